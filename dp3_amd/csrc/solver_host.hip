@@ -1,0 +1,669 @@
+// Host side of the solver library: device-resident problems in the
+// baseline-fastest layout, the iteration driver, and the C-ABI of
+// include/ddecal_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/ddecal_hip.h"
+#include "common.h"
+#include "solver_kernels.h"
+
+namespace ddh {
+
+namespace {
+thread_local std::string g_last_error;
+}
+void SetLastError(const std::string& message) { g_last_error = message; }
+const char* LastError() { return g_last_error.c_str(); }
+
+namespace {
+
+constexpr int kWave = 64;
+
+template <typename T>
+struct DeviceBuffer {
+  T* ptr = nullptr;
+  size_t count = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() { Free(); }
+  void Free() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+  }
+  void Resize(size_t n) {
+    if (n <= count && ptr) return;
+    Free();
+    if (n == 0) return;
+    DDH_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T)));
+    count = n;
+  }
+  void Upload(const std::vector<T>& v, hipStream_t s) {
+    Resize(std::max<size_t>(v.size(), 1));
+    if (!v.empty())
+      DDH_HIP_CHECK(hipMemcpyAsync(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+};
+
+struct SolverConfig {
+  ddh_config c;
+  std::vector<int32_t> n_sol_dir;
+  std::vector<int32_t> sol_start;
+  int n_sub = 0;
+};
+
+}  // namespace
+}  // namespace ddh
+
+struct ddh_problem {
+  // structure
+  int n_t = 0, n_bl = 0, bl_stride = 0, n_cb = 0, n_dir = 0, n_corr = 0, n_ant = 0;
+  int n_seg = 0;
+  bool f16 = false;
+  std::vector<int32_t> ant1, ant2, ch_per_cb, seg_t0, seg_sol;
+  std::vector<ddh::CbInfo> cbinfo;
+  size_t data_vis = 0, model_vis = 0;  // visibility counts of the device planes
+  // device
+  ddh::DeviceBuffer<unsigned char> data, model;
+  ddh::DeviceBuffer<float> scale;
+  ddh::DeviceBuffer<int32_t> d_ant1, d_ant2, d_csr_off, d_csr, d_seg_t0, d_seg_sol;
+  ddh::DeviceBuffer<ddh::CbInfo> d_cbinfo;
+  ddh::DeviceBuffer<float> staging;
+  ddh::DeviceBuffer<uint32_t> max_bits;
+  hipStream_t stream = nullptr;
+
+  size_t DataVisBytes() const { return size_t(n_corr) * 8; }
+  size_t ModelVisBytes() const { return f16 ? size_t(n_corr) * 4 : size_t(n_corr) * 8; }
+  size_t Rows(int cb) const { return size_t(n_t) * ch_per_cb[cb]; }
+};
+
+struct ddh_solver {
+  ddh::SolverConfig cfg;
+  hipStream_t stream = nullptr;
+  int waves = 4;
+  int use_graph = 0;
+  int poll = 8;
+  // per-solve device buffers
+  ddh::DeviceBuffer<double2> sol, next;
+  ddh::DeviceBuffer<float2> gains;
+  ddh::DeviceBuffer<double> conv, step_mags;
+  ddh::DeviceBuffer<float> partials;
+  ddh::DeviceBuffer<ddh::DeviceState> state;
+  ddh::DeviceBuffer<int32_t> n_sol_dir, sol_start;
+  ddh::DeviceState* host_state = nullptr;  // pinned
+  // timing of the last solve
+  std::vector<hipEvent_t> events;
+  int profile = 0;
+  double last_correlate_ms = 0.0;
+  int64_t last_correlate_launches = 0;
+  double last_solve_ms = 0.0;
+  // cached problem of the drop-in path
+  std::unique_ptr<ddh_problem, void (*)(ddh_problem*)> cached{nullptr, nullptr};
+};
+
+namespace ddh {
+namespace {
+
+void ValidateConfig(const ddh_config& c) {
+  DDH_REQUIRE(c.n_antennas > 1, -1, "n_antennas must be > 1");
+  DDH_REQUIRE(c.n_directions >= 1 && c.n_directions <= 16, -4,
+              "n_directions must be 1..16");
+  DDH_REQUIRE(c.n_channel_blocks >= 1, -1, "n_channel_blocks must be >= 1");
+  DDH_REQUIRE(c.n_correlations == 2 || c.n_correlations == 4, -1,
+              "n_correlations must be 2 (Duo) or 4 (Full)");
+  DDH_REQUIRE(c.n_solutions_per_direction != nullptr, -1,
+              "n_solutions_per_direction is required");
+  DDH_REQUIRE(c.max_iterations >= 1 && c.max_iterations <= 100000, -1,
+              "max_iterations must be 1..100000");
+  DDH_REQUIRE(c.min_iterations >= 0, -1, "min_iterations must be >= 0");
+  DDH_REQUIRE(c.step_size > 0.0, -1, "step_size must be > 0");
+  DDH_REQUIRE(c.model_storage == DDH_MODEL_C64 || c.model_storage == DDH_MODEL_F16, -1,
+              "model_storage must be DDH_MODEL_C64 or DDH_MODEL_F16");
+  for (int d = 0; d < c.n_directions; ++d) {
+    DDH_REQUIRE(c.n_solutions_per_direction[d] >= 1, -1,
+                "n_solutions_per_direction entries must be >= 1");
+    DDH_REQUIRE(c.n_antennas * c.n_solutions_per_direction[d] <= 1024, -4,
+                "n_antennas * solutions of one direction must be <= 1024");
+  }
+}
+
+// Builds segments: maximal time ranges in which every direction keeps one
+// solution index.
+void BuildSegments(ddh_problem& p, const uint32_t* sol_index, const SolverConfig& cfg) {
+  p.seg_t0.clear();
+  p.seg_sol.clear();
+  for (int t = 0; t < p.n_t; ++t) {
+    bool boundary = (t == 0);
+    for (int d = 0; d < p.n_dir && !boundary; ++d)
+      boundary = sol_index[d * p.n_t + t] != sol_index[d * p.n_t + t - 1];
+    if (boundary) {
+      p.seg_t0.push_back(t);
+      for (int d = 0; d < p.n_dir; ++d) {
+        const uint32_t s = sol_index[d * p.n_t + t];
+        DDH_REQUIRE(int(s) >= cfg.sol_start[d] && int(s) < cfg.sol_start[d] + cfg.n_sol_dir[d],
+                    -2, "solution index out of range for direction " + std::to_string(d));
+        p.seg_sol.push_back(int32_t(s));
+      }
+    }
+  }
+  p.seg_t0.push_back(p.n_t);
+  p.n_seg = int(p.seg_t0.size()) - 1;
+}
+
+ddh_problem* CreateProblem(const ddh_solver& s, int n_times, int n_baselines,
+                           const int32_t* antenna1, const int32_t* antenna2,
+                           const int32_t* channels_per_block, const uint32_t* sol_index) {
+  const ddh_config& c = s.cfg.c;
+  DDH_REQUIRE(n_times >= 1 && n_baselines >= 1, -1, "empty problem");
+  auto p = std::make_unique<ddh_problem>();
+  p->n_t = n_times;
+  p->n_bl = n_baselines;
+  p->bl_stride = (n_baselines + kWave - 1) / kWave * kWave;
+  p->n_cb = c.n_channel_blocks;
+  p->n_dir = c.n_directions;
+  p->n_corr = c.n_correlations;
+  p->n_ant = c.n_antennas;
+  p->f16 = c.model_storage == DDH_MODEL_F16;
+  p->stream = s.stream;
+  p->ant1.assign(antenna1, antenna1 + n_baselines);
+  p->ant2.assign(antenna2, antenna2 + n_baselines);
+  for (int b = 0; b < n_baselines; ++b) {
+    DDH_REQUIRE(p->ant1[b] >= 0 && p->ant1[b] < p->n_ant && p->ant2[b] >= 0 &&
+                    p->ant2[b] < p->n_ant,
+                -1, "antenna index out of range");
+    DDH_REQUIRE(p->ant1[b] != p->ant2[b], -1,
+                "auto-correlations are not solve data (SolveData drops them)");
+  }
+  p->ch_per_cb.assign(channels_per_block, channels_per_block + p->n_cb);
+  BuildSegments(*p, sol_index, s.cfg);
+  // Planes: per block one data plane, then n_dir model planes.
+  size_t data_off = 0, model_off = 0;
+  for (int cb = 0; cb < p->n_cb; ++cb) {
+    DDH_REQUIRE(p->ch_per_cb[cb] >= 1, -1, "channel blocks must hold >= 1 channel");
+    const size_t plane = p->Rows(cb) * p->bl_stride;
+    ddh::CbInfo info{};
+    info.n_ch = p->ch_per_cb[cb];
+    info.data_off = int64_t(data_off);
+    info.model_off = int64_t(model_off);
+    p->cbinfo.push_back(info);
+    data_off += plane;
+    model_off += plane * p->n_dir;
+  }
+  p->data_vis = data_off;
+  p->model_vis = model_off;
+  p->data.Resize(data_off * p->DataVisBytes());
+  p->model.Resize(model_off * p->ModelVisBytes());
+  // Padding lanes must read zeros.
+  DDH_HIP_CHECK(hipMemsetAsync(p->data.ptr, 0, data_off * p->DataVisBytes(), s.stream));
+  DDH_HIP_CHECK(hipMemsetAsync(p->model.ptr, 0, model_off * p->ModelVisBytes(), s.stream));
+  p->scale.Upload(std::vector<float>(size_t(p->n_cb) * p->n_dir, 1.0f), s.stream);
+  // Baseline tables padded to the stride.
+  std::vector<int32_t> a1(p->bl_stride, 0), a2(p->bl_stride, 0);
+  std::copy(p->ant1.begin(), p->ant1.end(), a1.begin());
+  std::copy(p->ant2.begin(), p->ant2.end(), a2.begin());
+  p->d_ant1.Upload(a1, s.stream);
+  p->d_ant2.Upload(a2, s.stream);
+  // Per-antenna baseline lists: (baseline << 1) | side, side 1 = second antenna.
+  std::vector<int32_t> off(p->n_ant + 1, 0), csr(size_t(2) * n_baselines);
+  for (int b = 0; b < n_baselines; ++b) {
+    off[p->ant1[b] + 1]++;
+    off[p->ant2[b] + 1]++;
+  }
+  for (int a = 0; a < p->n_ant; ++a) off[a + 1] += off[a];
+  std::vector<int32_t> fill(off.begin(), off.end() - 1);
+  for (int b = 0; b < n_baselines; ++b) {
+    csr[fill[p->ant1[b]]++] = (b << 1);
+    csr[fill[p->ant2[b]]++] = (b << 1) | 1;
+  }
+  p->d_csr_off.Upload(off, s.stream);
+  p->d_csr.Upload(csr, s.stream);
+  p->d_seg_t0.Upload(p->seg_t0, s.stream);
+  p->d_seg_sol.Upload(p->seg_sol, s.stream);
+  p->d_cbinfo.Upload(p->cbinfo, s.stream);
+  DDH_HIP_CHECK(hipStreamSynchronize(s.stream));
+  return p.release();
+}
+
+// Uploads one channel block given in reference order, [t][bl][ch][n_corr]
+// (data) and [d][t][bl][ch][n_corr] (model), converting to the device layout.
+void UploadBlock(ddh_problem& p, int cb, const float* data, const float* model) {
+  const int n_ch = p.ch_per_cb[cb];
+  const size_t n_vis = size_t(p.n_t) * p.n_bl * n_ch;
+  const size_t vis_floats = size_t(p.n_corr) * 2;
+  const size_t need = std::max(n_vis, n_vis * p.n_dir) * vis_floats;
+  p.staging.Resize(need);
+  // data
+  DDH_HIP_CHECK(hipMemcpyAsync(p.staging.ptr, data, n_vis * vis_floats * sizeof(float),
+                               hipMemcpyHostToDevice, p.stream));
+  LaunchRepack(p.staging.ptr, p.data.ptr + p.cbinfo[cb].data_off * p.DataVisBytes(), 1, p.n_t,
+               p.n_bl, n_ch, p.bl_stride, p.n_corr, false, nullptr, p.stream);
+  // model
+  DDH_HIP_CHECK(hipMemcpyAsync(p.staging.ptr, model,
+                               n_vis * p.n_dir * vis_floats * sizeof(float),
+                               hipMemcpyHostToDevice, p.stream));
+  float* scale = nullptr;
+  if (p.f16) {
+    p.max_bits.Resize(p.n_dir);
+    DDH_HIP_CHECK(hipMemsetAsync(p.max_bits.ptr, 0, p.n_dir * sizeof(uint32_t), p.stream));
+    LaunchPlaneMaxAbs(p.staging.ptr, int64_t(n_vis * vis_floats), p.n_dir, p.max_bits.ptr,
+                      p.stream);
+    scale = p.scale.ptr + size_t(cb) * p.n_dir;
+    LaunchScaleFromMax(p.max_bits.ptr, scale, p.n_dir, p.stream);
+  }
+  LaunchRepack(p.staging.ptr, p.model.ptr + p.cbinfo[cb].model_off * p.ModelVisBytes(),
+               p.n_dir, p.n_t, p.n_bl, n_ch, p.bl_stride, p.n_corr, p.f16, scale, p.stream);
+  // The staging buffer is reused by the next block.
+  DDH_HIP_CHECK(hipStreamSynchronize(p.stream));
+}
+
+// Structure of host SolveData channel blocks: n_t, baseline list, channels per
+// block and per-(direction, time) solution index. Verifies the
+// time -> baseline -> channel order of SolveData.cc:129-181.
+struct Structure {
+  int n_t = 0, n_bl = 0;
+  std::vector<int32_t> a1, a2, ch;
+  std::vector<uint32_t> sol_index;
+};
+
+Structure DetectStructure(const ddh_cb_view* cbs, int n_cb, int n_dir) {
+  Structure st;
+  for (int cb = 0; cb < n_cb; ++cb) {
+    const ddh_cb_view& v = cbs[cb];
+    DDH_REQUIRE(v.n_visibilities > 0 && v.data && v.model && v.antenna_pairs && v.solution_map,
+                -1, "channel block " + std::to_string(cb) + " is empty or incomplete");
+    const uint32_t* pr = v.antenna_pairs;
+    const int64_t nv = v.n_visibilities;
+    // Channels in block: run length of the first pair. Baselines: runs until
+    // the first pair comes back (the next time step) or the block ends.
+    int64_t run = 1;
+    while (run < nv && pr[2 * run] == pr[0] && pr[2 * run + 1] == pr[1]) ++run;
+    int64_t n_bl = 1;
+    while (n_bl * run < nv &&
+           !(pr[2 * n_bl * run] == pr[0] && pr[2 * n_bl * run + 1] == pr[1]))
+      ++n_bl;
+    int64_t n_t = nv / (n_bl * run);
+    if (n_bl == 1) {
+      // A single baseline cannot separate time from channel; treat every
+      // visibility as its own row.
+      run = 1;
+      n_t = nv;
+    }
+    DDH_REQUIRE(n_t * n_bl * run == nv, -2, "visibility count does not factor as time x baseline x channel");
+    if (cb == 0) {
+      st.n_t = int(n_t);
+      st.n_bl = int(n_bl);
+      for (int64_t b = 0; b < n_bl; ++b) {
+        st.a1.push_back(int32_t(pr[2 * b * run]));
+        st.a2.push_back(int32_t(pr[2 * b * run + 1]));
+      }
+      st.sol_index.resize(size_t(n_dir) * n_t);
+      for (int d = 0; d < n_dir; ++d)
+        for (int64_t t = 0; t < n_t; ++t)
+          st.sol_index[d * n_t + t] = v.solution_map[d * nv + t * n_bl * run];
+    } else {
+      DDH_REQUIRE(n_t == st.n_t && n_bl == st.n_bl, -2,
+                  "channel blocks differ in time steps or baselines");
+    }
+    st.ch.push_back(int32_t(run));
+    // verify pairs and the time-only dependence of the solution map
+    for (int64_t t = 0; t < n_t; ++t)
+      for (int64_t b = 0; b < n_bl; ++b)
+        for (int64_t c = 0; c < run; ++c) {
+          const int64_t i = (t * n_bl + b) * run + c;
+          DDH_REQUIRE(int32_t(pr[2 * i]) == st.a1[b] && int32_t(pr[2 * i + 1]) == st.a2[b], -2,
+                      "antenna pairs do not follow the time -> baseline -> channel order");
+        }
+    for (int d = 0; d < n_dir; ++d)
+      for (int64_t t = 0; t < n_t; ++t) {
+        const uint32_t s = st.sol_index[d * n_t + t];
+        const uint32_t* m = v.solution_map + d * nv + t * n_bl * run;
+        DDH_REQUIRE(m[0] == s && m[n_bl * run - 1] == s, -2,
+                    "solution map varies within a time step");
+      }
+  }
+  return st;
+}
+
+IterArgs MakeArgs(ddh_solver& s, ddh_problem& p) {
+  const ddh_config& c = s.cfg.c;
+  IterArgs a{};
+  a.data = p.data.ptr;
+  a.model = p.model.ptr;
+  a.model_scale = p.scale.ptr;
+  a.ant1 = p.d_ant1.ptr;
+  a.ant2 = p.d_ant2.ptr;
+  a.csr_off = p.d_csr_off.ptr;
+  a.csr = p.d_csr.ptr;
+  a.cbinfo = p.d_cbinfo.ptr;
+  a.seg_t0 = p.d_seg_t0.ptr;
+  a.seg_sol = p.d_seg_sol.ptr;
+  a.n_sol_dir = s.n_sol_dir.ptr;
+  a.sol_start = s.sol_start.ptr;
+  a.sol = s.sol.ptr;
+  a.next = s.next.ptr;
+  a.gains = s.gains.ptr;
+  a.conv = s.conv.ptr;
+  a.partials = s.partials.ptr;
+  a.state = s.state.ptr;
+  a.step_mags = s.step_mags.ptr;
+  a.n_t = p.n_t;
+  a.n_bl = p.n_bl;
+  a.bl_stride = p.bl_stride;
+  a.n_cb = p.n_cb;
+  a.n_dir = p.n_dir;
+  a.n_ant = p.n_ant;
+  a.n_sub = s.cfg.n_sub;
+  a.n_seg = p.n_seg;
+  a.max_iterations = c.max_iterations;
+  a.min_iterations = c.min_iterations;
+  a.accuracy = c.accuracy;
+  a.step_size = c.step_size;
+  a.step_diff_sigma = c.step_diff_sigma;
+  a.detect_stalling = c.detect_stalling;
+  a.phase_only = c.phase_only;
+  return a;
+}
+
+void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* result) {
+  const ddh_config& c = s.cfg.c;
+  DDH_REQUIRE(p.n_cb == c.n_channel_blocks && p.n_dir == c.n_directions &&
+                  p.n_corr == c.n_correlations && p.n_ant == c.n_antennas,
+              -1, "problem does not match the solver configuration");
+  DDH_REQUIRE(p.f16 == (c.model_storage == DDH_MODEL_F16), -1,
+              "problem model storage does not match the solver");
+  const size_t sol_per_cb = size_t(c.n_antennas) * s.cfg.n_sub * 2;
+  const size_t n_sol = sol_per_cb * c.n_channel_blocks;
+  s.sol.Resize(n_sol);
+  s.next.Resize(n_sol);
+  s.gains.Resize(n_sol);
+  s.conv.Resize(n_sol / 2);
+  s.step_mags.Resize(size_t(std::max(c.max_iterations, c.min_iterations)) + 2);
+  s.partials.Resize(size_t(p.n_cb) * p.n_seg * p.n_dir * p.n_corr * 3 * p.bl_stride);
+  s.state.Resize(1);
+  for (int cb = 0; cb < c.n_channel_blocks; ++cb)
+    DDH_HIP_CHECK(hipMemcpyAsync(s.sol.ptr + cb * sol_per_cb, solutions[cb],
+                                 sol_per_cb * sizeof(double2), hipMemcpyHostToDevice, s.stream));
+  const IterArgs a = MakeArgs(s, p);
+  const int limit = std::max(c.max_iterations, c.min_iterations);
+  const bool f16 = p.f16;
+
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  DDH_HIP_CHECK(hipEventCreate(&t0));
+  DDH_HIP_CHECK(hipEventCreate(&t1));
+  if (s.profile && int(s.events.size()) < 2 * limit) {
+    for (hipEvent_t e : s.events) (void)hipEventDestroy(e);
+    s.events.assign(2 * limit, nullptr);
+    for (hipEvent_t& e : s.events) DDH_HIP_CHECK(hipEventCreate(&e));
+  }
+  DDH_HIP_CHECK(hipEventRecord(t0, s.stream));
+  LaunchPrologue(a, s.stream);
+  int launched = 0;
+  while (launched < limit) {
+    const int k = std::min(s.poll, limit - launched);
+    for (int i = 0; i < k; ++i) {
+      const int it = launched + i;
+      if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it], s.stream));
+      LaunchCorrelate(a, p.n_corr, f16, s.waves, s.stream);
+      if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it + 1], s.stream));
+      LaunchAntennaSolve(a, p.n_corr, f16, s.stream);
+      LaunchAssign(a, s.stream);
+    }
+    launched += k;
+    DDH_HIP_CHECK(hipMemcpyAsync(s.host_state, s.state.ptr, sizeof(DeviceState),
+                                 hipMemcpyDeviceToHost, s.stream));
+    DDH_HIP_CHECK(hipStreamSynchronize(s.stream));
+    if (s.host_state->done) break;
+  }
+  DDH_HIP_CHECK(hipEventRecord(t1, s.stream));
+  for (int cb = 0; cb < c.n_channel_blocks; ++cb)
+    DDH_HIP_CHECK(hipMemcpyAsync(solutions[cb], s.sol.ptr + cb * sol_per_cb,
+                                 sol_per_cb * sizeof(double2), hipMemcpyDeviceToHost, s.stream));
+  DDH_HIP_CHECK(hipStreamSynchronize(s.stream));
+  const DeviceState st = *s.host_state;
+  DDH_REQUIRE(st.done, -3, "solver did not reach its stopping criterion");
+  float ms = 0.0f;
+  DDH_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+  s.last_solve_ms = ms;
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  s.last_correlate_ms = 0.0;
+  s.last_correlate_launches = st.iteration;
+  if (s.profile) {
+    for (int it = 0; it < st.iteration; ++it) {
+      float e = 0.0f;
+      DDH_HIP_CHECK(hipEventElapsedTime(&e, s.events[2 * it], s.events[2 * it + 1]));
+      s.last_correlate_ms += e;
+    }
+  }
+  if (result) {
+    result->iterations = st.has_converged ? st.iteration : st.iteration + 1;
+    result->constraint_iterations = 0;
+    result->has_converged = st.has_converged;
+    result->n_step_magnitudes = st.iteration;
+    result->last_step_magnitude = st.last_step;
+  }
+}
+
+}  // namespace
+}  // namespace ddh
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+using namespace ddh;
+
+extern "C" {
+
+const char* ddh_last_error(void) { return LastError(); }
+
+int ddh_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+void ddh_config_defaults(ddh_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->n_correlations = 2;
+  c->max_iterations = 100;
+  c->min_iterations = 0;
+  c->accuracy = 1.0e-5;
+  c->constraint_accuracy = 1.0e-4;
+  c->step_size = 0.2;
+  c->detect_stalling = 1;
+  c->step_diff_sigma = 0.1;
+  c->phase_only = 0;
+  c->model_storage = DDH_MODEL_C64;
+}
+
+ddh_solver* ddh_solver_create(const ddh_config* config) {
+  ddh_solver* out = nullptr;
+  const int rc = Guard([&] {
+    DDH_REQUIRE(config != nullptr, -1, "config is NULL");
+    ValidateConfig(*config);
+    int n_dev = 0;
+    DDH_REQUIRE(hipGetDeviceCount(&n_dev) == hipSuccess && n_dev > 0, -3,
+                "no HIP device available");
+    auto s = std::make_unique<ddh_solver>();
+    s->cfg.c = *config;
+    s->cfg.n_sol_dir.assign(config->n_solutions_per_direction,
+                            config->n_solutions_per_direction + config->n_directions);
+    s->cfg.c.n_solutions_per_direction = s->cfg.n_sol_dir.data();
+    int acc = 0;
+    for (int v : s->cfg.n_sol_dir) {
+      s->cfg.sol_start.push_back(acc);
+      acc += v;
+    }
+    s->cfg.n_sub = acc;
+    PrepareKernelAttributes();
+    DDH_HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    DDH_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&s->host_state), sizeof(DeviceState),
+                                hipHostMallocDefault));
+    s->n_sol_dir.Upload(s->cfg.n_sol_dir, s->stream);
+    s->sol_start.Upload(s->cfg.sol_start, s->stream);
+    DDH_HIP_CHECK(hipStreamSynchronize(s->stream));
+    out = s.release();
+  });
+  return rc == 0 ? out : nullptr;
+}
+
+void ddh_problem_destroy(ddh_problem* p) {
+  if (!p) return;
+  delete p;
+}
+
+void ddh_solver_destroy(ddh_solver* s) {
+  if (!s) return;
+  (void)hipStreamSynchronize(s->stream);
+  s->cached.reset();
+  for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
+  if (s->host_state) (void)hipHostFree(s->host_state);
+  s->sol.Free();
+  s->next.Free();
+  s->gains.Free();
+  s->conv.Free();
+  s->step_mags.Free();
+  s->partials.Free();
+  s->state.Free();
+  s->n_sol_dir.Free();
+  s->sol_start.Free();
+  (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+ddh_problem* ddh_problem_create(const ddh_solver* s, int32_t n_times, int32_t n_baselines,
+                                const int32_t* antenna1, const int32_t* antenna2,
+                                const int32_t* channels_per_block, const uint32_t* sol_index) {
+  ddh_problem* out = nullptr;
+  const int rc = Guard([&] {
+    DDH_REQUIRE(s && antenna1 && antenna2 && channels_per_block && sol_index, -1,
+                "NULL argument");
+    out = CreateProblem(*s, n_times, n_baselines, antenna1, antenna2, channels_per_block,
+                        sol_index);
+  });
+  return rc == 0 ? out : nullptr;
+}
+
+int ddh_problem_baseline_stride(const ddh_problem* p) { return p ? p->bl_stride : 0; }
+
+int ddh_problem_upload(ddh_problem* p, const ddh_cb_view* cbs) {
+  return Guard([&] {
+    DDH_REQUIRE(p && cbs, -1, "NULL argument");
+    for (int cb = 0; cb < p->n_cb; ++cb) {
+      const int64_t nv = int64_t(p->n_t) * p->n_bl * p->ch_per_cb[cb];
+      DDH_REQUIRE(cbs[cb].n_visibilities == nv, -2,
+                  "channel block " + std::to_string(cb) + " has the wrong visibility count");
+      UploadBlock(*p, cb, cbs[cb].data, cbs[cb].model);
+    }
+  });
+}
+
+void* ddh_problem_data_ptr(ddh_problem* p, int32_t cb) {
+  if (!p || cb < 0 || cb >= p->n_cb) return nullptr;
+  return p->data.ptr + p->cbinfo[cb].data_off * p->DataVisBytes();
+}
+
+void* ddh_problem_model_ptr(ddh_problem* p, int32_t cb, int32_t d) {
+  if (!p || cb < 0 || cb >= p->n_cb || d < 0 || d >= p->n_dir) return nullptr;
+  const size_t plane = p->Rows(cb) * p->bl_stride;
+  return p->model.ptr + (p->cbinfo[cb].model_off + d * plane) * p->ModelVisBytes();
+}
+
+int ddh_problem_set_model_scale(ddh_problem* p, const float* scale) {
+  return Guard([&] {
+    DDH_REQUIRE(p && scale, -1, "NULL argument");
+    DDH_HIP_CHECK(hipMemcpy(p->scale.ptr, scale, sizeof(float) * p->n_cb * p->n_dir,
+                            hipMemcpyHostToDevice));
+  });
+}
+
+int ddh_problem_download_plane(ddh_problem* p, int32_t cb, int32_t plane, void* host) {
+  return Guard([&] {
+    DDH_REQUIRE(p && host && cb >= 0 && cb < p->n_cb && plane >= 0 && plane <= p->n_dir, -1,
+                "bad argument");
+    const size_t n = p->Rows(cb) * p->bl_stride;
+    const void* src = plane == 0 ? ddh_problem_data_ptr(p, cb) : ddh_problem_model_ptr(p, cb, plane - 1);
+    const size_t bytes = n * (plane == 0 ? p->DataVisBytes() : p->ModelVisBytes());
+    DDH_HIP_CHECK(hipStreamSynchronize(p->stream));
+    DDH_HIP_CHECK(hipMemcpy(host, src, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int ddh_solve_problem(ddh_solver* s, ddh_problem* p, double* const* solutions,
+                      ddh_result* result) {
+  return Guard([&] {
+    DDH_REQUIRE(s && p && solutions, -1, "NULL argument");
+    Solve(*s, *p, solutions, result);
+  });
+}
+
+int ddh_solve(ddh_solver* s, const ddh_cb_view* cbs, int32_t n_cb, double* const* solutions,
+              double time, ddh_result* result) {
+  (void)time;
+  return Guard([&] {
+    DDH_REQUIRE(s && cbs && solutions, -1, "NULL argument");
+    DDH_REQUIRE(n_cb == s->cfg.c.n_channel_blocks, -1,
+                "number of channel blocks differs from Initialize");
+    const Structure st = DetectStructure(cbs, n_cb, s->cfg.c.n_directions);
+    ddh_problem* cached = s->cached.get();
+    bool reuse = cached && cached->n_t == st.n_t && cached->n_bl == st.n_bl &&
+                 cached->ant1 == st.a1 && cached->ant2 == st.a2 && cached->ch_per_cb == st.ch;
+    if (reuse) {
+      // the segmentation depends on the solution map
+      std::vector<int32_t> seg_t0, seg_sol;
+      ddh_problem probe;
+      probe.n_t = st.n_t;
+      probe.n_dir = s->cfg.c.n_directions;
+      BuildSegments(probe, st.sol_index.data(), s->cfg);
+      reuse = probe.seg_t0 == cached->seg_t0 && probe.seg_sol == cached->seg_sol;
+    }
+    if (!reuse) {
+      s->cached = std::unique_ptr<ddh_problem, void (*)(ddh_problem*)>(
+          CreateProblem(*s, st.n_t, st.n_bl, st.a1.data(), st.a2.data(), st.ch.data(),
+                        st.sol_index.data()),
+          &ddh_problem_destroy);
+    }
+    ddh_problem& p = *s->cached;
+    for (int cb = 0; cb < n_cb; ++cb) UploadBlock(p, cb, cbs[cb].data, cbs[cb].model);
+    Solve(*s, p, solutions, result);
+  });
+}
+
+int ddh_solver_last_timing(const ddh_solver* s, double* correlate_ms, int64_t* launches,
+                           double* solve_ms) {
+  if (!s) return -1;
+  if (correlate_ms) *correlate_ms = s->last_correlate_ms;
+  if (launches) *launches = s->last_correlate_launches;
+  if (solve_ms) *solve_ms = s->last_solve_ms;
+  return 0;
+}
+
+int ddh_set_schedule(ddh_solver* s, int32_t waves_per_tile, int32_t use_graph,
+                     int32_t poll_interval) {
+  return Guard([&] {
+    DDH_REQUIRE(s, -1, "NULL solver");
+    DDH_REQUIRE(waves_per_tile >= 1 && waves_per_tile <= 4, -1, "waves_per_tile must be 1..4");
+    DDH_REQUIRE(poll_interval >= 1, -1, "poll_interval must be >= 1");
+    s->waves = waves_per_tile;
+    s->use_graph = use_graph;
+    s->poll = poll_interval;
+  });
+}
+
+int ddh_set_profiling(ddh_solver* s, int32_t enable) {
+  if (!s) return -1;
+  s->profile = enable;
+  return 0;
+}
+
+}  // extern "C"

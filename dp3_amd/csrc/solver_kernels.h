@@ -1,0 +1,109 @@
+// Device kernels of the iterative diagonal solver.
+//
+// One solver iteration (IterativeDiagonalSolver.cc:67-94) is three launches:
+//
+//   correlate      streams the data plane and every model plane of a block
+//                  once. Per visibility it forms the full residual
+//                  R = D - sum_d G_a M_d G_b^H (SolveData.h:204-244) and, per
+//                  direction d and correlation k, accumulates the per-baseline
+//                  sums U_dk = sum R_k conj(M_dk) and S_dk = sum |M_dk|^2.
+//                  All directions are solved against residuals built from the
+//                  same old solutions (IterativeDiagonalSolver.cc:115-126), so
+//                  one pass serves every direction. The gains are constant
+//                  over a baseline's visibilities within a segment, so the
+//                  reference's per-visibility numerator/denominator terms
+//                  (:172-197) factor into these sums (see antenna_solve).
+//   antenna_solve  per (block, direction): gathers each antenna's baselines,
+//                  forms numerator and denominator (IterativeDiagonalSolver.cc
+//                  :134-138) from U, S and the gains in FP64, divides
+//                  (:206-221), applies the step (SolverBase.cc:88-158) and the
+//                  per-solution convergence term (SolverBase.cc:254-268).
+//   assign         one workgroup: FP64 convergence test (SolverBase.cc:296-306),
+//                  stall detection (:63-86), stopping criterion
+//                  (SolverBase.h:274-285), copy of the next solutions and the
+//                  NaN repair of the next iteration (SolverBase.cc:342-371).
+//
+// Every launch first reads the device-side `done` flag and returns when the
+// solve has already stopped, so the host may enqueue iterations ahead of the
+// convergence decision.
+#ifndef DP3_AMD_SOLVER_KERNELS_H
+#define DP3_AMD_SOLVER_KERNELS_H
+
+#include <cstdint>
+
+#include "common.h"
+
+namespace ddh {
+
+struct CbInfo {
+  int32_t n_ch;       // channels in this block
+  int32_t pad;
+  int64_t data_off;   // visibility offset of the data plane
+  int64_t model_off;  // visibility offset of direction 0's model plane
+};
+
+struct DeviceState {
+  int32_t iteration;
+  int32_t done;
+  int32_t has_converged;
+  int32_t pad;
+  double last_step;
+  double n_var_count;
+  double step_mean;
+  double step_var;
+};
+
+// Everything one iteration needs. Passed by value to each kernel.
+struct IterArgs {
+  // problem (device layout)
+  const void* data;          // Duo: float4 per vis; Full: 2 float4
+  const void* model;         // complex64 or FP16 planes
+  const float* model_scale;  // [n_cb][n_dir] (FP16 only)
+  const int32_t* ant1;       // [bl_stride]
+  const int32_t* ant2;
+  const int32_t* csr_off;    // [n_ant+1]
+  const int32_t* csr;        // (baseline << 1) | side
+  const CbInfo* cbinfo;      // [n_cb]
+  const int32_t* seg_t0;     // [n_seg+1]
+  const int32_t* seg_sol;    // [n_seg][n_dir]
+  // solver
+  const int32_t* n_sol_dir;  // [n_dir]
+  const int32_t* sol_start;  // [n_dir]
+  double2* sol;              // [n_cb][n_ant][n_sub][2]
+  double2* next;             // same
+  float2* gains;             // float copy of sol, read by correlate
+  double* conv;              // [n_cb][n_ant][n_sub]
+  float* partials;           // [n_cb][n_seg][n_dir][n_corr][3][bl_stride]
+  DeviceState* state;
+  double* step_mags;         // [max_iterations + 1]
+  int32_t n_t, n_bl, bl_stride, n_cb, n_dir, n_ant, n_sub, n_seg;
+  int32_t max_iterations, min_iterations;
+  double accuracy, step_size, step_diff_sigma;
+  int32_t detect_stalling, phase_only;
+};
+
+// Host-side launchers (defined in solver_kernels.hip).
+void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves,
+                     hipStream_t stream);
+void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, hipStream_t stream);
+void LaunchAssign(const IterArgs& a, hipStream_t stream);
+void LaunchPrologue(const IterArgs& a, hipStream_t stream);
+size_t CorrelateSharedBytes(int n_dir, int n_corr, int waves);
+size_t MaxSharedBytes();
+void PrepareKernelAttributes();
+
+// Layout conversion: reference order [plane][t][bl][ch][n_corr] complex64 ->
+// device order [plane][t][ch][bl_stride] (complex64 or FP16/scale).
+void LaunchRepack(const float* src, void* dst, int n_planes, int n_t, int n_bl,
+                  int n_ch, int bl_stride, int n_corr, bool f16,
+                  const float* plane_scale, hipStream_t stream);
+// Per-plane max |re|,|im| of reference-order planes (FP16 scale), written as
+// float bits into max_bits[n_planes] (which must be zeroed first).
+void LaunchPlaneMaxAbs(const float* src, int64_t plane_floats, int n_planes,
+                       uint32_t* max_bits, hipStream_t stream);
+void LaunchScaleFromMax(const uint32_t* max_bits, float* scale, int n,
+                        hipStream_t stream);
+
+}  // namespace ddh
+
+#endif

@@ -1,0 +1,301 @@
+"""ctypes bindings of libddecal_hip.so (include/ddecal_hip.h).
+
+The library is the product: there is no CPU fallback. Loading fails loudly
+when the shared library has not been built (python -m dp3_amd.build).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libddecal_hip.so")
+
+MODEL_C64 = 0
+MODEL_F16 = 1
+
+_LIB = None
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("n_antennas", C.c_int32),
+        ("n_directions", C.c_int32),
+        ("n_channel_blocks", C.c_int32),
+        ("n_correlations", C.c_int32),
+        ("n_solutions_per_direction", C.POINTER(C.c_int32)),
+        ("max_iterations", C.c_int32),
+        ("min_iterations", C.c_int32),
+        ("accuracy", C.c_double),
+        ("constraint_accuracy", C.c_double),
+        ("step_size", C.c_double),
+        ("detect_stalling", C.c_int32),
+        ("step_diff_sigma", C.c_double),
+        ("phase_only", C.c_int32),
+        ("model_storage", C.c_int32),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64),
+        ("constraint_iterations", C.c_int64),
+        ("has_converged", C.c_int32),
+        ("n_step_magnitudes", C.c_int32),
+        ("last_step_magnitude", C.c_double),
+    ]
+
+
+class CbView(C.Structure):
+    _fields_ = [
+        ("n_visibilities", C.c_int64),
+        ("data", C.c_void_p),
+        ("model", C.c_void_p),
+        ("antenna_pairs", C.c_void_p),
+        ("solution_map", C.c_void_p),
+    ]
+
+
+# Every exported symbol and its signature; tests check they are all present.
+SIGNATURES = {
+    "ddh_last_error": ([], C.c_char_p),
+    "ddh_device_count": ([], C.c_int),
+    "ddh_config_defaults": ([C.POINTER(Config)], None),
+    "ddh_solver_create": ([C.POINTER(Config)], C.c_void_p),
+    "ddh_solver_destroy": ([C.c_void_p], None),
+    "ddh_solve": ([C.c_void_p, C.POINTER(CbView), C.c_int32, C.c_void_p, C.c_double,
+                   C.POINTER(Result)], C.c_int),
+    "ddh_problem_create": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                            C.c_void_p, C.c_void_p], C.c_void_p),
+    "ddh_problem_destroy": ([C.c_void_p], None),
+    "ddh_problem_upload": ([C.c_void_p, C.POINTER(CbView)], C.c_int),
+    "ddh_problem_baseline_stride": ([C.c_void_p], C.c_int),
+    "ddh_problem_data_ptr": ([C.c_void_p, C.c_int32], C.c_void_p),
+    "ddh_problem_model_ptr": ([C.c_void_p, C.c_int32, C.c_int32], C.c_void_p),
+    "ddh_problem_set_model_scale": ([C.c_void_p, C.c_void_p], C.c_int),
+    "ddh_problem_download_plane": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
+    "ddh_solve_problem": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Result)], C.c_int),
+    "ddh_solver_last_timing": ([C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                C.POINTER(C.c_double)], C.c_int),
+    "ddh_set_schedule": ([C.c_void_p, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+    "ddh_set_profiling": ([C.c_void_p, C.c_int32], C.c_int),
+}
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise HipError(f"{LIB_PATH} is missing: build it with `python -m dp3_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = L
+    return _LIB
+
+
+def last_error():
+    msg = lib().ddh_last_error()
+    return msg.decode() if msg else ""
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise HipError(f"{what} failed ({rc}): {last_error()}")
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+@dataclass
+class SolveResult:
+    iterations: int
+    constraint_iterations: int
+    has_converged: bool
+    n_step_magnitudes: int
+    last_step_magnitude: float
+
+
+class Solver:
+    """IterativeDiagonalSolver on the GPU (one instance = one HIP stream)."""
+
+    def __init__(self, n_antennas, n_solutions_per_direction, n_channel_blocks, *,
+                 n_correlations=2, max_iterations=100, min_iterations=0, accuracy=1e-5,
+                 constraint_accuracy=1e-4, step_size=0.2, detect_stalling=True,
+                 step_diff_sigma=0.1, phase_only=False, model_storage=MODEL_C64):
+        L = lib()
+        self._nsd = np.ascontiguousarray(n_solutions_per_direction, dtype=np.int32)
+        cfg = Config()
+        L.ddh_config_defaults(C.byref(cfg))
+        cfg.n_antennas = n_antennas
+        cfg.n_directions = len(self._nsd)
+        cfg.n_channel_blocks = n_channel_blocks
+        cfg.n_correlations = n_correlations
+        cfg.n_solutions_per_direction = self._nsd.ctypes.data_as(C.POINTER(C.c_int32))
+        cfg.max_iterations = max_iterations
+        cfg.min_iterations = min_iterations
+        cfg.accuracy = accuracy
+        cfg.constraint_accuracy = constraint_accuracy
+        cfg.step_size = step_size
+        cfg.detect_stalling = int(detect_stalling)
+        cfg.step_diff_sigma = step_diff_sigma
+        cfg.phase_only = int(phase_only)
+        cfg.model_storage = model_storage
+        self.config = cfg
+        self.n_antennas = n_antennas
+        self.n_channel_blocks = n_channel_blocks
+        self.n_correlations = n_correlations
+        self.n_sub = int(self._nsd.sum())
+        self.model_storage = model_storage
+        h = L.ddh_solver_create(C.byref(cfg))
+        if not h:
+            raise HipError(f"ddh_solver_create failed: {last_error()}")
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().ddh_solver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def set_schedule(self, waves_per_tile=4, use_graph=0, poll_interval=8):
+        _check(lib().ddh_set_schedule(self.handle, waves_per_tile, use_graph, poll_interval),
+               "ddh_set_schedule")
+
+    def set_profiling(self, enable=True):
+        _check(lib().ddh_set_profiling(self.handle, int(enable)), "ddh_set_profiling")
+
+    def last_timing(self):
+        ms = C.c_double()
+        n = C.c_int64()
+        total = C.c_double()
+        _check(lib().ddh_solver_last_timing(self.handle, C.byref(ms), C.byref(n),
+                                            C.byref(total)), "ddh_solver_last_timing")
+        return ms.value, n.value, total.value
+
+    @staticmethod
+    def _sol_ptrs(solutions):
+        assert solutions.dtype == np.complex128 and solutions.flags.c_contiguous
+        n_cb = solutions.shape[0]
+        arr = (C.c_void_p * n_cb)(*[solutions[cb].ctypes.data for cb in range(n_cb)])
+        return arr
+
+    def solve(self, problem, solutions, time=0.0):
+        """Drop-in solve from a SolveData-layout problem (dp3_amd.synth.SolveProblem).
+        `solutions` [n_cb, n_ant*n_sub*2] complex128 is updated in place."""
+        views = (CbView * problem.n_channel_blocks)()
+        keep = []
+        n_dir = problem.n_directions
+        for cb in range(problem.n_channel_blocks):
+            v0, v1 = int(problem.vis_offsets[cb]), int(problem.vis_offsets[cb + 1])
+            d = np.ascontiguousarray(problem.data[v0:v1])
+            m = np.ascontiguousarray(problem.model[v0 * n_dir:v1 * n_dir])
+            pairs = np.ascontiguousarray(np.stack([problem.ant1[v0:v1], problem.ant2[v0:v1]],
+                                                  axis=1).astype(np.uint32))
+            sm = np.ascontiguousarray(problem.sol_map[v0 * n_dir:v1 * n_dir])
+            keep += [d, m, pairs, sm]
+            views[cb] = CbView(v1 - v0, d.ctypes.data, m.ctypes.data, pairs.ctypes.data,
+                               sm.ctypes.data)
+        res = Result()
+        ptrs = self._sol_ptrs(solutions)
+        _check(lib().ddh_solve(self.handle, views, problem.n_channel_blocks,
+                               C.cast(ptrs, C.c_void_p), time, C.byref(res)), "ddh_solve")
+        return SolveResult(res.iterations, res.constraint_iterations, bool(res.has_converged),
+                           res.n_step_magnitudes, res.last_step_magnitude)
+
+    def solve_problem(self, dproblem, solutions):
+        res = Result()
+        ptrs = self._sol_ptrs(solutions)
+        _check(lib().ddh_solve_problem(self.handle, dproblem.handle, C.cast(ptrs, C.c_void_p),
+                                       C.byref(res)), "ddh_solve_problem")
+        return SolveResult(res.iterations, res.constraint_iterations, bool(res.has_converged),
+                           res.n_step_magnitudes, res.last_step_magnitude)
+
+
+class DeviceProblem:
+    """A solution interval resident in device memory (library layout)."""
+
+    def __init__(self, solver: Solver, n_times, antenna1, antenna2, channels_per_block,
+                 sol_index):
+        a1 = np.ascontiguousarray(antenna1, dtype=np.int32)
+        a2 = np.ascontiguousarray(antenna2, dtype=np.int32)
+        ch = np.ascontiguousarray(channels_per_block, dtype=np.int32)
+        si = np.ascontiguousarray(sol_index, dtype=np.uint32)
+        h = lib().ddh_problem_create(solver.handle, n_times, len(a1), _ptr(a1), _ptr(a2),
+                                     _ptr(ch), _ptr(si))
+        if not h:
+            raise HipError(f"ddh_problem_create failed: {last_error()}")
+        self.handle = h
+        self.solver = solver
+        self.n_times = n_times
+        self.n_baselines = len(a1)
+        self.channels_per_block = ch
+        self.bl_stride = lib().ddh_problem_baseline_stride(h)
+
+    @classmethod
+    def from_problem(cls, solver, problem):
+        """Structure of a SolveData-layout problem (dp3_amd.synth.SolveProblem)."""
+        n_t, n_bl = problem.n_times, problem.n_baselines
+        ch = np.diff(problem.channel_begin).astype(np.int32)
+        v0 = 0
+        nv0 = int(problem.vis_offsets[1])
+        a1 = problem.ant1[0:n_bl * ch[0]:ch[0]]
+        a2 = problem.ant2[0:n_bl * ch[0]:ch[0]]
+        n_dir = problem.n_directions
+        sm = problem.sol_map[:nv0 * n_dir].reshape(n_dir, n_t, n_bl * ch[0])[:, :, 0]
+        obj = cls(solver, n_t, a1, a2, ch, sm)
+        del v0
+        return obj
+
+    def upload(self, problem):
+        views = (CbView * problem.n_channel_blocks)()
+        keep = []
+        n_dir = problem.n_directions
+        for cb in range(problem.n_channel_blocks):
+            v0, v1 = int(problem.vis_offsets[cb]), int(problem.vis_offsets[cb + 1])
+            d = np.ascontiguousarray(problem.data[v0:v1])
+            m = np.ascontiguousarray(problem.model[v0 * n_dir:v1 * n_dir])
+            keep += [d, m]
+            views[cb] = CbView(v1 - v0, d.ctypes.data, m.ctypes.data, None, None)
+        _check(lib().ddh_problem_upload(self.handle, views), "ddh_problem_upload")
+
+    def data_ptr(self, cb):
+        return lib().ddh_problem_data_ptr(self.handle, cb)
+
+    def model_ptr(self, cb, d):
+        return lib().ddh_problem_model_ptr(self.handle, cb, d)
+
+    def set_model_scale(self, scale):
+        s = np.ascontiguousarray(scale, dtype=np.float32)
+        _check(lib().ddh_problem_set_model_scale(self.handle, _ptr(s)),
+               "ddh_problem_set_model_scale")
+
+    def download_plane(self, cb, plane, n_corr=2, f16=False):
+        rows = self.n_times * int(self.channels_per_block[cb])
+        if f16 and plane > 0:
+            out = np.zeros((rows, self.bl_stride, n_corr, 2), np.float16)
+        else:
+            out = np.zeros((rows, self.bl_stride, n_corr), np.complex64)
+        _check(lib().ddh_problem_download_plane(self.handle, cb, plane, _ptr(out)),
+               "ddh_problem_download_plane")
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().ddh_problem_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()

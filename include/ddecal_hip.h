@@ -1,0 +1,150 @@
+// C-ABI of the MI355X-native DDECal iterative-diagonal solver and
+// point/Gaussian predict (library: dp3_amd/libddecal_hip.so).
+//
+// Every entry point uses plain C types. Functions returning int return 0 on
+// success and a negative status on error; ddh_last_error() then holds the
+// message (thread-local). NaN solutions are data, not errors.
+//
+// Reference interfaces replaced (lofar-astron/DP3 6.6.0):
+//   ddh_solver_create   <- SolverFactory CreateDiagonalSolver
+//                          (ddecal/SolverFactory.cc:86-146) + SolverBase
+//                          setters (SolverBase.h:109-177) + Initialize
+//                          (SolverBase.cc:41-61)
+//   ddh_solve           <- IterativeDiagonalSolver<VisMatrix>::Solve
+//                          (ddecal/gain_solvers/IterativeDiagonalSolver.cc:38-97;
+//                          SolverBase.h:205-219), taking SolveData's channel-
+//                          block arrays (SolveData.h:86-92) and the
+//                          solutions vector<vector<complex<double>>>.
+//   ddh_problem_*       <- SolveData regular layout (SolveData.cc:77-188), kept
+//                          device resident in a baseline-fastest layout.
+//   ddh_solve_problem   <- same Solve, on a device-resident problem.
+//   ddh_solve_batch     <- DDECal::doSolve's loop over solution intervals
+//                          (steps/DDECal.cc:595-708), all intervals at once.
+//   ddh_predict_*       <- Simulator::visit(PointSource|GaussianSource)
+//                          (base/Simulator.cc:110-468) + OnePredict's Stokes-I
+//                          copy (steps/OnePredict.cc:437-457).
+#ifndef DDECAL_HIP_H
+#define DDECAL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DDH_OK 0
+#define DDH_ERR_ARGUMENT -1
+#define DDH_ERR_LAYOUT -2
+#define DDH_ERR_DEVICE -3
+#define DDH_ERR_UNSUPPORTED -4
+
+// Model visibility storage on the device.
+#define DDH_MODEL_C64 0   // complex64 (reference precision)
+#define DDH_MODEL_F16 1   // complex FP16 with a per (block, direction) scale
+
+typedef struct ddh_solver ddh_solver;
+typedef struct ddh_problem ddh_problem;
+
+typedef struct {
+  int32_t n_antennas;
+  int32_t n_directions;
+  int32_t n_channel_blocks;
+  int32_t n_correlations;                    // 2 (Duo, XX/YY) or 4 (Full)
+  const int32_t* n_solutions_per_direction;  // [n_directions]
+  int32_t max_iterations;                    // SolverBase default 100
+  int32_t min_iterations;                    // default 0
+  double accuracy;                           // default 1e-5
+  double constraint_accuracy;                // default 1e-4
+  double step_size;                          // default 0.2
+  int32_t detect_stalling;                   // default 1
+  double step_diff_sigma;                    // default 0.1
+  int32_t phase_only;                        // default 0
+  int32_t model_storage;                     // DDH_MODEL_*
+} ddh_config;
+
+// Fills the SolverBase defaults (SolverBase.cc:27-39).
+void ddh_config_defaults(ddh_config* config);
+
+typedef struct {
+  int64_t iterations;             // SolveResult::iterations (max+1 if not converged)
+  int64_t constraint_iterations;  // SolveResult::constraint_iterations
+  int32_t has_converged;
+  int32_t n_step_magnitudes;
+  double last_step_magnitude;
+} ddh_result;
+
+// One channel block of a DuoSolveData / FullSolveData (host memory,
+// SolveData::ChannelBlockData): visibilities ordered time -> cross
+// baseline -> channel in block (SolveData.cc:129-181).
+typedef struct {
+  int64_t n_visibilities;
+  const float* data;             // [n_vis][n_corr] complex64 (&Visibility(0))
+  const float* model;            // [n_dir][n_vis][n_corr] complex64 (&ModelVisibility(0,0))
+  const uint32_t* antenna_pairs; // [n_vis][2] (antenna_indices_)
+  const uint32_t* solution_map;  // [n_dir][n_vis] (SolutionMapData())
+} ddh_cb_view;
+
+const char* ddh_last_error(void);
+int ddh_device_count(void);
+
+ddh_solver* ddh_solver_create(const ddh_config* config);
+void ddh_solver_destroy(ddh_solver* solver);
+
+// Drop-in solve from host SolveData arrays. solutions[cb] points at the
+// channel block's [n_ant][n_sub][2] complex128 vector: initial guess in,
+// result out. `time` is passed to constraints in the reference; no
+// constraints are supported here, so it is unused.
+int ddh_solve(ddh_solver* solver, const ddh_cb_view* channel_blocks,
+              int32_t n_channel_blocks, double* const* solutions, double time,
+              ddh_result* result);
+
+// Device-resident problem in the library's own layout: per channel block,
+// planes of [time][channel][baseline_stride] visibilities, baseline fastest.
+// sol_index[d * n_times + t] is the absolute solution index of direction d at
+// time step t (SolveData.cc:167-168).
+ddh_problem* ddh_problem_create(const ddh_solver* solver, int32_t n_times,
+                                int32_t n_baselines, const int32_t* antenna1,
+                                const int32_t* antenna2,
+                                const int32_t* channels_per_block,
+                                const uint32_t* sol_index);
+void ddh_problem_destroy(ddh_problem* problem);
+// Copies host SolveData channel blocks (layout as in ddh_cb_view, with the
+// problem's structure) into the device layout.
+int ddh_problem_upload(ddh_problem* problem, const ddh_cb_view* channel_blocks);
+// Copies host visibilities already in the reference DPBuffer order per
+// block: data [n_t][n_bl][n_ch_cb][n_corr], model [n_dir][n_t][n_bl][n_ch_cb][n_corr].
+int ddh_problem_baseline_stride(const ddh_problem* problem);
+// Device pointers of the data plane / a model plane of block cb (for device
+// producers such as the predict). Model planes of FP16 problems hold
+// half2 pairs; their scale is set by ddh_problem_set_model_scale.
+void* ddh_problem_data_ptr(ddh_problem* problem, int32_t cb);
+void* ddh_problem_model_ptr(ddh_problem* problem, int32_t cb, int32_t direction);
+int ddh_problem_set_model_scale(ddh_problem* problem, const float* scale /*[n_cb][n_dir]*/);
+// Reads back a device plane into host memory in the device layout.
+int ddh_problem_download_plane(ddh_problem* problem, int32_t cb, int32_t plane,
+                               void* host);
+
+// Solve a device-resident problem. solutions as in ddh_solve.
+int ddh_solve_problem(ddh_solver* solver, ddh_problem* problem,
+                      double* const* solutions, ddh_result* result);
+
+// Kernel-level timing of the last solve: total device time of the
+// correlate kernel and number of its launches that did work.
+int ddh_solver_last_timing(const ddh_solver* solver, double* correlate_ms,
+                           int64_t* correlate_launches, double* solve_ms);
+
+// Schedule of the correlate kernel (waves per baseline tile, 1..4), hipGraph
+// replay of the iteration body, and how many iterations are enqueued between
+// convergence polls.
+int ddh_set_schedule(ddh_solver* solver, int32_t waves_per_tile,
+                     int32_t use_graph, int32_t poll_interval);
+// Records HIP events around every correlate launch of the next solves
+// (ddh_solver_last_timing then reports their summed duration).
+int ddh_set_profiling(ddh_solver* solver, int32_t enable);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

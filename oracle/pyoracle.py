@@ -1,0 +1,261 @@
+"""ctypes bindings for the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the bench's
+CPU-baseline leg, never by the product package. The C++ sources restate
+lofar-astron/DP3's solver and predict (see oracle.h for the file:line map).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class _Settings(C.Structure):
+    _fields_ = [
+        ("n_antennas", C.c_int32),
+        ("n_directions", C.c_int32),
+        ("n_channel_blocks", C.c_int32),
+        ("n_correlations", C.c_int32),
+        ("n_solutions_per_direction", C.POINTER(C.c_int32)),
+        ("max_iterations", C.c_int32),
+        ("min_iterations", C.c_int32),
+        ("accuracy", C.c_double),
+        ("constraint_accuracy", C.c_double),
+        ("step_size", C.c_double),
+        ("detect_stalling", C.c_int32),
+        ("step_diff_sigma", C.c_double),
+        ("phase_only", C.c_int32),
+        ("fp64", C.c_int32),
+        ("n_threads", C.c_int32),
+    ]
+
+
+class _Result(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64),
+        ("constraint_iterations", C.c_int64),
+        ("has_converged", C.c_int32),
+        ("n_step_magnitudes", C.c_int32),
+    ]
+
+
+class Component(C.Structure):
+    _fields_ = [
+        ("ra", C.c_double), ("dec", C.c_double),
+        ("I", C.c_double), ("Q", C.c_double), ("U", C.c_double), ("V", C.c_double),
+        ("ref_freq", C.c_double),
+        ("n_spectral_terms", C.c_int32),
+        ("spectral_terms", C.POINTER(C.c_double)),
+        ("log_si", C.c_int32),
+        ("has_rm", C.c_int32),
+        ("pol_fraction", C.c_double), ("pol_angle", C.c_double), ("rm", C.c_double),
+        ("is_gaussian", C.c_int32),
+        ("major_axis", C.c_double), ("minor_axis", C.c_double),
+        ("position_angle", C.c_double),
+        ("pa_is_absolute", C.c_int32),
+    ]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
+        L = C.CDLL(path)
+        P = C.c_void_p
+        L.orc_solve.argtypes = [C.POINTER(_Settings), P, P, P, P, P, P, P, C.POINTER(_Result), P]
+        L.orc_solve.restype = C.c_int
+        L.orc_iterate_once.argtypes = [C.POINTER(_Settings), P, P, P, P, P, P, P, P]
+        L.orc_iterate_once.restype = C.c_int
+        L.orc_chi2.argtypes = [C.c_int32] * 5 + [P] * 7
+        L.orc_chi2.restype = C.c_double
+        L.orc_pack_solve_data.argtypes = [C.c_int32] * 7 + [P] * 11
+        L.orc_pack_solve_data.restype = C.c_int
+        L.orc_assign_and_weight.argtypes = [C.c_int32] * 4 + [P] * 4
+        L.orc_assign_and_weight.restype = None
+        L.orc_radec2lmn.argtypes = [C.c_double] * 4 + [P]
+        L.orc_radec2lmn.restype = None
+        L.orc_stokes.argtypes = [C.POINTER(Component), C.c_double, P]
+        L.orc_stokes.restype = None
+        L.orc_simulate.argtypes = ([C.c_double, C.c_double, C.c_int32, C.c_int32, P, P,
+                                    C.c_int32, P, P, P, P, C.c_int32, C.c_int32,
+                                    C.c_int32, C.c_int32, P, P])
+        L.orc_simulate.restype = None
+        L.orc_solvertester_shape.argtypes = [P] * 5
+        L.orc_solvertester_fill.argtypes = [P] * 4
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+@dataclass
+class SolveResult:
+    iterations: int
+    constraint_iterations: int
+    has_converged: bool
+    step_magnitudes: np.ndarray = field(default_factory=lambda: np.zeros(0))
+
+
+def solve(problem, solutions, *, max_iterations=50, min_iterations=0, accuracy=1e-5,
+          constraint_accuracy=1e-4, step_size=0.2, detect_stalling=True,
+          step_diff_sigma=0.1, phase_only=False, fp64=False, n_threads=1):
+    """Runs the oracle solver in place on `solutions` ([n_cb, n_ant*n_sub*2] complex128).
+
+    `problem` is a dp3_amd.synth.SolveProblem (reference SolveData layout)."""
+    n_sol = np.ascontiguousarray(problem.n_solutions_per_direction, dtype=np.int32)
+    s = _Settings(problem.n_antennas, problem.n_directions, problem.n_channel_blocks,
+                  problem.n_correlations, n_sol.ctypes.data_as(C.POINTER(C.c_int32)),
+                  max_iterations, min_iterations, accuracy, constraint_accuracy,
+                  step_size, int(detect_stalling), step_diff_sigma, int(phase_only),
+                  int(fp64), n_threads)
+    assert solutions.dtype == np.complex128 and solutions.flags.c_contiguous
+    res = _Result()
+    mags = np.zeros(max_iterations + 2)
+    rc = lib().orc_solve(C.byref(s), _p(problem.vis_offsets), _p(problem.data),
+                         _p(problem.model), _p(problem.ant1), _p(problem.ant2),
+                         _p(problem.sol_map), _p(solutions), C.byref(res), _p(mags))
+    if rc != 0:
+        raise RuntimeError(f"orc_solve failed ({rc})")
+    return SolveResult(int(res.iterations), int(res.constraint_iterations),
+                       bool(res.has_converged), mags[: res.n_step_magnitudes].copy())
+
+
+def iterate_once(problem, solutions, *, fp64=False, n_threads=1):
+    n_sol = np.ascontiguousarray(problem.n_solutions_per_direction, dtype=np.int32)
+    s = _Settings(problem.n_antennas, problem.n_directions, problem.n_channel_blocks,
+                  problem.n_correlations, n_sol.ctypes.data_as(C.POINTER(C.c_int32)),
+                  1, 0, 0.0, 0.0, 0.2, 0, 0.1, 0, int(fp64), n_threads)
+    sol = np.ascontiguousarray(solutions, dtype=np.complex128)
+    nxt = np.zeros_like(sol)
+    rc = lib().orc_iterate_once(C.byref(s), _p(problem.vis_offsets), _p(problem.data),
+                                _p(problem.model), _p(problem.ant1), _p(problem.ant2),
+                                _p(problem.sol_map), _p(sol), _p(nxt))
+    if rc != 0:
+        raise RuntimeError(f"orc_iterate_once failed ({rc})")
+    return nxt
+
+
+def chi2(problem, solutions):
+    sol = np.ascontiguousarray(solutions, dtype=np.complex128)
+    return lib().orc_chi2(problem.n_antennas, problem.n_directions,
+                          problem.n_channel_blocks, problem.n_correlations,
+                          int(np.sum(problem.n_solutions_per_direction)),
+                          _p(problem.vis_offsets), _p(problem.data), _p(problem.model),
+                          _p(problem.ant1), _p(problem.ant2), _p(problem.sol_map), _p(sol))
+
+
+def pack_solve_data(data, model, antennas1, antennas2, n_ant, n_cb, n_sol_per_dir, n_corr=2):
+    """SolveData regular constructor. data [n_t, n_bl, n_ch, 4] complex64,
+    model [n_dir, n_t, n_bl, n_ch, 4] complex64. Returns dict of packed arrays."""
+    data = np.ascontiguousarray(data, dtype=np.complex64)
+    model = np.ascontiguousarray(model, dtype=np.complex64)
+    n_t, n_bl, n_ch, _ = data.shape
+    n_dir = model.shape[0]
+    a1 = np.ascontiguousarray(antennas1, dtype=np.int32)
+    a2 = np.ascontiguousarray(antennas2, dtype=np.int32)
+    ns = np.ascontiguousarray(n_sol_per_dir, dtype=np.int32)
+    vo = np.zeros(n_cb + 1, dtype=np.int64)
+    L = lib()
+    rc = L.orc_pack_solve_data(n_t, n_bl, n_ch, n_dir, n_cb, n_corr, n_ant, _p(ns), _p(a1),
+                               _p(a2), None, None, _p(vo), None, None, None, None, None)
+    if rc != 0:
+        raise RuntimeError(f"orc_pack_solve_data failed ({rc})")
+    nv = int(vo[-1])
+    out_data = np.zeros((nv, n_corr), np.complex64)
+    out_model = np.zeros((nv * n_dir, n_corr), np.complex64)
+    o1 = np.zeros(nv, np.uint32)
+    o2 = np.zeros(nv, np.uint32)
+    osm = np.zeros(nv * n_dir, np.uint32)
+    rc = L.orc_pack_solve_data(n_t, n_bl, n_ch, n_dir, n_cb, n_corr, n_ant, _p(ns), _p(a1),
+                               _p(a2), _p(data), _p(model), _p(vo), _p(out_data),
+                               _p(out_model), _p(o1), _p(o2), _p(osm))
+    if rc != 0:
+        raise RuntimeError(f"orc_pack_solve_data failed ({rc})")
+    return dict(vis_offsets=vo, data=out_data, model=out_model, ant1=o1, ant2=o2,
+                sol_map=osm)
+
+
+def assign_and_weight(data, flags, weights, model):
+    """In place. data [n_t,n_bl,n_ch,4] c64, flags bool same, weights f32 same,
+    model [n_dir,n_t,n_bl,n_ch,4] c64."""
+    assert data.dtype == np.complex64 and data.flags.c_contiguous
+    assert model.dtype == np.complex64 and model.flags.c_contiguous
+    n_t, n_bl, n_ch, _ = data.shape
+    f = np.ascontiguousarray(flags, dtype=np.uint8)
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    lib().orc_assign_and_weight(n_t, n_bl, n_ch, model.shape[0], _p(data), _p(f), _p(w),
+                                _p(model))
+
+
+def radec2lmn(ref, direction):
+    out = np.zeros(3)
+    lib().orc_radec2lmn(ref[0], ref[1], direction[0], direction[1], _p(out))
+    return out
+
+
+def make_component(ra, dec, I, Q=0.0, U=0.0, V=0.0, *, gaussian=False, major=0.0,
+                   minor=0.0, pa=0.0, pa_absolute=False, spectral_terms=None,
+                   ref_freq=0.0, log_si=True, rm=None):
+    terms = np.ascontiguousarray(spectral_terms if spectral_terms is not None else [],
+                                 dtype=np.float64)
+    c = Component(ra, dec, I, Q, U, V, ref_freq, len(terms),
+                  terms.ctypes.data_as(C.POINTER(C.c_double)), int(log_si),
+                  int(rm is not None), *(rm if rm is not None else (0.0, 0.0, 0.0)),
+                  int(gaussian), major, minor, pa, int(pa_absolute))
+    c._keep = terms  # keep the coefficient array alive
+    return c
+
+
+def stokes(component, freq):
+    out = np.zeros(4)
+    lib().orc_stokes(C.byref(component), freq, _p(out))
+    return out
+
+
+def simulate(ref, station_uvw, baselines, freq, chan_width, components, *,
+             stokes_i_only=True, correct_freq_smearing=False,
+             correct_time_smearing=False, scaled_ncp_uvw=None):
+    """Returns buffer [n_bl, n_ch, n_corr] complex128 (n_corr 1 or 4)."""
+    uvw = np.ascontiguousarray(station_uvw, dtype=np.float64)
+    bl = np.asarray(baselines, dtype=np.int32)
+    p = np.ascontiguousarray(bl[:, 0])
+    q = np.ascontiguousarray(bl[:, 1])
+    f = np.ascontiguousarray(freq, dtype=np.float64)
+    w = np.ascontiguousarray(chan_width, dtype=np.float64)
+    ncp = None if scaled_ncp_uvw is None else np.ascontiguousarray(scaled_ncp_uvw, np.float64)
+    n_corr = 1 if stokes_i_only else 4
+    buf = np.zeros((len(bl), len(f), n_corr), np.complex128)
+    arr = (Component * len(components))(*components)
+    lib().orc_simulate(ref[0], ref[1], uvw.shape[0], len(bl), _p(p), _p(q), len(f), _p(f),
+                       _p(w), _p(uvw), _p(ncp), int(correct_time_smearing),
+                       int(correct_freq_smearing), int(stokes_i_only), len(components),
+                       C.cast(arr, C.c_void_p), _p(buf))
+    return buf
+
+
+def solvertester_shape():
+    v = [C.c_int32() for _ in range(5)]
+    lib().orc_solvertester_shape(*[C.byref(x) for x in v])
+    return tuple(int(x.value) for x in v)  # n_ant, n_dir, n_ch, n_t, n_cb
+
+
+def solvertester_fill(n_sol_per_dir):
+    n_ant, n_dir, n_ch, n_t, n_cb = solvertester_shape()
+    n_bl = n_ant * (n_ant - 1) // 2
+    ns = np.ascontiguousarray(n_sol_per_dir, dtype=np.int32)
+    n_sub = int(ns.sum())
+    data = np.zeros((n_t, n_bl, n_ch, 4), np.complex64)
+    model = np.zeros((n_dir, n_t, n_bl, n_ch, 4), np.complex64)
+    inp = np.zeros(n_ant * n_sub * 2, np.complex128)
+    lib().orc_solvertester_fill(_p(ns), _p(data), _p(model), _p(inp))
+    return data, model, inp

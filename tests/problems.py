@@ -1,0 +1,74 @@
+"""Seeded test problems shared by the oracle and GPU tests.
+
+Model visibilities here come from the CPU oracle's predict (oracle/), which
+is test infrastructure; the GPU tests feed the same inputs to the HIP path.
+"""
+import numpy as np
+
+from dp3_amd import synth as S
+from oracle import pyoracle as O
+
+
+def oracle_components(sky, d):
+    return [O.make_component(sky.ra[i], sky.dec[i], sky.flux[i],
+                             gaussian=bool(sky.is_gaussian[i]), major=sky.major[i],
+                             minor=sky.minor[i], pa=sky.pa[i])
+            for i in np.where(sky.direction == d)[0]]
+
+
+def oracle_model(w, geom, sky):
+    """[n_dir, n_t, n_bl, n_ch] complex128 Stokes-I model visibilities."""
+    n_t = geom.uvw.shape[0]
+    out = np.zeros((w.n_directions, n_t, len(geom.baselines), len(geom.freq)), np.complex128)
+    for d in range(w.n_directions):
+        comps = oracle_components(sky, d)
+        for t in range(n_t):
+            out[d, t] = O.simulate(geom.ref, geom.uvw[t], geom.baselines, geom.freq,
+                                   geom.chan_width, comps)[:, :, 0]
+    return out
+
+
+def small_problem(w=S.SMALL, n_corr=2, noise_sigma=None, seed=None):
+    """Returns (workload, geometry, sky, true gains, SolveProblem)."""
+    rng = np.random.default_rng(w.seed if seed is None else seed)
+    geom = S.make_geometry(w, rng)
+    sky = S.make_sky(w, geom, rng)
+    model = oracle_model(w, geom, sky)
+    gains = S.true_gains(w, rng)
+    P = S.assemble(w, geom, model, gains, rng, noise_sigma=noise_sigma, n_corr=n_corr)
+    return w, geom, sky, gains, P
+
+
+def solvertester_problem(n_sol_per_dir=(1, 2, 5), n_corr=2):
+    """SolverTester::FillDdIntervalData in SolveData layout + input solutions."""
+    n_ant, n_dir, n_ch, n_t, n_cb = O.solvertester_shape()
+    data, model, inp = O.solvertester_fill(n_sol_per_dir)
+    a1 = [a for a in range(n_ant) for b in range(a + 1, n_ant)]
+    a2 = [b for a in range(n_ant) for b in range(a + 1, n_ant)]
+    pk = O.pack_solve_data(data, model, a1, a2, n_ant, n_cb, n_sol_per_dir, n_corr=n_corr)
+    P = S.problem_from_solvertester(data, model, n_sol_per_dir, n_ant, n_cb, pk)
+    P.channel_begin = S.channel_begin(n_ch, n_cb)
+    return P, inp
+
+
+def check_dual_results(solutions, inp, n_ant, n_sub, n_cb, tolerance_percent):
+    """SolverTester::CheckDualResults (SolverTester.cc:409-439): |g|^2 and the
+    real part of g * conj(g_ant0) within `tolerance_percent` (BOOST_CHECK_CLOSE,
+    a relative tolerance in percent). Returns the worst relative error."""
+    s = solutions.reshape(n_cb, n_ant, n_sub, 2)
+    i = inp.reshape(n_ant, n_sub, 2)
+    worst = 0.0
+    for a, b in [(np.abs(s) ** 2, np.broadcast_to(np.abs(i) ** 2, s.shape)),
+                 ((s * np.conj(s[:, 0:1])).real,
+                  np.broadcast_to((i * np.conj(i[0:1])).real, s.shape))]:
+        rel = np.abs(a - b) / np.maximum(np.abs(a), np.abs(b))
+        worst = max(worst, float(rel.max()))
+    assert worst * 100.0 <= tolerance_percent, f"worst relative error {worst:.3e}"
+    return worst
+
+
+def phase_ref_relerr(sol_a, sol_b, n_ant, n_sub):
+    """Max |a_ref - b_ref| / |b_ref| after referencing both to antenna 0."""
+    ra = S.phase_referenced(sol_a, n_ant, n_sub)
+    rb = S.phase_referenced(sol_b, n_ant, n_sub)
+    return float(np.max(np.abs(ra - rb) / np.abs(rb)))

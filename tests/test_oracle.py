@@ -1,0 +1,319 @@
+"""Pins the CPU oracle against the reference's own known answers and against
+what the mathematics fixes, so that the oracle can judge the HIP path.
+
+  * predict:  tSimulator.cc known answers (|V| = 1, frequency smearing
+              0.759154, time smearing 0.893578 / 0.891990, radec2lmn closed
+              forms) and the Gaussian's analytic Fourier transform checked
+              by a brute-force numerical DFT over an (l, m) grid;
+  * solver:   SolverTester closed-form recovery (tSolvers.cc:280-310 with
+              CheckDualResults, SolverTester.cc:409-439), the stationarity of
+              chi^2 at the converged fixed point (finite differences of an
+              independent chi^2), exact invariance of noiseless true gains,
+              and the reference's iteration-count semantics;
+  * layout:   SolveData packing order, auto-correlation removal and block
+              edges (tSolveData.cc); AssignAndWeight NaN / flag handling
+              (tSolverTools.cc:127-213).
+"""
+import numpy as np
+import pytest
+
+from dp3_amd import synth as S
+from oracle import pyoracle as O
+from problems import check_dual_results, oracle_model, small_problem, solvertester_problem
+
+
+# ---------------------------------------------------------------- predict --
+REF = (0.5, 0.1)
+SRC = (REF[0] + 0.02, REF[1] + 0.02)
+N_ST = 4
+
+
+def _tsim_setup():
+    bls = [(a, b) for a in range(N_ST - 1) for b in range(a + 1, N_ST)]
+    uvw = np.array([[st * 5000.0, st * 1000.0, 0.0] for st in range(N_ST)])
+    freq = np.array([130e6, 131e6])
+    return bls, uvw, freq, np.full(2, 1e6)
+
+
+def test_radec2lmn_closed_forms():
+    # tSimulator.cc:157-197
+    np.testing.assert_allclose(O.radec2lmn((0, np.pi / 2), (0, np.pi / 2)), [0, 0, 1], atol=1e-6)
+    np.testing.assert_allclose(O.radec2lmn((0, np.pi / 2), (0, -np.pi / 2)), [0, 0, -1], atol=1e-6)
+    np.testing.assert_allclose(O.radec2lmn((0, np.pi / 2), (0, np.pi / 4)),
+                               [0, -np.sqrt(0.5), np.sqrt(0.5)], atol=1e-6)
+    np.testing.assert_allclose(O.radec2lmn((0, np.pi / 2), (0, -np.pi / 4)),
+                               [0, -np.sqrt(0.5), -np.sqrt(0.5)], atol=1e-6)
+    # unit norm for an arbitrary direction
+    lmn = O.radec2lmn((1.1, 0.4), (1.3, 0.7))
+    assert abs(np.dot(lmn, lmn) - 1.0) < 1e-12
+
+
+def test_simulator_known_answers():
+    bls, uvw, freq, width = _tsim_setup()
+    c = O.make_component(SRC[0], SRC[1], 1.0)
+    b = O.simulate(REF, uvw, bls, freq, width, [c])
+    np.testing.assert_allclose(np.abs(b[0, :, 0]), 1.0, rtol=1e-5)  # :80-81
+    b = O.simulate(REF, uvw, bls, freq, width, [c], stokes_i_only=False)
+    np.testing.assert_allclose(np.abs(b[0]), [[1, 0, 0, 1], [1, 0, 0, 1]], atol=1e-5)  # :101-108
+    b = O.simulate(REF, uvw, bls, freq, width, [c], correct_freq_smearing=True)
+    np.testing.assert_allclose(np.abs(b[N_ST - 1, :, 0]), 0.759154, rtol=1e-5)  # :129-130
+    b = O.simulate(REF, uvw, bls, freq, width, [c], correct_time_smearing=True,
+                   scaled_ncp_uvw=[0.0, 0.3, 0.0])
+    np.testing.assert_allclose(np.abs(b[N_ST - 1, :, 0]), [0.893578, 0.891990], rtol=1e-5)  # :153-154
+
+
+def test_point_source_phase_convention():
+    """V_pq = S exp(+2 pi i f/c ((u_q - u_p) l + (v_q - v_p) m + (w_q - w_p)(n - 1)))."""
+    bls, uvw, freq, width = _tsim_setup()
+    c = O.make_component(SRC[0], SRC[1], 2.5)
+    b = O.simulate(REF, uvw, bls, freq, width, [c])
+    l, m, n = O.radec2lmn(REF, SRC)
+    for i, (p, q) in enumerate(bls):
+        duvw = uvw[q] - uvw[p]
+        ph = 2 * np.pi * freq / 299792458.0 * (duvw[0] * l + duvw[1] * m + duvw[2] * (n - 1))
+        np.testing.assert_allclose(b[i, :, 0], 2.5 * np.exp(1j * ph), rtol=1e-12, atol=1e-12)
+
+
+def test_stokes_spectra():
+    c = O.make_component(0, 0, 2.0, spectral_terms=[-0.7, 0.1], ref_freq=150e6, log_si=True)
+    f = 120e6
+    x = np.log10(f / 150e6)
+    assert O.stokes(c, f)[0] == pytest.approx(2.0 * 10 ** (x * (-0.7 + 0.1 * x)), rel=1e-12)
+    c = O.make_component(0, 0, 2.0, spectral_terms=[0.5, 0.2], ref_freq=150e6, log_si=False)
+    x = f / 150e6 - 1
+    assert O.stokes(c, f)[0] == pytest.approx(2.0 + (0.5 + 0.2 * x) * x, rel=1e-12)
+    c = O.make_component(0, 0, 2.0, rm=(0.3, 0.2, 5.0))
+    lam = 299792458.0 / f
+    chi = 2 * (0.2 + 5.0 * lam * lam)
+    np.testing.assert_allclose(O.stokes(c, f)[1:3], [0.6 * np.cos(chi), 0.6 * np.sin(chi)])
+
+
+@pytest.mark.parametrize("pa", [0.0, 0.6, 2.2])
+def test_gaussian_matches_numerical_dft(pa):
+    """Brute-force sky integral of a Gaussian of unit integrated flux, major
+    axis at position angle `pa` (north through east), against the predict's
+    closed form. Source at the phase centre, w = 0, so only the envelope and
+    its orientation are tested."""
+    ref = (0.3, 0.8)
+    maj, mino = 4e-3, 1.5e-3  # FWHM (rad)
+    sig_maj = maj / (2 * np.sqrt(2 * np.log(2)))
+    sig_min = mino / (2 * np.sqrt(2 * np.log(2)))
+    # sky grid
+    h = 5e-5
+    g = np.arange(-6 * sig_maj, 6 * sig_maj + h / 2, h)
+    L, M = np.meshgrid(g, g, indexing="ij")
+    e_maj = np.array([np.sin(pa), np.cos(pa)])  # (l, m) direction of the major axis
+    x_maj = L * e_maj[0] + M * e_maj[1]
+    x_min = L * e_maj[1] - M * e_maj[0]
+    sky = np.exp(-0.5 * (x_maj / sig_maj) ** 2 - 0.5 * (x_min / sig_min) ** 2)
+    sky /= sky.sum()
+    uvw = np.array([[0.0, 0.0, 0.0], [60.0, 10.0, 0.0], [-25.0, 70.0, 0.0], [40.0, -55.0, 0.0]])
+    bls = [(0, 1), (0, 2), (0, 3), (1, 2)]
+    freq = np.array([150e6])
+    c = O.make_component(ref[0], ref[1], 1.0, gaussian=True, major=maj, minor=mino, pa=pa)
+    b = O.simulate(ref, uvw, bls, freq, np.ones(1), [c])[:, 0, 0]
+    lam = 299792458.0 / freq[0]
+    for i, (p, q) in enumerate(bls):
+        u, v = (uvw[q, :2] - uvw[p, :2]) / lam
+        brute = np.sum(sky * np.exp(2j * np.pi * (u * L + v * M)))
+        assert abs(b[i] - brute) < 2e-5, (i, b[i], brute)
+
+
+# ----------------------------------------------------------------- layout --
+def test_pack_solve_data_order_and_autos():
+    """tSolveData.cc regular case: 3 baselines, the last an auto-correlation;
+    7 channels in 2 blocks -> block sizes 3 and 4; order time -> baseline ->
+    channel; solution map t * n_sol / n_t + start (SolveData.cc:167-168)."""
+    n_t, n_ch, n_cb = 2, 7, 2
+    a1, a2 = [0, 0, 1], [1, 2, 1]
+    n_bl = 3
+    rng = np.random.default_rng(3)
+    data = (rng.normal(size=(n_t, n_bl, n_ch, 4)) + 1j * rng.normal(size=(n_t, n_bl, n_ch, 4))).astype(np.complex64)
+    model = (rng.normal(size=(2, n_t, n_bl, n_ch, 4)) + 1j * rng.normal(size=(2, n_t, n_bl, n_ch, 4))).astype(np.complex64)
+    for n_corr in (2, 4):
+        pk = O.pack_solve_data(data, model, a1, a2, 3, n_cb, [1, 2], n_corr=n_corr)
+        vo = pk["vis_offsets"]
+        np.testing.assert_array_equal(np.diff(vo), [n_t * 2 * 3, n_t * 2 * 4])
+        corr = [0, 3] if n_corr == 2 else [0, 1, 2, 3]
+        for cb, (c0, c1) in enumerate([(0, 3), (3, 7)]):
+            nc = c1 - c0
+            v0, nv = vo[cb], vo[cb + 1] - vo[cb]
+            for v in range(nv):
+                t = v // (2 * nc)
+                bl = (v // nc) % 2
+                ch = c0 + v % nc
+                np.testing.assert_array_equal(pk["data"][v0 + v], data[t, bl, ch, corr])
+                assert (pk["ant1"][v0 + v], pk["ant2"][v0 + v]) == (a1[bl], a2[bl])
+                for d in range(2):
+                    np.testing.assert_array_equal(pk["model"][v0 * 2 + d * nv + v],
+                                                  model[d, t, bl, ch, corr])
+                    expect = [0, 1 + t * 2 // n_t][d]
+                    assert pk["sol_map"][v0 * 2 + d * nv + v] == expect
+
+
+def test_assign_and_weight_nan_and_flags():
+    """tSolverTools.cc:127-213: one flag per (bl, ch) over correlations, data
+    NaN, model NaN or input flag zero data and every model; others are
+    multiplied by sqrt(weight)."""
+    n_t, n_bl, n_ch = 3, 1, 2
+    data = np.full((n_t, n_bl, n_ch, 4), 2 + 1j, np.complex64)
+    model = np.full((2, n_t, n_bl, n_ch, 4), 1 - 3j, np.complex64)
+    weights = np.full((n_t, n_bl, n_ch, 4), 4.0, np.float32)
+    weights[:, :, :, 2] = 9.0
+    flags = np.zeros((n_t, n_bl, n_ch, 4), bool)
+    data[0, 0, 0, 3] = np.complex64(complex(np.nan, 0))   # data NaN, first channel
+    model[0, 1, 0, 1, 0] = np.complex64(complex(0, np.nan))  # model NaN, last channel
+    flags[2, 0, 0, 1] = True                               # input flag
+    O.assign_and_weight(data, flags, weights, model)
+    sw = np.sqrt(weights)
+    expect_zero = np.zeros((n_t, n_bl, n_ch), bool)
+    expect_zero[0, 0, 0] = expect_zero[1, 0, 1] = expect_zero[2, 0, 0] = True
+    for t in range(n_t):
+        for ch in range(n_ch):
+            if expect_zero[t, 0, ch]:
+                assert np.all(data[t, 0, ch] == 0) and np.all(model[:, t, 0, ch] == 0)
+            else:
+                np.testing.assert_allclose(data[t, 0, ch], (2 + 1j) * sw[t, 0, ch])
+                np.testing.assert_allclose(model[:, t, 0, ch], np.broadcast_to((1 - 3j) * sw[t, 0, ch], (2, 4)))
+
+
+# ----------------------------------------------------------------- solver --
+@pytest.mark.parametrize("n_corr", [2, 4])
+def test_solvertester_recovery(n_corr):
+    """iterative_[duo_]diagonal_dd_intervals (tSolvers.cc:280-310): from unit
+    start, 100 iterations at accuracy 1e-8 recover the generating gains within
+    CheckDualResults(1e-2)."""
+    P, inp = solvertester_problem((1, 2, 5), n_corr=n_corr)
+    sol = P.unit_solutions()
+    r = O.solve(P, sol, max_iterations=100, accuracy=1e-8, step_size=0.2, n_threads=8)
+    assert r.iterations <= 101
+    check_dual_results(sol, inp, P.n_antennas, P.n_sub, P.n_channel_blocks, 1e-2)
+
+
+def _tiny(n_corr=2, nsd=None, noise=0.05, seed=11):
+    w = S.Workload("tiny", 7, 2, 1, 1, 6, 2, 3, 10, 1e-10, n_solutions_per_direction=nsd)
+    return small_problem(w, n_corr=n_corr, noise_sigma=noise, seed=seed)
+
+
+@pytest.mark.parametrize("n_corr,nsd", [(2, None), (4, None), (2, (1, 3))])
+def test_fixed_point_is_stationary_point_of_chi2(n_corr, nsd):
+    """At the converged fixed point every gain satisfies its least-squares
+    normal equation, so the gradient of chi^2 vanishes. chi^2 is an
+    independent FP64 function (orc_chi2); its gradient is taken by central
+    differences. A dropped conjugate or a swapped antenna in the update moves
+    the fixed point off the stationary point and fails this."""
+    w, geom, sky, gains, P = _tiny(n_corr, nsd)
+    sol = gains.copy()  # start near the solution to converge tightly
+    r = O.solve(P, sol, max_iterations=4000, accuracy=1e-13, fp64=True, detect_stalling=False)
+    assert r.has_converged
+    c0 = O.chi2(P, sol)
+    h = 1e-6
+    flat = sol.reshape(-1)
+    grad = np.zeros(flat.size, complex)
+    for i in range(flat.size):
+        for part, unit in ((0, 1.0), (1, 1j)):
+            flat[i] += h * unit
+            cp = O.chi2(P, sol)
+            flat[i] -= 2 * h * unit
+            cm = O.chi2(P, sol)
+            flat[i] += h * unit
+            if part == 0:
+                grad[i] += (cp - cm) / (2 * h)
+            else:
+                grad[i] += 1j * (cp - cm) / (2 * h)
+    # scale: gradient of chi^2 at the unit start
+    unit = P.unit_solutions().reshape(-1)
+    g_scale = 0.0
+    for i in range(0, unit.size, 7):
+        unit[i] += h
+        cp = O.chi2(P, unit.reshape(sol.shape))
+        unit[i] -= 2 * h
+        cm = O.chi2(P, unit.reshape(sol.shape))
+        unit[i] += h
+        g_scale = max(g_scale, abs(cp - cm) / (2 * h))
+    assert np.max(np.abs(grad)) < 1e-5 * g_scale, (np.max(np.abs(grad)), g_scale, c0)
+
+
+@pytest.mark.parametrize("n_corr", [2, 4])
+def test_noiseless_true_gains_are_a_fixed_point(n_corr):
+    w, geom, sky, gains, P = _tiny(n_corr, noise=0.0)
+    assert O.chi2(P, gains) < 1e-9 * np.sum(np.abs(P.data) ** 2)
+    # data and model are complex64, so "exact" holds to float rounding of the inputs
+    nxt = O.iterate_once(P, gains, fp64=True)
+    np.testing.assert_allclose(nxt, gains, rtol=1e-6, atol=1e-6)
+    # the reference precision mix keeps it to float rounding
+    nxt32 = O.iterate_once(P, gains)
+    np.testing.assert_allclose(nxt32, gains, rtol=1e-4, atol=1e-4)
+
+
+def test_single_direction_update_is_per_antenna_least_squares():
+    """One direction, from a given start: the next solution of antenna a is
+    the least-squares scale x minimising sum_b |D_ab - x M_ab conj(g_b)|^2
+    (and its mirror for the second antenna), solved here with numpy lstsq on
+    the explicit per-antenna system."""
+    w = S.Workload("1dir", 5, 1, 1, 0, 4, 1, 2, 10, 1e-10)
+    _, geom, sky, gains, P = small_problem(w, noise_sigma=0.02, seed=5)
+    rng = np.random.default_rng(2)
+    start = (1 + 0.2 * rng.normal(size=gains.shape) + 0.2j * rng.normal(size=gains.shape))
+    nxt = O.iterate_once(P, start, fp64=True)
+    g = start.reshape(P.n_antennas, 1, 2)
+    D = P.data.astype(np.complex128)
+    M = P.model.astype(np.complex128)
+    for a in range(P.n_antennas):
+        for p in range(2):
+            rows, rhs = [], []
+            for v in range(len(D)):
+                i, j = P.ant1[v], P.ant2[v]
+                if i == a:
+                    rows.append(M[v, p] * np.conj(g[j, 0, p]))
+                    rhs.append(D[v, p])
+                elif j == a:
+                    # D_v = g_i M conj(x)  ->  conj(D_v) = x conj(g_i M)
+                    rows.append(np.conj(g[i, 0, p] * M[v, p]))
+                    rhs.append(np.conj(D[v, p]))
+            x = np.linalg.lstsq(np.array(rows)[:, None], np.array(rhs), rcond=None)[0][0]
+            assert abs(nxt.reshape(P.n_antennas, 1, 2)[a, 0, p] - x) < 1e-10 * abs(x)
+
+
+def test_iteration_count_semantics():
+    w, geom, sky, gains, P = small_problem()
+    sol = P.unit_solutions()
+    assert O.solve(P, sol, max_iterations=50, min_iterations=10, accuracy=1e8).iterations == 10
+    sol = P.unit_solutions()
+    r = O.solve(P, sol, max_iterations=7, accuracy=1e-12)
+    assert r.iterations == 8 and not r.has_converged
+    # converged: iterations equals the number of iterations run
+    sol = gains.copy()
+    r = O.solve(P, sol, max_iterations=200, accuracy=1e-3)
+    assert r.has_converged and r.iterations == len(r.step_magnitudes)
+
+
+def test_fp32_oracle_tracks_fp64_oracle():
+    w, geom, sky, gains, P = small_problem()
+    a, b = P.unit_solutions(), P.unit_solutions()
+    O.solve(P, a, max_iterations=w.max_iterations, accuracy=w.tolerance)
+    O.solve(P, b, max_iterations=w.max_iterations, accuracy=w.tolerance, fp64=True)
+    ra = S.phase_referenced(a, P.n_antennas, P.n_sub)
+    rb = S.phase_referenced(b, P.n_antennas, P.n_sub)
+    assert np.max(np.abs(ra - rb) / np.abs(rb)) < 1e-4
+
+
+def test_threaded_oracle_matches_single_thread():
+    w, geom, sky, gains, P = small_problem()
+    a, b = P.unit_solutions(), P.unit_solutions()
+    O.solve(P, a, max_iterations=20, accuracy=w.tolerance, n_threads=1)
+    O.solve(P, b, max_iterations=20, accuracy=w.tolerance, n_threads=13)
+    ra = S.phase_referenced(a, P.n_antennas, P.n_sub)
+    rb = S.phase_referenced(b, P.n_antennas, P.n_sub)
+    # visibility ranges are summed separately: equal to float rounding
+    assert np.max(np.abs(ra - rb) / np.abs(rb)) < 1e-4
+
+
+def test_synthetic_problem_is_consistent():
+    """chi^2 of the true gains equals the injected noise power: data really is
+    sum_d G_a M_d G_b^H + noise in the layout the solver reads."""
+    w, geom, sky, gains, P = small_problem()
+    expect = P.data.size * 2 * w.noise_sigma ** 2
+    assert abs(O.chi2(P, gains) - expect) < 0.05 * expect
+    w2, _, _, g2, P2 = small_problem(noise_sigma=0.0)
+    assert O.chi2(P2, g2) < 1e-8 * np.sum(np.abs(P2.data) ** 2)
