@@ -90,15 +90,19 @@ struct ddh_solver {
   ddh::SolverConfig cfg;
   hipStream_t stream = nullptr;
   int waves = 4;
+  int unroll = 1;
   int use_graph = 0;
   int poll = 8;
   // per-solve device buffers
-  ddh::DeviceBuffer<double2> sol, next;
-  ddh::DeviceBuffer<float2> gains;
-  ddh::DeviceBuffer<double> conv, step_mags;
+  ddh::DeviceBuffer<double2> sol0, sol1;
+  ddh::DeviceBuffer<double> mean, wg_conv, wg_mean, step_mags;
+  ddh::DeviceBuffer<int32_t> wg_conv_n, wg_mean_n;
   ddh::DeviceBuffer<float> partials;
   ddh::DeviceBuffer<ddh::DeviceState> state;
   ddh::DeviceBuffer<int32_t> n_sol_dir, sol_start;
+  // captured iteration body (poll iterations), valid for graph_key
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<const void*> graph_key;
   ddh::DeviceState* host_state = nullptr;  // pinned
   // timing of the last solve
   std::vector<hipEvent_t> events;
@@ -348,10 +352,13 @@ IterArgs MakeArgs(ddh_solver& s, ddh_problem& p) {
   a.seg_sol = p.d_seg_sol.ptr;
   a.n_sol_dir = s.n_sol_dir.ptr;
   a.sol_start = s.sol_start.ptr;
-  a.sol = s.sol.ptr;
-  a.next = s.next.ptr;
-  a.gains = s.gains.ptr;
-  a.conv = s.conv.ptr;
+  a.sol0 = s.sol0.ptr;
+  a.sol1 = s.sol1.ptr;
+  a.mean = s.mean.ptr;
+  a.wg_conv = s.wg_conv.ptr;
+  a.wg_conv_n = s.wg_conv_n.ptr;
+  a.wg_mean = s.wg_mean.ptr;
+  a.wg_mean_n = s.wg_mean_n.ptr;
   a.partials = s.partials.ptr;
   a.state = s.state.ptr;
   a.step_mags = s.step_mags.ptr;
@@ -370,7 +377,43 @@ IterArgs MakeArgs(ddh_solver& s, ddh_problem& p) {
   a.step_diff_sigma = c.step_diff_sigma;
   a.detect_stalling = c.detect_stalling;
   a.phase_only = c.phase_only;
+  int max_nsd = 1;
+  for (int v : s.cfg.n_sol_dir) max_nsd = std::max(max_nsd, v);
+  a.split = 1;
+  while (a.split < 8 && c.n_antennas * max_nsd * a.split * 2 <= 1024) a.split *= 2;
+  a.max_nsd = max_nsd;
   return a;
+}
+
+void DestroyGraph(ddh_solver& s) {
+  if (s.graph_exec) (void)hipGraphExecDestroy(s.graph_exec);
+  s.graph_exec = nullptr;
+  s.graph_key.clear();
+}
+
+void LaunchIteration(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
+  LaunchCorrelate(a, p.n_corr, p.f16, s.waves, s.unroll, s.stream);
+  LaunchAntennaSolve(a, p.n_corr, p.f16, s.stream);
+  LaunchDecide(a, s.stream);
+}
+
+// Captures `poll` iterations into one graph; reused while the problem and
+// the solver buffers stay the same.
+void EnsureGraph(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
+  const std::vector<const void*> key = {&p, p.data.ptr, p.model.ptr, s.sol0.ptr, s.partials.ptr,
+                                        reinterpret_cast<const void*>(intptr_t(s.poll)),
+                                        reinterpret_cast<const void*>(intptr_t(s.waves)),
+                                        reinterpret_cast<const void*>(intptr_t(s.unroll))};
+  if (s.graph_exec && key == s.graph_key) return;
+  DestroyGraph(s);
+  hipGraph_t graph = nullptr;
+  DDH_HIP_CHECK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < s.poll; ++i) LaunchIteration(s, a, p);
+  DDH_HIP_CHECK(hipStreamEndCapture(s.stream, &graph));
+  const hipError_t e = hipGraphInstantiate(&s.graph_exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  DDH_HIP_CHECK(e);
+  s.graph_key = key;
 }
 
 void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* result) {
@@ -382,19 +425,25 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
               "problem model storage does not match the solver");
   const size_t sol_per_cb = size_t(c.n_antennas) * s.cfg.n_sub * 2;
   const size_t n_sol = sol_per_cb * c.n_channel_blocks;
-  s.sol.Resize(n_sol);
-  s.next.Resize(n_sol);
-  s.gains.Resize(n_sol);
-  s.conv.Resize(n_sol / 2);
+  const size_t n_wg = size_t(c.n_channel_blocks) * c.n_directions;
+  const void* before[2] = {s.sol0.ptr, s.partials.ptr};
+  s.sol0.Resize(n_sol);
+  s.sol1.Resize(n_sol);
+  s.mean.Resize(c.n_channel_blocks);
+  s.wg_conv.Resize(n_wg);
+  s.wg_conv_n.Resize(n_wg);
+  s.wg_mean.Resize(n_wg);
+  s.wg_mean_n.Resize(n_wg);
   s.step_mags.Resize(size_t(std::max(c.max_iterations, c.min_iterations)) + 2);
   s.partials.Resize(size_t(p.n_cb) * p.n_seg * p.n_dir * p.n_corr * 3 * p.bl_stride);
   s.state.Resize(1);
+  if (before[0] != s.sol0.ptr || before[1] != s.partials.ptr) DestroyGraph(s);
   for (int cb = 0; cb < c.n_channel_blocks; ++cb)
-    DDH_HIP_CHECK(hipMemcpyAsync(s.sol.ptr + cb * sol_per_cb, solutions[cb],
+    DDH_HIP_CHECK(hipMemcpyAsync(s.sol0.ptr + cb * sol_per_cb, solutions[cb],
                                  sol_per_cb * sizeof(double2), hipMemcpyHostToDevice, s.stream));
   const IterArgs a = MakeArgs(s, p);
   const int limit = std::max(c.max_iterations, c.min_iterations);
-  const bool f16 = p.f16;
+  const bool graph = s.use_graph && !s.profile;
 
   hipEvent_t t0 = nullptr, t1 = nullptr;
   DDH_HIP_CHECK(hipEventCreate(&t0));
@@ -404,32 +453,40 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
     s.events.assign(2 * limit, nullptr);
     for (hipEvent_t& e : s.events) DDH_HIP_CHECK(hipEventCreate(&e));
   }
+  if (graph) EnsureGraph(s, a, p);
   DDH_HIP_CHECK(hipEventRecord(t0, s.stream));
   LaunchPrologue(a, s.stream);
   int launched = 0;
   while (launched < limit) {
-    const int k = std::min(s.poll, limit - launched);
-    for (int i = 0; i < k; ++i) {
-      const int it = launched + i;
-      if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it], s.stream));
-      LaunchCorrelate(a, p.n_corr, f16, s.waves, s.stream);
-      if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it + 1], s.stream));
-      LaunchAntennaSolve(a, p.n_corr, f16, s.stream);
-      LaunchAssign(a, s.stream);
+    if (graph) {
+      DDH_HIP_CHECK(hipGraphLaunch(s.graph_exec, s.stream));
+      launched += s.poll;
+    } else {
+      const int k = std::min(s.poll, limit - launched);
+      for (int i = 0; i < k; ++i) {
+        const int it = launched + i;
+        if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it], s.stream));
+        LaunchCorrelate(a, p.n_corr, p.f16, s.waves, s.unroll, s.stream);
+        if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it + 1], s.stream));
+        LaunchAntennaSolve(a, p.n_corr, p.f16, s.stream);
+        LaunchDecide(a, s.stream);
+      }
+      launched += k;
     }
-    launched += k;
     DDH_HIP_CHECK(hipMemcpyAsync(s.host_state, s.state.ptr, sizeof(DeviceState),
                                  hipMemcpyDeviceToHost, s.stream));
     DDH_HIP_CHECK(hipStreamSynchronize(s.stream));
     if (s.host_state->done) break;
   }
   DDH_HIP_CHECK(hipEventRecord(t1, s.stream));
-  for (int cb = 0; cb < c.n_channel_blocks; ++cb)
-    DDH_HIP_CHECK(hipMemcpyAsync(solutions[cb], s.sol.ptr + cb * sol_per_cb,
-                                 sol_per_cb * sizeof(double2), hipMemcpyDeviceToHost, s.stream));
   DDH_HIP_CHECK(hipStreamSynchronize(s.stream));
   const DeviceState st = *s.host_state;
   DDH_REQUIRE(st.done, -3, "solver did not reach its stopping criterion");
+  const double2* final_sol = (st.iteration & 1) ? s.sol1.ptr : s.sol0.ptr;
+  for (int cb = 0; cb < c.n_channel_blocks; ++cb)
+    DDH_HIP_CHECK(hipMemcpyAsync(solutions[cb], final_sol + cb * sol_per_cb,
+                                 sol_per_cb * sizeof(double2), hipMemcpyDeviceToHost, s.stream));
+  DDH_HIP_CHECK(hipStreamSynchronize(s.stream));
   float ms = 0.0f;
   DDH_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
   s.last_solve_ms = ms;
@@ -454,6 +511,31 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
 }
 
 }  // namespace
+}  // namespace ddh
+
+namespace ddh {
+
+ProblemView GetProblemView(ddh_problem* p) {
+  ProblemView v;
+  v.model = p->model.ptr;
+  v.data = p->data.ptr;
+  v.scale = p->scale.ptr;
+  v.ant1 = p->d_ant1.ptr;
+  v.ant2 = p->d_ant2.ptr;
+  v.cbinfo = p->d_cbinfo.ptr;
+  v.n_t = p->n_t;
+  v.n_bl = p->n_bl;
+  v.bl_stride = p->bl_stride;
+  v.n_cb = p->n_cb;
+  v.n_dir = p->n_dir;
+  v.n_corr = p->n_corr;
+  v.n_ant = p->n_ant;
+  v.f16 = p->f16;
+  v.ch_per_cb = p->ch_per_cb;
+  v.cb_host = p->cbinfo;
+  v.stream = p->stream;
+  return v;
+}
 }  // namespace ddh
 
 // ---------------------------------------------------------------------------
@@ -526,12 +608,16 @@ void ddh_solver_destroy(ddh_solver* s) {
   if (!s) return;
   (void)hipStreamSynchronize(s->stream);
   s->cached.reset();
+  DestroyGraph(*s);
   for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
   if (s->host_state) (void)hipHostFree(s->host_state);
-  s->sol.Free();
-  s->next.Free();
-  s->gains.Free();
-  s->conv.Free();
+  s->sol0.Free();
+  s->sol1.Free();
+  s->mean.Free();
+  s->wg_conv.Free();
+  s->wg_conv_n.Free();
+  s->wg_mean.Free();
+  s->wg_mean_n.Free();
   s->step_mags.Free();
   s->partials.Free();
   s->state.Free();
@@ -648,12 +734,16 @@ int ddh_solver_last_timing(const ddh_solver* s, double* correlate_ms, int64_t* l
   return 0;
 }
 
-int ddh_set_schedule(ddh_solver* s, int32_t waves_per_tile, int32_t use_graph,
-                     int32_t poll_interval) {
+int ddh_set_schedule(ddh_solver* s, int32_t waves_per_tile, int32_t rows_per_trip,
+                     int32_t use_graph, int32_t poll_interval) {
   return Guard([&] {
     DDH_REQUIRE(s, -1, "NULL solver");
     DDH_REQUIRE(waves_per_tile >= 1 && waves_per_tile <= 4, -1, "waves_per_tile must be 1..4");
+    DDH_REQUIRE(rows_per_trip == 1 || rows_per_trip == 2, -1, "rows_per_trip must be 1 or 2");
+    DDH_REQUIRE(rows_per_trip == 1 || s->cfg.c.n_correlations == 2, -4,
+                "two rows per trip are built for Duo (2-correlation) data only");
     DDH_REQUIRE(poll_interval >= 1, -1, "poll_interval must be >= 1");
+    s->unroll = rows_per_trip;
     s->waves = waves_per_tile;
     s->use_graph = use_graph;
     s->poll = poll_interval;

@@ -17,11 +17,16 @@
 //                  forms numerator and denominator (IterativeDiagonalSolver.cc
 //                  :134-138) from U, S and the gains in FP64, divides
 //                  (:206-221), applies the step (SolverBase.cc:88-158) and the
-//                  per-solution convergence term (SolverBase.cc:254-268).
-//   assign         one workgroup: FP64 convergence test (SolverBase.cc:296-306),
+//                  per-solution convergence term (SolverBase.cc:254-268), with
+//                  per-workgroup partial sums.
+//   decide         one workgroup: FP64 convergence test (SolverBase.cc:296-306),
 //                  stall detection (:63-86), stopping criterion
-//                  (SolverBase.h:274-285), copy of the next solutions and the
-//                  NaN repair of the next iteration (SolverBase.cc:342-371).
+//                  (SolverBase.h:274-285) and the NaN-repair values of the next
+//                  iteration (SolverBase.cc:342-371), which consumers apply
+//                  when they read a solution.
+//
+// Solutions ping-pong between two buffers by iteration parity, so no copy
+// pass is needed.
 //
 // Every launch first reads the device-side `done` flag and returns when the
 // solve has already stopped, so the host may enqueue iterations ahead of the
@@ -30,8 +35,11 @@
 #define DP3_AMD_SOLVER_KERNELS_H
 
 #include <cstdint>
+#include <vector>
 
 #include "common.h"
+
+struct ddh_problem;
 
 namespace ddh {
 
@@ -69,10 +77,13 @@ struct IterArgs {
   // solver
   const int32_t* n_sol_dir;  // [n_dir]
   const int32_t* sol_start;  // [n_dir]
-  double2* sol;              // [n_cb][n_ant][n_sub][2]
-  double2* next;             // same
-  float2* gains;             // float copy of sol, read by correlate
-  double* conv;              // [n_cb][n_ant][n_sub]
+  double2* sol0;             // [n_cb][n_ant][n_sub][2], current at even iterations
+  double2* sol1;             // current at odd iterations
+  double* mean;              // [n_cb] repair value of the current solutions
+  double* wg_conv;           // [n_cb][n_dir] per-workgroup convergence sums
+  int32_t* wg_conv_n;
+  double* wg_mean;           // [n_cb][n_dir] per-workgroup sums of |pol 0|
+  int32_t* wg_mean_n;
   float* partials;           // [n_cb][n_seg][n_dir][n_corr][3][bl_stride]
   DeviceState* state;
   double* step_mags;         // [max_iterations + 1]
@@ -80,16 +91,19 @@ struct IterArgs {
   int32_t max_iterations, min_iterations;
   double accuracy, step_size, step_diff_sigma;
   int32_t detect_stalling, phase_only;
+  int32_t split;             // threads per (antenna, solution) in antenna_solve
+  int32_t max_nsd;           // largest n_sol_dir entry
 };
 
 // Host-side launchers (defined in solver_kernels.hip).
-void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves,
+void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, int unroll,
                      hipStream_t stream);
 void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, hipStream_t stream);
-void LaunchAssign(const IterArgs& a, hipStream_t stream);
+void LaunchDecide(const IterArgs& a, hipStream_t stream);
 void LaunchPrologue(const IterArgs& a, hipStream_t stream);
 size_t CorrelateSharedBytes(int n_dir, int n_corr, int waves);
 size_t MaxSharedBytes();
+size_t AntennaSolveSharedBytes(const IterArgs& a, int n_corr);
 void PrepareKernelAttributes();
 
 // Layout conversion: reference order [plane][t][bl][ch][n_corr] complex64 ->
@@ -103,6 +117,23 @@ void LaunchPlaneMaxAbs(const float* src, int64_t plane_floats, int n_planes,
                        uint32_t* max_bits, hipStream_t stream);
 void LaunchScaleFromMax(const uint32_t* max_bits, float* scale, int n,
                         hipStream_t stream);
+
+// Internals of a device problem, shared by the solver host code and the
+// predict (solver_host.hip owns the struct behind ddh_problem).
+struct ProblemView {
+  void* model;
+  void* data;
+  float* scale;
+  const int32_t* ant1;
+  const int32_t* ant2;
+  const CbInfo* cbinfo;
+  int n_t, n_bl, bl_stride, n_cb, n_dir, n_corr, n_ant;
+  bool f16;
+  std::vector<int32_t> ch_per_cb;
+  std::vector<CbInfo> cb_host;
+  hipStream_t stream;
+};
+ProblemView GetProblemView(ddh_problem* p);
 
 }  // namespace ddh
 

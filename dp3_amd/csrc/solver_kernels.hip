@@ -42,6 +42,9 @@ __device__ __forceinline__ uint32_t Float2ToHalfBits(float re, float im) {
   return __builtin_bit_cast(uint32_t, h);
 }
 
+__device__ __forceinline__ double2 FixedSolution(const double2* sol, size_t pair, int pol,
+                                                 double mean);
+
 // Loads the NC correlations of visibility `idx` from a plane.
 template <int NC, bool F16>
 __device__ __forceinline__ void LoadVis(const void* base, size_t idx, float2 (&v)[NC]) {
@@ -72,7 +75,7 @@ __device__ __forceinline__ void LoadVis(const void* base, size_t idx, float2 (&v
 // (time, channel) rows of the segment and are combined through LDS in a fixed
 // order, so results do not depend on scheduling.
 // ---------------------------------------------------------------------------
-template <int ND, int NC, bool F16>
+template <int ND, int NC, bool F16, int UNROLL>
 __global__ void __launch_bounds__(kCorrelateMaxThreads)
     CorrelateKernel(const IterArgs a) {
   if (a.state->done) return;
@@ -94,6 +97,8 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads)
   // Stage G_a,i * conj(G_b,j) of every (direction, correlation) for the 64
   // baselines of the tile (solutions cast to float, as SolveData.h:230-235).
   const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
+  const double2* sol = (a.state->iteration & 1) ? a.sol1 : a.sol0;
+  const double mean = a.mean[cb];
   for (int idx = threadIdx.x; idx < ND * NC * kWave; idx += blockDim.x) {
     const int l = idx & (kWave - 1);
     const int dk = idx / kWave;
@@ -103,9 +108,9 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads)
     float2 g = make_float2(0.0f, 0.0f);
     if (b < a.n_bl) {
       const int s = a.seg_sol[seg * ND + d];
-      const float2 ga = a.gains[(gbase + size_t(a.ant1[b]) * a.n_sub + s) * 2 + PolI<NC>(k)];
-      const float2 gb = a.gains[(gbase + size_t(a.ant2[b]) * a.n_sub + s) * 2 + PolJ<NC>(k)];
-      g = cmulc(ga, gb);
+      const double2 ga = FixedSolution(sol, gbase + size_t(a.ant1[b]) * a.n_sub + s, PolI<NC>(k), mean);
+      const double2 gb = FixedSolution(sol, gbase + size_t(a.ant2[b]) * a.n_sub + s, PolJ<NC>(k), mean);
+      g = cmulc(make_float2(float(ga.x), float(ga.y)), make_float2(float(gb.x), float(gb.y)));
       if constexpr (F16) {
         const float sc = a.model_scale[cb * ND + d];
         g.x *= sc;
@@ -128,32 +133,50 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads)
 
   const int r0 = a.seg_t0[seg] * n_ch;
   const int r1 = a.seg_t0[seg + 1] * n_ch;
-  for (int row = r0 + wave; row < r1; row += n_waves) {
-    const size_t e = size_t(row) * a.bl_stride + bl;
-    float2 r[NC];
-    LoadVis<NC, false>(a.data, info.data_off + e, r);
-    float2 m[ND][NC];
+  // UNROLL rows per trip: all loads of the trip are issued before any use.
+  // A row past the segment end reads as zeros, which adds exactly nothing.
+  for (int row0 = r0 + wave; row0 < r1; row0 += UNROLL * n_waves) {
+    float2 r[UNROLL][NC];
+    float2 m[UNROLL][ND][NC];
 #pragma unroll
-    for (int d = 0; d < ND; ++d)
-      LoadVis<NC, F16>(a.model, info.model_off + d * plane + e, m[d]);
-    // Full residual, directions subtracted in order (IterativeDiagonalSolver.cc:108-111).
+    for (int u = 0; u < UNROLL; ++u) {
+      const int row = row0 + u * n_waves;
+      if (u == 0 || row < r1) {
+        const size_t e = size_t(row) * a.bl_stride + bl;
+        LoadVis<NC, false>(a.data, info.data_off + e, r[u]);
 #pragma unroll
-    for (int d = 0; d < ND; ++d)
+        for (int d = 0; d < ND; ++d)
+          LoadVis<NC, F16>(a.model, info.model_off + d * plane + e, m[u][d]);
+      } else {
 #pragma unroll
-      for (int k = 0; k < NC; ++k) {
-        const float2 c = cmul(gg_s[(d * NC + k) * kWave + lane], m[d][k]);
-        r[k].x -= c.x;
-        r[k].y -= c.y;
+        for (int k = 0; k < NC; ++k) {
+          r[u][k] = make_float2(0.0f, 0.0f);
+#pragma unroll
+          for (int d = 0; d < ND; ++d) m[u][d][k] = make_float2(0.0f, 0.0f);
+        }
       }
+    }
 #pragma unroll
-    for (int d = 0; d < ND; ++d)
+    for (int u = 0; u < UNROLL; ++u) {
+      // Full residual, directions subtracted in order (IterativeDiagonalSolver.cc:108-111).
 #pragma unroll
-      for (int k = 0; k < NC; ++k) {
-        const float2 mm = m[d][k];
-        ur[d][k] += r[k].x * mm.x + r[k].y * mm.y;
-        ui[d][k] += r[k].y * mm.x - r[k].x * mm.y;
-        sm[d][k] += mm.x * mm.x + mm.y * mm.y;
-      }
+      for (int d = 0; d < ND; ++d)
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+          const float2 c = cmul(gg_s[(d * NC + k) * kWave + lane], m[u][d][k]);
+          r[u][k].x -= c.x;
+          r[u][k].y -= c.y;
+        }
+#pragma unroll
+      for (int d = 0; d < ND; ++d)
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+          const float2 mm = m[u][d][k];
+          ur[d][k] += r[u][k].x * mm.x + r[u][k].y * mm.y;
+          ui[d][k] += r[u][k].y * mm.x - r[u][k].x * mm.y;
+          sm[d][k] += mm.x * mm.x + mm.y * mm.y;
+        }
+    }
   }
 
   // Combine the waves: wave w-1 += wave w, for w = n_waves-1 .. 1.
@@ -197,37 +220,104 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads)
   }
 }
 
+// Solutions of MakeSolutionsFinite2Pol (SolverBase.cc:342-371), applied on
+// the fly: a pair with a non-finite entry reads as (mean, mean), where mean is
+// the block's average |pol-0 value| over finite pairs (:354), kept per block.
+__device__ __forceinline__ double2 FixedSolution(const double2* sol, size_t pair, int pol,
+                                                 double mean) {
+  const double2 g0 = sol[pair * 2];
+  const double2 g1 = sol[pair * 2 + 1];
+  if (!(cfinite(g0) && cfinite(g1))) return make_double2(mean, 0.0);
+  return pol ? g1 : g0;
+}
+
+// Block-wide deterministic sum (fixed tree) of one double and one int.
+__device__ void BlockSum(double& v, int& c, double* red, int* redc) {
+  red[threadIdx.x] = v;
+  redc[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[threadIdx.x] += red[threadIdx.x + s];
+      redc[threadIdx.x] += redc[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  v = red[0];
+  c = redc[0];
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
-// antenna_solve: grid (n_dir, n_cb); thread = (antenna, solution of d).
-// With T_k = U_k + G_a,i conj(G_b,j) S_k = sum_v R_d,k conj(M_k) (the residual
-// with direction d added back), the reference's sums become
+// antenna_solve: grid (n_dir, n_cb); `split` threads per (antenna, solution
+// of d) share that antenna's baseline list and are combined by a fixed
+// shuffle tree. With T_k = U_k + G_a,i conj(G_b,j) S_k = sum_v R_d,k conj(M_k)
+// (the residual with direction d added back), the reference's sums become
 //   antenna a as first antenna, pol p:  num += G_b,j T_k,  den += |G_b,j|^2 S_k
 //                                       over k with i(k) = p
 //   antenna b as second antenna, pol p: num += G_a,i conj(T_k),
 //                                       den += |G_a,i|^2 S_k over k with j(k) = p
-// (IterativeDiagonalSolver.cc:172-197, AddNormToDenominator :20-35).
+// (IterativeDiagonalSolver.cc:172-197, AddNormToDenominator :20-35). Then the
+// FP64 division (:206-221), the step (SolverBase.cc:88-158) and the
+// convergence term (:254-268).
+// Solutions ping-pong between sol0 and sol1 by iteration parity.
 // ---------------------------------------------------------------------------
 template <int NC, bool F16>
 __global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterArgs a) {
+  __shared__ double red[kBlockThreads];
+  __shared__ int redc[kBlockThreads];
   if (a.state->done) return;
+  const int iteration = a.state->iteration;
+  const double2* cur = (iteration & 1) ? a.sol1 : a.sol0;
+  double2* nxt = (iteration & 1) ? a.sol0 : a.sol1;
   const int d = blockIdx.x;
   const int cb = blockIdx.y;
   const int nsd = a.n_sol_dir[d];
   const int s0 = a.sol_start[d];
   const int n_items = a.n_ant * nsd;
-  const int item = threadIdx.x;
+  const int split = a.split;
+  const int item = threadIdx.x / split;
+  const int q = threadIdx.x - item * split;
   const bool active = item < n_items;
   const int ant = active ? item / nsd : 0;
   const int s_abs = s0 + (active ? item - ant * nsd : 0);
   const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
+  const double mean = a.mean[cb];
   float scale = 1.0f;
   if constexpr (F16) scale = a.model_scale[cb * a.n_dir + d];
   const double sc = scale;
   const double sc2 = sc * sc;
-
-  extern __shared__ float part_s[];  // [NC][3][bl_stride]
-  double nr[2] = {0.0, 0.0}, ni[2] = {0.0, 0.0}, dn[2] = {0.0, 0.0};
   const int bs = a.bl_stride;
+  // LDS: partials of one segment, baseline antennas, per-antenna baseline
+  // lists and the gains of this direction's solutions as the correlate pass
+  // used them (repaired, then cast to float).
+  extern __shared__ double smem_d[];
+  double2* g_s = reinterpret_cast<double2*>(smem_d);             // [n_ant][nsd][2]
+  float* part_s = reinterpret_cast<float*>(g_s + size_t(a.n_ant) * nsd * 2);  // [NC][3][bs]
+  int* a1_s = reinterpret_cast<int*>(part_s + NC * 3 * bs);      // [bs]
+  int* a2_s = a1_s + bs;                                         // [bs]
+  int* off_s = a2_s + bs;                                        // [n_ant + 1]
+  int* csr_s = off_s + a.n_ant + 1;                              // [2 n_bl]
+  for (int i = threadIdx.x; i < a.n_ant * nsd * 2; i += blockDim.x) {
+    const int an = i / (nsd * 2);
+    const int rem = i - an * nsd * 2;
+    const double2 g = FixedSolution(cur, gbase + size_t(an) * a.n_sub + s0 + (rem >> 1),
+                                    rem & 1, mean);
+    g_s[i] = make_double2(double(float(g.x)), double(float(g.y)));
+  }
+  for (int i = threadIdx.x; i < bs; i += blockDim.x) {
+    a1_s[i] = a.ant1[i];
+    a2_s[i] = a.ant2[i];
+  }
+  for (int i = threadIdx.x; i <= a.n_ant; i += blockDim.x) off_s[i] = a.csr_off[i];
+  for (int i = threadIdx.x; i < 2 * a.n_bl; i += blockDim.x) csr_s[i] = a.csr[i];
+  const int rel = s_abs - s0;
+  auto gain = [&](int an, int pol) { return g_s[(an * nsd + rel) * 2 + pol]; };
+
+  double nr[2] = {0.0, 0.0}, ni[2] = {0.0, 0.0}, dn[2] = {0.0, 0.0};
+  __syncthreads();
+  const double2 gant[2] = {active ? gain(ant, 0) : make_double2(0, 0),
+                           active ? gain(ant, 1) : make_double2(0, 0)};
   for (int seg = 0; seg < a.n_seg; ++seg) {
     const int s_seg = a.seg_sol[seg * a.n_dir + d];
     const float* src =
@@ -236,18 +326,12 @@ __global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterAr
     for (int i = threadIdx.x; i < NC * 3 * bs; i += blockDim.x) part_s[i] = src[i];
     __syncthreads();
     if (!active || s_seg != s_abs) continue;
-    const float2 ga_f[2] = {a.gains[(gbase + size_t(ant) * a.n_sub + s_abs) * 2],
-                            a.gains[(gbase + size_t(ant) * a.n_sub + s_abs) * 2 + 1]};
-    const double2 gant[2] = {make_double2(ga_f[0].x, ga_f[0].y),
-                             make_double2(ga_f[1].x, ga_f[1].y)};
-    for (int e = a.csr_off[ant]; e < a.csr_off[ant + 1]; ++e) {
-      const int code = a.csr[e];
+    for (int e = off_s[ant] + q; e < off_s[ant + 1]; e += split) {
+      const int code = csr_s[e];
       const int b = code >> 1;
       const int side = code & 1;
-      const int other = side ? a.ant1[b] : a.ant2[b];
-      const float2 go_f0 = a.gains[(gbase + size_t(other) * a.n_sub + s_abs) * 2];
-      const float2 go_f1 = a.gains[(gbase + size_t(other) * a.n_sub + s_abs) * 2 + 1];
-      const double2 go[2] = {make_double2(go_f0.x, go_f0.y), make_double2(go_f1.x, go_f1.y)};
+      const int other = side ? a1_s[b] : a2_s[b];
+      const double2 go[2] = {gain(other, 0), gain(other, 1)};
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
         const int i = PolI<NC>(k);
@@ -275,134 +359,112 @@ __global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterAr
       }
     }
   }
-  if (!active) return;
-
-  const size_t idx = (gbase + size_t(ant) * a.n_sub + s_abs) * 2;
-  const double nan = std::numeric_limits<double>::quiet_NaN();
-  double2 stepped[2];
-  double2 old[2];
+  // Combine the `split` lanes of an item (adjacent lanes, fixed tree).
+  for (int m = 1; m < split; m <<= 1) {
 #pragma unroll
-  for (int p = 0; p < 2; ++p) {
-    const double2 nx = dn[p] == 0.0 ? make_double2(nan, 0.0)
-                                    : make_double2(nr[p] / dn[p], ni[p] / dn[p]);
-    old[p] = a.sol[idx + p];
-    if (!a.phase_only) {
-      stepped[p] = make_double2(old[p].x * (1.0 - a.step_size) + nx.x * a.step_size,
-                                old[p].y * (1.0 - a.step_size) + nx.y * a.step_size);
-    } else {
-      // Step along the complex circle (SolverBase.cc:147-155).
-      const double phase_from = atan2(old[p].y, old[p].x);
-      double distance = atan2(nx.y, nx.x) - phase_from;
-      if (distance > M_PI)
-        distance -= 2.0 * M_PI;
-      else if (distance < -M_PI)
-        distance += 2.0 * M_PI;
-      double sn, cs;
-      sincos(phase_from + a.step_size * distance, &sn, &cs);
-      stepped[p] = make_double2(cs, sn);
+    for (int p = 0; p < 2; ++p) {
+      nr[p] += __shfl_xor(nr[p], m);
+      ni[p] += __shfl_xor(ni[p], m);
+      dn[p] += __shfl_xor(dn[p], m);
     }
-    a.next[idx + p] = stepped[p];
   }
-  // Convergence term of this solution pair (SolverBase.cc:254-268).
-  double term = -1.0;
-  const double n0 = cnorm(old[0]), n1 = cnorm(old[1]);
-  const double2 inv0 = make_double2(old[0].x / n0, -old[0].y / n0);
-  const double2 inv1 = make_double2(old[1].x / n1, -old[1].y / n1);
-  if (cfinite(inv0) && cfinite(inv1)) {
-    const double2 d0 = cmul(make_double2(stepped[0].x - old[0].x, stepped[0].y - old[0].y), inv0);
-    const double2 d1 = cmul(make_double2(stepped[1].x - old[1].x, stepped[1].y - old[1].y), inv1);
-    const double sumabs = hypot(d0.x, d0.y) + hypot(d1.x, d1.y);
-    if (isfinite(sumabs)) term = sumabs;
-  }
-  a.conv[gbase + size_t(ant) * a.n_sub + s_abs] = term;
-}
 
-// Block-wide deterministic sum (fixed tree) of one double and one int.
-__device__ void BlockSum(double& v, int& c, double* red, int* redc) {
-  red[threadIdx.x] = v;
-  redc[threadIdx.x] = c;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      red[threadIdx.x] += red[threadIdx.x + s];
-      redc[threadIdx.x] += redc[threadIdx.x + s];
+  double conv_sum = 0.0, mean_sum = 0.0;
+  int conv_n = 0, mean_n = 0;
+  if (active && q == 0) {
+    const size_t pair = gbase + size_t(ant) * a.n_sub + s_abs;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    double2 stepped[2];
+    double2 old[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const double2 nx = dn[p] == 0.0 ? make_double2(nan, 0.0)
+                                      : make_double2(nr[p] / dn[p], ni[p] / dn[p]);
+      old[p] = FixedSolution(cur, pair, p, mean);
+      if (!a.phase_only) {
+        stepped[p] = make_double2(old[p].x * (1.0 - a.step_size) + nx.x * a.step_size,
+                                  old[p].y * (1.0 - a.step_size) + nx.y * a.step_size);
+      } else {
+        // Step along the complex circle (SolverBase.cc:147-155).
+        const double phase_from = atan2(old[p].y, old[p].x);
+        double distance = atan2(nx.y, nx.x) - phase_from;
+        if (distance > M_PI)
+          distance -= 2.0 * M_PI;
+        else if (distance < -M_PI)
+          distance += 2.0 * M_PI;
+        double sn, cs;
+        sincos(phase_from + a.step_size * distance, &sn, &cs);
+        stepped[p] = make_double2(cs, sn);
+      }
+      nxt[pair * 2 + p] = stepped[p];
     }
-    __syncthreads();
-  }
-  v = red[0];
-  c = redc[0];
-  __syncthreads();
-}
-
-// MakeSolutionsFinite2Pol (SolverBase.cc:342-371) per block of solutions,
-// then the float copy read by correlate. The mean uses the pol-0 magnitude
-// for both polarizations (:354).
-__device__ void FiniteFixAndCast(const IterArgs& a, double* red, int* redc) {
-  const int n_pairs = a.n_ant * a.n_sub;
-  for (int cb = 0; cb < a.n_cb; ++cb) {
-    double2* s = a.sol + size_t(cb) * n_pairs * 2;
-    double sum = 0.0;
-    int count = 0;
-    for (int i = threadIdx.x; i < n_pairs; i += blockDim.x) {
-      if (cfinite(s[2 * i]) && cfinite(s[2 * i + 1])) {
-        sum += hypot(s[2 * i].x, s[2 * i].y);
-        ++count;
+    // Convergence term of this solution pair (SolverBase.cc:254-268).
+    const double n0 = cnorm(old[0]), n1 = cnorm(old[1]);
+    const double2 inv0 = make_double2(old[0].x / n0, -old[0].y / n0);
+    const double2 inv1 = make_double2(old[1].x / n1, -old[1].y / n1);
+    if (cfinite(inv0) && cfinite(inv1)) {
+      const double2 d0 =
+          cmul(make_double2(stepped[0].x - old[0].x, stepped[0].y - old[0].y), inv0);
+      const double2 d1 =
+          cmul(make_double2(stepped[1].x - old[1].x, stepped[1].y - old[1].y), inv1);
+      const double sumabs = hypot(d0.x, d0.y) + hypot(d1.x, d1.y);
+      if (isfinite(sumabs)) {
+        conv_sum = sumabs;
+        conv_n = 2;
       }
     }
-    BlockSum(sum, count, red, redc);
-    const double average = count == 0 ? 1.0 : sum / count;
-    float2* g = a.gains + size_t(cb) * n_pairs * 2;
-    for (int i = threadIdx.x; i < n_pairs; i += blockDim.x) {
-      double2 v0 = s[2 * i], v1 = s[2 * i + 1];
-      if (!cfinite(v0) || !cfinite(v1)) {
-        v0 = make_double2(average, 0.0);
-        v1 = make_double2(average, 0.0);
-        s[2 * i] = v0;
-        s[2 * i + 1] = v1;
-      }
-      g[2 * i] = make_float2(float(v0.x), float(v0.y));
-      g[2 * i + 1] = make_float2(float(v1.x), float(v1.y));
+    // Mean |pol 0| over finite pairs, for the next iteration's repair.
+    if (cfinite(stepped[0]) && cfinite(stepped[1])) {
+      mean_sum = hypot(stepped[0].x, stepped[0].y);
+      mean_n = 1;
     }
   }
-}
-
-__global__ void __launch_bounds__(kBlockThreads) PrologueKernel(const IterArgs a) {
-  __shared__ double red[kBlockThreads];
-  __shared__ int redc[kBlockThreads];
+  BlockSum(conv_sum, conv_n, red, redc);
+  BlockSum(mean_sum, mean_n, red, redc);
+  const int wg = cb * a.n_dir + d;
   if (threadIdx.x == 0) {
-    DeviceState st = {};
-    *a.state = st;
+    a.wg_conv[wg] = conv_sum;
+    a.wg_conv_n[wg] = conv_n;
+    a.wg_mean[wg] = mean_sum;
+    a.wg_mean_n[wg] = mean_n;
   }
-  FiniteFixAndCast(a, red, redc);
+
 }
 
-__global__ void __launch_bounds__(kBlockThreads) AssignKernel(const IterArgs a) {
+// One workgroup: the iteration's decision from the per-workgroup partials of
+// antenna_solve (the kernel boundary makes them visible without fences).
+__global__ void __launch_bounds__(kBlockThreads) DecideKernel(const IterArgs a) {
   __shared__ double red[kBlockThreads];
   __shared__ int redc[kBlockThreads];
-  __shared__ int stop_s;
   if (a.state->done) return;
-  const int n = a.n_cb * a.n_ant * a.n_sub;
+  const int n_wg = a.n_cb * a.n_dir;
   double sum = 0.0;
   int count = 0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const double t = a.conv[i];
-    if (t >= 0.0) {
-      sum += t;
-      count += 2;
-    }
+  for (int i = threadIdx.x; i < n_wg; i += blockDim.x) {
+    sum += a.wg_conv[i];
+    count += a.wg_conv_n[i];
   }
   BlockSum(sum, count, red, redc);
+  for (int c = threadIdx.x; c < a.n_cb; c += blockDim.x) {
+    double ms = 0.0;
+    int mn = 0;
+    for (int dd = 0; dd < a.n_dir; ++dd) {
+      ms += a.wg_mean[c * a.n_dir + dd];
+      mn += a.wg_mean_n[c * a.n_dir + dd];
+    }
+    a.mean[c] = mn == 0 ? 1.0 : ms / mn;
+  }
   if (threadIdx.x == 0) {
     DeviceState st = *a.state;
-    const int iteration = st.iteration + 1;
+    const int it = st.iteration + 1;
     // AssignSolutions (SolverBase.cc:296-306); no constraints are attached, so
     // constraints are satisfied and the plain accuracy applies.
     const double step_magnitude = count == 0 ? 0.0 : sum / a.step_size / count;
-    a.step_mags[iteration - 1] = step_magnitude;
+    a.step_mags[it - 1] = step_magnitude;
     const bool converged = step_magnitude <= a.accuracy;
     bool stalled = false;
     if (a.detect_stalling) {
-      if (iteration == 1) {
+      if (it == 1) {
         st.n_var_count = 1.0;
         st.step_mean = step_magnitude;
         st.step_var = 0.0;
@@ -413,23 +475,42 @@ __global__ void __launch_bounds__(kBlockThreads) AssignKernel(const IterArgs a) 
         const double d2 = step_magnitude - st.step_mean;
         st.step_var += d1 * d2;
       }
-      if (iteration >= 30)
+      if (it >= 30)
         stalled = st.step_mean < a.step_diff_sigma * sqrt(st.step_var / st.n_var_count);
     }
-    const bool ready = iteration >= a.max_iterations || converged || stalled;
-    const bool stop = iteration >= a.min_iterations && ready;
-    st.iteration = iteration;
+    const bool ready = it >= a.max_iterations || converged || stalled;
+    const bool stop = it >= a.min_iterations && ready;
+    st.iteration = it;
     st.has_converged = converged ? 1 : 0;
     st.last_step = step_magnitude;
     st.done = stop ? 1 : 0;
     *a.state = st;
-    stop_s = stop ? 1 : 0;
   }
-  __syncthreads();
-  const int total = n * 2;
-  for (int i = threadIdx.x; i < total; i += blockDim.x) a.sol[i] = a.next[i];
-  __syncthreads();
-  if (!stop_s) FiniteFixAndCast(a, red, redc);
+}
+
+// Resets the solve state and computes the repair means of the initial
+// solutions (the first MakeSolutionsFinite2Pol of the loop).
+__global__ void __launch_bounds__(kBlockThreads) PrologueKernel(const IterArgs a) {
+  __shared__ double red[kBlockThreads];
+  __shared__ int redc[kBlockThreads];
+  if (threadIdx.x == 0) {
+    DeviceState st = {};
+    *a.state = st;
+  }
+  const int n_pairs = a.n_ant * a.n_sub;
+  for (int cb = 0; cb < a.n_cb; ++cb) {
+    const double2* s = a.sol0 + size_t(cb) * n_pairs * 2;
+    double sum = 0.0;
+    int count = 0;
+    for (int i = threadIdx.x; i < n_pairs; i += blockDim.x) {
+      if (cfinite(s[2 * i]) && cfinite(s[2 * i + 1])) {
+        sum += hypot(s[2 * i].x, s[2 * i].y);
+        ++count;
+      }
+    }
+    BlockSum(sum, count, red, redc);
+    if (threadIdx.x == 0) a.mean[cb] = count == 0 ? 1.0 : sum / count;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -505,20 +586,26 @@ __global__ void ScaleFromMaxKernel(const uint32_t* max_bits, float* scale, int n
 
 using KernelFn = void (*)(const IterArgs);
 
-template <int NC, bool F16, int... ND>
+template <int NC, bool F16, int UNROLL, int... ND>
 constexpr auto MakeCorrelateTable(std::integer_sequence<int, ND...>) {
-  return std::array<KernelFn, sizeof...(ND)>{&CorrelateKernel<ND + 1, NC, F16>...};
+  return std::array<KernelFn, sizeof...(ND)>{&CorrelateKernel<ND + 1, NC, F16, UNROLL>...};
 }
 
-const auto kDuoC64 = MakeCorrelateTable<2, false>(std::make_integer_sequence<int, kMaxDirections>{});
-const auto kDuoF16 = MakeCorrelateTable<2, true>(std::make_integer_sequence<int, kMaxDirections>{});
-const auto kFullC64 = MakeCorrelateTable<4, false>(std::make_integer_sequence<int, kMaxDirections>{});
-const auto kFullF16 = MakeCorrelateTable<4, true>(std::make_integer_sequence<int, kMaxDirections>{});
+using Dirs = std::make_integer_sequence<int, kMaxDirections>;
+const auto kDuoC64 = MakeCorrelateTable<2, false, 1>(Dirs{});
+const auto kDuoF16 = MakeCorrelateTable<2, true, 1>(Dirs{});
+const auto kFullC64 = MakeCorrelateTable<4, false, 1>(Dirs{});
+const auto kFullF16 = MakeCorrelateTable<4, true, 1>(Dirs{});
+const auto kDuoC64U2 = MakeCorrelateTable<2, false, 2>(Dirs{});
+const auto kDuoF16U2 = MakeCorrelateTable<2, true, 2>(Dirs{});
 
-KernelFn CorrelateFor(int n_dir, int n_corr, bool f16) {
+KernelFn CorrelateFor(int n_dir, int n_corr, bool f16, int unroll) {
   DDH_REQUIRE(n_dir >= 1 && n_dir <= kMaxDirections, -4,
               "number of directions must be 1.." + std::to_string(kMaxDirections));
-  if (n_corr == 2) return f16 ? kDuoF16[n_dir - 1] : kDuoC64[n_dir - 1];
+  if (n_corr == 2) {
+    if (unroll == 2) return f16 ? kDuoF16U2[n_dir - 1] : kDuoC64U2[n_dir - 1];
+    return f16 ? kDuoF16[n_dir - 1] : kDuoC64[n_dir - 1];
+  }
   if (n_corr == 4) return f16 ? kFullF16[n_dir - 1] : kFullC64[n_dir - 1];
   throw Error(-1, "n_correlations must be 2 or 4");
 }
@@ -533,6 +620,7 @@ size_t CorrelateSharedBytes(int n_dir, int n_corr, int waves) {
 
 namespace {
 int g_max_lds = 64 * 1024;
+int g_max_lds_solve = 160 * 1024;
 }
 
 size_t MaxSharedBytes() { return size_t(g_max_lds); }
@@ -546,21 +634,28 @@ void PrepareKernelAttributes() {
   DDH_HIP_CHECK(hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   g_max_lds = std::min(limit, 160 * 1024);
   const int max_lds = g_max_lds;
-  for (const auto* table : {&kDuoC64, &kDuoF16, &kFullC64, &kFullF16})
+  for (const auto* table : {&kDuoC64, &kDuoF16, &kFullC64, &kFullF16, &kDuoC64U2, &kDuoF16U2})
     for (KernelFn f : *table)
       DDH_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(f),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  // antenna_solve also holds static LDS for its reductions
   for (const void* f : {reinterpret_cast<const void*>(&AntennaSolveKernel<2, false>),
                         reinterpret_cast<const void*>(&AntennaSolveKernel<2, true>),
                         reinterpret_cast<const void*>(&AntennaSolveKernel<4, false>),
-                        reinterpret_cast<const void*>(&AntennaSolveKernel<4, true>)})
-    DDH_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+                        reinterpret_cast<const void*>(&AntennaSolveKernel<4, true>)}) {
+    hipFuncAttributes attr{};
+    DDH_HIP_CHECK(hipFuncGetAttributes(&attr, f));
+    const int dyn = max_lds - int(attr.sharedSizeBytes);
+    g_max_lds_solve = std::min(g_max_lds_solve, dyn);
+    DDH_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+  }
   done = true;
 }
 
-void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, hipStream_t stream) {
+void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, int unroll,
+                     hipStream_t stream) {
   DDH_REQUIRE(waves >= 1 && waves * kWave <= kCorrelateMaxThreads, -1, "bad waves per tile");
-  const KernelFn f = CorrelateFor(a.n_dir, n_corr, f16);
+  const KernelFn f = CorrelateFor(a.n_dir, n_corr, f16, unroll);
   const dim3 grid(a.bl_stride / kWave, a.n_seg, a.n_cb);
   const size_t lds = CorrelateSharedBytes(a.n_dir, n_corr, waves);
   DDH_REQUIRE(lds <= MaxSharedBytes(), -4, "correlate tile needs more LDS than the device has");
@@ -568,9 +663,16 @@ void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, hipStre
   DDH_HIP_CHECK(hipGetLastError());
 }
 
+size_t AntennaSolveSharedBytes(const IterArgs& a, int n_corr) {
+  const int max_nsd = std::max(1, a.max_nsd);
+  return size_t(a.n_ant) * max_nsd * 2 * sizeof(double2) +
+         size_t(n_corr) * 3 * a.bl_stride * sizeof(float) +
+         (size_t(2) * a.bl_stride + a.n_ant + 1 + 2 * size_t(a.n_bl)) * sizeof(int);
+}
+
 void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, hipStream_t stream) {
-  const size_t lds = size_t(n_corr) * 3 * a.bl_stride * sizeof(float);
-  DDH_REQUIRE(lds <= MaxSharedBytes(), -4, "too many baselines for the antenna solve LDS");
+  const size_t lds = AntennaSolveSharedBytes(a, n_corr);
+  DDH_REQUIRE(lds <= size_t(g_max_lds_solve), -4, "too many baselines for the antenna solve LDS");
   const dim3 grid(a.n_dir, a.n_cb);
   if (n_corr == 2) {
     if (f16)
@@ -586,8 +688,8 @@ void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, hipStream_t str
   DDH_HIP_CHECK(hipGetLastError());
 }
 
-void LaunchAssign(const IterArgs& a, hipStream_t stream) {
-  hipLaunchKernelGGL(AssignKernel, dim3(1), dim3(kBlockThreads), 0, stream, a);
+void LaunchDecide(const IterArgs& a, hipStream_t stream) {
+  hipLaunchKernelGGL(DecideKernel, dim3(1), dim3(256), 0, stream, a);
   DDH_HIP_CHECK(hipGetLastError());
 }
 

