@@ -416,13 +416,18 @@ void EnsureGraph(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
   s.graph_key = key;
 }
 
-void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* result) {
+void CheckMatch(const ddh_solver& s, const ddh_problem& p) {
   const ddh_config& c = s.cfg.c;
   DDH_REQUIRE(p.n_cb == c.n_channel_blocks && p.n_dir == c.n_directions &&
                   p.n_corr == c.n_correlations && p.n_ant == c.n_antennas,
               -1, "problem does not match the solver configuration");
   DDH_REQUIRE(p.f16 == (c.model_storage == DDH_MODEL_F16), -1,
               "problem model storage does not match the solver");
+}
+
+// Sizes the solver's per-solve device buffers for problem p.
+void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
+  const ddh_config& c = s.cfg.c;
   const size_t sol_per_cb = size_t(c.n_antennas) * s.cfg.n_sub * 2;
   const size_t n_sol = sol_per_cb * c.n_channel_blocks;
   const size_t n_wg = size_t(c.n_channel_blocks) * c.n_directions;
@@ -438,6 +443,13 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
   s.partials.Resize(size_t(p.n_cb) * p.n_seg * p.n_dir * p.n_corr * 3 * p.bl_stride);
   s.state.Resize(1);
   if (before[0] != s.sol0.ptr || before[1] != s.partials.ptr) DestroyGraph(s);
+}
+
+void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* result) {
+  const ddh_config& c = s.cfg.c;
+  CheckMatch(s, p);
+  const size_t sol_per_cb = size_t(c.n_antennas) * s.cfg.n_sub * 2;
+  PrepareBuffers(s, p);
   for (int cb = 0; cb < c.n_channel_blocks; ++cb)
     DDH_HIP_CHECK(hipMemcpyAsync(s.sol0.ptr + cb * sol_per_cb, solutions[cb],
                                  sol_per_cb * sizeof(double2), hipMemcpyHostToDevice, s.stream));
@@ -754,6 +766,347 @@ int ddh_set_profiling(ddh_solver* s, int32_t enable) {
   if (!s) return -1;
   s->profile = enable;
   return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched solves: many solution intervals per launch.
+// ---------------------------------------------------------------------------
+struct ddh_batch {
+  std::vector<ddh_solver*> solvers;
+  std::vector<ddh_problem*> problems;
+  hipStream_t stream = nullptr;
+  int n_corr = 2;
+  bool f16 = false;
+  int waves = 2, unroll = 1, use_graph = 1, poll = 8;
+  int limit = 0;  // largest iteration limit over the intervals
+  size_t correlate_lds = 0, solve_lds = 0;
+  std::vector<ddh::IterArgs> args;
+  std::vector<ddh::BatchTile> tiles, items, pro_items;
+  ddh::DeviceBuffer<ddh::IterArgs> d_args;
+  ddh::DeviceBuffer<ddh::BatchTile> d_tiles, d_items, d_pro_items;
+  ddh::DeviceBuffer<ddh::DeviceState> d_states;
+  ddh::DeviceState* host_states = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_poll = 0, graph_waves = 0, graph_unroll = 0;
+  double last_ms = 0.0;
+};
+
+namespace ddh {
+namespace {
+
+void DestroyBatchGraph(ddh_batch& b) {
+  if (b.graph_exec) (void)hipGraphExecDestroy(b.graph_exec);
+  b.graph_exec = nullptr;
+}
+
+void BuildBatch(ddh_batch& b) {
+  const int n = int(b.solvers.size());
+  b.args.clear();
+  b.tiles.clear();
+  b.items.clear();
+  b.pro_items.clear();
+  b.d_states.Resize(n);
+  b.limit = 0;
+  b.correlate_lds = 0;
+  b.solve_lds = 0;
+  struct Costed {
+    BatchTile t;
+    double cost;
+  };
+  std::vector<Costed> costed;
+  for (int i = 0; i < n; ++i) {
+    ddh_solver& s = *b.solvers[i];
+    ddh_problem& p = *b.problems[i];
+    CheckMatch(s, p);
+    PrepareBuffers(s, p);
+    IterArgs a = MakeArgs(s, p);
+    a.state = b.d_states.ptr + i;
+    b.args.push_back(a);
+    b.limit = std::max(b.limit, std::max(s.cfg.c.max_iterations, s.cfg.c.min_iterations));
+    b.correlate_lds = std::max(b.correlate_lds, CorrelateSharedBytes(p.n_dir, p.n_corr, b.waves));
+    b.solve_lds = std::max(b.solve_lds, AntennaSolveSharedBytes(a, p.n_corr));
+    const double vis_bytes = double(p.DataVisBytes() + p.n_dir * p.ModelVisBytes());
+    for (int cb = 0; cb < p.n_cb; ++cb) {
+      for (int seg = 0; seg < p.n_seg; ++seg) {
+        const double rows = double(p.seg_t0[seg + 1] - p.seg_t0[seg]) * p.ch_per_cb[cb];
+        for (int tile = 0; tile < p.bl_stride / kWave; ++tile)
+          costed.push_back({BatchTile{i, cb, seg, tile}, rows * vis_bytes});
+      }
+      for (int d = 0; d < p.n_dir; ++d) b.items.push_back(BatchTile{i, cb, d, 0});
+      b.pro_items.push_back(BatchTile{i, cb, 0, 0});
+    }
+  }
+  // Longest tiles first: workgroups are dispatched in order, so this is a
+  // greedy longest-processing-time schedule over the compute units.
+  std::stable_sort(costed.begin(), costed.end(),
+                   [](const Costed& x, const Costed& y) { return x.cost > y.cost; });
+  for (const Costed& c : costed) b.tiles.push_back(c.t);
+  b.d_args.Upload(b.args, b.stream);
+  b.d_tiles.Upload(b.tiles, b.stream);
+  b.d_items.Upload(b.items, b.stream);
+  b.d_pro_items.Upload(b.pro_items, b.stream);
+  DDH_HIP_CHECK(hipStreamSynchronize(b.stream));
+  DestroyBatchGraph(b);
+}
+
+void LaunchBatchIteration(ddh_batch& b) {
+  LaunchBatch(b.d_args.ptr, b.d_tiles.ptr, int(b.tiles.size()), b.d_items.ptr,
+              int(b.items.size()), int(b.solvers.size()), b.correlate_lds, b.solve_lds, b.n_corr,
+              b.f16, b.waves, b.unroll, b.stream);
+}
+
+void EnsureBatchGraph(ddh_batch& b) {
+  if (b.graph_exec && b.graph_poll == b.poll && b.graph_waves == b.waves &&
+      b.graph_unroll == b.unroll)
+    return;
+  DestroyBatchGraph(b);
+  hipGraph_t graph = nullptr;
+  DDH_HIP_CHECK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < b.poll; ++i) LaunchBatchIteration(b);
+  DDH_HIP_CHECK(hipStreamEndCapture(b.stream, &graph));
+  const hipError_t e = hipGraphInstantiate(&b.graph_exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  DDH_HIP_CHECK(e);
+  b.graph_poll = b.poll;
+  b.graph_waves = b.waves;
+  b.graph_unroll = b.unroll;
+}
+
+void SolveBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results) {
+  const int n = int(b.solvers.size());
+  for (int i = 0; i < n; ++i) {
+    const ddh_solver& s = *b.solvers[i];
+    const size_t per_cb = size_t(s.cfg.c.n_antennas) * s.cfg.n_sub * 2;
+    for (int cb = 0; cb < s.cfg.c.n_channel_blocks; ++cb)
+      DDH_HIP_CHECK(hipMemcpyAsync(b.args[i].sol0 + cb * per_cb, solutions[i][cb],
+                                   per_cb * sizeof(double2), hipMemcpyHostToDevice, b.stream));
+  }
+  const bool graph = b.use_graph != 0;
+  if (graph) EnsureBatchGraph(b);
+  hipEvent_t t0, t1;
+  DDH_HIP_CHECK(hipEventCreate(&t0));
+  DDH_HIP_CHECK(hipEventCreate(&t1));
+  DDH_HIP_CHECK(hipEventRecord(t0, b.stream));
+  LaunchBatchPrologue(b.d_args.ptr, b.d_pro_items.ptr, int(b.pro_items.size()), b.stream);
+  int launched = 0;
+  bool all_done = false;
+  while (launched < b.limit && !all_done) {
+    if (graph) {
+      DDH_HIP_CHECK(hipGraphLaunch(b.graph_exec, b.stream));
+      launched += b.poll;
+    } else {
+      const int k = std::min(b.poll, b.limit - launched);
+      for (int i = 0; i < k; ++i) LaunchBatchIteration(b);
+      launched += k;
+    }
+    DDH_HIP_CHECK(hipMemcpyAsync(b.host_states, b.d_states.ptr, n * sizeof(DeviceState),
+                                 hipMemcpyDeviceToHost, b.stream));
+    DDH_HIP_CHECK(hipStreamSynchronize(b.stream));
+    all_done = true;
+    for (int i = 0; i < n; ++i) all_done = all_done && b.host_states[i].done;
+  }
+  DDH_HIP_CHECK(hipEventRecord(t1, b.stream));
+  DDH_HIP_CHECK(hipStreamSynchronize(b.stream));
+  DDH_REQUIRE(all_done, -3, "a batched interval did not reach its stopping criterion");
+  for (int i = 0; i < n; ++i) {
+    const ddh_solver& s = *b.solvers[i];
+    const DeviceState st = b.host_states[i];
+    const size_t per_cb = size_t(s.cfg.c.n_antennas) * s.cfg.n_sub * 2;
+    const double2* fin = (st.iteration & 1) ? b.args[i].sol1 : b.args[i].sol0;
+    for (int cb = 0; cb < s.cfg.c.n_channel_blocks; ++cb)
+      DDH_HIP_CHECK(hipMemcpyAsync(solutions[i][cb], fin + cb * per_cb, per_cb * sizeof(double2),
+                                   hipMemcpyDeviceToHost, b.stream));
+    if (results) {
+      results[i].iterations = st.has_converged ? st.iteration : st.iteration + 1;
+      results[i].constraint_iterations = 0;
+      results[i].has_converged = st.has_converged;
+      results[i].n_step_magnitudes = st.iteration;
+      results[i].last_step_magnitude = st.last_step;
+    }
+  }
+  DDH_HIP_CHECK(hipStreamSynchronize(b.stream));
+  float ms = 0.0f;
+  DDH_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+  b.last_ms = ms;
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+}
+
+// Mean duration of `reps` correlate launches under the current schedule.
+template <typename LaunchFn>
+double TimeLaunches(hipStream_t stream, int reps, LaunchFn&& launch) {
+  hipEvent_t e0, e1;
+  DDH_HIP_CHECK(hipEventCreate(&e0));
+  DDH_HIP_CHECK(hipEventCreate(&e1));
+  launch();  // warm-up
+  DDH_HIP_CHECK(hipEventRecord(e0, stream));
+  for (int r = 0; r < reps; ++r) launch();
+  DDH_HIP_CHECK(hipEventRecord(e1, stream));
+  DDH_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  DDH_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / reps;
+}
+
+// Candidate correlate schedules: (waves per tile, rows per trip).
+std::vector<std::pair<int, int>> ScheduleCandidates(int n_corr) {
+  std::vector<std::pair<int, int>> c;
+  for (int w : {1, 2, 3, 4})
+    for (int u : {1, 2})
+      if (u == 1 || n_corr == 2) c.push_back({w, u});
+  return c;
+}
+
+}  // namespace
+}  // namespace ddh
+
+extern "C" {
+
+ddh_batch* ddh_batch_create(ddh_solver* const* solvers, ddh_problem* const* problems,
+                            int32_t n_intervals) {
+  ddh_batch* out = nullptr;
+  const int rc = Guard([&] {
+    DDH_REQUIRE(solvers && problems && n_intervals >= 1, -1, "empty batch");
+    auto b = std::make_unique<ddh_batch>();
+    b->solvers.assign(solvers, solvers + n_intervals);
+    b->problems.assign(problems, problems + n_intervals);
+    for (int i = 0; i < n_intervals; ++i)
+      DDH_REQUIRE(b->solvers[i] && b->problems[i], -1, "NULL interval in batch");
+    b->n_corr = b->problems[0]->n_corr;
+    b->f16 = b->problems[0]->f16;
+    for (int i = 0; i < n_intervals; ++i)
+      DDH_REQUIRE(b->problems[i]->n_corr == b->n_corr && b->problems[i]->f16 == b->f16, -4,
+                  "all intervals of a batch share correlations and model storage");
+    DDH_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    DDH_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b->host_states),
+                                sizeof(DeviceState) * n_intervals, hipHostMallocDefault));
+    // solvers' buffers are used from the batch stream
+    for (ddh_solver* s : b->solvers) DDH_HIP_CHECK(hipStreamSynchronize(s->stream));
+    BuildBatch(*b);
+    out = b.release();
+  });
+  return rc == 0 ? out : nullptr;
+}
+
+void ddh_batch_destroy(ddh_batch* b) {
+  if (!b) return;
+  (void)hipStreamSynchronize(b->stream);
+  DestroyBatchGraph(*b);
+  if (b->host_states) (void)hipHostFree(b->host_states);
+  b->d_args.Free();
+  b->d_tiles.Free();
+  b->d_items.Free();
+  b->d_pro_items.Free();
+  b->d_states.Free();
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int ddh_batch_set_schedule(ddh_batch* b, int32_t waves_per_tile, int32_t rows_per_trip,
+                           int32_t use_graph, int32_t poll_interval) {
+  return Guard([&] {
+    DDH_REQUIRE(b, -1, "NULL batch");
+    DDH_REQUIRE(waves_per_tile >= 1 && waves_per_tile <= 4, -1, "waves_per_tile must be 1..4");
+    DDH_REQUIRE(rows_per_trip == 1 || rows_per_trip == 2, -1, "rows_per_trip must be 1 or 2");
+    DDH_REQUIRE(rows_per_trip == 1 || b->n_corr == 2, -4,
+                "two rows per trip are built for Duo (2-correlation) data only");
+    DDH_REQUIRE(poll_interval >= 1, -1, "poll_interval must be >= 1");
+    b->waves = waves_per_tile;
+    b->unroll = rows_per_trip;
+    b->use_graph = use_graph;
+    b->poll = poll_interval;
+    BuildBatch(*b);
+  });
+}
+
+int ddh_batch_solve(ddh_batch* b, double* const* const* solutions, ddh_result* results) {
+  return Guard([&] {
+    DDH_REQUIRE(b && solutions, -1, "NULL argument");
+    SolveBatch(*b, solutions, results);
+  });
+}
+
+int ddh_batch_last_time(const ddh_batch* b, double* ms) {
+  if (!b || !ms) return -1;
+  *ms = b->last_ms;
+  return 0;
+}
+
+int ddh_batch_autotune(ddh_batch* b, int32_t* waves_out, int32_t* rows_out, double* best_ms) {
+  return Guard([&] {
+    DDH_REQUIRE(b, -1, "NULL batch");
+    // unit gains and a reset state make every interval live for the timing
+    for (size_t i = 0; i < b->solvers.size(); ++i) {
+      const ddh_solver& s = *b->solvers[i];
+      const size_t n = size_t(s.cfg.c.n_antennas) * s.cfg.n_sub * 2 * s.cfg.c.n_channel_blocks;
+      std::vector<double2> ones(n, make_double2(1.0, 0.0));
+      DDH_HIP_CHECK(hipMemcpyAsync(b->args[i].sol0, ones.data(), n * sizeof(double2),
+                                   hipMemcpyHostToDevice, b->stream));
+      DDH_HIP_CHECK(hipStreamSynchronize(b->stream));
+    }
+    LaunchBatchPrologue(b->d_args.ptr, b->d_pro_items.ptr, int(b->pro_items.size()), b->stream);
+    double best = 1e30;
+    int bw = b->waves, bu = b->unroll;
+    for (const auto& cand : ScheduleCandidates(b->n_corr)) {
+      b->waves = cand.first;
+      b->unroll = cand.second;
+      BuildBatch(*b);
+      const double ms = TimeLaunches(b->stream, 5, [&] {
+        // correlate only: it does not change the solve state
+        LaunchBatch(b->d_args.ptr, b->d_tiles.ptr, int(b->tiles.size()), b->d_items.ptr, 0,
+                    0, b->correlate_lds, b->solve_lds, b->n_corr, b->f16, b->waves, b->unroll,
+                    b->stream);
+      });
+      if (ms < best) {
+        best = ms;
+        bw = cand.first;
+        bu = cand.second;
+      }
+    }
+    b->waves = bw;
+    b->unroll = bu;
+    BuildBatch(*b);
+    if (waves_out) *waves_out = bw;
+    if (rows_out) *rows_out = bu;
+    if (best_ms) *best_ms = best;
+  });
+}
+
+int ddh_autotune(ddh_solver* s, ddh_problem* p, int32_t* waves_out, int32_t* rows_out,
+                 double* best_ms) {
+  return Guard([&] {
+    DDH_REQUIRE(s && p, -1, "NULL argument");
+    CheckMatch(*s, *p);
+    PrepareBuffers(*s, *p);
+    const size_t n = size_t(s->cfg.c.n_antennas) * s->cfg.n_sub * 2 * s->cfg.c.n_channel_blocks;
+    std::vector<double2> ones(n, make_double2(1.0, 0.0));
+    DDH_HIP_CHECK(hipMemcpyAsync(s->sol0.ptr, ones.data(), n * sizeof(double2),
+                                 hipMemcpyHostToDevice, s->stream));
+    IterArgs a = MakeArgs(*s, *p);
+    LaunchPrologue(a, s->stream);
+    double best = 1e30;
+    int bw = s->waves, bu = s->unroll;
+    for (const auto& cand : ScheduleCandidates(p->n_corr)) {
+      const double ms = TimeLaunches(s->stream, 5, [&] {
+        LaunchCorrelate(a, p->n_corr, p->f16, cand.first, cand.second, s->stream);
+      });
+      if (ms < best) {
+        best = ms;
+        bw = cand.first;
+        bu = cand.second;
+      }
+    }
+    s->waves = bw;
+    s->unroll = bu;
+    DestroyGraph(*s);
+    if (waves_out) *waves_out = bw;
+    if (rows_out) *rows_out = bu;
+    if (best_ms) *best_ms = best;
+  });
 }
 
 }  // extern "C"

@@ -95,6 +95,16 @@ struct IterArgs {
   int32_t max_nsd;           // largest n_sol_dir entry
 };
 
+// One unit of batched work: correlate tiles use all four fields; antenna
+// solve items use (interval, cb) with the direction in `seg`; prologue items
+// use (interval, cb).
+struct BatchTile {
+  int32_t interval;
+  int32_t cb;
+  int32_t seg;
+  int32_t tile;
+};
+
 // Host-side launchers (defined in solver_kernels.hip).
 void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, int unroll,
                      hipStream_t stream);
@@ -105,6 +115,14 @@ size_t CorrelateSharedBytes(int n_dir, int n_corr, int waves);
 size_t MaxSharedBytes();
 size_t AntennaSolveSharedBytes(const IterArgs& a, int n_corr);
 void PrepareKernelAttributes();
+// One batched iteration: correlate over all tiles, antenna solve over all
+// items, then one decide workgroup per interval.
+void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, int n_tiles,
+                 const BatchTile* d_items, int n_items, int n_intervals, size_t correlate_lds,
+                 size_t solve_lds, int n_corr, bool f16, int waves, int unroll,
+                 hipStream_t stream);
+void LaunchBatchPrologue(const IterArgs* d_args, const BatchTile* d_items, int n_items,
+                         hipStream_t stream);
 
 // Layout conversion: reference order [plane][t][bl][ch][n_corr] complex64 ->
 // device order [plane][t][ch][bl_stride] (complex64 or FP16/scale).

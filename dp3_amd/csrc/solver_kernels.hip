@@ -76,12 +76,8 @@ __device__ __forceinline__ void LoadVis(const void* base, size_t idx, float2 (&v
 // order, so results do not depend on scheduling.
 // ---------------------------------------------------------------------------
 template <int ND, int NC, bool F16, int UNROLL>
-__global__ void __launch_bounds__(kCorrelateMaxThreads)
-    CorrelateKernel(const IterArgs a) {
-  if (a.state->done) return;
-  const int tile = blockIdx.x;
-  const int seg = blockIdx.y;
-  const int cb = blockIdx.z;
+__device__ __forceinline__ void CorrelateTile(const IterArgs& a, const int tile, const int seg,
+                                              const int cb) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int n_waves = blockDim.x / kWave;
@@ -93,33 +89,6 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads)
   extern __shared__ float smem[];
   float2* gg_s = reinterpret_cast<float2*>(smem);  // [ND][NC][64]
   float* red_s = smem + 2 * ND * NC * kWave;       // [3*ND*NC][64]
-
-  // Stage G_a,i * conj(G_b,j) of every (direction, correlation) for the 64
-  // baselines of the tile (solutions cast to float, as SolveData.h:230-235).
-  const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
-  const double2* sol = (a.state->iteration & 1) ? a.sol1 : a.sol0;
-  const double mean = a.mean[cb];
-  for (int idx = threadIdx.x; idx < ND * NC * kWave; idx += blockDim.x) {
-    const int l = idx & (kWave - 1);
-    const int dk = idx / kWave;
-    const int d = dk / NC;
-    const int k = dk - d * NC;
-    const int b = tile * kWave + l;
-    float2 g = make_float2(0.0f, 0.0f);
-    if (b < a.n_bl) {
-      const int s = a.seg_sol[seg * ND + d];
-      const double2 ga = FixedSolution(sol, gbase + size_t(a.ant1[b]) * a.n_sub + s, PolI<NC>(k), mean);
-      const double2 gb = FixedSolution(sol, gbase + size_t(a.ant2[b]) * a.n_sub + s, PolJ<NC>(k), mean);
-      g = cmulc(make_float2(float(ga.x), float(ga.y)), make_float2(float(gb.x), float(gb.y)));
-      if constexpr (F16) {
-        const float sc = a.model_scale[cb * ND + d];
-        g.x *= sc;
-        g.y *= sc;
-      }
-    }
-    gg_s[idx] = g;
-  }
-  __syncthreads();
 
   float ur[ND][NC], ui[ND][NC], sm[ND][NC];
 #pragma unroll
@@ -133,49 +102,111 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads)
 
   const int r0 = a.seg_t0[seg] * n_ch;
   const int r1 = a.seg_t0[seg + 1] * n_ch;
-  // UNROLL rows per trip: all loads of the trip are issued before any use.
-  // A row past the segment end reads as zeros, which adds exactly nothing.
-  for (int row0 = r0 + wave; row0 < r1; row0 += UNROLL * n_waves) {
-    float2 r[UNROLL][NC];
-    float2 m[UNROLL][ND][NC];
+  // Rows of the segment, strided by the number of waves. With UNROLL == 2 the
+  // loop is software pipelined: the next row's loads are in flight while the
+  // current row is computed. A row past the segment end reads as zeros, which
+  // adds exactly nothing.
+  auto load_row = [&](int row, float2 (&r)[NC], float2 (&m)[ND][NC]) {
+    if (row < r1) {
+      const size_t e = size_t(row) * a.bl_stride + bl;
+      LoadVis<NC, false>(a.data, info.data_off + e, r);
 #pragma unroll
-    for (int u = 0; u < UNROLL; ++u) {
-      const int row = row0 + u * n_waves;
-      if (u == 0 || row < r1) {
-        const size_t e = size_t(row) * a.bl_stride + bl;
-        LoadVis<NC, false>(a.data, info.data_off + e, r[u]);
+      for (int d = 0; d < ND; ++d) LoadVis<NC, F16>(a.model, info.model_off + d * plane + e, m[d]);
+    } else {
 #pragma unroll
-        for (int d = 0; d < ND; ++d)
-          LoadVis<NC, F16>(a.model, info.model_off + d * plane + e, m[u][d]);
-      } else {
+      for (int k = 0; k < NC; ++k) {
+        r[k] = make_float2(0.0f, 0.0f);
 #pragma unroll
-        for (int k = 0; k < NC; ++k) {
-          r[u][k] = make_float2(0.0f, 0.0f);
-#pragma unroll
-          for (int d = 0; d < ND; ++d) m[u][d][k] = make_float2(0.0f, 0.0f);
-        }
+        for (int d = 0; d < ND; ++d) m[d][k] = make_float2(0.0f, 0.0f);
       }
     }
+  };
+  auto accumulate = [&](float2 (&r)[NC], const float2 (&m)[ND][NC]) {
+    // Full residual, directions subtracted in order (IterativeDiagonalSolver.cc:108-111).
 #pragma unroll
-    for (int u = 0; u < UNROLL; ++u) {
-      // Full residual, directions subtracted in order (IterativeDiagonalSolver.cc:108-111).
+    for (int d = 0; d < ND; ++d)
 #pragma unroll
-      for (int d = 0; d < ND; ++d)
+      for (int k = 0; k < NC; ++k) {
+        const float2 c = cmul(gg_s[(d * NC + k) * kWave + lane], m[d][k]);
+        r[k].x -= c.x;
+        r[k].y -= c.y;
+      }
 #pragma unroll
-        for (int k = 0; k < NC; ++k) {
-          const float2 c = cmul(gg_s[(d * NC + k) * kWave + lane], m[u][d][k]);
-          r[u][k].x -= c.x;
-          r[u][k].y -= c.y;
-        }
+    for (int d = 0; d < ND; ++d)
 #pragma unroll
-      for (int d = 0; d < ND; ++d)
+      for (int k = 0; k < NC; ++k) {
+        const float2 mm = m[d][k];
+        ur[d][k] += r[k].x * mm.x + r[k].y * mm.y;
+        ui[d][k] += r[k].y * mm.x - r[k].x * mm.y;
+        sm[d][k] += mm.x * mm.x + mm.y * mm.y;
+      }
+  };
+  // The first rows' loads do not depend on the gains: issue them before the
+  // staging so that its memory round trips overlap with them.
+  float2 ra[NC], rb[NC];
+  float2 ma[ND][NC], mb[ND][NC];
+  load_row(r0 + wave, ra, ma);
+  // Stage G_a,i * conj(G_b,j) of every (direction, correlation) for the 64
+  // baselines of the tile (solutions repaired, then cast to float, as
+  // SolveData.h:230-235). Wave 0 does it, lane = baseline; the loads of all
+  // directions are independent and issued together, so staging costs about
+  // two memory round trips instead of one per loop trip.
+  if (wave == 0) {
+    const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
+    const double2* sol = (a.state->iteration & 1) ? a.sol1 : a.sol0;
+    const double mean = a.mean[cb];
+    const bool valid = bl < a.n_bl;
+    const int a1 = valid ? a.ant1[bl] : 0;
+    const int a2 = valid ? a.ant2[bl] : 0;
+    double2 g1[ND][2], g2[ND][2];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) {
-          const float2 mm = m[u][d][k];
-          ur[d][k] += r[u][k].x * mm.x + r[u][k].y * mm.y;
-          ui[d][k] += r[u][k].y * mm.x - r[u][k].x * mm.y;
-          sm[d][k] += mm.x * mm.x + mm.y * mm.y;
-        }
+    for (int d = 0; d < ND; ++d) {
+      const int s = a.seg_sol[seg * ND + d];
+      const size_t p1 = gbase + size_t(a1) * a.n_sub + s;
+      const size_t p2 = gbase + size_t(a2) * a.n_sub + s;
+      g1[d][0] = sol[p1 * 2];
+      g1[d][1] = sol[p1 * 2 + 1];
+      g2[d][0] = sol[p2 * 2];
+      g2[d][1] = sol[p2 * 2 + 1];
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      // MakeSolutionsFinite2Pol: a non-finite pair reads as (mean, mean).
+      if (!(cfinite(g1[d][0]) && cfinite(g1[d][1])))
+        g1[d][0] = g1[d][1] = make_double2(mean, 0.0);
+      if (!(cfinite(g2[d][0]) && cfinite(g2[d][1])))
+        g2[d][0] = g2[d][1] = make_double2(mean, 0.0);
+      float sc = 1.0f;
+      if constexpr (F16) sc = a.model_scale[cb * ND + d];
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        const double2 ga = g1[d][PolI<NC>(k)];
+        const double2 gb = g2[d][PolJ<NC>(k)];
+        float2 g = cmulc(make_float2(float(ga.x), float(ga.y)),
+                         make_float2(float(gb.x), float(gb.y)));
+        g.x *= sc;
+        g.y *= sc;
+        gg_s[(d * NC + k) * kWave + lane] = valid ? g : make_float2(0.0f, 0.0f);
+      }
+    }
+  }
+  __syncthreads();
+
+  if constexpr (UNROLL == 1) {
+    accumulate(ra, ma);
+    for (int row = r0 + wave + n_waves; row < r1; row += n_waves) {
+      float2 r[NC];
+      float2 m[ND][NC];
+      load_row(row, r, m);
+      accumulate(r, m);
+    }
+  } else {
+    int row = r0 + wave;
+    for (; row < r1; row += 2 * n_waves) {
+      load_row(row + n_waves, rb, mb);
+      accumulate(ra, ma);
+      load_row(row + 2 * n_waves, ra, ma);
+      accumulate(rb, mb);
     }
   }
 
@@ -220,6 +251,12 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads)
   }
 }
 
+template <int ND, int NC, bool F16, int UNROLL>
+__global__ void __launch_bounds__(kCorrelateMaxThreads) CorrelateKernel(const IterArgs a) {
+  if (a.state->done) return;
+  CorrelateTile<ND, NC, F16, UNROLL>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 // Solutions of MakeSolutionsFinite2Pol (SolverBase.cc:342-371), applied on
 // the fly: a pair with a non-finite entry reads as (mean, mean), where mean is
 // the block's average |pol-0 value| over finite pairs (:354), kept per block.
@@ -229,6 +266,61 @@ __device__ __forceinline__ double2 FixedSolution(const double2* sol, size_t pair
   const double2 g1 = sol[pair * 2 + 1];
   if (!(cfinite(g0) && cfinite(g1))) return make_double2(mean, 0.0);
   return pol ? g1 : g0;
+}
+
+// Copies n 4-byte words from global memory to LDS with 16-byte loads, four
+// independent loads in flight per thread (src and dst 16-byte aligned).
+template <typename T>
+__device__ void StageWords(const T* __restrict__ src, T* dst, int n) {
+  static_assert(sizeof(T) == 4, "4-byte words");
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  const int n4 = n / 4;
+  const u4* s4 = reinterpret_cast<const u4*>(src);
+  u4* d4 = reinterpret_cast<u4*>(dst);
+  const int bd = blockDim.x;
+  int i = threadIdx.x;
+  for (; i + 3 * bd < n4; i += 4 * bd) {
+    const u4 v0 = s4[i], v1 = s4[i + bd], v2 = s4[i + 2 * bd], v3 = s4[i + 3 * bd];
+    d4[i] = v0;
+    d4[i + bd] = v1;
+    d4[i + 2 * bd] = v2;
+    d4[i + 3 * bd] = v3;
+  }
+  for (; i < n4; i += bd) d4[i] = s4[i];
+  for (int j = n4 * 4 + threadIdx.x; j < n; j += bd) dst[j] = src[j];
+}
+
+// Block-wide deterministic sums of two (double, int) pairs: a fixed xor
+// shuffle tree inside each wave, then the per-wave results in wave order.
+// `red` needs 2 * (blockDim.x / 64) doubles, `redc` as many ints.
+__device__ void BlockSum2(double& v, int& c, double& w, int& e, double* red, int* redc) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v += __shfl_xor(v, off);
+    c += __shfl_xor(c, off);
+    w += __shfl_xor(w, off);
+    e += __shfl_xor(e, off);
+  }
+  const int n_waves = blockDim.x / 64;
+  const int wave = threadIdx.x / 64;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    red[wave] = v;
+    redc[wave] = c;
+    red[n_waves + wave] = w;
+    redc[n_waves + wave] = e;
+  }
+  __syncthreads();
+  v = 0.0;
+  c = 0;
+  w = 0.0;
+  e = 0;
+  for (int i = 0; i < n_waves; ++i) {
+    v += red[i];
+    c += redc[i];
+    w += red[n_waves + i];
+    e += redc[n_waves + i];
+  }
+  __syncthreads();
 }
 
 // Block-wide deterministic sum (fixed tree) of one double and one int.
@@ -263,15 +355,12 @@ __device__ void BlockSum(double& v, int& c, double* red, int* redc) {
 // Solutions ping-pong between sol0 and sol1 by iteration parity.
 // ---------------------------------------------------------------------------
 template <int NC, bool F16>
-__global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterArgs a) {
-  __shared__ double red[kBlockThreads];
-  __shared__ int redc[kBlockThreads];
-  if (a.state->done) return;
+__device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d, const int cb) {
+  __shared__ double red[2 * kBlockThreads / 64];
+  __shared__ int redc[2 * kBlockThreads / 64];
   const int iteration = a.state->iteration;
   const double2* cur = (iteration & 1) ? a.sol1 : a.sol0;
   double2* nxt = (iteration & 1) ? a.sol0 : a.sol1;
-  const int d = blockIdx.x;
-  const int cb = blockIdx.y;
   const int nsd = a.n_sol_dir[d];
   const int s0 = a.sol_start[d];
   const int n_items = a.n_ant * nsd;
@@ -296,21 +385,20 @@ __global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterAr
   float* part_s = reinterpret_cast<float*>(g_s + size_t(a.n_ant) * nsd * 2);  // [NC][3][bs]
   int* a1_s = reinterpret_cast<int*>(part_s + NC * 3 * bs);      // [bs]
   int* a2_s = a1_s + bs;                                         // [bs]
-  int* off_s = a2_s + bs;                                        // [n_ant + 1]
-  int* csr_s = off_s + a.n_ant + 1;                              // [2 n_bl]
-  for (int i = threadIdx.x; i < a.n_ant * nsd * 2; i += blockDim.x) {
-    const int an = i / (nsd * 2);
-    const int rem = i - an * nsd * 2;
-    const double2 g = FixedSolution(cur, gbase + size_t(an) * a.n_sub + s0 + (rem >> 1),
-                                    rem & 1, mean);
-    g_s[i] = make_double2(double(float(g.x)), double(float(g.y)));
+  int* csr_s = a2_s + bs;                                        // [2 n_bl], padded to 4
+  int* off_s = csr_s + ((2 * a.n_bl + 3) & ~3);                  // [n_ant + 1]
+  for (int i = threadIdx.x; i < a.n_ant * nsd; i += blockDim.x) {
+    const int an = i / nsd;
+    const size_t pair = gbase + size_t(an) * a.n_sub + s0 + (i - an * nsd);
+    double2 g0 = cur[pair * 2], g1 = cur[pair * 2 + 1];
+    if (!(cfinite(g0) && cfinite(g1))) g0 = g1 = make_double2(mean, 0.0);
+    g_s[2 * i] = make_double2(double(float(g0.x)), double(float(g0.y)));
+    g_s[2 * i + 1] = make_double2(double(float(g1.x)), double(float(g1.y)));
   }
-  for (int i = threadIdx.x; i < bs; i += blockDim.x) {
-    a1_s[i] = a.ant1[i];
-    a2_s[i] = a.ant2[i];
-  }
+  StageWords(a.ant1, a1_s, bs);
+  StageWords(a.ant2, a2_s, bs);
+  StageWords(a.csr, csr_s, 2 * a.n_bl);
   for (int i = threadIdx.x; i <= a.n_ant; i += blockDim.x) off_s[i] = a.csr_off[i];
-  for (int i = threadIdx.x; i < 2 * a.n_bl; i += blockDim.x) csr_s[i] = a.csr[i];
   const int rel = s_abs - s0;
   auto gain = [&](int an, int pol) { return g_s[(an * nsd + rel) * 2 + pol]; };
 
@@ -323,7 +411,7 @@ __global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterAr
     const float* src =
         a.partials + ((size_t(cb) * a.n_seg + seg) * a.n_dir + d) * NC * 3 * size_t(bs);
     __syncthreads();
-    for (int i = threadIdx.x; i < NC * 3 * bs; i += blockDim.x) part_s[i] = src[i];
+    StageWords(src, part_s, NC * 3 * bs);
     __syncthreads();
     if (!active || s_seg != s_abs) continue;
     for (int e = off_s[ant] + q; e < off_s[ant + 1]; e += split) {
@@ -419,8 +507,7 @@ __global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterAr
       mean_n = 1;
     }
   }
-  BlockSum(conv_sum, conv_n, red, redc);
-  BlockSum(mean_sum, mean_n, red, redc);
+  BlockSum2(conv_sum, conv_n, mean_sum, mean_n, red, redc);
   const int wg = cb * a.n_dir + d;
   if (threadIdx.x == 0) {
     a.wg_conv[wg] = conv_sum;
@@ -431,12 +518,17 @@ __global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterAr
 
 }
 
+template <int NC, bool F16>
+__global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterArgs a) {
+  if (a.state->done) return;
+  AntennaSolveBlock<NC, F16>(a, blockIdx.x, blockIdx.y);
+}
+
 // One workgroup: the iteration's decision from the per-workgroup partials of
 // antenna_solve (the kernel boundary makes them visible without fences).
-__global__ void __launch_bounds__(kBlockThreads) DecideKernel(const IterArgs a) {
+__device__ __forceinline__ void DecideBlock(const IterArgs& a) {
   __shared__ double red[kBlockThreads];
   __shared__ int redc[kBlockThreads];
-  if (a.state->done) return;
   const int n_wg = a.n_cb * a.n_dir;
   double sum = 0.0;
   int count = 0;
@@ -488,29 +580,84 @@ __global__ void __launch_bounds__(kBlockThreads) DecideKernel(const IterArgs a) 
   }
 }
 
+__global__ void __launch_bounds__(kBlockThreads) DecideKernel(const IterArgs a) {
+  if (a.state->done) return;
+  DecideBlock(a);
+}
+
 // Resets the solve state and computes the repair means of the initial
-// solutions (the first MakeSolutionsFinite2Pol of the loop).
-__global__ void __launch_bounds__(kBlockThreads) PrologueKernel(const IterArgs a) {
-  __shared__ double red[kBlockThreads];
-  __shared__ int redc[kBlockThreads];
-  if (threadIdx.x == 0) {
+// solutions (the first MakeSolutionsFinite2Pol of the loop); one workgroup
+// per channel block.
+__device__ __forceinline__ void PrologueBlock(const IterArgs& a, const int cb) {
+  __shared__ double red[2 * kBlockThreads / 64];
+  __shared__ int redc[2 * kBlockThreads / 64];
+  if (cb == 0 && threadIdx.x == 0) {
     DeviceState st = {};
     *a.state = st;
   }
   const int n_pairs = a.n_ant * a.n_sub;
-  for (int cb = 0; cb < a.n_cb; ++cb) {
-    const double2* s = a.sol0 + size_t(cb) * n_pairs * 2;
-    double sum = 0.0;
-    int count = 0;
-    for (int i = threadIdx.x; i < n_pairs; i += blockDim.x) {
-      if (cfinite(s[2 * i]) && cfinite(s[2 * i + 1])) {
-        sum += hypot(s[2 * i].x, s[2 * i].y);
-        ++count;
-      }
+  const double2* s = a.sol0 + size_t(cb) * n_pairs * 2;
+  double sum = 0.0, unused = 0.0;
+  int count = 0, unused_n = 0;
+  for (int i = threadIdx.x; i < n_pairs; i += blockDim.x) {
+    if (cfinite(s[2 * i]) && cfinite(s[2 * i + 1])) {
+      sum += hypot(s[2 * i].x, s[2 * i].y);
+      ++count;
     }
-    BlockSum(sum, count, red, redc);
-    if (threadIdx.x == 0) a.mean[cb] = count == 0 ? 1.0 : sum / count;
   }
+  BlockSum2(sum, count, unused, unused_n, red, redc);
+  if (threadIdx.x == 0) a.mean[cb] = count == 0 ? 1.0 : sum / count;
+}
+
+__global__ void __launch_bounds__(kBlockThreads) PrologueKernel(const IterArgs a) {
+  PrologueBlock(a, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Batched kernels: many solution intervals (each with its own IterArgs,
+// shapes and device state) in one launch per kernel. A tile / item list maps
+// blockIdx.x to (interval, block, segment, tile); the direction count is a
+// per-interval switch over the compiled variants.
+// ---------------------------------------------------------------------------
+template <int NC, bool F16, int UNROLL>
+__global__ void __launch_bounds__(kCorrelateMaxThreads)
+    BatchCorrelateKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ tiles) {
+  const BatchTile t = tiles[blockIdx.x];
+  const IterArgs& a = args[t.interval];
+  if (a.state->done) return;
+  switch (a.n_dir) {
+#define DDH_CASE(N) \
+  case N:           \
+    CorrelateTile<N, NC, F16, UNROLL>(a, t.tile, t.seg, t.cb); \
+    break;
+    DDH_CASE(1) DDH_CASE(2) DDH_CASE(3) DDH_CASE(4) DDH_CASE(5) DDH_CASE(6) DDH_CASE(7)
+    DDH_CASE(8) DDH_CASE(9) DDH_CASE(10) DDH_CASE(11) DDH_CASE(12) DDH_CASE(13)
+    DDH_CASE(14) DDH_CASE(15) DDH_CASE(16)
+#undef DDH_CASE
+    default:
+      break;
+  }
+}
+
+template <int NC, bool F16>
+__global__ void __launch_bounds__(kBlockThreads)
+    BatchAntennaSolveKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ items) {
+  const BatchTile t = items[blockIdx.x];
+  const IterArgs& a = args[t.interval];
+  if (a.state->done) return;
+  AntennaSolveBlock<NC, F16>(a, t.seg, t.cb);  // seg holds the direction
+}
+
+__global__ void __launch_bounds__(256) BatchDecideKernel(const IterArgs* __restrict__ args) {
+  const IterArgs& a = args[blockIdx.x];
+  if (a.state->done) return;
+  DecideBlock(a);
+}
+
+__global__ void __launch_bounds__(kBlockThreads)
+    BatchPrologueKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ items) {
+  const BatchTile t = items[blockIdx.x];
+  PrologueBlock(args[t.interval], t.cb);
 }
 
 // ---------------------------------------------------------------------------
@@ -649,7 +796,65 @@ void PrepareKernelAttributes() {
     g_max_lds_solve = std::min(g_max_lds_solve, dyn);
     DDH_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
   }
+  for (const void* f : {reinterpret_cast<const void*>(&BatchCorrelateKernel<2, false, 1>),
+                        reinterpret_cast<const void*>(&BatchCorrelateKernel<2, false, 2>),
+                        reinterpret_cast<const void*>(&BatchCorrelateKernel<2, true, 1>),
+                        reinterpret_cast<const void*>(&BatchCorrelateKernel<2, true, 2>),
+                        reinterpret_cast<const void*>(&BatchCorrelateKernel<4, false, 1>)})
+    DDH_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  for (const void* f : {reinterpret_cast<const void*>(&BatchAntennaSolveKernel<2, false>),
+                        reinterpret_cast<const void*>(&BatchAntennaSolveKernel<2, true>),
+                        reinterpret_cast<const void*>(&BatchAntennaSolveKernel<4, false>)}) {
+    hipFuncAttributes attr{};
+    DDH_HIP_CHECK(hipFuncGetAttributes(&attr, f));
+    const int dyn = max_lds - int(attr.sharedSizeBytes);
+    g_max_lds_solve = std::min(g_max_lds_solve, dyn);
+    DDH_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+  }
   done = true;
+}
+
+void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, int n_tiles,
+                 const BatchTile* d_items, int n_items, int n_intervals, size_t correlate_lds,
+                 size_t solve_lds, int n_corr, bool f16, int waves, int unroll,
+                 hipStream_t stream) {
+  DDH_REQUIRE(correlate_lds <= MaxSharedBytes(), -4, "batch correlate LDS exceeds the device");
+  DDH_REQUIRE(solve_lds <= size_t(g_max_lds_solve), -4, "batch antenna solve LDS exceeds the device");
+  DDH_REQUIRE(n_corr == 2 || (!f16 && unroll == 1), -4,
+              "batched Full (4-correlation) data runs complex64 with one row per trip");
+  const dim3 cgrid(n_tiles), cblock(waves * kWave);
+  if (n_corr == 2) {
+    if (f16) {
+      if (unroll == 2)
+        hipLaunchKernelGGL((BatchCorrelateKernel<2, true, 2>), cgrid, cblock, correlate_lds, stream, d_args, d_tiles);
+      else
+        hipLaunchKernelGGL((BatchCorrelateKernel<2, true, 1>), cgrid, cblock, correlate_lds, stream, d_args, d_tiles);
+      if (n_items > 0)
+        hipLaunchKernelGGL((BatchAntennaSolveKernel<2, true>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
+    } else {
+      if (unroll == 2)
+        hipLaunchKernelGGL((BatchCorrelateKernel<2, false, 2>), cgrid, cblock, correlate_lds, stream, d_args, d_tiles);
+      else
+        hipLaunchKernelGGL((BatchCorrelateKernel<2, false, 1>), cgrid, cblock, correlate_lds, stream, d_args, d_tiles);
+      if (n_items > 0)
+        hipLaunchKernelGGL((BatchAntennaSolveKernel<2, false>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
+    }
+  } else {
+    hipLaunchKernelGGL((BatchCorrelateKernel<4, false, 1>), cgrid, cblock, correlate_lds, stream, d_args, d_tiles);
+    if (n_items > 0)
+      hipLaunchKernelGGL((BatchAntennaSolveKernel<4, false>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
+  }
+  DDH_HIP_CHECK(hipGetLastError());
+  if (n_intervals > 0)
+    hipLaunchKernelGGL(BatchDecideKernel, dim3(n_intervals), dim3(256), 0, stream, d_args);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchBatchPrologue(const IterArgs* d_args, const BatchTile* d_items, int n_items,
+                         hipStream_t stream) {
+  hipLaunchKernelGGL(BatchPrologueKernel, dim3(n_items), dim3(kBlockThreads), 0, stream, d_args,
+                     d_items);
+  DDH_HIP_CHECK(hipGetLastError());
 }
 
 void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, int unroll,
@@ -667,7 +872,8 @@ size_t AntennaSolveSharedBytes(const IterArgs& a, int n_corr) {
   const int max_nsd = std::max(1, a.max_nsd);
   return size_t(a.n_ant) * max_nsd * 2 * sizeof(double2) +
          size_t(n_corr) * 3 * a.bl_stride * sizeof(float) +
-         (size_t(2) * a.bl_stride + a.n_ant + 1 + 2 * size_t(a.n_bl)) * sizeof(int);
+         (size_t(2) * a.bl_stride + ((2 * size_t(a.n_bl) + 3) & ~size_t(3)) + a.n_ant + 1) *
+             sizeof(int);
 }
 
 void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, hipStream_t stream) {
@@ -694,7 +900,7 @@ void LaunchDecide(const IterArgs& a, hipStream_t stream) {
 }
 
 void LaunchPrologue(const IterArgs& a, hipStream_t stream) {
-  hipLaunchKernelGGL(PrologueKernel, dim3(1), dim3(kBlockThreads), 0, stream, a);
+  hipLaunchKernelGGL(PrologueKernel, dim3(a.n_cb), dim3(kBlockThreads), 0, stream, a);
   DDH_HIP_CHECK(hipGetLastError());
 }
 

@@ -120,6 +120,16 @@ SIGNATURES = {
                                      C.c_float, C.c_uint32], C.c_int),
     "ddh_calibrate_bandwidth": ([C.c_int64, C.c_int32, C.POINTER(C.c_double),
                                  C.POINTER(C.c_double)], C.c_int),
+    "ddh_batch_create": ([C.c_void_p, C.c_void_p, C.c_int32], C.c_void_p),
+    "ddh_batch_destroy": ([C.c_void_p], None),
+    "ddh_batch_set_schedule": ([C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32],
+                               C.c_int),
+    "ddh_batch_solve": ([C.c_void_p, C.c_void_p, C.POINTER(Result)], C.c_int),
+    "ddh_batch_last_time": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "ddh_autotune": ([C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                      C.POINTER(C.c_double)], C.c_int),
+    "ddh_batch_autotune": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                            C.POINTER(C.c_double)], C.c_int),
 }
 
 
@@ -268,6 +278,70 @@ class Solver:
                                        C.byref(res)), "ddh_solve_problem")
         return SolveResult(res.iterations, res.constraint_iterations, bool(res.has_converged),
                            res.n_step_magnitudes, res.last_step_magnitude)
+
+
+def autotune(solver, dproblem):
+    """Picks the fastest correlate schedule for this problem; returns
+    (waves_per_tile, rows_per_trip, correlate ms)."""
+    w, r, ms = C.c_int32(), C.c_int32(), C.c_double()
+    _check(lib().ddh_autotune(solver.handle, dproblem.handle, C.byref(w), C.byref(r),
+                              C.byref(ms)), "ddh_autotune")
+    return w.value, r.value, ms.value
+
+
+class Batch:
+    """Many solution intervals solved together (one launch per kernel per
+    iteration over all intervals)."""
+
+    def __init__(self, solvers, dproblems):
+        self.solvers = list(solvers)
+        self.dproblems = list(dproblems)
+        n = len(self.solvers)
+        sh = (C.c_void_p * n)(*[s.handle for s in self.solvers])
+        ph = (C.c_void_p * n)(*[p.handle for p in self.dproblems])
+        h = lib().ddh_batch_create(C.cast(sh, C.c_void_p), C.cast(ph, C.c_void_p), n)
+        if not h:
+            raise HipError(f"ddh_batch_create failed: {last_error()}")
+        self.handle = h
+
+    def set_schedule(self, waves_per_tile=2, rows_per_trip=1, use_graph=1, poll_interval=8):
+        _check(lib().ddh_batch_set_schedule(self.handle, waves_per_tile, rows_per_trip,
+                                            use_graph, poll_interval), "ddh_batch_set_schedule")
+
+    def autotune(self):
+        w, r, ms = C.c_int32(), C.c_int32(), C.c_double()
+        _check(lib().ddh_batch_autotune(self.handle, C.byref(w), C.byref(r), C.byref(ms)),
+               "ddh_batch_autotune")
+        return w.value, r.value, ms.value
+
+    def solve(self, solutions):
+        """solutions: list of [n_cb, n_ant*n_sub*2] complex128 arrays, in place."""
+        n = len(self.solvers)
+        assert len(solutions) == n
+        inner = []
+        for sol in solutions:
+            assert sol.dtype == np.complex128 and sol.flags.c_contiguous
+            inner.append((C.c_void_p * sol.shape[0])(*[sol[cb].ctypes.data
+                                                       for cb in range(sol.shape[0])]))
+        outer = (C.c_void_p * n)(*[C.cast(x, C.c_void_p) for x in inner])
+        res = (Result * n)()
+        _check(lib().ddh_batch_solve(self.handle, C.cast(outer, C.c_void_p), res),
+               "ddh_batch_solve")
+        return [SolveResult(r.iterations, r.constraint_iterations, bool(r.has_converged),
+                            r.n_step_magnitudes, r.last_step_magnitude) for r in res]
+
+    def last_time_ms(self):
+        ms = C.c_double()
+        _check(lib().ddh_batch_last_time(self.handle, C.byref(ms)), "ddh_batch_last_time")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().ddh_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
 
 
 class DeviceProblem:

@@ -182,6 +182,30 @@ int ddh_set_schedule(ddh_solver* solver, int32_t waves_per_tile,
 // (ddh_solver_last_timing then reports their summed duration).
 int ddh_set_profiling(ddh_solver* solver, int32_t enable);
 
+// Batched solves: many solution intervals (each its own solver
+// configuration and device problem; all with the same correlation count and
+// model storage) advance together, one launch per kernel per iteration over
+// all intervals' tiles, until every interval met its stopping criterion.
+// Replaces DDECal's serial loop over buffered solution intervals
+// (steps/DDECal.cc:595-708). Solvers and problems must outlive the batch.
+typedef struct ddh_batch ddh_batch;
+ddh_batch* ddh_batch_create(ddh_solver* const* solvers, ddh_problem* const* problems,
+                            int32_t n_intervals);
+void ddh_batch_destroy(ddh_batch* batch);
+int ddh_batch_set_schedule(ddh_batch* batch, int32_t waves_per_tile, int32_t rows_per_trip,
+                           int32_t use_graph, int32_t poll_interval);
+// solutions[i][cb] -> interval i, channel block cb ([n_ant][n_sub][2] complex128).
+int ddh_batch_solve(ddh_batch* batch, double* const* const* solutions, ddh_result* results);
+int ddh_batch_last_time(const ddh_batch* batch, double* ms);
+
+// Schedule autotuner: times the correlate kernel under every candidate
+// schedule (waves per tile x rows per trip) on the actual shapes and keeps
+// the fastest. Leaves the solve state reset (unit gains).
+int ddh_autotune(ddh_solver* solver, ddh_problem* problem, int32_t* waves_per_tile,
+                 int32_t* rows_per_trip, double* best_ms);
+int ddh_batch_autotune(ddh_batch* batch, int32_t* waves_per_tile, int32_t* rows_per_trip,
+                       double* best_ms);
+
 // Bandwidth calibration: a plain copy and a plain streaming read over
 // `bytes` (each repeated `repeats` times). copy_gbs counts bytes read plus
 // written. No counterpart in the reference; used to put the solver's HBM
