@@ -1,0 +1,599 @@
+// Warm-start network: a 3-layer perceptron (n_in -> hidden -> hidden -> n_out,
+// ReLU) trained and evaluated on the GPU with bf16 MFMA GEMMs (FP32
+// accumulation, FP32 master weights, Adam). It maps, per (channel block,
+// antenna, solution), the previous interval's solution and summaries of the
+// current interval's data/model correlation to a multiplicative correction of
+// the initial gains; the plain solver then runs to its stopping criterion
+// from that start.
+//
+// No counterpart in the reference: DP3's only warm starts are
+// propagatesolutions and H5Parm initial solutions (steps/DDECal.cc:483-531).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "../../include/ddecal_hip.h"
+#include "common.h"
+
+namespace ddh {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float float16_v __attribute__((ext_vector_type(16)));
+
+// FP32 -> bf16 bits, round to nearest even (NaN kept quiet).
+__host__ __device__ inline uint16_t ToBf16(float f) {
+  const uint32_t u = __builtin_bit_cast(uint32_t, f);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return uint16_t((u >> 16) | 0x40);
+  const uint32_t rounding = 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t((u + rounding) >> 16);
+}
+__host__ __device__ inline float FromBf16(uint16_t b) {
+  return __builtin_bit_cast(float, uint32_t(b) << 16);
+}
+
+// C[M x N] (fp32, row-major, ldc) = opA(A)[M x K] * opB(B)[K x N] (+ bias[N])
+// (ReLU) for bf16 operands with arbitrary strides:
+//   opA(A)(m, k) = trans_a ? A[k * lda + m] : A[m * lda + k]
+//   opB(B)(k, n) = trans_b ? B[n * ldb + k] : B[k * ldb + n]
+// One wave per 32 x 32 tile of C; K advances 16 per v_mfma_f32_32x32x16_bf16.
+// Operand layout of that instruction (wave64): lane l supplies row (A) or
+// column (B) l % 32 and the 8 k values 8 * (l / 32) .. + 7; the 16 results of
+// lane l are C(8 * (i / 4) + 4 * (l / 32) + i % 4, l % 32), i = 0..15.
+__global__ void __launch_bounds__(256) GemmBf16Kernel(const uint16_t* __restrict__ A, int lda,
+                                                      int trans_a,
+                                                      const uint16_t* __restrict__ B, int ldb,
+                                                      int trans_b, float* __restrict__ C,
+                                                      int ldc, int M, int N, int K,
+                                                      const float* __restrict__ bias, int relu,
+                                                      int accumulate) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int tiles_n = (N + 31) / 32;
+  const int tile = blockIdx.x * 4 + wave;
+  const int tm = tile / tiles_n;
+  const int tn = tile - tm * tiles_n;
+  if (tm * 32 >= M) return;
+  const int m = tm * 32 + (lane & 31);
+  const int n = tn * 32 + (lane & 31);
+  const int kg = (lane >> 5) * 8;
+  float16_v acc;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    bf16x8 a, b;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = k0 + kg + j;
+      uint16_t av = 0, bv = 0;
+      if (m < M && k < K) av = trans_a ? A[size_t(k) * lda + m] : A[size_t(m) * lda + k];
+      if (n < N && k < K) bv = trans_b ? B[size_t(n) * ldb + k] : B[size_t(k) * ldb + n];
+      a[j] = __builtin_bit_cast(__bf16, av);
+      b[j] = __builtin_bit_cast(__bf16, bv);
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int row = tm * 32 + 8 * (i / 4) + 4 * (lane >> 5) + (i % 4);
+    const int col = tn * 32 + (lane & 31);
+    if (row < M && col < N) {
+      float v = acc[i];
+      if (bias) v += bias[col];
+      if (relu) v = fmaxf(v, 0.0f);
+      float* out = C + size_t(row) * ldc + col;
+      *out = accumulate ? *out + v : v;
+    }
+  }
+}
+
+__global__ void ToBf16Kernel(const float* __restrict__ x, uint16_t* __restrict__ y, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = ToBf16(x[i]);
+}
+
+// dY = 2 (Y - T) / (n * n_out); also per-row squared error for the loss.
+__global__ void MseGradKernel(const float* __restrict__ y, const float* __restrict__ t,
+                              float* __restrict__ dy, float* __restrict__ sq, int64_t n_rows,
+                              int n_out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n_rows * n_out) return;
+  const float e = y[i] - t[i];
+  dy[i] = 2.0f * e / float(n_rows * n_out);
+  sq[i] = e * e;
+}
+
+// g *= (h > 0): ReLU backward through the stored post-activation h.
+__global__ void ReluBackwardKernel(float* __restrict__ g, const float* __restrict__ h,
+                                   int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n && h[i] <= 0.0f) g[i] = 0.0f;
+}
+
+// Column sums (bias gradients), fixed order per column.
+__global__ void ColumnSumKernel(const float* __restrict__ g, int64_t n_rows, int n_cols,
+                                float* __restrict__ out) {
+  const int c = blockIdx.x;
+  __shared__ float red[256];
+  float acc = 0.0f;
+  for (int64_t r = threadIdx.x; r < n_rows; r += blockDim.x) acc += g[r * n_cols + c];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = red[0];
+}
+
+__global__ void SumKernel(const float* __restrict__ x, int64_t n, float* __restrict__ out) {
+  __shared__ float red[256];
+  float acc = 0.0f;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) acc += x[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// Adam on FP32 master weights; refreshes the bf16 copy.
+__global__ void AdamKernel(float* __restrict__ w, const float* __restrict__ g,
+                           float* __restrict__ m, float* __restrict__ v,
+                           uint16_t* __restrict__ w_bf16, int64_t n, float lr, float beta1,
+                           float beta2, float eps, float bias1, float bias2) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = beta1 * m[i] + (1.0f - beta1) * gi;
+  const float vi = beta2 * v[i] + (1.0f - beta2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float upd = lr * (mi / bias1) / (sqrtf(vi / bias2) + eps);
+  w[i] -= upd;
+  if (w_bf16) w_bf16[i] = ToBf16(w[i]);
+}
+
+__global__ void GatherRowsKernel(const float* __restrict__ src, const int32_t* __restrict__ idx,
+                                 int64_t n_rows, int n_cols, float* __restrict__ dst) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n_rows * n_cols) return;
+  const int64_t r = i / n_cols;
+  const int c = int(i - r * n_cols);
+  dst[i] = src[int64_t(idx[r]) * n_cols + c];
+}
+
+// ---- warm-start features -------------------------------------------------
+// Per solution pair (cb, antenna, sub) and polarization p, with g0 the
+// previous interval's solution on the current interval:
+//   r1 = T(g0) / g0    the undamped one-step update (data/model correlation)
+//   ra = g_a / g0      after the first stretch of damped solver iterations
+//   rb = g_b / g0      after the second stretch
+//   s  = log10(den / mean den over the antennas of this (cb, sub))
+// Features (kWarmFeatures, zero padded): for p = 0, 1:
+//   [log|r1|, arg r1, log|ra|, arg ra, log|rb|, arg rb, s].
+constexpr int kWarmFeatures = 16;
+constexpr int kWarmOutputs = 4;
+
+__device__ inline void LogRatio(double2 a, double2 b, float& lmag, float& ang) {
+  // log(a / b) = log|a| - log|b| + i (arg a - arg b), wrapped to (-pi, pi]
+  const double na = hypot(a.x, a.y), nb = hypot(b.x, b.y);
+  if (!(na > 0.0 && nb > 0.0 && isfinite(na) && isfinite(nb))) {
+    lmag = 0.0f;
+    ang = 0.0f;
+    return;
+  }
+  lmag = float(log(na / nb));
+  // arg(a conj(b))
+  const double re = a.x * b.x + a.y * b.y, im = a.y * b.x - a.x * b.y;
+  ang = float(atan2(im, re));
+}
+
+__global__ void WarmFeaturesKernel(const double2* __restrict__ g0,
+                                   const double2* __restrict__ next1,
+                                   const double2* __restrict__ ga,
+                                   const double2* __restrict__ gb,
+                                   const double* __restrict__ den, int n_cb, int n_ant,
+                                   int n_sub, float* __restrict__ feat) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t n_pairs = int64_t(n_cb) * n_ant * n_sub;
+  if (i >= n_pairs) return;
+  const int s = int(i % n_sub);
+  const int cb = int(i / (int64_t(n_ant) * n_sub));
+  float f[kWarmFeatures] = {};
+  for (int p = 0; p < 2; ++p) {
+    float* fp = f + 7 * p;
+    const double2 base = g0[i * 2 + p];
+    LogRatio(next1[i * 2 + p], base, fp[0], fp[1]);
+    LogRatio(ga[i * 2 + p], base, fp[2], fp[3]);
+    LogRatio(gb[i * 2 + p], base, fp[4], fp[5]);
+    double mean = 0.0;
+    int cnt = 0;
+    for (int a = 0; a < n_ant; ++a) {
+      const double dv = den[((int64_t(cb) * n_ant + a) * n_sub + s) * 2 + p];
+      if (dv > 0.0 && isfinite(dv)) {
+        mean += dv;
+        ++cnt;
+      }
+    }
+    const double dv = den[i * 2 + p];
+    fp[6] = (cnt > 0 && dv > 0.0 && isfinite(dv)) ? float(log10(dv / (mean / cnt))) : -3.0f;
+  }
+  for (int k = 0; k < kWarmFeatures; ++k) feat[i * kWarmFeatures + k] = isfinite(f[k]) ? f[k] : 0.0f;
+}
+
+// init = prev * exp(d_log_mag + i d_arg) per polarization; non-finite prev
+// falls back to 1.
+__global__ void WarmApplyKernel(const double2* __restrict__ prev, const float* __restrict__ pred,
+                                int64_t n_pairs, double2* __restrict__ init) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n_pairs) return;
+  for (int p = 0; p < 2; ++p) {
+    double2 g = prev[i * 2 + p];
+    if (!(isfinite(g.x) && isfinite(g.y))) g = make_double2(1.0, 0.0);
+    const double lm = pred[i * kWarmOutputs + 2 * p];
+    const double an = pred[i * kWarmOutputs + 2 * p + 1];
+    double sn, cs;
+    sincos(an, &sn, &cs);
+    const double sc = exp(lm);
+    init[i * 2 + p] = make_double2(sc * (g.x * cs - g.y * sn), sc * (g.x * sn + g.y * cs));
+  }
+}
+
+template <typename T>
+struct Buf {
+  T* ptr = nullptr;
+  size_t n = 0;
+  ~Buf() {
+    if (ptr) (void)hipFree(ptr);
+  }
+  T* Get(size_t count) {
+    if (count > n) {
+      if (ptr) DDH_HIP_CHECK(hipFree(ptr));
+      ptr = nullptr;
+      DDH_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&ptr), std::max<size_t>(count, 1) * sizeof(T)));
+      n = count;
+    }
+    return ptr;
+  }
+};
+
+inline unsigned Blocks(int64_t n, int t = 256) { return unsigned((n + t - 1) / t); }
+
+void Gemm(const uint16_t* A, int lda, bool ta, const uint16_t* B, int ldb, bool tb, float* C,
+          int ldc, int M, int N, int K, const float* bias, bool relu, bool accumulate,
+          hipStream_t s) {
+  const int tiles = ((M + 31) / 32) * ((N + 31) / 32);
+  hipLaunchKernelGGL(GemmBf16Kernel, dim3((tiles + 3) / 4), dim3(256), 0, s, A, lda, int(ta), B,
+                     ldb, int(tb), C, ldc, M, N, K, bias, int(relu), int(accumulate));
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void ToBf16(const float* x, uint16_t* y, int64_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(ToBf16Kernel, dim3(Blocks(n)), dim3(256), 0, s, x, y, n);
+}
+
+}  // namespace
+}  // namespace ddh
+
+struct ddh_mlp {
+  int n_in = 0, n_hidden = 0, n_out = 0;
+  hipStream_t stream = nullptr;
+  // parameter block: W1 [n_in x h], b1 [h], W2 [h x h], b2 [h], W3 [h x n_out], b3 [n_out]
+  int64_t off[7] = {};
+  ddh::Buf<float> w, g, m, v;
+  ddh::Buf<uint16_t> w16;
+  int64_t step = 0;
+  // activations
+  ddh::Buf<float> x, t, h1, h2, y, dy, dh1, dh2, sq, loss;
+  ddh::Buf<uint16_t> x16, h1_16, h2_16, dy16, dh1_16, dh2_16;
+  ddh::Buf<float> data_x, data_t;
+  ddh::Buf<int32_t> perm;
+
+  int64_t n_params() const { return off[6]; }
+};
+
+namespace ddh {
+namespace {
+
+void Forward(ddh_mlp& net, int64_t n, bool keep) {
+  (void)keep;
+  hipStream_t s = net.stream;
+  const int h = net.n_hidden;
+  ToBf16(net.x.ptr, net.x16.ptr, n * net.n_in, s);
+  // H1 = relu(X W1 + b1)
+  Gemm(net.x16.ptr, net.n_in, false, net.w16.ptr + net.off[0], h, false, net.h1.ptr, h, int(n), h,
+       net.n_in, net.w.ptr + net.off[1], true, false, s);
+  ToBf16(net.h1.ptr, net.h1_16.ptr, n * h, s);
+  Gemm(net.h1_16.ptr, h, false, net.w16.ptr + net.off[2], h, false, net.h2.ptr, h, int(n), h, h,
+       net.w.ptr + net.off[3], true, false, s);
+  ToBf16(net.h2.ptr, net.h2_16.ptr, n * h, s);
+  Gemm(net.h2_16.ptr, h, false, net.w16.ptr + net.off[4], net.n_out, false, net.y.ptr, net.n_out,
+       int(n), net.n_out, h, net.w.ptr + net.off[5], false, false, s);
+}
+
+// Gradients of the mean squared error over n rows into net.g; loss into net.loss[0].
+void Backward(ddh_mlp& net, int64_t n) {
+  hipStream_t s = net.stream;
+  const int h = net.n_hidden, no = net.n_out, ni = net.n_in;
+  hipLaunchKernelGGL(MseGradKernel, dim3(Blocks(n * no)), dim3(256), 0, s, net.y.ptr, net.t.ptr,
+                     net.dy.ptr, net.sq.ptr, n, no);
+  hipLaunchKernelGGL(SumKernel, dim3(1), dim3(256), 0, s, net.sq.ptr, n * no, net.loss.ptr);
+  ToBf16(net.dy.ptr, net.dy16.ptr, n * no, s);
+  // dW3 = H2^T dY ; db3 = colsum(dY)
+  Gemm(net.h2_16.ptr, h, true, net.dy16.ptr, no, false, net.g.ptr + net.off[4], no, h, no, int(n),
+       nullptr, false, false, s);
+  hipLaunchKernelGGL(ColumnSumKernel, dim3(no), dim3(256), 0, s, net.dy.ptr, n, no,
+                     net.g.ptr + net.off[5]);
+  // dH2 = dY W3^T * relu'(H2)
+  Gemm(net.dy16.ptr, no, false, net.w16.ptr + net.off[4], no, true, net.dh2.ptr, h, int(n), h, no,
+       nullptr, false, false, s);
+  hipLaunchKernelGGL(ReluBackwardKernel, dim3(Blocks(n * h)), dim3(256), 0, s, net.dh2.ptr,
+                     net.h2.ptr, n * h);
+  ToBf16(net.dh2.ptr, net.dh2_16.ptr, n * h, s);
+  Gemm(net.h1_16.ptr, h, true, net.dh2_16.ptr, h, false, net.g.ptr + net.off[2], h, h, h, int(n),
+       nullptr, false, false, s);
+  hipLaunchKernelGGL(ColumnSumKernel, dim3(h), dim3(256), 0, s, net.dh2.ptr, n, h,
+                     net.g.ptr + net.off[3]);
+  // dH1 = dH2 W2^T * relu'(H1)
+  Gemm(net.dh2_16.ptr, h, false, net.w16.ptr + net.off[2], h, true, net.dh1.ptr, h, int(n), h, h,
+       nullptr, false, false, s);
+  hipLaunchKernelGGL(ReluBackwardKernel, dim3(Blocks(n * h)), dim3(256), 0, s, net.dh1.ptr,
+                     net.h1.ptr, n * h);
+  ToBf16(net.dh1.ptr, net.dh1_16.ptr, n * h, s);
+  Gemm(net.x16.ptr, ni, true, net.dh1_16.ptr, h, false, net.g.ptr + net.off[0], h, ni, h, int(n),
+       nullptr, false, false, s);
+  hipLaunchKernelGGL(ColumnSumKernel, dim3(h), dim3(256), 0, s, net.dh1.ptr, n, h,
+                     net.g.ptr + net.off[1]);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void Reserve(ddh_mlp& net, int64_t n) {
+  const int h = net.n_hidden;
+  net.x.Get(n * net.n_in);
+  net.x16.Get(n * net.n_in);
+  net.t.Get(n * net.n_out);
+  net.h1.Get(n * h);
+  net.h2.Get(n * h);
+  net.h1_16.Get(n * h);
+  net.h2_16.Get(n * h);
+  net.y.Get(n * net.n_out);
+  net.dy.Get(n * net.n_out);
+  net.dy16.Get(n * net.n_out);
+  net.sq.Get(n * net.n_out);
+  net.dh1.Get(n * h);
+  net.dh2.Get(n * h);
+  net.dh1_16.Get(n * h);
+  net.dh2_16.Get(n * h);
+  net.loss.Get(1);
+}
+
+}  // namespace
+}  // namespace ddh
+
+using namespace ddh;
+
+extern "C" {
+
+ddh_mlp* ddh_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_out, uint32_t seed) {
+  ddh_mlp* out = nullptr;
+  const int rc = Guard([&] {
+    DDH_REQUIRE(n_in >= 1 && n_hidden >= 1 && n_out >= 1, -1, "bad layer sizes");
+    auto net = std::make_unique<ddh_mlp>();
+    net->n_in = n_in;
+    net->n_hidden = n_hidden;
+    net->n_out = n_out;
+    const int64_t h = n_hidden;
+    const int64_t sizes[6] = {n_in * h, h, h * h, h, h * n_out, n_out};
+    net->off[0] = 0;
+    for (int i = 0; i < 6; ++i) net->off[i + 1] = net->off[i] + sizes[i];
+    const int64_t np = net->n_params();
+    // He-normal weights, zero biases (host RNG, seeded)
+    std::vector<float> w(np, 0.0f);
+    std::mt19937 rng(seed);
+    const int fan_in[3] = {n_in, n_hidden, n_hidden};
+    for (int layer = 0; layer < 3; ++layer) {
+      std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / fan_in[layer]));
+      for (int64_t i = net->off[2 * layer]; i < net->off[2 * layer + 1]; ++i) w[i] = nd(rng);
+    }
+    // the output layer starts small: the initial correction is near zero
+    for (int64_t i = net->off[4]; i < net->off[5]; ++i) w[i] *= 0.01f;
+    DDH_HIP_CHECK(hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking));
+    net->w.Get(np);
+    net->g.Get(np);
+    net->m.Get(np);
+    net->v.Get(np);
+    net->w16.Get(np);
+    DDH_HIP_CHECK(hipMemcpy(net->w.ptr, w.data(), np * sizeof(float), hipMemcpyHostToDevice));
+    DDH_HIP_CHECK(hipMemset(net->m.ptr, 0, np * sizeof(float)));
+    DDH_HIP_CHECK(hipMemset(net->v.ptr, 0, np * sizeof(float)));
+    ToBf16(net->w.ptr, net->w16.ptr, np, net->stream);
+    DDH_HIP_CHECK(hipStreamSynchronize(net->stream));
+    out = net.release();
+  });
+  return rc == 0 ? out : nullptr;
+}
+
+void ddh_mlp_destroy(ddh_mlp* net) {
+  if (!net) return;
+  (void)hipStreamSynchronize(net->stream);
+  (void)hipStreamDestroy(net->stream);
+  delete net;
+}
+
+int64_t ddh_mlp_num_params(const ddh_mlp* net) { return net ? net->n_params() : 0; }
+
+int ddh_mlp_get_params(const ddh_mlp* net, float* params) {
+  return Guard([&] {
+    DDH_REQUIRE(net && params, -1, "NULL argument");
+    DDH_HIP_CHECK(hipMemcpy(params, net->w.ptr, net->n_params() * sizeof(float),
+                            hipMemcpyDeviceToHost));
+  });
+}
+
+int ddh_mlp_set_params(ddh_mlp* net, const float* params) {
+  return Guard([&] {
+    DDH_REQUIRE(net && params, -1, "NULL argument");
+    DDH_HIP_CHECK(hipMemcpy(net->w.ptr, params, net->n_params() * sizeof(float),
+                            hipMemcpyHostToDevice));
+    ToBf16(net->w.ptr, net->w16.ptr, net->n_params(), net->stream);
+    DDH_HIP_CHECK(hipStreamSynchronize(net->stream));
+  });
+}
+
+int ddh_mlp_forward(ddh_mlp* net, const float* x, int64_t n, float* y) {
+  return Guard([&] {
+    DDH_REQUIRE(net && x && y && n >= 1, -1, "bad argument");
+    Reserve(*net, n);
+    DDH_HIP_CHECK(hipMemcpyAsync(net->x.ptr, x, n * net->n_in * sizeof(float),
+                                 hipMemcpyHostToDevice, net->stream));
+    Forward(*net, n, false);
+    DDH_HIP_CHECK(hipMemcpyAsync(y, net->y.ptr, n * net->n_out * sizeof(float),
+                                 hipMemcpyDeviceToHost, net->stream));
+    DDH_HIP_CHECK(hipStreamSynchronize(net->stream));
+  });
+}
+
+int ddh_mlp_loss_and_grads(ddh_mlp* net, const float* x, const float* t, int64_t n,
+                           float* loss, float* grads) {
+  return Guard([&] {
+    DDH_REQUIRE(net && x && t && n >= 1, -1, "bad argument");
+    Reserve(*net, n);
+    DDH_HIP_CHECK(hipMemcpyAsync(net->x.ptr, x, n * net->n_in * sizeof(float),
+                                 hipMemcpyHostToDevice, net->stream));
+    DDH_HIP_CHECK(hipMemcpyAsync(net->t.ptr, t, n * net->n_out * sizeof(float),
+                                 hipMemcpyHostToDevice, net->stream));
+    Forward(*net, n, true);
+    Backward(*net, n);
+    float sq = 0.0f;
+    DDH_HIP_CHECK(hipMemcpyAsync(&sq, net->loss.ptr, sizeof(float), hipMemcpyDeviceToHost,
+                                 net->stream));
+    if (grads)
+      DDH_HIP_CHECK(hipMemcpyAsync(grads, net->g.ptr, net->n_params() * sizeof(float),
+                                   hipMemcpyDeviceToHost, net->stream));
+    DDH_HIP_CHECK(hipStreamSynchronize(net->stream));
+    if (loss) *loss = sq / float(n * net->n_out);
+  });
+}
+
+// Minibatch Adam over (x [n x n_in], t [n x n_out]) for `epochs` passes with a
+// seeded shuffle. Returns the mean loss of the last epoch.
+int ddh_mlp_train(ddh_mlp* net, const float* x, const float* t, int64_t n, int32_t epochs,
+                  int32_t batch, float lr, uint32_t seed, float* last_epoch_loss) {
+  return Guard([&] {
+    DDH_REQUIRE(net && x && t && n >= 1 && epochs >= 1 && batch >= 1, -1, "bad argument");
+    hipStream_t s = net->stream;
+    float* dx = net->data_x.Get(n * net->n_in);
+    float* dt = net->data_t.Get(n * net->n_out);
+    DDH_HIP_CHECK(hipMemcpyAsync(dx, x, n * net->n_in * sizeof(float), hipMemcpyHostToDevice, s));
+    DDH_HIP_CHECK(hipMemcpyAsync(dt, t, n * net->n_out * sizeof(float), hipMemcpyHostToDevice, s));
+    Reserve(*net, batch);
+    int32_t* dperm = net->perm.Get(n);
+    std::vector<int32_t> perm(n);
+    std::mt19937 rng(seed);
+    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    double epoch_loss = 0.0;
+    for (int e = 0; e < epochs; ++e) {
+      for (int64_t i = 0; i < n; ++i) perm[i] = int32_t(i);
+      std::shuffle(perm.begin(), perm.end(), rng);
+      DDH_HIP_CHECK(hipMemcpyAsync(dperm, perm.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+      DDH_HIP_CHECK(hipStreamSynchronize(s));  // perm is reused next epoch
+      epoch_loss = 0.0;
+      int64_t batches = 0;
+      for (int64_t b0 = 0; b0 + batch <= n; b0 += batch) {
+        hipLaunchKernelGGL(GatherRowsKernel, dim3(Blocks(int64_t(batch) * net->n_in)), dim3(256), 0,
+                           s, dx, dperm + b0, int64_t(batch), net->n_in, net->x.ptr);
+        hipLaunchKernelGGL(GatherRowsKernel, dim3(Blocks(int64_t(batch) * net->n_out)), dim3(256),
+                           0, s, dt, dperm + b0, int64_t(batch), net->n_out, net->t.ptr);
+        Forward(*net, batch, true);
+        Backward(*net, batch);
+        net->step += 1;
+        const float bias1 = 1.0f - std::pow(beta1, float(net->step));
+        const float bias2 = 1.0f - std::pow(beta2, float(net->step));
+        hipLaunchKernelGGL(AdamKernel, dim3(Blocks(net->n_params())), dim3(256), 0, s, net->w.ptr,
+                           net->g.ptr, net->m.ptr, net->v.ptr, net->w16.ptr, net->n_params(), lr,
+                           beta1, beta2, eps, bias1, bias2);
+        float sq = 0.0f;
+        if (e == epochs - 1) {
+          DDH_HIP_CHECK(hipMemcpyAsync(&sq, net->loss.ptr, sizeof(float), hipMemcpyDeviceToHost, s));
+          DDH_HIP_CHECK(hipStreamSynchronize(s));
+          epoch_loss += sq / float(int64_t(batch) * net->n_out);
+        }
+        ++batches;
+      }
+      if (batches > 0) epoch_loss /= batches;
+    }
+    DDH_HIP_CHECK(hipStreamSynchronize(s));
+    if (last_epoch_loss) *last_epoch_loss = float(epoch_loss);
+  });
+}
+
+// C = A B with bf16 operands (given as FP32, rounded to bf16 on the device)
+// and FP32 accumulation; A [M x K], B [K x N] row-major. Exposes the MFMA
+// GEMM for testing.
+int ddh_gemm_bf16(const float* a, const float* b, float* c, int32_t M, int32_t N, int32_t K,
+                  int32_t trans_a, int32_t trans_b) {
+  return Guard([&] {
+    DDH_REQUIRE(a && b && c && M >= 1 && N >= 1 && K >= 1, -1, "bad argument");
+    Buf<float> fa, fb, fc;
+    Buf<uint16_t> ha, hb;
+    const size_t na = size_t(M) * K, nb = size_t(K) * N;
+    DDH_HIP_CHECK(hipMemcpy(fa.Get(na), a, na * sizeof(float), hipMemcpyHostToDevice));
+    DDH_HIP_CHECK(hipMemcpy(fb.Get(nb), b, nb * sizeof(float), hipMemcpyHostToDevice));
+    ToBf16(fa.ptr, ha.Get(na), na, nullptr);
+    ToBf16(fb.ptr, hb.Get(nb), nb, nullptr);
+    const int lda = trans_a ? M : K;
+    const int ldb = trans_b ? K : N;
+    Gemm(ha.ptr, lda, trans_a, hb.ptr, ldb, trans_b, fc.Get(size_t(M) * N), N, M, N, K, nullptr,
+         false, false, nullptr);
+    DDH_HIP_CHECK(hipMemcpy(c, fc.ptr, size_t(M) * N * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+int ddh_warmstart_features(const double* g0, const double* next1, const double* ga,
+                           const double* gb, const double* den, int32_t n_cb, int32_t n_ant,
+                           int32_t n_sub, float* features) {
+  return Guard([&] {
+    DDH_REQUIRE(g0 && next1 && ga && gb && den && features, -1, "NULL argument");
+    const int64_t n_pairs = int64_t(n_cb) * n_ant * n_sub;
+    Buf<double2> d0, d1, da, db;
+    Buf<double> dd;
+    Buf<float> df;
+    const size_t bytes = n_pairs * 2 * sizeof(double2);
+    DDH_HIP_CHECK(hipMemcpy(d0.Get(n_pairs * 2), g0, bytes, hipMemcpyHostToDevice));
+    DDH_HIP_CHECK(hipMemcpy(d1.Get(n_pairs * 2), next1, bytes, hipMemcpyHostToDevice));
+    DDH_HIP_CHECK(hipMemcpy(da.Get(n_pairs * 2), ga, bytes, hipMemcpyHostToDevice));
+    DDH_HIP_CHECK(hipMemcpy(db.Get(n_pairs * 2), gb, bytes, hipMemcpyHostToDevice));
+    DDH_HIP_CHECK(hipMemcpy(dd.Get(n_pairs * 2), den, n_pairs * 2 * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(WarmFeaturesKernel, dim3(Blocks(n_pairs)), dim3(256), 0, nullptr, d0.ptr,
+                       d1.ptr, da.ptr, db.ptr, dd.ptr, n_cb, n_ant, n_sub,
+                       df.Get(n_pairs * kWarmFeatures));
+    DDH_HIP_CHECK(hipGetLastError());
+    DDH_HIP_CHECK(hipMemcpy(features, df.ptr, n_pairs * kWarmFeatures * sizeof(float),
+                            hipMemcpyDeviceToHost));
+  });
+}
+
+int ddh_warmstart_apply(const double* prev, const float* pred, int64_t n_pairs, double* init) {
+  return Guard([&] {
+    DDH_REQUIRE(prev && pred && init, -1, "NULL argument");
+    Buf<double2> dp, di;
+    Buf<float> dpred;
+    DDH_HIP_CHECK(hipMemcpy(dp.Get(n_pairs * 2), prev, n_pairs * 2 * sizeof(double2), hipMemcpyHostToDevice));
+    DDH_HIP_CHECK(hipMemcpy(dpred.Get(n_pairs * kWarmOutputs), pred, n_pairs * kWarmOutputs * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(WarmApplyKernel, dim3(Blocks(n_pairs)), dim3(256), 0, nullptr, dp.ptr,
+                       dpred.ptr, n_pairs, di.Get(n_pairs * 2));
+    DDH_HIP_CHECK(hipGetLastError());
+    DDH_HIP_CHECK(hipMemcpy(init, di.ptr, n_pairs * 2 * sizeof(double2), hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"

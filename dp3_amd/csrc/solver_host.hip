@@ -100,6 +100,7 @@ struct ddh_solver {
   ddh::DeviceBuffer<float> partials;
   ddh::DeviceBuffer<ddh::DeviceState> state;
   ddh::DeviceBuffer<int32_t> n_sol_dir, sol_start;
+  ddh::DeviceBuffer<double> den, chi2;  // one-step denominators; chi^2 partials + total
   // captured iteration body (poll iterations), valid for graph_key
   hipGraphExec_t graph_exec = nullptr;
   std::vector<const void*> graph_key;
@@ -377,6 +378,7 @@ IterArgs MakeArgs(ddh_solver& s, ddh_problem& p) {
   a.step_diff_sigma = c.step_diff_sigma;
   a.detect_stalling = c.detect_stalling;
   a.phase_only = c.phase_only;
+  a.den_out = nullptr;
   int max_nsd = 1;
   for (int v : s.cfg.n_sol_dir) max_nsd = std::max(max_nsd, v);
   a.split = 1;
@@ -635,6 +637,8 @@ void ddh_solver_destroy(ddh_solver* s) {
   s->state.Free();
   s->n_sol_dir.Free();
   s->sol_start.Free();
+  s->den.Free();
+  s->chi2.Free();
   (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -791,6 +795,10 @@ struct ddh_batch {
   hipGraphExec_t graph_exec = nullptr;
   int graph_poll = 0, graph_waves = 0, graph_unroll = 0;
   double last_ms = 0.0;
+  int profile = 0;
+  std::vector<hipEvent_t> events;
+  double last_correlate_ms = 0.0;
+  int64_t last_correlate_launches = 0;
 };
 
 namespace ddh {
@@ -883,8 +891,13 @@ void SolveBatch(ddh_batch& b, double* const* const* solutions, ddh_result* resul
       DDH_HIP_CHECK(hipMemcpyAsync(b.args[i].sol0 + cb * per_cb, solutions[i][cb],
                                    per_cb * sizeof(double2), hipMemcpyHostToDevice, b.stream));
   }
-  const bool graph = b.use_graph != 0;
+  const bool graph = b.use_graph != 0 && !b.profile;
   if (graph) EnsureBatchGraph(b);
+  if (b.profile && int(b.events.size()) < 2 * b.limit) {
+    for (hipEvent_t e : b.events) (void)hipEventDestroy(e);
+    b.events.assign(2 * b.limit, nullptr);
+    for (hipEvent_t& e : b.events) DDH_HIP_CHECK(hipEventCreate(&e));
+  }
   hipEvent_t t0, t1;
   DDH_HIP_CHECK(hipEventCreate(&t0));
   DDH_HIP_CHECK(hipEventCreate(&t1));
@@ -898,7 +911,20 @@ void SolveBatch(ddh_batch& b, double* const* const* solutions, ddh_result* resul
       launched += b.poll;
     } else {
       const int k = std::min(b.poll, b.limit - launched);
-      for (int i = 0; i < k; ++i) LaunchBatchIteration(b);
+      for (int i = 0; i < k; ++i) {
+        const int it = launched + i;
+        if (b.profile) {
+          // correlate alone between the events, then the rest of the iteration
+          DDH_HIP_CHECK(hipEventRecord(b.events[2 * it], b.stream));
+          LaunchBatch(b.d_args.ptr, b.d_tiles.ptr, int(b.tiles.size()), b.d_items.ptr, 0, 0,
+                      b.correlate_lds, b.solve_lds, b.n_corr, b.f16, b.waves, b.unroll, b.stream);
+          DDH_HIP_CHECK(hipEventRecord(b.events[2 * it + 1], b.stream));
+          LaunchBatchTail(b.d_args.ptr, b.d_items.ptr, int(b.items.size()), int(b.solvers.size()),
+                          b.solve_lds, b.n_corr, b.f16, b.stream);
+        } else {
+          LaunchBatchIteration(b);
+        }
+      }
       launched += k;
     }
     DDH_HIP_CHECK(hipMemcpyAsync(b.host_states, b.d_states.ptr, n * sizeof(DeviceState),
@@ -930,6 +956,19 @@ void SolveBatch(ddh_batch& b, double* const* const* solutions, ddh_result* resul
   float ms = 0.0f;
   DDH_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
   b.last_ms = ms;
+  b.last_correlate_ms = 0.0;
+  b.last_correlate_launches = 0;
+  if (b.profile) {
+    // launches after the last interval stopped do no work: count those that did
+    int worked = 0;
+    for (int i = 0; i < n; ++i) worked = std::max(worked, b.host_states[i].iteration);
+    for (int it = 0; it < worked; ++it) {
+      float e = 0.0f;
+      DDH_HIP_CHECK(hipEventElapsedTime(&e, b.events[2 * it], b.events[2 * it + 1]));
+      b.last_correlate_ms += e;
+    }
+    b.last_correlate_launches = worked;
+  }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
 }
@@ -996,6 +1035,7 @@ void ddh_batch_destroy(ddh_batch* b) {
   if (!b) return;
   (void)hipStreamSynchronize(b->stream);
   DestroyBatchGraph(*b);
+  for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
   if (b->host_states) (void)hipHostFree(b->host_states);
   b->d_args.Free();
   b->d_tiles.Free();
@@ -1023,6 +1063,40 @@ int ddh_batch_set_schedule(ddh_batch* b, int32_t waves_per_tile, int32_t rows_pe
   });
 }
 
+int ddh_batch_rebind(ddh_batch* b, ddh_problem* const* problems) {
+  return Guard([&] {
+    DDH_REQUIRE(b && problems, -1, "NULL argument");
+    const int n = int(b->solvers.size());
+    for (int i = 0; i < n; ++i) {
+      ddh_problem* p = problems[i];
+      const ddh_problem* o = b->problems[i];
+      DDH_REQUIRE(p, -1, "NULL problem");
+      DDH_REQUIRE(p->n_t == o->n_t && p->n_bl == o->n_bl && p->n_cb == o->n_cb &&
+                      p->n_dir == o->n_dir && p->n_corr == o->n_corr && p->f16 == o->f16 &&
+                      p->bl_stride == o->bl_stride && p->ch_per_cb == o->ch_per_cb &&
+                      p->seg_t0 == o->seg_t0 && p->seg_sol == o->seg_sol,
+                  -2, "rebinding needs problems of the same shapes");
+    }
+    std::vector<IterArgs> args(n);
+    for (int i = 0; i < n; ++i) {
+      ddh_solver& s = *b->solvers[i];
+      CheckMatch(s, *problems[i]);
+      PrepareBuffers(s, *problems[i]);
+      args[i] = MakeArgs(s, *problems[i]);
+      args[i].state = b->args[i].state;
+      DDH_REQUIRE(AntennaSolveSharedBytes(args[i], b->n_corr) <= b->solve_lds, -2,
+                  "rebinding needs problems of the same shapes");
+    }
+    for (int i = 0; i < n; ++i) b->problems[i] = problems[i];
+    b->args = args;
+    // Same shapes: same tiles, same buffer sizes, so the captured graph (which
+    // reads the interval arguments from d_args) stays valid.
+    DDH_HIP_CHECK(hipMemcpyAsync(b->d_args.ptr, b->args.data(), n * sizeof(IterArgs),
+                                 hipMemcpyHostToDevice, b->stream));
+    DDH_HIP_CHECK(hipStreamSynchronize(b->stream));
+  });
+}
+
 int ddh_batch_solve(ddh_batch* b, double* const* const* solutions, ddh_result* results) {
   return Guard([&] {
     DDH_REQUIRE(b && solutions, -1, "NULL argument");
@@ -1033,6 +1107,19 @@ int ddh_batch_solve(ddh_batch* b, double* const* const* solutions, ddh_result* r
 int ddh_batch_last_time(const ddh_batch* b, double* ms) {
   if (!b || !ms) return -1;
   *ms = b->last_ms;
+  return 0;
+}
+
+int ddh_batch_set_profiling(ddh_batch* b, int32_t enable) {
+  if (!b) return -1;
+  b->profile = enable;
+  return 0;
+}
+
+int ddh_batch_last_timing(const ddh_batch* b, double* correlate_ms, int64_t* launches) {
+  if (!b) return -1;
+  if (correlate_ms) *correlate_ms = b->last_correlate_ms;
+  if (launches) *launches = b->last_correlate_launches;
   return 0;
 }
 
@@ -1106,6 +1193,61 @@ int ddh_autotune(ddh_solver* s, ddh_problem* p, int32_t* waves_out, int32_t* row
     if (waves_out) *waves_out = bw;
     if (rows_out) *rows_out = bu;
     if (best_ms) *best_ms = best;
+  });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int ddh_one_step(ddh_solver* s, ddh_problem* p, double* const* solutions_in,
+                 double* const* next_out, double* const* den_out) {
+  return Guard([&] {
+    DDH_REQUIRE(s && p && solutions_in && next_out, -1, "NULL argument");
+    CheckMatch(*s, *p);
+    PrepareBuffers(*s, *p);
+    const ddh_config& c = s->cfg.c;
+    const size_t per_cb = size_t(c.n_antennas) * s->cfg.n_sub * 2;
+    s->den.Resize(per_cb * c.n_channel_blocks);
+    for (int cb = 0; cb < c.n_channel_blocks; ++cb)
+      DDH_HIP_CHECK(hipMemcpyAsync(s->sol0.ptr + cb * per_cb, solutions_in[cb],
+                                   per_cb * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+    IterArgs a = MakeArgs(*s, *p);
+    a.step_size = 1.0;  // stepped == next: the undamped update
+    a.phase_only = 0;
+    a.den_out = s->den.ptr;
+    LaunchPrologue(a, s->stream);
+    LaunchCorrelate(a, p->n_corr, p->f16, s->waves, s->unroll, s->stream);
+    LaunchAntennaSolve(a, p->n_corr, p->f16, s->stream);
+    for (int cb = 0; cb < c.n_channel_blocks; ++cb) {
+      DDH_HIP_CHECK(hipMemcpyAsync(next_out[cb], s->sol1.ptr + cb * per_cb,
+                                   per_cb * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+      if (den_out)
+        DDH_HIP_CHECK(hipMemcpyAsync(den_out[cb], s->den.ptr + cb * per_cb, per_cb * sizeof(double),
+                                     hipMemcpyDeviceToHost, s->stream));
+    }
+    DDH_HIP_CHECK(hipStreamSynchronize(s->stream));
+  });
+}
+
+int ddh_chi2(ddh_solver* s, ddh_problem* p, double* const* solutions, double* chi2) {
+  return Guard([&] {
+    DDH_REQUIRE(s && p && solutions && chi2, -1, "NULL argument");
+    CheckMatch(*s, *p);
+    PrepareBuffers(*s, *p);
+    const ddh_config& c = s->cfg.c;
+    const size_t per_cb = size_t(c.n_antennas) * s->cfg.n_sub * 2;
+    for (int cb = 0; cb < c.n_channel_blocks; ++cb)
+      DDH_HIP_CHECK(hipMemcpyAsync(s->sol1.ptr + cb * per_cb, solutions[cb],
+                                   per_cb * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+    const IterArgs a = MakeArgs(*s, *p);
+    const int n_partial = Chi2Partials(a);
+    s->chi2.Resize(size_t(n_partial) + 1);
+    LaunchChi2(a, p->n_corr, p->f16, s->sol1.ptr, s->chi2.ptr, n_partial,
+               s->chi2.ptr + n_partial, s->stream);
+    DDH_HIP_CHECK(hipMemcpyAsync(chi2, s->chi2.ptr + n_partial, sizeof(double),
+                                 hipMemcpyDeviceToHost, s->stream));
+    DDH_HIP_CHECK(hipStreamSynchronize(s->stream));
   });
 }
 

@@ -91,6 +91,7 @@ struct IterArgs {
   int32_t max_iterations, min_iterations;
   double accuracy, step_size, step_diff_sigma;
   int32_t detect_stalling, phase_only;
+  double* den_out;           // optional [n_cb][n_ant][n_sub][2] denominators
   int32_t split;             // threads per (antenna, solution) in antenna_solve
   int32_t max_nsd;           // largest n_sol_dir entry
 };
@@ -111,6 +112,12 @@ void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, int unr
 void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, hipStream_t stream);
 void LaunchDecide(const IterArgs& a, hipStream_t stream);
 void LaunchPrologue(const IterArgs& a, hipStream_t stream);
+// FP64 chi^2 of the residual with solutions `sol` (current buffer of `a`):
+// per-workgroup partial sums into `partial`, then their fixed-order sum into
+// `out`.
+void LaunchChi2(const IterArgs& a, int n_corr, bool f16, const double2* sol, double* partial,
+                int n_partial, double* out, hipStream_t stream);
+int Chi2Partials(const IterArgs& a);
 size_t CorrelateSharedBytes(int n_dir, int n_corr, int waves);
 size_t MaxSharedBytes();
 size_t AntennaSolveSharedBytes(const IterArgs& a, int n_corr);
@@ -123,6 +130,9 @@ void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, int n_tiles,
                  hipStream_t stream);
 void LaunchBatchPrologue(const IterArgs* d_args, const BatchTile* d_items, int n_items,
                          hipStream_t stream);
+void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_items,
+                     int n_intervals, size_t solve_lds, int n_corr, bool f16,
+                     hipStream_t stream);
 
 // Layout conversion: reference order [plane][t][bl][ch][n_corr] complex64 ->
 // device order [plane][t][ch][bl_stride] (complex64 or FP16/scale).

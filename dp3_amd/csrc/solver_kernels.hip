@@ -485,6 +485,7 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
         stepped[p] = make_double2(cs, sn);
       }
       nxt[pair * 2 + p] = stepped[p];
+      if (a.den_out) a.den_out[pair * 2 + p] = dn[p];
     }
     // Convergence term of this solution pair (SolverBase.cc:254-268).
     const double n0 = cnorm(old[0]), n1 = cnorm(old[1]);
@@ -611,6 +612,72 @@ __device__ __forceinline__ void PrologueBlock(const IterArgs& a, const int cb) {
 
 __global__ void __launch_bounds__(kBlockThreads) PrologueKernel(const IterArgs a) {
   PrologueBlock(a, blockIdx.x);
+}
+
+// chi^2 = sum over blocks, visibilities and correlations of
+// |D - sum_d Diag(g_a) M_d Diag(g_b)^H|^2 in FP64 (gains unrepaired, as
+// returned by the solver). Grid (bl tiles, blocks); one partial per block.
+template <int NC, bool F16>
+__global__ void __launch_bounds__(256) Chi2Kernel(const IterArgs a, const double2* sol,
+                                                  double* partial) {
+  __shared__ double red[256 / 64];
+  const int tile = blockIdx.x;
+  const int cb = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x / 64;
+  const int n_waves = blockDim.x / 64;
+  const int bl = tile * 64 + lane;
+  const CbInfo info = a.cbinfo[cb];
+  const size_t plane = size_t(a.n_t) * info.n_ch * a.bl_stride;
+  double acc = 0.0;
+  if (bl < a.n_bl) {
+    const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
+    const int a1 = a.ant1[bl], a2 = a.ant2[bl];
+    for (int seg = 0; seg < a.n_seg; ++seg) {
+      const int r0 = a.seg_t0[seg] * info.n_ch, r1 = a.seg_t0[seg + 1] * info.n_ch;
+      for (int row = r0 + wave; row < r1; row += n_waves) {
+        const size_t e = size_t(row) * a.bl_stride + bl;
+        float2 df[NC];
+        LoadVis<NC, false>(a.data, info.data_off + e, df);
+        double2 r[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) r[k] = make_double2(df[k].x, df[k].y);
+        for (int d = 0; d < a.n_dir; ++d) {
+          const int s = a.seg_sol[seg * a.n_dir + d];
+          float2 mf[NC];
+          LoadVis<NC, F16>(a.model, info.model_off + d * plane + e, mf);
+          const double sc = F16 ? double(a.model_scale[cb * a.n_dir + d]) : 1.0;
+#pragma unroll
+          for (int k = 0; k < NC; ++k) {
+            const double2 ga = sol[(gbase + size_t(a1) * a.n_sub + s) * 2 + PolI<NC>(k)];
+            const double2 gb = sol[(gbase + size_t(a2) * a.n_sub + s) * 2 + PolJ<NC>(k)];
+            const double2 m = make_double2(sc * mf[k].x, sc * mf[k].y);
+            const double2 c = cmulc(cmul(ga, m), gb);
+            r[k].x -= c.x;
+            r[k].y -= c.y;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < NC; ++k) acc += cnorm(r[k]);
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (lane == 0) red[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < n_waves; ++w) t += red[w];
+    partial[size_t(cb) * gridDim.x + tile] = t;
+  }
+}
+
+__global__ void SumPartialsKernel(const double* partial, int n, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < n; ++i) t += partial[i];
+    *out = t;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -822,6 +889,10 @@ void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, int n_tiles,
   DDH_REQUIRE(solve_lds <= size_t(g_max_lds_solve), -4, "batch antenna solve LDS exceeds the device");
   DDH_REQUIRE(n_corr == 2 || (!f16 && unroll == 1), -4,
               "batched Full (4-correlation) data runs complex64 with one row per trip");
+  if (n_tiles == 0) {
+    LaunchBatchTail(d_args, d_items, n_items, n_intervals, solve_lds, n_corr, f16, stream);
+    return;
+  }
   const dim3 cgrid(n_tiles), cblock(waves * kWave);
   if (n_corr == 2) {
     if (f16) {
@@ -845,6 +916,25 @@ void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, int n_tiles,
       hipLaunchKernelGGL((BatchAntennaSolveKernel<4, false>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
   }
   DDH_HIP_CHECK(hipGetLastError());
+  if (n_intervals > 0)
+    hipLaunchKernelGGL(BatchDecideKernel, dim3(n_intervals), dim3(256), 0, stream, d_args);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_items,
+                     int n_intervals, size_t solve_lds, int n_corr, bool f16,
+                     hipStream_t stream) {
+  // antenna solve + decide only (n_tiles = 0 skips the correlate)
+  if (n_items > 0) {
+    if (n_corr == 2) {
+      if (f16)
+        hipLaunchKernelGGL((BatchAntennaSolveKernel<2, true>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
+      else
+        hipLaunchKernelGGL((BatchAntennaSolveKernel<2, false>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
+    } else {
+      hipLaunchKernelGGL((BatchAntennaSolveKernel<4, false>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
+    }
+  }
   if (n_intervals > 0)
     hipLaunchKernelGGL(BatchDecideKernel, dim3(n_intervals), dim3(256), 0, stream, d_args);
   DDH_HIP_CHECK(hipGetLastError());
@@ -891,6 +981,28 @@ void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, hipStream_t str
     else
       hipLaunchKernelGGL((AntennaSolveKernel<4, false>), grid, dim3(kBlockThreads), lds, stream, a);
   }
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+int Chi2Partials(const IterArgs& a) { return (a.bl_stride / kWave) * a.n_cb; }
+
+void LaunchChi2(const IterArgs& a, int n_corr, bool f16, const double2* sol, double* partial,
+                int n_partial, double* out, hipStream_t stream) {
+  DDH_REQUIRE(n_partial >= Chi2Partials(a), -1, "chi2 partial buffer too small");
+  const dim3 grid(a.bl_stride / kWave, a.n_cb);
+  if (n_corr == 2) {
+    if (f16)
+      hipLaunchKernelGGL((Chi2Kernel<2, true>), grid, dim3(256), 0, stream, a, sol, partial);
+    else
+      hipLaunchKernelGGL((Chi2Kernel<2, false>), grid, dim3(256), 0, stream, a, sol, partial);
+  } else {
+    if (f16)
+      hipLaunchKernelGGL((Chi2Kernel<4, true>), grid, dim3(256), 0, stream, a, sol, partial);
+    else
+      hipLaunchKernelGGL((Chi2Kernel<4, false>), grid, dim3(256), 0, stream, a, sol, partial);
+  }
+  hipLaunchKernelGGL(SumPartialsKernel, dim3(1), dim3(64), 0, stream, partial,
+                     Chi2Partials(a), out);
   DDH_HIP_CHECK(hipGetLastError());
 }
 

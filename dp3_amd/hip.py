@@ -125,12 +125,144 @@ SIGNATURES = {
     "ddh_batch_set_schedule": ([C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32],
                                C.c_int),
     "ddh_batch_solve": ([C.c_void_p, C.c_void_p, C.POINTER(Result)], C.c_int),
+    "ddh_batch_rebind": ([C.c_void_p, C.c_void_p], C.c_int),
     "ddh_batch_last_time": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "ddh_batch_set_profiling": ([C.c_void_p, C.c_int32], C.c_int),
+    "ddh_batch_last_timing": ([C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
+                              C.c_int),
     "ddh_autotune": ([C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                       C.POINTER(C.c_double)], C.c_int),
     "ddh_batch_autotune": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                             C.POINTER(C.c_double)], C.c_int),
+    "ddh_one_step": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "ddh_chi2": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "ddh_mlp_create": ([C.c_int32, C.c_int32, C.c_int32, C.c_uint32], C.c_void_p),
+    "ddh_mlp_destroy": ([C.c_void_p], None),
+    "ddh_mlp_num_params": ([C.c_void_p], C.c_int64),
+    "ddh_mlp_get_params": ([C.c_void_p, C.c_void_p], C.c_int),
+    "ddh_mlp_set_params": ([C.c_void_p, C.c_void_p], C.c_int),
+    "ddh_mlp_forward": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
+    "ddh_mlp_loss_and_grads": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                C.POINTER(C.c_float), C.c_void_p], C.c_int),
+    "ddh_mlp_train": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
+                       C.c_float, C.c_uint32, C.POINTER(C.c_float)], C.c_int),
+    "ddh_gemm_bf16": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                       C.c_int32, C.c_int32], C.c_int),
+    "ddh_warmstart_features": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_int32, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
+    "ddh_warmstart_apply": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
 }
+
+N_WARM_FEATURES = 16
+N_WARM_OUTPUTS = 4
+
+
+def _rows_ptrs(a):
+    return (C.c_void_p * a.shape[0])(*[a[i].ctypes.data for i in range(a.shape[0])])
+
+
+def one_step(solver, dproblem, solutions):
+    """Undamped solver update from `solutions` and its denominators."""
+    sol = np.ascontiguousarray(solutions, dtype=np.complex128)
+    nxt = np.zeros_like(sol)
+    den = np.zeros(sol.shape, np.float64)
+    _check(lib().ddh_one_step(solver.handle, dproblem.handle, C.cast(_rows_ptrs(sol), C.c_void_p),
+                              C.cast(_rows_ptrs(nxt), C.c_void_p),
+                              C.cast(_rows_ptrs(den), C.c_void_p)), "ddh_one_step")
+    return nxt, den
+
+
+def chi2(solver, dproblem, solutions):
+    sol = np.ascontiguousarray(solutions, dtype=np.complex128)
+    out = C.c_double()
+    _check(lib().ddh_chi2(solver.handle, dproblem.handle, C.cast(_rows_ptrs(sol), C.c_void_p),
+                          C.byref(out)), "ddh_chi2")
+    return out.value
+
+
+def gemm_bf16(a, b, trans_a=False, trans_b=False):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    M = a.shape[1] if trans_a else a.shape[0]
+    K = a.shape[0] if trans_a else a.shape[1]
+    N = b.shape[0] if trans_b else b.shape[1]
+    c = np.zeros((M, N), np.float32)
+    _check(lib().ddh_gemm_bf16(_ptr(a), _ptr(b), _ptr(c), M, N, K, int(trans_a), int(trans_b)),
+           "ddh_gemm_bf16")
+    return c
+
+
+def warmstart_features(g0, next1, ga, gb, den, n_cb, n_ant, n_sub):
+    arrs = [np.ascontiguousarray(x, np.complex128) for x in (g0, next1, ga, gb)]
+    d = np.ascontiguousarray(den, np.float64)
+    out = np.zeros((n_cb * n_ant * n_sub, N_WARM_FEATURES), np.float32)
+    _check(lib().ddh_warmstart_features(*[_ptr(a) for a in arrs], _ptr(d), n_cb, n_ant, n_sub,
+                                        _ptr(out)), "ddh_warmstart_features")
+    return out
+
+
+def warmstart_apply(prev, pred):
+    p = np.ascontiguousarray(prev, np.complex128)
+    q = np.ascontiguousarray(pred, np.float32)
+    out = np.zeros_like(p)
+    _check(lib().ddh_warmstart_apply(_ptr(p), _ptr(q), p.size // 2, _ptr(out)),
+           "ddh_warmstart_apply")
+    return out
+
+
+class MLP:
+    """3-layer perceptron on the GPU (bf16 MFMA GEMMs, FP32 accumulation)."""
+
+    def __init__(self, n_in, n_hidden, n_out, seed=0):
+        h = lib().ddh_mlp_create(n_in, n_hidden, n_out, seed)
+        if not h:
+            raise HipError(f"ddh_mlp_create failed: {last_error()}")
+        self.handle = h
+        self.n_in, self.n_hidden, self.n_out = n_in, n_hidden, n_out
+
+    def num_params(self):
+        return int(lib().ddh_mlp_num_params(self.handle))
+
+    def get_params(self):
+        p = np.zeros(self.num_params(), np.float32)
+        _check(lib().ddh_mlp_get_params(self.handle, _ptr(p)), "ddh_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.num_params()
+        _check(lib().ddh_mlp_set_params(self.handle, _ptr(p)), "ddh_mlp_set_params")
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], self.n_out), np.float32)
+        _check(lib().ddh_mlp_forward(self.handle, _ptr(x), x.shape[0], _ptr(y)), "ddh_mlp_forward")
+        return y
+
+    def loss_and_grads(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.num_params(), np.float32)
+        loss = C.c_float()
+        _check(lib().ddh_mlp_loss_and_grads(self.handle, _ptr(x), _ptr(t), x.shape[0],
+                                            C.byref(loss), _ptr(g)), "ddh_mlp_loss_and_grads")
+        return loss.value, g
+
+    def train(self, x, t, epochs=5, batch=4096, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = C.c_float()
+        _check(lib().ddh_mlp_train(self.handle, _ptr(x), _ptr(t), x.shape[0], epochs, batch, lr,
+                                   seed, C.byref(loss)), "ddh_mlp_train")
+        return loss.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().ddh_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
 
 
 def calibrate_bandwidth(nbytes=1 << 30, repeats=10):
@@ -304,6 +436,13 @@ class Batch:
             raise HipError(f"ddh_batch_create failed: {last_error()}")
         self.handle = h
 
+    def rebind(self, dproblems):
+        """Next intervals of the same shapes; keeps buffers, schedule and graph."""
+        self.dproblems = list(dproblems)
+        n = len(self.dproblems)
+        ph = (C.c_void_p * n)(*[p.handle for p in self.dproblems])
+        _check(lib().ddh_batch_rebind(self.handle, C.cast(ph, C.c_void_p)), "ddh_batch_rebind")
+
     def set_schedule(self, waves_per_tile=2, rows_per_trip=1, use_graph=1, poll_interval=8):
         _check(lib().ddh_batch_set_schedule(self.handle, waves_per_tile, rows_per_trip,
                                             use_graph, poll_interval), "ddh_batch_set_schedule")
@@ -334,6 +473,15 @@ class Batch:
         ms = C.c_double()
         _check(lib().ddh_batch_last_time(self.handle, C.byref(ms)), "ddh_batch_last_time")
         return ms.value
+
+    def set_profiling(self, enable=True):
+        _check(lib().ddh_batch_set_profiling(self.handle, int(enable)), "ddh_batch_set_profiling")
+
+    def last_timing(self):
+        ms, n = C.c_double(), C.c_int64()
+        _check(lib().ddh_batch_last_timing(self.handle, C.byref(ms), C.byref(n)),
+               "ddh_batch_last_timing")
+        return ms.value, n.value
 
     def close(self):
         if getattr(self, "handle", None):

@@ -131,11 +131,18 @@ def lm_to_radec(ref, l, m):
     return ra, dec
 
 
-def make_geometry(w: Workload, rng: np.random.Generator, n_times=None) -> Geometry:
+def make_geometry(w: Workload, rng: np.random.Generator, n_times=None, start_step=0,
+                  station_enu=None) -> Geometry:
+    """Station UVW for n_times integrations starting `start_step` integrations
+    after the first. A given station_enu keeps the layout of another interval
+    (no random draws are then made)."""
     n_times = w.solint if n_times is None else n_times
-    r = np.minimum(np.exp(rng.normal(np.log(400.0), 1.0, w.n_stations)), 3000.0)
-    az = rng.uniform(0, 2 * np.pi, w.n_stations)
-    enu = np.stack([r * np.cos(az), r * np.sin(az), np.zeros(w.n_stations)], axis=1)
+    if station_enu is None:
+        r = np.minimum(np.exp(rng.normal(np.log(400.0), 1.0, w.n_stations)), 3000.0)
+        az = rng.uniform(0, 2 * np.pi, w.n_stations)
+        enu = np.stack([r * np.cos(az), r * np.sin(az), np.zeros(w.n_stations)], axis=1)
+    else:
+        enu = np.asarray(station_enu, np.float64)
     lat = 52.91 * DEG
     # ENU -> local equatorial XYZ
     X = -np.sin(lat) * enu[:, 1] + np.cos(lat) * enu[:, 2]
@@ -145,7 +152,7 @@ def make_geometry(w: Workload, rng: np.random.Generator, n_times=None) -> Geomet
     dec0 = ref[1]
     uvw = np.zeros((n_times, w.n_stations, 3))
     for t in range(n_times):
-        H = -0.3 + EARTH_ROTATION * w.integration_time * t
+        H = -0.3 + EARTH_ROTATION * w.integration_time * (start_step + t)
         sh, ch = np.sin(H), np.cos(H)
         uvw[t, :, 0] = sh * X + ch * Y
         uvw[t, :, 1] = -np.sin(dec0) * ch * X + np.sin(dec0) * sh * Y + np.cos(dec0) * Z

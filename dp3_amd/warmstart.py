@@ -1,0 +1,164 @@
+"""Learned warm start for consecutive solution intervals.
+
+DP3 starts each interval from unit gains or from the previous interval's
+solutions (propagatesolutions, steps/DDECal.cc:487-499). This module adds a
+third option: a small network, trained on the solver's own outputs, maps per
+(channel block, antenna, solution)
+  * the undamped one-step update of the previous solutions on the current
+    interval (the data/model correlation summary, ddh_one_step), relative to
+    the previous solutions, with its denominator as a confidence, and
+  * the previous interval's drift
+to a multiplicative correction of the previous solutions. The plain solver
+then runs to its stopping criterion from that start, so the network only
+decides where the iteration begins, never when it ends.
+
+Everything numerical runs on the GPU through libddecal_hip (features, network
+forward/backward in bf16 MFMA GEMMs, Adam, application of the correction).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import hip as H
+from . import synth as S
+
+
+def sibling(solver: H.Solver, max_iterations, accuracy, detect_stalling=True):
+    """A solver with the same shapes and step as `solver` but its own stopping
+    criterion (device problems of one are valid for the other)."""
+    c = solver.config
+    return H.Solver(solver.n_antennas, solver._nsd, solver.n_channel_blocks,
+                    n_correlations=solver.n_correlations, max_iterations=max_iterations,
+                    accuracy=accuracy, step_size=c.step_size, detect_stalling=detect_stalling,
+                    phase_only=bool(c.phase_only), model_storage=solver.model_storage)
+
+
+class WarmStart:
+    """Features: the undamped one-step update of the previous solutions g0 on
+    the current interval and two iterates of the damped solver (after
+    `stretch` and 2 * `stretch` iterations), all relative to g0, plus the
+    update's denominators. Output: log(g_start / g0) per polarization."""
+
+    N_IN = H.N_WARM_FEATURES
+    N_OUT = H.N_WARM_OUTPUTS
+
+    def __init__(self, hidden=256, seed=0, stretch=4):
+        self.net = H.MLP(self.N_IN, hidden, self.N_OUT, seed)
+        self.stretch = stretch
+        self.trained = False
+        self._aux = {}
+
+    def _aux_solver(self, solver):
+        key = id(solver)
+        if key not in self._aux:
+            self._aux[key] = (solver, sibling(solver, self.stretch, 0.0, False))
+        return self._aux[key][1]
+
+    def features(self, solver, dproblem, g0):
+        """Features [n_pairs, 16] of the current interval."""
+        nxt, den = H.one_step(solver, dproblem, g0)
+        aux = self._aux_solver(solver)
+        ga = np.array(g0, np.complex128, copy=True)
+        aux.solve_problem(dproblem, ga)
+        gb = ga.copy()
+        aux.solve_problem(dproblem, gb)
+        return H.warmstart_features(g0, nxt, ga, gb, den, solver.n_channel_blocks,
+                                    solver.n_antennas, solver.n_sub)
+
+    @staticmethod
+    def targets(g0, final):
+        """log(final / g0) per polarization as (log-magnitude, phase) pairs."""
+        p = np.asarray(g0, np.complex128).reshape(-1, 2)
+        f = np.asarray(final, np.complex128).reshape(-1, 2)
+        with np.errstate(all="ignore"):
+            r = f / p
+            out = np.stack([np.log(np.abs(r[:, 0])), np.angle(r[:, 0]),
+                            np.log(np.abs(r[:, 1])), np.angle(r[:, 1])], axis=1)
+        out[~np.isfinite(out)] = 0.0
+        return out.astype(np.float32)
+
+    def fit(self, x, t, epochs=10, batch=4096, lr=1e-3, seed=0):
+        loss = self.net.train(x, t, epochs=epochs, batch=batch, lr=lr, seed=seed)
+        self.trained = True
+        return loss
+
+    def initial(self, solver, dproblem, g0):
+        x = self.features(solver, dproblem, g0)
+        pred = self.net.forward(x)
+        return H.warmstart_apply(g0, pred).reshape(np.asarray(g0).shape)
+
+
+def drifting_gains(w: S.Workload, rng, n_intervals, sigma_phase=0.05, sigma_amp=0.005):
+    """Gains per interval: amplitude and phase random walks from the usual
+    initial draw (amplitude 1 +- 0.1, phase uniform +-pi)."""
+    shape = (w.n_channel_blocks, w.n_stations, w.n_sub, 2)
+    amp = rng.uniform(0.9, 1.1, shape)
+    ph = rng.uniform(-np.pi, np.pi, shape)
+    out = []
+    for _ in range(n_intervals):
+        out.append((amp * np.exp(1j * ph)).reshape(w.n_channel_blocks, -1))
+        amp = amp + rng.normal(0.0, sigma_amp, shape)
+        ph = ph + rng.normal(0.0, sigma_phase, shape)
+    return out
+
+
+class IntervalSequence:
+    """Consecutive solution intervals of one observation, built on the GPU:
+    same stations and sky, UVW advancing with earth rotation, drifting gains."""
+
+    def __init__(self, w: S.Workload, n_intervals, seed=0, storage=H.MODEL_C64,
+                 sigma_phase=0.05, sigma_amp=0.005, solver_kwargs=None, build_from=0):
+        rng = np.random.default_rng(seed)
+        g0 = S.make_geometry(w, rng)
+        self.w = w
+        self.sky = S.make_sky(w, g0, rng)
+        self.gains = drifting_gains(w, rng, n_intervals, sigma_phase, sigma_amp)
+        kw = dict(max_iterations=w.max_iterations, accuracy=w.tolerance, step_size=w.step_size,
+                  model_storage=storage)
+        kw.update(solver_kwargs or {})
+        self.solver = H.Solver(w.n_stations, w.sols_per_dir(), w.n_channel_blocks, **kw)
+        ch = np.diff(S.channel_begin(w.n_channels, w.n_channel_blocks)).astype(np.int32)
+        n_t = w.solint
+        start = np.concatenate([[0], np.cumsum(w.sols_per_dir())[:-1]])
+        self.sol_index = np.zeros((w.n_directions, n_t), np.uint32)
+        for d in range(w.n_directions):
+            self.sol_index[d] = (np.arange(n_t) * w.sols_per_dir()[d]) // n_t + start[d]
+        comps = H.components_from_sky(self.sky)
+        self.problems = []
+        for i in range(n_intervals):
+            if i < build_from:  # same random draws, no device problem
+                self.problems.append(None)
+                continue
+            geom = S.make_geometry(w, rng, start_step=i * n_t, station_enu=g0.station_enu)
+            dp = H.DeviceProblem(self.solver, n_t, geom.baselines[:, 0], geom.baselines[:, 1],
+                                 ch, self.sol_index)
+            dp.predict(geom.ref, geom.uvw, geom.freq, geom.chan_width, comps)
+            dp.synthesize_data(self.gains[i], self.sol_index, w.noise_sigma, seed=seed * 1000 + i)
+            self.problems.append(dp)
+
+    def unit(self):
+        return np.ones((self.w.n_channel_blocks, self.w.n_stations * self.w.n_sub * 2),
+                       np.complex128)
+
+
+def training_set(seq: IntervalSequence, ws: WarmStart, first, last, converge_iterations=600):
+    """Solves intervals [first, last) to convergence (tolerance 1e-9, at most
+    `converge_iterations`) from the previous interval's solutions and collects
+    (features, targets) for the learned start. Returns x, t and the last
+    converged solutions."""
+    long = sibling(seq.solver, converge_iterations, 1e-9, False)
+    prev = None
+    xs, ts = [], []
+    for i in range(first, last):
+        dp = seq.problems[i]
+        if prev is None:
+            sol = seq.unit()
+            long.solve_problem(dp, sol)
+            prev = sol
+            continue
+        xs.append(ws.features(seq.solver, dp, prev))
+        sol = prev.copy()
+        long.solve_problem(dp, sol)
+        ts.append(ws.targets(prev, sol))
+        prev = sol
+    return np.concatenate(xs), np.concatenate(ts), prev

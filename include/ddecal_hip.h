@@ -182,6 +182,19 @@ int ddh_set_schedule(ddh_solver* solver, int32_t waves_per_tile,
 // (ddh_solver_last_timing then reports their summed duration).
 int ddh_set_profiling(ddh_solver* solver, int32_t enable);
 
+// One undamped solver update from `solutions_in` (the solution of the
+// Jacobi step, IterativeDiagonalSolver.cc:206-221, without the damping of
+// SolverBase::Step), with its denominators: next_out[cb] like the solutions,
+// den_out[cb] -> [n_ant][n_sub][2] doubles (may be NULL). Used as the
+// data/model correlation summary of the warm start.
+int ddh_one_step(ddh_solver* solver, ddh_problem* problem, double* const* solutions_in,
+                 double* const* next_out, double* const* den_out);
+
+// FP64 chi^2 of the residual D - sum_d G_a M_d G_b^H of a device problem for
+// the given solutions (no counterpart in the reference, which computes none).
+int ddh_chi2(ddh_solver* solver, ddh_problem* problem, double* const* solutions,
+             double* chi2);
+
 // Batched solves: many solution intervals (each its own solver
 // configuration and device problem; all with the same correlation count and
 // model storage) advance together, one launch per kernel per iteration over
@@ -194,9 +207,17 @@ ddh_batch* ddh_batch_create(ddh_solver* const* solvers, ddh_problem* const* prob
 void ddh_batch_destroy(ddh_batch* batch);
 int ddh_batch_set_schedule(ddh_batch* batch, int32_t waves_per_tile, int32_t rows_per_trip,
                            int32_t use_graph, int32_t poll_interval);
+// Points the batch at other problems of the same shapes (the next intervals
+// of the same sequences), keeping buffers, schedule and captured graph.
+int ddh_batch_rebind(ddh_batch* batch, ddh_problem* const* problems);
 // solutions[i][cb] -> interval i, channel block cb ([n_ant][n_sub][2] complex128).
 int ddh_batch_solve(ddh_batch* batch, double* const* const* solutions, ddh_result* results);
 int ddh_batch_last_time(const ddh_batch* batch, double* ms);
+// HIP events around every batched correlate launch of the next solves (graph
+// replay off while enabled); ddh_batch_last_timing reports their summed
+// duration and the number of launches that did work.
+int ddh_batch_set_profiling(ddh_batch* batch, int32_t enable);
+int ddh_batch_last_timing(const ddh_batch* batch, double* correlate_ms, int64_t* launches);
 
 // Schedule autotuner: times the correlate kernel under every candidate
 // schedule (waves per tile x rows per trip) on the actual shapes and keeps
@@ -205,6 +226,37 @@ int ddh_autotune(ddh_solver* solver, ddh_problem* problem, int32_t* waves_per_ti
                  int32_t* rows_per_trip, double* best_ms);
 int ddh_batch_autotune(ddh_batch* batch, int32_t* waves_per_tile, int32_t* rows_per_trip,
                        double* best_ms);
+
+// Warm-start network (no counterpart in the reference): a 3-layer
+// perceptron with bf16 weights on the GPU (bf16 MFMA GEMMs, FP32
+// accumulation, FP32 master weights, Adam). Parameters are one flat FP32
+// array: W1 [n_in x h], b1 [h], W2 [h x h], b2 [h], W3 [h x n_out], b3 [n_out].
+typedef struct ddh_mlp ddh_mlp;
+ddh_mlp* ddh_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_out, uint32_t seed);
+void ddh_mlp_destroy(ddh_mlp* net);
+int64_t ddh_mlp_num_params(const ddh_mlp* net);
+int ddh_mlp_get_params(const ddh_mlp* net, float* params);
+int ddh_mlp_set_params(ddh_mlp* net, const float* params);
+int ddh_mlp_forward(ddh_mlp* net, const float* x, int64_t n, float* y);
+// Mean squared error over n rows and its gradient w.r.t. the parameters.
+int ddh_mlp_loss_and_grads(ddh_mlp* net, const float* x, const float* t, int64_t n,
+                           float* loss, float* grads);
+int ddh_mlp_train(ddh_mlp* net, const float* x, const float* t, int64_t n, int32_t epochs,
+                  int32_t batch, float lr, uint32_t seed, float* last_epoch_loss);
+// C [M x N] = op(A) op(B) through the bf16 MFMA GEMM (operands rounded to
+// bf16, FP32 accumulation); op transposes when the flag is set (A given as
+// [K x M], B as [N x K]).
+int ddh_gemm_bf16(const float* a, const float* b, float* c, int32_t M, int32_t N, int32_t K,
+                  int32_t trans_a, int32_t trans_b);
+// Features per solution pair (16 floats) on the current interval: the
+// undamped one-step update of the previous solutions g0 (next1, with its
+// denominators den) and two later iterates ga, gb of the damped solver, each
+// relative to g0; and the initial solutions g0 * exp(pred) from the
+// network's 4 outputs per pair.
+int ddh_warmstart_features(const double* g0, const double* next1, const double* ga,
+                           const double* gb, const double* den, int32_t n_cb, int32_t n_ant,
+                           int32_t n_sub, float* features);
+int ddh_warmstart_apply(const double* prev, const float* pred, int64_t n_pairs, double* init);
 
 // Bandwidth calibration: a plain copy and a plain streaming read over
 // `bytes` (each repeated `repeats` times). copy_gbs counts bytes read plus

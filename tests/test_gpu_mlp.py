@@ -1,0 +1,123 @@
+"""Warm-start network on the GPU: bf16 MFMA GEMM against numpy, forward and
+backward against the FP64 oracle (same bf16 rounding points), training, the
+one-step estimate and chi^2 against the CPU oracle, and the warm start's gain
+over starting from the previous solutions (propagatesolutions) and from unity."""
+import numpy as np
+import pytest
+
+from dp3_amd import hip as H
+from dp3_amd import synth as S
+from dp3_amd import warmstart as WS
+from oracle import mlp_oracle as MO
+from oracle import pyoracle as O
+from problems import small_problem
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [(64, 64, 16, 0, 0), (100, 37, 53, 0, 0),
+                                         (33, 256, 256, 1, 0), (256, 4, 300, 0, 1),
+                                         (5, 7, 3, 1, 1)])
+def test_gemm_bf16_matches_numpy(M, N, K, ta, tb):
+    print(f"case: gemm {M}x{N}x{K} ta={ta} tb={tb}", flush=True)
+    rng = np.random.default_rng(M * 1000 + N)
+    a = rng.normal(size=(K, M) if ta else (M, K)).astype(np.float32)
+    b = rng.normal(size=(N, K) if tb else (K, N)).astype(np.float32)
+    c = H.gemm_bf16(a, b, bool(ta), bool(tb))
+    ar, br = MO.bf16(a), MO.bf16(b)
+    ref = (ar.T if ta else ar) @ (br.T if tb else br)
+    scale = np.abs(ar).max() * np.abs(br).max() * K
+    assert np.abs(c - ref).max() <= 1e-6 * scale + 1e-5 * np.abs(ref).max()
+
+
+def test_mlp_gradients_match_fp64_oracle():
+    print("case: mlp gradients", flush=True)
+    n_in, h, n_out, n = 16, 64, 4, 333
+    net = H.MLP(n_in, h, n_out, seed=3)
+    rng = np.random.default_rng(0)
+    params = (0.2 * rng.normal(size=net.num_params())).astype(np.float32)
+    net.set_params(params)
+    x = rng.normal(size=(n, n_in)).astype(np.float32)
+    t = rng.normal(size=(n, n_out)).astype(np.float32)
+    loss, g = net.loss_and_grads(x, t)
+    ref_loss, ref_g, ref_y = MO.loss_and_grads(params, x, t, n_in, h, n_out)
+    np.testing.assert_allclose(net.forward(x), ref_y, rtol=0, atol=1e-4 * np.abs(ref_y).max())
+    assert abs(loss - ref_loss) <= 1e-5 * ref_loss
+    # per parameter block, relative to the block's scale
+    sizes = [n_in * h, h, h * h, h, h * n_out, n_out]
+    o = 0
+    for sz in sizes:
+        gb, rb = g[o:o + sz], ref_g[o:o + sz]
+        assert np.abs(gb - rb).max() <= 2e-3 * np.abs(rb).max() + 1e-9, o
+        o += sz
+
+
+def test_mlp_training_fits_a_smooth_map():
+    print("case: mlp training", flush=True)
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(20000, 16)).astype(np.float32)
+    t = np.stack([np.tanh(x[:, 0]), 0.5 * x[:, 1] * x[:, 2], np.sin(x[:, 3]), x[:, 4] ** 2 / 4],
+                 axis=1).astype(np.float32)
+    net = H.MLP(16, 128, 4, seed=0)
+    l0, _ = net.loss_and_grads(x[:4096], t[:4096])
+    loss = net.train(x, t, epochs=20, batch=1024, lr=3e-3, seed=0)
+    l1, _ = net.loss_and_grads(x[:4096], t[:4096])
+    print(f"loss {l0:.4f} -> {l1:.4f} (last epoch {loss:.4f})")
+    assert l1 < 0.1 * l0
+
+
+@pytest.fixture(scope="module")
+def small():
+    return small_problem()
+
+
+def test_one_step_and_chi2_match_oracle(small):
+    print("case: one step + chi2", flush=True)
+    w, geom, sky, gains, P = small
+    s = H.Solver(P.n_antennas, P.n_solutions_per_direction, P.n_channel_blocks)
+    dp = H.DeviceProblem.from_problem(s, P)
+    dp.upload(P)
+    rng = np.random.default_rng(4)
+    start = gains * (1 + 0.05 * rng.normal(size=gains.shape))
+    nxt, den = H.one_step(s, dp, start)
+    ref = O.iterate_once(P, start, fp64=True)
+    assert np.max(np.abs(nxt - ref) / np.abs(ref)) <= 1e-4
+    assert np.all(den > 0)
+    c = H.chi2(s, dp, gains)
+    assert abs(c - O.chi2(P, gains)) <= 1e-9 * O.chi2(P, gains)
+
+
+def test_warm_start_reduces_iterations():
+    """Consecutive intervals with drifting gains: train on the first intervals'
+    converged solver outputs, evaluate on the rest. Every start converges to
+    the same chi^2 (the plain solver decides when to stop); the learned start
+    needs fewer iterations than the previous solutions, which need fewer than
+    unity."""
+    print("case: warm start sequence", flush=True)
+    w = S.Workload("ws", 16, 3, 2, 1, 16, 4, 4, 400, 1e-6, seed=5)
+    seq = WS.IntervalSequence(w, 36, seed=5)
+    ws = WS.WarmStart(hidden=256, seed=0)
+    x, t, prev = WS.training_set(seq, ws, 0, 26)
+    loss = ws.fit(x, t, epochs=30, batch=1024, lr=1e-3)
+    print(f"train samples {len(x)}, last epoch loss {loss:.3e}, target var {t.var():.3e}")
+    its = {"unit": [], "previous": [], "learned": []}
+    chis = {k: [] for k in its}
+    for i in range(26, 36):
+        dp = seq.problems[i]
+        starts = {"unit": seq.unit(), "previous": prev.copy(),
+                  "learned": ws.initial(seq.solver, dp, prev)}
+        finals = {}
+        for k, sol in starts.items():
+            r = seq.solver.solve_problem(dp, sol)
+            its[k].append(r.iterations)
+            chis[k].append(H.chi2(seq.solver, dp, sol))
+            finals[k] = sol
+        prev = finals["previous"]
+    for k in its:
+        print(f"{k:9s} mean iterations {np.mean(its[k]):6.1f}  mean chi2 {np.mean(chis[k]):.6e}")
+    for k in ("unit", "learned"):
+        np.testing.assert_allclose(chis[k], chis["previous"], rtol=1e-3)
+    # the learned start includes 2 x stretch damped iterations of its own
+    learned_total = np.mean(its["learned"]) + 2 * ws.stretch + 1
+    print(f"learned incl. feature iterations {learned_total:.1f}")
+    assert learned_total < np.mean(its["previous"]) < np.mean(its["unit"])

@@ -317,3 +317,31 @@ def test_synthetic_problem_is_consistent():
     assert abs(O.chi2(P, gains) - expect) < 0.05 * expect
     w2, _, _, g2, P2 = small_problem(noise_sigma=0.0)
     assert O.chi2(P2, g2) < 1e-8 * np.sum(np.abs(P2.data) ** 2)
+
+
+def test_mlp_oracle_backward_matches_finite_differences():
+    """The FP64 network oracle's backward pass (rounding off) against central
+    differences of its own loss."""
+    from oracle import mlp_oracle as MO
+    n_in, h, n_out, n = 5, 7, 3, 11
+    rng = np.random.default_rng(0)
+    params = rng.normal(size=n_in * h + h + h * h + h + h * n_out + n_out)
+    x = rng.normal(size=(n, n_in))
+    t = rng.normal(size=(n, n_out))
+    _, g, _ = MO.loss_and_grads(params, x, t, n_in, h, n_out, rounding=False)
+    eps = 1e-6
+    for i in range(0, params.size, 3):
+        p1, p2 = params.copy(), params.copy()
+        p1[i] += eps
+        p2[i] -= eps
+        l1 = MO.loss_and_grads(p1, x, t, n_in, h, n_out, rounding=False)[0]
+        l2 = MO.loss_and_grads(p2, x, t, n_in, h, n_out, rounding=False)[0]
+        assert abs((l1 - l2) / (2 * eps) - g[i]) < 1e-6 * max(1.0, abs(g[i]))
+
+
+def test_bf16_rounding_is_round_to_nearest_even():
+    from oracle import mlp_oracle as MO
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, 1.0 + 2 ** -9, -2.5, 3.0e38], np.float32)
+    r = MO.bf16(x)
+    # 1 + 2^-9 is a tie between 1 and 1 + 2^-7 -> even (1.0); 1 + 3*2^-9 rounds up
+    np.testing.assert_array_equal(r[:5], [1.0, 1.0, 1.0 + 2 ** -7, 1.0, -2.5])
