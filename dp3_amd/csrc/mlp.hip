@@ -19,6 +19,7 @@
 
 #include "../../include/ddecal_hip.h"
 #include "common.h"
+#include "warm.h"
 
 namespace ddh {
 namespace {
@@ -179,8 +180,6 @@ __global__ void GatherRowsKernel(const float* __restrict__ src, const int32_t* _
 //   s  = log10(den / mean den over the antennas of this (cb, sub))
 // Features (kWarmFeatures, zero padded): for p = 0, 1:
 //   [log|r1|, arg r1, log|ra|, arg ra, log|rb|, arg rb, s].
-constexpr int kWarmFeatures = 16;
-constexpr int kWarmOutputs = 4;
 
 __device__ inline void LogRatio(double2 a, double2 b, float& lmag, float& ang) {
   // log(a / b) = log|a| - log|b| + i (arg a - arg b), wrapped to (-pi, pi]
@@ -304,9 +303,7 @@ struct ddh_mlp {
 namespace ddh {
 namespace {
 
-void Forward(ddh_mlp& net, int64_t n, bool keep) {
-  (void)keep;
-  hipStream_t s = net.stream;
+void Forward(ddh_mlp& net, int64_t n, hipStream_t s) {
   const int h = net.n_hidden;
   ToBf16(net.x.ptr, net.x16.ptr, n * net.n_in, s);
   // H1 = relu(X W1 + b1)
@@ -377,6 +374,41 @@ void Reserve(ddh_mlp& net, int64_t n) {
 }
 
 }  // namespace
+
+void CheckWarmNetwork(const ddh_mlp& net) {
+  DDH_REQUIRE(net.n_in == kWarmFeatures && net.n_out == kWarmOutputs, -1,
+              "the warm-start network maps 16 features to 4 outputs");
+}
+
+void WaitForNetwork(ddh_mlp& net) { DDH_HIP_CHECK(hipStreamSynchronize(net.stream)); }
+
+float* MlpInputBuffer(ddh_mlp& net, int64_t n) {
+  Reserve(net, n);
+  return net.x.ptr;
+}
+
+const float* MlpForwardDevice(ddh_mlp& net, int64_t n, hipStream_t stream) {
+  Forward(net, n, stream);
+  DDH_HIP_CHECK(hipGetLastError());
+  return net.y.ptr;
+}
+
+void LaunchWarmFeatures(const double2* g0, const double2* next1, const double2* ga,
+                        const double2* gb, const double* den, int n_cb, int n_ant, int n_sub,
+                        float* features, hipStream_t stream) {
+  const int64_t n_pairs = int64_t(n_cb) * n_ant * n_sub;
+  hipLaunchKernelGGL(WarmFeaturesKernel, dim3(Blocks(n_pairs)), dim3(256), 0, stream, g0, next1,
+                     ga, gb, den, n_cb, n_ant, n_sub, features);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchWarmApply(const double2* g0, const float* pred, int64_t n_pairs, double2* init,
+                     hipStream_t stream) {
+  hipLaunchKernelGGL(WarmApplyKernel, dim3(Blocks(n_pairs)), dim3(256), 0, stream, g0, pred,
+                     n_pairs, init);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace ddh
 
 using namespace ddh;
@@ -455,7 +487,7 @@ int ddh_mlp_forward(ddh_mlp* net, const float* x, int64_t n, float* y) {
     Reserve(*net, n);
     DDH_HIP_CHECK(hipMemcpyAsync(net->x.ptr, x, n * net->n_in * sizeof(float),
                                  hipMemcpyHostToDevice, net->stream));
-    Forward(*net, n, false);
+    Forward(*net, n, net->stream);
     DDH_HIP_CHECK(hipMemcpyAsync(y, net->y.ptr, n * net->n_out * sizeof(float),
                                  hipMemcpyDeviceToHost, net->stream));
     DDH_HIP_CHECK(hipStreamSynchronize(net->stream));
@@ -471,7 +503,7 @@ int ddh_mlp_loss_and_grads(ddh_mlp* net, const float* x, const float* t, int64_t
                                  hipMemcpyHostToDevice, net->stream));
     DDH_HIP_CHECK(hipMemcpyAsync(net->t.ptr, t, n * net->n_out * sizeof(float),
                                  hipMemcpyHostToDevice, net->stream));
-    Forward(*net, n, true);
+    Forward(*net, n, net->stream);
     Backward(*net, n);
     float sq = 0.0f;
     DDH_HIP_CHECK(hipMemcpyAsync(&sq, net->loss.ptr, sizeof(float), hipMemcpyDeviceToHost,
@@ -513,7 +545,7 @@ int ddh_mlp_train(ddh_mlp* net, const float* x, const float* t, int64_t n, int32
                            s, dx, dperm + b0, int64_t(batch), net->n_in, net->x.ptr);
         hipLaunchKernelGGL(GatherRowsKernel, dim3(Blocks(int64_t(batch) * net->n_out)), dim3(256),
                            0, s, dt, dperm + b0, int64_t(batch), net->n_out, net->t.ptr);
-        Forward(*net, batch, true);
+        Forward(*net, batch, net->stream);
         Backward(*net, batch);
         net->step += 1;
         const float bias1 = 1.0f - std::pow(beta1, float(net->step));

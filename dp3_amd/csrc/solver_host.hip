@@ -14,6 +14,7 @@
 #include "../../include/ddecal_hip.h"
 #include "common.h"
 #include "solver_kernels.h"
+#include "warm.h"
 
 namespace ddh {
 
@@ -799,6 +800,12 @@ struct ddh_batch {
   std::vector<hipEvent_t> events;
   double last_correlate_ms = 0.0;
   int64_t last_correlate_launches = 0;
+  // learned warm start: argument variants and per-interval solution stages
+  ddh::DeviceBuffer<ddh::IterArgs> d_args_step, d_args_aux;
+  ddh::DeviceBuffer<double2> warm_g0, warm_next, warm_a, warm_b;
+  ddh::DeviceBuffer<double> warm_den;
+  hipEvent_t ev_warm0 = nullptr, ev_warm1 = nullptr;
+  double last_warm_ms = 0.0;
 };
 
 namespace ddh {
@@ -882,15 +889,130 @@ void EnsureBatchGraph(ddh_batch& b) {
   b.graph_unroll = b.unroll;
 }
 
-void SolveBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results) {
+// Complex solution values of interval i over all its channel blocks.
+size_t SolutionCount(const ddh_batch& b, int i) {
+  const ddh_solver& s = *b.solvers[i];
+  return size_t(s.cfg.c.n_channel_blocks) * s.cfg.c.n_antennas * s.cfg.n_sub * 2;
+}
+
+// Host solutions[i][cb] -> dst(i) on the batch stream.
+template <typename Dst>
+void UploadSolutions(ddh_batch& b, double* const* const* solutions, Dst&& dst) {
   const int n = int(b.solvers.size());
   for (int i = 0; i < n; ++i) {
     const ddh_solver& s = *b.solvers[i];
     const size_t per_cb = size_t(s.cfg.c.n_antennas) * s.cfg.n_sub * 2;
     for (int cb = 0; cb < s.cfg.c.n_channel_blocks; ++cb)
-      DDH_HIP_CHECK(hipMemcpyAsync(b.args[i].sol0 + cb * per_cb, solutions[i][cb],
+      DDH_HIP_CHECK(hipMemcpyAsync(dst(i) + cb * per_cb, solutions[i][cb],
                                    per_cb * sizeof(double2), hipMemcpyHostToDevice, b.stream));
   }
+}
+
+// Runs the iterations from the solutions in each interval's sol0 to every
+// interval's stopping criterion and downloads the results.
+void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results);
+
+void SolveBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results) {
+  UploadSolutions(b, solutions, [&](int i) { return b.args[i].sol0; });
+  b.last_warm_ms = 0.0;
+  RunBatch(b, solutions, results);
+}
+
+// The learned warm start on the device, for every interval at once:
+//   next1 = one undamped update of g0 (with its denominators),
+//   ga    = `stretch` damped iterations from g0, gb = `stretch` more from ga,
+//   x     = features(g0, next1, ga, gb, den), pred = network(x),
+//   start = g0 * exp(pred),
+// then the solve from `start`. Matches the per-interval host sequence
+// ddh_one_step, two ddh_solve_problem calls of `stretch` iterations at
+// accuracy 0 without stall detection, ddh_warmstart_features,
+// ddh_mlp_forward and ddh_warmstart_apply.
+void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* const* solutions,
+                    ddh_result* results) {
+  CheckWarmNetwork(net);
+  const int n = int(b.solvers.size());
+  std::vector<size_t> off(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    DDH_REQUIRE(stretch >= 1 && stretch <= std::max(b.solvers[i]->cfg.c.max_iterations,
+                                                    b.solvers[i]->cfg.c.min_iterations),
+                -1, "stretch must be 1..max_iterations");
+    off[i + 1] = off[i] + SolutionCount(b, i);
+  }
+  const size_t total = off[n];
+  b.warm_g0.Resize(total);
+  b.warm_next.Resize(total);
+  b.warm_a.Resize(total);
+  b.warm_b.Resize(total);
+  b.warm_den.Resize(total);
+  // undamped single step with denominators; fixed-length damped stretches
+  std::vector<IterArgs> step(b.args), aux(b.args);
+  for (int i = 0; i < n; ++i) {
+    step[i].step_size = 1.0;
+    step[i].phase_only = 0;
+    step[i].den_out = b.warm_den.ptr + off[i];
+    aux[i].max_iterations = stretch;
+    aux[i].min_iterations = 0;
+    aux[i].accuracy = 0.0;
+    aux[i].detect_stalling = 0;
+  }
+  b.d_args_step.Upload(step, b.stream);
+  b.d_args_aux.Upload(aux, b.stream);
+  if (!b.ev_warm0) DDH_HIP_CHECK(hipEventCreate(&b.ev_warm0));
+  if (!b.ev_warm1) DDH_HIP_CHECK(hipEventCreate(&b.ev_warm1));
+  WaitForNetwork(net);  // weights of any pending training
+  DDH_HIP_CHECK(hipEventRecord(b.ev_warm0, b.stream));
+  UploadSolutions(b, solutions, [&](int i) { return b.warm_g0.ptr + off[i]; });
+  const int n_pro = int(b.pro_items.size());
+  auto copy_into_sol0 = [&](const double2* src) {
+    for (int i = 0; i < n; ++i)
+      DDH_HIP_CHECK(hipMemcpyAsync(b.args[i].sol0, src + off[i], SolutionCount(b, i) * sizeof(double2),
+                                   hipMemcpyDeviceToDevice, b.stream));
+  };
+  auto run_aux = [&](double2* out) {
+    LaunchBatchPrologue(b.d_args_aux.ptr, b.d_pro_items.ptr, n_pro, b.stream);
+    for (int k = 0; k < stretch; ++k)
+      LaunchBatch(b.d_args_aux.ptr, b.d_tiles.ptr, int(b.tiles.size()), b.d_items.ptr,
+                  int(b.items.size()), n, b.correlate_lds, b.solve_lds, b.n_corr, b.f16, b.waves,
+                  b.unroll, b.stream);
+    for (int i = 0; i < n; ++i)
+      LaunchSnapshot(b.d_args_aux.ptr, i, int64_t(SolutionCount(b, i)), out + off[i], b.stream);
+  };
+  // next1: iteration 0 reads sol0 and writes sol1; no decision is taken
+  copy_into_sol0(b.warm_g0.ptr);
+  LaunchBatchPrologue(b.d_args_step.ptr, b.d_pro_items.ptr, n_pro, b.stream);
+  LaunchBatch(b.d_args_step.ptr, b.d_tiles.ptr, int(b.tiles.size()), b.d_items.ptr,
+              int(b.items.size()), 0, b.correlate_lds, b.solve_lds, b.n_corr, b.f16, b.waves,
+              b.unroll, b.stream);
+  for (int i = 0; i < n; ++i)
+    DDH_HIP_CHECK(hipMemcpyAsync(b.warm_next.ptr + off[i], b.args[i].sol1,
+                                 SolutionCount(b, i) * sizeof(double2), hipMemcpyDeviceToDevice,
+                                 b.stream));
+  run_aux(b.warm_a.ptr);  // sol0 still holds g0
+  copy_into_sol0(b.warm_a.ptr);
+  run_aux(b.warm_b.ptr);
+  // features of all intervals as consecutive rows, one forward pass
+  const int64_t rows = int64_t(total / 2);
+  float* x = MlpInputBuffer(net, rows);
+  for (int i = 0; i < n; ++i) {
+    const ddh_config& c = b.solvers[i]->cfg.c;
+    LaunchWarmFeatures(b.warm_g0.ptr + off[i], b.warm_next.ptr + off[i], b.warm_a.ptr + off[i],
+                       b.warm_b.ptr + off[i], b.warm_den.ptr + off[i], c.n_channel_blocks,
+                       c.n_antennas, b.solvers[i]->cfg.n_sub, x + int64_t(off[i] / 2) * kWarmFeatures,
+                       b.stream);
+  }
+  const float* pred = MlpForwardDevice(net, rows, b.stream);
+  for (int i = 0; i < n; ++i)
+    LaunchWarmApply(b.warm_g0.ptr + off[i], pred + int64_t(off[i] / 2) * kWarmOutputs,
+                    int64_t(SolutionCount(b, i) / 2), b.args[i].sol0, b.stream);
+  DDH_HIP_CHECK(hipEventRecord(b.ev_warm1, b.stream));
+  RunBatch(b, solutions, results);
+  float ms = 0.0f;
+  DDH_HIP_CHECK(hipEventElapsedTime(&ms, b.ev_warm0, b.ev_warm1));
+  b.last_warm_ms = ms;
+}
+
+void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results) {
+  const int n = int(b.solvers.size());
   const bool graph = b.use_graph != 0 && !b.profile;
   if (graph) EnsureBatchGraph(b);
   if (b.profile && int(b.events.size()) < 2 * b.limit) {
@@ -1036,6 +1158,8 @@ void ddh_batch_destroy(ddh_batch* b) {
   (void)hipStreamSynchronize(b->stream);
   DestroyBatchGraph(*b);
   for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+  if (b->ev_warm0) (void)hipEventDestroy(b->ev_warm0);
+  if (b->ev_warm1) (void)hipEventDestroy(b->ev_warm1);
   if (b->host_states) (void)hipHostFree(b->host_states);
   b->d_args.Free();
   b->d_tiles.Free();
@@ -1102,6 +1226,20 @@ int ddh_batch_solve(ddh_batch* b, double* const* const* solutions, ddh_result* r
     DDH_REQUIRE(b && solutions, -1, "NULL argument");
     SolveBatch(*b, solutions, results);
   });
+}
+
+int ddh_batch_solve_warm(ddh_batch* b, ddh_mlp* net, int32_t stretch,
+                         double* const* const* solutions, ddh_result* results) {
+  return Guard([&] {
+    DDH_REQUIRE(b && net && solutions, -1, "NULL argument");
+    SolveBatchWarm(*b, *net, stretch, solutions, results);
+  });
+}
+
+int ddh_batch_last_warm_time(const ddh_batch* b, double* ms) {
+  if (!b || !ms) return -1;
+  *ms = b->last_warm_ms;
+  return 0;
 }
 
 int ddh_batch_last_time(const ddh_batch* b, double* ms) {

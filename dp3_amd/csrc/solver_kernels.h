@@ -128,6 +128,10 @@ void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, int n_tiles,
                  const BatchTile* d_items, int n_items, int n_intervals, size_t correlate_lds,
                  size_t solve_lds, int n_corr, bool f16, int waves, int unroll,
                  hipStream_t stream);
+// Current solutions (by iteration parity) of batched interval `interval`
+// into dst, `count` complex values.
+void LaunchSnapshot(const IterArgs* d_args, int interval, int64_t count, double2* dst,
+                    hipStream_t stream);
 void LaunchBatchPrologue(const IterArgs* d_args, const BatchTile* d_items, int n_items,
                          hipStream_t stream);
 void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_items,

@@ -727,6 +727,17 @@ __global__ void __launch_bounds__(kBlockThreads)
   PrologueBlock(args[t.interval], t.cb);
 }
 
+// Copies the current solutions of one batched interval (the buffer its
+// iteration count selects) to dst.
+__global__ void SnapshotKernel(const IterArgs* __restrict__ args, int interval, int64_t count,
+                               double2* __restrict__ dst) {
+  const IterArgs& a = args[interval];
+  const double2* src = (a.state->iteration & 1) ? a.sol1 : a.sol0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += int64_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // Layout conversion for host-provided SolveData.
 // ---------------------------------------------------------------------------
@@ -937,6 +948,14 @@ void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_ite
   }
   if (n_intervals > 0)
     hipLaunchKernelGGL(BatchDecideKernel, dim3(n_intervals), dim3(256), 0, stream, d_args);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchSnapshot(const IterArgs* d_args, int interval, int64_t count, double2* dst,
+                    hipStream_t stream) {
+  const int64_t blocks = std::min<int64_t>((count + 255) / 256, 1024);
+  hipLaunchKernelGGL(SnapshotKernel, dim3(unsigned(std::max<int64_t>(blocks, 1))), dim3(256), 0,
+                     stream, d_args, interval, count, dst);
   DDH_HIP_CHECK(hipGetLastError());
 }
 

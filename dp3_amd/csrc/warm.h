@@ -1,0 +1,35 @@
+// Device-side pieces of the learned warm start, shared by the network
+// (mlp.hip) and the batched warm solve (solver_host.hip). Internal to the
+// library; the C-ABI is include/ddecal_hip.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+struct ddh_mlp;
+
+namespace ddh {
+
+constexpr int kWarmFeatures = 16;
+constexpr int kWarmOutputs = 4;
+
+// Throws unless the network maps kWarmFeatures inputs to kWarmOutputs outputs.
+void CheckWarmNetwork(const ddh_mlp& net);
+// Waits for work queued on the network's own stream (training).
+void WaitForNetwork(ddh_mlp& net);
+// Device input rows [n][n_in] of the next MlpForwardDevice; valid until the
+// network is used again.
+float* MlpInputBuffer(ddh_mlp& net, int64_t n);
+// Forward pass over the n rows of the input buffer on `stream`; returns the
+// device outputs [n][n_out].
+const float* MlpForwardDevice(ddh_mlp& net, int64_t n, hipStream_t stream);
+// Features [n_pairs][kWarmFeatures] of one interval (n_pairs = n_cb * n_ant * n_sub).
+void LaunchWarmFeatures(const double2* g0, const double2* next1, const double2* ga,
+                        const double2* gb, const double* den, int n_cb, int n_ant, int n_sub,
+                        float* features, hipStream_t stream);
+// init = g0 * exp(pred) per polarization.
+void LaunchWarmApply(const double2* g0, const float* pred, int64_t n_pairs, double2* init,
+                     hipStream_t stream);
+
+}  // namespace ddh

@@ -126,6 +126,9 @@ SIGNATURES = {
                                C.c_int),
     "ddh_batch_solve": ([C.c_void_p, C.c_void_p, C.POINTER(Result)], C.c_int),
     "ddh_batch_rebind": ([C.c_void_p, C.c_void_p], C.c_int),
+    "ddh_batch_solve_warm": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(Result)],
+                             C.c_int),
+    "ddh_batch_last_warm_time": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
     "ddh_batch_last_time": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
     "ddh_batch_set_profiling": ([C.c_void_p, C.c_int32], C.c_int),
     "ddh_batch_last_timing": ([C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
@@ -453,8 +456,7 @@ class Batch:
                "ddh_batch_autotune")
         return w.value, r.value, ms.value
 
-    def solve(self, solutions):
-        """solutions: list of [n_cb, n_ant*n_sub*2] complex128 arrays, in place."""
+    def _pointers(self, solutions):
         n = len(self.solvers)
         assert len(solutions) == n
         inner = []
@@ -463,11 +465,34 @@ class Batch:
             inner.append((C.c_void_p * sol.shape[0])(*[sol[cb].ctypes.data
                                                        for cb in range(sol.shape[0])]))
         outer = (C.c_void_p * n)(*[C.cast(x, C.c_void_p) for x in inner])
-        res = (Result * n)()
-        _check(lib().ddh_batch_solve(self.handle, C.cast(outer, C.c_void_p), res),
-               "ddh_batch_solve")
+        return inner, outer, (Result * n)()
+
+    @staticmethod
+    def _results(res):
         return [SolveResult(r.iterations, r.constraint_iterations, bool(r.has_converged),
                             r.n_step_magnitudes, r.last_step_magnitude) for r in res]
+
+    def solve(self, solutions):
+        """solutions: list of [n_cb, n_ant*n_sub*2] complex128 arrays, in place."""
+        inner, outer, res = self._pointers(solutions)
+        _check(lib().ddh_batch_solve(self.handle, C.cast(outer, C.c_void_p), res),
+               "ddh_batch_solve")
+        return self._results(res)
+
+    def solve_warm(self, net, solutions, stretch=4):
+        """Learned warm start on the device from the previous solutions held in
+        `solutions` (replaced by the final solutions)."""
+        inner, outer, res = self._pointers(solutions)
+        _check(lib().ddh_batch_solve_warm(self.handle, net.handle, stretch,
+                                          C.cast(outer, C.c_void_p), res),
+               "ddh_batch_solve_warm")
+        return self._results(res)
+
+    def last_warm_ms(self):
+        ms = C.c_double()
+        _check(lib().ddh_batch_last_warm_time(self.handle, C.byref(ms)),
+               "ddh_batch_last_warm_time")
+        return ms.value
 
     def last_time_ms(self):
         ms = C.c_double()

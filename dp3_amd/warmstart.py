@@ -103,49 +103,80 @@ def drifting_gains(w: S.Workload, rng, n_intervals, sigma_phase=0.05, sigma_amp=
 
 
 class IntervalSequence:
-    """Consecutive solution intervals of one observation, built on the GPU:
-    same stations and sky, UVW advancing with earth rotation, drifting gains."""
+    """Consecutive solution intervals of one observation, built on the GPU on
+    first use: same stations and sky, UVW advancing with earth rotation,
+    drifting gains. `problems[i]` builds interval i (predict + data synthesis
+    on the device) and keeps it resident until `release(i)`; the result does
+    not depend on the order in which intervals are built."""
 
     def __init__(self, w: S.Workload, n_intervals, seed=0, storage=H.MODEL_C64,
-                 sigma_phase=0.05, sigma_amp=0.005, solver_kwargs=None, build_from=0):
+                 sigma_phase=0.05, sigma_amp=0.005, solver_kwargs=None):
         rng = np.random.default_rng(seed)
-        g0 = S.make_geometry(w, rng)
         self.w = w
-        self.sky = S.make_sky(w, g0, rng)
+        self.seed = seed
+        self.n_intervals = n_intervals
+        self.geom0 = S.make_geometry(w, rng)
+        self.sky = S.make_sky(w, self.geom0, rng)
         self.gains = drifting_gains(w, rng, n_intervals, sigma_phase, sigma_amp)
         kw = dict(max_iterations=w.max_iterations, accuracy=w.tolerance, step_size=w.step_size,
                   model_storage=storage)
         kw.update(solver_kwargs or {})
         self.solver = H.Solver(w.n_stations, w.sols_per_dir(), w.n_channel_blocks, **kw)
-        ch = np.diff(S.channel_begin(w.n_channels, w.n_channel_blocks)).astype(np.int32)
+        self._ch = np.diff(S.channel_begin(w.n_channels, w.n_channel_blocks)).astype(np.int32)
         n_t = w.solint
         start = np.concatenate([[0], np.cumsum(w.sols_per_dir())[:-1]])
         self.sol_index = np.zeros((w.n_directions, n_t), np.uint32)
         for d in range(w.n_directions):
             self.sol_index[d] = (np.arange(n_t) * w.sols_per_dir()[d]) // n_t + start[d]
-        comps = H.components_from_sky(self.sky)
-        self.problems = []
-        for i in range(n_intervals):
-            if i < build_from:  # same random draws, no device problem
-                self.problems.append(None)
-                continue
-            geom = S.make_geometry(w, rng, start_step=i * n_t, station_enu=g0.station_enu)
-            dp = H.DeviceProblem(self.solver, n_t, geom.baselines[:, 0], geom.baselines[:, 1],
-                                 ch, self.sol_index)
-            dp.predict(geom.ref, geom.uvw, geom.freq, geom.chan_width, comps)
-            dp.synthesize_data(self.gains[i], self.sol_index, w.noise_sigma, seed=seed * 1000 + i)
-            self.problems.append(dp)
+        self._comps = H.components_from_sky(self.sky)
+        self.problems = _LazyProblems(self)
+
+    def build(self, i):
+        w = self.w
+        geom = S.make_geometry(w, None, start_step=i * w.solint, station_enu=self.geom0.station_enu)
+        dp = H.DeviceProblem(self.solver, w.solint, geom.baselines[:, 0], geom.baselines[:, 1],
+                             self._ch, self.sol_index)
+        dp.predict(geom.ref, geom.uvw, geom.freq, geom.chan_width, self._comps)
+        dp.synthesize_data(self.gains[i], self.sol_index, w.noise_sigma,
+                           seed=self.seed * 1000 + i)
+        return dp
+
+    def release(self, i):
+        self.problems.release(i)
 
     def unit(self):
         return np.ones((self.w.n_channel_blocks, self.w.n_stations * self.w.n_sub * 2),
                        np.complex128)
 
 
-def training_set(seq: IntervalSequence, ws: WarmStart, first, last, converge_iterations=600):
+class _LazyProblems:
+    def __init__(self, seq):
+        self._seq = seq
+        self._built = {}
+
+    def __len__(self):
+        return self._seq.n_intervals
+
+    def __getitem__(self, i):
+        if not 0 <= i < self._seq.n_intervals:
+            raise IndexError(i)
+        if i not in self._built:
+            self._built[i] = self._seq.build(i)
+        return self._built[i]
+
+    def release(self, i):
+        dp = self._built.pop(i, None)
+        if dp is not None:
+            dp.close()
+
+
+def training_set(seq: IntervalSequence, ws: WarmStart, first, last, converge_iterations=600,
+                 release=False):
     """Solves intervals [first, last) to convergence (tolerance 1e-9, at most
     `converge_iterations`) from the previous interval's solutions and collects
     (features, targets) for the learned start. Returns x, t and the last
-    converged solutions."""
+    converged solutions. With `release`, each interval's device problem is
+    freed once used."""
     long = sibling(seq.solver, converge_iterations, 1e-9, False)
     prev = None
     xs, ts = [], []
@@ -155,10 +186,14 @@ def training_set(seq: IntervalSequence, ws: WarmStart, first, last, converge_ite
             sol = seq.unit()
             long.solve_problem(dp, sol)
             prev = sol
+            if release:
+                seq.release(i)
             continue
         xs.append(ws.features(seq.solver, dp, prev))
         sol = prev.copy()
         long.solve_problem(dp, sol)
         ts.append(ws.targets(prev, sol))
         prev = sol
+        if release:
+            seq.release(i)
     return np.concatenate(xs), np.concatenate(ts), prev

@@ -213,6 +213,23 @@ int ddh_batch_rebind(ddh_batch* batch, ddh_problem* const* problems);
 // solutions[i][cb] -> interval i, channel block cb ([n_ant][n_sub][2] complex128).
 int ddh_batch_solve(ddh_batch* batch, double* const* const* solutions, ddh_result* results);
 int ddh_batch_last_time(const ddh_batch* batch, double* ms);
+// Learned warm start for every interval of the batch, all on the device:
+// solutions[i] hold the previous interval's solutions g0 on entry and the
+// final solutions on return. From g0 the batch runs one undamped update
+// (with its denominators) and two stretches of `stretch` damped iterations,
+// turns them into features, runs the network (ddh_mlp, 16 -> 4) and starts
+// the solve from g0 * exp(output). Same arithmetic as the per-interval
+// sequence ddh_one_step, 2 x ddh_solve_problem(max `stretch`, accuracy 0,
+// no stall detection), ddh_warmstart_features, ddh_mlp_forward and
+// ddh_warmstart_apply. The network must not be in use on another stream.
+// No counterpart in the reference (DDECal.cc:487-499 offers unit or
+// previous solutions only).
+typedef struct ddh_mlp ddh_mlp;
+int ddh_batch_solve_warm(ddh_batch* batch, ddh_mlp* net, int32_t stretch,
+                         double* const* const* solutions, ddh_result* results);
+// Device time of the warm-start phase of the last ddh_batch_solve_warm
+// (ddh_batch_last_time covers the solve from the learned start).
+int ddh_batch_last_warm_time(const ddh_batch* batch, double* ms);
 // HIP events around every batched correlate launch of the next solves (graph
 // replay off while enabled); ddh_batch_last_timing reports their summed
 // duration and the number of launches that did work.
@@ -231,7 +248,6 @@ int ddh_batch_autotune(ddh_batch* batch, int32_t* waves_per_tile, int32_t* rows_
 // perceptron with bf16 weights on the GPU (bf16 MFMA GEMMs, FP32
 // accumulation, FP32 master weights, Adam). Parameters are one flat FP32
 // array: W1 [n_in x h], b1 [h], W2 [h x h], b2 [h], W3 [h x n_out], b3 [n_out].
-typedef struct ddh_mlp ddh_mlp;
 ddh_mlp* ddh_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_out, uint32_t seed);
 void ddh_mlp_destroy(ddh_mlp* net);
 int64_t ddh_mlp_num_params(const ddh_mlp* net);

@@ -121,3 +121,40 @@ def test_warm_start_reduces_iterations():
     learned_total = np.mean(its["learned"]) + 2 * ws.stretch + 1
     print(f"learned incl. feature iterations {learned_total:.1f}")
     assert learned_total < np.mean(its["previous"]) < np.mean(its["unit"])
+
+
+def test_batched_warm_start_matches_host_sequence():
+    """ddh_batch_solve_warm (features, network and start for every interval on
+    the device) gives the same starts and solves as the per-interval host
+    sequence one_step -> 2 x stretch iterations -> features -> forward ->
+    apply, followed by a batched solve. Same schedule (4 waves per tile, one
+    row per trip) on both sides, so the arithmetic is the same."""
+    print("case: batched warm start vs host sequence", flush=True)
+    w = S.Workload("wsb", 12, 3, 2, 1, 12, 3, 4, 60, 1e-5, seed=9)
+    seqs = [WS.IntervalSequence(w, 2, seed=20 + k) for k in range(3)]
+    ws = WS.WarmStart(hidden=64, seed=1, stretch=3)
+    prevs = []
+    for q in seqs:
+        g = q.unit()
+        q.solver.solve_problem(q.problems[0], g)
+        prevs.append(g)
+    solvers = [q.solver for q in seqs]
+    dps = [q.problems[1] for q in seqs]
+    host = [ws.initial(q.solver, q.problems[1], prevs[k]) for k, q in enumerate(seqs)]
+    b1 = H.Batch(solvers, dps)
+    b1.set_schedule(4, 1, 1, 8)
+    r_host = b1.solve(host)
+    dev = [p.copy() for p in prevs]
+    b2 = H.Batch(solvers, dps)
+    b2.set_schedule(4, 1, 1, 8)
+    r_dev = b2.solve_warm(ws.net, dev, stretch=ws.stretch)
+    assert b2.last_warm_ms() > 0.0
+    for k in range(len(seqs)):
+        assert r_dev[k].iterations == r_host[k].iterations
+        np.testing.assert_allclose(dev[k], host[k], rtol=1e-9, atol=1e-12)
+    # a second call on the same batch (rebound to the same problems) repeats exactly
+    again = [p.copy() for p in prevs]
+    b2.rebind(dps)
+    b2.solve_warm(ws.net, again, stretch=ws.stretch)
+    for k in range(len(seqs)):
+        np.testing.assert_array_equal(again[k], dev[k])
