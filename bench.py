@@ -37,7 +37,10 @@ correlate over FP16 planes: algorithmic bytes per launch = streams x n_vis x
 (16 + n_dir x 8) (data complex64, model 2 x half2), over its mean duration
 from HIP events on the batch stream around each launch that did work.
 `roofline_plain` is the complex64 correlate of the plain form
-(n_vis x (16 + n_dir x 16) per launch). `cpu_baseline` times the CPU oracle
+(n_vis x (16 + n_dir x 16) per launch). Both are measured on 50-iteration
+solves at accuracy 0, so that every launch does the full work. `traffic` is
+read from the committed rocprofv3 --pmc summary of the same command, when
+present. `cpu_baseline` times the CPU oracle
 (a restatement of DP3's IterativeDiagonalSolver, threads over channel
 blocks) on a bounded sample of plain-form iterations and scales it to
 solves/s at the plain form's mean iteration count.
@@ -61,6 +64,22 @@ METRIC = "DDECal solves/s (62 st, 8 dir, 240 ch): plain vs batched+warm-start, s
 HBM_PEAK_GBS = 8000.0
 MAX_ITERATIONS = 1000
 TOLERANCE = 1e-6
+ROOF_ITERATIONS = 50
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "correlate_traffic.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (lines "<kernel> traffic_bytes_per_launch <N>"), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == kernel and parts[1] == "traffic_bytes_per_launch":
+                    return int(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def algorithmic_bytes(w, model_bytes):
@@ -153,11 +172,20 @@ def main():
                                                   accuracy=TOLERANCE))
            for b in range(B)]
 
+    # inputs of the evaluated intervals go resident first (contiguous device
+    # memory); training intervals then pass through one scratch problem each
+    for i in range(first, n_total):
+        for b in range(B):
+            _ = s16[b].problems[i]
+            _ = s64[b].problems[i]
+    log(f"built {2 * B * (n_total - first)} resident intervals "
+        f"({time.perf_counter() - t_setup:.1f} s)")
+
     # ---- learned warm start: trained on the solver's own converged outputs
     ws = WS.WarmStart(hidden=256, seed=args.seed, stretch=args.stretch)
     xs, ts, prevs = [], [], []
     for q in s16:
-        x, t, prev = WS.training_set(q, ws, 0, TR, converge_iterations=1000, release=True)
+        x, t, prev = WS.training_set(q, ws, 0, TR, converge_iterations=1000, transient=True)
         xs.append(x)
         ts.append(t)
         prevs.append(prev)
@@ -176,7 +204,6 @@ def main():
     chain = [p.copy() for p in prevs]
     for i in range(first, n_total):
         for b in range(B):
-            _ = s16[b].problems[i]
             g = chain[b].copy()
             long[b].solve_problem(s64[b].problems[i], g)
             conv[(b, i)] = H.chi2(s64[b].solver, s64[b].problems[i], g)
@@ -216,13 +243,16 @@ def main():
     v_nonet, _, _, chi_nonet, its_nonet = run_batched(False)
     log(f"optimized {value:.2f} solves/s, without network {v_nonet:.2f}")
 
-    # roofline of the batched FP16 correlate: one more, profiled, batch
-    batch.set_profiling(True)
-    batch.set_schedule(tuned[0], tuned[1], 0, 8)
-    batch.rebind([q.problems[n_total - 1] for q in s16])
-    batch.solve([q.unit() for q in s16])
-    corr_ms, corr_n = batch.last_timing()
-    batch.set_profiling(False)
+    # roofline of the batched FP16 correlate: a fixed-length profiled batch in
+    # which every interval is live in every launch (the last launches of this
+    # kernel in the run)
+    rb = H.Batch([WS.sibling(q.solver, ROOF_ITERATIONS, 0.0, False) for q in s16],
+                 [q.problems[n_total - 1] for q in s16])
+    rb.set_schedule(tuned[0], tuned[1], 0, 8)
+    rb.set_profiling(True)
+    rb.solve([q.unit() for q in s16])
+    corr_ms, corr_n = rb.last_timing()
+    rb.close()
     avg_ms = corr_ms / max(corr_n, 1)
     bytes_opt = B * algorithmic_bytes(w, 8)
     achieved = bytes_opt / (avg_ms * 1e-3) / 1e9
@@ -253,8 +283,8 @@ def main():
         results[name] = (B * T / t_all, chis, its)
         log(f"{name}: {results[name][0]:.2f} solves/s, mean iterations {np.mean(its):.1f}, "
             f"chi2 excess max {np.max(chis):.2e}")
-    # roofline of the complex64 correlate (plain form)
-    s = WS.sibling(plain, MAX_ITERATIONS, TOLERANCE)
+    # roofline of the complex64 correlate (plain form), fixed length as above
+    s = WS.sibling(plain, ROOF_ITERATIONS, 0.0, False)
     s.set_schedule(waves_per_tile=p_tuned[0], rows_per_trip=p_tuned[1], use_graph=0,
                    poll_interval=8)
     s.set_profiling(True)
@@ -314,7 +344,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic("batched_correlate_f16"),
             "kernel": "batched correlate, FP16 model planes",
             "algorithmic_bytes_per_launch": bytes_opt,
             "mean_launch_ms": round(avg_ms, 5),
@@ -325,6 +355,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(p_achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic("correlate_c64"),
             "kernel": "correlate, complex64 model planes",
             "algorithmic_bytes_per_launch": algorithmic_bytes(w, 16),
             "mean_launch_ms": round(p_avg, 5),

@@ -196,6 +196,10 @@ ddh_problem* CreateProblem(const ddh_solver& s, int n_times, int n_baselines,
   for (int cb = 0; cb < p->n_cb; ++cb) {
     DDH_REQUIRE(p->ch_per_cb[cb] >= 1, -1, "channel blocks must hold >= 1 channel");
     const size_t plane = p->Rows(cb) * p->bl_stride;
+    // Kernels address one plane with 32-bit byte offsets (with room for the
+    // rows a wave reads past a segment's end).
+    DDH_REQUIRE((plane + 16 * size_t(p->bl_stride)) * p->DataVisBytes() < (size_t(1) << 32), -4,
+                "a channel block plane must stay below 4 GiB");
     ddh::CbInfo info{};
     info.n_ch = p->ch_per_cb[cb];
     info.data_off = int64_t(data_off);
@@ -757,8 +761,6 @@ int ddh_set_schedule(ddh_solver* s, int32_t waves_per_tile, int32_t rows_per_tri
     DDH_REQUIRE(s, -1, "NULL solver");
     DDH_REQUIRE(waves_per_tile >= 1 && waves_per_tile <= 4, -1, "waves_per_tile must be 1..4");
     DDH_REQUIRE(rows_per_trip == 1 || rows_per_trip == 2, -1, "rows_per_trip must be 1 or 2");
-    DDH_REQUIRE(rows_per_trip == 1 || s->cfg.c.n_correlations == 2, -4,
-                "two rows per trip are built for Duo (2-correlation) data only");
     DDH_REQUIRE(poll_interval >= 1, -1, "poll_interval must be >= 1");
     s->unroll = rows_per_trip;
     s->waves = waves_per_tile;
@@ -786,8 +788,9 @@ struct ddh_batch {
   bool f16 = false;
   int waves = 2, unroll = 1, use_graph = 1, poll = 8;
   int limit = 0;  // largest iteration limit over the intervals
-  size_t correlate_lds = 0, solve_lds = 0;
+  size_t solve_lds = 0;
   std::vector<ddh::IterArgs> args;
+  std::vector<ddh::TileGroup> groups;  // tiles of one direction count each
   std::vector<ddh::BatchTile> tiles, items, pro_items;
   ddh::DeviceBuffer<ddh::IterArgs> d_args;
   ddh::DeviceBuffer<ddh::BatchTile> d_tiles, d_items, d_pro_items;
@@ -824,10 +827,10 @@ void BuildBatch(ddh_batch& b) {
   b.pro_items.clear();
   b.d_states.Resize(n);
   b.limit = 0;
-  b.correlate_lds = 0;
   b.solve_lds = 0;
   struct Costed {
     BatchTile t;
+    int n_dir;
     double cost;
   };
   std::vector<Costed> costed;
@@ -840,24 +843,31 @@ void BuildBatch(ddh_batch& b) {
     a.state = b.d_states.ptr + i;
     b.args.push_back(a);
     b.limit = std::max(b.limit, std::max(s.cfg.c.max_iterations, s.cfg.c.min_iterations));
-    b.correlate_lds = std::max(b.correlate_lds, CorrelateSharedBytes(p.n_dir, p.n_corr, b.waves));
     b.solve_lds = std::max(b.solve_lds, AntennaSolveSharedBytes(a, p.n_corr));
     const double vis_bytes = double(p.DataVisBytes() + p.n_dir * p.ModelVisBytes());
     for (int cb = 0; cb < p.n_cb; ++cb) {
       for (int seg = 0; seg < p.n_seg; ++seg) {
         const double rows = double(p.seg_t0[seg + 1] - p.seg_t0[seg]) * p.ch_per_cb[cb];
         for (int tile = 0; tile < p.bl_stride / kWave; ++tile)
-          costed.push_back({BatchTile{i, cb, seg, tile}, rows * vis_bytes});
+          costed.push_back({BatchTile{i, cb, seg, tile}, p.n_dir, rows * vis_bytes});
       }
       for (int d = 0; d < p.n_dir; ++d) b.items.push_back(BatchTile{i, cb, d, 0});
       b.pro_items.push_back(BatchTile{i, cb, 0, 0});
     }
   }
-  // Longest tiles first: workgroups are dispatched in order, so this is a
-  // greedy longest-processing-time schedule over the compute units.
-  std::stable_sort(costed.begin(), costed.end(),
-                   [](const Costed& x, const Costed& y) { return x.cost > y.cost; });
-  for (const Costed& c : costed) b.tiles.push_back(c.t);
+  // One correlate launch per direction count; inside it longest tiles first:
+  // workgroups are dispatched in order, so this is a greedy
+  // longest-processing-time schedule over the compute units.
+  std::stable_sort(costed.begin(), costed.end(), [](const Costed& x, const Costed& y) {
+    return x.n_dir != y.n_dir ? x.n_dir > y.n_dir : x.cost > y.cost;
+  });
+  b.groups.clear();
+  for (const Costed& c : costed) {
+    if (b.groups.empty() || b.groups.back().n_dir != c.n_dir)
+      b.groups.push_back(TileGroup{c.n_dir, int32_t(b.tiles.size()), 0});
+    ++b.groups.back().count;
+    b.tiles.push_back(c.t);
+  }
   b.d_args.Upload(b.args, b.stream);
   b.d_tiles.Upload(b.tiles, b.stream);
   b.d_items.Upload(b.items, b.stream);
@@ -867,9 +877,9 @@ void BuildBatch(ddh_batch& b) {
 }
 
 void LaunchBatchIteration(ddh_batch& b) {
-  LaunchBatch(b.d_args.ptr, b.d_tiles.ptr, int(b.tiles.size()), b.d_items.ptr,
-              int(b.items.size()), int(b.solvers.size()), b.correlate_lds, b.solve_lds, b.n_corr,
-              b.f16, b.waves, b.unroll, b.stream);
+  LaunchBatch(b.d_args.ptr, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), b.d_items.ptr,
+              int(b.items.size()), int(b.solvers.size()), b.solve_lds, b.n_corr, b.f16, b.waves,
+              b.unroll, b.stream);
 }
 
 void EnsureBatchGraph(ddh_batch& b) {
@@ -971,8 +981,8 @@ void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* cons
   auto run_aux = [&](double2* out) {
     LaunchBatchPrologue(b.d_args_aux.ptr, b.d_pro_items.ptr, n_pro, b.stream);
     for (int k = 0; k < stretch; ++k)
-      LaunchBatch(b.d_args_aux.ptr, b.d_tiles.ptr, int(b.tiles.size()), b.d_items.ptr,
-                  int(b.items.size()), n, b.correlate_lds, b.solve_lds, b.n_corr, b.f16, b.waves,
+      LaunchBatch(b.d_args_aux.ptr, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()),
+                  b.d_items.ptr, int(b.items.size()), n, b.solve_lds, b.n_corr, b.f16, b.waves,
                   b.unroll, b.stream);
     for (int i = 0; i < n; ++i)
       LaunchSnapshot(b.d_args_aux.ptr, i, int64_t(SolutionCount(b, i)), out + off[i], b.stream);
@@ -980,8 +990,8 @@ void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* cons
   // next1: iteration 0 reads sol0 and writes sol1; no decision is taken
   copy_into_sol0(b.warm_g0.ptr);
   LaunchBatchPrologue(b.d_args_step.ptr, b.d_pro_items.ptr, n_pro, b.stream);
-  LaunchBatch(b.d_args_step.ptr, b.d_tiles.ptr, int(b.tiles.size()), b.d_items.ptr,
-              int(b.items.size()), 0, b.correlate_lds, b.solve_lds, b.n_corr, b.f16, b.waves,
+  LaunchBatch(b.d_args_step.ptr, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()),
+              b.d_items.ptr, int(b.items.size()), 0, b.solve_lds, b.n_corr, b.f16, b.waves,
               b.unroll, b.stream);
   for (int i = 0; i < n; ++i)
     DDH_HIP_CHECK(hipMemcpyAsync(b.warm_next.ptr + off[i], b.args[i].sol1,
@@ -1038,8 +1048,9 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
         if (b.profile) {
           // correlate alone between the events, then the rest of the iteration
           DDH_HIP_CHECK(hipEventRecord(b.events[2 * it], b.stream));
-          LaunchBatch(b.d_args.ptr, b.d_tiles.ptr, int(b.tiles.size()), b.d_items.ptr, 0, 0,
-                      b.correlate_lds, b.solve_lds, b.n_corr, b.f16, b.waves, b.unroll, b.stream);
+          LaunchBatch(b.d_args.ptr, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()),
+                      b.d_items.ptr, 0, 0, b.solve_lds, b.n_corr, b.f16, b.waves, b.unroll,
+                      b.stream);
           DDH_HIP_CHECK(hipEventRecord(b.events[2 * it + 1], b.stream));
           LaunchBatchTail(b.d_args.ptr, b.d_items.ptr, int(b.items.size()), int(b.solvers.size()),
                           b.solve_lds, b.n_corr, b.f16, b.stream);
@@ -1116,9 +1127,9 @@ double TimeLaunches(hipStream_t stream, int reps, LaunchFn&& launch) {
 // Candidate correlate schedules: (waves per tile, rows per trip).
 std::vector<std::pair<int, int>> ScheduleCandidates(int n_corr) {
   std::vector<std::pair<int, int>> c;
+  (void)n_corr;
   for (int w : {1, 2, 3, 4})
-    for (int u : {1, 2})
-      if (u == 1 || n_corr == 2) c.push_back({w, u});
+    for (int u : {1, 2}) c.push_back({w, u});
   return c;
 }
 
@@ -1176,8 +1187,6 @@ int ddh_batch_set_schedule(ddh_batch* b, int32_t waves_per_tile, int32_t rows_pe
     DDH_REQUIRE(b, -1, "NULL batch");
     DDH_REQUIRE(waves_per_tile >= 1 && waves_per_tile <= 4, -1, "waves_per_tile must be 1..4");
     DDH_REQUIRE(rows_per_trip == 1 || rows_per_trip == 2, -1, "rows_per_trip must be 1 or 2");
-    DDH_REQUIRE(rows_per_trip == 1 || b->n_corr == 2, -4,
-                "two rows per trip are built for Duo (2-correlation) data only");
     DDH_REQUIRE(poll_interval >= 1, -1, "poll_interval must be >= 1");
     b->waves = waves_per_tile;
     b->unroll = rows_per_trip;
@@ -1282,8 +1291,8 @@ int ddh_batch_autotune(ddh_batch* b, int32_t* waves_out, int32_t* rows_out, doub
       BuildBatch(*b);
       const double ms = TimeLaunches(b->stream, 5, [&] {
         // correlate only: it does not change the solve state
-        LaunchBatch(b->d_args.ptr, b->d_tiles.ptr, int(b->tiles.size()), b->d_items.ptr, 0,
-                    0, b->correlate_lds, b->solve_lds, b->n_corr, b->f16, b->waves, b->unroll,
+        LaunchBatch(b->d_args.ptr, b->d_tiles.ptr, b->groups.data(), int(b->groups.size()),
+                    b->d_items.ptr, 0, 0, b->solve_lds, b->n_corr, b->f16, b->waves, b->unroll,
                     b->stream);
       });
       if (ms < best) {

@@ -107,7 +107,7 @@ struct BatchTile {
 };
 
 // Host-side launchers (defined in solver_kernels.hip).
-void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, int unroll,
+void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, int rows,
                      hipStream_t stream);
 void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, hipStream_t stream);
 void LaunchDecide(const IterArgs& a, hipStream_t stream);
@@ -124,9 +124,18 @@ size_t AntennaSolveSharedBytes(const IterArgs& a, int n_corr);
 void PrepareKernelAttributes();
 // One batched iteration: correlate over all tiles, antenna solve over all
 // items, then one decide workgroup per interval.
-void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, int n_tiles,
-                 const BatchTile* d_items, int n_items, int n_intervals, size_t correlate_lds,
-                 size_t solve_lds, int n_corr, bool f16, int waves, int unroll,
+// Tiles of one direction count: entries [first, first + count) of a tile list.
+struct TileGroup {
+  int32_t n_dir;
+  int32_t first;
+  int32_t count;
+};
+// One batched iteration: the correlate once per tile group, then the antenna
+// solve over n_items items and the decision for n_intervals intervals (zero
+// counts skip a kernel).
+void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
+                 int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
+                 size_t solve_lds, int n_corr, bool f16, int waves, int rows,
                  hipStream_t stream);
 // Current solutions (by iteration parity) of batched interval `interval`
 // into dst, `count` complex values.

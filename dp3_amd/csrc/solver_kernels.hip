@@ -69,25 +69,95 @@ __device__ __forceinline__ void LoadVis(const void* base, size_t idx, float2 (&v
   }
 }
 
+// One visibility of a plane as loaded: complex64 planes as float4 vectors,
+// FP16 planes as packed half2 words, converted to float when used (fewer
+// registers per row in flight).
+template <int NC, bool F16>
+struct RawVis {
+  float4_v v[NC / 2];
+};
+template <>
+struct RawVis<2, true> {
+  uint2_v v;
+};
+template <>
+struct RawVis<4, true> {
+  uint4_v v;
+};
+
+template <int NC, bool F16>
+constexpr uint32_t VisBytes() {
+  return F16 ? uint32_t(NC) * 4u : uint32_t(NC) * 8u;
+}
+
+// Plane reads go through buffer resources (base, size) in scalar registers:
+// one 32-bit offset register per lane serves every plane, and a read at or
+// past the size returns zeros, so rows beyond the end of a segment need no
+// branch. 0x00020000 is the gfx9 resource word 3 (32-bit float data format);
+// cache policy 2 marks the reads non-temporal.
+constexpr int kResourceWord3 = 0x00020000;
+constexpr int kNonTemporal = 2;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t PlaneResource(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), short(0), int(bytes),
+                                           kResourceWord3);
+}
+
+template <int NC, bool F16>
+__device__ __forceinline__ void LoadRaw(__amdgpu_buffer_rsrc_t rsrc, uint32_t byte_off,
+                                        RawVis<NC, F16>& r) {
+  if constexpr (!F16) {
+#pragma unroll
+    for (int h = 0; h < NC / 2; ++h)
+      r.v[h] = __builtin_bit_cast(
+          float4_v, __builtin_amdgcn_raw_buffer_load_b128(rsrc, byte_off + 16u * h, 0, kNonTemporal));
+  } else if constexpr (NC == 2) {
+    r.v = __builtin_bit_cast(uint2_v,
+                             __builtin_amdgcn_raw_buffer_load_b64(rsrc, byte_off, 0, kNonTemporal));
+  } else {
+    r.v = __builtin_bit_cast(uint4_v,
+                             __builtin_amdgcn_raw_buffer_load_b128(rsrc, byte_off, 0, kNonTemporal));
+  }
+}
+
+// Correlation k of a loaded visibility as float2.
+template <int NC, bool F16>
+__device__ __forceinline__ float2 RawGet(const RawVis<NC, F16>& r, int k) {
+  if constexpr (!F16) {
+    const float4_v x = r.v[k / 2];
+    return (k & 1) ? make_float2(x.z, x.w) : make_float2(x.x, x.y);
+  } else if constexpr (NC == 2) {
+    return HalfBitsToFloat2(k ? r.v.y : r.v.x);
+  } else {
+    return HalfBitsToFloat2(r.v[k]);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // correlate: grid (bl tiles, segments, channel blocks), block = waves x 64.
 // Lane l of every wave owns baseline tile*64+l; the waves of a block split the
 // (time, channel) rows of the segment and are combined through LDS in a fixed
-// order, so results do not depend on scheduling.
+// order, so results do not depend on scheduling. Each wave keeps ROWS rows of
+// loads in flight (a ring of row registers refilled as each row is used).
 // ---------------------------------------------------------------------------
-template <int ND, int NC, bool F16, int UNROLL>
+template <int ND, int NC, bool F16, int ROWS>
 __device__ __forceinline__ void CorrelateTile(const IterArgs& a, const int tile, const int seg,
                                               const int cb) {
+  constexpr uint32_t kDataBytes = VisBytes<NC, false>();
+  constexpr uint32_t kModelBytes = VisBytes<NC, F16>();
   const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kWave);
   const int n_waves = blockDim.x / kWave;
-  const CbInfo info = a.cbinfo[cb];
+  const CbInfo info = LoadConst(a.cbinfo + cb);
   const int n_ch = info.n_ch;
   const size_t plane = size_t(a.n_t) * n_ch * a.bl_stride;
   const int bl = tile * kWave + lane;
+  const char* dbase = reinterpret_cast<const char*>(a.data) + size_t(info.data_off) * kDataBytes;
+  const char* mbase = reinterpret_cast<const char*>(a.model) + size_t(info.model_off) * kModelBytes;
+  const size_t mplane = plane * kModelBytes;
 
   extern __shared__ float smem[];
-  float2* gg_s = reinterpret_cast<float2*>(smem);  // [ND][NC][64]
+  float2* gg_s = reinterpret_cast<float2*>(smem);  // [ND][64][NC]
   float* red_s = smem + 2 * ND * NC * kWave;       // [3*ND*NC][64]
 
   float ur[ND][NC], ui[ND][NC], sm[ND][NC];
@@ -100,34 +170,40 @@ __device__ __forceinline__ void CorrelateTile(const IterArgs& a, const int tile,
       sm[d][k] = 0.0f;
     }
 
-  const int r0 = a.seg_t0[seg] * n_ch;
-  const int r1 = a.seg_t0[seg + 1] * n_ch;
-  // Rows of the segment, strided by the number of waves. With UNROLL == 2 the
-  // loop is software pipelined: the next row's loads are in flight while the
-  // current row is computed. A row past the segment end reads as zeros, which
-  // adds exactly nothing.
-  auto load_row = [&](int row, float2 (&r)[NC], float2 (&m)[ND][NC]) {
-    if (row < r1) {
-      const size_t e = size_t(row) * a.bl_stride + bl;
-      LoadVis<NC, false>(a.data, info.data_off + e, r);
+  const int r0 = AsConst(a.seg_t0)[seg] * n_ch;
+  const int r1 = AsConst(a.seg_t0)[seg + 1] * n_ch;
+  // Resources end at the segment's last row: a row past it reads as zeros,
+  // which adds exactly nothing.
+  const uint32_t row_vis = uint32_t(r1) * uint32_t(a.bl_stride);
+  const __amdgpu_buffer_rsrc_t drsrc = PlaneResource(dbase, row_vis * kDataBytes);
+  __amdgpu_buffer_rsrc_t mrsrc[ND];
 #pragma unroll
-      for (int d = 0; d < ND; ++d) LoadVis<NC, F16>(a.model, info.model_off + d * plane + e, m[d]);
-    } else {
-#pragma unroll
-      for (int k = 0; k < NC; ++k) {
-        r[k] = make_float2(0.0f, 0.0f);
-#pragma unroll
-        for (int d = 0; d < ND; ++d) m[d][k] = make_float2(0.0f, 0.0f);
-      }
-    }
+  for (int d = 0; d < ND; ++d) mrsrc[d] = PlaneResource(mbase + d * mplane, row_vis * kModelBytes);
+  struct Row {
+    RawVis<NC, false> d;
+    RawVis<NC, F16> m[ND];
   };
-  auto accumulate = [&](float2 (&r)[NC], const float2 (&m)[ND][NC]) {
+  auto load_row = [&](int row, Row& r) {
+    const uint32_t e = uint32_t(row) * uint32_t(a.bl_stride) + uint32_t(bl);
+    LoadRaw<NC, false>(drsrc, e * kDataBytes, r.d);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) LoadRaw<NC, F16>(mrsrc[d], e * kModelBytes, r.m[d]);
+  };
+  auto accumulate = [&](const Row& row) {
+    float2 r[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) r[k] = RawGet<NC, false>(row.d, k);
+    // The gain products are read from LDS for every row: an index the
+    // compiler cannot see through keeps it from hoisting all ND * NC of them
+    // into registers for the whole loop, which would cost occupancy.
+    int gi = lane * NC;
+    asm volatile("" : "+v"(gi));
     // Full residual, directions subtracted in order (IterativeDiagonalSolver.cc:108-111).
 #pragma unroll
     for (int d = 0; d < ND; ++d)
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
-        const float2 c = cmul(gg_s[(d * NC + k) * kWave + lane], m[d][k]);
+        const float2 c = cmul(gg_s[d * kWave * NC + gi + k], RawGet<NC, F16>(row.m[d], k));
         r[k].x -= c.x;
         r[k].y -= c.y;
       }
@@ -135,7 +211,7 @@ __device__ __forceinline__ void CorrelateTile(const IterArgs& a, const int tile,
     for (int d = 0; d < ND; ++d)
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
-        const float2 mm = m[d][k];
+        const float2 mm = RawGet<NC, F16>(row.m[d], k);
         ur[d][k] += r[k].x * mm.x + r[k].y * mm.y;
         ui[d][k] += r[k].y * mm.x - r[k].x * mm.y;
         sm[d][k] += mm.x * mm.x + mm.y * mm.y;
@@ -143,9 +219,9 @@ __device__ __forceinline__ void CorrelateTile(const IterArgs& a, const int tile,
   };
   // The first rows' loads do not depend on the gains: issue them before the
   // staging so that its memory round trips overlap with them.
-  float2 ra[NC], rb[NC];
-  float2 ma[ND][NC], mb[ND][NC];
-  load_row(r0 + wave, ra, ma);
+  Row ring[ROWS];
+#pragma unroll
+  for (int u = 0; u < ROWS; ++u) load_row(r0 + wave + u * n_waves, ring[u]);
   // Stage G_a,i * conj(G_b,j) of every (direction, correlation) for the 64
   // baselines of the tile (solutions repaired, then cast to float, as
   // SolveData.h:230-235). Wave 0 does it, lane = baseline; the loads of all
@@ -153,60 +229,65 @@ __device__ __forceinline__ void CorrelateTile(const IterArgs& a, const int tile,
   // two memory round trips instead of one per loop trip.
   if (wave == 0) {
     const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
-    const double2* sol = (a.state->iteration & 1) ? a.sol1 : a.sol0;
-    const double mean = a.mean[cb];
+    const double2* sol = (AsConst(a.state)->iteration & 1) ? a.sol1 : a.sol0;
+    const double mean = AsConst(a.mean)[cb];
     const bool valid = bl < a.n_bl;
-    const int a1 = valid ? a.ant1[bl] : 0;
-    const int a2 = valid ? a.ant2[bl] : 0;
-    double2 g1[ND][2], g2[ND][2];
+    const int a1 = valid ? AsGlobal(a.ant1)[bl] : 0;
+    const int a2 = valid ? AsGlobal(a.ant2)[bl] : 0;
+    // Directions in groups: each group's loads are issued together (one
+    // memory round trip per group) and a compiler barrier between groups
+    // bounds the registers they occupy.
+    constexpr int kGroup = F16 ? 4 : 2;
 #pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      const int s = a.seg_sol[seg * ND + d];
-      const size_t p1 = gbase + size_t(a1) * a.n_sub + s;
-      const size_t p2 = gbase + size_t(a2) * a.n_sub + s;
-      g1[d][0] = sol[p1 * 2];
-      g1[d][1] = sol[p1 * 2 + 1];
-      g2[d][0] = sol[p2 * 2];
-      g2[d][1] = sol[p2 * 2 + 1];
-    }
+    for (int d0 = 0; d0 < ND; d0 += kGroup) {
+      double2 g1[kGroup][2], g2[kGroup][2];
 #pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      // MakeSolutionsFinite2Pol: a non-finite pair reads as (mean, mean).
-      if (!(cfinite(g1[d][0]) && cfinite(g1[d][1])))
-        g1[d][0] = g1[d][1] = make_double2(mean, 0.0);
-      if (!(cfinite(g2[d][0]) && cfinite(g2[d][1])))
-        g2[d][0] = g2[d][1] = make_double2(mean, 0.0);
-      float sc = 1.0f;
-      if constexpr (F16) sc = a.model_scale[cb * ND + d];
-#pragma unroll
-      for (int k = 0; k < NC; ++k) {
-        const double2 ga = g1[d][PolI<NC>(k)];
-        const double2 gb = g2[d][PolJ<NC>(k)];
-        float2 g = cmulc(make_float2(float(ga.x), float(ga.y)),
-                         make_float2(float(gb.x), float(gb.y)));
-        g.x *= sc;
-        g.y *= sc;
-        gg_s[(d * NC + k) * kWave + lane] = valid ? g : make_float2(0.0f, 0.0f);
+      for (int dd = 0; dd < kGroup; ++dd) {
+        if (d0 + dd >= ND) break;
+        const int s = AsConst(a.seg_sol)[seg * ND + d0 + dd];
+        const size_t p1 = gbase + size_t(a1) * a.n_sub + s;
+        const size_t p2 = gbase + size_t(a2) * a.n_sub + s;
+        g1[dd][0] = LoadGlobal(sol + p1 * 2);
+        g1[dd][1] = LoadGlobal(sol + p1 * 2 + 1);
+        g2[dd][0] = LoadGlobal(sol + p2 * 2);
+        g2[dd][1] = LoadGlobal(sol + p2 * 2 + 1);
       }
+#pragma unroll
+      for (int dd = 0; dd < kGroup; ++dd) {
+        if (d0 + dd >= ND) break;
+        const int d = d0 + dd;
+        // MakeSolutionsFinite2Pol: a non-finite pair reads as (mean, mean).
+        if (!(cfinite(g1[dd][0]) && cfinite(g1[dd][1])))
+          g1[dd][0] = g1[dd][1] = make_double2(mean, 0.0);
+        if (!(cfinite(g2[dd][0]) && cfinite(g2[dd][1])))
+          g2[dd][0] = g2[dd][1] = make_double2(mean, 0.0);
+        float sc = 1.0f;
+        if constexpr (F16) sc = AsConst(a.model_scale)[cb * ND + d];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+          const double2 ga = g1[dd][PolI<NC>(k)];
+          const double2 gb = g2[dd][PolJ<NC>(k)];
+          float2 g = cmulc(make_float2(float(ga.x), float(ga.y)),
+                           make_float2(float(gb.x), float(gb.y)));
+          g.x *= sc;
+          g.y *= sc;
+          gg_s[(d * kWave + lane) * NC + k] = valid ? g : make_float2(0.0f, 0.0f);
+        }
+      }
+      asm volatile("" ::: "memory");
     }
   }
   __syncthreads();
 
-  if constexpr (UNROLL == 1) {
-    accumulate(ra, ma);
-    for (int row = r0 + wave + n_waves; row < r1; row += n_waves) {
-      float2 r[NC];
-      float2 m[ND][NC];
-      load_row(row, r, m);
-      accumulate(r, m);
-    }
-  } else {
-    int row = r0 + wave;
-    for (; row < r1; row += 2 * n_waves) {
-      load_row(row + n_waves, rb, mb);
-      accumulate(ra, ma);
-      load_row(row + 2 * n_waves, ra, ma);
-      accumulate(rb, mb);
+  // The scheduling barrier keeps each refill behind the use of its slot, so a
+  // wave holds exactly ROWS rows of registers (hoisting the refills above the
+  // arithmetic would need fresh registers for every slot).
+  for (int row = r0 + wave; row < r1; row += ROWS * n_waves) {
+#pragma unroll
+    for (int u = 0; u < ROWS; ++u) {
+      accumulate(ring[u]);
+      __builtin_amdgcn_sched_barrier(0);
+      load_row(row + (u + ROWS) * n_waves, ring[u]);
     }
   }
 
@@ -238,7 +319,8 @@ __device__ __forceinline__ void CorrelateTile(const IterArgs& a, const int tile,
     __syncthreads();
   }
   if (wave == 0) {
-    float* out = a.partials + (size_t(cb) * a.n_seg + seg) * ND * NC * 3 * a.bl_stride;
+    GlobalT<float>* out =
+        AsGlobal(a.partials) + (size_t(cb) * a.n_seg + seg) * ND * NC * 3 * a.bl_stride;
 #pragma unroll
     for (int d = 0; d < ND; ++d)
 #pragma unroll
@@ -251,10 +333,11 @@ __device__ __forceinline__ void CorrelateTile(const IterArgs& a, const int tile,
   }
 }
 
-template <int ND, int NC, bool F16, int UNROLL>
-__global__ void __launch_bounds__(kCorrelateMaxThreads) CorrelateKernel(const IterArgs a) {
+template <int ND, int NC, bool F16, int ROWS>
+__global__ void __launch_bounds__(kCorrelateMaxThreads)
+    CorrelateKernel(const IterArgs a) {
   if (a.state->done) return;
-  CorrelateTile<ND, NC, F16, UNROLL>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+  CorrelateTile<ND, NC, F16, ROWS>(a, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Solutions of MakeSolutionsFinite2Pol (SolverBase.cc:342-371), applied on
@@ -683,27 +766,17 @@ __global__ void SumPartialsKernel(const double* partial, int n, double* out) {
 // ---------------------------------------------------------------------------
 // Batched kernels: many solution intervals (each with its own IterArgs,
 // shapes and device state) in one launch per kernel. A tile / item list maps
-// blockIdx.x to (interval, block, segment, tile); the direction count is a
-// per-interval switch over the compiled variants.
+// blockIdx.x to (interval, block, segment, tile). The correlate is compiled
+// per direction count (its registers scale with it), so a batch launches it
+// once per direction count present, over that count's tiles.
 // ---------------------------------------------------------------------------
-template <int NC, bool F16, int UNROLL>
+template <int ND, int NC, bool F16, int ROWS>
 __global__ void __launch_bounds__(kCorrelateMaxThreads)
     BatchCorrelateKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ tiles) {
-  const BatchTile t = tiles[blockIdx.x];
-  const IterArgs& a = args[t.interval];
-  if (a.state->done) return;
-  switch (a.n_dir) {
-#define DDH_CASE(N) \
-  case N:           \
-    CorrelateTile<N, NC, F16, UNROLL>(a, t.tile, t.seg, t.cb); \
-    break;
-    DDH_CASE(1) DDH_CASE(2) DDH_CASE(3) DDH_CASE(4) DDH_CASE(5) DDH_CASE(6) DDH_CASE(7)
-    DDH_CASE(8) DDH_CASE(9) DDH_CASE(10) DDH_CASE(11) DDH_CASE(12) DDH_CASE(13)
-    DDH_CASE(14) DDH_CASE(15) DDH_CASE(16)
-#undef DDH_CASE
-    default:
-      break;
-  }
+  const BatchTile t = LoadConst(tiles + blockIdx.x);
+  const IterArgs a = LoadConst(args + t.interval);
+  if (AsConst(a.state)->done) return;
+  CorrelateTile<ND, NC, F16, ROWS>(a, t.tile, t.seg, t.cb);
 }
 
 template <int NC, bool F16>
@@ -810,29 +883,46 @@ __global__ void ScaleFromMaxKernel(const uint32_t* max_bits, float* scale, int n
 }
 
 using KernelFn = void (*)(const IterArgs);
+using BatchKernelFn = void (*)(const IterArgs*, const BatchTile*);
+constexpr int kMaxRows = 2;
 
-template <int NC, bool F16, int UNROLL, int... ND>
+template <int NC, bool F16, int ROWS, int... ND>
 constexpr auto MakeCorrelateTable(std::integer_sequence<int, ND...>) {
-  return std::array<KernelFn, sizeof...(ND)>{&CorrelateKernel<ND + 1, NC, F16, UNROLL>...};
+  return std::array<KernelFn, sizeof...(ND)>{&CorrelateKernel<ND + 1, NC, F16, ROWS>...};
+}
+template <int NC, bool F16, int ROWS, int... ND>
+constexpr auto MakeBatchCorrelateTable(std::integer_sequence<int, ND...>) {
+  return std::array<BatchKernelFn, sizeof...(ND)>{&BatchCorrelateKernel<ND + 1, NC, F16, ROWS>...};
 }
 
 using Dirs = std::make_integer_sequence<int, kMaxDirections>;
-const auto kDuoC64 = MakeCorrelateTable<2, false, 1>(Dirs{});
-const auto kDuoF16 = MakeCorrelateTable<2, true, 1>(Dirs{});
-const auto kFullC64 = MakeCorrelateTable<4, false, 1>(Dirs{});
-const auto kFullF16 = MakeCorrelateTable<4, true, 1>(Dirs{});
-const auto kDuoC64U2 = MakeCorrelateTable<2, false, 2>(Dirs{});
-const auto kDuoF16U2 = MakeCorrelateTable<2, true, 2>(Dirs{});
+// [n_corr == 4][f16][rows - 1][n_dir - 1]
+const std::array<KernelFn, kMaxDirections> kCorrelate[2][2][kMaxRows] = {
+    {{MakeCorrelateTable<2, false, 1>(Dirs{}), MakeCorrelateTable<2, false, 2>(Dirs{})},
+     {MakeCorrelateTable<2, true, 1>(Dirs{}), MakeCorrelateTable<2, true, 2>(Dirs{})}},
+    {{MakeCorrelateTable<4, false, 1>(Dirs{}), MakeCorrelateTable<4, false, 2>(Dirs{})},
+     {MakeCorrelateTable<4, true, 1>(Dirs{}), MakeCorrelateTable<4, true, 2>(Dirs{})}}};
+const std::array<BatchKernelFn, kMaxDirections> kBatchCorrelate[2][2][kMaxRows] = {
+    {{MakeBatchCorrelateTable<2, false, 1>(Dirs{}), MakeBatchCorrelateTable<2, false, 2>(Dirs{})},
+     {MakeBatchCorrelateTable<2, true, 1>(Dirs{}), MakeBatchCorrelateTable<2, true, 2>(Dirs{})}},
+    {{MakeBatchCorrelateTable<4, false, 1>(Dirs{}), MakeBatchCorrelateTable<4, false, 2>(Dirs{})},
+     {MakeBatchCorrelateTable<4, true, 1>(Dirs{}), MakeBatchCorrelateTable<4, true, 2>(Dirs{})}}};
 
-KernelFn CorrelateFor(int n_dir, int n_corr, bool f16, int unroll) {
+void CheckCorrelateVariant(int n_dir, int n_corr, int rows) {
   DDH_REQUIRE(n_dir >= 1 && n_dir <= kMaxDirections, -4,
               "number of directions must be 1.." + std::to_string(kMaxDirections));
-  if (n_corr == 2) {
-    if (unroll == 2) return f16 ? kDuoF16U2[n_dir - 1] : kDuoC64U2[n_dir - 1];
-    return f16 ? kDuoF16[n_dir - 1] : kDuoC64[n_dir - 1];
-  }
-  if (n_corr == 4) return f16 ? kFullF16[n_dir - 1] : kFullC64[n_dir - 1];
-  throw Error(-1, "n_correlations must be 2 or 4");
+  DDH_REQUIRE(n_corr == 2 || n_corr == 4, -1, "n_correlations must be 2 or 4");
+  DDH_REQUIRE(rows >= 1 && rows <= kMaxRows, -1, "rows per trip must be 1 or 2");
+}
+
+KernelFn CorrelateFor(int n_dir, int n_corr, bool f16, int rows) {
+  CheckCorrelateVariant(n_dir, n_corr, rows);
+  return kCorrelate[n_corr == 4][f16][rows - 1][n_dir - 1];
+}
+
+BatchKernelFn BatchCorrelateFor(int n_dir, int n_corr, bool f16, int rows) {
+  CheckCorrelateVariant(n_dir, n_corr, rows);
+  return kBatchCorrelate[n_corr == 4][f16][rows - 1][n_dir - 1];
 }
 
 }  // namespace
@@ -859,10 +949,16 @@ void PrepareKernelAttributes() {
   DDH_HIP_CHECK(hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   g_max_lds = std::min(limit, 160 * 1024);
   const int max_lds = g_max_lds;
-  for (const auto* table : {&kDuoC64, &kDuoF16, &kFullC64, &kFullF16, &kDuoC64U2, &kDuoF16U2})
-    for (KernelFn f : *table)
-      DDH_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(f),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  for (int c = 0; c < 2; ++c)
+    for (int h = 0; h < 2; ++h)
+      for (int r = 0; r < kMaxRows; ++r)
+        for (int d = 0; d < kMaxDirections; ++d) {
+          DDH_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kCorrelate[c][h][r][d]),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+          DDH_HIP_CHECK(hipFuncSetAttribute(
+              reinterpret_cast<const void*>(kBatchCorrelate[c][h][r][d]),
+              hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+        }
   // antenna_solve also holds static LDS for its reductions
   for (const void* f : {reinterpret_cast<const void*>(&AntennaSolveKernel<2, false>),
                         reinterpret_cast<const void*>(&AntennaSolveKernel<2, true>),
@@ -874,15 +970,10 @@ void PrepareKernelAttributes() {
     g_max_lds_solve = std::min(g_max_lds_solve, dyn);
     DDH_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
   }
-  for (const void* f : {reinterpret_cast<const void*>(&BatchCorrelateKernel<2, false, 1>),
-                        reinterpret_cast<const void*>(&BatchCorrelateKernel<2, false, 2>),
-                        reinterpret_cast<const void*>(&BatchCorrelateKernel<2, true, 1>),
-                        reinterpret_cast<const void*>(&BatchCorrelateKernel<2, true, 2>),
-                        reinterpret_cast<const void*>(&BatchCorrelateKernel<4, false, 1>)})
-    DDH_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
   for (const void* f : {reinterpret_cast<const void*>(&BatchAntennaSolveKernel<2, false>),
                         reinterpret_cast<const void*>(&BatchAntennaSolveKernel<2, true>),
-                        reinterpret_cast<const void*>(&BatchAntennaSolveKernel<4, false>)}) {
+                        reinterpret_cast<const void*>(&BatchAntennaSolveKernel<4, false>),
+                        reinterpret_cast<const void*>(&BatchAntennaSolveKernel<4, true>)}) {
     hipFuncAttributes attr{};
     DDH_HIP_CHECK(hipFuncGetAttributes(&attr, f));
     const int dyn = max_lds - int(attr.sharedSizeBytes);
@@ -892,50 +983,28 @@ void PrepareKernelAttributes() {
   done = true;
 }
 
-void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, int n_tiles,
-                 const BatchTile* d_items, int n_items, int n_intervals, size_t correlate_lds,
-                 size_t solve_lds, int n_corr, bool f16, int waves, int unroll,
+void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
+                 int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
+                 size_t solve_lds, int n_corr, bool f16, int waves, int rows,
                  hipStream_t stream) {
-  DDH_REQUIRE(correlate_lds <= MaxSharedBytes(), -4, "batch correlate LDS exceeds the device");
-  DDH_REQUIRE(solve_lds <= size_t(g_max_lds_solve), -4, "batch antenna solve LDS exceeds the device");
-  DDH_REQUIRE(n_corr == 2 || (!f16 && unroll == 1), -4,
-              "batched Full (4-correlation) data runs complex64 with one row per trip");
-  if (n_tiles == 0) {
-    LaunchBatchTail(d_args, d_items, n_items, n_intervals, solve_lds, n_corr, f16, stream);
-    return;
-  }
-  const dim3 cgrid(n_tiles), cblock(waves * kWave);
-  if (n_corr == 2) {
-    if (f16) {
-      if (unroll == 2)
-        hipLaunchKernelGGL((BatchCorrelateKernel<2, true, 2>), cgrid, cblock, correlate_lds, stream, d_args, d_tiles);
-      else
-        hipLaunchKernelGGL((BatchCorrelateKernel<2, true, 1>), cgrid, cblock, correlate_lds, stream, d_args, d_tiles);
-      if (n_items > 0)
-        hipLaunchKernelGGL((BatchAntennaSolveKernel<2, true>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
-    } else {
-      if (unroll == 2)
-        hipLaunchKernelGGL((BatchCorrelateKernel<2, false, 2>), cgrid, cblock, correlate_lds, stream, d_args, d_tiles);
-      else
-        hipLaunchKernelGGL((BatchCorrelateKernel<2, false, 1>), cgrid, cblock, correlate_lds, stream, d_args, d_tiles);
-      if (n_items > 0)
-        hipLaunchKernelGGL((BatchAntennaSolveKernel<2, false>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
-    }
-  } else {
-    hipLaunchKernelGGL((BatchCorrelateKernel<4, false, 1>), cgrid, cblock, correlate_lds, stream, d_args, d_tiles);
-    if (n_items > 0)
-      hipLaunchKernelGGL((BatchAntennaSolveKernel<4, false>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
+  DDH_REQUIRE(waves >= 1 && waves * kWave <= kCorrelateMaxThreads, -1, "bad waves per tile");
+  for (int g = 0; g < n_groups; ++g) {
+    const TileGroup& grp = groups[g];
+    if (grp.count == 0) continue;
+    const size_t lds = CorrelateSharedBytes(grp.n_dir, n_corr, waves);
+    DDH_REQUIRE(lds <= MaxSharedBytes(), -4, "batch correlate LDS exceeds the device");
+    hipLaunchKernelGGL(BatchCorrelateFor(grp.n_dir, n_corr, f16, rows), dim3(grp.count),
+                       dim3(waves * kWave), lds, stream, d_args, d_tiles + grp.first);
   }
   DDH_HIP_CHECK(hipGetLastError());
-  if (n_intervals > 0)
-    hipLaunchKernelGGL(BatchDecideKernel, dim3(n_intervals), dim3(256), 0, stream, d_args);
-  DDH_HIP_CHECK(hipGetLastError());
+  LaunchBatchTail(d_args, d_items, n_items, n_intervals, solve_lds, n_corr, f16, stream);
 }
 
 void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_items,
                      int n_intervals, size_t solve_lds, int n_corr, bool f16,
                      hipStream_t stream) {
-  // antenna solve + decide only (n_tiles = 0 skips the correlate)
+  // antenna solve + decide only
+  DDH_REQUIRE(solve_lds <= size_t(g_max_lds_solve), -4, "batch antenna solve LDS exceeds the device");
   if (n_items > 0) {
     if (n_corr == 2) {
       if (f16)
@@ -943,7 +1012,10 @@ void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_ite
       else
         hipLaunchKernelGGL((BatchAntennaSolveKernel<2, false>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
     } else {
-      hipLaunchKernelGGL((BatchAntennaSolveKernel<4, false>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
+      if (f16)
+        hipLaunchKernelGGL((BatchAntennaSolveKernel<4, true>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
+      else
+        hipLaunchKernelGGL((BatchAntennaSolveKernel<4, false>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
     }
   }
   if (n_intervals > 0)
@@ -966,10 +1038,10 @@ void LaunchBatchPrologue(const IterArgs* d_args, const BatchTile* d_items, int n
   DDH_HIP_CHECK(hipGetLastError());
 }
 
-void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, int unroll,
+void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, int rows,
                      hipStream_t stream) {
   DDH_REQUIRE(waves >= 1 && waves * kWave <= kCorrelateMaxThreads, -1, "bad waves per tile");
-  const KernelFn f = CorrelateFor(a.n_dir, n_corr, f16, unroll);
+  const KernelFn f = CorrelateFor(a.n_dir, n_corr, f16, rows);
   const dim3 grid(a.bl_stride / kWave, a.n_seg, a.n_cb);
   const size_t lds = CorrelateSharedBytes(a.n_dir, n_corr, waves);
   DDH_REQUIRE(lds <= MaxSharedBytes(), -4, "correlate tile needs more LDS than the device has");

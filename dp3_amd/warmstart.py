@@ -131,11 +131,15 @@ class IntervalSequence:
         self._comps = H.components_from_sky(self.sky)
         self.problems = _LazyProblems(self)
 
-    def build(self, i):
+    def build(self, i, into=None):
+        """Interval i as a device problem; `into` (a problem of this sequence)
+        is refilled instead of allocating a new one."""
         w = self.w
         geom = S.make_geometry(w, None, start_step=i * w.solint, station_enu=self.geom0.station_enu)
-        dp = H.DeviceProblem(self.solver, w.solint, geom.baselines[:, 0], geom.baselines[:, 1],
-                             self._ch, self.sol_index)
+        dp = into
+        if dp is None:
+            dp = H.DeviceProblem(self.solver, w.solint, geom.baselines[:, 0], geom.baselines[:, 1],
+                                 self._ch, self.sol_index)
         dp.predict(geom.ref, geom.uvw, geom.freq, geom.chan_width, self._comps)
         dp.synthesize_data(self.gains[i], self.sol_index, w.noise_sigma,
                            seed=self.seed * 1000 + i)
@@ -143,6 +147,12 @@ class IntervalSequence:
 
     def release(self, i):
         self.problems.release(i)
+
+    def transient(self, i):
+        """Interval i in one scratch problem reused by every call (no device
+        allocation after the first), valid until the next call."""
+        self._scratch = self.build(i, into=getattr(self, "_scratch", None))
+        return self._scratch
 
     def unit(self):
         return np.ones((self.w.n_channel_blocks, self.w.n_stations * self.w.n_sub * 2),
@@ -171,29 +181,25 @@ class _LazyProblems:
 
 
 def training_set(seq: IntervalSequence, ws: WarmStart, first, last, converge_iterations=600,
-                 release=False):
+                 transient=False):
     """Solves intervals [first, last) to convergence (tolerance 1e-9, at most
     `converge_iterations`) from the previous interval's solutions and collects
     (features, targets) for the learned start. Returns x, t and the last
-    converged solutions. With `release`, each interval's device problem is
-    freed once used."""
+    converged solutions. With `transient`, the intervals are built one after
+    another into one scratch problem instead of staying resident."""
     long = sibling(seq.solver, converge_iterations, 1e-9, False)
     prev = None
     xs, ts = [], []
     for i in range(first, last):
-        dp = seq.problems[i]
+        dp = seq.transient(i) if transient else seq.problems[i]
         if prev is None:
             sol = seq.unit()
             long.solve_problem(dp, sol)
             prev = sol
-            if release:
-                seq.release(i)
             continue
         xs.append(ws.features(seq.solver, dp, prev))
         sol = prev.copy()
         long.solve_problem(dp, sol)
         ts.append(ws.targets(prev, sol))
         prev = sol
-        if release:
-            seq.release(i)
     return np.concatenate(xs), np.concatenate(ts), prev
