@@ -1,0 +1,123 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures of tests/golden/ with the CPU oracle (oracle/,
+a restatement of DP3's IterativeDiagonalSolver, SolveData packing and
+Simulator predict; the reference itself cannot be built here).
+
+  golden_small_duo.npz / golden_small_full.npz
+      A reduced config[0]-style interval (8 stations, 2 directions, one point
+      and one Gaussian component each, 8 channels in 2 blocks, solution
+      interval 2, step 0.2, at most 200 iterations, tolerance 1e-5, seed 0;
+      it converges in 170):
+      SolveData arrays, true gains, and the oracle's final gains (reference
+      precision mix, one thread; and FP64), iteration counts and chi^2.
+  golden_solvertester.npz
+      SolverTester's dd-interval case (tSolvers.cc:280-310, solutions per
+      direction 1, 2, 5): the oracle's final gains and iterations. The input
+      is regenerated bit-exactly by the oracle's mt19937 fixture.
+  golden_predict.npz
+      Station UVW, channels and components (points, Gaussians with relative
+      and absolute position angle, spectral index, rotation measure, full
+      Stokes) with the oracle's predicted visibilities.
+
+Run from the repository root: python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from dp3_amd import synth as S  # noqa: E402
+from oracle import pyoracle as O  # noqa: E402
+from problems import small_problem, solvertester_problem  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+SMALL = S.Workload("golden8", 8, 2, 1, 1, 8, 2, 2, 200, 1e-5, seed=0)
+
+
+def problem_arrays(P):
+    return dict(n_antennas=P.n_antennas, n_directions=P.n_directions,
+                n_channel_blocks=P.n_channel_blocks, n_correlations=P.n_correlations,
+                n_solutions_per_direction=np.asarray(P.n_solutions_per_direction, np.int32),
+                vis_offsets=P.vis_offsets, data=P.data, model=P.model, ant1=P.ant1, ant2=P.ant2,
+                sol_map=P.sol_map, n_times=P.n_times, n_baselines=P.n_baselines,
+                channel_begin=P.channel_begin)
+
+
+def small(n_corr):
+    w, geom, sky, gains, P = small_problem(SMALL, n_corr=n_corr)
+    out = problem_arrays(P)
+    out["true_gains"] = gains
+    for tag, fp64 in (("", False), ("_fp64", True)):
+        sol = P.unit_solutions()
+        r = O.solve(P, sol, max_iterations=SMALL.max_iterations, accuracy=SMALL.tolerance,
+                    fp64=fp64, n_threads=1)
+        out["final" + tag] = sol
+        out["iterations" + tag] = r.iterations
+        out["chi2" + tag] = O.chi2(P, sol)
+    out["max_iterations"] = SMALL.max_iterations
+    out["accuracy"] = SMALL.tolerance
+    out["step_size"] = SMALL.step_size
+    name = "golden_small_duo.npz" if n_corr == 2 else "golden_small_full.npz"
+    np.savez_compressed(os.path.join(OUT, name), **out)
+    print(f"{name}: iterations {out['iterations']}, chi2 {out['chi2']:.6g}")
+
+
+def solvertester():
+    P, inp = solvertester_problem((1, 2, 5), n_corr=2)
+    sol = P.unit_solutions()
+    r = O.solve(P, sol, max_iterations=100, accuracy=1e-8, step_size=0.2, n_threads=1)
+    np.savez_compressed(os.path.join(OUT, "golden_solvertester.npz"), final=sol,
+                        iterations=r.iterations, input_solutions=inp,
+                        n_solutions_per_direction=np.array([1, 2, 5], np.int32),
+                        max_iterations=100, accuracy=1e-8, step_size=0.2)
+    print(f"golden_solvertester.npz: iterations {r.iterations}")
+
+
+def predict():
+    rng = np.random.default_rng(7)
+    w = S.Workload("golden-predict", 6, 1, 1, 0, 5, 1, 2, 1, 1e-5, seed=7)
+    geom = S.make_geometry(w, rng)
+    ra0, dec0 = geom.ref
+    specs = [
+        dict(ra=ra0 + 0.01, dec=dec0 - 0.005, I=1.5),
+        dict(ra=ra0 - 0.02, dec=dec0 + 0.01, I=2.0, Q=0.3, U=-0.2, V=0.1,
+             spectral_terms=[-0.7, 0.1], ref_freq=130e6, log_si=True),
+        dict(ra=ra0 + 0.015, dec=dec0 + 0.02, I=0.8, Q=0.4, U=0.1, rm=(0.2, 12.0, 0.5),
+             ref_freq=135e6),
+        dict(ra=ra0 - 0.005, dec=dec0 - 0.015, I=1.2, gaussian=True, major=40 / 206265.0,
+             minor=15 / 206265.0, pa=0.6),
+        dict(ra=ra0 + 0.03, dec=dec0 - 0.02, I=0.9, gaussian=True, major=25 / 206265.0,
+             minor=10 / 206265.0, pa=1.1, pa_absolute=True),
+    ]
+    comps = [O.make_component(**c) for c in specs]
+    vis = np.stack([O.simulate(geom.ref, geom.uvw[t], geom.baselines, geom.freq,
+                               geom.chan_width, comps, stokes_i_only=False,
+                               correct_freq_smearing=True)
+                    for t in range(geom.uvw.shape[0])])
+    arrays = {k: np.array([c.get(k, d) for c in specs], np.float64)
+              for k, d in (("ra", 0.0), ("dec", 0.0), ("I", 0.0), ("Q", 0.0), ("U", 0.0),
+                           ("V", 0.0), ("major", 0.0), ("minor", 0.0), ("pa", 0.0),
+                           ("ref_freq", 0.0))}
+    arrays["gaussian"] = np.array([c.get("gaussian", False) for c in specs])
+    arrays["pa_absolute"] = np.array([c.get("pa_absolute", False) for c in specs])
+    arrays["spectral_terms"] = np.array([(c.get("spectral_terms") or []) + [0.0] * (2 - len(
+        c.get("spectral_terms") or [])) for c in specs])
+    arrays["n_spectral_terms"] = np.array([len(c.get("spectral_terms") or []) for c in specs])
+    arrays["log_si"] = np.array([c.get("log_si", True) for c in specs])
+    arrays["rm"] = np.array([c.get("rm") or (0.0, 0.0, 0.0) for c in specs])
+    arrays["has_rm"] = np.array([c.get("rm") is not None for c in specs])
+    np.savez_compressed(os.path.join(OUT, "golden_predict.npz"), ref=np.array(geom.ref),
+                        uvw=geom.uvw, baselines=geom.baselines, freq=geom.freq,
+                        chan_width=geom.chan_width, visibilities=vis, **arrays)
+    print(f"golden_predict.npz: {vis.shape}")
+
+
+if __name__ == "__main__":
+    small(2)
+    small(4)
+    solvertester()
+    predict()

@@ -3,6 +3,8 @@
 Model visibilities here come from the CPU oracle's predict (oracle/), which
 is test infrastructure; the GPU tests feed the same inputs to the HIP path.
 """
+import os
+
 import numpy as np
 
 from dp3_amd import synth as S
@@ -72,3 +74,37 @@ def phase_ref_relerr(sol_a, sol_b, n_ant, n_sub):
     ra = S.phase_referenced(sol_a, n_ant, n_sub)
     rb = S.phase_referenced(sol_b, n_ant, n_sub)
     return float(np.max(np.abs(ra - rb) / np.abs(rb)))
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def golden(name):
+    """Arrays of a committed fixture in tests/golden/ (numpy, no pickles)."""
+    with np.load(os.path.join(GOLDEN, name), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def golden_problem(name):
+    """(SolveProblem, fixture arrays) of golden_small_{duo,full}.npz."""
+    g = golden(name)
+    P = S.SolveProblem(int(g["n_antennas"]), int(g["n_directions"]), int(g["n_channel_blocks"]),
+                       int(g["n_correlations"]), g["n_solutions_per_direction"].astype(np.int64),
+                       g["vis_offsets"], g["data"], g["model"], g["ant1"], g["ant2"],
+                       g["sol_map"], int(g["n_times"]), int(g["n_baselines"]),
+                       g["channel_begin"])
+    return P, g
+
+
+def golden_components(g):
+    """Oracle components of golden_predict.npz."""
+    out = []
+    for i in range(len(g["ra"])):
+        terms = list(g["spectral_terms"][i][: int(g["n_spectral_terms"][i])])
+        out.append(O.make_component(
+            g["ra"][i], g["dec"][i], g["I"][i], g["Q"][i], g["U"][i], g["V"][i],
+            gaussian=bool(g["gaussian"][i]), major=g["major"][i], minor=g["minor"][i],
+            pa=g["pa"][i], pa_absolute=bool(g["pa_absolute"][i]), spectral_terms=terms or None,
+            ref_freq=g["ref_freq"][i], log_si=bool(g["log_si"][i]),
+            rm=tuple(g["rm"][i]) if g["has_rm"][i] else None))
+    return out
