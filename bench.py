@@ -65,6 +65,7 @@ HBM_PEAK_GBS = 8000.0
 MAX_ITERATIONS = 1000
 TOLERANCE = 1e-6
 ROOF_ITERATIONS = 50
+GRAPH = 1
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "correlate_traffic.txt")
 
 
@@ -148,6 +149,9 @@ def main():
     ap.add_argument("--cpu-iterations", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch iterations directly instead of replaying captured hipGraphs "
+                         "(same kernels; for profilers that do not follow graph launches)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also time a plain copy and a plain read of 1 GiB")
     args = ap.parse_args()
@@ -157,6 +161,8 @@ def main():
               "the one-GPU number is the per-GPU throughput)", file=sys.stderr)
         sys.exit(2)
 
+    global GRAPH
+    GRAPH = 0 if args.no_graph else 1
     w = S.LOFAR
     B, T, TR, WU = args.streams, args.steps, args.train, args.warmup
     n_total = TR + WU + T
@@ -217,7 +223,7 @@ def main():
     # ---- optimized form: batched + FP16 + learned warm start on the device
     batch = H.Batch([q.solver for q in s16], [q.problems[first] for q in s16])
     tuned = batch.autotune()
-    batch.set_schedule(tuned[0], tuned[1], 1, 8)
+    batch.set_schedule(tuned[0], tuned[1], GRAPH, 8)
     log(f"autotuned batch schedule: waves {tuned[0]}, rows {tuned[1]}, correlate {tuned[2]:.4f} ms")
 
     def run_batched(use_net):
@@ -265,7 +271,7 @@ def main():
     results = {}
     for name, (limit, start) in forms.items():
         s = WS.sibling(plain, limit, TOLERANCE)
-        s.set_schedule(waves_per_tile=p_tuned[0], rows_per_trip=p_tuned[1], use_graph=1,
+        s.set_schedule(waves_per_tile=p_tuned[0], rows_per_trip=p_tuned[1], use_graph=GRAPH,
                        poll_interval=8)
         prev = [p.copy() for p in prevs]
         t_all, chis, its = 0.0, [], []

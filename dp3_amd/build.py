@@ -68,7 +68,30 @@ def build(verbose=True, force=False):
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT}")
+    build_cpp(verbose)
     return OUT
+
+
+CPP = os.path.join(HERE, "cpp")
+CPP_OUT = os.path.join(HERE, "libddecal_hip_solver.so")
+CXX = os.environ.get("CXX", "g++")
+
+
+def build_cpp(verbose=True):
+    """The C++ host layer (dp3_amd/cpp/: SolverBase-compatible
+    IterativeDiagonalSolverHip over the C-ABI) as libddecal_hip_solver.so,
+    linked against libddecal_hip.so (found next to it at run time)."""
+    tmp = CPP_OUT + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
+           os.path.join(CPP, "iterative_diagonal_solver_hip.cc"), "-L" + HERE, "-lddecal_hip",
+           "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ host layer build failed:\n{r.stderr}")
+    os.replace(tmp, CPP_OUT)
+    if verbose:
+        print(f"built {CPP_OUT}")
+    return CPP_OUT
 
 
 if __name__ == "__main__":

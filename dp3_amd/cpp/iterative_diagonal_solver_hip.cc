@@ -1,0 +1,121 @@
+#include "iterative_diagonal_solver_hip.h"
+
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/ddecal_hip.h"
+
+namespace dp3amd {
+
+namespace {
+
+[[noreturn]] void ThrowLastError(const char* what) {
+  throw std::runtime_error(std::string(what) + ": " + ddh_last_error());
+}
+
+}  // namespace
+
+IterativeDiagonalSolverHip::IterativeDiagonalSolverHip(int model_storage)
+    : model_storage_(model_storage) {
+  if (model_storage != DDH_MODEL_C64 && model_storage != DDH_MODEL_F16)
+    throw std::runtime_error("Unknown model storage for the HIP iterative solver");
+}
+
+IterativeDiagonalSolverHip::~IterativeDiagonalSolverHip() {
+  if (solver_) ddh_solver_destroy(solver_);
+}
+
+void IterativeDiagonalSolverHip::Initialize(size_t n_antennas,
+                                            const std::vector<size_t>& n_solutions_per_direction,
+                                            size_t n_channel_blocks) {
+  SolverBase::Initialize(n_antennas, n_solutions_per_direction, n_channel_blocks);
+  if (solver_) ddh_solver_destroy(solver_);
+  solver_ = nullptr;
+  key_.clear();
+}
+
+ddh_solver* IterativeDiagonalSolverHip::DeviceSolver(int n_correlations) {
+  if (NAntennas() == 0) throw std::runtime_error("Solve called before Initialize");
+  double acc = GetAccuracy(), cacc = GetConstraintAccuracy(), step = GetStepSize(),
+         sigma = GetStepDiffSigma();
+  std::vector<int64_t> key = {n_correlations, int64_t(GetMaxIterations()),
+                              int64_t(GetMinIterations()), GetDetectStalling(), GetPhaseOnly(),
+                              model_storage_};
+  for (double v : {acc, cacc, step, sigma}) {
+    int64_t bits;
+    std::memcpy(&bits, &v, sizeof bits);
+    key.push_back(bits);
+  }
+  if (solver_ && key == key_) return solver_;
+  if (solver_) ddh_solver_destroy(solver_);
+  solver_ = nullptr;
+  std::vector<int32_t> n_sol(NSolutionsPerDirection().begin(), NSolutionsPerDirection().end());
+  ddh_config c;
+  ddh_config_defaults(&c);
+  c.n_antennas = int32_t(NAntennas());
+  c.n_directions = int32_t(n_sol.size());
+  c.n_channel_blocks = int32_t(NChannelBlocks());
+  c.n_correlations = n_correlations;
+  c.n_solutions_per_direction = n_sol.data();
+  c.max_iterations = int32_t(GetMaxIterations());
+  c.min_iterations = int32_t(GetMinIterations());
+  c.accuracy = acc;
+  c.constraint_accuracy = cacc;
+  c.step_size = step;
+  c.detect_stalling = GetDetectStalling() ? 1 : 0;
+  c.step_diff_sigma = sigma;
+  c.phase_only = GetPhaseOnly() ? 1 : 0;
+  c.model_storage = model_storage_;
+  solver_ = ddh_solver_create(&c);
+  if (!solver_) ThrowLastError("Could not create the HIP solver");
+  key_ = key;
+  return solver_;
+}
+
+template <typename SolveDataType>
+SolverBase::SolveResult IterativeDiagonalSolverHip::SolveImpl(
+    const SolveDataType& data, std::vector<std::vector<DComplex>>& solutions,
+    int n_correlations) {
+  if (data.NChannelBlocks() != NChannelBlocks() || solutions.size() != NChannelBlocks())
+    throw std::runtime_error("Solve data or solutions do not match the initialized channel blocks");
+  const size_t per_cb = NAntennas() * NSubSolutions() * NSolutionPolarizations();
+  std::vector<ddh_cb_view> views(NChannelBlocks());
+  std::vector<double*> sol_ptrs(NChannelBlocks());
+  for (size_t cb = 0; cb < NChannelBlocks(); ++cb) {
+    const auto& block = data.ChannelBlock(cb);
+    if (solutions[cb].size() != per_cb)
+      throw std::runtime_error("Solution vector of a channel block has the wrong size");
+    if (block.NDirections() != NSolutionsPerDirection().size())
+      throw std::runtime_error("Solve data has a different number of directions");
+    ddh_cb_view& v = views[cb];
+    v.n_visibilities = int64_t(block.NVisibilities());
+    const bool empty = block.NVisibilities() == 0;
+    v.data = empty ? nullptr : reinterpret_cast<const float*>(&block.Visibility(0));
+    v.model = empty ? nullptr : reinterpret_cast<const float*>(&block.ModelVisibility(0, 0));
+    v.antenna_pairs = empty ? nullptr : reinterpret_cast<const uint32_t*>(block.AntennaIndices());
+    v.solution_map = block.SolutionMapData();
+    sol_ptrs[cb] = reinterpret_cast<double*>(solutions[cb].data());
+  }
+  ddh_result r{};
+  if (ddh_solve(DeviceSolver(n_correlations), views.data(), int32_t(views.size()),
+                sol_ptrs.data(), 0.0, &r) != DDH_OK)
+    ThrowLastError("HIP solve failed");
+  last_converged_ = r.has_converged != 0;
+  SolveResult out;
+  out.iterations = size_t(r.iterations);
+  out.constraint_iterations = size_t(r.constraint_iterations);
+  return out;
+}
+
+SolverBase::SolveResult IterativeDiagonalSolverHip::Solve(
+    const DuoSolveData& data, std::vector<std::vector<DComplex>>& solutions, double) {
+  return SolveImpl(data, solutions, 2);
+}
+
+SolverBase::SolveResult IterativeDiagonalSolverHip::Solve(
+    const FullSolveData& data, std::vector<std::vector<DComplex>>& solutions, double) {
+  return SolveImpl(data, solutions, 4);
+}
+
+}  // namespace dp3amd

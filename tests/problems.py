@@ -55,15 +55,16 @@ def solvertester_problem(n_sol_per_dir=(1, 2, 5), n_corr=2):
 
 def check_dual_results(solutions, inp, n_ant, n_sub, n_cb, tolerance_percent):
     """SolverTester::CheckDualResults (SolverTester.cc:409-439): |g|^2 and the
-    real part of g * conj(g_ant0) within `tolerance_percent` (BOOST_CHECK_CLOSE,
-    a relative tolerance in percent). Returns the worst relative error."""
+    real part of g * conj(g_ant0) within `tolerance_percent` (BOOST_CHECK_CLOSE:
+    the difference relative to each of the two values, in percent). Returns
+    the worst relative error."""
     s = solutions.reshape(n_cb, n_ant, n_sub, 2)
     i = inp.reshape(n_ant, n_sub, 2)
     worst = 0.0
     for a, b in [(np.abs(s) ** 2, np.broadcast_to(np.abs(i) ** 2, s.shape)),
                  ((s * np.conj(s[:, 0:1])).real,
                   np.broadcast_to((i * np.conj(i[0:1])).real, s.shape))]:
-        rel = np.abs(a - b) / np.maximum(np.abs(a), np.abs(b))
+        rel = np.abs(a - b) / np.minimum(np.abs(a), np.abs(b))
         worst = max(worst, float(rel.max()))
     assert worst * 100.0 <= tolerance_percent, f"worst relative error {worst:.3e}"
     return worst

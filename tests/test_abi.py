@@ -39,3 +39,17 @@ def test_oracle_exports_every_declared_function():
     names = declared(os.path.join(ROOT, "oracle", "oracle.h"))
     missing = exported(os.path.join(ROOT, "oracle", "liboracle.so"), names)
     assert not missing, missing
+
+
+def test_cpp_host_layer_links():
+    """The C++ host layer and its test binary resolve every shared library
+    they need (nothing is run)."""
+    import subprocess
+    for path in (os.path.join(ROOT, "dp3_amd", "libddecal_hip_solver.so"),
+                 os.path.join(ROOT, "tests", "cpp", "test_iterative_diagonal_solver_hip")):
+        if not os.path.exists(path):
+            pytest.fail(f"{path} is not built (python __graft_entry__.py)")
+        out = subprocess.run(["ldd", path], capture_output=True, text=True).stdout
+        assert "not found" not in out, out
+    lib = ctypes.CDLL(os.path.join(ROOT, "dp3_amd", "libddecal_hip_solver.so"))
+    assert lib is not None
