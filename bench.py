@@ -95,26 +95,7 @@ def log(msg):
 def cpu_baseline(dp, w, iterations_per_solve, sample_iterations, threads):
     """Oracle (reference loop structure) on `sample_iterations` iterations."""
     from oracle import pyoracle as O
-    cbe = S.channel_begin(w.n_channels, w.n_channel_blocks)
-    n_t, n_bl = dp.n_times, dp.n_baselines
-    datas, models, a1s, a2s, sms, offs = [], [], [], [], [], [0]
-    bl = np.array([(a, b) for a in range(w.n_stations) for b in range(a + 1, w.n_stations)])
-    for cb in range(w.n_channel_blocks):
-        nc = int(cbe[cb + 1] - cbe[cb])
-        d = dp.download_plane(cb, 0).reshape(n_t, nc, dp.bl_stride, 2)[:, :, :n_bl]
-        datas.append(np.ascontiguousarray(d.transpose(0, 2, 1, 3)).reshape(-1, 2))
-        for di in range(w.n_directions):
-            m = dp.download_plane(cb, di + 1).reshape(n_t, nc, dp.bl_stride, 2)[:, :, :n_bl]
-            models.append(np.ascontiguousarray(m.transpose(0, 2, 1, 3)).reshape(-1, 2))
-        nv = n_t * n_bl * nc
-        offs.append(offs[-1] + nv)
-        sms.append(np.repeat(np.arange(w.n_directions, dtype=np.uint32), nv))
-        a1s.append(np.repeat(np.tile(bl[:, 0], n_t), nc).astype(np.uint32))
-        a2s.append(np.repeat(np.tile(bl[:, 1], n_t), nc).astype(np.uint32))
-    P = S.SolveProblem(w.n_stations, w.n_directions, w.n_channel_blocks, 2,
-                       w.sols_per_dir(), np.array(offs, np.int64), np.concatenate(datas),
-                       np.concatenate(models), np.concatenate(a1s), np.concatenate(a2s),
-                       np.concatenate(sms), n_t, n_bl, cbe)
+    P = S.download_problem(dp, w)
     sol = P.unit_solutions()
     t0 = time.perf_counter()
     O.solve(P, sol, max_iterations=sample_iterations, accuracy=0.0, detect_stalling=False,

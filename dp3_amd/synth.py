@@ -275,6 +275,38 @@ def problem_from_solvertester(data, model, n_sol_per_dir, n_ant, n_cb, packed):
                         packed["sol_map"], data.shape[0], data.shape[1])
 
 
+def download_problem(dp, w: Workload, baselines=None):
+    """The SolveData-layout copy (SolveProblem) of a device problem built for
+    workload w with the regular baseline order (a < b), complex64 planes:
+    each plane is read back and reordered time -> baseline -> channel."""
+    cbe = channel_begin(w.n_channels, w.n_channel_blocks)
+    n_t, n_bl = dp.n_times, dp.n_baselines
+    if baselines is None:
+        baselines = np.array([(a, b) for a in range(w.n_stations)
+                              for b in range(a + 1, w.n_stations)])
+    datas, models, a1s, a2s, sms, offs = [], [], [], [], [], [0]
+    spd = w.sols_per_dir()
+    start = np.concatenate([[0], np.cumsum(spd)[:-1]])
+    for cb in range(w.n_channel_blocks):
+        nc = int(cbe[cb + 1] - cbe[cb])
+        d = dp.download_plane(cb, 0).reshape(n_t, nc, dp.bl_stride, 2)[:, :, :n_bl]
+        datas.append(np.ascontiguousarray(d.transpose(0, 2, 1, 3)).reshape(-1, 2))
+        for di in range(w.n_directions):
+            m = dp.download_plane(cb, di + 1).reshape(n_t, nc, dp.bl_stride, 2)[:, :, :n_bl]
+            models.append(np.ascontiguousarray(m.transpose(0, 2, 1, 3)).reshape(-1, 2))
+        nv = n_t * n_bl * nc
+        offs.append(offs[-1] + nv)
+        t_of_vis = np.repeat(np.arange(n_t), n_bl * nc)
+        sms.append(np.concatenate([(t_of_vis * spd[di]) // n_t + start[di]
+                                   for di in range(w.n_directions)]).astype(np.uint32))
+        a1s.append(np.repeat(np.tile(baselines[:, 0], n_t), nc).astype(np.uint32))
+        a2s.append(np.repeat(np.tile(baselines[:, 1], n_t), nc).astype(np.uint32))
+    return SolveProblem(w.n_stations, w.n_directions, w.n_channel_blocks, 2, spd,
+                        np.array(offs, np.int64), np.concatenate(datas), np.concatenate(models),
+                        np.concatenate(a1s), np.concatenate(a2s), np.concatenate(sms), n_t, n_bl,
+                        cbe)
+
+
 def phase_referenced(solutions, n_ant, n_sub):
     """Solutions [n_cb, n_ant*n_sub*2] -> g * conj(g_ant0) / |g_ant0| per
     (cb, sub-solution, pol), removing the unconstrained common phase."""

@@ -18,9 +18,9 @@
 //   ddh_problem_*       <- SolveData regular layout (SolveData.cc:77-188), kept
 //                          device resident in a baseline-fastest layout.
 //   ddh_solve_problem   <- same Solve, on a device-resident problem.
-//   ddh_solve_batch     <- DDECal::doSolve's loop over solution intervals
+//   ddh_batch_*         <- DDECal::doSolve's loop over solution intervals
 //                          (steps/DDECal.cc:595-708), all intervals at once.
-//   ddh_predict_*       <- Simulator::visit(PointSource|GaussianSource)
+//   ddh_predict         <- Simulator::visit(PointSource|GaussianSource)
 //                          (base/Simulator.cc:110-468) + OnePredict's Stokes-I
 //                          copy (steps/OnePredict.cc:437-457).
 #ifndef DDECAL_HIP_H
@@ -112,8 +112,8 @@ void ddh_problem_destroy(ddh_problem* problem);
 // Copies host SolveData channel blocks (layout as in ddh_cb_view, with the
 // problem's structure) into the device layout.
 int ddh_problem_upload(ddh_problem* problem, const ddh_cb_view* channel_blocks);
-// Copies host visibilities already in the reference DPBuffer order per
-// block: data [n_t][n_bl][n_ch_cb][n_corr], model [n_dir][n_t][n_bl][n_ch_cb][n_corr].
+// Baselines per (time, channel) row of a device plane: n_baselines rounded
+// up to a multiple of 64, the padding reading as zeros.
 int ddh_problem_baseline_stride(const ddh_problem* problem);
 // Device pointers of the data plane / a model plane of block cb (for device
 // producers such as the predict). Model planes of FP16 problems hold

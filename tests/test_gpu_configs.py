@@ -1,0 +1,93 @@
+"""The parity shapes of the batched and FP16 configurations:
+
+* uneven batch: 64 independent intervals in one batch with hipGraph-captured
+  iterations and an autotuned schedule; stations 24-62, solution interval
+  2-20, channel blocks of 8-48 channels, 3-12 directions (complex64).
+* FP16 model at many directions: 16 directions with FP16 model planes (per
+  block and direction scale), FP32 accumulation, FP64 convergence test,
+  against the FP64 oracle.
+Data are predicted and synthesized on the device; the oracle gets the same
+visibilities read back into the SolveData layout."""
+import numpy as np
+import pytest
+
+from dp3_amd import hip as H
+from dp3_amd import synth as S
+from dp3_amd import warmstart as WS
+from oracle import pyoracle as O
+from problems import phase_ref_relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def uneven_workloads(n=64, seed=2024):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        n_st = int(rng.integers(24, 63))
+        n_dir = int(rng.integers(3, 13))
+        n_cb = int(rng.integers(1, 3))
+        ch = int(rng.integers(8, 49))
+        solint = int(rng.integers(2, 21))
+        out.append(S.Workload(f"u{i}", n_st, n_dir, 2, 0, ch * n_cb, n_cb, solint, 25, 1e-5,
+                              seed=i))
+    return out
+
+
+def test_uneven_batch_of_64_intervals():
+    print("case: 64 uneven intervals, batched", flush=True)
+    ws = uneven_workloads()
+    seqs = [WS.IntervalSequence(w, 1, seed=100 + i) for i, w in enumerate(ws)]
+    solvers = [q.solver for q in seqs]
+    problems = [q.problems[0] for q in seqs]
+    batch = H.Batch(solvers, problems)
+    waves, rows, ms = batch.autotune()
+    print(f"autotuned waves {waves} rows {rows}: correlate {ms:.3f} ms over 64 intervals")
+    batch.set_schedule(waves, rows, 1, 8)
+    starts = [q.unit() for q in seqs]
+    res = batch.solve(starts)
+    for i, (q, s) in enumerate(zip(seqs, solvers)):
+        s.set_schedule(waves_per_tile=waves, rows_per_trip=rows, use_graph=0, poll_interval=8)
+        one = q.unit()
+        r = s.solve_problem(q.problems[0], one)
+        assert r.iterations == res[i].iterations, i
+        np.testing.assert_array_equal(starts[i], one)
+    # the two smallest intervals against the oracle
+    order = np.argsort([w.n_baselines * w.n_channels * w.solint * w.n_directions for w in ws])
+    for i in order[:2]:
+        w = ws[i]
+        P = S.download_problem(problems[i], w)
+        ref = P.unit_solutions()
+        rr = O.solve(P, ref, max_iterations=w.max_iterations, accuracy=w.tolerance, n_threads=4)
+        err = phase_ref_relerr(starts[i], ref, P.n_antennas, P.n_sub)
+        print(f"interval {i} ({w.n_stations} st, {w.n_directions} dir, solint {w.solint}): "
+              f"iterations {res[i].iterations} / oracle {rr.iterations}, gain err {err:.2e}")
+        assert res[i].iterations == rr.iterations
+        assert err <= 1e-4
+
+
+def test_sixteen_directions_fp16_model_against_fp64_oracle():
+    print("case: 16 directions, FP16 model", flush=True)
+    w = S.Workload("dir16", 24, 16, 2, 1, 48, 8, 4, 80, 1e-6, seed=3)
+    q16 = WS.IntervalSequence(w, 1, seed=33, storage=H.MODEL_F16)
+    q64 = WS.IntervalSequence(w, 1, seed=33, storage=H.MODEL_C64)
+    P = S.download_problem(q64.problems[0], w)
+    ref64 = P.unit_solutions()
+    O.solve(P, ref64, max_iterations=w.max_iterations, accuracy=w.tolerance, fp64=True,
+            n_threads=8)
+    ref32 = P.unit_solutions()
+    r_ref = O.solve(P, ref32, max_iterations=w.max_iterations, accuracy=w.tolerance, n_threads=8)
+    g16 = q16.unit()
+    q16.solver.solve_problem(q16.problems[0], g16)
+    g64 = q64.unit()
+    r64 = q64.solver.solve_problem(q64.problems[0], g64)
+    c = {k: O.chi2(P, g) for k, g in (("fp64 oracle", ref64), ("oracle", ref32),
+                                      ("gpu c64", g64), ("gpu fp16", g16))}
+    e16 = phase_ref_relerr(g16, ref64, P.n_antennas, P.n_sub)
+    e64 = phase_ref_relerr(g64, ref64, P.n_antennas, P.n_sub)
+    print(f"gain error vs FP64 oracle: fp16 model {e16:.2e}, complex64 model {e64:.2e}; "
+          f"chi2 {c}")
+    assert phase_ref_relerr(g64, ref32, P.n_antennas, P.n_sub) <= 1e-4
+    assert r64.iterations <= r_ref.iterations
+    assert e16 <= 2e-3
+    assert abs(c["gpu fp16"] - c["fp64 oracle"]) <= 1e-3 * c["fp64 oracle"]
