@@ -96,6 +96,7 @@ struct ddh_solver {
   int poll = 8;
   // per-solve device buffers
   ddh::DeviceBuffer<double2> sol0, sol1;
+  ddh::DeviceBuffer<float2> gram;  // Gram iteration cross products
   ddh::DeviceBuffer<double> mean, wg_conv, wg_mean, step_mags;
   ddh::DeviceBuffer<int32_t> wg_conv_n, wg_mean_n;
   ddh::DeviceBuffer<float> partials;
@@ -120,6 +121,11 @@ namespace ddh {
 namespace {
 
 void ValidateConfig(const ddh_config& c) {
+  DDH_REQUIRE(c.iteration == DDH_ITERATION_STREAM || c.iteration == DDH_ITERATION_GRAM, -1,
+              "unknown iteration kind");
+  DDH_REQUIRE(c.iteration != DDH_ITERATION_GRAM ||
+                  (c.n_correlations == 2 && c.n_directions <= kMaxGramDirections),
+              -4, "the Gram iteration is built for Duo data with at most 8 directions");
   DDH_REQUIRE(c.n_antennas > 1, -1, "n_antennas must be > 1");
   DDH_REQUIRE(c.n_directions >= 1 && c.n_directions <= 16, -4,
               "n_directions must be 1..16");
@@ -384,6 +390,7 @@ IterArgs MakeArgs(ddh_solver& s, ddh_problem& p) {
   a.detect_stalling = c.detect_stalling;
   a.phase_only = c.phase_only;
   a.den_out = nullptr;
+  a.gram = s.gram.ptr;
   int max_nsd = 1;
   for (int v : s.cfg.n_sol_dir) max_nsd = std::max(max_nsd, v);
   a.split = 1;
@@ -398,8 +405,24 @@ void DestroyGraph(ddh_solver& s) {
   s.graph_key.clear();
 }
 
+bool GramMode(const ddh_solver& s) { return s.cfg.c.iteration == DDH_ITERATION_GRAM; }
+
+// The correlate partials of one iteration, by the configured iteration kind.
+void LaunchCorrelation(const ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
+  if (GramMode(s))
+    LaunchApplyGram(a, p.f16, s.stream);
+  else
+    LaunchCorrelate(a, p.n_corr, p.f16, s.waves, s.unroll, s.stream);
+}
+
+// Gram iteration: the cross products of the problem, before the first iteration.
+void LaunchSolvePrologue(const ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
+  LaunchPrologue(a, s.stream);
+  if (GramMode(s)) LaunchGram(a, p.f16, s.waves, s.stream);
+}
+
 void LaunchIteration(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
-  LaunchCorrelate(a, p.n_corr, p.f16, s.waves, s.unroll, s.stream);
+  LaunchCorrelation(s, a, p);
   LaunchAntennaSolve(a, p.n_corr, p.f16, s.stream);
   LaunchDecide(a, s.stream);
 }
@@ -408,6 +431,7 @@ void LaunchIteration(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
 // the solver buffers stay the same.
 void EnsureGraph(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
   const std::vector<const void*> key = {&p, p.data.ptr, p.model.ptr, s.sol0.ptr, s.partials.ptr,
+                                        s.gram.ptr,
                                         reinterpret_cast<const void*>(intptr_t(s.poll)),
                                         reinterpret_cast<const void*>(intptr_t(s.waves)),
                                         reinterpret_cast<const void*>(intptr_t(s.unroll))};
@@ -438,7 +462,7 @@ void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
   const size_t sol_per_cb = size_t(c.n_antennas) * s.cfg.n_sub * 2;
   const size_t n_sol = sol_per_cb * c.n_channel_blocks;
   const size_t n_wg = size_t(c.n_channel_blocks) * c.n_directions;
-  const void* before[2] = {s.sol0.ptr, s.partials.ptr};
+  const void* before[3] = {s.sol0.ptr, s.partials.ptr, s.gram.ptr};
   s.sol0.Resize(n_sol);
   s.sol1.Resize(n_sol);
   s.mean.Resize(c.n_channel_blocks);
@@ -449,7 +473,10 @@ void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
   s.step_mags.Resize(size_t(std::max(c.max_iterations, c.min_iterations)) + 2);
   s.partials.Resize(size_t(p.n_cb) * p.n_seg * p.n_dir * p.n_corr * 3 * p.bl_stride);
   s.state.Resize(1);
-  if (before[0] != s.sol0.ptr || before[1] != s.partials.ptr) DestroyGraph(s);
+  if (c.iteration == DDH_ITERATION_GRAM)
+    s.gram.Resize(size_t(p.n_cb) * p.n_seg * 2 * GramTerms(p.n_dir) * p.bl_stride);
+  if (before[0] != s.sol0.ptr || before[1] != s.partials.ptr || before[2] != s.gram.ptr)
+    DestroyGraph(s);
 }
 
 void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* result) {
@@ -474,7 +501,7 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
   }
   if (graph) EnsureGraph(s, a, p);
   DDH_HIP_CHECK(hipEventRecord(t0, s.stream));
-  LaunchPrologue(a, s.stream);
+  LaunchSolvePrologue(s, a, p);
   int launched = 0;
   while (launched < limit) {
     if (graph) {
@@ -485,7 +512,7 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
       for (int i = 0; i < k; ++i) {
         const int it = launched + i;
         if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it], s.stream));
-        LaunchCorrelate(a, p.n_corr, p.f16, s.waves, s.unroll, s.stream);
+        LaunchCorrelation(s, a, p);
         if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it + 1], s.stream));
         LaunchAntennaSolve(a, p.n_corr, p.f16, s.stream);
         LaunchDecide(a, s.stream);
@@ -585,6 +612,7 @@ void ddh_config_defaults(ddh_config* c) {
   c->step_diff_sigma = 0.1;
   c->phase_only = 0;
   c->model_storage = DDH_MODEL_C64;
+  c->iteration = DDH_ITERATION_STREAM;
 }
 
 ddh_solver* ddh_solver_create(const ddh_config* config) {
@@ -786,6 +814,7 @@ struct ddh_batch {
   hipStream_t stream = nullptr;
   int n_corr = 2;
   bool f16 = false;
+  bool gram = false;  // Gram iteration
   int waves = 2, unroll = 1, use_graph = 1, poll = 8;
   int limit = 0;  // largest iteration limit over the intervals
   size_t solve_lds = 0;
@@ -876,10 +905,28 @@ void BuildBatch(ddh_batch& b) {
   DestroyBatchGraph(b);
 }
 
+// One batched iteration over `args` (the batch's arguments or a variant of
+// them): the correlate (Gram iteration: apply) of every tile group, then the
+// antenna solve over n_items items and the decision for n_intervals intervals.
+void LaunchBatchStep(ddh_batch& b, const IterArgs* args, int n_items, int n_intervals) {
+  if (b.gram)
+    LaunchBatchApply(args, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), b.d_items.ptr,
+                     n_items, n_intervals, b.solve_lds, b.f16, b.stream);
+  else
+    LaunchBatch(args, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), b.d_items.ptr,
+                n_items, n_intervals, b.solve_lds, b.n_corr, b.f16, b.waves, b.unroll, b.stream);
+}
+
+// Gram iteration: the cross products of every interval (shared by all
+// argument variants, which point at the same buffers).
+void LaunchBatchCrossProducts(ddh_batch& b) {
+  if (b.gram)
+    LaunchBatchGram(b.d_args.ptr, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), b.f16,
+                    b.waves, b.stream);
+}
+
 void LaunchBatchIteration(ddh_batch& b) {
-  LaunchBatch(b.d_args.ptr, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), b.d_items.ptr,
-              int(b.items.size()), int(b.solvers.size()), b.solve_lds, b.n_corr, b.f16, b.waves,
-              b.unroll, b.stream);
+  LaunchBatchStep(b, b.d_args.ptr, int(b.items.size()), int(b.solvers.size()));
 }
 
 void EnsureBatchGraph(ddh_batch& b) {
@@ -920,12 +967,13 @@ void UploadSolutions(ddh_batch& b, double* const* const* solutions, Dst&& dst) {
 
 // Runs the iterations from the solutions in each interval's sol0 to every
 // interval's stopping criterion and downloads the results.
-void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results);
+void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results,
+              bool cross_products);
 
 void SolveBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results) {
   UploadSolutions(b, solutions, [&](int i) { return b.args[i].sol0; });
   b.last_warm_ms = 0.0;
-  RunBatch(b, solutions, results);
+  RunBatch(b, solutions, results, true);
 }
 
 // The learned warm start on the device, for every interval at once:
@@ -981,18 +1029,15 @@ void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* cons
   auto run_aux = [&](double2* out) {
     LaunchBatchPrologue(b.d_args_aux.ptr, b.d_pro_items.ptr, n_pro, b.stream);
     for (int k = 0; k < stretch; ++k)
-      LaunchBatch(b.d_args_aux.ptr, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()),
-                  b.d_items.ptr, int(b.items.size()), n, b.solve_lds, b.n_corr, b.f16, b.waves,
-                  b.unroll, b.stream);
+      LaunchBatchStep(b, b.d_args_aux.ptr, int(b.items.size()), n);
     for (int i = 0; i < n; ++i)
       LaunchSnapshot(b.d_args_aux.ptr, i, int64_t(SolutionCount(b, i)), out + off[i], b.stream);
   };
   // next1: iteration 0 reads sol0 and writes sol1; no decision is taken
   copy_into_sol0(b.warm_g0.ptr);
   LaunchBatchPrologue(b.d_args_step.ptr, b.d_pro_items.ptr, n_pro, b.stream);
-  LaunchBatch(b.d_args_step.ptr, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()),
-              b.d_items.ptr, int(b.items.size()), 0, b.solve_lds, b.n_corr, b.f16, b.waves,
-              b.unroll, b.stream);
+  LaunchBatchCrossProducts(b);  // once for the whole warm start and solve
+  LaunchBatchStep(b, b.d_args_step.ptr, int(b.items.size()), 0);
   for (int i = 0; i < n; ++i)
     DDH_HIP_CHECK(hipMemcpyAsync(b.warm_next.ptr + off[i], b.args[i].sol1,
                                  SolutionCount(b, i) * sizeof(double2), hipMemcpyDeviceToDevice,
@@ -1015,13 +1060,14 @@ void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* cons
     LaunchWarmApply(b.warm_g0.ptr + off[i], pred + int64_t(off[i] / 2) * kWarmOutputs,
                     int64_t(SolutionCount(b, i) / 2), b.args[i].sol0, b.stream);
   DDH_HIP_CHECK(hipEventRecord(b.ev_warm1, b.stream));
-  RunBatch(b, solutions, results);
+  RunBatch(b, solutions, results, false);
   float ms = 0.0f;
   DDH_HIP_CHECK(hipEventElapsedTime(&ms, b.ev_warm0, b.ev_warm1));
   b.last_warm_ms = ms;
 }
 
-void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results) {
+void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results,
+              bool cross_products) {
   const int n = int(b.solvers.size());
   const bool graph = b.use_graph != 0 && !b.profile;
   if (graph) EnsureBatchGraph(b);
@@ -1035,6 +1081,7 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
   DDH_HIP_CHECK(hipEventCreate(&t1));
   DDH_HIP_CHECK(hipEventRecord(t0, b.stream));
   LaunchBatchPrologue(b.d_args.ptr, b.d_pro_items.ptr, int(b.pro_items.size()), b.stream);
+  if (cross_products) LaunchBatchCrossProducts(b);
   int launched = 0;
   bool all_done = false;
   while (launched < b.limit && !all_done) {
@@ -1048,9 +1095,7 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
         if (b.profile) {
           // correlate alone between the events, then the rest of the iteration
           DDH_HIP_CHECK(hipEventRecord(b.events[2 * it], b.stream));
-          LaunchBatch(b.d_args.ptr, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()),
-                      b.d_items.ptr, 0, 0, b.solve_lds, b.n_corr, b.f16, b.waves, b.unroll,
-                      b.stream);
+          LaunchBatchStep(b, b.d_args.ptr, 0, 0);
           DDH_HIP_CHECK(hipEventRecord(b.events[2 * it + 1], b.stream));
           LaunchBatchTail(b.d_args.ptr, b.d_items.ptr, int(b.items.size()), int(b.solvers.size()),
                           b.solve_lds, b.n_corr, b.f16, b.stream);
@@ -1150,9 +1195,13 @@ ddh_batch* ddh_batch_create(ddh_solver* const* solvers, ddh_problem* const* prob
       DDH_REQUIRE(b->solvers[i] && b->problems[i], -1, "NULL interval in batch");
     b->n_corr = b->problems[0]->n_corr;
     b->f16 = b->problems[0]->f16;
-    for (int i = 0; i < n_intervals; ++i)
+    b->gram = GramMode(*b->solvers[0]);
+    for (int i = 0; i < n_intervals; ++i) {
       DDH_REQUIRE(b->problems[i]->n_corr == b->n_corr && b->problems[i]->f16 == b->f16, -4,
                   "all intervals of a batch share correlations and model storage");
+      DDH_REQUIRE(GramMode(*b->solvers[i]) == b->gram, -4,
+                  "all intervals of a batch share the iteration kind");
+    }
     DDH_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     DDH_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b->host_states),
                                 sizeof(DeviceState) * n_intervals, hipHostMallocDefault));
@@ -1290,10 +1339,12 @@ int ddh_batch_autotune(ddh_batch* b, int32_t* waves_out, int32_t* rows_out, doub
       b->unroll = cand.second;
       BuildBatch(*b);
       const double ms = TimeLaunches(b->stream, 5, [&] {
-        // correlate only: it does not change the solve state
-        LaunchBatch(b->d_args.ptr, b->d_tiles.ptr, b->groups.data(), int(b->groups.size()),
-                    b->d_items.ptr, 0, 0, b->solve_lds, b->n_corr, b->f16, b->waves, b->unroll,
-                    b->stream);
+        // correlate only (Gram iteration: the cross products, which the
+        // schedule shapes): neither changes the solve state
+        if (b->gram)
+          LaunchBatchCrossProducts(*b);
+        else
+          LaunchBatchStep(*b, b->d_args.ptr, 0, 0);
       });
       if (ms < best) {
         best = ms;
@@ -1325,8 +1376,12 @@ int ddh_autotune(ddh_solver* s, ddh_problem* p, int32_t* waves_out, int32_t* row
     double best = 1e30;
     int bw = s->waves, bu = s->unroll;
     for (const auto& cand : ScheduleCandidates(p->n_corr)) {
+      // Gram iteration: the schedule shapes the once-per-solve cross products
       const double ms = TimeLaunches(s->stream, 5, [&] {
-        LaunchCorrelate(a, p->n_corr, p->f16, cand.first, cand.second, s->stream);
+        if (GramMode(*s))
+          LaunchGram(a, p->f16, cand.first, s->stream);
+        else
+          LaunchCorrelate(a, p->n_corr, p->f16, cand.first, cand.second, s->stream);
       });
       if (ms < best) {
         best = ms;
@@ -1363,8 +1418,8 @@ int ddh_one_step(ddh_solver* s, ddh_problem* p, double* const* solutions_in,
     a.step_size = 1.0;  // stepped == next: the undamped update
     a.phase_only = 0;
     a.den_out = s->den.ptr;
-    LaunchPrologue(a, s->stream);
-    LaunchCorrelate(a, p->n_corr, p->f16, s->waves, s->unroll, s->stream);
+    LaunchSolvePrologue(*s, a, *p);
+    LaunchCorrelation(*s, a, *p);
     LaunchAntennaSolve(a, p->n_corr, p->f16, s->stream);
     for (int cb = 0; cb < c.n_channel_blocks; ++cb) {
       DDH_HIP_CHECK(hipMemcpyAsync(next_out[cb], s->sol1.ptr + cb * per_cb,

@@ -94,7 +94,16 @@ struct IterArgs {
   double* den_out;           // optional [n_cb][n_ant][n_sub][2] denominators
   int32_t split;             // threads per (antenna, solution) in antenna_solve
   int32_t max_nsd;           // largest n_sol_dir entry
+  // Gram iteration (Duo, n_dir <= kMaxGramDirections): data/model cross
+  // products C_d and G_e,d per (block, segment, correlation, baseline), made
+  // once per solve; [n_cb][n_seg][2][n_dir + n_dir(n_dir+1)/2][bl_stride].
+  float2* gram;
 };
+
+// Largest direction count of the Gram iteration (its per-lane accumulators
+// grow with n_dir^2).
+constexpr int kMaxGramDirections = 8;
+inline int GramTerms(int n_dir) { return n_dir + n_dir * (n_dir + 1) / 2; }
 
 // One unit of batched work: correlate tiles use all four fields; antenna
 // solve items use (interval, cb) with the direction in `seg`; prologue items
@@ -122,8 +131,6 @@ size_t CorrelateSharedBytes(int n_dir, int n_corr, int waves);
 size_t MaxSharedBytes();
 size_t AntennaSolveSharedBytes(const IterArgs& a, int n_corr);
 void PrepareKernelAttributes();
-// One batched iteration: correlate over all tiles, antenna solve over all
-// items, then one decide workgroup per interval.
 // Tiles of one direction count: entries [first, first + count) of a tile list.
 struct TileGroup {
   int32_t n_dir;
@@ -137,6 +144,17 @@ void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, const TileGro
                  int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
                  size_t solve_lds, int n_corr, bool f16, int waves, int rows,
                  hipStream_t stream);
+// Gram iteration: the cross products once per solve, then per iteration the
+// correlate partials from them and the current gains.
+void LaunchGram(const IterArgs& a, bool f16, int waves, hipStream_t stream);
+void LaunchApplyGram(const IterArgs& a, bool f16, hipStream_t stream);
+size_t GramSharedBytes(int n_dir, int waves);
+void LaunchBatchGram(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
+                     int n_groups, bool f16, int waves, hipStream_t stream);
+// One batched Gram iteration: apply per tile group, then antenna solve and decision.
+void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
+                      int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
+                      size_t solve_lds, bool f16, hipStream_t stream);
 // Current solutions (by iteration parity) of batched interval `interval`
 // into dst, `count` complex values.
 void LaunchSnapshot(const IterArgs* d_args, int interval, int64_t count, double2* dst,

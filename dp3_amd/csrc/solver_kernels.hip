@@ -340,6 +340,221 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads)
   CorrelateTile<ND, NC, F16, ROWS>(a, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
+// ---------------------------------------------------------------------------
+// Gram iteration (Duo, n_dir <= kMaxGramDirections). Within a solution
+// segment the gain products gg_e of a baseline are constant, so the
+// correlate's sums factor once more:
+//   U_d,k = sum_rows R_all,k conj(M_d,k) = C_d,k - sum_e gg_e,k G_e,d,k
+//   S_d,k = G_d,d,k,  C_d,k = sum_rows D_k conj(M_d,k),
+//   G_e,d,k = sum_rows M_e,k conj(M_d,k) = conj(G_d,e,k).
+// C and G do not depend on the gains: gram makes them once per solve (one
+// pass over the data and model planes) and apply evaluates U and S from them
+// and the current gains every iteration, without visibility traffic, into
+// the partials layout of correlate (antenna_solve is unchanged). For FP16
+// planes both hold the unscaled half values and apply scales gg, as
+// correlate does. Layout [cb][seg][k][q][bl_stride], q < ND: C_d,
+// q = ND + d(d+1)/2 + e: G_e,d (e <= d).
+// ---------------------------------------------------------------------------
+constexpr int kGramChunk = 8;
+
+template <int ND>
+constexpr int GramPair(int e, int d) {
+  return ND + d * (d + 1) / 2 + e;
+}
+
+template <int ND, bool F16>
+__device__ __forceinline__ void GramTile(const IterArgs& a, const int tile, const int seg,
+                                         const int cb) {
+  constexpr int NC = 2;
+  constexpr int NQ = ND + ND * (ND + 1) / 2;
+  constexpr uint32_t kDataBytes = VisBytes<NC, false>();
+  constexpr uint32_t kModelBytes = VisBytes<NC, F16>();
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kWave);
+  const int n_waves = blockDim.x / kWave;
+  const CbInfo info = LoadConst(a.cbinfo + cb);
+  const int n_ch = info.n_ch;
+  const size_t plane = size_t(a.n_t) * n_ch * a.bl_stride;
+  const int bl = tile * kWave + lane;
+  const char* dbase = reinterpret_cast<const char*>(a.data) + size_t(info.data_off) * kDataBytes;
+  const char* mbase = reinterpret_cast<const char*>(a.model) + size_t(info.model_off) * kModelBytes;
+  const size_t mplane = plane * kModelBytes;
+  const int r0 = AsConst(a.seg_t0)[seg] * n_ch;
+  const int r1 = AsConst(a.seg_t0)[seg + 1] * n_ch;
+  const uint32_t row_vis = uint32_t(r1) * uint32_t(a.bl_stride);
+  const __amdgpu_buffer_rsrc_t drsrc = PlaneResource(dbase, row_vis * kDataBytes);
+  __amdgpu_buffer_rsrc_t mrsrc[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) mrsrc[d] = PlaneResource(mbase + d * mplane, row_vis * kModelBytes);
+
+  float acc[NQ][NC][2];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+#pragma unroll
+    for (int k = 0; k < NC; ++k) acc[q][k][0] = acc[q][k][1] = 0.0f;
+
+  for (int row = r0 + wave; row < r1; row += n_waves) {
+    const uint32_t e = uint32_t(row) * uint32_t(a.bl_stride) + uint32_t(bl);
+    RawVis<NC, false> dr;
+    RawVis<NC, F16> mr[ND];
+    LoadRaw<NC, false>(drsrc, e * kDataBytes, dr);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) LoadRaw<NC, F16>(mrsrc[d], e * kModelBytes, mr[d]);
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const float2 dv = RawGet<NC, false>(dr, k);
+      float2 m[ND];
+#pragma unroll
+      for (int d = 0; d < ND; ++d) m[d] = RawGet<NC, F16>(mr[d], k);
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        const float2 c = cmulc(dv, m[d]);
+        acc[d][k][0] += c.x;
+        acc[d][k][1] += c.y;
+#pragma unroll
+        for (int f = 0; f <= d; ++f) {
+          const float2 g = cmulc(m[f], m[d]);
+          acc[GramPair<ND>(f, d)][k][0] += g.x;
+          acc[GramPair<ND>(f, d)][k][1] += g.y;
+        }
+      }
+    }
+  }
+
+  // Combine the waves in a fixed order (wave w-1 += wave w), as correlate,
+  // kGramChunk terms at a time to bound the LDS.
+  extern __shared__ float smem[];
+#pragma unroll
+  for (int q0 = 0; q0 < NQ; q0 += kGramChunk) {
+    for (int w = n_waves - 1; w >= 1; --w) {
+      if (wave == w) {
+#pragma unroll
+        for (int q = q0; q < q0 + kGramChunk && q < NQ; ++q)
+#pragma unroll
+          for (int k = 0; k < NC; ++k) {
+            smem[(((q - q0) * NC + k) * 2 + 0) * kWave + lane] = acc[q][k][0];
+            smem[(((q - q0) * NC + k) * 2 + 1) * kWave + lane] = acc[q][k][1];
+          }
+      }
+      __syncthreads();
+      if (wave == w - 1) {
+#pragma unroll
+        for (int q = q0; q < q0 + kGramChunk && q < NQ; ++q)
+#pragma unroll
+          for (int k = 0; k < NC; ++k) {
+            acc[q][k][0] += smem[(((q - q0) * NC + k) * 2 + 0) * kWave + lane];
+            acc[q][k][1] += smem[(((q - q0) * NC + k) * 2 + 1) * kWave + lane];
+          }
+      }
+      __syncthreads();
+    }
+  }
+  if (wave == 0) {
+    GlobalT<float>* out = (GlobalT<float>*)AsGlobal(
+        a.gram + size_t(cb * a.n_seg + seg) * NC * NQ * a.bl_stride + bl);
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        out[2 * size_t(k * NQ + q) * a.bl_stride + 0] = acc[q][k][0];
+        out[2 * size_t(k * NQ + q) * a.bl_stride + 1] = acc[q][k][1];
+      }
+  }
+}
+
+template <int ND, bool F16>
+__device__ __forceinline__ void ApplyGramTile(const IterArgs& a, const int tile, const int seg,
+                                              const int cb) {
+  constexpr int NC = 2;
+  constexpr int NQ = ND + ND * (ND + 1) / 2;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int bl = tile * kWave + lane;
+  const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
+  const double2* sol = (AsConst(a.state)->iteration & 1) ? a.sol1 : a.sol0;
+  const double mean = AsConst(a.mean)[cb];
+  const bool valid = bl < a.n_bl;
+  const int a1 = valid ? AsGlobal(a.ant1)[bl] : 0;
+  const int a2 = valid ? AsGlobal(a.ant2)[bl] : 0;
+  // gg_d,k (repaired, cast to float, FP16 scale folded in), as correlate stages it
+  float2 gg[ND][NC];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const int s = AsConst(a.seg_sol)[seg * ND + d];
+    const size_t p1 = gbase + size_t(a1) * a.n_sub + s;
+    const size_t p2 = gbase + size_t(a2) * a.n_sub + s;
+    double2 g1[2] = {LoadGlobal(sol + p1 * 2), LoadGlobal(sol + p1 * 2 + 1)};
+    double2 g2[2] = {LoadGlobal(sol + p2 * 2), LoadGlobal(sol + p2 * 2 + 1)};
+    if (!(cfinite(g1[0]) && cfinite(g1[1]))) g1[0] = g1[1] = make_double2(mean, 0.0);
+    if (!(cfinite(g2[0]) && cfinite(g2[1]))) g2[0] = g2[1] = make_double2(mean, 0.0);
+    float sc = 1.0f;
+    if constexpr (F16) sc = AsConst(a.model_scale)[cb * ND + d];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      float2 g = cmulc(make_float2(float(g1[k].x), float(g1[k].y)),
+                       make_float2(float(g2[k].x), float(g2[k].y)));
+      gg[d][k] = valid ? make_float2(g.x * sc, g.y * sc) : make_float2(0.0f, 0.0f);
+    }
+  }
+  const GlobalT<float>* in = (const GlobalT<float>*)AsGlobal(
+      a.gram + size_t(cb * a.n_seg + seg) * NC * NQ * a.bl_stride + bl);
+  GlobalT<float>* out =
+      AsGlobal(a.partials) + (size_t(cb) * a.n_seg + seg) * ND * NC * 3 * a.bl_stride + bl;
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    float2 q[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      const GlobalT<float>* f = in + 2 * size_t(k * NQ + i) * a.bl_stride;
+      q[i] = make_float2(f[0], f[1]);
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      float2 u = q[d];
+      // directions subtracted in order, as the residual is formed
+#pragma unroll
+      for (int e = 0; e < ND; ++e) {
+        const float2 g = e <= d ? q[GramPair<ND>(e, d)]
+                                : make_float2(q[GramPair<ND>(d, e)].x, -q[GramPair<ND>(d, e)].y);
+        const float2 c = cmul(gg[e][k], g);
+        u.x -= c.x;
+        u.y -= c.y;
+      }
+      const size_t v = size_t(d * NC + k) * 3;
+      out[(v + 0) * a.bl_stride] = u.x;
+      out[(v + 1) * a.bl_stride] = u.y;
+      out[(v + 2) * a.bl_stride] = q[GramPair<ND>(d, d)].x;
+    }
+  }
+}
+
+template <int ND, bool F16>
+__global__ void __launch_bounds__(kCorrelateMaxThreads) GramKernel(const IterArgs a) {
+  GramTile<ND, F16>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <int ND, bool F16>
+__global__ void __launch_bounds__(kWave) ApplyGramKernel(const IterArgs a) {
+  if (AsConst(a.state)->done) return;
+  ApplyGramTile<ND, F16>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <int ND, bool F16>
+__global__ void __launch_bounds__(kCorrelateMaxThreads)
+    BatchGramKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ tiles) {
+  const BatchTile t = LoadConst(tiles + blockIdx.x);
+  const IterArgs a = LoadConst(args + t.interval);
+  GramTile<ND, F16>(a, t.tile, t.seg, t.cb);
+}
+
+template <int ND, bool F16>
+__global__ void __launch_bounds__(kWave)
+    BatchApplyGramKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ tiles) {
+  const BatchTile t = LoadConst(tiles + blockIdx.x);
+  const IterArgs a = LoadConst(args + t.interval);
+  if (AsConst(a.state)->done) return;
+  ApplyGramTile<ND, F16>(a, t.tile, t.seg, t.cb);
+}
+
 // Solutions of MakeSolutionsFinite2Pol (SolverBase.cc:342-371), applied on
 // the fly: a pair with a non-finite entry reads as (mean, mean), where mean is
 // the block's average |pol-0 value| over finite pairs (:354), kept per block.
@@ -925,7 +1140,87 @@ BatchKernelFn BatchCorrelateFor(int n_dir, int n_corr, bool f16, int rows) {
   return kBatchCorrelate[n_corr == 4][f16][rows - 1][n_dir - 1];
 }
 
+template <bool F16, int... ND>
+constexpr auto MakeGramTable(std::integer_sequence<int, ND...>) {
+  return std::array<KernelFn, sizeof...(ND)>{&GramKernel<ND + 1, F16>...};
+}
+template <bool F16, int... ND>
+constexpr auto MakeApplyGramTable(std::integer_sequence<int, ND...>) {
+  return std::array<KernelFn, sizeof...(ND)>{&ApplyGramKernel<ND + 1, F16>...};
+}
+template <bool F16, int... ND>
+constexpr auto MakeBatchGramTable(std::integer_sequence<int, ND...>) {
+  return std::array<BatchKernelFn, sizeof...(ND)>{&BatchGramKernel<ND + 1, F16>...};
+}
+template <bool F16, int... ND>
+constexpr auto MakeBatchApplyGramTable(std::integer_sequence<int, ND...>) {
+  return std::array<BatchKernelFn, sizeof...(ND)>{&BatchApplyGramKernel<ND + 1, F16>...};
+}
+using GramDirs = std::make_integer_sequence<int, kMaxGramDirections>;
+// [f16][n_dir - 1]
+const std::array<KernelFn, kMaxGramDirections> kGram[2] = {MakeGramTable<false>(GramDirs{}),
+                                                          MakeGramTable<true>(GramDirs{})};
+const std::array<KernelFn, kMaxGramDirections> kApplyGram[2] = {
+    MakeApplyGramTable<false>(GramDirs{}), MakeApplyGramTable<true>(GramDirs{})};
+const std::array<BatchKernelFn, kMaxGramDirections> kBatchGram[2] = {
+    MakeBatchGramTable<false>(GramDirs{}), MakeBatchGramTable<true>(GramDirs{})};
+const std::array<BatchKernelFn, kMaxGramDirections> kBatchApplyGram[2] = {
+    MakeBatchApplyGramTable<false>(GramDirs{}), MakeBatchApplyGramTable<true>(GramDirs{})};
+
+void CheckGramDirections(int n_dir) {
+  DDH_REQUIRE(n_dir >= 1 && n_dir <= kMaxGramDirections, -4,
+              "the Gram iteration supports 1.." + std::to_string(kMaxGramDirections) +
+                  " directions");
+}
+
 }  // namespace
+
+size_t GramSharedBytes(int n_dir, int waves) {
+  (void)n_dir;
+  return waves > 1 ? size_t(kGramChunk) * 2 * 2 * kWave * sizeof(float) : 0;
+}
+
+void LaunchGram(const IterArgs& a, bool f16, int waves, hipStream_t stream) {
+  CheckGramDirections(a.n_dir);
+  DDH_REQUIRE(waves >= 1 && waves * kWave <= kCorrelateMaxThreads, -1, "bad waves per tile");
+  const dim3 grid(a.bl_stride / kWave, a.n_seg, a.n_cb);
+  hipLaunchKernelGGL(kGram[f16][a.n_dir - 1], grid, dim3(waves * kWave),
+                     GramSharedBytes(a.n_dir, waves), stream, a);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchApplyGram(const IterArgs& a, bool f16, hipStream_t stream) {
+  CheckGramDirections(a.n_dir);
+  const dim3 grid(a.bl_stride / kWave, a.n_seg, a.n_cb);
+  hipLaunchKernelGGL(kApplyGram[f16][a.n_dir - 1], grid, dim3(kWave), 0, stream, a);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchBatchGram(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
+                     int n_groups, bool f16, int waves, hipStream_t stream) {
+  DDH_REQUIRE(waves >= 1 && waves * kWave <= kCorrelateMaxThreads, -1, "bad waves per tile");
+  for (int g = 0; g < n_groups; ++g) {
+    if (groups[g].count == 0) continue;
+    CheckGramDirections(groups[g].n_dir);
+    hipLaunchKernelGGL(kBatchGram[f16][groups[g].n_dir - 1], dim3(groups[g].count),
+                       dim3(waves * kWave), GramSharedBytes(groups[g].n_dir, waves), stream,
+                       d_args, d_tiles + groups[g].first);
+  }
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
+                      int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
+                      size_t solve_lds, bool f16, hipStream_t stream) {
+  for (int g = 0; g < n_groups; ++g) {
+    if (groups[g].count == 0) continue;
+    CheckGramDirections(groups[g].n_dir);
+    hipLaunchKernelGGL(kBatchApplyGram[f16][groups[g].n_dir - 1], dim3(groups[g].count),
+                       dim3(kWave), 0, stream, d_args, d_tiles + groups[g].first);
+  }
+  DDH_HIP_CHECK(hipGetLastError());
+  LaunchBatchTail(d_args, d_items, n_items, n_intervals, solve_lds, 2, f16, stream);
+}
 
 size_t CorrelateSharedBytes(int n_dir, int n_corr, int waves) {
   size_t bytes = size_t(n_dir) * n_corr * kWave * sizeof(float2);

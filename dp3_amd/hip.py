@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(HERE, "libddecal_hip.so")
 
 MODEL_C64 = 0
 MODEL_F16 = 1
+ITERATION_STREAM = 0
+ITERATION_GRAM = 1
 
 _LIB = None
 
@@ -36,6 +38,7 @@ class Config(C.Structure):
         ("step_diff_sigma", C.c_double),
         ("phase_only", C.c_int32),
         ("model_storage", C.c_int32),
+        ("iteration", C.c_int32),
     ]
 
 
@@ -323,7 +326,8 @@ class Solver:
     def __init__(self, n_antennas, n_solutions_per_direction, n_channel_blocks, *,
                  n_correlations=2, max_iterations=100, min_iterations=0, accuracy=1e-5,
                  constraint_accuracy=1e-4, step_size=0.2, detect_stalling=True,
-                 step_diff_sigma=0.1, phase_only=False, model_storage=MODEL_C64):
+                 step_diff_sigma=0.1, phase_only=False, model_storage=MODEL_C64,
+                 iteration=ITERATION_STREAM):
         L = lib()
         self._nsd = np.ascontiguousarray(n_solutions_per_direction, dtype=np.int32)
         cfg = Config()
@@ -342,12 +346,14 @@ class Solver:
         cfg.step_diff_sigma = step_diff_sigma
         cfg.phase_only = int(phase_only)
         cfg.model_storage = model_storage
+        cfg.iteration = iteration
         self.config = cfg
         self.n_antennas = n_antennas
         self.n_channel_blocks = n_channel_blocks
         self.n_correlations = n_correlations
         self.n_sub = int(self._nsd.sum())
         self.model_storage = model_storage
+        self.iteration = iteration
         h = L.ddh_solver_create(C.byref(cfg))
         if not h:
             raise HipError(f"ddh_solver_create failed: {last_error()}")

@@ -23,14 +23,15 @@ from . import hip as H
 from . import synth as S
 
 
-def sibling(solver: H.Solver, max_iterations, accuracy, detect_stalling=True):
+def sibling(solver: H.Solver, max_iterations, accuracy, detect_stalling=True, iteration=None):
     """A solver with the same shapes and step as `solver` but its own stopping
     criterion (device problems of one are valid for the other)."""
     c = solver.config
     return H.Solver(solver.n_antennas, solver._nsd, solver.n_channel_blocks,
                     n_correlations=solver.n_correlations, max_iterations=max_iterations,
                     accuracy=accuracy, step_size=c.step_size, detect_stalling=detect_stalling,
-                    phase_only=bool(c.phase_only), model_storage=solver.model_storage)
+                    phase_only=bool(c.phase_only), model_storage=solver.model_storage,
+                    iteration=solver.iteration if iteration is None else iteration)
 
 
 class WarmStart:

@@ -43,6 +43,12 @@ extern "C" {
 #define DDH_MODEL_C64 0   // complex64 (reference precision)
 #define DDH_MODEL_F16 1   // complex FP16 with a per (block, direction) scale
 
+// How an iteration correlates data and model (the results agree to rounding).
+#define DDH_ITERATION_STREAM 0  // stream data and model planes every iteration
+#define DDH_ITERATION_GRAM 1    // per-baseline data/model cross products made once
+                                // per solve, applied to the gains every
+                                // iteration (Duo, up to 8 directions)
+
 typedef struct ddh_solver ddh_solver;
 typedef struct ddh_problem ddh_problem;
 
@@ -61,6 +67,7 @@ typedef struct {
   double step_diff_sigma;                    // default 0.1
   int32_t phase_only;                        // default 0
   int32_t model_storage;                     // DDH_MODEL_*
+  int32_t iteration;                         // DDH_ITERATION_*
 } ddh_config;
 
 // Fills the SolverBase defaults (SolverBase.cc:27-39).

@@ -6,6 +6,7 @@ import pytest
 
 from dp3_amd import hip as H
 from dp3_amd import synth as S
+from dp3_amd import warmstart as WS
 from oracle import pyoracle as O
 from problems import phase_ref_relerr, small_problem
 
@@ -87,3 +88,26 @@ def test_autotune(intervals):
     s, dp = pairs[0]
     w2, r2, ms2 = H.autotune(s, dp)
     assert 1 <= w2 <= 4 and r2 in (1, 2) and ms2 > 0
+
+
+def test_gram_batch_equals_single_gram_solves():
+    """Batched Gram iterations (cross products per interval, one apply launch
+    per direction count) give the same results as single Gram solves."""
+    print("case: gram batch", flush=True)
+    ws = [S.Workload(f"g{i}", 10 + 3 * i, 2 + i, 1, 1, 8 * (1 + i % 2), 2, 2 + i, 40, 1e-5,
+                     seed=i) for i in range(4)]
+    seqs = [WS.IntervalSequence(w, 1, seed=50 + i, solver_kwargs=dict(iteration=H.ITERATION_GRAM))
+            for i, w in enumerate(ws)]
+    singles = []
+    for q in seqs:
+        q.solver.set_schedule(waves_per_tile=2, rows_per_trip=1, use_graph=0, poll_interval=8)
+        g = q.unit()
+        r = q.solver.solve_problem(q.problems[0], g)
+        singles.append((g, r.iterations))
+    b = H.Batch([q.solver for q in seqs], [q.problems[0] for q in seqs])
+    b.set_schedule(2, 1, 1, 8)
+    got = [q.unit() for q in seqs]
+    res = b.solve(got)
+    for (g, its), x, r in zip(singles, got, res):
+        assert r.iterations == its
+        np.testing.assert_array_equal(x, g)

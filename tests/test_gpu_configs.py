@@ -67,27 +67,31 @@ def test_uneven_batch_of_64_intervals():
 
 
 def test_sixteen_directions_fp16_model_against_fp64_oracle():
+    """62 stations and 16 directions (one point and one Gaussian each) at a
+    reduced channel count. Sixteen directions converge slowly (the step
+    magnitude is still ~2e-3 after 400 iterations), so the comparison is at a
+    fixed count: 60 iterations, accuracy 0, no stall detection, complex64 and
+    FP16 model planes against the FP64 oracle after the same iterations."""
     print("case: 16 directions, FP16 model", flush=True)
-    w = S.Workload("dir16", 24, 16, 2, 1, 48, 8, 4, 80, 1e-6, seed=3)
-    q16 = WS.IntervalSequence(w, 1, seed=33, storage=H.MODEL_F16)
-    q64 = WS.IntervalSequence(w, 1, seed=33, storage=H.MODEL_C64)
+    w = S.Workload("dir16", 62, 16, 1, 1, 24, 4, 4, 60, 0.0, seed=3)
+    kw = dict(solver_kwargs=dict(detect_stalling=False))
+    q16 = WS.IntervalSequence(w, 1, seed=33, storage=H.MODEL_F16, **kw)
+    q64 = WS.IntervalSequence(w, 1, seed=33, storage=H.MODEL_C64, **kw)
     P = S.download_problem(q64.problems[0], w)
     ref64 = P.unit_solutions()
-    O.solve(P, ref64, max_iterations=w.max_iterations, accuracy=w.tolerance, fp64=True,
-            n_threads=8)
-    ref32 = P.unit_solutions()
-    r_ref = O.solve(P, ref32, max_iterations=w.max_iterations, accuracy=w.tolerance, n_threads=8)
+    r_ref = O.solve(P, ref64, max_iterations=w.max_iterations, accuracy=0.0, fp64=True,
+                    detect_stalling=False, n_threads=8)
     g16 = q16.unit()
-    q16.solver.solve_problem(q16.problems[0], g16)
+    r16 = q16.solver.solve_problem(q16.problems[0], g16)
     g64 = q64.unit()
     r64 = q64.solver.solve_problem(q64.problems[0], g64)
-    c = {k: O.chi2(P, g) for k, g in (("fp64 oracle", ref64), ("oracle", ref32),
-                                      ("gpu c64", g64), ("gpu fp16", g16))}
+    c = {k: O.chi2(P, g) for k, g in (("fp64 oracle", ref64), ("gpu c64", g64), ("gpu fp16", g16))}
     e16 = phase_ref_relerr(g16, ref64, P.n_antennas, P.n_sub)
     e64 = phase_ref_relerr(g64, ref64, P.n_antennas, P.n_sub)
-    print(f"gain error vs FP64 oracle: fp16 model {e16:.2e}, complex64 model {e64:.2e}; "
-          f"chi2 {c}")
-    assert phase_ref_relerr(g64, ref32, P.n_antennas, P.n_sub) <= 1e-4
-    assert r64.iterations <= r_ref.iterations
-    assert e16 <= 2e-3
-    assert abs(c["gpu fp16"] - c["fp64 oracle"]) <= 1e-3 * c["fp64 oracle"]
+    print(f"after {r_ref.iterations - 1} iterations: gain error vs FP64 oracle: complex64 model "
+          f"{e64:.2e}, FP16 model {e16:.2e}; chi2 {c}")
+    assert r_ref.iterations == r64.iterations == r16.iterations == w.max_iterations + 1
+    assert e64 <= 1e-4
+    assert e16 <= 5e-3
+    for k in ("gpu c64", "gpu fp16"):
+        assert abs(c[k] - c["fp64 oracle"]) <= 1e-3 * c["fp64 oracle"], k

@@ -182,3 +182,60 @@ def test_schedule_does_not_change_results_beyond_rounding(small, waves, rows, gr
     sol2 = P.unit_solutions()
     s.solve(P, sol2)
     np.testing.assert_array_equal(sol, sol2)
+
+
+@pytest.mark.parametrize("storage", [H.MODEL_C64, H.MODEL_F16])
+def test_gram_iteration_matches_oracle(small, storage):
+    """The Gram iteration (cross products once per solve, applied to the gains
+    each iteration) against the oracle: same tolerances as the streaming
+    iteration (FP16 model: against the FP64 oracle)."""
+    w, geom, sky, gains, P = small
+    print(f"case: gram iteration storage={storage}", flush=True)
+    sol = P.unit_solutions()
+    r = gpu_solver(P, max_iterations=w.max_iterations, accuracy=w.tolerance,
+                   model_storage=storage, iteration=H.ITERATION_GRAM).solve(P, sol)
+    if storage == H.MODEL_C64:
+        ref = P.unit_solutions()
+        rr = O.solve(P, ref, max_iterations=w.max_iterations, accuracy=w.tolerance)
+        assert r.iterations == rr.iterations
+        tol = GAIN_TOL
+    else:
+        ref = P.unit_solutions()
+        O.solve(P, ref, max_iterations=w.max_iterations, accuracy=w.tolerance, fp64=True)
+        tol = 2e-3
+    err = phase_ref_relerr(sol, ref, P.n_antennas, P.n_sub)
+    c, cr = O.chi2(P, sol), O.chi2(P, ref)
+    print(f"gram: iterations {r.iterations}, gain err {err:.2e}, chi2 {c:.6g} / {cr:.6g}")
+    assert err <= tol
+    assert abs(c - cr) <= CHI2_TOL * cr
+    # streaming and Gram iterations agree
+    s2 = P.unit_solutions()
+    gpu_solver(P, max_iterations=w.max_iterations, accuracy=w.tolerance,
+               model_storage=storage).solve(P, s2)
+    assert phase_ref_relerr(sol, s2, P.n_antennas, P.n_sub) <= tol
+
+
+def test_gram_iteration_dd_intervals():
+    """Direction-dependent solution intervals (segments with several solution
+    indices) under the Gram iteration: SolverTester recovery and oracle parity."""
+    print("case: gram iteration solvertester dd(1,2,5)", flush=True)
+    P, inp = solvertester_problem((1, 2, 5), n_corr=2)
+    sol = P.unit_solutions()
+    s = gpu_solver(P, max_iterations=100, accuracy=1e-8, step_size=0.2,
+                   iteration=H.ITERATION_GRAM)
+    r = s.solve(P, sol)
+    check_dual_results(sol, inp, P.n_antennas, P.n_sub, P.n_channel_blocks, 1e-2)
+    ref = P.unit_solutions()
+    r_ref = O.solve(P, ref, max_iterations=100, accuracy=1e-8, step_size=0.2, n_threads=4)
+    print(f"iterations gram {r.iterations} oracle {r_ref.iterations}")
+    assert r.iterations <= r_ref.iterations
+    assert phase_ref_relerr(sol, ref, P.n_antennas, P.n_sub) <= GAIN_TOL
+
+
+def test_gram_iteration_rejects_unsupported_shapes(small):
+    w, geom, sky, gains, P = small
+    with pytest.raises(H.HipError):
+        H.Solver(P.n_antennas, P.n_solutions_per_direction, P.n_channel_blocks,
+                 n_correlations=4, iteration=H.ITERATION_GRAM)
+    with pytest.raises(H.HipError):
+        H.Solver(P.n_antennas, [1] * 9, P.n_channel_blocks, iteration=H.ITERATION_GRAM)
