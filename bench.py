@@ -21,29 +21,35 @@ evaluated in FP64 on the complex64 model and compared with the interval's
 converged chi^2 (a 1000-iteration solve at tolerance 1e-9).
 
   * plain         the HIP solver with the reference's settings: unit gains,
-                  complex64 model planes, one interval per solve.
+                  complex64 model planes, one interval per solve, streaming
+                  iteration (data and model planes read every iteration).
   * optimized     batched (one launch per kernel per iteration over all
                   streams), FP16 model planes (per block/direction scale),
                   learned warm start from the previous interval's solutions
                   computed on the device (one undamped step + 2 x 4 damped
                   iterations -> features -> 3-layer 256-hidden bf16 MLP),
-                  inside the timed region. `value` is this form.
-Also reported: the plain form at the 100-iteration limit; previous-solution
-starts (DP3's propagatesolutions) unbatched in complex64; the optimized form
-without the network (batched + FP16 + previous solutions).
+                  inside the timed region, and the Gram iteration (per-baseline
+                  data/model cross products made once per solve, applied to
+                  the gains every iteration). `value` is this form.
+Also reported: the optimized form with the streaming iteration (the three
+changes alone), the optimized form without the network (previous
+solutions), the plain form at the 100-iteration limit and previous-solution
+starts (DP3's propagatesolutions) unbatched in complex64.
 
-`roofline` is for the dominant kernel of the optimized form, the batched
-correlate over FP16 planes: algorithmic bytes per launch = streams x n_vis x
-(16 + n_dir x 8) (data complex64, model 2 x half2), over its mean duration
-from HIP events on the batch stream around each launch that did work.
-`roofline_plain` is the complex64 correlate of the plain form
-(n_vis x (16 + n_dir x 16) per launch). Both are measured on 50-iteration
-solves at accuracy 0, so that every launch does the full work. `traffic` is
-read from the committed rocprofv3 --pmc summary of the same command, when
-present. `cpu_baseline` times the CPU oracle
-(a restatement of DP3's IterativeDiagonalSolver, threads over channel
-blocks) on a bounded sample of plain-form iterations and scales it to
-solves/s at the plain form's mean iteration count.
+`roofline` is for the dominant kernel of the optimized form, by time per
+batched iteration: the Gram apply (algorithmic bytes: the cross products read
+and the correlate partials written) or the antenna solve + decision (partials
+read, solutions read and written). Times come from HIP events on the batch
+stream around each launch of a fixed-length (50-iteration, accuracy 0) solve
+in which every interval is live in every launch. `roofline_other` holds the
+others, including the batched FP16 correlate of the streaming form
+(streams x n_vis x (16 + n_dir x 8) bytes); `roofline_plain` is the complex64
+correlate of the plain form (n_vis x (16 + n_dir x 16)). `traffic` is read
+from the committed rocprofv3 --pmc summary of the same command, when present.
+`cpu_baseline` times the CPU oracle (a restatement of DP3's
+IterativeDiagonalSolver, threads over channel blocks) on a bounded sample of
+plain-form iterations and scales it to solves/s at the plain form's mean
+iteration count.
 """
 import argparse
 import json
@@ -66,7 +72,7 @@ MAX_ITERATIONS = 1000
 TOLERANCE = 1e-6
 ROOF_ITERATIONS = 50
 GRAPH = 1
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "correlate_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_per_launch.txt")
 
 
 def measured_traffic(kernel):
@@ -201,13 +207,9 @@ def main():
     def excess(b, i, g):
         return H.chi2(s64[b].solver, s64[b].problems[i], g) / conv[(b, i)] - 1.0
 
-    # ---- optimized form: batched + FP16 + learned warm start on the device
-    batch = H.Batch([q.solver for q in s16], [q.problems[first] for q in s16])
-    tuned = batch.autotune()
-    batch.set_schedule(tuned[0], tuned[1], GRAPH, 8)
-    log(f"autotuned batch schedule: waves {tuned[0]}, rows {tuned[1]}, correlate {tuned[2]:.4f} ms")
-
-    def run_batched(use_net):
+    # ---- optimized forms: batched + FP16 + learned warm start on the device,
+    # with the streaming and the Gram iteration
+    def run_batched(batch, use_net):
         prev = [p.copy() for p in prevs]
         t_all, warm_ms, chis, its = 0.0, 0.0, [], []
         for i in range(first, n_total):
@@ -224,25 +226,70 @@ def main():
                 for b in range(B):
                     chis.append(excess(b, i, prev[b]))
                     its.append(res[b].iterations)
-        return B * T / t_all, t_all, warm_ms, chis, its
+        return {"value": B * T / t_all, "seconds": t_all, "warm_ms": warm_ms, "chi2": chis,
+                "iterations": its}
 
-    value, t_opt, warm_ms, chi_opt, its_opt = run_batched(True)
-    v_nonet, _, _, chi_nonet, its_nonet = run_batched(False)
-    log(f"optimized {value:.2f} solves/s, without network {v_nonet:.2f}")
+    def kernel_times(iteration, tuned):
+        """Mean correlate (or apply) and antenna solve + decision per batched
+        iteration over a fixed-length profiled solve in which every interval is
+        live in every launch (the last launches of these kernels in the run)."""
+        rb = H.Batch([WS.sibling(q.solver, ROOF_ITERATIONS, 0.0, False, iteration=iteration)
+                      for q in s16], [q.problems[n_total - 1] for q in s16])
+        rb.set_schedule(tuned[0], tuned[1], 0, 8)
+        rb.set_profiling(True)
+        rb.solve([q.unit() for q in s16])
+        corr_ms, solve_ms, n_it = rb.last_timing()
+        rb.close()
+        return corr_ms / max(n_it, 1), solve_ms / max(n_it, 1)
 
-    # roofline of the batched FP16 correlate: a fixed-length profiled batch in
-    # which every interval is live in every launch (the last launches of this
-    # kernel in the run)
-    rb = H.Batch([WS.sibling(q.solver, ROOF_ITERATIONS, 0.0, False) for q in s16],
-                 [q.problems[n_total - 1] for q in s16])
-    rb.set_schedule(tuned[0], tuned[1], 0, 8)
-    rb.set_profiling(True)
-    rb.solve([q.unit() for q in s16])
-    corr_ms, corr_n = rb.last_timing()
-    rb.close()
-    avg_ms = corr_ms / max(corr_n, 1)
-    bytes_opt = B * algorithmic_bytes(w, 8)
-    achieved = bytes_opt / (avg_ms * 1e-3) / 1e9
+    opt = {}
+    for name, iteration in (("gram", H.ITERATION_GRAM), ("streaming", H.ITERATION_STREAM)):
+        solvers = [WS.sibling(q.solver, MAX_ITERATIONS, TOLERANCE, iteration=iteration)
+                   for q in s16]
+        batch = H.Batch(solvers, [q.problems[first] for q in s16])
+        tuned = batch.autotune()
+        batch.set_schedule(tuned[0], tuned[1], GRAPH, 8)
+        r = run_batched(batch, True)
+        r["without_network"] = run_batched(batch, False)
+        r["schedule"] = tuned
+        r["kernels_ms"] = kernel_times(iteration, tuned)
+        opt[name] = r
+        log(f"optimized ({name} iteration): {r['value']:.2f} solves/s, without network "
+            f"{r['without_network']['value']:.2f}; per batched iteration: correlate "
+            f"{r['kernels_ms'][0] * 1e3:.1f} us, antenna solve + decision "
+            f"{r['kernels_ms'][1] * 1e3:.1f} us")
+        batch.close()
+    best = opt["gram"]
+    value, t_opt, warm_ms = best["value"], best["seconds"], best["warm_ms"]
+
+    n_vis = w.n_baselines * w.n_channels * w.solint
+    bl_stride = -(-w.n_baselines // 64) * 64
+    partial_bytes = w.n_channel_blocks * w.n_directions * 2 * 3 * bl_stride * 4
+    solution_bytes = w.n_channel_blocks * w.n_stations * w.n_sub * 2 * 16
+    gram_bytes = w.n_channel_blocks * 2 * (w.n_directions + w.n_directions *
+                                           (w.n_directions + 1) // 2) * bl_stride * 8
+
+    def roof(name, nbytes, ms, **extra):
+        achieved = nbytes / (ms * 1e-3) / 1e9
+        out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+               "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+               "traffic": measured_traffic(name), "kernel": name,
+               "algorithmic_bytes_per_launch": int(nbytes), "mean_launch_ms": round(ms, 5)}
+        out.update(extra)
+        return out
+
+    g_apply, g_solve = opt["gram"]["kernels_ms"]
+    s_corr, s_solve = opt["streaming"]["kernels_ms"]
+    rooflines = {
+        "gram_apply": roof("gram_apply", B * (gram_bytes + partial_bytes), g_apply,
+                           note="per-baseline cross products read, correlate partials written"),
+        "antenna_solve": roof("antenna_solve", B * (partial_bytes + 2 * solution_bytes), g_solve,
+                              note="partials read, solutions read and written; with the "
+                                   "decision kernel; bound by reduction latency, not HBM"),
+        "batched_correlate_f16": roof("batched_correlate_f16", B * n_vis * (16 + w.n_directions * 8),
+                                      s_corr),
+    }
+    dominant = "gram_apply" if g_apply >= g_solve else "antenna_solve"
 
     # ---- plain forms: complex64, one interval at a time
     plain = s64[0].solver
@@ -282,6 +329,7 @@ def main():
     p_achieved = algorithmic_bytes(w, 16) / (p_avg * 1e-3) / 1e9
 
     v_plain = results["plain"][0]
+    p_roof = roof("correlate_c64", algorithmic_bytes(w, 16), p_avg)
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -302,51 +350,42 @@ def main():
                          f"step 0.2, tolerance 1e-6, iteration limit {MAX_ITERATIONS}"),
             "form": (f"optimized: {B} interval streams batched, FP16 model planes with FP32 "
                      "accumulation and FP64 convergence test, learned warm start (3-layer "
-                     "256-hidden bf16 MLP) on the device, runs to tolerance"),
+                     "256-hidden bf16 MLP) on the device, Gram iteration, runs to tolerance"),
             "step": f"one interval of each of {B} streams (a batch of {B} solves)",
             "streams": B,
             "train_intervals_per_stream": TR,
             "train_samples": int(len(x)),
             "warm_start_stretch": ws.stretch,
             "warm_start_device_ms_per_step": round(warm_ms / T, 3),
-            "optimized": summary(chi_opt, its_opt),
+            "optimized": summary(best["chi2"], best["iterations"]),
+            "speedup_vs_plain": round(value / v_plain, 3),
             "plain_solves_per_s": round(v_plain, 3),
             "plain": summary(results["plain"][1], results["plain"][2]),
-            "speedup_vs_plain": round(value / v_plain, 3),
+            "optimized_streaming_solves_per_s": round(opt["streaming"]["value"], 3),
+            "optimized_streaming": summary(opt["streaming"]["chi2"], opt["streaming"]["iterations"]),
+            "speedup_streaming_vs_plain": round(opt["streaming"]["value"] / v_plain, 3),
+            "optimized_without_network_solves_per_s": round(best["without_network"]["value"], 3),
+            "optimized_without_network": summary(best["without_network"]["chi2"],
+                                                 best["without_network"]["iterations"]),
             "plain_100_iterations_solves_per_s": round(results["plain_100"][0], 3),
             "plain_100_iterations": summary(results["plain_100"][1], results["plain_100"][2]),
             "plain_previous_start_solves_per_s": round(results["plain_previous"][0], 3),
             "plain_previous_start": summary(results["plain_previous"][1],
                                             results["plain_previous"][2]),
-            "speedup_vs_plain_previous_start": round(value / results["plain_previous"][0], 3),
-            "optimized_without_network_solves_per_s": round(v_nonet, 3),
-            "optimized_without_network": summary(chi_nonet, its_nonet),
-            "batch_schedule": {"waves_per_tile": tuned[0], "rows_per_trip": tuned[1]},
+            "per_batched_iteration_us": {
+                "gram_apply": round(g_apply * 1e3, 1), "gram_antenna_solve": round(g_solve * 1e3, 1),
+                "streaming_correlate": round(s_corr * 1e3, 1),
+                "streaming_antenna_solve": round(s_solve * 1e3, 1)},
+            "batch_schedule": {"waves_per_tile": opt["gram"]["schedule"][0],
+                               "rows_per_trip": opt["gram"]["schedule"][1]},
+            "batch_schedule_streaming": {"waves_per_tile": opt["streaming"]["schedule"][0],
+                                         "rows_per_trip": opt["streaming"]["schedule"][1]},
             "plain_schedule": {"waves_per_tile": p_tuned[0], "rows_per_trip": p_tuned[1]},
             "setup_s": round(t_setup, 1),
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic("batched_correlate_f16"),
-            "kernel": "batched correlate, FP16 model planes",
-            "algorithmic_bytes_per_launch": bytes_opt,
-            "mean_launch_ms": round(avg_ms, 5),
-        },
-        "roofline_plain": {
-            "bound": "hbm",
-            "achieved": round(p_achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(p_achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic("correlate_c64"),
-            "kernel": "correlate, complex64 model planes",
-            "algorithmic_bytes_per_launch": algorithmic_bytes(w, 16),
-            "mean_launch_ms": round(p_avg, 5),
-        },
+        "roofline": rooflines[dominant],
+        "roofline_plain": p_roof,
+        "roofline_other": {k: v for k, v in rooflines.items() if k != dominant},
     }
     if args.calibrate:
         copy_gbs, read_gbs = H.calibrate_bandwidth(1 << 30, 10)

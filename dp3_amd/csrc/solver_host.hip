@@ -228,8 +228,9 @@ ddh_problem* CreateProblem(const ddh_solver& s, int n_times, int n_baselines,
   std::copy(p->ant2.begin(), p->ant2.end(), a2.begin());
   p->d_ant1.Upload(a1, s.stream);
   p->d_ant2.Upload(a2, s.stream);
-  // Per-antenna baseline lists: (baseline << 1) | side, side 1 = second antenna.
-  std::vector<int32_t> off(p->n_ant + 1, 0), csr(size_t(2) * n_baselines);
+  // Per-antenna baseline lists, two words per entry: (baseline << 1) | side
+  // (side 1 = second antenna) and the baseline's other antenna.
+  std::vector<int32_t> off(p->n_ant + 1, 0), csr(size_t(4) * n_baselines);
   for (int b = 0; b < n_baselines; ++b) {
     off[p->ant1[b] + 1]++;
     off[p->ant2[b] + 1]++;
@@ -237,8 +238,11 @@ ddh_problem* CreateProblem(const ddh_solver& s, int n_times, int n_baselines,
   for (int a = 0; a < p->n_ant; ++a) off[a + 1] += off[a];
   std::vector<int32_t> fill(off.begin(), off.end() - 1);
   for (int b = 0; b < n_baselines; ++b) {
-    csr[fill[p->ant1[b]]++] = (b << 1);
-    csr[fill[p->ant2[b]]++] = (b << 1) | 1;
+    const int e1 = fill[p->ant1[b]]++, e2 = fill[p->ant2[b]]++;
+    csr[2 * e1] = (b << 1);
+    csr[2 * e1 + 1] = p->ant2[b];
+    csr[2 * e2] = (b << 1) | 1;
+    csr[2 * e2 + 1] = p->ant1[b];
   }
   p->d_csr_off.Upload(off, s.stream);
   p->d_csr.Upload(csr, s.stream);
@@ -393,8 +397,9 @@ IterArgs MakeArgs(ddh_solver& s, ddh_problem& p) {
   a.gram = s.gram.ptr;
   int max_nsd = 1;
   for (int v : s.cfg.n_sol_dir) max_nsd = std::max(max_nsd, v);
+  const int threads = AntennaSolveThreads(c.n_antennas, max_nsd);
   a.split = 1;
-  while (a.split < 8 && c.n_antennas * max_nsd * a.split * 2 <= 1024) a.split *= 2;
+  while (a.split < 8 && c.n_antennas * max_nsd * a.split * 2 <= threads) a.split *= 2;
   a.max_nsd = max_nsd;
   return a;
 }
@@ -818,6 +823,7 @@ struct ddh_batch {
   int waves = 2, unroll = 1, use_graph = 1, poll = 8;
   int limit = 0;  // largest iteration limit over the intervals
   size_t solve_lds = 0;
+  int solve_threads = 0;
   std::vector<ddh::IterArgs> args;
   std::vector<ddh::TileGroup> groups;  // tiles of one direction count each
   std::vector<ddh::BatchTile> tiles, items, pro_items;
@@ -831,6 +837,7 @@ struct ddh_batch {
   int profile = 0;
   std::vector<hipEvent_t> events;
   double last_correlate_ms = 0.0;
+  double last_solve_ms = 0.0;  // antenna solve + decision
   int64_t last_correlate_launches = 0;
   // learned warm start: argument variants and per-interval solution stages
   ddh::DeviceBuffer<ddh::IterArgs> d_args_step, d_args_aux;
@@ -857,6 +864,7 @@ void BuildBatch(ddh_batch& b) {
   b.d_states.Resize(n);
   b.limit = 0;
   b.solve_lds = 0;
+  b.solve_threads = 0;
   struct Costed {
     BatchTile t;
     int n_dir;
@@ -873,6 +881,7 @@ void BuildBatch(ddh_batch& b) {
     b.args.push_back(a);
     b.limit = std::max(b.limit, std::max(s.cfg.c.max_iterations, s.cfg.c.min_iterations));
     b.solve_lds = std::max(b.solve_lds, AntennaSolveSharedBytes(a, p.n_corr));
+    b.solve_threads = std::max(b.solve_threads, AntennaSolveThreads(a.n_ant, a.max_nsd));
     const double vis_bytes = double(p.DataVisBytes() + p.n_dir * p.ModelVisBytes());
     for (int cb = 0; cb < p.n_cb; ++cb) {
       for (int seg = 0; seg < p.n_seg; ++seg) {
@@ -911,10 +920,11 @@ void BuildBatch(ddh_batch& b) {
 void LaunchBatchStep(ddh_batch& b, const IterArgs* args, int n_items, int n_intervals) {
   if (b.gram)
     LaunchBatchApply(args, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), b.d_items.ptr,
-                     n_items, n_intervals, b.solve_lds, b.f16, b.stream);
+                     n_items, n_intervals, b.solve_lds, b.solve_threads, b.f16, b.stream);
   else
     LaunchBatch(args, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), b.d_items.ptr,
-                n_items, n_intervals, b.solve_lds, b.n_corr, b.f16, b.waves, b.unroll, b.stream);
+                n_items, n_intervals, b.solve_lds, b.solve_threads, b.n_corr, b.f16, b.waves,
+                b.unroll, b.stream);
 }
 
 // Gram iteration: the cross products of every interval (shared by all
@@ -1071,9 +1081,9 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
   const int n = int(b.solvers.size());
   const bool graph = b.use_graph != 0 && !b.profile;
   if (graph) EnsureBatchGraph(b);
-  if (b.profile && int(b.events.size()) < 2 * b.limit) {
+  if (b.profile && int(b.events.size()) < 3 * b.limit) {
     for (hipEvent_t e : b.events) (void)hipEventDestroy(e);
-    b.events.assign(2 * b.limit, nullptr);
+    b.events.assign(3 * b.limit, nullptr);
     for (hipEvent_t& e : b.events) DDH_HIP_CHECK(hipEventCreate(&e));
   }
   hipEvent_t t0, t1;
@@ -1093,12 +1103,14 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
       for (int i = 0; i < k; ++i) {
         const int it = launched + i;
         if (b.profile) {
-          // correlate alone between the events, then the rest of the iteration
-          DDH_HIP_CHECK(hipEventRecord(b.events[2 * it], b.stream));
+          // correlate alone between the first two events, antenna solve and
+          // decision between the last two
+          DDH_HIP_CHECK(hipEventRecord(b.events[3 * it], b.stream));
           LaunchBatchStep(b, b.d_args.ptr, 0, 0);
-          DDH_HIP_CHECK(hipEventRecord(b.events[2 * it + 1], b.stream));
+          DDH_HIP_CHECK(hipEventRecord(b.events[3 * it + 1], b.stream));
           LaunchBatchTail(b.d_args.ptr, b.d_items.ptr, int(b.items.size()), int(b.solvers.size()),
-                          b.solve_lds, b.n_corr, b.f16, b.stream);
+                          b.solve_lds, b.solve_threads, b.n_corr, b.f16, b.stream);
+          DDH_HIP_CHECK(hipEventRecord(b.events[3 * it + 2], b.stream));
         } else {
           LaunchBatchIteration(b);
         }
@@ -1135,15 +1147,18 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
   DDH_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
   b.last_ms = ms;
   b.last_correlate_ms = 0.0;
+  b.last_solve_ms = 0.0;
   b.last_correlate_launches = 0;
   if (b.profile) {
     // launches after the last interval stopped do no work: count those that did
     int worked = 0;
     for (int i = 0; i < n; ++i) worked = std::max(worked, b.host_states[i].iteration);
     for (int it = 0; it < worked; ++it) {
-      float e = 0.0f;
-      DDH_HIP_CHECK(hipEventElapsedTime(&e, b.events[2 * it], b.events[2 * it + 1]));
+      float e = 0.0f, f = 0.0f;
+      DDH_HIP_CHECK(hipEventElapsedTime(&e, b.events[3 * it], b.events[3 * it + 1]));
+      DDH_HIP_CHECK(hipEventElapsedTime(&f, b.events[3 * it + 1], b.events[3 * it + 2]));
       b.last_correlate_ms += e;
+      b.last_solve_ms += f;
     }
     b.last_correlate_launches = worked;
   }
@@ -1312,9 +1327,11 @@ int ddh_batch_set_profiling(ddh_batch* b, int32_t enable) {
   return 0;
 }
 
-int ddh_batch_last_timing(const ddh_batch* b, double* correlate_ms, int64_t* launches) {
+int ddh_batch_last_timing(const ddh_batch* b, double* correlate_ms, double* solve_ms,
+                          int64_t* launches) {
   if (!b) return -1;
   if (correlate_ms) *correlate_ms = b->last_correlate_ms;
+  if (solve_ms) *solve_ms = b->last_solve_ms;
   if (launches) *launches = b->last_correlate_launches;
   return 0;
 }

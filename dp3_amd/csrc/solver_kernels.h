@@ -70,7 +70,7 @@ struct IterArgs {
   const int32_t* ant1;       // [bl_stride]
   const int32_t* ant2;
   const int32_t* csr_off;    // [n_ant+1]
-  const int32_t* csr;        // (baseline << 1) | side
+  const int32_t* csr;        // per entry: (baseline << 1) | side, other antenna
   const CbInfo* cbinfo;      // [n_cb]
   const int32_t* seg_t0;     // [n_seg+1]
   const int32_t* seg_sol;    // [n_seg][n_dir]
@@ -130,6 +130,8 @@ int Chi2Partials(const IterArgs& a);
 size_t CorrelateSharedBytes(int n_dir, int n_corr, int waves);
 size_t MaxSharedBytes();
 size_t AntennaSolveSharedBytes(const IterArgs& a, int n_corr);
+// Block size of antenna_solve: about four threads per (antenna, solution).
+int AntennaSolveThreads(int n_ant, int max_nsd);
 void PrepareKernelAttributes();
 // Tiles of one direction count: entries [first, first + count) of a tile list.
 struct TileGroup {
@@ -142,7 +144,7 @@ struct TileGroup {
 // counts skip a kernel).
 void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
                  int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
-                 size_t solve_lds, int n_corr, bool f16, int waves, int rows,
+                 size_t solve_lds, int solve_threads, int n_corr, bool f16, int waves, int rows,
                  hipStream_t stream);
 // Gram iteration: the cross products once per solve, then per iteration the
 // correlate partials from them and the current gains.
@@ -154,7 +156,7 @@ void LaunchBatchGram(const IterArgs* d_args, const BatchTile* d_tiles, const Til
 // One batched Gram iteration: apply per tile group, then antenna solve and decision.
 void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
                       int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
-                      size_t solve_lds, bool f16, hipStream_t stream);
+                      size_t solve_lds, int solve_threads, bool f16, hipStream_t stream);
 // Current solutions (by iteration parity) of batched interval `interval`
 // into dst, `count` complex values.
 void LaunchSnapshot(const IterArgs* d_args, int interval, int64_t count, double2* dst,
@@ -162,7 +164,7 @@ void LaunchSnapshot(const IterArgs* d_args, int interval, int64_t count, double2
 void LaunchBatchPrologue(const IterArgs* d_args, const BatchTile* d_items, int n_items,
                          hipStream_t stream);
 void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_items,
-                     int n_intervals, size_t solve_lds, int n_corr, bool f16,
+                     int n_intervals, size_t solve_lds, int solve_threads, int n_corr, bool f16,
                      hipStream_t stream);
 
 // Layout conversion: reference order [plane][t][bl][ch][n_corr] complex64 ->

@@ -675,28 +675,27 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
   const double sc = scale;
   const double sc2 = sc * sc;
   const int bs = a.bl_stride;
-  // LDS: partials of one segment, baseline antennas, per-antenna baseline
-  // lists and the gains of this direction's solutions as the correlate pass
-  // used them (repaired, then cast to float).
+  // LDS: the gains of this direction's solutions as the correlate pass used
+  // them (repaired, then cast to float) and the partials of one segment. The
+  // per-antenna baseline lists are read from global memory (cached): each
+  // entry holds (baseline << 1 | side, other antenna).
   extern __shared__ double smem_d[];
   double2* g_s = reinterpret_cast<double2*>(smem_d);             // [n_ant][nsd][2]
   float* part_s = reinterpret_cast<float*>(g_s + size_t(a.n_ant) * nsd * 2);  // [NC][3][bs]
-  int* a1_s = reinterpret_cast<int*>(part_s + NC * 3 * bs);      // [bs]
-  int* a2_s = a1_s + bs;                                         // [bs]
-  int* csr_s = a2_s + bs;                                        // [2 n_bl], padded to 4
-  int* off_s = csr_s + ((2 * a.n_bl + 3) & ~3);                  // [n_ant + 1]
   for (int i = threadIdx.x; i < a.n_ant * nsd; i += blockDim.x) {
     const int an = i / nsd;
     const size_t pair = gbase + size_t(an) * a.n_sub + s0 + (i - an * nsd);
-    double2 g0 = cur[pair * 2], g1 = cur[pair * 2 + 1];
+    double2 g0 = LoadGlobal(cur + pair * 2), g1 = LoadGlobal(cur + pair * 2 + 1);
     if (!(cfinite(g0) && cfinite(g1))) g0 = g1 = make_double2(mean, 0.0);
     g_s[2 * i] = make_double2(double(float(g0.x)), double(float(g0.y)));
     g_s[2 * i + 1] = make_double2(double(float(g1.x)), double(float(g1.y)));
   }
-  StageWords(a.ant1, a1_s, bs);
-  StageWords(a.ant2, a2_s, bs);
-  StageWords(a.csr, csr_s, 2 * a.n_bl);
-  for (int i = threadIdx.x; i <= a.n_ant; i += blockDim.x) off_s[i] = a.csr_off[i];
+  int e_begin = 0, e_end = 0;
+  if (active) {
+    e_begin = AsGlobal(a.csr_off)[ant];
+    e_end = AsGlobal(a.csr_off)[ant + 1];
+  }
+  const GlobalT<int32_t>* csr = AsGlobal(a.csr);
   const int rel = s_abs - s0;
   auto gain = [&](int an, int pol) { return g_s[(an * nsd + rel) * 2 + pol]; };
 
@@ -705,18 +704,18 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
   const double2 gant[2] = {active ? gain(ant, 0) : make_double2(0, 0),
                            active ? gain(ant, 1) : make_double2(0, 0)};
   for (int seg = 0; seg < a.n_seg; ++seg) {
-    const int s_seg = a.seg_sol[seg * a.n_dir + d];
+    const int s_seg = AsConst(a.seg_sol)[seg * a.n_dir + d];
     const float* src =
         a.partials + ((size_t(cb) * a.n_seg + seg) * a.n_dir + d) * NC * 3 * size_t(bs);
-    __syncthreads();
+    if (seg > 0) __syncthreads();
     StageWords(src, part_s, NC * 3 * bs);
     __syncthreads();
     if (!active || s_seg != s_abs) continue;
-    for (int e = off_s[ant] + q; e < off_s[ant + 1]; e += split) {
-      const int code = csr_s[e];
+    for (int e = e_begin + q; e < e_end; e += split) {
+      const int code = csr[2 * e];
+      const int other = csr[2 * e + 1];
       const int b = code >> 1;
       const int side = code & 1;
-      const int other = side ? a1_s[b] : a2_s[b];
       const double2 go[2] = {gain(other, 0), gain(other, 1)};
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
@@ -1211,7 +1210,7 @@ void LaunchBatchGram(const IterArgs* d_args, const BatchTile* d_tiles, const Til
 
 void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
                       int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
-                      size_t solve_lds, bool f16, hipStream_t stream) {
+                      size_t solve_lds, int solve_threads, bool f16, hipStream_t stream) {
   for (int g = 0; g < n_groups; ++g) {
     if (groups[g].count == 0) continue;
     CheckGramDirections(groups[g].n_dir);
@@ -1219,7 +1218,8 @@ void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const Ti
                        dim3(kWave), 0, stream, d_args, d_tiles + groups[g].first);
   }
   DDH_HIP_CHECK(hipGetLastError());
-  LaunchBatchTail(d_args, d_items, n_items, n_intervals, solve_lds, 2, f16, stream);
+  LaunchBatchTail(d_args, d_items, n_items, n_intervals, solve_lds, solve_threads, 2, f16,
+                  stream);
 }
 
 size_t CorrelateSharedBytes(int n_dir, int n_corr, int waves) {
@@ -1280,7 +1280,7 @@ void PrepareKernelAttributes() {
 
 void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
                  int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
-                 size_t solve_lds, int n_corr, bool f16, int waves, int rows,
+                 size_t solve_lds, int solve_threads, int n_corr, bool f16, int waves, int rows,
                  hipStream_t stream) {
   DDH_REQUIRE(waves >= 1 && waves * kWave <= kCorrelateMaxThreads, -1, "bad waves per tile");
   for (int g = 0; g < n_groups; ++g) {
@@ -1292,25 +1292,28 @@ void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, const TileGro
                        dim3(waves * kWave), lds, stream, d_args, d_tiles + grp.first);
   }
   DDH_HIP_CHECK(hipGetLastError());
-  LaunchBatchTail(d_args, d_items, n_items, n_intervals, solve_lds, n_corr, f16, stream);
+  LaunchBatchTail(d_args, d_items, n_items, n_intervals, solve_lds, solve_threads, n_corr, f16,
+                  stream);
 }
 
 void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_items,
-                     int n_intervals, size_t solve_lds, int n_corr, bool f16,
+                     int n_intervals, size_t solve_lds, int solve_threads, int n_corr, bool f16,
                      hipStream_t stream) {
+  DDH_REQUIRE(solve_threads >= kWave && solve_threads <= kBlockThreads, -1,
+              "bad antenna solve block size");
   // antenna solve + decide only
   DDH_REQUIRE(solve_lds <= size_t(g_max_lds_solve), -4, "batch antenna solve LDS exceeds the device");
   if (n_items > 0) {
     if (n_corr == 2) {
       if (f16)
-        hipLaunchKernelGGL((BatchAntennaSolveKernel<2, true>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
+        hipLaunchKernelGGL((BatchAntennaSolveKernel<2, true>), dim3(n_items), dim3(solve_threads), solve_lds, stream, d_args, d_items);
       else
-        hipLaunchKernelGGL((BatchAntennaSolveKernel<2, false>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
+        hipLaunchKernelGGL((BatchAntennaSolveKernel<2, false>), dim3(n_items), dim3(solve_threads), solve_lds, stream, d_args, d_items);
     } else {
       if (f16)
-        hipLaunchKernelGGL((BatchAntennaSolveKernel<4, true>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
+        hipLaunchKernelGGL((BatchAntennaSolveKernel<4, true>), dim3(n_items), dim3(solve_threads), solve_lds, stream, d_args, d_items);
       else
-        hipLaunchKernelGGL((BatchAntennaSolveKernel<4, false>), dim3(n_items), dim3(kBlockThreads), solve_lds, stream, d_args, d_items);
+        hipLaunchKernelGGL((BatchAntennaSolveKernel<4, false>), dim3(n_items), dim3(solve_threads), solve_lds, stream, d_args, d_items);
     }
   }
   if (n_intervals > 0)
@@ -1347,25 +1350,31 @@ void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, int row
 size_t AntennaSolveSharedBytes(const IterArgs& a, int n_corr) {
   const int max_nsd = std::max(1, a.max_nsd);
   return size_t(a.n_ant) * max_nsd * 2 * sizeof(double2) +
-         size_t(n_corr) * 3 * a.bl_stride * sizeof(float) +
-         (size_t(2) * a.bl_stride + ((2 * size_t(a.n_bl) + 3) & ~size_t(3)) + a.n_ant + 1) *
-             sizeof(int);
+         size_t(n_corr) * 3 * a.bl_stride * sizeof(float);
+}
+
+int AntennaSolveThreads(int n_ant, int max_nsd) {
+  // about four threads per (antenna, solution), whole waves, at most 1024
+  const int items = n_ant * std::max(1, max_nsd);
+  const int t = ((items * 4 + kWave - 1) / kWave) * kWave;
+  return std::min(kBlockThreads, std::max(kWave, t));
 }
 
 void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, hipStream_t stream) {
   const size_t lds = AntennaSolveSharedBytes(a, n_corr);
+  const dim3 block(AntennaSolveThreads(a.n_ant, a.max_nsd));
   DDH_REQUIRE(lds <= size_t(g_max_lds_solve), -4, "too many baselines for the antenna solve LDS");
   const dim3 grid(a.n_dir, a.n_cb);
   if (n_corr == 2) {
     if (f16)
-      hipLaunchKernelGGL((AntennaSolveKernel<2, true>), grid, dim3(kBlockThreads), lds, stream, a);
+      hipLaunchKernelGGL((AntennaSolveKernel<2, true>), grid, block, lds, stream, a);
     else
-      hipLaunchKernelGGL((AntennaSolveKernel<2, false>), grid, dim3(kBlockThreads), lds, stream, a);
+      hipLaunchKernelGGL((AntennaSolveKernel<2, false>), grid, block, lds, stream, a);
   } else {
     if (f16)
-      hipLaunchKernelGGL((AntennaSolveKernel<4, true>), grid, dim3(kBlockThreads), lds, stream, a);
+      hipLaunchKernelGGL((AntennaSolveKernel<4, true>), grid, block, lds, stream, a);
     else
-      hipLaunchKernelGGL((AntennaSolveKernel<4, false>), grid, dim3(kBlockThreads), lds, stream, a);
+      hipLaunchKernelGGL((AntennaSolveKernel<4, false>), grid, block, lds, stream, a);
   }
   DDH_HIP_CHECK(hipGetLastError());
 }

@@ -134,7 +134,8 @@ SIGNATURES = {
     "ddh_batch_last_warm_time": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
     "ddh_batch_last_time": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
     "ddh_batch_set_profiling": ([C.c_void_p, C.c_int32], C.c_int),
-    "ddh_batch_last_timing": ([C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
+    "ddh_batch_last_timing": ([C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                               C.POINTER(C.c_int64)],
                               C.c_int),
     "ddh_autotune": ([C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                       C.POINTER(C.c_double)], C.c_int),
@@ -509,10 +510,12 @@ class Batch:
         _check(lib().ddh_batch_set_profiling(self.handle, int(enable)), "ddh_batch_set_profiling")
 
     def last_timing(self):
-        ms, n = C.c_double(), C.c_int64()
-        _check(lib().ddh_batch_last_timing(self.handle, C.byref(ms), C.byref(n)),
+        """(correlate ms, antenna solve + decision ms, iterations that did work)
+        of the last profiled solve."""
+        ms, sm, n = C.c_double(), C.c_double(), C.c_int64()
+        _check(lib().ddh_batch_last_timing(self.handle, C.byref(ms), C.byref(sm), C.byref(n)),
                "ddh_batch_last_timing")
-        return ms.value, n.value
+        return ms.value, sm.value, n.value
 
     def close(self):
         if getattr(self, "handle", None):

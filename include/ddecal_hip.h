@@ -237,11 +237,13 @@ int ddh_batch_solve_warm(ddh_batch* batch, ddh_mlp* net, int32_t stretch,
 // Device time of the warm-start phase of the last ddh_batch_solve_warm
 // (ddh_batch_last_time covers the solve from the learned start).
 int ddh_batch_last_warm_time(const ddh_batch* batch, double* ms);
-// HIP events around every batched correlate launch of the next solves (graph
-// replay off while enabled); ddh_batch_last_timing reports their summed
-// duration and the number of launches that did work.
+// HIP events around every batched correlate launch (Gram iteration: apply)
+// and every antenna solve + decision of the next solves (graph replay off
+// while enabled); ddh_batch_last_timing reports their summed durations and
+// the number of iterations that did work.
 int ddh_batch_set_profiling(ddh_batch* batch, int32_t enable);
-int ddh_batch_last_timing(const ddh_batch* batch, double* correlate_ms, int64_t* launches);
+int ddh_batch_last_timing(const ddh_batch* batch, double* correlate_ms, double* solve_ms,
+                          int64_t* launches);
 
 // Schedule autotuner: times the correlate kernel under every candidate
 // schedule (waves per tile x rows per trip) on the actual shapes and keeps
