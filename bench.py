@@ -328,6 +328,31 @@ def main():
     p_avg = pms / max(pn, 1)
     p_achieved = algorithmic_bytes(w, 16) / (p_avg * 1e-3) / 1e9
 
+    # ---- the step after the solve, DDECal::CorrectAndSubtractModels, on one
+    # interval of the same shape in DPBuffer order (autocorrelations included)
+    n_bl_all = w.n_stations * (w.n_stations + 1) // 2
+    a1 = np.array([a for a in range(w.n_stations) for b in range(a, w.n_stations)], np.int32)
+    a2 = np.array([b for a in range(w.n_stations) for b in range(a, w.n_stations)], np.int32)
+    rng = np.random.default_rng(args.seed)
+    vis_shape = (w.solint, n_bl_all, w.n_channels, 4)
+    sub_data = rng.standard_normal(vis_shape + (2,), dtype=np.float32).view(np.complex64)[..., 0]
+    sub_model = rng.standard_normal((w.n_directions,) + vis_shape + (2,),
+                                    dtype=np.float32).view(np.complex64)[..., 0]
+    sub_sol = np.exp(1j * rng.uniform(-np.pi, np.pi, (w.n_channel_blocks, w.n_stations,
+                                                       w.n_directions, 2)))
+    cbs = S.channel_begin(w.n_channels, w.n_channel_blocks).astype(np.int32)
+    for _ in range(2):  # the second call is timed
+        _, _, sub_ms = H.correct_and_subtract(sub_data, sub_model, sub_sol, a1, a2, cbs,
+                                              w.n_stations, subtract=True, keep_model=False)
+    del sub_data, sub_model
+    sub_vis = w.solint * n_bl_all * w.n_channels
+    components = {"correct_and_subtract": roof(
+        "correct_and_subtract", sub_vis * (64 + w.n_directions * 32), sub_ms,
+        note=(f"DDECal::CorrectAndSubtractModels over one interval: {sub_vis} visibilities "
+              "(autocorrelations included), 4 correlations, diagonal solutions, subtract; "
+              "data read and written, every model read"))}
+    log(f"correct and subtract: {sub_ms * 1e3:.1f} us per interval")
+
     v_plain = results["plain"][0]
     p_roof = roof("correlate_c64", algorithmic_bytes(w, 16), p_avg)
     line = {
@@ -386,6 +411,7 @@ def main():
         "roofline": rooflines[dominant],
         "roofline_plain": p_roof,
         "roofline_other": {k: v for k, v in rooflines.items() if k != dominant},
+        "components": components,
     }
     if args.calibrate:
         copy_gbs, read_gbs = H.calibrate_bandwidth(1 << 30, 10)

@@ -17,7 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libddecal_hip.so")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["solver_kernels.hip", "solver_host.hip", "predict.hip", "calibrate.hip", "mlp.hip"]
+SOURCES = ["solver_kernels.hip", "solver_host.hip", "predict.hip", "calibrate.hip", "mlp.hip",
+           "subtract.hip"]
 HEADERS = ["common.h", "solver_kernels.h", "warm.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

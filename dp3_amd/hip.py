@@ -129,6 +129,10 @@ SIGNATURES = {
                                C.c_int),
     "ddh_batch_solve": ([C.c_void_p, C.c_void_p, C.POINTER(Result)], C.c_int),
     "ddh_batch_rebind": ([C.c_void_p, C.c_void_p], C.c_int),
+    "ddh_correct_and_subtract_device": ([C.c_int32] * 7 + [C.c_void_p] * 6 + [C.c_int32] * 2
+                                        + [C.c_void_p], C.c_int),
+    "ddh_correct_and_subtract": ([C.c_int32] * 7 + [C.c_void_p] * 6 + [C.c_int32] * 2
+                                 + [C.POINTER(C.c_double)], C.c_int),
     "ddh_batch_solve_warm": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(Result)],
                              C.c_int),
     "ddh_batch_last_warm_time": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
@@ -623,3 +627,24 @@ class DeviceProblem:
 
     def __del__(self):
         self.close()
+
+
+def correct_and_subtract(data, model, solutions, antenna1, antenna2, channel_block_start,
+                         n_antennas, subtract=True, keep_model=False):
+    """DDECal::CorrectAndSubtractModels on the GPU for host arrays: data
+    [n_t, n_bl, n_ch, 4] and model [n_dir, n_t, n_bl, n_ch, 4] complex64,
+    solutions [n_cb, n_ant, n_dir, n_pol] complex128. Returns (data, model,
+    kernel ms) with the outputs as new arrays."""
+    d = np.ascontiguousarray(data, dtype=np.complex64).copy()
+    m = np.ascontiguousarray(model, dtype=np.complex64).copy()
+    s = np.ascontiguousarray(solutions, dtype=np.complex128)
+    n_dir, n_t, n_bl, n_ch, _ = m.shape
+    a1 = np.ascontiguousarray(antenna1, dtype=np.int32)
+    a2 = np.ascontiguousarray(antenna2, dtype=np.int32)
+    cbs = np.ascontiguousarray(channel_block_start, dtype=np.int32)
+    ms = C.c_double()
+    _check(lib().ddh_correct_and_subtract(n_t, n_bl, n_ch, n_dir, n_antennas, s.shape[0],
+                                          s.shape[-1], _ptr(a1), _ptr(a2), _ptr(cbs), _ptr(s),
+                                          _ptr(d), _ptr(m), int(subtract), int(keep_model),
+                                          C.byref(ms)), "ddh_correct_and_subtract")
+    return d, m, ms.value

@@ -253,6 +253,35 @@ int ddh_autotune(ddh_solver* solver, ddh_problem* problem, int32_t* waves_per_ti
 int ddh_batch_autotune(ddh_batch* batch, int32_t* waves_per_tile, int32_t* rows_per_trip,
                        double* best_ms);
 
+// Applies the solutions of an interval to the model data of every direction
+// and subtracts the sum from the data: DDECal::CorrectAndSubtractModels
+// (steps/DDECal.cc:1074-1124, ApplySolutionAndSubtract :1031-1072, the 2x2
+// products of :992-1020 in FP64). Buffers in DPBuffer order over the time
+// steps of the interval, unweighted:
+//   data  [n_times][n_baselines][n_channels][4] complex64 (data minus the
+//         corrected models when subtract != 0)
+//   model [n_directions][n_times][n_baselines][n_channels][4] complex64
+//         (replaced by the corrected models when keep_model != 0)
+//   solutions [n_channel_blocks][n_antennas][n_directions][n_pol] complex128,
+//         n_pol 1 (scalar), 2 (diagonal) or 4 (full Jones)
+//   channel_block_start [n_channel_blocks + 1], antenna1/2 [n_baselines]
+// The _device form takes device pointers for everything and enqueues on
+// `stream` (a hipStream_t, NULL for the default stream); the host form copies
+// host arrays in and out and reports the kernel's device time.
+int ddh_correct_and_subtract_device(int32_t n_times, int32_t n_baselines, int32_t n_channels,
+                                    int32_t n_directions, int32_t n_antennas,
+                                    int32_t n_channel_blocks, int32_t n_pol,
+                                    const int32_t* antenna1, const int32_t* antenna2,
+                                    const int32_t* channel_block_start, const double* solutions,
+                                    float* data, float* model, int32_t subtract,
+                                    int32_t keep_model, void* stream);
+int ddh_correct_and_subtract(int32_t n_times, int32_t n_baselines, int32_t n_channels,
+                             int32_t n_directions, int32_t n_antennas, int32_t n_channel_blocks,
+                             int32_t n_pol, const int32_t* antenna1, const int32_t* antenna2,
+                             const int32_t* channel_block_start, const double* solutions,
+                             float* data, float* model, int32_t subtract, int32_t keep_model,
+                             double* kernel_ms);
+
 // Warm-start network (no counterpart in the reference): a 3-layer
 // perceptron with bf16 weights on the GPU (bf16 MFMA GEMMs, FP32
 // accumulation, FP32 master weights, Adam). Parameters are one flat FP32

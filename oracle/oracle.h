@@ -17,6 +17,7 @@
 //   base/PointSource.cc:51-99                          (Stokes spectrum)
 //   steps/OnePredict.cc:437-457                        (copy to DPBuffer)
 //   ddecal/test/unit/SolverTester.cc:95-202,348-363    (test fixture data)
+//   steps/DDECal.cc:992-1124                    (correct and subtract models)
 //
 // All entry points are extern "C" with plain pointers so that Python (ctypes)
 // can drive them. Complex numbers are interleaved (re, im).
@@ -148,6 +149,19 @@ void orc_solvertester_shape(int32_t* n_ant, int32_t* n_dir, int32_t* n_ch,
                             int32_t* n_t, int32_t* n_cb);
 void orc_solvertester_fill(const int32_t* n_solutions_per_direction,
                            float* data, float* model, double* input_solutions);
+
+// Restatement of DDECal::CorrectAndSubtractModels (steps/DDECal.cc:992-1124)
+// over one solution interval: data [n_t][n_bl][n_ch][4] and model
+// [n_dir][n_t][n_bl][n_ch][4] complex64 (unweighted), solutions
+// [n_cb][n_ant][n_dir][n_pol] complex128 (n_pol 1 scalar, 2 diagonal,
+// 4 full Jones), channel_block_start [n_cb + 1]. With subtract the corrected
+// models are subtracted from the data; with keep_model they replace the
+// model data.
+void orc_correct_and_subtract(int32_t n_t, int32_t n_bl, int32_t n_ch, int32_t n_dir,
+                              int32_t n_ant, int32_t n_cb, int32_t n_pol,
+                              const int32_t* antenna1, const int32_t* antenna2,
+                              const int32_t* channel_block_start, const double* solutions,
+                              float* data, float* model, int32_t subtract, int32_t keep_model);
 
 #ifdef __cplusplus
 }

@@ -90,6 +90,8 @@ def lib():
         L.orc_simulate.restype = None
         L.orc_solvertester_shape.argtypes = [P] * 5
         L.orc_solvertester_fill.argtypes = [P] * 4
+        L.orc_correct_and_subtract.argtypes = [C.c_int32] * 7 + [P] * 6 + [C.c_int32] * 2
+        L.orc_correct_and_subtract.restype = None
         _LIB = L
     return _LIB
 
@@ -259,3 +261,22 @@ def solvertester_fill(n_sol_per_dir):
     inp = np.zeros(n_ant * n_sub * 2, np.complex128)
     lib().orc_solvertester_fill(_p(ns), _p(data), _p(model), _p(inp))
     return data, model, inp
+
+
+def correct_and_subtract(data, model, solutions, antenna1, antenna2, channel_block_start,
+                         n_ant, subtract=True, keep_model=False):
+    """DDECal::CorrectAndSubtractModels over one interval, in place on copies:
+    data [n_t, n_bl, n_ch, 4] and model [n_dir, n_t, n_bl, n_ch, 4] complex64,
+    solutions [n_cb, n_ant, n_dir, n_pol] complex128. Returns (data, model)."""
+    d = np.ascontiguousarray(data, dtype=np.complex64).copy()
+    m = np.ascontiguousarray(model, dtype=np.complex64).copy()
+    s = np.ascontiguousarray(solutions, dtype=np.complex128)
+    n_dir, n_t, n_bl, n_ch, _ = m.shape
+    n_cb = s.shape[0]
+    n_pol = s.shape[-1]
+    a1 = np.ascontiguousarray(antenna1, dtype=np.int32)
+    a2 = np.ascontiguousarray(antenna2, dtype=np.int32)
+    cbs = np.ascontiguousarray(channel_block_start, dtype=np.int32)
+    lib().orc_correct_and_subtract(n_t, n_bl, n_ch, n_dir, n_ant, n_cb, n_pol, _p(a1), _p(a2),
+                                   _p(cbs), _p(s), _p(d), _p(m), int(subtract), int(keep_model))
+    return d, m

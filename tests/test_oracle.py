@@ -345,3 +345,68 @@ def test_bf16_rounding_is_round_to_nearest_even():
     r = MO.bf16(x)
     # 1 + 2^-9 is a tie between 1 and 1 + 2^-7 -> even (1.0); 1 + 3*2^-9 rounds up
     np.testing.assert_array_equal(r[:5], [1.0, 1.0, 1.0 + 2 ** -7, 1.0, -2.5])
+
+
+# ---- correct and subtract (DDECal.cc:992-1124) ----------------------------
+
+def _subtract_case(n_pol, seed=3):
+    rng = np.random.default_rng(seed)
+    n_t, n_ant, n_ch, n_dir, n_cb = 2, 5, 6, 3, 2
+    bl = [(a, b) for a in range(n_ant) for b in range(a, n_ant)]  # with autocorrelations
+    a1 = np.array([x for x, _ in bl], np.int32)
+    a2 = np.array([y for _, y in bl], np.int32)
+    n_bl = len(bl)
+    cplx = lambda *s: (rng.normal(size=s) + 1j * rng.normal(size=s))
+    data = cplx(n_t, n_bl, n_ch, 4).astype(np.complex64)
+    model = cplx(n_dir, n_t, n_bl, n_ch, 4).astype(np.complex64)
+    sol = cplx(n_cb, n_ant, n_dir, n_pol)
+    cbs = np.array([0, 4, n_ch], np.int32)  # uneven blocks
+    return data, model, sol, a1, a2, cbs, n_ant
+
+
+@pytest.mark.parametrize("n_pol", [1, 2, 4])
+def test_correct_and_subtract_against_matrix_products(n_pol):
+    """The oracle equals sum_d S1 M_d S2^H per visibility, with S the scalar,
+    diagonal or full 2x2 solution of each antenna and the block of each
+    channel (closed form with numpy matrices)."""
+    data, model, sol, a1, a2, cbs, n_ant = _subtract_case(n_pol)
+    d_out, m_out = O.correct_and_subtract(data, model, sol, a1, a2, cbs, n_ant,
+                                          subtract=True, keep_model=True)
+    n_dir, n_t, n_bl, n_ch, _ = model.shape
+    cb_of = np.searchsorted(cbs, np.arange(n_ch), side="right") - 1
+
+    def mat(s):
+        if n_pol == 1:
+            return s[0] * np.eye(2)
+        if n_pol == 2:
+            return np.diag(s)
+        return s.reshape(2, 2)
+
+    ref_d = data.astype(np.complex128).copy()
+    for t in range(n_t):
+        for b in range(n_bl):
+            for c in range(n_ch):
+                tot = np.zeros((2, 2), complex)
+                for d in range(n_dir):
+                    s1 = mat(sol[cb_of[c], a1[b], d])
+                    s2 = mat(sol[cb_of[c], a2[b], d])
+                    corr = s1 @ model[d, t, b, c].astype(np.complex128).reshape(2, 2) @ s2.conj().T
+                    np.testing.assert_allclose(m_out[d, t, b, c], corr.ravel(), rtol=2e-6,
+                                               atol=2e-6)
+                    tot += corr
+                ref_d[t, b, c] -= tot.ravel()
+    np.testing.assert_allclose(d_out, ref_d, rtol=0, atol=2e-5)
+
+
+def test_correct_and_subtract_flags():
+    """Unit solutions subtract the plain model sum; subtract=False and
+    keep_model=False leave the inputs untouched."""
+    data, model, sol, a1, a2, cbs, n_ant = _subtract_case(2)
+    ones = np.ones_like(sol)
+    d_out, m_out = O.correct_and_subtract(data, model, ones, a1, a2, cbs, n_ant)
+    np.testing.assert_allclose(d_out, data - model.sum(axis=0), atol=1e-5)
+    np.testing.assert_array_equal(m_out, model)
+    d2, m2 = O.correct_and_subtract(data, model, sol, a1, a2, cbs, n_ant, subtract=False,
+                                    keep_model=False)
+    np.testing.assert_array_equal(d2, data)
+    np.testing.assert_array_equal(m2, model)
