@@ -344,14 +344,36 @@ def main():
     for _ in range(2):  # the second call is timed
         _, _, sub_ms = H.correct_and_subtract(sub_data, sub_model, sub_sol, a1, a2, cbs,
                                               w.n_stations, subtract=True, keep_model=False)
-    del sub_data, sub_model
+    # ---- the step before the solve, AssignAndWeight + SolveData packing, on
+    # the same raw buffers into an FP16 device problem (the optimized form's
+    # storage); flags on 1% of the correlations, uniform weights
+    sub_w = rng.uniform(0.5, 2.0, vis_shape).astype(np.float32)
+    sub_f = rng.random(vis_shape) < 0.01
+    cross = a1 != a2
+    ing = H.DeviceProblem(s16[0].solver, w.solint, a1[cross], a2[cross], np.diff(cbs),
+                          s16[0].sol_index)
+    for _ in range(2):  # the second call is timed
+        ing_ms = ing.assign_and_weight(sub_data, sub_model, a1, a2, flags=sub_f, weights=sub_w)
+    ing.close()
+    del sub_data, sub_model, sub_w, sub_f
     sub_vis = w.solint * n_bl_all * w.n_channels
-    components = {"correct_and_subtract": roof(
-        "correct_and_subtract", sub_vis * (64 + w.n_directions * 32), sub_ms,
-        note=(f"DDECal::CorrectAndSubtractModels over one interval: {sub_vis} visibilities "
-              "(autocorrelations included), 4 correlations, diagonal solutions, subtract; "
-              "data read and written, every model read"))}
-    log(f"correct and subtract: {sub_ms * 1e3:.1f} us per interval")
+    cross_vis = w.solint * int(cross.sum()) * w.n_channels
+    components = {
+        "correct_and_subtract": roof(
+            "correct_and_subtract", sub_vis * (64 + w.n_directions * 32), sub_ms,
+            note=(f"DDECal::CorrectAndSubtractModels over one interval: {sub_vis} visibilities "
+                  "(autocorrelations included), 4 correlations, diagonal solutions, subtract; "
+                  "data read and written, every model read")),
+        "assign_and_weight": roof(
+            "assign_and_weight",
+            sub_vis * (32 + 4 + 16 + w.n_directions * 32) + cross_vis * (16 + w.n_directions * 8),
+            ing_ms,
+            note=(f"AssignAndWeight + SolveData packing of one interval into an FP16 device "
+                  f"problem: {sub_vis} raw visibilities (data, flags, weights, every model "
+                  f"read once), {cross_vis} packed Duo visibilities written; two passes over "
+                  "the raw buffers (the first finds the FP16 scale)"))}
+    log(f"correct and subtract: {sub_ms * 1e3:.1f} us per interval; assign and weight "
+        f"(FP16 problem): {ing_ms * 1e3:.1f} us per interval")
 
     v_plain = results["plain"][0]
     p_roof = roof("correlate_c64", algorithmic_bytes(w, 16), p_avg)

@@ -18,7 +18,7 @@ OUT = os.path.join(HERE, "libddecal_hip.so")
 BUILD = os.path.join(HERE, "_build")
 
 SOURCES = ["solver_kernels.hip", "solver_host.hip", "predict.hip", "calibrate.hip", "mlp.hip",
-           "subtract.hip"]
+           "subtract.hip", "ingest.hip"]
 HEADERS = ["common.h", "solver_kernels.h", "warm.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -80,6 +80,12 @@ struct ddh_problem {
   ddh::DeviceBuffer<ddh::CbInfo> d_cbinfo;
   ddh::DeviceBuffer<float> staging;
   ddh::DeviceBuffer<uint32_t> max_bits;
+  // raw-buffer ingest: the DPBuffer baseline list the map was built for,
+  // DPBuffer baseline -> solve-data baseline (-1: auto-correlation), first
+  // channel per block, and staging of host buffers
+  std::vector<int32_t> ingest_a1, ingest_a2, ingest_cross, ingest_ch_start, ingest_ch_cb;
+  ddh::DeviceBuffer<int32_t> d_ingest_cross, d_ingest_ch_start, d_ingest_ch_cb;
+  ddh::DeviceBuffer<unsigned char> ingest_staging;
   hipStream_t stream = nullptr;
 
   size_t DataVisBytes() const { return size_t(n_corr) * 8; }
@@ -566,6 +572,73 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
 
 namespace ddh {
 
+// Maps the DPBuffer baseline list (auto-correlations included) of raw
+// interval buffers onto the problem's baselines: the cross-correlations must
+// be the problem's baselines in order (SolveData.cc:96-100, 129-137).
+void PrepareIngest(ddh_problem& p, int n_bl_all, const int32_t* a1, const int32_t* a2) {
+  DDH_REQUIRE(n_bl_all >= 1 && a1 && a2, -1, "empty baseline list");
+  if (p.ingest_a1.size() == size_t(n_bl_all) && std::equal(a1, a1 + n_bl_all, p.ingest_a1.begin()) &&
+      std::equal(a2, a2 + n_bl_all, p.ingest_a2.begin()))
+    return;
+  std::vector<int32_t> cross(n_bl_all, -1);
+  int k = 0;
+  for (int b = 0; b < n_bl_all; ++b) {
+    DDH_REQUIRE(a1[b] >= 0 && a1[b] < p.n_ant && a2[b] >= 0 && a2[b] < p.n_ant, -1,
+                "antenna index out of range");
+    if (a1[b] == a2[b]) continue;
+    DDH_REQUIRE(k < p.n_bl && p.ant1[k] == a1[b] && p.ant2[k] == a2[b], -2,
+                "the cross-correlations of the buffers differ from the problem's baselines");
+    cross[b] = k++;
+  }
+  DDH_REQUIRE(k == p.n_bl, -2, "the buffers hold fewer cross-correlations than the problem");
+  std::vector<int32_t> ch_start(p.n_cb + 1, 0), ch_cb;
+  for (int cb = 0; cb < p.n_cb; ++cb) {
+    ch_start[cb + 1] = ch_start[cb] + p.ch_per_cb[cb];
+    ch_cb.insert(ch_cb.end(), p.ch_per_cb[cb], cb);
+  }
+  p.ingest_cross = std::move(cross);
+  p.ingest_ch_start = std::move(ch_start);
+  p.ingest_ch_cb = std::move(ch_cb);
+  p.d_ingest_cross.Upload(p.ingest_cross, p.stream);
+  p.d_ingest_ch_start.Upload(p.ingest_ch_start, p.stream);
+  p.d_ingest_ch_cb.Upload(p.ingest_ch_cb, p.stream);
+  p.max_bits.Resize(size_t(p.n_cb) * p.n_dir);
+  DDH_HIP_CHECK(hipStreamSynchronize(p.stream));
+  p.ingest_a1.assign(a1, a1 + n_bl_all);
+  p.ingest_a2.assign(a2, a2 + n_bl_all);
+}
+
+void AssignAndWeighInto(ddh_problem& p, int n_bl_all, const int32_t* a1, const int32_t* a2,
+                        const float* data, const uint8_t* flags, const float* weights,
+                        const float* model, int linear, hipStream_t stream) {
+  DDH_REQUIRE(data && model, -1, "NULL data or model");
+  auto aligned = [](const void* ptr, uintptr_t a) { return (reinterpret_cast<uintptr_t>(ptr) % a) == 0; };
+  DDH_REQUIRE(aligned(data, 16) && aligned(model, 16) && aligned(weights, 16) && aligned(flags, 4),
+              -1, "data, model and weights must be 16-byte aligned, flags 4-byte aligned");
+  PrepareIngest(p, n_bl_all, a1, a2);
+  IngestArgs g{};
+  g.data = data;
+  g.flags = flags;
+  g.weights = weights;
+  g.model = model;
+  g.cross = p.d_ingest_cross.ptr;
+  g.ch_start = p.d_ingest_ch_start.ptr;
+  g.ch_cb = p.d_ingest_ch_cb.ptr;
+  g.cbinfo = p.d_cbinfo.ptr;
+  g.out_data = p.data.ptr;
+  g.out_model = p.model.ptr;
+  g.scale = p.scale.ptr;
+  g.max_bits = p.max_bits.ptr;
+  g.n_t = p.n_t;
+  g.n_bl_all = n_bl_all;
+  g.n_ch = p.ingest_ch_start.back();
+  g.n_cb = p.n_cb;
+  g.n_dir = p.n_dir;
+  g.bl_stride = p.bl_stride;
+  g.linear = linear ? 1 : 0;
+  LaunchAssignAndWeigh(g, p.n_corr, p.f16, stream);
+}
+
 ProblemView GetProblemView(ddh_problem* p) {
   ProblemView v;
   v.model = p->model.ptr;
@@ -695,6 +768,62 @@ ddh_problem* ddh_problem_create(const ddh_solver* s, int32_t n_times, int32_t n_
 }
 
 int ddh_problem_baseline_stride(const ddh_problem* p) { return p ? p->bl_stride : 0; }
+
+int ddh_problem_assign_and_weight_device(ddh_problem* p, int32_t n_baselines,
+                                         const int32_t* antenna1, const int32_t* antenna2,
+                                         const float* data, const uint8_t* flags,
+                                         const float* weights, const float* model,
+                                         int32_t linear_weighting, void* stream) {
+  return Guard([&] {
+    DDH_REQUIRE(p, -1, "NULL problem");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    AssignAndWeighInto(*p, n_baselines, antenna1, antenna2, data, flags, weights, model,
+                       linear_weighting, st);
+  });
+}
+
+int ddh_problem_assign_and_weight(ddh_problem* p, int32_t n_baselines, const int32_t* antenna1,
+                                  const int32_t* antenna2, const float* data,
+                                  const uint8_t* flags, const float* weights, const float* model,
+                                  int32_t linear_weighting, double* kernel_ms) {
+  return Guard([&] {
+    DDH_REQUIRE(p && data && model, -1, "NULL argument");
+    DDH_REQUIRE(n_baselines >= 1, -1, "empty baseline list");
+    const int n_ch = std::accumulate(p->ch_per_cb.begin(), p->ch_per_cb.end(), 0);
+    const size_t n_vis = size_t(p->n_t) * n_baselines * n_ch;
+    // staging: data, models, weights, flags (each 256-byte aligned)
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t b_data = up(n_vis * 32), b_model = up(n_vis * 32 * p->n_dir);
+    const size_t b_w = weights ? up(n_vis * 16) : 0, b_f = flags ? up(n_vis * 4) : 0;
+    p->ingest_staging.Resize(b_data + b_model + b_w + b_f);
+    unsigned char* base = p->ingest_staging.ptr;
+    float* d_data = reinterpret_cast<float*>(base);
+    float* d_model = reinterpret_cast<float*>(base + b_data);
+    float* d_w = weights ? reinterpret_cast<float*>(base + b_data + b_model) : nullptr;
+    uint8_t* d_f = flags ? base + b_data + b_model + b_w : nullptr;
+    DDH_HIP_CHECK(hipMemcpyAsync(d_data, data, n_vis * 32, hipMemcpyHostToDevice, p->stream));
+    DDH_HIP_CHECK(hipMemcpyAsync(d_model, model, n_vis * 32 * p->n_dir, hipMemcpyHostToDevice,
+                                 p->stream));
+    if (weights)
+      DDH_HIP_CHECK(hipMemcpyAsync(d_w, weights, n_vis * 16, hipMemcpyHostToDevice, p->stream));
+    if (flags)
+      DDH_HIP_CHECK(hipMemcpyAsync(d_f, flags, n_vis * 4, hipMemcpyHostToDevice, p->stream));
+    PrepareIngest(*p, n_baselines, antenna1, antenna2);
+    hipEvent_t e0, e1;
+    DDH_HIP_CHECK(hipEventCreate(&e0));
+    DDH_HIP_CHECK(hipEventCreate(&e1));
+    DDH_HIP_CHECK(hipEventRecord(e0, p->stream));
+    AssignAndWeighInto(*p, n_baselines, antenna1, antenna2, d_data, d_f, d_w, d_model,
+                       linear_weighting, p->stream);
+    DDH_HIP_CHECK(hipEventRecord(e1, p->stream));
+    DDH_HIP_CHECK(hipStreamSynchronize(p->stream));
+    float ms = 0.0f;
+    DDH_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (kernel_ms) *kernel_ms = ms;
+  });
+}
 
 int ddh_problem_upload(ddh_problem* p, const ddh_cb_view* cbs) {
   return Guard([&] {

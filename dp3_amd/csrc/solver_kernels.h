@@ -179,6 +179,25 @@ void LaunchPlaneMaxAbs(const float* src, int64_t plane_floats, int n_planes,
 void LaunchScaleFromMax(const uint32_t* max_bits, float* scale, int n,
                         hipStream_t stream);
 
+// Weighting and packing of raw interval buffers into a device problem
+// (AssignAndWeight + SolveData, defined in ingest.hip).
+struct IngestArgs {
+  const float* data;       // [n_t][n_bl_all][n_ch][4] complex64
+  const uint8_t* flags;    // [n_t][n_bl_all][n_ch][4] or null
+  const float* weights;    // [n_t][n_bl_all][n_ch][4] or null (unit weights)
+  const float* model;      // [n_dir][n_t][n_bl_all][n_ch][4] complex64
+  const int32_t* cross;    // [n_bl_all] solve-data baseline index, -1 for autos
+  const int32_t* ch_start; // [n_cb + 1] first channel of each block
+  const int32_t* ch_cb;    // [n_ch] block of each channel
+  const CbInfo* cbinfo;    // [n_cb] plane offsets of the problem
+  void* out_data;          // problem data planes
+  void* out_model;         // problem model planes
+  const float* scale;      // [n_cb][n_dir] FP16 scale (written by the first pass)
+  uint32_t* max_bits;      // [n_cb][n_dir] workspace of the FP16 maxima
+  int32_t n_t, n_bl_all, n_ch, n_cb, n_dir, bl_stride, linear;
+};
+void LaunchAssignAndWeigh(const IngestArgs& g, int n_corr, bool f16, hipStream_t stream);
+
 // Internals of a device problem, shared by the solver host code and the
 // predict (solver_host.hip owns the struct behind ddh_problem).
 struct ProblemView {

@@ -129,6 +129,10 @@ SIGNATURES = {
                                C.c_int),
     "ddh_batch_solve": ([C.c_void_p, C.c_void_p, C.POINTER(Result)], C.c_int),
     "ddh_batch_rebind": ([C.c_void_p, C.c_void_p], C.c_int),
+    "ddh_problem_assign_and_weight_device": ([C.c_void_p, C.c_int32] + [C.c_void_p] * 6
+                                             + [C.c_int32, C.c_void_p], C.c_int),
+    "ddh_problem_assign_and_weight": ([C.c_void_p, C.c_int32] + [C.c_void_p] * 6
+                                      + [C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "ddh_correct_and_subtract_device": ([C.c_int32] * 7 + [C.c_void_p] * 6 + [C.c_int32] * 2
                                         + [C.c_void_p], C.c_int),
     "ddh_correct_and_subtract": ([C.c_int32] * 7 + [C.c_void_p] * 6 + [C.c_int32] * 2
@@ -598,6 +602,32 @@ class DeviceProblem:
         _check(lib().ddh_problem_synthesize_data(self.handle, C.cast(ptrs, C.c_void_p), n_sub,
                                                  _ptr(si), float(noise_sigma), seed),
                "ddh_problem_synthesize_data")
+
+    def assign_and_weight(self, data, model, antenna1, antenna2, flags=None, weights=None,
+                          linear_weighting=False):
+        """Fills the problem from an interval's raw buffers (AssignAndWeight +
+        SolveData packing on the device): data [n_t, n_bl, n_ch, 4] and model
+        [n_dir, n_t, n_bl, n_ch, 4] complex64, flags (bool) and weights
+        (float32) [n_t, n_bl, n_ch, 4] or None, over every baseline of the
+        buffers with antenna1/antenna2 [n_bl]. Returns the kernels' device
+        time in ms."""
+        d = np.ascontiguousarray(data, dtype=np.complex64)
+        m = np.ascontiguousarray(model, dtype=np.complex64)
+        a1 = np.ascontiguousarray(antenna1, dtype=np.int32)
+        a2 = np.ascontiguousarray(antenna2, dtype=np.int32)
+        n_t, n_bl, n_ch, n_c = d.shape
+        assert n_t == self.n_times and n_c == 4 and m.shape[1:] == d.shape
+        assert len(a1) == n_bl and len(a2) == n_bl
+        f = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+        for x in (f, w):
+            assert x is None or x.shape == d.shape
+        ms = C.c_double()
+        _check(lib().ddh_problem_assign_and_weight(
+            self.handle, n_bl, _ptr(a1), _ptr(a2), _ptr(d), _ptr(f), _ptr(w), _ptr(m),
+            int(linear_weighting), C.byref(ms)),
+            "ddh_problem_assign_and_weight")
+        return ms.value
 
     def data_ptr(self, cb):
         return lib().ddh_problem_data_ptr(self.handle, cb)

@@ -132,6 +132,37 @@ int ddh_problem_set_model_scale(ddh_problem* problem, const float* scale /*[n_cb
 int ddh_problem_download_plane(ddh_problem* problem, int32_t cb, int32_t plane,
                                void* host);
 
+// Fills every data and model plane of `problem` from the raw buffers of its
+// interval: AssignAndWeight (ddecal/gain_solvers/SolverTools.cc:81-167,
+// linear_weighting_mode as linear_weighting; Weigh :48-79) followed by the
+// regular SolveData constructor (ddecal/gain_solvers/SolveData.cc:77-188), in
+// one pass on the device. Buffers in DPBuffer order over the interval's time
+// steps, every baseline of the buffer (auto-correlations included):
+//   data    [n_times][n_baselines][n_channels][4] complex64
+//   flags   [n_times][n_baselines][n_channels][4] bytes, or NULL (none set)
+//   weights [n_times][n_baselines][n_channels][4] float, or NULL (unit)
+//   model   [n_directions][n_times][n_baselines][n_channels][4] complex64
+// with n_channels the sum of the problem's channels per block, and
+// antenna1/antenna2 [n_baselines] host arrays whose cross-correlations are
+// the problem's baselines in order. A visibility with a set flag or a
+// non-finite value in any correlation of the data or of a model is zeroed in
+// the data and every model. FP16 problems get their per-(block, direction)
+// model scale from the weighted models (an extra pass over the buffers).
+// The _device form takes device pointers (data, model, weights 16-byte
+// aligned, flags 4-byte aligned) and enqueues on `stream` (a hipStream_t;
+// NULL: the problem's stream, which is its solver's); the host form copies
+// the buffers in and reports the kernels' device time.
+int ddh_problem_assign_and_weight_device(ddh_problem* problem, int32_t n_baselines,
+                                         const int32_t* antenna1, const int32_t* antenna2,
+                                         const float* data, const uint8_t* flags,
+                                         const float* weights, const float* model,
+                                         int32_t linear_weighting, void* stream);
+int ddh_problem_assign_and_weight(ddh_problem* problem, int32_t n_baselines,
+                                  const int32_t* antenna1, const int32_t* antenna2,
+                                  const float* data, const uint8_t* flags,
+                                  const float* weights, const float* model,
+                                  int32_t linear_weighting, double* kernel_ms);
+
 // Solve a device-resident problem. solutions as in ddh_solve.
 int ddh_solve_problem(ddh_solver* solver, ddh_problem* problem,
                       double* const* solutions, ddh_result* result);

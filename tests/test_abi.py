@@ -53,3 +53,27 @@ def test_cpp_host_layer_links():
         assert "not found" not in out, out
     lib = ctypes.CDLL(os.path.join(ROOT, "dp3_amd", "libddecal_hip_solver.so"))
     assert lib is not None
+
+
+def declared_arity(header):
+    """Parameter count of every ddh_ function declared in `header`."""
+    text = open(header).read()
+    text = re.sub(r"//[^\n]*", "", text)
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(ddh_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", text):
+        params = m.group(2).strip()
+        out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_python_binding_matches_declared_arity():
+    """dp3_amd.hip's ctypes signatures take as many arguments as the C
+    declarations (a mismatch only shows at call time on a GPU)."""
+    from dp3_amd import hip as H
+    arity = declared_arity(os.path.join(ROOT, "include", "ddecal_hip.h"))
+    wrong = {name: (len(args), arity[name]) for name, (args, _) in H.SIGNATURES.items()
+             if name in arity and len(args) != arity[name]}
+    assert not wrong, f"ctypes argument count differs from the header: {wrong}"
+    unknown = [n for n in H.SIGNATURES if n not in arity]
+    assert not unknown, f"bound but not declared: {unknown}"
