@@ -24,8 +24,9 @@
 // of the finite weighted model values of the block maps to 2^14 (the same
 // rule as uploads of packed blocks), found by a first pass over the buffers.
 //
-// Threads walk the raw visibilities in buffer order, so reads are contiguous;
-// each write lands in the row of its channel. HBM bound: per visibility 32 B data, 4 B flags, 16 B
+// A workgroup takes a tile of 16 baselines x 32 channels of one time step:
+// it reads runs of each baseline's row and, through LDS, stores runs of each
+// channel's row. HBM bound: per visibility 32 B data, 4 B flags, 16 B
 // weights and n_dir x 32 B models read; 16 B (Duo) or 32 B (Full) written for
 // the data and n_dir times 16 B / 32 B (complex64) or 8 B / 16 B (FP16) for
 // the models, auto-correlations excepted.
@@ -45,19 +46,10 @@ typedef _Float16 half2_v __attribute__((ext_vector_type(2)));
 typedef unsigned int uint2_v __attribute__((ext_vector_type(2)));
 typedef unsigned int uint4_v __attribute__((ext_vector_type(4)));
 
-constexpr int kIngestThreads = 256;
+constexpr int kIngestThreads = 512;
+// (block, direction) entries of the FP16 scale pass held in LDS
+constexpr int kIngestMaxScale = 2048;
 constexpr int kIngestMaxDirections = 16;
-// LDS of the FP16 scale pass: one word per (block, direction)
-constexpr size_t kIngestMaxLds = 32 * 1024;
-
-struct Vis4 {
-  float4_v lo;  // correlations 0, 1
-  float4_v hi;  // correlations 2, 3
-};
-
-__device__ __forceinline__ Vis4 LoadVis4(const float4_v* p, size_t v) {
-  return {p[2 * v], p[2 * v + 1]};
-}
 
 __device__ __forceinline__ bool Finite4(float4_v x) {
   return isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(x.w);
@@ -70,18 +62,19 @@ __device__ __forceinline__ uint32_t HalfBits(float re, float im) {
   return __builtin_bit_cast(uint32_t, h);
 }
 
-// Weighted, flag-zeroed visibility: correlation c scaled by w[c].
-__device__ __forceinline__ Vis4 Weigh(Vis4 x, float4_v w, bool flagged) {
-  const float4_v wl = {w.x, w.x, w.y, w.y};
-  const float4_v wh = {w.z, w.z, w.w, w.w};
-  const float4_v zero = {0.0f, 0.0f, 0.0f, 0.0f};
-  return {flagged ? zero : x.lo * wl, flagged ? zero : x.hi * wh};
+// Visibility x (correlations 0..3) weighted by w (per correlation) and
+// zeroed when flagged.
+__device__ __forceinline__ void Weigh(const float4_v (&x)[2], float4_v w, bool flagged,
+                                      float4_v (&out)[2]) {
+  const float4_v z = {0.0f, 0.0f, 0.0f, 0.0f};
+  out[0] = flagged ? z : x[0] * float4_v{w.x, w.x, w.y, w.y};
+  out[1] = flagged ? z : x[1] * float4_v{w.z, w.z, w.w, w.w};
 }
 
 // Largest finite |re|, |im| over the correlations a plane keeps.
 template <int NC>
-__device__ __forceinline__ float MaxAbs(Vis4 x, float m) {
-  float v[8] = {x.lo.x, x.lo.y, x.lo.z, x.lo.w, x.hi.x, x.hi.y, x.hi.z, x.hi.w};
+__device__ __forceinline__ float MaxAbs(const float4_v (&x)[2], float m) {
+  const float v[8] = {x[0].x, x[0].y, x[0].z, x[0].w, x[1].x, x[1].y, x[1].z, x[1].w};
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     if (NC == 2 && i >= 2 && i < 6) continue;  // Duo keeps correlations 0 and 3
@@ -91,94 +84,174 @@ __device__ __forceinline__ float MaxAbs(Vis4 x, float m) {
   return m;
 }
 
-// Stores one packed visibility at element `idx` of a plane.
+// Packed form of one visibility in a plane of the problem.
 template <int NC, bool F16>
-__device__ __forceinline__ void StoreVis(void* base, size_t idx, Vis4 x, float inv) {
-  if constexpr (!F16) {
-    float4_v* o = reinterpret_cast<float4_v*>(base);
-    if constexpr (NC == 2) {
-      o[idx] = float4_v{x.lo.x, x.lo.y, x.hi.z, x.hi.w};
-    } else {
-      o[2 * idx] = x.lo;
-      o[2 * idx + 1] = x.hi;
-    }
+struct Packed;
+template <>
+struct Packed<2, false> {
+  float4_v v;  // correlations 0 and 3
+};
+template <>
+struct Packed<4, false> {
+  float4_v v[2];
+};
+template <>
+struct Packed<2, true> {
+  uint2_v v;
+};
+template <>
+struct Packed<4, true> {
+  uint4_v v;
+};
+
+template <int NC, bool F16>
+__device__ __forceinline__ Packed<NC, F16> Pack(const float4_v (&x)[2], float inv) {
+  Packed<NC, F16> p;
+  if constexpr (!F16 && NC == 2) {
+    p.v = float4_v{x[0].x, x[0].y, x[1].z, x[1].w};
+  } else if constexpr (!F16) {
+    p.v[0] = x[0];
+    p.v[1] = x[1];
+  } else if constexpr (NC == 2) {
+    p.v = uint2_v{HalfBits(x[0].x * inv, x[0].y * inv), HalfBits(x[1].z * inv, x[1].w * inv)};
   } else {
-    if constexpr (NC == 2) {
-      reinterpret_cast<uint2_v*>(base)[idx] =
-          uint2_v{HalfBits(x.lo.x * inv, x.lo.y * inv), HalfBits(x.hi.z * inv, x.hi.w * inv)};
-    } else {
-      reinterpret_cast<uint4_v*>(base)[idx] =
-          uint4_v{HalfBits(x.lo.x * inv, x.lo.y * inv), HalfBits(x.lo.z * inv, x.lo.w * inv),
-                  HalfBits(x.hi.x * inv, x.hi.y * inv), HalfBits(x.hi.z * inv, x.hi.w * inv)};
-    }
+    p.v = uint4_v{HalfBits(x[0].x * inv, x[0].y * inv), HalfBits(x[0].z * inv, x[0].w * inv),
+                  HalfBits(x[1].x * inv, x[1].y * inv), HalfBits(x[1].z * inv, x[1].w * inv)};
   }
+  return p;
 }
+
+// Tile of one workgroup: kTileBl consecutive buffer baselines (time step t)
+// by kTileCh consecutive channels, one visibility per thread.
+constexpr int kTileCh = 32;
+constexpr int kTileBl = kIngestThreads / kTileCh;
 
 // MAX: per-(block, direction) maxima of the weighted models only.
 // Otherwise: writes the data plane and model planes of each block.
-// Threads walk the visibilities in DPBuffer order (contiguous reads; each
-// write lands in the row of its channel). NDB >= n_dir bounds the directions
-// held in registers: every load of a visibility is issued before the first is
-// used. MAX keeps per-workgroup maxima in LDS (s_max, n_cb * n_dir words),
-// merged into max_bits with one atomic per entry and workgroup.
+//
+// Read phase: thread i takes baseline i / kTileCh and channel i % kTileCh of
+// the tile, so each wave reads runs of kTileCh x 32 B of consecutive
+// baselines' rows. NDB >= n_dir bounds the directions held in registers: every
+// load of a visibility is issued before the first is used. Write phase, per
+// plane: the packed values pass through LDS, and thread j stores channel
+// j / kTileBl, baseline j % kTileBl, so the stores of a channel row are
+// contiguous. The grid is a multiple of the channel-group count, so every
+// thread keeps one channel (one block) for the whole launch; MAX keeps its
+// maxima in registers and merges them through LDS (s_max, n_cb * n_dir words)
+// into max_bits once per workgroup.
 template <int NC, bool F16, bool MAX, int NDB>
 __global__ void __launch_bounds__(kIngestThreads) AssignWeighKernel(IngestArgs g) {
-  extern __shared__ uint32_t s_max[];
+  using P = Packed<NC, F16>;
+  using PD = Packed<NC, false>;  // data planes are complex64 in every storage mode
+  __shared__ P s_model[2][kIngestThreads];
+  __shared__ PD s_data[kIngestThreads];
+  __shared__ uint32_t s_max[kIngestMaxScale];
   const int n_max = MAX ? g.n_cb * g.n_dir : 0;
   for (int e = threadIdx.x; e < n_max; e += kIngestThreads) s_max[e] = 0u;
-  if constexpr (MAX) __syncthreads();
+
+  const int n_cg = (g.n_ch + kTileCh - 1) / kTileCh;
+  const int n_blg = (g.n_bl_all + kTileBl - 1) / kTileBl;
+  const int cg = int(blockIdx.x % n_cg);
+  // read-phase coordinates
+  const int r_bl = threadIdx.x / kTileCh;
+  const int r_ch = cg * kTileCh + int(threadIdx.x % kTileCh);
+  // write-phase coordinates
+  const int w_ch_l = threadIdx.x / kTileBl;
+  const int w_bl = threadIdx.x % kTileBl;
+  const int w_ch = cg * kTileCh + w_ch_l;
+  const bool w_ch_ok = w_ch < g.n_ch;
+  const int w_cb = w_ch_ok ? g.ch_cb[w_ch] : 0;
+  const int w_c0 = g.ch_start[w_cb];
+  const int w_nch = g.ch_start[w_cb + 1] - w_c0;
+  const CbInfo w_info = g.cbinfo[w_cb];
+  const size_t w_plane = size_t(g.n_t) * w_nch * g.bl_stride;
+  const int r_cb = r_ch < g.n_ch ? g.ch_cb[r_ch] : 0;
+
+  const int64_t n_units = int64_t(g.n_t) * n_blg * n_cg;
+  const int64_t n_vis_all = int64_t(g.n_t) * g.n_bl_all * g.n_ch;
   const float4_v* data = reinterpret_cast<const float4_v*>(g.data);
   const float4_v* model = reinterpret_cast<const float4_v*>(g.model);
-  const int64_t n_vis_all = int64_t(g.n_t) * g.n_bl_all * g.n_ch;
   const float4_v zero = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int64_t v = int64_t(blockIdx.x) * kIngestThreads + threadIdx.x; v < n_vis_all;
-       v += int64_t(gridDim.x) * kIngestThreads) {
-    const int ch = int(v % g.n_ch);
-    const int64_t tb = v / g.n_ch;
-    const int bl = int(tb % g.n_bl_all);
-    const int t = int(tb / g.n_bl_all);
-    const int cross = g.cross[bl];
-    if (cross < 0) continue;  // auto-correlation: not solve data
-    const Vis4 x = LoadVis4(data, size_t(v));
-    const uint32_t fl = g.flags ? reinterpret_cast<const uint32_t*>(g.flags)[v] : 0u;
+  float mx[NDB];
+#pragma unroll
+  for (int d = 0; d < NDB; ++d) mx[d] = 0.0f;
+  if constexpr (MAX) __syncthreads();
+
+  for (int64_t u = blockIdx.x; u < n_units; u += gridDim.x) {
+    const int64_t tg = u / n_cg;
+    const int t = int(tg / n_blg);
+    const int bl0 = int(tg % n_blg) * kTileBl;
+    // ---- read phase
+    const int bl = bl0 + r_bl;
+    const bool r_ok = bl < g.n_bl_all && r_ch < g.n_ch && g.cross[bl] >= 0;
+    float4_v x[2] = {zero, zero};
     float4_v w = {1.0f, 1.0f, 1.0f, 1.0f};
-    if (g.weights) w = reinterpret_cast<const float4_v*>(g.weights)[v];
-    Vis4 m[NDB];
-#pragma unroll
-    for (int d = 0; d < NDB; ++d)
-      m[d] = d < g.n_dir ? LoadVis4(model, size_t(d) * n_vis_all + v) : Vis4{zero, zero};
-    bool flagged = fl != 0u || !Finite4(x.lo) || !Finite4(x.hi);
-#pragma unroll
-    for (int d = 0; d < NDB; ++d) flagged = flagged || !Finite4(m[d].lo) || !Finite4(m[d].hi);
-    if (g.weights && !g.linear) w = float4_v{sqrtf(w.x), sqrtf(w.y), sqrtf(w.z), sqrtf(w.w)};
-    const int cb = g.ch_cb[ch];
-    if constexpr (MAX) {
+    float4_v m[NDB][2];
+    bool flagged = false;
+    if (r_ok) {
+      const int64_t v = (int64_t(t) * g.n_bl_all + bl) * g.n_ch + r_ch;
+      x[0] = data[2 * v];
+      x[1] = data[2 * v + 1];
+      const uint32_t fl = g.flags ? reinterpret_cast<const uint32_t*>(g.flags)[v] : 0u;
+      if (g.weights) w = reinterpret_cast<const float4_v*>(g.weights)[v];
 #pragma unroll
       for (int d = 0; d < NDB; ++d) {
-        if (d < g.n_dir) {
-          const float mv = MaxAbs<NC>(Weigh(m[d], w, flagged), 0.0f);
-          if (mv > 0.0f) atomicMax(&s_max[cb * g.n_dir + d], __float_as_uint(mv));
+        const size_t mv = size_t(d) * n_vis_all + v;
+        m[d][0] = d < g.n_dir ? model[2 * mv] : zero;
+        m[d][1] = d < g.n_dir ? model[2 * mv + 1] : zero;
+      }
+      flagged = fl != 0u || !Finite4(x[0]) || !Finite4(x[1]);
+#pragma unroll
+      for (int d = 0; d < NDB; ++d) flagged = flagged || !Finite4(m[d][0]) || !Finite4(m[d][1]);
+      if (g.weights && !g.linear) w = float4_v{sqrtf(w.x), sqrtf(w.y), sqrtf(w.z), sqrtf(w.w)};
+    } else {
+#pragma unroll
+      for (int d = 0; d < NDB; ++d) m[d][0] = m[d][1] = zero;
+    }
+    float4_v o[2];
+    if constexpr (MAX) {
+      if (r_ok) {
+#pragma unroll
+        for (int d = 0; d < NDB; ++d) {
+          if (d < g.n_dir) {
+            Weigh(m[d], w, flagged, o);
+            mx[d] = MaxAbs<NC>(o, mx[d]);
+          }
         }
       }
     } else {
-      const int c0 = g.ch_start[cb];
-      const int n_ch_cb = g.ch_start[cb + 1] - c0;
-      const CbInfo info = g.cbinfo[cb];
-      const size_t out = (size_t(t) * n_ch_cb + (ch - c0)) * g.bl_stride + cross;
-      const size_t plane = size_t(g.n_t) * n_ch_cb * g.bl_stride;
-      // data planes are complex64 in every storage mode
-      StoreVis<NC, false>(g.out_data, size_t(info.data_off) + out, Weigh(x, w, flagged), 1.0f);
+      // ---- write phase through LDS (every thread reaches every barrier)
+      const int s_idx = (threadIdx.x % kTileCh) * kTileBl + r_bl;  // [channel][baseline]
+      const int wbl = bl0 + w_bl;
+      const int w_cross = (wbl < g.n_bl_all && w_ch_ok) ? g.cross[wbl] : -1;
+      const size_t out = (size_t(t) * w_nch + (w_ch - w_c0)) * g.bl_stride + w_cross;
+      Weigh(x, w, flagged, o);
+      s_data[s_idx] = Pack<NC, false>(o, 1.0f);
+      __syncthreads();
+      if (w_cross >= 0)
+        reinterpret_cast<PD*>(g.out_data)[size_t(w_info.data_off) + out] = s_data[threadIdx.x];
 #pragma unroll
       for (int d = 0; d < NDB; ++d) {
         if (d < g.n_dir) {
-          const float inv = F16 ? 1.0f / g.scale[cb * g.n_dir + d] : 1.0f;
-          StoreVis<NC, F16>(g.out_model, size_t(info.model_off) + d * plane + out,
-                            Weigh(m[d], w, flagged), inv);
+          const float inv = F16 ? 1.0f / g.scale[r_cb * g.n_dir + d] : 1.0f;
+          Weigh(m[d], w, flagged, o);
+          s_model[d & 1][s_idx] = Pack<NC, F16>(o, inv);
+          __syncthreads();
+          if (w_cross >= 0)
+            reinterpret_cast<P*>(g.out_model)[size_t(w_info.model_off) + d * w_plane + out] =
+                s_model[d & 1][threadIdx.x];
         }
       }
+      __syncthreads();  // the next unit reuses the LDS tiles
     }
   }
   if constexpr (MAX) {
+    if (r_ch < g.n_ch) {
+#pragma unroll
+      for (int d = 0; d < NDB; ++d)
+        if (d < g.n_dir && mx[d] > 0.0f)
+          atomicMax(&s_max[r_cb * g.n_dir + d], __float_as_uint(mx[d]));
+    }
     __syncthreads();
     for (int e = threadIdx.x; e < n_max; e += kIngestThreads)
       if (s_max[e] != 0u) atomicMax(&g.max_bits[e], s_max[e]);
@@ -201,15 +274,17 @@ IngestFn PickIngest(int n_corr, bool f16) {
 
 template <bool MAX>
 void LaunchAssignWeighPass(const IngestArgs& g, int n_corr, bool f16, hipStream_t stream) {
-  const int64_t n_vis = int64_t(g.n_t) * g.n_bl_all * g.n_ch;
-  // grid-stride: about eight workgroups per compute unit keep the loads in
-  // flight; fewer workgroups also mean fewer merges of the LDS maxima
-  const int64_t blocks = std::min<int64_t>((n_vis + kIngestThreads - 1) / kIngestThreads, 2048);
-  const size_t lds = MAX ? size_t(g.n_cb) * g.n_dir * sizeof(uint32_t) : 0;
+  const int64_t n_cg = (g.n_ch + kTileCh - 1) / kTileCh;
+  const int64_t n_units = int64_t(g.n_t) * ((g.n_bl_all + kTileBl - 1) / kTileBl) * n_cg;
+  // grid-stride over tiles: about four workgroups per compute unit keep the
+  // loads in flight; a multiple of the channel-group count fixes each
+  // thread's channel
+  const int64_t cap = std::max<int64_t>(1, 1024 / n_cg) * n_cg;
+  const int64_t blocks = std::min<int64_t>(n_units, cap);
   const IngestFn kernel = g.n_dir <= 4   ? PickIngest<MAX, 4>(n_corr, f16)
                           : g.n_dir <= 8 ? PickIngest<MAX, 8>(n_corr, f16)
                                          : PickIngest<MAX, kIngestMaxDirections>(n_corr, f16);
-  hipLaunchKernelGGL(kernel, dim3(unsigned(blocks)), dim3(kIngestThreads), lds, stream, g);
+  hipLaunchKernelGGL(kernel, dim3(unsigned(blocks)), dim3(kIngestThreads), 0, stream, g);
   DDH_HIP_CHECK(hipGetLastError());
 }
 
@@ -222,8 +297,8 @@ void LaunchAssignAndWeigh(const IngestArgs& g, int n_corr, bool f16, hipStream_t
   if (g.n_t == 0 || g.n_bl_all == 0 || g.n_ch == 0) return;
   if (f16) {
     const int n = g.n_cb * g.n_dir;
-    DDH_REQUIRE(size_t(n) * sizeof(uint32_t) <= kIngestMaxLds, -1,
-                "too many channel blocks x directions for the FP16 scale pass");
+    DDH_REQUIRE(n <= kIngestMaxScale, -1,
+                "FP16 ingest supports at most 2048 channel blocks x directions");
     hipLaunchKernelGGL(FillU32Kernel, dim3((n + 255) / 256), dim3(256), 0, stream, g.max_bits, n,
                        0u);
     DDH_HIP_CHECK(hipGetLastError());
