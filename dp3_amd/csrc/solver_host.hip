@@ -132,6 +132,9 @@ void ValidateConfig(const ddh_config& c) {
   DDH_REQUIRE(c.iteration != DDH_ITERATION_GRAM ||
                   (c.n_correlations == 2 && c.n_directions <= kMaxGramDirections),
               -4, "the Gram iteration is built for Duo data with at most 8 directions");
+  DDH_REQUIRE(c.iteration != DDH_ITERATION_GRAM ||
+                  ApplyGramSharedBytes(c.n_antennas, c.n_directions) <= kApplyGramMaxShared,
+              -4, "the Gram iteration stages at most 4096 antenna x direction gains");
   DDH_REQUIRE(c.n_antennas > 1, -1, "n_antennas must be > 1");
   DDH_REQUIRE(c.n_directions >= 1 && c.n_directions <= 16, -4,
               "n_directions must be 1..16");
@@ -1047,13 +1050,17 @@ void BuildBatch(ddh_batch& b) {
 // them): the correlate (Gram iteration: apply) of every tile group, then the
 // antenna solve over n_items items and the decision for n_intervals intervals.
 void LaunchBatchStep(ddh_batch& b, const IterArgs* args, int n_items, int n_intervals) {
-  if (b.gram)
-    LaunchBatchApply(args, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), b.d_items.ptr,
-                     n_items, n_intervals, b.solve_lds, b.solve_threads, b.f16, b.stream);
-  else
+  if (b.gram) {
+    int apply_ant = 0;
+    for (const IterArgs& a : b.args) apply_ant = std::max(apply_ant, a.n_ant);
+    LaunchBatchApply(args, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), apply_ant,
+                     b.d_items.ptr, n_items, n_intervals, b.solve_lds, b.solve_threads, b.f16,
+                     b.stream);
+  } else {
     LaunchBatch(args, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), b.d_items.ptr,
                 n_items, n_intervals, b.solve_lds, b.solve_threads, b.n_corr, b.f16, b.waves,
                 b.unroll, b.stream);
+  }
 }
 
 // Gram iteration: the cross products of every interval (shared by all

@@ -150,13 +150,18 @@ void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, const TileGro
 // correlate partials from them and the current gains.
 void LaunchGram(const IterArgs& a, bool f16, int waves, hipStream_t stream);
 void LaunchApplyGram(const IterArgs& a, bool f16, hipStream_t stream);
+// LDS of the Gram apply: the segment's gains of every antenna and direction.
+size_t ApplyGramSharedBytes(int n_ant, int n_dir);
+constexpr size_t kApplyGramMaxShared = 64 * 1024;
 size_t GramSharedBytes(int n_dir, int waves);
 void LaunchBatchGram(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
                      int n_groups, bool f16, int waves, hipStream_t stream);
 // One batched Gram iteration: apply per tile group, then antenna solve and decision.
+// apply_ant: the largest antenna count of the batch (sizes the apply's LDS).
 void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
-                      int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
-                      size_t solve_lds, int solve_threads, bool f16, hipStream_t stream);
+                      int n_groups, int apply_ant, const BatchTile* d_items, int n_items,
+                      int n_intervals, size_t solve_lds, int solve_threads, bool f16,
+                      hipStream_t stream);
 // Current solutions (by iteration parity) of batched interval `interval`
 // into dst, `count` complex values.
 void LaunchSnapshot(const IterArgs* d_args, int interval, int64_t count, double2* dst,

@@ -15,6 +15,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kMaxDirections = 16;
 constexpr int kCorrelateMaxThreads = 256;
+constexpr int kApplyThreads = 2 * kWave;  // Gram apply: one wave per Duo correlation
 constexpr int kBlockThreads = 1024;
 
 // Correlation k of the visibility matrix maps to gain polarizations (i, j):
@@ -465,65 +466,69 @@ __device__ __forceinline__ void GramTile(const IterArgs& a, const int tile, cons
 template <int ND, bool F16>
 __device__ __forceinline__ void ApplyGramTile(const IterArgs& a, const int tile, const int seg,
                                               const int cb) {
+  // One wave per correlation k: each wave issues its cross-product loads
+  // (8-byte vector loads) at once; the segment's gains are staged in LDS
+  // once per workgroup.
   constexpr int NC = 2;
   constexpr int NQ = ND + ND * (ND + 1) / 2;
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  extern __shared__ float2 s_gain[];  // [n_ant][ND][NC], repaired, cast to float
   const int lane = threadIdx.x & (kWave - 1);
+  const int k = int(threadIdx.x >> 6);  // kApplyThreads = NC waves
   const int bl = tile * kWave + lane;
-  const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
-  const double2* sol = (AsConst(a.state)->iteration & 1) ? a.sol1 : a.sol0;
-  const double mean = AsConst(a.mean)[cb];
   const bool valid = bl < a.n_bl;
   const int a1 = valid ? AsGlobal(a.ant1)[bl] : 0;
   const int a2 = valid ? AsGlobal(a.ant2)[bl] : 0;
-  // gg_d,k (repaired, cast to float, FP16 scale folded in), as correlate stages it
-  float2 gg[ND][NC];
+  const GlobalT<f2v>* in = (const GlobalT<f2v>*)AsGlobal(
+      a.gram + (size_t(cb * a.n_seg + seg) * NC * NQ + size_t(k) * NQ) * a.bl_stride + bl);
+  f2v q[NQ];
 #pragma unroll
-  for (int d = 0; d < ND; ++d) {
-    const int s = AsConst(a.seg_sol)[seg * ND + d];
-    const size_t p1 = gbase + size_t(a1) * a.n_sub + s;
-    const size_t p2 = gbase + size_t(a2) * a.n_sub + s;
-    double2 g1[2] = {LoadGlobal(sol + p1 * 2), LoadGlobal(sol + p1 * 2 + 1)};
-    double2 g2[2] = {LoadGlobal(sol + p2 * 2), LoadGlobal(sol + p2 * 2 + 1)};
-    if (!(cfinite(g1[0]) && cfinite(g1[1]))) g1[0] = g1[1] = make_double2(mean, 0.0);
-    if (!(cfinite(g2[0]) && cfinite(g2[1]))) g2[0] = g2[1] = make_double2(mean, 0.0);
-    float sc = 1.0f;
-    if constexpr (F16) sc = AsConst(a.model_scale)[cb * ND + d];
-#pragma unroll
-    for (int k = 0; k < NC; ++k) {
-      float2 g = cmulc(make_float2(float(g1[k].x), float(g1[k].y)),
-                       make_float2(float(g2[k].x), float(g2[k].y)));
-      gg[d][k] = valid ? make_float2(g.x * sc, g.y * sc) : make_float2(0.0f, 0.0f);
+  for (int i = 0; i < NQ; ++i) q[i] = in[size_t(i) * a.bl_stride];
+  // The segment's gains of every antenna and direction, repaired (a pair with
+  // a non-finite entry reads as the block mean) and cast to float, as
+  // correlate stages them.
+  {
+    const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
+    const double2* sol = (AsConst(a.state)->iteration & 1) ? a.sol1 : a.sol0;
+    const double mean = AsConst(a.mean)[cb];
+    for (int e = threadIdx.x; e < a.n_ant * ND; e += kApplyThreads) {
+      const int an = e / ND;
+      const int d = e - an * ND;
+      const size_t p = gbase + size_t(an) * a.n_sub + AsConst(a.seg_sol)[seg * ND + d];
+      double2 g[2] = {LoadGlobal(sol + p * 2), LoadGlobal(sol + p * 2 + 1)};
+      if (!(cfinite(g[0]) && cfinite(g[1]))) g[0] = g[1] = make_double2(mean, 0.0);
+      s_gain[e * NC + 0] = make_float2(float(g[0].x), float(g[0].y));
+      s_gain[e * NC + 1] = make_float2(float(g[1].x), float(g[1].y));
     }
   }
-  const GlobalT<float>* in = (const GlobalT<float>*)AsGlobal(
-      a.gram + size_t(cb * a.n_seg + seg) * NC * NQ * a.bl_stride + bl);
+  __syncthreads();
+  // gg_d,k with the FP16 scale folded in
+  float2 gg[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    float sc = 1.0f;
+    if constexpr (F16) sc = AsConst(a.model_scale)[cb * ND + d];
+    const float2 g = cmulc(s_gain[(a1 * ND + d) * NC + k], s_gain[(a2 * ND + d) * NC + k]);
+    gg[d] = valid ? make_float2(g.x * sc, g.y * sc) : make_float2(0.0f, 0.0f);
+  }
   GlobalT<float>* out =
       AsGlobal(a.partials) + (size_t(cb) * a.n_seg + seg) * ND * NC * 3 * a.bl_stride + bl;
 #pragma unroll
-  for (int k = 0; k < NC; ++k) {
-    float2 q[NQ];
+  for (int d = 0; d < ND; ++d) {
+    float2 u = make_float2(q[d].x, q[d].y);
+    // directions subtracted in order, as the residual is formed
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) {
-      const GlobalT<float>* f = in + 2 * size_t(k * NQ + i) * a.bl_stride;
-      q[i] = make_float2(f[0], f[1]);
+    for (int e = 0; e < ND; ++e) {
+      const f2v ge = e <= d ? q[GramPair<ND>(e, d)] : q[GramPair<ND>(d, e)];
+      const float2 g = make_float2(ge.x, e <= d ? ge.y : -ge.y);
+      const float2 c = cmul(gg[e], g);
+      u.x -= c.x;
+      u.y -= c.y;
     }
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      float2 u = q[d];
-      // directions subtracted in order, as the residual is formed
-#pragma unroll
-      for (int e = 0; e < ND; ++e) {
-        const float2 g = e <= d ? q[GramPair<ND>(e, d)]
-                                : make_float2(q[GramPair<ND>(d, e)].x, -q[GramPair<ND>(d, e)].y);
-        const float2 c = cmul(gg[e][k], g);
-        u.x -= c.x;
-        u.y -= c.y;
-      }
-      const size_t v = size_t(d * NC + k) * 3;
-      out[(v + 0) * a.bl_stride] = u.x;
-      out[(v + 1) * a.bl_stride] = u.y;
-      out[(v + 2) * a.bl_stride] = q[GramPair<ND>(d, d)].x;
-    }
+    const size_t v = size_t(d * NC + k) * 3;
+    out[(v + 0) * a.bl_stride] = u.x;
+    out[(v + 1) * a.bl_stride] = u.y;
+    out[(v + 2) * a.bl_stride] = q[GramPair<ND>(d, d)].x;
   }
 }
 
@@ -533,7 +538,7 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads) GramKernel(const IterArg
 }
 
 template <int ND, bool F16>
-__global__ void __launch_bounds__(kWave) ApplyGramKernel(const IterArgs a) {
+__global__ void __launch_bounds__(kApplyThreads) ApplyGramKernel(const IterArgs a) {
   if (AsConst(a.state)->done) return;
   ApplyGramTile<ND, F16>(a, blockIdx.x, blockIdx.y, blockIdx.z);
 }
@@ -547,7 +552,7 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads)
 }
 
 template <int ND, bool F16>
-__global__ void __launch_bounds__(kWave)
+__global__ void __launch_bounds__(kApplyThreads)
     BatchApplyGramKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ tiles) {
   const BatchTile t = LoadConst(tiles + blockIdx.x);
   const IterArgs a = LoadConst(args + t.interval);
@@ -1188,10 +1193,15 @@ void LaunchGram(const IterArgs& a, bool f16, int waves, hipStream_t stream) {
   DDH_HIP_CHECK(hipGetLastError());
 }
 
+size_t ApplyGramSharedBytes(int n_ant, int n_dir) {
+  return size_t(n_ant) * n_dir * 2 * sizeof(float2);
+}
+
 void LaunchApplyGram(const IterArgs& a, bool f16, hipStream_t stream) {
   CheckGramDirections(a.n_dir);
   const dim3 grid(a.bl_stride / kWave, a.n_seg, a.n_cb);
-  hipLaunchKernelGGL(kApplyGram[f16][a.n_dir - 1], grid, dim3(kWave), 0, stream, a);
+  hipLaunchKernelGGL(kApplyGram[f16][a.n_dir - 1], grid, dim3(kApplyThreads),
+                     ApplyGramSharedBytes(a.n_ant, a.n_dir), stream, a);
   DDH_HIP_CHECK(hipGetLastError());
 }
 
@@ -1209,13 +1219,15 @@ void LaunchBatchGram(const IterArgs* d_args, const BatchTile* d_tiles, const Til
 }
 
 void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
-                      int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
-                      size_t solve_lds, int solve_threads, bool f16, hipStream_t stream) {
+                      int n_groups, int apply_ant, const BatchTile* d_items, int n_items,
+                      int n_intervals, size_t solve_lds, int solve_threads, bool f16,
+                      hipStream_t stream) {
   for (int g = 0; g < n_groups; ++g) {
     if (groups[g].count == 0) continue;
     CheckGramDirections(groups[g].n_dir);
     hipLaunchKernelGGL(kBatchApplyGram[f16][groups[g].n_dir - 1], dim3(groups[g].count),
-                       dim3(kWave), 0, stream, d_args, d_tiles + groups[g].first);
+                       dim3(kApplyThreads), ApplyGramSharedBytes(apply_ant, groups[g].n_dir),
+                       stream, d_args, d_tiles + groups[g].first);
   }
   DDH_HIP_CHECK(hipGetLastError());
   LaunchBatchTail(d_args, d_items, n_items, n_intervals, solve_lds, solve_threads, 2, f16,
@@ -1254,6 +1266,15 @@ void PrepareKernelAttributes() {
               reinterpret_cast<const void*>(kBatchCorrelate[c][h][r][d]),
               hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
         }
+  for (int h = 0; h < 2; ++h)
+    for (int d = 0; d < kMaxGramDirections; ++d) {
+      DDH_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kApplyGram[h][d]),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        int(kApplyGramMaxShared)));
+      DDH_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kBatchApplyGram[h][d]),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        int(kApplyGramMaxShared)));
+    }
   // antenna_solve also holds static LDS for its reductions
   for (const void* f : {reinterpret_cast<const void*>(&AntennaSolveKernel<2, false>),
                         reinterpret_cast<const void*>(&AntennaSolveKernel<2, true>),
