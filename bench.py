@@ -366,12 +366,13 @@ def main():
                   "data read and written, every model read")),
         "assign_and_weight": roof(
             "assign_and_weight",
-            sub_vis * (32 + 4 + 16 + w.n_directions * 32) + cross_vis * (16 + w.n_directions * 8),
+            cross_vis * (32 + 4 + 16 + w.n_directions * 32 + 16 + w.n_directions * 8),
             ing_ms,
             note=(f"AssignAndWeight + SolveData packing of one interval into an FP16 device "
-                  f"problem: {sub_vis} raw visibilities (data, flags, weights, every model "
-                  f"read once), {cross_vis} packed Duo visibilities written; two passes over "
-                  "the raw buffers (the first finds the FP16 scale)"))}
+                  f"problem: the {cross_vis} raw cross-correlation visibilities (data, flags, "
+                  "weights, every model) read once and written packed (Duo); the autocorrelations "
+                  "are not read; two passes over the raw buffers (the first finds the FP16 "
+                  "scale)"))}
     log(f"correct and subtract: {sub_ms * 1e3:.1f} us per interval; assign and weight "
         f"(FP16 problem): {ing_ms * 1e3:.1f} us per interval")
 

@@ -26,10 +26,10 @@
 //
 // A workgroup takes a tile of 16 baselines x 32 channels of one time step:
 // it reads runs of each baseline's row and, through LDS, stores runs of each
-// channel's row. HBM bound: per visibility 32 B data, 4 B flags, 16 B
-// weights and n_dir x 32 B models read; 16 B (Duo) or 32 B (Full) written for
-// the data and n_dir times 16 B / 32 B (complex64) or 8 B / 16 B (FP16) for
-// the models, auto-correlations excepted.
+// channel's row. HBM bound: per cross-correlation visibility 32 B data, 4 B
+// flags, 16 B weights and n_dir x 32 B models read (auto-correlations are
+// not read); 16 B (Duo) or 32 B (Full) written for the data and n_dir times
+// 16 B / 32 B (complex64) or 8 B / 16 B (FP16) for the models.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
