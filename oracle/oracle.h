@@ -18,6 +18,8 @@
 //   steps/OnePredict.cc:437-457                        (copy to DPBuffer)
 //   ddecal/test/unit/SolverTester.cc:95-202,348-363    (test fixture data)
 //   steps/DDECal.cc:992-1124                    (correct and subtract models)
+//   ddecal/constraints/SmoothnessConstraint.cc:50-97, KernelSmoother.h
+//                                                   (smoothness constraint)
 //
 // All entry points are extern "C" with plain pointers so that Python (ctypes)
 // can drive them. Complex numbers are interleaved (re, im).
@@ -56,6 +58,24 @@ typedef struct {
   int32_t n_step_magnitudes;
 } orc_result;
 
+// Smoothness constraint (SmoothnessConstraint.h:23-75, Gaussian kernel).
+typedef struct {
+  const double* frequencies;           // [n_cb] channel-block frequencies, ascending (Hz)
+  double bandwidth_hz;                 // kernel size (> 0)
+  double bandwidth_ref_frequency_hz;   // 0: the same kernel size at every frequency
+  double spectral_exponent;
+  int32_t kernel_truncation;
+  const double* weights;               // [n_ant][n_cb] (SetWeights), NULL: ones
+  const double* sub_solution_weights;  // [n_sub][n_ant][n_cb] (SetSubSolutionWeights) or NULL
+  const double* antenna_factors;       // [n_ant] (SetAntennaFactors), NULL: ones
+  const double* dd_factors;            // [n_sub] (SetDdSmoothingFactors), NULL: ones
+} orc_smoothness;
+
+// SmoothnessConstraint::Apply on solutions [n_cb][n_ant][n_sub][n_pol]
+// complex128, in place.
+int orc_smooth(const orc_smoothness* constraint, int32_t n_cb, int32_t n_ant, int32_t n_sub,
+               int32_t n_pol, double* solutions);
+
 // Channel-block data in the reference SolveData layout, concatenated over
 // channel blocks. vis_offsets[cb] .. vis_offsets[cb+1] are the visibilities of
 // block cb.
@@ -70,6 +90,13 @@ int orc_solve(const orc_settings* settings, const int64_t* vis_offsets,
               const float* data, const float* model, const uint32_t* ant1,
               const uint32_t* ant2, const uint32_t* sol_map, double* solutions,
               orc_result* result, double* step_magnitudes);
+// orc_solve with a smoothness constraint applied to the next solutions after
+// every step (SolverBase::ApplyConstraints, SolverBase.cc:166-190; the
+// constraint is always satisfied).
+int orc_solve_constrained(const orc_settings* settings, const orc_smoothness* smoothness,
+                          const int64_t* vis_offsets, const float* data, const float* model,
+                          const uint32_t* ant1, const uint32_t* ant2, const uint32_t* sol_map,
+                          double* solutions, orc_result* result, double* step_magnitudes);
 
 // One PerformIteration + SolveDirection pass for all blocks without step:
 // writes next_solutions [n_cb][n_ant][n_sub][2] complex128 from solutions.

@@ -45,6 +45,48 @@ class _Result(C.Structure):
     ]
 
 
+class _Smoothness(C.Structure):
+    _fields_ = [
+        ("frequencies", C.c_void_p),
+        ("bandwidth_hz", C.c_double),
+        ("bandwidth_ref_frequency_hz", C.c_double),
+        ("spectral_exponent", C.c_double),
+        ("kernel_truncation", C.c_int32),
+        ("weights", C.c_void_p),
+        ("sub_solution_weights", C.c_void_p),
+        ("antenna_factors", C.c_void_p),
+        ("dd_factors", C.c_void_p),
+    ]
+
+
+@dataclass
+class Smoothness:
+    """Smoothness constraint settings (SmoothnessConstraint.h:23-75):
+    channel-block frequencies (Hz), kernel bandwidth, reference frequency,
+    spectral exponent, truncation, weights [n_ant, n_cb] or per sub-solution
+    [n_sub, n_ant, n_cb], antenna factors [n_ant], dd factors [n_sub]."""
+    frequencies: np.ndarray
+    bandwidth_hz: float
+    bandwidth_ref_frequency_hz: float = 0.0
+    spectral_exponent: float = 0.0
+    kernel_truncation: bool = True
+    weights: np.ndarray | None = None
+    sub_solution_weights: np.ndarray | None = None
+    antenna_factors: np.ndarray | None = None
+    dd_factors: np.ndarray | None = None
+
+    def struct(self):
+        """The C struct and the arrays it points to (kept alive by the caller)."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                for a in (self.frequencies, self.weights, self.sub_solution_weights,
+                          self.antenna_factors, self.dd_factors)]
+        ptr = [None if a is None else a.ctypes.data for a in arrs]
+        st = _Smoothness(ptr[0], self.bandwidth_hz, self.bandwidth_ref_frequency_hz,
+                         self.spectral_exponent, int(self.kernel_truncation), ptr[1], ptr[2],
+                         ptr[3], ptr[4])
+        return st, arrs
+
+
 class Component(C.Structure):
     _fields_ = [
         ("ra", C.c_double), ("dec", C.c_double),
@@ -72,6 +114,11 @@ def lib():
         P = C.c_void_p
         L.orc_solve.argtypes = [C.POINTER(_Settings), P, P, P, P, P, P, P, C.POINTER(_Result), P]
         L.orc_solve.restype = C.c_int
+        L.orc_solve_constrained.argtypes = [C.POINTER(_Settings), C.POINTER(_Smoothness), P, P,
+                                            P, P, P, P, P, C.POINTER(_Result), P]
+        L.orc_solve_constrained.restype = C.c_int
+        L.orc_smooth.argtypes = [C.POINTER(_Smoothness)] + [C.c_int32] * 4 + [P]
+        L.orc_smooth.restype = C.c_int
         L.orc_iterate_once.argtypes = [C.POINTER(_Settings), P, P, P, P, P, P, P, P]
         L.orc_iterate_once.restype = C.c_int
         L.orc_chi2.argtypes = [C.c_int32] * 5 + [P] * 7
@@ -108,10 +155,24 @@ class SolveResult:
     step_magnitudes: np.ndarray = field(default_factory=lambda: np.zeros(0))
 
 
+def smooth(solutions, smoothness: Smoothness):
+    """SmoothnessConstraint::Apply in place on solutions [n_cb, n_ant, n_sub,
+    n_pol] complex128."""
+    assert solutions.dtype == np.complex128 and solutions.flags.c_contiguous
+    n_cb, n_ant, n_sub, n_pol = solutions.shape
+    st, keep = smoothness.struct()
+    rc = lib().orc_smooth(C.byref(st), n_cb, n_ant, n_sub, n_pol, _p(solutions))
+    del keep
+    if rc != 0:
+        raise RuntimeError(f"orc_smooth failed ({rc})")
+
+
 def solve(problem, solutions, *, max_iterations=50, min_iterations=0, accuracy=1e-5,
           constraint_accuracy=1e-4, step_size=0.2, detect_stalling=True,
-          step_diff_sigma=0.1, phase_only=False, fp64=False, n_threads=1):
-    """Runs the oracle solver in place on `solutions` ([n_cb, n_ant*n_sub*2] complex128).
+          step_diff_sigma=0.1, phase_only=False, fp64=False, n_threads=1,
+          smoothness: Smoothness | None = None):
+    """Runs the oracle solver in place on `solutions` ([n_cb, n_ant*n_sub*2] complex128),
+    with the smoothness constraint when given.
 
     `problem` is a dp3_amd.synth.SolveProblem (reference SolveData layout)."""
     n_sol = np.ascontiguousarray(problem.n_solutions_per_direction, dtype=np.int32)
@@ -123,9 +184,17 @@ def solve(problem, solutions, *, max_iterations=50, min_iterations=0, accuracy=1
     assert solutions.dtype == np.complex128 and solutions.flags.c_contiguous
     res = _Result()
     mags = np.zeros(max_iterations + 2)
-    rc = lib().orc_solve(C.byref(s), _p(problem.vis_offsets), _p(problem.data),
-                         _p(problem.model), _p(problem.ant1), _p(problem.ant2),
-                         _p(problem.sol_map), _p(solutions), C.byref(res), _p(mags))
+    if smoothness is None:
+        rc = lib().orc_solve(C.byref(s), _p(problem.vis_offsets), _p(problem.data),
+                             _p(problem.model), _p(problem.ant1), _p(problem.ant2),
+                             _p(problem.sol_map), _p(solutions), C.byref(res), _p(mags))
+    else:
+        st, keep = smoothness.struct()
+        rc = lib().orc_solve_constrained(C.byref(s), C.byref(st), _p(problem.vis_offsets),
+                                         _p(problem.data), _p(problem.model), _p(problem.ant1),
+                                         _p(problem.ant2), _p(problem.sol_map), _p(solutions),
+                                         C.byref(res), _p(mags))
+        del keep
     if rc != 0:
         raise RuntimeError(f"orc_solve failed ({rc})")
     return SolveResult(int(res.iterations), int(res.constraint_iterations),

@@ -409,7 +409,8 @@ class IterativeDiagonalOracle {
 
   // IterativeDiagonalSolver.cc:38-97 and SolverBase.cc:192-213.
   void Solve(const std::vector<ChannelBlock<V>>& cbs, double* sol_flat,
-             orc_result* result, double* step_mags_out) {
+             orc_result* result, double* step_mags_out,
+             const orc_smoothness* smoothness = nullptr) {
     const size_t size = SolSize();
     std::vector<std::vector<DComplex>> solutions(n_cb_);
     for (size_t cb = 0; cb != n_cb_; ++cb)
@@ -433,7 +434,12 @@ class IterativeDiagonalOracle {
                            &next[cb * size]);
       });
       Step(solutions, next);
-      constraints_satisfied = true;  // no constraints attached
+      // ApplyConstraints (SolverBase.cc:174-190): the smoothness constraint
+      // is always satisfied; without constraints the same holds.
+      if (smoothness)
+        orc_smooth(smoothness, int32_t(n_cb_), int32_t(n_ant_), int32_t(n_sub_), 2,
+                   reinterpret_cast<double*>(next.data()));
+      constraints_satisfied = true;
       has_converged = AssignSolutions(solutions, next, !constraints_satisfied,
                                       step_magnitudes);
       iteration++;
@@ -508,11 +514,11 @@ template <typename V>
 int SolveTyped(const orc_settings* s, const int64_t* vo, const float* data,
                const float* model, const uint32_t* a1, const uint32_t* a2,
                const uint32_t* sm, double* solutions, orc_result* result,
-               double* step_mags) {
+               double* step_mags, const orc_smoothness* smoothness = nullptr) {
   const std::vector<ChannelBlock<V>> cbs =
       MakeBlocks<V>(*s, vo, data, model, a1, a2, sm);
   IterativeDiagonalOracle<V> solver(*s);
-  solver.Solve(cbs, solutions, result, step_mags);
+  solver.Solve(cbs, solutions, result, step_mags, smoothness);
   return 0;
 }
 
@@ -551,6 +557,26 @@ extern "C" int orc_solve(const orc_settings* s, const int64_t* vo,
     return SolveTyped<Full<float>>(s, vo, data, model, a1, a2, sm, solutions,
                                    result, step_mags);
   }
+  return -2;
+}
+
+extern "C" int orc_solve_constrained(const orc_settings* s, const orc_smoothness* c,
+                                     const int64_t* vo, const float* data, const float* model,
+                                     const uint32_t* a1, const uint32_t* a2, const uint32_t* sm,
+                                     double* solutions, orc_result* result, double* step_mags) {
+  if (!s || !c || !c->frequencies || !(c->bandwidth_hz > 0.0) || s->n_directions <= 0 ||
+      s->n_channel_blocks <= 0 || s->n_antennas <= 0)
+    return -1;
+  if (s->n_correlations == 2)
+    return s->fp64 ? SolveTyped<Duo<double>>(s, vo, data, model, a1, a2, sm, solutions, result,
+                                             step_mags, c)
+                   : SolveTyped<Duo<float>>(s, vo, data, model, a1, a2, sm, solutions, result,
+                                            step_mags, c);
+  if (s->n_correlations == 4)
+    return s->fp64 ? SolveTyped<Full<double>>(s, vo, data, model, a1, a2, sm, solutions,
+                                              result, step_mags, c)
+                   : SolveTyped<Full<float>>(s, vo, data, model, a1, a2, sm, solutions, result,
+                                             step_mags, c);
   return -2;
 }
 
