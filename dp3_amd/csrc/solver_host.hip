@@ -121,6 +121,15 @@ struct ddh_solver {
   double last_solve_ms = 0.0;
   // cached problem of the drop-in path
   std::unique_ptr<ddh_problem, void (*)(ddh_problem*)> cached{nullptr, nullptr};
+  // smoothness constraint (on when sm_freq is set): the kernel windows and
+  // frequency corrections per block, weights, factors, smoothed solutions
+  ddh::DeviceBuffer<double> sm_freq, sm_corr, sm_weights, sm_ant_factor, sm_dd_factor;
+  ddh::DeviceBuffer<int32_t> sm_window;
+  ddh::DeviceBuffer<double2> sm_out;
+  bool sm_on = false;
+  bool sm_sub_weights = false;
+  double sm_bandwidth = 0.0;
+  int sm_truncate = 1;
 };
 
 namespace ddh {
@@ -410,6 +419,18 @@ IterArgs MakeArgs(ddh_solver& s, ddh_problem& p) {
   a.split = 1;
   while (a.split < 8 && c.n_antennas * max_nsd * a.split * 2 <= threads) a.split *= 2;
   a.max_nsd = max_nsd;
+  if (s.sm_on) {
+    a.sm_freq = s.sm_freq.ptr;
+    a.sm_corr = s.sm_corr.ptr;
+    a.sm_window = s.sm_window.ptr;
+    a.sm_weights = s.sm_weights.ptr;
+    a.sm_ant_factor = s.sm_ant_factor.ptr;
+    a.sm_dd_factor = s.sm_dd_factor.ptr;
+    a.sm_out = s.sm_out.ptr;
+    a.sm_bandwidth = s.sm_bandwidth;
+    a.sm_truncate = s.sm_truncate;
+    a.sm_sub_weights = s.sm_sub_weights ? 1 : 0;
+  }
   return a;
 }
 
@@ -438,6 +459,7 @@ void LaunchSolvePrologue(const ddh_solver& s, const IterArgs& a, const ddh_probl
 void LaunchIteration(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
   LaunchCorrelation(s, a, p);
   LaunchAntennaSolve(a, p.n_corr, p.f16, s.stream);
+  LaunchSmoothing(a, s.stream);
   LaunchDecide(a, s.stream);
 }
 
@@ -445,7 +467,7 @@ void LaunchIteration(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
 // the solver buffers stay the same.
 void EnsureGraph(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
   const std::vector<const void*> key = {&p, p.data.ptr, p.model.ptr, s.sol0.ptr, s.partials.ptr,
-                                        s.gram.ptr,
+                                        s.gram.ptr, s.sm_out.ptr, s.sm_weights.ptr,
                                         reinterpret_cast<const void*>(intptr_t(s.poll)),
                                         reinterpret_cast<const void*>(intptr_t(s.waves)),
                                         reinterpret_cast<const void*>(intptr_t(s.unroll))};
@@ -529,6 +551,7 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
         LaunchCorrelation(s, a, p);
         if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it + 1], s.stream));
         LaunchAntennaSolve(a, p.n_corr, p.f16, s.stream);
+        LaunchSmoothing(a, s.stream);
         LaunchDecide(a, s.stream);
       }
       launched += k;
@@ -920,6 +943,97 @@ int ddh_solver_last_timing(const ddh_solver* s, double* correlate_ms, int64_t* l
   return 0;
 }
 
+// New constraint weights. A captured iteration graph holds the weights
+// address and kind by value: it is kept while both stay the same (DDECal sets
+// the weights once per interval), and rebuilt otherwise. Batches take the
+// arguments at creation or rebind.
+static void UploadConstraintWeights(ddh_solver& s, const std::vector<double>& w, bool per_sub) {
+  const double* before = s.sm_weights.ptr;
+  s.sm_weights.Upload(w, s.stream);
+  if (s.sm_weights.ptr != before || s.sm_sub_weights != per_sub) DestroyGraph(s);
+  s.sm_sub_weights = per_sub;
+  DDH_HIP_CHECK(hipStreamSynchronize(s.stream));
+}
+
+int ddh_solver_set_smoothness(ddh_solver* s, const double* channel_block_frequencies,
+                              double bandwidth_hz, double bandwidth_ref_frequency_hz,
+                              double spectral_exponent, int32_t kernel_truncation,
+                              const double* antenna_factors, const double* dd_factors) {
+  return Guard([&] {
+    DDH_REQUIRE(s, -1, "NULL solver");
+    DestroyGraph(*s);
+    if (!channel_block_frequencies) {
+      s->sm_on = false;
+      return;
+    }
+    const ddh_config& c = s->cfg.c;
+    const int n_cb = c.n_channel_blocks, n_ant = c.n_antennas, n_sub = s->cfg.n_sub;
+    DDH_REQUIRE(bandwidth_hz > 0.0, -1, "the smoothing bandwidth must be > 0");
+    DDH_REQUIRE(c.n_correlations == 2 || c.n_correlations == 4, -1, "bad correlations");
+    std::vector<double> f(channel_block_frequencies, channel_block_frequencies + n_cb);
+    for (int i = 1; i < n_cb; ++i)
+      DDH_REQUIRE(f[i] >= f[i - 1], -1, "channel-block frequencies must ascend");
+    // KernelSmoother's truncation windows move up the grid with the output
+    // block; they and the frequency correction of the kernel size depend only
+    // on the grid (KernelSmoother.h, Smooth).
+    std::vector<int32_t> window(size_t(2) * n_cb);
+    std::vector<double> corr(n_cb);
+    size_t left = 0, right = 0;
+    const size_t n = size_t(n_cb);
+    while (right != n && f[right] < f[0] + bandwidth_hz * 0.5) ++right;
+    right = std::min(right + 1, n);
+    for (size_t i = 0; i != n; ++i) {
+      const double local_bw = bandwidth_ref_frequency_hz == 0.0
+                                  ? bandwidth_hz
+                                  : bandwidth_hz * f[i] / bandwidth_ref_frequency_hz;
+      size_t first = 0, last = n;
+      if (kernel_truncation) {
+        while (f[left] < f[i] - local_bw * 0.5) ++left;
+        while (right != n && f[right] < f[i] + local_bw * 0.5) ++right;
+        first = left > 0 ? left - 1 : 0;
+        last = right < n ? right + 1 : n;
+      }
+      window[2 * i] = int32_t(first);
+      window[2 * i + 1] = int32_t(last);
+      corr[i] = std::pow(local_bw / bandwidth_hz, spectral_exponent);
+    }
+    std::vector<double> ant_f(n_ant, 1.0), dd_f(n_sub, 1.0);
+    if (antenna_factors) ant_f.assign(antenna_factors, antenna_factors + n_ant);
+    if (dd_factors) dd_f.assign(dd_factors, dd_factors + n_sub);
+    s->sm_freq.Upload(f, s->stream);
+    s->sm_corr.Upload(corr, s->stream);
+    s->sm_window.Upload(window, s->stream);
+    s->sm_ant_factor.Upload(ant_f, s->stream);
+    s->sm_dd_factor.Upload(dd_f, s->stream);
+    if (!s->sm_on) {
+      // unit weights until the caller sets them
+      s->sm_weights.Upload(std::vector<double>(size_t(n_ant) * n_cb, 1.0), s->stream);
+      s->sm_sub_weights = false;
+    }
+    s->sm_out.Resize(size_t(n_cb) * n_ant * n_sub * 2);
+    s->sm_bandwidth = bandwidth_hz;
+    s->sm_truncate = kernel_truncation ? 1 : 0;
+    DDH_HIP_CHECK(hipStreamSynchronize(s->stream));
+    s->sm_on = true;
+  });
+}
+
+int ddh_solver_set_constraint_weights(ddh_solver* s, const double* weights) {
+  return Guard([&] {
+    DDH_REQUIRE(s && weights, -1, "NULL argument");
+    const size_t n = size_t(s->cfg.c.n_antennas) * s->cfg.c.n_channel_blocks;
+    UploadConstraintWeights(*s, std::vector<double>(weights, weights + n), false);
+  });
+}
+
+int ddh_solver_set_sub_solution_weights(ddh_solver* s, const double* weights) {
+  return Guard([&] {
+    DDH_REQUIRE(s && weights, -1, "NULL argument");
+    const size_t n = size_t(s->cfg.n_sub) * s->cfg.c.n_antennas * s->cfg.c.n_channel_blocks;
+    UploadConstraintWeights(*s, std::vector<double>(weights, weights + n), true);
+  });
+}
+
 int ddh_set_schedule(ddh_solver* s, int32_t waves_per_tile, int32_t rows_per_trip,
                      int32_t use_graph, int32_t poll_interval) {
   return Guard([&] {
@@ -958,6 +1072,7 @@ struct ddh_batch {
   int solve_threads = 0;
   std::vector<ddh::IterArgs> args;
   std::vector<ddh::TileGroup> groups;  // tiles of one direction count each
+  ddh::SmoothLaunch smooth{};          // smoothness constraint launches (none: 0 intervals)
   std::vector<ddh::BatchTile> tiles, items, pro_items;
   ddh::DeviceBuffer<ddh::IterArgs> d_args;
   ddh::DeviceBuffer<ddh::BatchTile> d_tiles, d_items, d_pro_items;
@@ -1038,6 +1153,15 @@ void BuildBatch(ddh_batch& b) {
     ++b.groups.back().count;
     b.tiles.push_back(c.t);
   }
+  b.smooth = SmoothLaunch{};
+  for (const IterArgs& a : b.args) {
+    if (!a.sm_freq) continue;
+    b.smooth.n_intervals = int(b.args.size());
+    b.smooth.max_elements =
+        std::max(b.smooth.max_elements, int64_t(a.n_cb) * a.n_ant * a.n_sub * 2);
+    b.smooth.finish_threads =
+        std::max(b.smooth.finish_threads, SmoothFinishThreads(a.n_ant, a.max_nsd));
+  }
   b.d_args.Upload(b.args, b.stream);
   b.d_tiles.Upload(b.tiles, b.stream);
   b.d_items.Upload(b.items, b.stream);
@@ -1055,11 +1179,11 @@ void LaunchBatchStep(ddh_batch& b, const IterArgs* args, int n_items, int n_inte
     for (const IterArgs& a : b.args) apply_ant = std::max(apply_ant, a.n_ant);
     LaunchBatchApply(args, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), apply_ant,
                      b.d_items.ptr, n_items, n_intervals, b.solve_lds, b.solve_threads, b.f16,
-                     b.stream);
+                     b.smooth, b.stream);
   } else {
     LaunchBatch(args, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), b.d_items.ptr,
                 n_items, n_intervals, b.solve_lds, b.solve_threads, b.n_corr, b.f16, b.waves,
-                b.unroll, b.stream);
+                b.unroll, b.smooth, b.stream);
   }
 }
 
@@ -1245,7 +1369,7 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
           LaunchBatchStep(b, b.d_args.ptr, 0, 0);
           DDH_HIP_CHECK(hipEventRecord(b.events[3 * it + 1], b.stream));
           LaunchBatchTail(b.d_args.ptr, b.d_items.ptr, int(b.items.size()), int(b.solvers.size()),
-                          b.solve_lds, b.solve_threads, b.n_corr, b.f16, b.stream);
+                          b.solve_lds, b.solve_threads, b.n_corr, b.f16, b.smooth, b.stream);
           DDH_HIP_CHECK(hipEventRecord(b.events[3 * it + 2], b.stream));
         } else {
           LaunchBatchIteration(b);

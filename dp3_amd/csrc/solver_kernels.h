@@ -94,6 +94,16 @@ struct IterArgs {
   double* den_out;           // optional [n_cb][n_ant][n_sub][2] denominators
   int32_t split;             // threads per (antenna, solution) in antenna_solve
   int32_t max_nsd;           // largest n_sol_dir entry
+  // Smoothness constraint on the stepped solutions (null sm_freq: none).
+  const double* sm_freq;        // [n_cb] channel-block frequencies (Hz)
+  const double* sm_corr;        // [n_cb] frequency correction of the kernel size per block
+  const int32_t* sm_window;     // [n_cb][2] kernel window [first, last) per block
+  const double* sm_weights;     // [n_ant][n_cb], or [n_sub][n_ant][n_cb] (sm_sub_weights)
+  const double* sm_ant_factor;  // [n_ant]
+  const double* sm_dd_factor;   // [n_sub]
+  double2* sm_out;              // [n_cb][n_ant][n_sub][2] smoothed solutions
+  double sm_bandwidth;
+  int32_t sm_truncate, sm_sub_weights;
   // Gram iteration (Duo, n_dir <= kMaxGramDirections): data/model cross
   // products C_d and G_e,d per (block, segment, correlation, baseline), made
   // once per solve; [n_cb][n_seg][2][n_dir + n_dir(n_dir+1)/2][bl_stride].
@@ -139,13 +149,24 @@ struct TileGroup {
   int32_t first;
   int32_t count;
 };
+// Smoothness constraint launches of a batch (n_intervals == 0: none).
+struct SmoothLaunch {
+  int32_t n_intervals;
+  int64_t max_elements;   // largest n_cb * n_ant * n_sub * 2 of the batch
+  int32_t finish_threads; // block size of the per-(direction, block) pass
+};
+// Smoothness constraint of one solve: the smoothed series, then per
+// (direction, block) the smoothed solutions written back with their
+// convergence partials (replacing antenna_solve's).
+void LaunchSmoothing(const IterArgs& a, hipStream_t stream);
+int SmoothFinishThreads(int n_ant, int max_nsd);
 // One batched iteration: the correlate once per tile group, then the antenna
 // solve over n_items items and the decision for n_intervals intervals (zero
 // counts skip a kernel).
 void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
                  int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
                  size_t solve_lds, int solve_threads, int n_corr, bool f16, int waves, int rows,
-                 hipStream_t stream);
+                 const SmoothLaunch& smooth, hipStream_t stream);
 // Gram iteration: the cross products once per solve, then per iteration the
 // correlate partials from them and the current gains.
 void LaunchGram(const IterArgs& a, bool f16, int waves, hipStream_t stream);
@@ -161,7 +182,7 @@ void LaunchBatchGram(const IterArgs* d_args, const BatchTile* d_tiles, const Til
 void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
                       int n_groups, int apply_ant, const BatchTile* d_items, int n_items,
                       int n_intervals, size_t solve_lds, int solve_threads, bool f16,
-                      hipStream_t stream);
+                      const SmoothLaunch& smooth, hipStream_t stream);
 // Current solutions (by iteration parity) of batched interval `interval`
 // into dst, `count` complex values.
 void LaunchSnapshot(const IterArgs* d_args, int interval, int64_t count, double2* dst,
@@ -170,7 +191,7 @@ void LaunchBatchPrologue(const IterArgs* d_args, const BatchTile* d_items, int n
                          hipStream_t stream);
 void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_items,
                      int n_intervals, size_t solve_lds, int solve_threads, int n_corr, bool f16,
-                     hipStream_t stream);
+                     const SmoothLaunch& smooth, hipStream_t stream);
 
 // Layout conversion: reference order [plane][t][bl][ch][n_corr] complex64 ->
 // device order [plane][t][ch][bl_stride] (complex64 or FP16/scale).

@@ -643,6 +643,30 @@ __device__ void BlockSum(double& v, int& c, double* red, int* redc) {
   __syncthreads();
 }
 
+// Convergence term of one solution pair (SolverBase.cc:254-268) between its
+// repaired old value and its next value, and the next value's |pol 0| for the
+// repair mean (SolverBase.cc:342-371), added to the running sums.
+__device__ __forceinline__ void AddPairTerms(const double2 (&old)[2], const double2 (&next)[2],
+                                             double& conv_sum, int& conv_n, double& mean_sum,
+                                             int& mean_n) {
+  const double n0 = cnorm(old[0]), n1 = cnorm(old[1]);
+  const double2 inv0 = make_double2(old[0].x / n0, -old[0].y / n0);
+  const double2 inv1 = make_double2(old[1].x / n1, -old[1].y / n1);
+  if (cfinite(inv0) && cfinite(inv1)) {
+    const double2 d0 = cmul(make_double2(next[0].x - old[0].x, next[0].y - old[0].y), inv0);
+    const double2 d1 = cmul(make_double2(next[1].x - old[1].x, next[1].y - old[1].y), inv1);
+    const double sumabs = hypot(d0.x, d0.y) + hypot(d1.x, d1.y);
+    if (isfinite(sumabs)) {
+      conv_sum += sumabs;
+      conv_n += 2;
+    }
+  }
+  if (cfinite(next[0]) && cfinite(next[1])) {
+    mean_sum += hypot(next[0].x, next[0].y);
+    mean_n += 1;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // antenna_solve: grid (n_dir, n_cb); `split` threads per (antenna, solution
 // of d) share that antenna's baseline list and are combined by a fixed
@@ -789,26 +813,7 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
       nxt[pair * 2 + p] = stepped[p];
       if (a.den_out) a.den_out[pair * 2 + p] = dn[p];
     }
-    // Convergence term of this solution pair (SolverBase.cc:254-268).
-    const double n0 = cnorm(old[0]), n1 = cnorm(old[1]);
-    const double2 inv0 = make_double2(old[0].x / n0, -old[0].y / n0);
-    const double2 inv1 = make_double2(old[1].x / n1, -old[1].y / n1);
-    if (cfinite(inv0) && cfinite(inv1)) {
-      const double2 d0 =
-          cmul(make_double2(stepped[0].x - old[0].x, stepped[0].y - old[0].y), inv0);
-      const double2 d1 =
-          cmul(make_double2(stepped[1].x - old[1].x, stepped[1].y - old[1].y), inv1);
-      const double sumabs = hypot(d0.x, d0.y) + hypot(d1.x, d1.y);
-      if (isfinite(sumabs)) {
-        conv_sum = sumabs;
-        conv_n = 2;
-      }
-    }
-    // Mean |pol 0| over finite pairs, for the next iteration's repair.
-    if (cfinite(stepped[0]) && cfinite(stepped[1])) {
-      mean_sum = hypot(stepped[0].x, stepped[0].y);
-      mean_n = 1;
-    }
+    AddPairTerms(old, stepped, conv_sum, conv_n, mean_sum, mean_n);
   }
   BlockSum2(conv_sum, conv_n, mean_sum, mean_n, red, redc);
   const int wg = cb * a.n_dir + d;
@@ -825,6 +830,113 @@ template <int NC, bool F16>
 __global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterArgs a) {
   if (a.state->done) return;
   AntennaSolveBlock<NC, F16>(a, blockIdx.x, blockIdx.y);
+}
+
+// ---------------------------------------------------------------------------
+// Smoothness constraint (SmoothnessConstraint.cc:50-97 with the Gaussian
+// kernel of KernelSmoother.h): each (antenna, sub-solution, polarization)
+// series of the stepped solutions over the channel blocks is replaced by its
+// weighted kernel mean; non-finite values enter with weight 0, and a block
+// whose weights sum to 0 becomes NaN. The kernel windows [first, last) and
+// the frequency corrections of each output block depend only on the
+// frequencies; the host prepares them. The size factor of a series is
+// (1 / dd_factor) / antenna_factor. One thread per (output block, series);
+// the results go to sm_out, and a second pass per (direction, block) writes
+// them back with the convergence partials, which replace antenna_solve's.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void SmoothElement(const IterArgs& a, const int64_t e) {
+  const int n_series = a.n_ant * a.n_sub * 2;
+  const int i = int(e / n_series);
+  const int idx = int(e - int64_t(i) * n_series);
+  const int sub = (idx >> 1) % a.n_sub;
+  const int ant = idx / (a.n_sub * 2);
+  const double2* nxt = (a.state->iteration & 1) ? a.sol0 : a.sol1;
+  const double* w = a.sm_weights + (a.sm_sub_weights ? size_t(sub) * a.n_ant * a.n_cb : 0) +
+                    size_t(ant) * a.n_cb;
+  const double correction = a.sm_corr[i] * ((1.0 / a.sm_dd_factor[sub]) / a.sm_ant_factor[ant]);
+  const double fi = a.sm_freq[i];
+  double sr = 0.0, si = 0.0, ws = 0.0;
+  for (int j = a.sm_window[2 * i]; j < a.sm_window[2 * i + 1]; ++j) {
+    const double2 v = nxt[size_t(j) * n_series + idx];
+    const bool finite = cfinite(v);
+    const double x = ((fi - a.sm_freq[j]) * correction) / a.sm_bandwidth;
+    const double k = (a.sm_truncate && (x < -1.0 || x > 1.0)) ? 0.0 : exp(-x * x * 9.0);
+    const double wt = k * (finite ? w[j] : 0.0);
+    sr += (finite ? v.x : 0.0) * wt;
+    si += (finite ? v.y : 0.0) * wt;
+    ws += wt;
+  }
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  a.sm_out[e] = ws == 0.0 ? make_double2(nan, nan) : make_double2(sr / ws, si / ws);
+}
+
+__device__ __forceinline__ void SmoothFinishBlock(const IterArgs& a, const int d, const int cb) {
+  __shared__ double red[2 * kBlockThreads / 64];
+  __shared__ int redc[2 * kBlockThreads / 64];
+  const int iteration = a.state->iteration;
+  const double2* cur = (iteration & 1) ? a.sol1 : a.sol0;
+  double2* nxt = (iteration & 1) ? a.sol0 : a.sol1;
+  const int nsd = a.n_sol_dir[d];
+  const int s0 = a.sol_start[d];
+  const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
+  const double mean = a.mean[cb];
+  double conv_sum = 0.0, mean_sum = 0.0;
+  int conv_n = 0, mean_n = 0;
+  for (int item = threadIdx.x; item < a.n_ant * nsd; item += blockDim.x) {
+    const int ant = item / nsd;
+    const size_t pair = gbase + size_t(ant) * a.n_sub + s0 + (item - ant * nsd);
+    double2 next[2], old[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      next[p] = a.sm_out[pair * 2 + p];
+      nxt[pair * 2 + p] = next[p];
+      old[p] = FixedSolution(cur, pair, p, mean);
+    }
+    AddPairTerms(old, next, conv_sum, conv_n, mean_sum, mean_n);
+  }
+  BlockSum2(conv_sum, conv_n, mean_sum, mean_n, red, redc);
+  const int wg = cb * a.n_dir + d;
+  if (threadIdx.x == 0) {
+    a.wg_conv[wg] = conv_sum;
+    a.wg_conv_n[wg] = conv_n;
+    a.wg_mean[wg] = mean_sum;
+    a.wg_mean_n[wg] = mean_n;
+  }
+}
+
+constexpr int kSmoothThreads = 256;
+
+__global__ void __launch_bounds__(kSmoothThreads) SmoothKernel(const IterArgs a) {
+  if (a.state->done) return;
+  const int64_t n = int64_t(a.n_cb) * a.n_ant * a.n_sub * 2;
+  for (int64_t e = int64_t(blockIdx.x) * kSmoothThreads + threadIdx.x; e < n;
+       e += int64_t(gridDim.x) * kSmoothThreads)
+    SmoothElement(a, e);
+}
+
+__global__ void __launch_bounds__(kBlockThreads) SmoothFinishKernel(const IterArgs a) {
+  if (a.state->done) return;
+  SmoothFinishBlock(a, blockIdx.x, blockIdx.y);
+}
+
+// Batched forms: blockIdx.y selects the interval (SmoothKernel), the
+// antenna-solve items select (interval, block, direction) (finish).
+__global__ void __launch_bounds__(kSmoothThreads)
+    BatchSmoothKernel(const IterArgs* __restrict__ args) {
+  const IterArgs& a = args[blockIdx.y];
+  if (!a.sm_freq || a.state->done) return;
+  const int64_t n = int64_t(a.n_cb) * a.n_ant * a.n_sub * 2;
+  for (int64_t e = int64_t(blockIdx.x) * kSmoothThreads + threadIdx.x; e < n;
+       e += int64_t(gridDim.x) * kSmoothThreads)
+    SmoothElement(a, e);
+}
+
+__global__ void __launch_bounds__(kBlockThreads)
+    BatchSmoothFinishKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ items) {
+  const BatchTile t = items[blockIdx.x];
+  const IterArgs& a = args[t.interval];
+  if (!a.sm_freq || a.state->done) return;
+  SmoothFinishBlock(a, t.seg, t.cb);  // seg holds the direction
 }
 
 // One workgroup: the iteration's decision from the per-workgroup partials of
@@ -1221,7 +1333,7 @@ void LaunchBatchGram(const IterArgs* d_args, const BatchTile* d_tiles, const Til
 void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
                       int n_groups, int apply_ant, const BatchTile* d_items, int n_items,
                       int n_intervals, size_t solve_lds, int solve_threads, bool f16,
-                      hipStream_t stream) {
+                      const SmoothLaunch& smooth, hipStream_t stream) {
   for (int g = 0; g < n_groups; ++g) {
     if (groups[g].count == 0) continue;
     CheckGramDirections(groups[g].n_dir);
@@ -1230,7 +1342,7 @@ void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const Ti
                        stream, d_args, d_tiles + groups[g].first);
   }
   DDH_HIP_CHECK(hipGetLastError());
-  LaunchBatchTail(d_args, d_items, n_items, n_intervals, solve_lds, solve_threads, 2, f16,
+  LaunchBatchTail(d_args, d_items, n_items, n_intervals, solve_lds, solve_threads, 2, f16, smooth,
                   stream);
 }
 
@@ -1302,7 +1414,7 @@ void PrepareKernelAttributes() {
 void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
                  int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
                  size_t solve_lds, int solve_threads, int n_corr, bool f16, int waves, int rows,
-                 hipStream_t stream) {
+                 const SmoothLaunch& smooth, hipStream_t stream) {
   DDH_REQUIRE(waves >= 1 && waves * kWave <= kCorrelateMaxThreads, -1, "bad waves per tile");
   for (int g = 0; g < n_groups; ++g) {
     const TileGroup& grp = groups[g];
@@ -1314,12 +1426,12 @@ void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, const TileGro
   }
   DDH_HIP_CHECK(hipGetLastError());
   LaunchBatchTail(d_args, d_items, n_items, n_intervals, solve_lds, solve_threads, n_corr, f16,
-                  stream);
+                  smooth, stream);
 }
 
 void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_items,
                      int n_intervals, size_t solve_lds, int solve_threads, int n_corr, bool f16,
-                     hipStream_t stream) {
+                     const SmoothLaunch& smooth, hipStream_t stream) {
   DDH_REQUIRE(solve_threads >= kWave && solve_threads <= kBlockThreads, -1,
               "bad antenna solve block size");
   // antenna solve + decide only
@@ -1336,6 +1448,17 @@ void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_ite
       else
         hipLaunchKernelGGL((BatchAntennaSolveKernel<4, false>), dim3(n_items), dim3(solve_threads), solve_lds, stream, d_args, d_items);
     }
+  }
+  if (n_items > 0 && smooth.n_intervals > 0) {
+    DDH_REQUIRE(smooth.finish_threads >= kWave && smooth.finish_threads <= kBlockThreads, -1,
+                "bad smoothing block size");
+    const int64_t blocks = std::min<int64_t>((smooth.max_elements + kSmoothThreads - 1) /
+                                                 kSmoothThreads, 1024);
+    hipLaunchKernelGGL(BatchSmoothKernel, dim3(unsigned(std::max<int64_t>(blocks, 1)),
+                                               smooth.n_intervals),
+                       dim3(kSmoothThreads), 0, stream, d_args);
+    hipLaunchKernelGGL(BatchSmoothFinishKernel, dim3(n_items), dim3(smooth.finish_threads), 0,
+                       stream, d_args, d_items);
   }
   if (n_intervals > 0)
     hipLaunchKernelGGL(BatchDecideKernel, dim3(n_intervals), dim3(256), 0, stream, d_args);
@@ -1419,6 +1542,22 @@ void LaunchChi2(const IterArgs& a, int n_corr, bool f16, const double2* sol, dou
   }
   hipLaunchKernelGGL(SumPartialsKernel, dim3(1), dim3(64), 0, stream, partial,
                      Chi2Partials(a), out);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+int SmoothFinishThreads(int n_ant, int max_nsd) {
+  const int items = n_ant * std::max(1, max_nsd);
+  return std::min(kBlockThreads, std::max(kWave, (items + kWave - 1) / kWave * kWave));
+}
+
+void LaunchSmoothing(const IterArgs& a, hipStream_t stream) {
+  if (!a.sm_freq) return;
+  const int64_t n = int64_t(a.n_cb) * a.n_ant * a.n_sub * 2;
+  const int64_t blocks = std::min<int64_t>((n + kSmoothThreads - 1) / kSmoothThreads, 1024);
+  hipLaunchKernelGGL(SmoothKernel, dim3(unsigned(std::max<int64_t>(blocks, 1))),
+                     dim3(kSmoothThreads), 0, stream, a);
+  hipLaunchKernelGGL(SmoothFinishKernel, dim3(a.n_dir, a.n_cb),
+                     dim3(SmoothFinishThreads(a.n_ant, a.max_nsd)), 0, stream, a);
   DDH_HIP_CHECK(hipGetLastError());
 }
 

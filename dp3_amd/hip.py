@@ -129,6 +129,10 @@ SIGNATURES = {
                                C.c_int),
     "ddh_batch_solve": ([C.c_void_p, C.c_void_p, C.POINTER(Result)], C.c_int),
     "ddh_batch_rebind": ([C.c_void_p, C.c_void_p], C.c_int),
+    "ddh_solver_set_smoothness": ([C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                   C.c_int32, C.c_void_p, C.c_void_p], C.c_int),
+    "ddh_solver_set_constraint_weights": ([C.c_void_p, C.c_void_p], C.c_int),
+    "ddh_solver_set_sub_solution_weights": ([C.c_void_p, C.c_void_p], C.c_int),
     "ddh_problem_assign_and_weight_device": ([C.c_void_p, C.c_int32] + [C.c_void_p] * 6
                                              + [C.c_int32, C.c_void_p], C.c_int),
     "ddh_problem_assign_and_weight": ([C.c_void_p, C.c_int32] + [C.c_void_p] * 6
@@ -420,6 +424,32 @@ class Solver:
                                C.cast(ptrs, C.c_void_p), time, C.byref(res)), "ddh_solve")
         return SolveResult(res.iterations, res.constraint_iterations, bool(res.has_converged),
                            res.n_step_magnitudes, res.last_step_magnitude)
+
+    def set_smoothness(self, frequencies, bandwidth_hz, bandwidth_ref_frequency_hz=0.0,
+                       spectral_exponent=0.0, kernel_truncation=True, antenna_factors=None,
+                       dd_factors=None):
+        """Smoothness constraint on the device (None frequencies: off)."""
+        keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                for a in (frequencies, antenna_factors, dd_factors)]
+        _check(lib().ddh_solver_set_smoothness(self.handle, _ptr(keep[0]), float(bandwidth_hz),
+                                               float(bandwidth_ref_frequency_hz),
+                                               float(spectral_exponent), int(kernel_truncation),
+                                               _ptr(keep[1]), _ptr(keep[2])),
+               "ddh_solver_set_smoothness")
+
+    def set_constraint_weights(self, weights):
+        """[n_antennas, n_channel_blocks] weights of the smoothness constraint."""
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        assert w.size == self.n_antennas * self.n_channel_blocks
+        _check(lib().ddh_solver_set_constraint_weights(self.handle, _ptr(w)),
+               "ddh_solver_set_constraint_weights")
+
+    def set_sub_solution_weights(self, weights):
+        """[n_sub, n_antennas, n_channel_blocks] weights of the smoothness constraint."""
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        assert w.size == int(self._nsd.sum()) * self.n_antennas * self.n_channel_blocks
+        _check(lib().ddh_solver_set_sub_solution_weights(self.handle, _ptr(w)),
+               "ddh_solver_set_sub_solution_weights")
 
     def solve_problem(self, dproblem, solutions):
         res = Result()

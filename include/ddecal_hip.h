@@ -204,6 +204,25 @@ int ddh_problem_synthesize_data(ddh_problem* problem, double* const* gains,
                                 int32_t n_sub, const uint32_t* sol_index,
                                 float noise_sigma, uint32_t seed);
 
+// Smoothness constraint (ddecal/constraints/SmoothnessConstraint.h:23-75 with
+// the Gaussian kernel of KernelSmoother.h), applied on the device to the
+// next solutions after every step, before the convergence test
+// (SolverBase::ApplyConstraints, SolverBase.cc:174-190; the constraint is
+// always satisfied). Replaces the constructor, Initialize (the channel-block
+// frequencies, ascending, Hz), SetAntennaFactors ([n_antennas], NULL: ones)
+// and SetDdSmoothingFactors ([n_sub], NULL: ones) of SmoothnessConstraint.cc:
+// 11-46 and SolverFactory.cc:427-435. NULL frequencies switch the constraint
+// off. Weights default to ones until set.
+int ddh_solver_set_smoothness(ddh_solver* solver, const double* channel_block_frequencies,
+                              double bandwidth_hz, double bandwidth_ref_frequency_hz,
+                              double spectral_exponent, int32_t kernel_truncation,
+                              const double* antenna_factors, const double* dd_factors);
+// Constraint::SetWeights (DDECal.cc:615-621): [n_antennas][n_channel_blocks].
+int ddh_solver_set_constraint_weights(ddh_solver* solver, const double* weights);
+// Constraint::SetSubSolutionWeights (SolverBase.cc:436-442):
+// [n_sub][n_antennas][n_channel_blocks], used instead of the plain weights.
+int ddh_solver_set_sub_solution_weights(ddh_solver* solver, const double* weights);
+
 // Kernel-level timing of the last solve: total device time of the
 // correlate kernel and number of its launches that did work.
 int ddh_solver_last_timing(const ddh_solver* solver, double* correlate_ms,
