@@ -70,7 +70,72 @@ ddh_solver* IterativeDiagonalSolverHip::DeviceSolver(int n_correlations) {
   solver_ = ddh_solver_create(&c);
   if (!solver_) ThrowLastError("Could not create the HIP solver");
   key_ = key;
+  ApplyConstraint(solver_);
   return solver_;
+}
+
+void IterativeDiagonalSolverHip::CheckSmoothness(const Smoothness& m) const {
+  if (m.channel_block_frequencies.empty() || NAntennas() == 0) return;  // checked at Solve
+  if (m.channel_block_frequencies.size() != NChannelBlocks())
+    throw std::runtime_error("Smoothness constraint: one frequency per channel block is needed");
+  if (!m.antenna_factors.empty() && m.antenna_factors.size() != NAntennas())
+    throw std::runtime_error("Smoothness constraint: one factor per antenna is needed");
+  if (!m.dd_factors.empty() && m.dd_factors.size() != NSubSolutions())
+    throw std::runtime_error("Invalid setting for the dd factors option of smoothness constraint");
+}
+
+void IterativeDiagonalSolverHip::ApplyConstraint(ddh_solver* solver) const {
+  const Smoothness& m = smoothness_;
+  if (m.channel_block_frequencies.empty()) {
+    if (ddh_solver_set_smoothness(solver, nullptr, 0.0, 0.0, 0.0, 0, nullptr, nullptr) != DDH_OK)
+      ThrowLastError("Could not remove the smoothness constraint");
+    return;
+  }
+  CheckSmoothness(m);
+  if (ddh_solver_set_smoothness(solver, m.channel_block_frequencies.data(), m.bandwidth_hz,
+                                m.bandwidth_ref_frequency_hz, m.spectral_exponent,
+                                m.kernel_truncation ? 1 : 0,
+                                m.antenna_factors.empty() ? nullptr : m.antenna_factors.data(),
+                                m.dd_factors.empty() ? nullptr : m.dd_factors.data()) != DDH_OK)
+    ThrowLastError("Could not set the smoothness constraint");
+  if (!sub_solution_weights_.empty()) {
+    if (ddh_solver_set_sub_solution_weights(solver, sub_solution_weights_.data()) != DDH_OK)
+      ThrowLastError("Could not set the constraint weights");
+  } else if (!constraint_weights_.empty()) {
+    if (ddh_solver_set_constraint_weights(solver, constraint_weights_.data()) != DDH_OK)
+      ThrowLastError("Could not set the constraint weights");
+  }
+}
+
+void IterativeDiagonalSolverHip::SetSmoothness(const Smoothness& smoothness) {
+  CheckSmoothness(smoothness);
+  smoothness_ = smoothness;
+  if (solver_) ApplyConstraint(solver_);
+}
+
+void IterativeDiagonalSolverHip::SetConstraintWeights(const std::vector<double>& weights) {
+  if (weights.size() != NAntennas() * NChannelBlocks())
+    throw std::runtime_error("Constraint weights: n_antennas x n_channel_blocks values needed");
+  constraint_weights_ = weights;
+  sub_solution_weights_.clear();
+  if (solver_ && !smoothness_.channel_block_frequencies.empty())
+    if (ddh_solver_set_constraint_weights(solver_, constraint_weights_.data()) != DDH_OK)
+      ThrowLastError("Could not set the constraint weights");
+}
+
+void IterativeDiagonalSolverHip::SetDdConstraintWeights(
+    const std::vector<std::vector<double>>& weights) {
+  if (weights.size() != NSubSolutions())
+    throw std::runtime_error("Dd constraint weights: one vector per sub-solution needed");
+  sub_solution_weights_.clear();
+  for (const std::vector<double>& w : weights) {
+    if (w.size() != NAntennas() * NChannelBlocks())
+      throw std::runtime_error("Dd constraint weights: n_antennas x n_channel_blocks values needed");
+    sub_solution_weights_.insert(sub_solution_weights_.end(), w.begin(), w.end());
+  }
+  if (solver_ && !smoothness_.channel_block_frequencies.empty())
+    if (ddh_solver_set_sub_solution_weights(solver_, sub_solution_weights_.data()) != DDH_OK)
+      ThrowLastError("Could not set the constraint weights");
 }
 
 template <typename SolveDataType>

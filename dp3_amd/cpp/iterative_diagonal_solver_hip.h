@@ -9,7 +9,8 @@
 // errors raise std::runtime_error and unsupported data kinds
 // std::logic_error. The device solver is created on the first Solve after a
 // settings change and cached, as the reference caches its buffers
-// (keep_host_buffers, Settings.cc:212).
+// (keep_host_buffers, Settings.cc:212). The one constraint supported is the
+// smoothness constraint (SetSmoothness), which the device loop applies.
 #ifndef DP3AMD_ITERATIVE_DIAGONAL_SOLVER_HIP_H
 #define DP3AMD_ITERATIVE_DIAGONAL_SOLVER_HIP_H
 
@@ -46,16 +47,42 @@ class IterativeDiagonalSolverHip final : public SolverBase {
   // MakeResult).
   bool LastSolveConverged() const { return last_converged_; }
 
+  // The smoothness constraint, applied on the device after every step: the
+  // parameters of DP3's SmoothnessConstraint (SmoothnessConstraint.h:23-75,
+  // Gaussian kernel), its Initialize frequencies (channel-block frequencies,
+  // ascending) and its SetAntennaFactors / SetDdSmoothingFactors (empty:
+  // ones). Empty frequencies remove the constraint.
+  struct Smoothness {
+    std::vector<double> channel_block_frequencies;
+    double bandwidth_hz = 0.0;
+    double bandwidth_ref_frequency_hz = 0.0;
+    double spectral_exponent = 0.0;
+    bool kernel_truncation = true;
+    std::vector<double> antenna_factors;
+    std::vector<double> dd_factors;
+  };
+  void SetSmoothness(const Smoothness& smoothness);
+  // Constraint::SetWeights: [n_antennas][n_channel_blocks] (DDECal.cc:615-621).
+  void SetConstraintWeights(const std::vector<double>& weights);
+  // SolverBase::SetDdConstraintWeights (SolverBase.cc:436-442): per
+  // sub-solution [n_antennas][n_channel_blocks] weights.
+  void SetDdConstraintWeights(const std::vector<std::vector<double>>& weights);
+
  private:
   template <typename SolveDataType>
   SolveResult SolveImpl(const SolveDataType& data, std::vector<std::vector<DComplex>>& solutions,
                         int n_correlations);
   ddh_solver* DeviceSolver(int n_correlations);
+  void ApplyConstraint(ddh_solver* solver) const;
+  void CheckSmoothness(const Smoothness& smoothness) const;
 
   int model_storage_;
   ddh_solver* solver_ = nullptr;
   std::vector<int64_t> key_;  // settings the cached device solver was made with
   bool last_converged_ = false;
+  Smoothness smoothness_;
+  std::vector<double> constraint_weights_;       // [n_ant][n_cb], empty: ones
+  std::vector<double> sub_solution_weights_;     // [n_sub][n_ant][n_cb], empty: none
 };
 
 }  // namespace dp3amd

@@ -149,7 +149,8 @@ struct Fixture {
   }
 
   // The CPU restatement of the reference solver on the same data.
-  std::vector<std::vector<DComplex>> OracleSolve(int n_corr, orc_result* result) const {
+  std::vector<std::vector<DComplex>> OracleSolve(int n_corr, orc_result* result,
+                                                 const orc_smoothness* smoothness = nullptr) const {
     orc_settings s{};
     s.n_antennas = n_ant;
     s.n_directions = n_dir;
@@ -165,8 +166,13 @@ struct Fixture {
     s.n_threads = 4;
     std::vector<double> flat(size_t(n_cb) * n_ant * NSub() * 4);
     for (size_t i = 0; i < flat.size(); i += 2) flat[i] = 1.0;
-    orc_solve(&s, offsets.data(), data.data(), model.data(), ant1.data(), ant2.data(),
-              sol_map.data(), flat.data(), result, nullptr);
+    if (smoothness)
+      orc_solve_constrained(&s, smoothness, offsets.data(), data.data(), model.data(),
+                            ant1.data(), ant2.data(), sol_map.data(), flat.data(), result,
+                            nullptr);
+    else
+      orc_solve(&s, offsets.data(), data.data(), model.data(), ant1.data(), ant2.data(),
+                sol_map.data(), flat.data(), result, nullptr);
     std::vector<std::vector<DComplex>> out = UnitSolutions();
     for (int cb = 0; cb < n_cb; ++cb)
       for (size_t i = 0; i < out[cb].size(); ++i)
@@ -212,6 +218,59 @@ void DdIntervals(int n_corr) {
   // The factored per-baseline sums with an FP64 antenna reduction reach the
   // accuracy no later than the reference's FP32 per-visibility sums.
   CHECK(int64_t(result.iterations) <= oracle_result.iterations);
+}
+
+// The smoothness constraint through the SolverBase-compatible interface
+// (SetSmoothness, SetDdConstraintWeights), against the oracle's constrained
+// solve on the same fixture. The fixture's gains do not vary over the
+// channel blocks, so the constraint leaves them in place: the checks are
+// parity with the oracle and CheckDualResults.
+void SmoothnessConstraint() {
+  Fixture f({1, 2, 5}, 2);
+  const DuoSolveData data = f.Make<DuoSolveData>(2);
+  IterativeDiagonalSolverHip solver;
+  f.InitializeSolver(solver);
+  IterativeDiagonalSolverHip::Smoothness m;
+  for (int cb = 0; cb < f.n_cb; ++cb) m.channel_block_frequencies.push_back(120e6 + 7.5e6 * cb);
+  m.bandwidth_hz = 10e6;
+  m.spectral_exponent = -1.0;
+  for (int a = 0; a < f.n_ant; ++a) m.antenna_factors.push_back(0.8 + 0.01 * a);
+  for (int sub = 0; sub < f.NSub(); ++sub) m.dd_factors.push_back(1.0 + 0.1 * sub);
+  solver.SetSmoothness(m);
+  std::vector<std::vector<double>> weights(f.NSub());
+  for (int sub = 0; sub < f.NSub(); ++sub)
+    for (int i = 0; i < f.n_ant * f.n_cb; ++i) weights[sub].push_back(0.5 + 0.01 * ((i + sub) % 37));
+  solver.SetDdConstraintWeights(weights);
+  auto solutions = f.UnitSolutions();
+  const SolverBase::SolveResult result = solver.Solve(data, solutions, 0.0);
+  f.CheckDualResults(solutions, 1.0e-2);
+  std::vector<double> flat_weights;
+  for (const auto& w : weights) flat_weights.insert(flat_weights.end(), w.begin(), w.end());
+  orc_smoothness c{};
+  c.frequencies = m.channel_block_frequencies.data();
+  c.bandwidth_hz = m.bandwidth_hz;
+  c.bandwidth_ref_frequency_hz = m.bandwidth_ref_frequency_hz;
+  c.spectral_exponent = m.spectral_exponent;
+  c.kernel_truncation = 1;
+  c.sub_solution_weights = flat_weights.data();
+  c.antenna_factors = m.antenna_factors.data();
+  c.dd_factors = m.dd_factors.data();
+  orc_result oracle_result{};
+  const auto reference = f.OracleSolve(2, &oracle_result, &c);
+  const double err = PhaseReferencedError(solutions, reference, f.n_ant, f.NSub());
+  std::printf("smoothness: iterations %zu (oracle %lld), gain error vs oracle %.2e\n",
+              result.iterations, static_cast<long long>(oracle_result.iterations), err);
+  CHECK(err <= 1e-4);
+  CHECK(int64_t(result.iterations) <= oracle_result.iterations);
+  // a wrong dd factor count is a settings error (SmoothnessConstraint.cc:36-45)
+  m.dd_factors.pop_back();
+  bool thrown = false;
+  try {
+    solver.SetSmoothness(m);
+  } catch (const std::runtime_error&) {
+    thrown = true;
+  }
+  CHECK(thrown);
 }
 
 void IterationSemantics() {
@@ -284,6 +343,7 @@ int main() {
       {"iterative_duo_diagonal_dd_intervals", [] { DdIntervals<DuoSolveData>(2); }},
       {"iterative_diagonal_dd_intervals", [] { DdIntervals<FullSolveData>(4); }},
       {"iteration_semantics", IterationSemantics},
+      {"smoothness_constraint", SmoothnessConstraint},
       {"errors", Errors},
   };
   for (const auto& c : cases) {
