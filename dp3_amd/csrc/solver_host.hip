@@ -150,6 +150,8 @@ void ValidateConfig(const ddh_config& c) {
   DDH_REQUIRE(c.n_channel_blocks >= 1, -1, "n_channel_blocks must be >= 1");
   DDH_REQUIRE(c.n_correlations == 2 || c.n_correlations == 4, -1,
               "n_correlations must be 2 (Duo) or 4 (Full)");
+  DDH_REQUIRE(c.n_solution_polarizations == 1 || c.n_solution_polarizations == 2, -1,
+              "n_solution_polarizations must be 1 (scalar) or 2 (diagonal)");
   DDH_REQUIRE(c.n_solutions_per_direction != nullptr, -1,
               "n_solutions_per_direction is required");
   DDH_REQUIRE(c.max_iterations >= 1 && c.max_iterations <= 100000, -1,
@@ -419,6 +421,7 @@ IterArgs MakeArgs(ddh_solver& s, ddh_problem& p) {
   a.split = 1;
   while (a.split < 8 && c.n_antennas * max_nsd * a.split * 2 <= threads) a.split *= 2;
   a.max_nsd = max_nsd;
+  a.scalar = c.n_solution_polarizations == 1 ? 1 : 0;
   if (s.sm_on) {
     a.sm_freq = s.sm_freq.ptr;
     a.sm_corr = s.sm_corr.ptr;
@@ -515,14 +518,49 @@ void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
     DestroyGraph(s);
 }
 
+// Solutions cross the C-ABI in the caller's layout: per (antenna,
+// sub-solution) two values (diagonal solver) or one (scalar solver). The
+// device holds two per pair; a scalar solver keeps them equal.
+void SolutionsIn(const ddh_solver& s, double2* dev, const double* host, size_t n_pairs,
+                 hipStream_t stream) {
+  if (s.cfg.c.n_solution_polarizations == 2) {
+    DDH_HIP_CHECK(hipMemcpyAsync(dev, host, n_pairs * 2 * sizeof(double2), hipMemcpyHostToDevice,
+                                 stream));
+    return;
+  }
+  std::vector<double2> tmp(n_pairs * 2);
+  for (size_t i = 0; i < n_pairs; ++i)
+    tmp[2 * i] = tmp[2 * i + 1] = make_double2(host[2 * i], host[2 * i + 1]);
+  DDH_HIP_CHECK(hipMemcpyAsync(dev, tmp.data(), tmp.size() * sizeof(double2),
+                               hipMemcpyHostToDevice, stream));
+  DDH_HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+// Device pairs to the caller's layout (complex values, or with `real` one
+// double per polarization, as the one-step denominators).
+void SolutionsOut(const ddh_solver& s, double* host, const void* dev, size_t n_pairs,
+                  hipStream_t stream, bool real = false) {
+  const size_t value = real ? sizeof(double) : sizeof(double2);
+  if (s.cfg.c.n_solution_polarizations == 2) {
+    DDH_HIP_CHECK(hipMemcpyAsync(host, dev, n_pairs * 2 * value, hipMemcpyDeviceToHost, stream));
+    return;
+  }
+  std::vector<double> tmp(n_pairs * 2 * value / sizeof(double));
+  DDH_HIP_CHECK(hipMemcpyAsync(tmp.data(), dev, n_pairs * 2 * value, hipMemcpyDeviceToHost,
+                               stream));
+  DDH_HIP_CHECK(hipStreamSynchronize(stream));
+  const size_t words = value / sizeof(double);
+  for (size_t i = 0; i < n_pairs; ++i)
+    for (size_t w = 0; w < words; ++w) host[i * words + w] = tmp[2 * i * words + w];
+}
+
 void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* result) {
   const ddh_config& c = s.cfg.c;
   CheckMatch(s, p);
   const size_t sol_per_cb = size_t(c.n_antennas) * s.cfg.n_sub * 2;
   PrepareBuffers(s, p);
   for (int cb = 0; cb < c.n_channel_blocks; ++cb)
-    DDH_HIP_CHECK(hipMemcpyAsync(s.sol0.ptr + cb * sol_per_cb, solutions[cb],
-                                 sol_per_cb * sizeof(double2), hipMemcpyHostToDevice, s.stream));
+    SolutionsIn(s, s.sol0.ptr + cb * sol_per_cb, solutions[cb], sol_per_cb / 2, s.stream);
   const IterArgs a = MakeArgs(s, p);
   const int limit = std::max(c.max_iterations, c.min_iterations);
   const bool graph = s.use_graph && !s.profile;
@@ -567,8 +605,7 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
   DDH_REQUIRE(st.done, -3, "solver did not reach its stopping criterion");
   const double2* final_sol = (st.iteration & 1) ? s.sol1.ptr : s.sol0.ptr;
   for (int cb = 0; cb < c.n_channel_blocks; ++cb)
-    DDH_HIP_CHECK(hipMemcpyAsync(solutions[cb], final_sol + cb * sol_per_cb,
-                                 sol_per_cb * sizeof(double2), hipMemcpyDeviceToHost, s.stream));
+    SolutionsOut(s, solutions[cb], final_sol + cb * sol_per_cb, sol_per_cb / 2, s.stream);
   DDH_HIP_CHECK(hipStreamSynchronize(s.stream));
   float ms = 0.0f;
   DDH_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
@@ -717,6 +754,7 @@ void ddh_config_defaults(ddh_config* c) {
   c->phase_only = 0;
   c->model_storage = DDH_MODEL_C64;
   c->iteration = DDH_ITERATION_STREAM;
+  c->n_solution_polarizations = 2;
 }
 
 ddh_solver* ddh_solver_create(const ddh_config* config) {
@@ -1230,8 +1268,7 @@ void UploadSolutions(ddh_batch& b, double* const* const* solutions, Dst&& dst) {
     const ddh_solver& s = *b.solvers[i];
     const size_t per_cb = size_t(s.cfg.c.n_antennas) * s.cfg.n_sub * 2;
     for (int cb = 0; cb < s.cfg.c.n_channel_blocks; ++cb)
-      DDH_HIP_CHECK(hipMemcpyAsync(dst(i) + cb * per_cb, solutions[i][cb],
-                                   per_cb * sizeof(double2), hipMemcpyHostToDevice, b.stream));
+      SolutionsIn(s, dst(i) + cb * per_cb, solutions[i][cb], per_cb / 2, b.stream);
   }
 }
 
@@ -1392,8 +1429,7 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
     const size_t per_cb = size_t(s.cfg.c.n_antennas) * s.cfg.n_sub * 2;
     const double2* fin = (st.iteration & 1) ? b.args[i].sol1 : b.args[i].sol0;
     for (int cb = 0; cb < s.cfg.c.n_channel_blocks; ++cb)
-      DDH_HIP_CHECK(hipMemcpyAsync(solutions[i][cb], fin + cb * per_cb, per_cb * sizeof(double2),
-                                   hipMemcpyDeviceToHost, b.stream));
+      SolutionsOut(s, solutions[i][cb], fin + cb * per_cb, per_cb / 2, b.stream);
     if (results) {
       results[i].iterations = st.has_converged ? st.iteration : st.iteration + 1;
       results[i].constraint_iterations = 0;
@@ -1689,8 +1725,7 @@ int ddh_one_step(ddh_solver* s, ddh_problem* p, double* const* solutions_in,
     const size_t per_cb = size_t(c.n_antennas) * s->cfg.n_sub * 2;
     s->den.Resize(per_cb * c.n_channel_blocks);
     for (int cb = 0; cb < c.n_channel_blocks; ++cb)
-      DDH_HIP_CHECK(hipMemcpyAsync(s->sol0.ptr + cb * per_cb, solutions_in[cb],
-                                   per_cb * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+      SolutionsIn(*s, s->sol0.ptr + cb * per_cb, solutions_in[cb], per_cb / 2, s->stream);
     IterArgs a = MakeArgs(*s, *p);
     a.step_size = 1.0;  // stepped == next: the undamped update
     a.phase_only = 0;
@@ -1699,11 +1734,9 @@ int ddh_one_step(ddh_solver* s, ddh_problem* p, double* const* solutions_in,
     LaunchCorrelation(*s, a, *p);
     LaunchAntennaSolve(a, p->n_corr, p->f16, s->stream);
     for (int cb = 0; cb < c.n_channel_blocks; ++cb) {
-      DDH_HIP_CHECK(hipMemcpyAsync(next_out[cb], s->sol1.ptr + cb * per_cb,
-                                   per_cb * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+      SolutionsOut(*s, next_out[cb], s->sol1.ptr + cb * per_cb, per_cb / 2, s->stream);
       if (den_out)
-        DDH_HIP_CHECK(hipMemcpyAsync(den_out[cb], s->den.ptr + cb * per_cb, per_cb * sizeof(double),
-                                     hipMemcpyDeviceToHost, s->stream));
+        SolutionsOut(*s, den_out[cb], s->den.ptr + cb * per_cb, per_cb / 2, s->stream, true);
     }
     DDH_HIP_CHECK(hipStreamSynchronize(s->stream));
   });
@@ -1717,8 +1750,7 @@ int ddh_chi2(ddh_solver* s, ddh_problem* p, double* const* solutions, double* ch
     const ddh_config& c = s->cfg.c;
     const size_t per_cb = size_t(c.n_antennas) * s->cfg.n_sub * 2;
     for (int cb = 0; cb < c.n_channel_blocks; ++cb)
-      DDH_HIP_CHECK(hipMemcpyAsync(s->sol1.ptr + cb * per_cb, solutions[cb],
-                                   per_cb * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+      SolutionsIn(*s, s->sol1.ptr + cb * per_cb, solutions[cb], per_cb / 2, s->stream);
     const IterArgs a = MakeArgs(*s, *p);
     const int n_partial = Chi2Partials(a);
     s->chi2.Resize(size_t(n_partial) + 1);

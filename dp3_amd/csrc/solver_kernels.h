@@ -94,6 +94,7 @@ struct IterArgs {
   double* den_out;           // optional [n_cb][n_ant][n_sub][2] denominators
   int32_t split;             // threads per (antenna, solution) in antenna_solve
   int32_t max_nsd;           // largest n_sol_dir entry
+  int32_t scalar;            // 1: one gain for both polarizations (scalar solver)
   // Smoothness constraint on the stepped solutions (null sm_freq: none).
   const double* sm_freq;        // [n_cb] channel-block frequencies (Hz)
   const double* sm_corr;        // [n_cb] frequency correction of the kernel size per block

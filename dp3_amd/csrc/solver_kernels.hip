@@ -643,12 +643,41 @@ __device__ void BlockSum(double& v, int& c, double* red, int* redc) {
   __syncthreads();
 }
 
+// a / b, scaled by the larger part of b (Smith's algorithm).
+__device__ __forceinline__ double2 CDiv(double2 a, double2 b) {
+  if (fabs(b.y) <= fabs(b.x)) {
+    const double r = b.y / b.x, den = b.x + b.y * r;
+    return make_double2((a.x + a.y * r) / den, (a.y - a.x * r) / den);
+  }
+  const double r = b.x / b.y, den = b.x * r + b.y;
+  return make_double2((a.x * r + a.y) / den, (a.y * r - a.x) / den);
+}
+
 // Convergence term of one solution pair (SolverBase.cc:254-268) between its
 // repaired old value and its next value, and the next value's |pol 0| for the
-// repair mean (SolverBase.cc:342-371), added to the running sums.
+// repair mean (SolverBase.cc:342-371), added to the running sums. A scalar
+// solver's pair holds one gain twice: its term is that of one solution,
+// |(next - old) / old| unless old is 0 (:244-253), and its mean is over the
+// finite solutions (:308-333).
 __device__ __forceinline__ void AddPairTerms(const double2 (&old)[2], const double2 (&next)[2],
-                                             double& conv_sum, int& conv_n, double& mean_sum,
-                                             int& mean_n) {
+                                             bool scalar, double& conv_sum, int& conv_n,
+                                             double& mean_sum, int& mean_n) {
+  if (scalar) {
+    if (old[0].x != 0.0 || old[0].y != 0.0) {
+      const double2 q =
+          CDiv(make_double2(next[0].x - old[0].x, next[0].y - old[0].y), old[0]);
+      const double a = hypot(q.x, q.y);
+      if (isfinite(a)) {
+        conv_sum += a;
+        conv_n += 1;
+      }
+    }
+    if (cfinite(next[0])) {
+      mean_sum += hypot(next[0].x, next[0].y);
+      mean_n += 1;
+    }
+    return;
+  }
   const double n0 = cnorm(old[0]), n1 = cnorm(old[1]);
   const double2 inv0 = make_double2(old[0].x / n0, -old[0].y / n0);
   const double2 inv1 = make_double2(old[1].x / n1, -old[1].y / n1);
@@ -783,6 +812,13 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
     }
   }
 
+  // Scalar solver: one gain for both polarizations, the sums of both
+  // (IterativeScalarSolver.cc:142-173: Trace and Norm over all correlations).
+  if (a.scalar) {
+    nr[0] = nr[1] = nr[0] + nr[1];
+    ni[0] = ni[1] = ni[0] + ni[1];
+    dn[0] = dn[1] = dn[0] + dn[1];
+  }
   double conv_sum = 0.0, mean_sum = 0.0;
   int conv_n = 0, mean_n = 0;
   if (active && q == 0) {
@@ -813,7 +849,7 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
       nxt[pair * 2 + p] = stepped[p];
       if (a.den_out) a.den_out[pair * 2 + p] = dn[p];
     }
-    AddPairTerms(old, stepped, conv_sum, conv_n, mean_sum, mean_n);
+    AddPairTerms(old, stepped, a.scalar != 0, conv_sum, conv_n, mean_sum, mean_n);
   }
   BlockSum2(conv_sum, conv_n, mean_sum, mean_n, red, redc);
   const int wg = cb * a.n_dir + d;
@@ -892,7 +928,7 @@ __device__ __forceinline__ void SmoothFinishBlock(const IterArgs& a, const int d
       nxt[pair * 2 + p] = next[p];
       old[p] = FixedSolution(cur, pair, p, mean);
     }
-    AddPairTerms(old, next, conv_sum, conv_n, mean_sum, mean_n);
+    AddPairTerms(old, next, a.scalar != 0, conv_sum, conv_n, mean_sum, mean_n);
   }
   BlockSum2(conv_sum, conv_n, mean_sum, mean_n, red, redc);
   const int wg = cb * a.n_dir + d;

@@ -39,6 +39,7 @@ class Config(C.Structure):
         ("phase_only", C.c_int32),
         ("model_storage", C.c_int32),
         ("iteration", C.c_int32),
+        ("n_solution_polarizations", C.c_int32),
     ]
 
 
@@ -340,7 +341,7 @@ class Solver:
                  n_correlations=2, max_iterations=100, min_iterations=0, accuracy=1e-5,
                  constraint_accuracy=1e-4, step_size=0.2, detect_stalling=True,
                  step_diff_sigma=0.1, phase_only=False, model_storage=MODEL_C64,
-                 iteration=ITERATION_STREAM):
+                 iteration=ITERATION_STREAM, n_solution_polarizations=2):
         L = lib()
         self._nsd = np.ascontiguousarray(n_solutions_per_direction, dtype=np.int32)
         cfg = Config()
@@ -360,6 +361,8 @@ class Solver:
         cfg.phase_only = int(phase_only)
         cfg.model_storage = model_storage
         cfg.iteration = iteration
+        cfg.n_solution_polarizations = n_solution_polarizations
+        self.n_solution_polarizations = n_solution_polarizations
         self.config = cfg
         self.n_antennas = n_antennas
         self.n_channel_blocks = n_channel_blocks

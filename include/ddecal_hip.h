@@ -68,6 +68,10 @@ typedef struct {
   int32_t phase_only;                        // default 0
   int32_t model_storage;                     // DDH_MODEL_*
   int32_t iteration;                         // DDH_ITERATION_*
+  // 2: IterativeDiagonalSolver (default); 1: IterativeScalarSolver
+  // (IterativeScalarSolver.cc:30-226), one gain per antenna and solution for
+  // both correlations, solutions [n_ant][n_sub] per channel block
+  int32_t n_solution_polarizations;
 } ddh_config;
 
 // Fills the SolverBase defaults (SolverBase.cc:27-39).

@@ -168,32 +168,13 @@ extern "C" void orc_solvertester_shape(int32_t* n_ant, int32_t* n_dir,
   *n_cb = kNChannelBlocks;
 }
 
-// SetDiagonalSolutions (SolverTester.cc:348-363) followed by
-// FillDdIntervalData(leakage_only = false) (:95-202). Weights are 1 and
-// nothing is flagged, so the AssignAndWeight pass at :199 is the identity.
-extern "C" void orc_solvertester_fill(const int32_t* n_sol_dir, float* data,
-                                      float* model, double* input_solutions) {
-  size_t n_solutions = 0;
-  for (int d = 0; d != kNDirections; ++d) n_solutions += n_sol_dir[d];
-  std::vector<CF> input(kNAntennas * n_solutions * 2);
-  {
-    std::mt19937 mt;
-    std::uniform_real_distribution<float> uniform_sols(1.0, 2.0);
-    for (size_t ant = 0; ant != kNAntennas; ++ant) {
-      for (size_t pol = 0; pol != 2; ++pol) {
-        for (size_t s = 0; s != n_solutions; ++s) {
-          const size_t index = (ant * n_solutions + s) * 2 + pol;
-          input[index] = {uniform_sols(mt), uniform_sols(mt)};
-          if (s != 0) input[index] *= 0.5f;
-        }
-      }
-    }
-  }
-  for (size_t i = 0; i != input.size(); ++i) {
-    input_solutions[2 * i] = input[i].real();
-    input_solutions[2 * i + 1] = input[i].imag();
-  }
+namespace {
 
+// FillDdIntervalData(leakage_only = false) (SolverTester.cc:95-202) for the
+// input solutions `input` ([ant][sub][2]). Weights are 1 and nothing is
+// flagged, so the AssignAndWeight pass at :199 is the identity.
+void FillDdIntervalData(const int32_t* n_sol_dir, const std::vector<CF>& input,
+                        size_t n_solutions, float* data, float* model) {
   std::uniform_real_distribution<float> uniform_data(-1.0, 1.0);
   std::mt19937 mt(0);
   std::vector<std::vector<size_t>> solution_indices(kNDirections);
@@ -248,4 +229,59 @@ extern "C" void orc_solvertester_fill(const int32_t* n_sol_dir, float* data,
       }
     }
   }
+}
+
+void CopyOut(const std::vector<CF>& input, double* input_solutions) {
+  for (size_t i = 0; i != input.size(); ++i) {
+    input_solutions[2 * i] = input[i].real();
+    input_solutions[2 * i + 1] = input[i].imag();
+  }
+}
+
+size_t CountSolutions(const int32_t* n_sol_dir) {
+  size_t n = 0;
+  for (int d = 0; d != kNDirections; ++d) n += n_sol_dir[d];
+  return n;
+}
+
+}  // namespace
+
+// SetDiagonalSolutions (SolverTester.cc:348-363), then FillDdIntervalData.
+extern "C" void orc_solvertester_fill(const int32_t* n_sol_dir, float* data,
+                                      float* model, double* input_solutions) {
+  const size_t n_solutions = CountSolutions(n_sol_dir);
+  std::vector<CF> input(kNAntennas * n_solutions * 2);
+  std::mt19937 mt;
+  std::uniform_real_distribution<float> uniform_sols(1.0, 2.0);
+  for (size_t ant = 0; ant != kNAntennas; ++ant) {
+    for (size_t pol = 0; pol != 2; ++pol) {
+      for (size_t s = 0; s != n_solutions; ++s) {
+        const size_t index = (ant * n_solutions + s) * 2 + pol;
+        input[index] = {uniform_sols(mt), uniform_sols(mt)};
+        if (s != 0) input[index] *= 0.5f;
+      }
+    }
+  }
+  CopyOut(input, input_solutions);
+  FillDdIntervalData(n_sol_dir, input, n_solutions, data, model);
+}
+
+// SetScalarSolutions (SolverTester.cc:331-346: one draw per (antenna,
+// solution), copied to the second polarization), then FillDdIntervalData.
+extern "C" void orc_solvertester_fill_scalar(const int32_t* n_sol_dir, float* data,
+                                             float* model, double* input_solutions) {
+  const size_t n_solutions = CountSolutions(n_sol_dir);
+  std::vector<CF> input(kNAntennas * n_solutions * 2);
+  std::mt19937 mt;
+  std::uniform_real_distribution<float> uniform_sols(1.0, 2.0);
+  for (size_t ant = 0; ant != kNAntennas; ++ant) {
+    for (size_t s = 0; s != n_solutions; ++s) {
+      const size_t index = (ant * n_solutions + s) * 2;
+      input[index] = {uniform_sols(mt), uniform_sols(mt)};
+      if (s != 0) input[index] *= 0.5f;
+      input[index + 1] = input[index];
+    }
+  }
+  CopyOut(input, input_solutions);
+  FillDdIntervalData(n_sol_dir, input, n_solutions, data, model);
 }

@@ -7,6 +7,7 @@
 //
 // It restates, in plain C++, the reference algorithm of lofar-astron/DP3 6.6.0:
 //   ddecal/gain_solvers/IterativeDiagonalSolver.cc:20-222   (solve loop)
+//   ddecal/gain_solvers/IterativeScalarSolver.cc:30-226     (scalar solve)
 //   ddecal/gain_solvers/SolverBase.cc:63-371, SolverBase.h:274-285
 //                                                   (step, convergence, stall,
 //                                                    NaN repair, result)
@@ -90,6 +91,13 @@ int orc_solve(const orc_settings* settings, const int64_t* vis_offsets,
               const float* data, const float* model, const uint32_t* ant1,
               const uint32_t* ant2, const uint32_t* sol_map, double* solutions,
               orc_result* result, double* step_magnitudes);
+// IterativeScalarSolver (IterativeScalarSolver.cc:30-226): one solution
+// polarization, solutions [n_cb][n_ant][n_sub] complex128; with a smoothness
+// constraint when `smoothness` is not NULL.
+int orc_solve_scalar(const orc_settings* settings, const orc_smoothness* smoothness,
+                     const int64_t* vis_offsets, const float* data, const float* model,
+                     const uint32_t* ant1, const uint32_t* ant2, const uint32_t* sol_map,
+                     double* solutions, orc_result* result, double* step_magnitudes);
 // orc_solve with a smoothness constraint applied to the next solutions after
 // every step (SolverBase::ApplyConstraints, SolverBase.cc:166-190; the
 // constraint is always satisfied).
@@ -176,6 +184,11 @@ void orc_solvertester_shape(int32_t* n_ant, int32_t* n_dir, int32_t* n_ch,
                             int32_t* n_t, int32_t* n_cb);
 void orc_solvertester_fill(const int32_t* n_solutions_per_direction,
                            float* data, float* model, double* input_solutions);
+// SolverTester::SetScalarSolutions (SolverTester.cc:331-346) instead of
+// SetDiagonalSolutions, then the same data (input_solutions [ant][sub][2]
+// with equal polarizations).
+void orc_solvertester_fill_scalar(const int32_t* n_solutions_per_direction, float* data,
+                                  float* model, double* input_solutions);
 
 // Restatement of DDECal::CorrectAndSubtractModels (steps/DDECal.cc:992-1124)
 // over one solution interval: data [n_t][n_bl][n_ch][4] and model

@@ -137,6 +137,10 @@ def lib():
         L.orc_simulate.restype = None
         L.orc_solvertester_shape.argtypes = [P] * 5
         L.orc_solvertester_fill.argtypes = [P] * 4
+        L.orc_solvertester_fill_scalar.argtypes = [P] * 4
+        L.orc_solve_scalar.argtypes = [C.POINTER(_Settings), C.c_void_p, P, P, P, P, P, P, P,
+                                       C.POINTER(_Result), P]
+        L.orc_solve_scalar.restype = C.c_int
         L.orc_correct_and_subtract.argtypes = [C.c_int32] * 7 + [P] * 6 + [C.c_int32] * 2
         L.orc_correct_and_subtract.restype = None
         _LIB = L
@@ -170,9 +174,10 @@ def smooth(solutions, smoothness: Smoothness):
 def solve(problem, solutions, *, max_iterations=50, min_iterations=0, accuracy=1e-5,
           constraint_accuracy=1e-4, step_size=0.2, detect_stalling=True,
           step_diff_sigma=0.1, phase_only=False, fp64=False, n_threads=1,
-          smoothness: Smoothness | None = None):
-    """Runs the oracle solver in place on `solutions` ([n_cb, n_ant*n_sub*2] complex128),
-    with the smoothness constraint when given.
+          smoothness: Smoothness | None = None, scalar=False):
+    """Runs the oracle solver in place on `solutions` ([n_cb, n_ant*n_sub*2] complex128,
+    or [n_cb, n_ant*n_sub] for the scalar solver), with the smoothness constraint when
+    given.
 
     `problem` is a dp3_amd.synth.SolveProblem (reference SolveData layout)."""
     n_sol = np.ascontiguousarray(problem.n_solutions_per_direction, dtype=np.int32)
@@ -184,7 +189,14 @@ def solve(problem, solutions, *, max_iterations=50, min_iterations=0, accuracy=1
     assert solutions.dtype == np.complex128 and solutions.flags.c_contiguous
     res = _Result()
     mags = np.zeros(max_iterations + 2)
-    if smoothness is None:
+    if scalar:
+        st, keep = smoothness.struct() if smoothness is not None else (None, None)
+        rc = lib().orc_solve_scalar(C.byref(s), None if st is None else C.addressof(st),
+                                    _p(problem.vis_offsets), _p(problem.data),
+                                    _p(problem.model), _p(problem.ant1), _p(problem.ant2),
+                                    _p(problem.sol_map), _p(solutions), C.byref(res), _p(mags))
+        del keep
+    elif smoothness is None:
         rc = lib().orc_solve(C.byref(s), _p(problem.vis_offsets), _p(problem.data),
                              _p(problem.model), _p(problem.ant1), _p(problem.ant2),
                              _p(problem.sol_map), _p(solutions), C.byref(res), _p(mags))
@@ -320,7 +332,9 @@ def solvertester_shape():
     return tuple(int(x.value) for x in v)  # n_ant, n_dir, n_ch, n_t, n_cb
 
 
-def solvertester_fill(n_sol_per_dir):
+def solvertester_fill(n_sol_per_dir, scalar=False):
+    """SolverTester data for SetDiagonalSolutions, or SetScalarSolutions when
+    `scalar` (input solutions with equal polarizations)."""
     n_ant, n_dir, n_ch, n_t, n_cb = solvertester_shape()
     n_bl = n_ant * (n_ant - 1) // 2
     ns = np.ascontiguousarray(n_sol_per_dir, dtype=np.int32)
@@ -328,7 +342,8 @@ def solvertester_fill(n_sol_per_dir):
     data = np.zeros((n_t, n_bl, n_ch, 4), np.complex64)
     model = np.zeros((n_dir, n_t, n_bl, n_ch, 4), np.complex64)
     inp = np.zeros(n_ant * n_sub * 2, np.complex128)
-    lib().orc_solvertester_fill(_p(ns), _p(data), _p(model), _p(inp))
+    fill = lib().orc_solvertester_fill_scalar if scalar else lib().orc_solvertester_fill
+    fill(_p(ns), _p(data), _p(model), _p(inp))
     return data, model, inp
 
 

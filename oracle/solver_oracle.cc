@@ -3,6 +3,8 @@
 //
 // Reference (lofar-astron/DP3 6.6.0):
 //   ddecal/gain_solvers/IterativeDiagonalSolver.cc:20-222
+//   ddecal/gain_solvers/IterativeScalarSolver.cc:30-226 (scalar mode: one
+//       solution polarization, Trace / Norm sums in double accumulators)
 //   ddecal/gain_solvers/SolveData.h:204-244 (DiagonalAddOrSubtractDirection)
 //   ddecal/gain_solvers/SolverBase.cc:63-371, SolverBase.h:274-285
 //
@@ -129,6 +131,41 @@ inline void AccumulateAnt2(const Full<T>& r, const Full<T>& m,
   den[1] += std::norm(c1) + std::norm(c3);
 }
 
+// IterativeScalarSolver::SolveDirection (IterativeScalarSolver.cc:142-173):
+// with C1 = M^H g_b and C2 = M g_a, antenna 1 adds Trace(R C1) and Norm(C1),
+// antenna 2 adds Trace(R^H C2) and Norm(C2); the products are float, the
+// sums double.
+template <typename T>
+inline void AccumulateScalar(const Duo<T>& r, const Duo<T>& m, std::complex<T> g1,
+                             std::complex<T> g2, DComplex& num1, double& den1, DComplex& num2,
+                             double& den2) {
+  std::complex<T> c1[2], c2[2];
+  for (int p = 0; p != 2; ++p) {
+    c1[p] = std::conj(m.v[p]) * g2;
+    c2[p] = m.v[p] * g1;
+  }
+  num1 += DComplex(r.v[0] * c1[0] + r.v[1] * c1[1]);
+  den1 += double(std::norm(c1[0]) + std::norm(c1[1]));
+  num2 += DComplex(std::conj(r.v[0]) * c2[0] + std::conj(r.v[1]) * c2[1]);
+  den2 += double(std::norm(c2[0]) + std::norm(c2[1]));
+}
+template <typename T>
+inline void AccumulateScalar(const Full<T>& r, const Full<T>& m, std::complex<T> g1,
+                             std::complex<T> g2, DComplex& num1, double& den1, DComplex& num2,
+                             double& den2) {
+  // C1 = M^H g_b = [conj(M0) conj(M2); conj(M1) conj(M3)] g_b, C2 = M g_a
+  const std::complex<T> c1[4] = {std::conj(m.v[0]) * g2, std::conj(m.v[2]) * g2,
+                                 std::conj(m.v[1]) * g2, std::conj(m.v[3]) * g2};
+  const std::complex<T> c2[4] = {m.v[0] * g1, m.v[1] * g1, m.v[2] * g1, m.v[3] * g1};
+  // Trace(R C1) = R0 C1_0 + R1 C1_2 + R2 C1_1 + R3 C1_3
+  num1 += DComplex((r.v[0] * c1[0] + r.v[1] * c1[2]) + (r.v[2] * c1[1] + r.v[3] * c1[3]));
+  den1 += double((std::norm(c1[0]) + std::norm(c1[1])) + (std::norm(c1[2]) + std::norm(c1[3])));
+  // Trace(R^H C2), R^H = [conj(R0) conj(R2); conj(R1) conj(R3)]
+  num2 += DComplex((std::conj(r.v[0]) * c2[0] + std::conj(r.v[2]) * c2[2]) +
+                   (std::conj(r.v[1]) * c2[1] + std::conj(r.v[3]) * c2[3]));
+  den2 += double((std::norm(c2[0]) + std::norm(c2[1])) + (std::norm(c2[2]) + std::norm(c2[3])));
+}
+
 template <typename V>
 struct ChannelBlock {
   size_t n_vis = 0;
@@ -178,8 +215,9 @@ class IterativeDiagonalOracle {
   using T = decltype(V().v[0].real());
   using C = std::complex<T>;
 
-  IterativeDiagonalOracle(const orc_settings& s)
+  IterativeDiagonalOracle(const orc_settings& s, bool scalar = false)
       : s_(s),
+        np_(scalar ? 1 : 2),
         n_ant_(s.n_antennas),
         n_dir_(s.n_directions),
         n_cb_(s.n_channel_blocks),
@@ -197,7 +235,7 @@ class IterativeDiagonalOracle {
     n_block_threads_ = std::min<size_t>(n_threads, n_cb_);
   }
 
-  size_t SolSize() const { return n_ant_ * n_sub_ * 2; }
+  size_t SolSize() const { return n_ant_ * n_sub_ * np_; }
 
   // IterativeDiagonalSolver.cc:100-127
   void PerformIteration(const ChannelBlock<V>& cb, std::vector<V>& residual,
@@ -222,10 +260,12 @@ class IterativeDiagonalOracle {
         const uint32_t a1 = cb.ant1[v];
         const uint32_t a2 = cb.ant2[v];
         const uint32_t si = cb.sol_map[d * cb.n_vis + v];
-        const DComplex* s1 = &solutions[(a1 * n_sub_ + si) * 2];
-        const DComplex* s2 = &solutions[(a2 * n_sub_ + si) * 2];
-        const C ga[2] = {static_cast<C>(s1[0]), static_cast<C>(s1[1])};
-        const C gb[2] = {static_cast<C>(s2[0]), static_cast<C>(s2[1])};
+        const DComplex* s1 = &solutions[(a1 * n_sub_ + si) * np_];
+        const DComplex* s2 = &solutions[(a2 * n_sub_ + si) * np_];
+        // scalar: model * g_a * conj(g_b) per element (IterativeScalarSolver.cc:
+        // 209-222), the same products as Diag(g_a) M Diag(g_b)^H with equal pols
+        const C ga[2] = {static_cast<C>(s1[0]), static_cast<C>(s1[np_ - 1])};
+        const C gb[2] = {static_cast<C>(s2[0]), static_cast<C>(s2[np_ - 1])};
         const V contribution = Contribution(ga, cb.model[d * cb.n_vis + v], gb);
         if (Add)
           AddTo(residual[v], contribution);
@@ -235,10 +275,52 @@ class IterativeDiagonalOracle {
     });
   }
 
+  // IterativeScalarSolver.cc:117-194
+  void SolveDirectionScalar(const ChannelBlock<V>& cb, const std::vector<V>& residual,
+                            size_t d, const DComplex* solutions, DComplex* next) const {
+    const size_t n_dir_sol = n_sol_dir_[d];
+    const uint32_t sol0 = cb.n_vis ? cb.sol_map[d * cb.n_vis] : sol_start_[d];
+    const size_t n_parts = std::max<size_t>(1, std::min(n_sub_threads_, cb.n_vis));
+    std::vector<std::vector<DComplex>> num(n_parts, std::vector<DComplex>(n_ant_ * n_dir_sol));
+    std::vector<std::vector<double>> den(n_parts, std::vector<double>(n_ant_ * n_dir_sol));
+    StaticFor(cb.n_vis, n_parts, [&](size_t b, size_t e, size_t part) {
+      DComplex* lnum = num[part].data();
+      double* lden = den[part].data();
+      for (size_t v = b; v != e; ++v) {
+        const uint32_t a1 = cb.ant1[v];
+        const uint32_t a2 = cb.ant2[v];
+        const uint32_t si = cb.sol_map[d * cb.n_vis + v];
+        const C g1 = static_cast<C>(solutions[a1 * n_sub_ + si]);
+        const C g2 = static_cast<C>(solutions[a2 * n_sub_ + si]);
+        const uint32_t rel = si - sol0;
+        const size_t i1 = a1 * n_dir_sol + rel, i2 = a2 * n_dir_sol + rel;
+        AccumulateScalar(residual[v], cb.model[d * cb.n_vis + v], g1, g2, lnum[i1], lden[i1],
+                         lnum[i2], lden[i2]);
+      }
+    });
+    for (size_t part = 1; part < n_parts; ++part)
+      for (size_t i = 0; i != num[0].size(); ++i) {
+        num[0][i] += num[part][i];
+        den[0][i] += den[part][i];
+      }
+    for (size_t ant = 0; ant != n_ant_; ++ant)
+      for (size_t rel = 0; rel != n_dir_sol; ++rel) {
+        const size_t index = ant * n_dir_sol + rel;
+        DComplex& out = next[ant * n_sub_ + rel + sol0];
+        // a zero denominator gives NaN (IterativeScalarSolver.cc:184-189)
+        out = den[0][index] == 0.0 ? DComplex(std::numeric_limits<float>::quiet_NaN(), 0.0)
+                                   : num[0][index] / den[0][index];
+      }
+  }
+
   // IterativeDiagonalSolver.cc:130-222
   void SolveDirection(const ChannelBlock<V>& cb, const std::vector<V>& residual,
                       size_t d, const DComplex* solutions,
                       DComplex* next) const {
+    if (np_ == 1) {
+      SolveDirectionScalar(cb, residual, d, solutions, next);
+      return;
+    }
     const size_t n_dir_sol = n_sol_dir_[d];
     const uint32_t sol0 = cb.n_vis ? cb.sol_map[d * cb.n_vis] : sol_start_[d];
     const size_t n_parts =
@@ -312,6 +394,20 @@ class IterativeDiagonalOracle {
     }
   }
 
+  // SolverBase.cc:308-333
+  static void MakeSolutionsFinite1Pol(std::vector<DComplex>& sv) {
+    size_t count = 0;
+    double average = 0.0;
+    for (const DComplex& v : sv)
+      if (IsFinite(v)) {
+        average += std::abs(v);
+        ++count;
+      }
+    average = count == 0 ? 1.0 : average / count;
+    for (DComplex& v : sv)
+      if (!IsFinite(v)) v = average;
+  }
+
   static bool IsFinite(const DComplex& c) {
     return std::isfinite(c.real()) && std::isfinite(c.imag());
   }
@@ -351,6 +447,21 @@ class IterativeDiagonalOracle {
     const size_t size = SolSize();
     for (size_t cb = 0; cb != n_cb_; ++cb) {
       const DComplex* ns_cb = &next[cb * size];
+      if (np_ == 1) {
+        // one solution polarization (SolverBase.cc:244-253)
+        for (size_t i = 0; i != size; ++i) {
+          const DComplex s0 = solutions[cb][i];
+          if (s0 != 0.0) {
+            const double a = std::abs((ns_cb[i] - s0) / s0);
+            if (std::isfinite(a)) {
+              avg_abs_diff += a;
+              ++n;
+            }
+          }
+        }
+        std::copy_n(ns_cb, size, solutions[cb].data());
+        continue;
+      }
       for (size_t i = 0; i != size; i += 2) {
         const DComplex s[2] = {solutions[cb][i], solutions[cb][i + 1]};
         const DComplex s_inv[2] = {1.0 / s[0], 1.0 / s[1]};
@@ -427,7 +538,12 @@ class IterativeDiagonalOracle {
     std::vector<double> step_magnitudes;
     step_magnitudes.reserve(s_.max_iterations + 1);
     do {
-      for (std::vector<DComplex>& sv : solutions) MakeSolutionsFinite2Pol(sv);
+      for (std::vector<DComplex>& sv : solutions) {
+        if (np_ == 1)
+          MakeSolutionsFinite1Pol(sv);
+        else
+          MakeSolutionsFinite2Pol(sv);
+      }
       StaticFor(n_cb_, n_block_threads_, [&](size_t b, size_t e, size_t) {
         for (size_t cb = b; cb < e; ++cb)
           PerformIteration(cbs[cb], residual[cb], solutions[cb].data(),
@@ -437,7 +553,7 @@ class IterativeDiagonalOracle {
       // ApplyConstraints (SolverBase.cc:174-190): the smoothness constraint
       // is always satisfied; without constraints the same holds.
       if (smoothness)
-        orc_smooth(smoothness, int32_t(n_cb_), int32_t(n_ant_), int32_t(n_sub_), 2,
+        orc_smooth(smoothness, int32_t(n_cb_), int32_t(n_ant_), int32_t(n_sub_), int32_t(np_),
                    reinterpret_cast<double*>(next.data()));
       constraints_satisfied = true;
       has_converged = AssignSolutions(solutions, next, !constraints_satisfied,
@@ -476,6 +592,7 @@ class IterativeDiagonalOracle {
 
  private:
   orc_settings s_;
+  size_t np_;  // solution polarizations: 2 (diagonal) or 1 (scalar)
   size_t n_ant_, n_dir_, n_cb_, n_sub_;
   std::vector<size_t> n_sol_dir_;
   std::vector<size_t> sol_start_;
@@ -514,10 +631,11 @@ template <typename V>
 int SolveTyped(const orc_settings* s, const int64_t* vo, const float* data,
                const float* model, const uint32_t* a1, const uint32_t* a2,
                const uint32_t* sm, double* solutions, orc_result* result,
-               double* step_mags, const orc_smoothness* smoothness = nullptr) {
+               double* step_mags, const orc_smoothness* smoothness = nullptr,
+               bool scalar = false) {
   const std::vector<ChannelBlock<V>> cbs =
       MakeBlocks<V>(*s, vo, data, model, a1, a2, sm);
-  IterativeDiagonalOracle<V> solver(*s);
+  IterativeDiagonalOracle<V> solver(*s, scalar);
   solver.Solve(cbs, solutions, result, step_mags, smoothness);
   return 0;
 }
@@ -577,6 +695,25 @@ extern "C" int orc_solve_constrained(const orc_settings* s, const orc_smoothness
                                               result, step_mags, c)
                    : SolveTyped<Full<float>>(s, vo, data, model, a1, a2, sm, solutions, result,
                                              step_mags, c);
+  return -2;
+}
+
+extern "C" int orc_solve_scalar(const orc_settings* s, const orc_smoothness* c,
+                                const int64_t* vo, const float* data, const float* model,
+                                const uint32_t* a1, const uint32_t* a2, const uint32_t* sm,
+                                double* solutions, orc_result* result, double* step_mags) {
+  if (!s || s->n_directions <= 0 || s->n_channel_blocks <= 0 || s->n_antennas <= 0) return -1;
+  if (c && (!c->frequencies || !(c->bandwidth_hz > 0.0))) return -1;
+  if (s->n_correlations == 2)
+    return s->fp64 ? SolveTyped<Duo<double>>(s, vo, data, model, a1, a2, sm, solutions, result,
+                                             step_mags, c, true)
+                   : SolveTyped<Duo<float>>(s, vo, data, model, a1, a2, sm, solutions, result,
+                                            step_mags, c, true);
+  if (s->n_correlations == 4)
+    return s->fp64 ? SolveTyped<Full<double>>(s, vo, data, model, a1, a2, sm, solutions,
+                                              result, step_mags, c, true)
+                   : SolveTyped<Full<float>>(s, vo, data, model, a1, a2, sm, solutions, result,
+                                             step_mags, c, true);
   return -2;
 }
 

@@ -41,16 +41,36 @@ def small_problem(w=S.SMALL, n_corr=2, noise_sigma=None, seed=None):
     return w, geom, sky, gains, P
 
 
-def solvertester_problem(n_sol_per_dir=(1, 2, 5), n_corr=2):
-    """SolverTester::FillDdIntervalData in SolveData layout + input solutions."""
+def solvertester_problem(n_sol_per_dir=(1, 2, 5), n_corr=2, scalar=False):
+    """SolverTester::FillDdIntervalData in SolveData layout + input solutions
+    (SetScalarSolutions when `scalar`, else SetDiagonalSolutions)."""
     n_ant, n_dir, n_ch, n_t, n_cb = O.solvertester_shape()
-    data, model, inp = O.solvertester_fill(n_sol_per_dir)
+    data, model, inp = O.solvertester_fill(n_sol_per_dir, scalar=scalar)
     a1 = [a for a in range(n_ant) for b in range(a + 1, n_ant)]
     a2 = [b for a in range(n_ant) for b in range(a + 1, n_ant)]
     pk = O.pack_solve_data(data, model, a1, a2, n_ant, n_cb, n_sol_per_dir, n_corr=n_corr)
     P = S.problem_from_solvertester(data, model, n_sol_per_dir, n_ant, n_cb, pk)
     P.channel_begin = S.channel_begin(n_ch, n_cb)
     return P, inp
+
+
+def check_scalar_results(solutions, inp, n_ant, n_sub, n_cb, tolerance):
+    """SolverTester::CheckScalarResults (SolverTester.cc:382-407): |g|^2 within
+    `tolerance` percent, and the antenna-0-referenced product within
+    `tolerance` absolute in real and imaginary part. solutions [n_cb,
+    n_ant*n_sub] (one polarization), inp [n_ant*n_sub*2]."""
+    sol = np.asarray(solutions).reshape(n_cb, n_ant, n_sub)
+    x = np.asarray(inp).reshape(n_ant, n_sub, 2)[:, :, 0]
+    for cb in range(n_cb):
+        s0, i0 = sol[cb, 0], x[0]
+        for ant in range(n_ant):
+            s, i = sol[cb, ant], x[ant]
+            a, b = np.abs(s) ** 2, np.abs(i) ** 2
+            d = np.abs(a - b)
+            assert np.all(d <= tolerance / 100 * a) and np.all(d <= tolerance / 100 * b), (cb, ant)
+            rs, ri = s * np.conj(s0), i * np.conj(i0)
+            assert np.all(np.abs(rs.real - ri.real) < tolerance), (cb, ant)
+            assert np.all(np.abs(rs.imag - ri.imag) < tolerance), (cb, ant)
 
 
 def check_dual_results(solutions, inp, n_ant, n_sub, n_cb, tolerance_percent):
