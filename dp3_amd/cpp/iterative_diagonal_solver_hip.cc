@@ -16,17 +16,17 @@ namespace {
 
 }  // namespace
 
-IterativeDiagonalSolverHip::IterativeDiagonalSolverHip(int model_storage)
-    : model_storage_(model_storage) {
+IterativeSolverHip::IterativeSolverHip(int model_storage, int n_solution_polarizations)
+    : model_storage_(model_storage), n_solution_polarizations_(n_solution_polarizations) {
   if (model_storage != DDH_MODEL_C64 && model_storage != DDH_MODEL_F16)
     throw std::runtime_error("Unknown model storage for the HIP iterative solver");
 }
 
-IterativeDiagonalSolverHip::~IterativeDiagonalSolverHip() {
+IterativeSolverHip::~IterativeSolverHip() {
   if (solver_) ddh_solver_destroy(solver_);
 }
 
-void IterativeDiagonalSolverHip::Initialize(size_t n_antennas,
+void IterativeSolverHip::Initialize(size_t n_antennas,
                                             const std::vector<size_t>& n_solutions_per_direction,
                                             size_t n_channel_blocks) {
   SolverBase::Initialize(n_antennas, n_solutions_per_direction, n_channel_blocks);
@@ -35,7 +35,7 @@ void IterativeDiagonalSolverHip::Initialize(size_t n_antennas,
   key_.clear();
 }
 
-ddh_solver* IterativeDiagonalSolverHip::DeviceSolver(int n_correlations) {
+ddh_solver* IterativeSolverHip::DeviceSolver(int n_correlations) {
   if (NAntennas() == 0) throw std::runtime_error("Solve called before Initialize");
   double acc = GetAccuracy(), cacc = GetConstraintAccuracy(), step = GetStepSize(),
          sigma = GetStepDiffSigma();
@@ -67,6 +67,7 @@ ddh_solver* IterativeDiagonalSolverHip::DeviceSolver(int n_correlations) {
   c.step_diff_sigma = sigma;
   c.phase_only = GetPhaseOnly() ? 1 : 0;
   c.model_storage = model_storage_;
+  c.n_solution_polarizations = n_solution_polarizations_;
   solver_ = ddh_solver_create(&c);
   if (!solver_) ThrowLastError("Could not create the HIP solver");
   key_ = key;
@@ -74,7 +75,7 @@ ddh_solver* IterativeDiagonalSolverHip::DeviceSolver(int n_correlations) {
   return solver_;
 }
 
-void IterativeDiagonalSolverHip::CheckSmoothness(const Smoothness& m) const {
+void IterativeSolverHip::CheckSmoothness(const Smoothness& m) const {
   if (m.channel_block_frequencies.empty() || NAntennas() == 0) return;  // checked at Solve
   if (m.channel_block_frequencies.size() != NChannelBlocks())
     throw std::runtime_error("Smoothness constraint: one frequency per channel block is needed");
@@ -84,7 +85,7 @@ void IterativeDiagonalSolverHip::CheckSmoothness(const Smoothness& m) const {
     throw std::runtime_error("Invalid setting for the dd factors option of smoothness constraint");
 }
 
-void IterativeDiagonalSolverHip::ApplyConstraint(ddh_solver* solver) const {
+void IterativeSolverHip::ApplyConstraint(ddh_solver* solver) const {
   const Smoothness& m = smoothness_;
   if (m.channel_block_frequencies.empty()) {
     if (ddh_solver_set_smoothness(solver, nullptr, 0.0, 0.0, 0.0, 0, nullptr, nullptr) != DDH_OK)
@@ -107,13 +108,13 @@ void IterativeDiagonalSolverHip::ApplyConstraint(ddh_solver* solver) const {
   }
 }
 
-void IterativeDiagonalSolverHip::SetSmoothness(const Smoothness& smoothness) {
+void IterativeSolverHip::SetSmoothness(const Smoothness& smoothness) {
   CheckSmoothness(smoothness);
   smoothness_ = smoothness;
   if (solver_) ApplyConstraint(solver_);
 }
 
-void IterativeDiagonalSolverHip::SetConstraintWeights(const std::vector<double>& weights) {
+void IterativeSolverHip::SetConstraintWeights(const std::vector<double>& weights) {
   if (weights.size() != NAntennas() * NChannelBlocks())
     throw std::runtime_error("Constraint weights: n_antennas x n_channel_blocks values needed");
   constraint_weights_ = weights;
@@ -123,7 +124,7 @@ void IterativeDiagonalSolverHip::SetConstraintWeights(const std::vector<double>&
       ThrowLastError("Could not set the constraint weights");
 }
 
-void IterativeDiagonalSolverHip::SetDdConstraintWeights(
+void IterativeSolverHip::SetDdConstraintWeights(
     const std::vector<std::vector<double>>& weights) {
   if (weights.size() != NSubSolutions())
     throw std::runtime_error("Dd constraint weights: one vector per sub-solution needed");
@@ -139,7 +140,7 @@ void IterativeDiagonalSolverHip::SetDdConstraintWeights(
 }
 
 template <typename SolveDataType>
-SolverBase::SolveResult IterativeDiagonalSolverHip::SolveImpl(
+SolverBase::SolveResult IterativeSolverHip::SolveImpl(
     const SolveDataType& data, std::vector<std::vector<DComplex>>& solutions,
     int n_correlations) {
   if (data.NChannelBlocks() != NChannelBlocks() || solutions.size() != NChannelBlocks())
@@ -173,12 +174,12 @@ SolverBase::SolveResult IterativeDiagonalSolverHip::SolveImpl(
   return out;
 }
 
-SolverBase::SolveResult IterativeDiagonalSolverHip::Solve(
+SolverBase::SolveResult IterativeSolverHip::Solve(
     const DuoSolveData& data, std::vector<std::vector<DComplex>>& solutions, double) {
   return SolveImpl(data, solutions, 2);
 }
 
-SolverBase::SolveResult IterativeDiagonalSolverHip::Solve(
+SolverBase::SolveResult IterativeSolverHip::Solve(
     const FullSolveData& data, std::vector<std::vector<DComplex>>& solutions, double) {
   return SolveImpl(data, solutions, 4);
 }

@@ -1,8 +1,9 @@
-// IterativeDiagonalSolverHip: DP3's IterativeDiagonalSolver
-// (ddecal/gain_solvers/IterativeDiagonalSolver.h:13-41) behind the same
+// IterativeDiagonalSolverHip and IterativeScalarSolverHip: DP3's
+// IterativeDiagonalSolver (ddecal/gain_solvers/IterativeDiagonalSolver.h:
+// 13-41) and IterativeScalarSolver (IterativeScalarSolver.h) behind the same
 // SolverBase interface, solving on an MI355X through libddecal_hip's C-ABI
-// (include/ddecal_hip.h). Duo (MC2x2FDiag) and Full (MC2x2F) data, two
-// solution polarizations, direction-dependent solution intervals.
+// (include/ddecal_hip.h). Duo (MC2x2FDiag) and Full (MC2x2F) data, two or
+// one solution polarizations, direction-dependent solution intervals.
 //
 // Behaviour as the reference: the iteration count is max+1 when the solve
 // does not converge (SolverBase.cc:197-200), NaN solutions are data, settings
@@ -24,15 +25,13 @@ struct ddh_solver;
 
 namespace dp3amd {
 
-class IterativeDiagonalSolverHip final : public SolverBase {
+// The shared implementation; n_solution_polarizations is 2 (diagonal) or 1
+// (scalar).
+class IterativeSolverHip : public SolverBase {
  public:
-  // model_storage: 0 keeps complex64 model visibilities on the device (the
-  // reference precision), 1 stores them as FP16 with a per (block,
-  // direction) scale.
-  explicit IterativeDiagonalSolverHip(int model_storage = 0);
-  ~IterativeDiagonalSolverHip() override;
+  ~IterativeSolverHip() override;
 
-  size_t NSolutionPolarizations() const override { return 2; }
+  size_t NSolutionPolarizations() const override { return n_solution_polarizations_; }
   bool SupportsDdSolutionIntervals() const override { return true; }
 
   void Initialize(size_t n_antennas, const std::vector<size_t>& n_solutions_per_direction,
@@ -68,6 +67,12 @@ class IterativeDiagonalSolverHip final : public SolverBase {
   // sub-solution [n_antennas][n_channel_blocks] weights.
   void SetDdConstraintWeights(const std::vector<std::vector<double>>& weights);
 
+ protected:
+  // model_storage: 0 keeps complex64 model visibilities on the device (the
+  // reference precision), 1 stores them as FP16 with a per (block,
+  // direction) scale.
+  IterativeSolverHip(int model_storage, int n_solution_polarizations);
+
  private:
   template <typename SolveDataType>
   SolveResult SolveImpl(const SolveDataType& data, std::vector<std::vector<DComplex>>& solutions,
@@ -77,12 +82,26 @@ class IterativeDiagonalSolverHip final : public SolverBase {
   void CheckSmoothness(const Smoothness& smoothness) const;
 
   int model_storage_;
+  int n_solution_polarizations_;
   ddh_solver* solver_ = nullptr;
   std::vector<int64_t> key_;  // settings the cached device solver was made with
   bool last_converged_ = false;
   Smoothness smoothness_;
   std::vector<double> constraint_weights_;       // [n_ant][n_cb], empty: ones
   std::vector<double> sub_solution_weights_;     // [n_sub][n_ant][n_cb], empty: none
+};
+
+// IterativeDiagonalSolver: two solution polarizations.
+class IterativeDiagonalSolverHip final : public IterativeSolverHip {
+ public:
+  explicit IterativeDiagonalSolverHip(int model_storage = 0) : IterativeSolverHip(model_storage, 2) {}
+};
+
+// IterativeScalarSolver: one gain per antenna and solution for both
+// correlations (NSolutionPolarizations() == 1).
+class IterativeScalarSolverHip final : public IterativeSolverHip {
+ public:
+  explicit IterativeScalarSolverHip(int model_storage = 0) : IterativeSolverHip(model_storage, 1) {}
 };
 
 }  // namespace dp3amd

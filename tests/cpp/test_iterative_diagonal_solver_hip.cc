@@ -22,6 +22,7 @@
 using dp3amd::DuoSolveData;
 using dp3amd::FullSolveData;
 using dp3amd::IterativeDiagonalSolverHip;
+using dp3amd::IterativeScalarSolverHip;
 using dp3amd::SolverBase;
 using DComplex = std::complex<double>;
 
@@ -57,7 +58,10 @@ struct Fixture {
   std::vector<uint32_t> ant1, ant2, sol_map;
   std::vector<double> input;  // [n_ant][n_sub][2]
 
-  Fixture(std::vector<int32_t> n_solutions, int n_corr) : n_sol(std::move(n_solutions)) {
+  int n_pol = 2;  // solution polarizations of the solver under test
+
+  Fixture(std::vector<int32_t> n_solutions, int n_corr, bool scalar = false)
+      : n_sol(std::move(n_solutions)), n_pol(scalar ? 1 : 2) {
     orc_solvertester_shape(&n_ant, &n_dir, &n_ch, &n_t, &n_cb);
     const int n_bl = n_ant * (n_ant - 1) / 2;
     int n_sub = 0;
@@ -65,7 +69,11 @@ struct Fixture {
     std::vector<float> raw(size_t(n_t) * n_bl * n_ch * 8);
     std::vector<float> raw_model(raw.size() * n_dir);
     input.resize(size_t(n_ant) * n_sub * 4);
-    orc_solvertester_fill(n_sol.data(), raw.data(), raw_model.data(), input.data());
+    // SetScalarSolutions or SetDiagonalSolutions (SolverTester.cc:331-363)
+    if (scalar)
+      orc_solvertester_fill_scalar(n_sol.data(), raw.data(), raw_model.data(), input.data());
+    else
+      orc_solvertester_fill(n_sol.data(), raw.data(), raw_model.data(), input.data());
     std::vector<int32_t> a1, a2;
     for (int a = 0; a < n_ant; ++a)
       for (int b = a + 1; b < n_ant; ++b) {
@@ -118,8 +126,8 @@ struct Fixture {
   }
 
   std::vector<std::vector<DComplex>> UnitSolutions() const {
-    return std::vector<std::vector<DComplex>>(n_cb,
-                                              std::vector<DComplex>(size_t(n_ant) * NSub() * 2, 1.0));
+    return std::vector<std::vector<DComplex>>(
+        n_cb, std::vector<DComplex>(size_t(n_ant) * NSub() * n_pol, 1.0));
   }
 
   void InitializeSolver(SolverBase& solver) const {
@@ -148,6 +156,21 @@ struct Fixture {
         }
   }
 
+  // SolverTester::CheckScalarResults (SolverTester.cc:382-407).
+  void CheckScalarResults(const std::vector<std::vector<DComplex>>& sol, double tolerance) const {
+    const int n_sub = NSub();
+    const DComplex* in = reinterpret_cast<const DComplex*>(input.data());
+    for (int ch = 0; ch < n_cb; ++ch)
+      for (int ant = 0; ant < n_ant; ++ant)
+        for (int s = 0; s < n_sub; ++s) {
+          const DComplex sol0 = sol[ch][s], inp0 = in[s * 2];
+          const DComplex g = sol[ch][s + ant * n_sub], i = in[(s + ant * n_sub) * 2];
+          CHECK(Close(std::norm(g), std::norm(i), tolerance));
+          CHECK(std::abs((g * std::conj(sol0)).real() - (i * std::conj(inp0)).real()) < tolerance);
+          CHECK(std::abs((g * std::conj(sol0)).imag() - (i * std::conj(inp0)).imag()) < tolerance);
+        }
+  }
+
   // The CPU restatement of the reference solver on the same data.
   std::vector<std::vector<DComplex>> OracleSolve(int n_corr, orc_result* result,
                                                  const orc_smoothness* smoothness = nullptr) const {
@@ -164,9 +187,12 @@ struct Fixture {
     s.detect_stalling = 1;
     s.step_diff_sigma = 0.1;
     s.n_threads = 4;
-    std::vector<double> flat(size_t(n_cb) * n_ant * NSub() * 4);
+    std::vector<double> flat(size_t(n_cb) * n_ant * NSub() * n_pol * 2);
     for (size_t i = 0; i < flat.size(); i += 2) flat[i] = 1.0;
-    if (smoothness)
+    if (n_pol == 1)
+      orc_solve_scalar(&s, smoothness, offsets.data(), data.data(), model.data(), ant1.data(),
+                       ant2.data(), sol_map.data(), flat.data(), result, nullptr);
+    else if (smoothness)
       orc_solve_constrained(&s, smoothness, offsets.data(), data.data(), model.data(),
                             ant1.data(), ant2.data(), sol_map.data(), flat.data(), result,
                             nullptr);
@@ -273,6 +299,36 @@ void SmoothnessConstraint() {
   CHECK(thrown);
 }
 
+// tSolvers.cc:155-200 (iterative_scalar_dd_intervals,
+// iterative_duo_scalar_dd_intervals): SetScalarSolutions, CheckScalarResults
+// at 1e-3; and parity with the oracle's scalar solve.
+template <typename SolveDataType>
+void ScalarDdIntervals(int n_corr) {
+  Fixture f({1, 2, 5}, n_corr, true);
+  IterativeScalarSolverHip solver;
+  f.InitializeSolver(solver);
+  CHECK(solver.NSolutionPolarizations() == 1u);
+  const SolveDataType data = f.Make<SolveDataType>(n_corr);
+  auto solutions = f.UnitSolutions();
+  const SolverBase::SolveResult result = solver.Solve(data, solutions, 0.0);
+  f.CheckScalarResults(solutions, 1.0e-3);
+  orc_result oracle_result{};
+  const auto reference = f.OracleSolve(n_corr, &oracle_result);
+  double worst = 0.0;
+  for (size_t cb = 0; cb < solutions.size(); ++cb)
+    for (size_t i = 0; i < solutions[cb].size(); ++i) {
+      const DComplex r0 = reference[cb][i % f.NSub()], s0 = solutions[cb][i % f.NSub()];
+      const DComplex ra = solutions[cb][i] * std::conj(s0) / std::abs(s0);
+      const DComplex rb = reference[cb][i] * std::conj(r0) / std::abs(r0);
+      worst = std::max(worst, std::abs(ra - rb) / std::abs(rb));
+    }
+  std::printf("scalar dd intervals n_corr=%d: iterations %zu (oracle %lld), gain error vs oracle "
+              "%.2e\n",
+              n_corr, result.iterations, static_cast<long long>(oracle_result.iterations), worst);
+  CHECK(worst <= 1e-4);
+  CHECK(int64_t(result.iterations) <= oracle_result.iterations);
+}
+
 void IterationSemantics() {
   Fixture f({1, 1, 1}, 2);
   const DuoSolveData data = f.Make<DuoSolveData>(2);
@@ -342,6 +398,8 @@ int main() {
   const std::vector<std::pair<const char*, std::function<void()>>> cases = {
       {"iterative_duo_diagonal_dd_intervals", [] { DdIntervals<DuoSolveData>(2); }},
       {"iterative_diagonal_dd_intervals", [] { DdIntervals<FullSolveData>(4); }},
+      {"iterative_duo_scalar_dd_intervals", [] { ScalarDdIntervals<DuoSolveData>(2); }},
+      {"iterative_scalar_dd_intervals", [] { ScalarDdIntervals<FullSolveData>(4); }},
       {"iteration_semantics", IterationSemantics},
       {"smoothness_constraint", SmoothnessConstraint},
       {"errors", Errors},
