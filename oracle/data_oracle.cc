@@ -170,11 +170,12 @@ extern "C" void orc_solvertester_shape(int32_t* n_ant, int32_t* n_dir,
 
 namespace {
 
-// FillDdIntervalData(leakage_only = false) (SolverTester.cc:95-202) for the
-// input solutions `input` ([ant][sub][2]). Weights are 1 and nothing is
-// flagged, so the AssignAndWeight pass at :199 is the identity.
+// FillDdIntervalData (SolverTester.cc:95-202) for the input solutions
+// `input` ([ant][sub][2]): diagonal gains, or with `leakage` the leakage
+// terms of J = [[1, l_0], [l_1, 1]]. Weights are 1 and nothing is flagged,
+// so the AssignAndWeight pass at :199 is the identity.
 void FillDdIntervalData(const int32_t* n_sol_dir, const std::vector<CF>& input,
-                        size_t n_solutions, float* data, float* model) {
+                        size_t n_solutions, float* data, float* model, bool leakage = false) {
   std::uniform_real_distribution<float> uniform_data(-1.0, 1.0);
   std::mt19937 mt(0);
   std::vector<std::vector<size_t>> solution_indices(kNDirections);
@@ -217,6 +218,20 @@ void FillDdIntervalData(const int32_t* n_sol_dir, const std::vector<CF>& input,
                               CD(input[(a1 * n_solutions + si) * 2 + 1])};
             const CD gb[2] = {CD(input[(a2 * n_solutions + si) * 2 + 0]),
                               CD(input[(a2 * n_solutions + si) * 2 + 1])};
+            if (leakage) {
+              // left * val.MultiplyHerm(right) with left = [[1, ga0], [ga1, 1]],
+              // right = [[1, gb0], [gb1, 1]]: val * right^H, then left * that
+              const CD left[4] = {1.0, ga[0], ga[1], 1.0};
+              const CD right_h[4] = {1.0, std::conj(gb[1]), std::conj(gb[0]), 1.0};
+              CD vr[4];
+              for (int i = 0; i != 2; ++i)
+                for (int j = 0; j != 2; ++j)
+                  vr[2 * i + j] = CD(val[2 * i]) * right_h[j] + CD(val[2 * i + 1]) * right_h[2 + j];
+              for (int i = 0; i != 2; ++i)
+                for (int j = 0; j != 2; ++j)
+                  sum[2 * i + j] += left[2 * i] * vr[j] + left[2 * i + 1] * vr[2 + j];
+              continue;
+            }
             // left * val.MultiplyHerm(right) = Diag(ga) * val * Diag(gb)^H
             for (int i = 0; i != 2; ++i)
               for (int j = 0; j != 2; ++j)
@@ -284,4 +299,25 @@ extern "C" void orc_solvertester_fill_scalar(const int32_t* n_sol_dir, float* da
   }
   CopyOut(input, input_solutions);
   FillDdIntervalData(n_sol_dir, input, n_solutions, data, model);
+}
+
+// SetLeakageSolutions (SolverTester.cc:365-379), then FillDdIntervalData with
+// leakage_only.
+extern "C" void orc_solvertester_fill_leakage(const int32_t* n_sol_dir, float* data,
+                                              float* model, double* input_solutions) {
+  const size_t n_solutions = CountSolutions(n_sol_dir);
+  std::vector<CF> input(kNAntennas * n_solutions * 2);
+  std::mt19937 mt;
+  std::uniform_real_distribution<float> uniform_sols(0.0, 0.5);
+  for (size_t ant = 0; ant != kNAntennas; ++ant) {
+    for (size_t pol = 0; pol != 2; ++pol) {
+      for (size_t s = 0; s != n_solutions; ++s) {
+        const size_t index = (ant * n_solutions + s) * 2 + pol;
+        input[index] = {uniform_sols(mt), uniform_sols(mt)};
+        if (s != 0) input[index] *= 0.5f;
+      }
+    }
+  }
+  CopyOut(input, input_solutions);
+  FillDdIntervalData(n_sol_dir, input, n_solutions, data, model, true);
 }

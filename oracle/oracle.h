@@ -98,6 +98,14 @@ int orc_solve_scalar(const orc_settings* settings, const orc_smoothness* smoothn
                      const int64_t* vis_offsets, const float* data, const float* model,
                      const uint32_t* ant1, const uint32_t* ant2, const uint32_t* sol_map,
                      double* solutions, orc_result* result, double* step_magnitudes);
+// IterativeFullJonesSolver (IterativeFullJonesSolver.cc:18-224): 2x2
+// solutions [n_cb][n_ant][n_sub][4] complex128 (row-major 00, 01, 10, 11),
+// Full data only (n_correlations 4, else -2); with a smoothness constraint
+// when `smoothness` is not NULL.
+int orc_solve_full_jones(const orc_settings* settings, const orc_smoothness* smoothness,
+                         const int64_t* vis_offsets, const float* data, const float* model,
+                         const uint32_t* ant1, const uint32_t* ant2, const uint32_t* sol_map,
+                         double* solutions, orc_result* result, double* step_magnitudes);
 // orc_solve with a smoothness constraint applied to the next solutions after
 // every step (SolverBase::ApplyConstraints, SolverBase.cc:166-190; the
 // constraint is always satisfied).
@@ -119,6 +127,12 @@ double orc_chi2(int32_t n_ant, int32_t n_dir, int32_t n_cb, int32_t n_corr,
                 int32_t n_sub, const int64_t* vis_offsets, const float* data,
                 const float* model, const uint32_t* ant1, const uint32_t* ant2,
                 const uint32_t* sol_map, const double* solutions);
+// The same for full-Jones solutions [n_cb][n_ant][n_sub][4] (residual
+// D - sum_d J_a M_d J_b^H); n_corr must be 4 (NaN otherwise).
+double orc_chi2_full_jones(int32_t n_ant, int32_t n_dir, int32_t n_cb, int32_t n_corr,
+                           int32_t n_sub, const int64_t* vis_offsets, const float* data,
+                           const float* model, const uint32_t* ant1, const uint32_t* ant2,
+                           const uint32_t* sol_map, const double* solutions);
 
 // Restatement of SolveData's regular constructor (SolveData.cc:77-188).
 // Inputs per time step t: data [n_t][n_bl][n_ch][4] complex64 and
@@ -189,6 +203,12 @@ void orc_solvertester_fill(const int32_t* n_solutions_per_direction,
 // with equal polarizations).
 void orc_solvertester_fill_scalar(const int32_t* n_solutions_per_direction, float* data,
                                   float* model, double* input_solutions);
+// SolverTester::SetLeakageSolutions (SolverTester.cc:365-379: off-diagonal
+// leakage terms per (antenna, solution)), then FillDdIntervalData with
+// leakage_only (:159-172: J = [[1, l_0], [l_1, 1]] on both sides);
+// input_solutions [ant][sub][2] holds (l_0, l_1).
+void orc_solvertester_fill_leakage(const int32_t* n_solutions_per_direction, float* data,
+                                   float* model, double* input_solutions);
 
 // Restatement of DDECal::CorrectAndSubtractModels (steps/DDECal.cc:992-1124)
 // over one solution interval: data [n_t][n_bl][n_ch][4] and model

@@ -141,6 +141,11 @@ def lib():
         L.orc_solve_scalar.argtypes = [C.POINTER(_Settings), C.c_void_p, P, P, P, P, P, P, P,
                                        C.POINTER(_Result), P]
         L.orc_solve_scalar.restype = C.c_int
+        L.orc_solve_full_jones.argtypes = L.orc_solve_scalar.argtypes
+        L.orc_solve_full_jones.restype = C.c_int
+        L.orc_chi2_full_jones.argtypes = L.orc_chi2.argtypes
+        L.orc_chi2_full_jones.restype = C.c_double
+        L.orc_solvertester_fill_leakage.argtypes = [P] * 4
         L.orc_correct_and_subtract.argtypes = [C.c_int32] * 7 + [P] * 6 + [C.c_int32] * 2
         L.orc_correct_and_subtract.restype = None
         _LIB = L
@@ -174,10 +179,10 @@ def smooth(solutions, smoothness: Smoothness):
 def solve(problem, solutions, *, max_iterations=50, min_iterations=0, accuracy=1e-5,
           constraint_accuracy=1e-4, step_size=0.2, detect_stalling=True,
           step_diff_sigma=0.1, phase_only=False, fp64=False, n_threads=1,
-          smoothness: Smoothness | None = None, scalar=False):
+          smoothness: Smoothness | None = None, scalar=False, full_jones=False):
     """Runs the oracle solver in place on `solutions` ([n_cb, n_ant*n_sub*2] complex128,
-    or [n_cb, n_ant*n_sub] for the scalar solver), with the smoothness constraint when
-    given.
+    [n_cb, n_ant*n_sub] for the scalar solver, [n_cb, n_ant*n_sub*4] (row-major 2x2) for
+    the full-Jones solver), with the smoothness constraint when given.
 
     `problem` is a dp3_amd.synth.SolveProblem (reference SolveData layout)."""
     n_sol = np.ascontiguousarray(problem.n_solutions_per_direction, dtype=np.int32)
@@ -189,9 +194,10 @@ def solve(problem, solutions, *, max_iterations=50, min_iterations=0, accuracy=1
     assert solutions.dtype == np.complex128 and solutions.flags.c_contiguous
     res = _Result()
     mags = np.zeros(max_iterations + 2)
-    if scalar:
+    if scalar or full_jones:
         st, keep = smoothness.struct() if smoothness is not None else (None, None)
-        rc = lib().orc_solve_scalar(C.byref(s), None if st is None else C.addressof(st),
+        fn = lib().orc_solve_full_jones if full_jones else lib().orc_solve_scalar
+        rc = fn(C.byref(s), None if st is None else C.addressof(st),
                                     _p(problem.vis_offsets), _p(problem.data),
                                     _p(problem.model), _p(problem.ant1), _p(problem.ant2),
                                     _p(problem.sol_map), _p(solutions), C.byref(res), _p(mags))
@@ -228,9 +234,10 @@ def iterate_once(problem, solutions, *, fp64=False, n_threads=1):
     return nxt
 
 
-def chi2(problem, solutions):
+def chi2(problem, solutions, full_jones=False):
     sol = np.ascontiguousarray(solutions, dtype=np.complex128)
-    return lib().orc_chi2(problem.n_antennas, problem.n_directions,
+    fn = lib().orc_chi2_full_jones if full_jones else lib().orc_chi2
+    return fn(problem.n_antennas, problem.n_directions,
                           problem.n_channel_blocks, problem.n_correlations,
                           int(np.sum(problem.n_solutions_per_direction)),
                           _p(problem.vis_offsets), _p(problem.data), _p(problem.model),
@@ -332,9 +339,10 @@ def solvertester_shape():
     return tuple(int(x.value) for x in v)  # n_ant, n_dir, n_ch, n_t, n_cb
 
 
-def solvertester_fill(n_sol_per_dir, scalar=False):
-    """SolverTester data for SetDiagonalSolutions, or SetScalarSolutions when
-    `scalar` (input solutions with equal polarizations)."""
+def solvertester_fill(n_sol_per_dir, scalar=False, leakage=False):
+    """SolverTester data for SetDiagonalSolutions, SetScalarSolutions when
+    `scalar` (input solutions with equal polarizations), or SetLeakageSolutions
+    with leakage-only data when `leakage` (inputs are the off-diagonal terms)."""
     n_ant, n_dir, n_ch, n_t, n_cb = solvertester_shape()
     n_bl = n_ant * (n_ant - 1) // 2
     ns = np.ascontiguousarray(n_sol_per_dir, dtype=np.int32)
@@ -342,7 +350,8 @@ def solvertester_fill(n_sol_per_dir, scalar=False):
     data = np.zeros((n_t, n_bl, n_ch, 4), np.complex64)
     model = np.zeros((n_dir, n_t, n_bl, n_ch, 4), np.complex64)
     inp = np.zeros(n_ant * n_sub * 2, np.complex128)
-    fill = lib().orc_solvertester_fill_scalar if scalar else lib().orc_solvertester_fill
+    fill = (lib().orc_solvertester_fill_leakage if leakage else
+            lib().orc_solvertester_fill_scalar if scalar else lib().orc_solvertester_fill)
     fill(_p(ns), _p(data), _p(model), _p(inp))
     return data, model, inp
 

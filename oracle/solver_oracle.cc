@@ -5,6 +5,10 @@
 //   ddecal/gain_solvers/IterativeDiagonalSolver.cc:20-222
 //   ddecal/gain_solvers/IterativeScalarSolver.cc:30-226 (scalar mode: one
 //       solution polarization, Trace / Norm sums in double accumulators)
+//   ddecal/gain_solvers/IterativeFullJonesSolver.cc:18-224 (full-Jones mode:
+//       2x2 solutions, float numerator and denominator matrices, float
+//       inverse; SolverBase.cc:269-283 and :373-405 for its convergence term
+//       and MakeSolutionsFinite4Pol)
 //   ddecal/gain_solvers/SolveData.h:204-244 (DiagonalAddOrSubtractDirection)
 //   ddecal/gain_solvers/SolverBase.cc:63-371, SolverBase.h:274-285
 //
@@ -66,6 +70,45 @@ inline Full<T> Contribution(const std::complex<T> a[2], const Full<T>& m,
     for (int j = 0; j != 2; ++j)
       r.v[2 * i + j] = (a[i] * m.v[2 * i + j]) * std::conj(b[j]);
   return r;
+}
+
+// Full 2x2 matrices, row-major (00, 01, 10, 11) as the visibility layout.
+// Products follow the textbook definitions; the inverse is the adjugate over
+// the determinant and fails when the determinant is exactly zero (the
+// aocommon Matrix2x2::Invert contract the reference relies on).
+template <typename T>
+struct Mat2 {
+  std::complex<T> m[4];
+};
+template <typename T>
+inline Mat2<T> Mul(const Mat2<T>& a, const Mat2<T>& b) {
+  return {{a.m[0] * b.m[0] + a.m[1] * b.m[2], a.m[0] * b.m[1] + a.m[1] * b.m[3],
+           a.m[2] * b.m[0] + a.m[3] * b.m[2], a.m[2] * b.m[1] + a.m[3] * b.m[3]}};
+}
+template <typename T>
+inline Mat2<T> HermTranspose(const Mat2<T>& a) {
+  return {{std::conj(a.m[0]), std::conj(a.m[2]), std::conj(a.m[1]), std::conj(a.m[3])}};
+}
+// a * b^H (MC2x2::MultiplyHerm)
+template <typename T>
+inline Mat2<T> MultiplyHerm(const Mat2<T>& a, const Mat2<T>& b) {
+  return Mul(a, HermTranspose(b));
+}
+template <typename T>
+inline bool Invert(Mat2<T>& a) {
+  const std::complex<T> det = a.m[0] * a.m[3] - a.m[1] * a.m[2];
+  if (det == std::complex<T>(0)) return false;
+  const std::complex<T> r = std::complex<T>(1) / det;
+  const std::complex<T> t = a.m[3] * r;
+  a.m[1] = -a.m[1] * r;
+  a.m[2] = -a.m[2] * r;
+  a.m[3] = a.m[0] * r;
+  a.m[0] = t;
+  return true;
+}
+template <typename T>
+inline void AddTo(Mat2<T>& x, const Mat2<T>& c) {
+  for (int k = 0; k != 4; ++k) x.m[k] += c.m[k];
 }
 
 template <typename V>
@@ -215,9 +258,9 @@ class IterativeDiagonalOracle {
   using T = decltype(V().v[0].real());
   using C = std::complex<T>;
 
-  IterativeDiagonalOracle(const orc_settings& s, bool scalar = false)
+  IterativeDiagonalOracle(const orc_settings& s, int n_pol = 2)
       : s_(s),
-        np_(scalar ? 1 : 2),
+        np_(n_pol),
         n_ant_(s.n_antennas),
         n_dir_(s.n_directions),
         n_cb_(s.n_channel_blocks),
@@ -262,6 +305,21 @@ class IterativeDiagonalOracle {
         const uint32_t si = cb.sol_map[d * cb.n_vis + v];
         const DComplex* s1 = &solutions[(a1 * n_sub_ + si) * np_];
         const DComplex* s2 = &solutions[(a2 * n_sub_ + si) * np_];
+        if constexpr (V::kCorr == 4) {
+          if (np_ == 4) {
+            // solution_1 * model.MultiplyHerm(solution_2)
+            // (IterativeFullJonesSolver.cc:196-212), in float
+            const Mat2<T> term = Mul(ToMat(s1), MultiplyHerm(ToMat(cb.model[d * cb.n_vis + v]),
+                                                             ToMat(s2)));
+            for (int k = 0; k != 4; ++k) {
+              if (Add)
+                residual[v].v[k] += term.m[k];
+              else
+                residual[v].v[k] -= term.m[k];
+            }
+            continue;
+          }
+        }
         // scalar: model * g_a * conj(g_b) per element (IterativeScalarSolver.cc:
         // 209-222), the same products as Diag(g_a) M Diag(g_b)^H with equal pols
         const C ga[2] = {static_cast<C>(s1[0]), static_cast<C>(s1[np_ - 1])};
@@ -273,6 +331,69 @@ class IterativeDiagonalOracle {
           SubFrom(residual[v], contribution);
       }
     });
+  }
+
+  static Mat2<T> ToMat(const DComplex* s) {
+    return {{C(s[0]), C(s[1]), C(s[2]), C(s[3])}};
+  }
+  static Mat2<T> ToMat(const V& x) {
+    Mat2<T> r;
+    for (int k = 0; k != 4; ++k) r.m[k] = x.v[k];
+    return r;
+  }
+
+  // IterativeFullJonesSolver.cc:99-188: per antenna and solution the float
+  // sums numerator = sum D J_b M^H (or D^H J_a M) and denominator =
+  // sum (J_b M^H)^H (J_b M^H) (or (J_a M)^H (J_a M)); the solution is
+  // numerator * denominator^-1, NaN when the denominator is singular.
+  void SolveDirectionFullJones(const ChannelBlock<V>& cb, const std::vector<V>& residual,
+                               size_t d, const DComplex* solutions, DComplex* next) const {
+    const size_t n_dir_sol = n_sol_dir_[d];
+    const uint32_t sol0 = cb.n_vis ? cb.sol_map[d * cb.n_vis] : sol_start_[d];
+    const size_t n_parts = std::max<size_t>(1, std::min(n_sub_threads_, cb.n_vis));
+    const Mat2<T> zero{};
+    std::vector<std::vector<Mat2<T>>> num(n_parts,
+                                          std::vector<Mat2<T>>(n_ant_ * n_dir_sol, zero));
+    std::vector<std::vector<Mat2<T>>> den(n_parts,
+                                          std::vector<Mat2<T>>(n_ant_ * n_dir_sol, zero));
+    StaticFor(cb.n_vis, n_parts, [&](size_t b, size_t e, size_t part) {
+      Mat2<T>* lnum = num[part].data();
+      Mat2<T>* lden = den[part].data();
+      for (size_t v = b; v != e; ++v) {
+        const uint32_t a1 = cb.ant1[v];
+        const uint32_t a2 = cb.ant2[v];
+        const uint32_t si = cb.sol_map[d * cb.n_vis + v];
+        const Mat2<T> j1 = ToMat(&solutions[(a1 * n_sub_ + si) * 4]);
+        const Mat2<T> j2 = ToMat(&solutions[(a2 * n_sub_ + si) * 4]);
+        const Mat2<T> data = ToMat(residual[v]);
+        const Mat2<T> model = ToMat(cb.model[d * cb.n_vis + v]);
+        const uint32_t rel = si - sol0;
+        const Mat2<T> cor_model_herm_1 = MultiplyHerm(j2, model);
+        AddTo(lnum[a1 * n_dir_sol + rel], Mul(data, cor_model_herm_1));
+        AddTo(lden[a1 * n_dir_sol + rel], Mul(HermTranspose(cor_model_herm_1), cor_model_herm_1));
+        const Mat2<T> cor_model_2 = Mul(j1, model);
+        AddTo(lnum[a2 * n_dir_sol + rel], Mul(HermTranspose(data), cor_model_2));
+        AddTo(lden[a2 * n_dir_sol + rel], Mul(HermTranspose(cor_model_2), cor_model_2));
+      }
+    });
+    for (size_t part = 1; part < n_parts; ++part)
+      for (size_t i = 0; i != num[0].size(); ++i) {
+        AddTo(num[0][i], num[part][i]);
+        AddTo(den[0][i], den[part][i]);
+      }
+    const T nan = std::numeric_limits<T>::quiet_NaN();
+    for (size_t ant = 0; ant != n_ant_; ++ant)
+      for (size_t rel = 0; rel != n_dir_sol; ++rel) {
+        const size_t index = ant * n_dir_sol + rel;
+        Mat2<T> inv = den[0][index];
+        Mat2<T> result;
+        if (Invert(inv))
+          result = Mul(num[0][index], inv);
+        else
+          for (int k = 0; k != 4; ++k) result.m[k] = std::complex<T>(nan, nan);
+        DComplex* out = &next[(ant * n_sub_ + rel + sol0) * 4];
+        for (int k = 0; k != 4; ++k) out[k] = DComplex(result.m[k]);
+      }
   }
 
   // IterativeScalarSolver.cc:117-194
@@ -320,6 +441,12 @@ class IterativeDiagonalOracle {
     if (np_ == 1) {
       SolveDirectionScalar(cb, residual, d, solutions, next);
       return;
+    }
+    if constexpr (V::kCorr == 4) {
+      if (np_ == 4) {
+        SolveDirectionFullJones(cb, residual, d, solutions, next);
+        return;
+      }
     }
     const size_t n_dir_sol = n_sol_dir_[d];
     const uint32_t sol0 = cb.n_vis ? cb.sol_map[d * cb.n_vis] : sol_start_[d];
@@ -408,6 +535,32 @@ class IterativeDiagonalOracle {
       if (!IsFinite(v)) v = average;
   }
 
+  // SolverBase.cc:373-405. As in the reference, the average of every
+  // polarization accumulates |entry 0| (:384), so a non-finite matrix becomes
+  // the mean |J_00| in all four entries; with no finite matrix, the identity.
+  static void MakeSolutionsFinite4Pol(std::vector<DComplex>& sv) {
+    size_t count = 0;
+    double average[4] = {0.0, 0.0, 0.0, 0.0};
+    for (size_t i = 0; i < sv.size(); i += 4) {
+      if (IsFinite(sv[i]) && IsFinite(sv[i + 1]) && IsFinite(sv[i + 2]) && IsFinite(sv[i + 3])) {
+        for (size_t p = 0; p != 4; ++p) average[p] += std::abs(sv[i]);
+        ++count;
+      }
+    }
+    if (count == 0) {
+      average[0] = 1.0;
+      average[1] = 0.0;
+      average[2] = 0.0;
+      average[3] = 1.0;
+    } else {
+      for (size_t p = 0; p != 4; ++p) average[p] /= count;
+    }
+    for (size_t i = 0; i < sv.size(); i += 4) {
+      if (!(IsFinite(sv[i]) && IsFinite(sv[i + 1]) && IsFinite(sv[i + 2]) && IsFinite(sv[i + 3])))
+        for (size_t p = 0; p != 4; ++p) sv[i + p] = average[p];
+    }
+  }
+
   static bool IsFinite(const DComplex& c) {
     return std::isfinite(c.real()) && std::isfinite(c.imag());
   }
@@ -456,6 +609,28 @@ class IterativeDiagonalOracle {
             if (std::isfinite(a)) {
               avg_abs_diff += a;
               ++n;
+            }
+          }
+        }
+        std::copy_n(ns_cb, size, solutions[cb].data());
+        continue;
+      }
+      if (np_ == 4) {
+        // four solution polarizations (SolverBase.cc:269-283): the entries of
+        // (new - old) * old^-1, skipped when old is singular
+        for (size_t i = 0; i != size; i += 4) {
+          const Mat2<double> s{{solutions[cb][i], solutions[cb][i + 1], solutions[cb][i + 2],
+                                solutions[cb][i + 3]}};
+          Mat2<double> s_inv = s;
+          if (Invert(s_inv)) {
+            Mat2<double> ns{{ns_cb[i] - s.m[0], ns_cb[i + 1] - s.m[1], ns_cb[i + 2] - s.m[2],
+                             ns_cb[i + 3] - s.m[3]}};
+            ns = Mul(ns, s_inv);
+            const double sumabs =
+                std::abs(ns.m[0]) + std::abs(ns.m[1]) + std::abs(ns.m[2]) + std::abs(ns.m[3]);
+            if (std::isfinite(sumabs)) {
+              avg_abs_diff += sumabs;
+              n += 4;
             }
           }
         }
@@ -541,6 +716,8 @@ class IterativeDiagonalOracle {
       for (std::vector<DComplex>& sv : solutions) {
         if (np_ == 1)
           MakeSolutionsFinite1Pol(sv);
+        else if (np_ == 4)
+          MakeSolutionsFinite4Pol(sv);
         else
           MakeSolutionsFinite2Pol(sv);
       }
@@ -592,7 +769,7 @@ class IterativeDiagonalOracle {
 
  private:
   orc_settings s_;
-  size_t np_;  // solution polarizations: 2 (diagonal) or 1 (scalar)
+  size_t np_;  // solution polarizations: 2 (diagonal), 1 (scalar) or 4 (full-Jones)
   size_t n_ant_, n_dir_, n_cb_, n_sub_;
   std::vector<size_t> n_sol_dir_;
   std::vector<size_t> sol_start_;
@@ -632,10 +809,10 @@ int SolveTyped(const orc_settings* s, const int64_t* vo, const float* data,
                const float* model, const uint32_t* a1, const uint32_t* a2,
                const uint32_t* sm, double* solutions, orc_result* result,
                double* step_mags, const orc_smoothness* smoothness = nullptr,
-               bool scalar = false) {
+               int n_pol = 2) {
   const std::vector<ChannelBlock<V>> cbs =
       MakeBlocks<V>(*s, vo, data, model, a1, a2, sm);
-  IterativeDiagonalOracle<V> solver(*s, scalar);
+  IterativeDiagonalOracle<V> solver(*s, n_pol);
   solver.Solve(cbs, solutions, result, step_mags, smoothness);
   return 0;
 }
@@ -706,15 +883,29 @@ extern "C" int orc_solve_scalar(const orc_settings* s, const orc_smoothness* c,
   if (c && (!c->frequencies || !(c->bandwidth_hz > 0.0))) return -1;
   if (s->n_correlations == 2)
     return s->fp64 ? SolveTyped<Duo<double>>(s, vo, data, model, a1, a2, sm, solutions, result,
-                                             step_mags, c, true)
+                                             step_mags, c, 1)
                    : SolveTyped<Duo<float>>(s, vo, data, model, a1, a2, sm, solutions, result,
-                                            step_mags, c, true);
+                                            step_mags, c, 1);
   if (s->n_correlations == 4)
     return s->fp64 ? SolveTyped<Full<double>>(s, vo, data, model, a1, a2, sm, solutions,
-                                              result, step_mags, c, true)
+                                              result, step_mags, c, 1)
                    : SolveTyped<Full<float>>(s, vo, data, model, a1, a2, sm, solutions, result,
-                                             step_mags, c, true);
+                                             step_mags, c, 1);
   return -2;
+}
+
+extern "C" int orc_solve_full_jones(const orc_settings* s, const orc_smoothness* c,
+                                    const int64_t* vo, const float* data, const float* model,
+                                    const uint32_t* a1, const uint32_t* a2, const uint32_t* sm,
+                                    double* solutions, orc_result* result, double* step_mags) {
+  if (!s || s->n_directions <= 0 || s->n_channel_blocks <= 0 || s->n_antennas <= 0) return -1;
+  if (c && (!c->frequencies || !(c->bandwidth_hz > 0.0))) return -1;
+  // the full-Jones solver takes FullSolveData only (IterativeFullJonesSolver.h)
+  if (s->n_correlations != 4) return -2;
+  return s->fp64 ? SolveTyped<Full<double>>(s, vo, data, model, a1, a2, sm, solutions, result,
+                                            step_mags, c, 4)
+                 : SolveTyped<Full<float>>(s, vo, data, model, a1, a2, sm, solutions, result,
+                                           step_mags, c, 4);
 }
 
 extern "C" int orc_iterate_once(const orc_settings* s, const int64_t* vo,
@@ -769,6 +960,43 @@ extern "C" double orc_chi2(int32_t n_ant, int32_t n_dir, int32_t n_cb,
         }
       }
       for (int k = 0; k != n_corr; ++k) chi2 += std::norm(r[k]);
+    }
+  }
+  return chi2;
+}
+
+// orc_chi2 for full-Jones solutions ([cb][ant][sub][4]):
+// |D - sum_d J_a M_d J_b^H|^2, everything in double.
+extern "C" double orc_chi2_full_jones(int32_t n_ant, int32_t n_dir, int32_t n_cb,
+                                      int32_t n_corr, int32_t n_sub, const int64_t* vo,
+                                      const float* data, const float* model,
+                                      const uint32_t* ant1, const uint32_t* ant2,
+                                      const uint32_t* sol_map, const double* solutions) {
+  if (n_corr != 4) return std::numeric_limits<double>::quiet_NaN();
+  const DComplex* sol = reinterpret_cast<const DComplex*>(solutions);
+  const size_t size = size_t(n_ant) * n_sub * 4;
+  double chi2 = 0.0;
+  for (int32_t cb = 0; cb != n_cb; ++cb) {
+    const size_t v0 = vo[cb];
+    const size_t nv = vo[cb + 1] - v0;
+    const DComplex* s = sol + cb * size;
+    for (size_t v = 0; v != nv; ++v) {
+      Mat2<double> r;
+      for (int k = 0; k != 4; ++k)
+        r.m[k] = DComplex(data[((v0 + v) * 4 + k) * 2], data[((v0 + v) * 4 + k) * 2 + 1]);
+      const uint32_t a1 = ant1[v0 + v], a2 = ant2[v0 + v];
+      for (int32_t d = 0; d != n_dir; ++d) {
+        const uint32_t si = sol_map[v0 * n_dir + size_t(d) * nv + v];
+        const DComplex* ja = &s[(a1 * size_t(n_sub) + si) * 4];
+        const DComplex* jb = &s[(a2 * size_t(n_sub) + si) * 4];
+        const float* m = &model[(v0 * n_dir + size_t(d) * nv + v) * 8];
+        const Mat2<double> mm{{DComplex(m[0], m[1]), DComplex(m[2], m[3]), DComplex(m[4], m[5]),
+                               DComplex(m[6], m[7])}};
+        const Mat2<double> term = Mul(Mat2<double>{{ja[0], ja[1], ja[2], ja[3]}},
+                                      MultiplyHerm(mm, Mat2<double>{{jb[0], jb[1], jb[2], jb[3]}}));
+        for (int k = 0; k != 4; ++k) r.m[k] -= term.m[k];
+      }
+      for (int k = 0; k != 4; ++k) chi2 += std::norm(r.m[k]);
     }
   }
   return chi2;

@@ -41,11 +41,12 @@ def small_problem(w=S.SMALL, n_corr=2, noise_sigma=None, seed=None):
     return w, geom, sky, gains, P
 
 
-def solvertester_problem(n_sol_per_dir=(1, 2, 5), n_corr=2, scalar=False):
+def solvertester_problem(n_sol_per_dir=(1, 2, 5), n_corr=2, scalar=False, leakage=False):
     """SolverTester::FillDdIntervalData in SolveData layout + input solutions
-    (SetScalarSolutions when `scalar`, else SetDiagonalSolutions)."""
+    (SetScalarSolutions when `scalar`, SetLeakageSolutions with leakage-only
+    data when `leakage`, else SetDiagonalSolutions)."""
     n_ant, n_dir, n_ch, n_t, n_cb = O.solvertester_shape()
-    data, model, inp = O.solvertester_fill(n_sol_per_dir, scalar=scalar)
+    data, model, inp = O.solvertester_fill(n_sol_per_dir, scalar=scalar, leakage=leakage)
     a1 = [a for a in range(n_ant) for b in range(a + 1, n_ant)]
     a2 = [b for a in range(n_ant) for b in range(a + 1, n_ant)]
     pk = O.pack_solve_data(data, model, a1, a2, n_ant, n_cb, n_sol_per_dir, n_corr=n_corr)
@@ -129,3 +130,24 @@ def golden_components(g):
             ref_freq=g["ref_freq"][i], log_si=bool(g["log_si"][i]),
             rm=tuple(g["rm"][i]) if g["has_rm"][i] else None))
     return out
+
+
+def unit_matrices(n_cb, n_ant, n_sub):
+    """Identity 2x2 solutions [n_cb, n_ant*n_sub*4] (the reference's full-Jones
+    tests start from unit matrices, tSolvers.cc:374-381)."""
+    s = np.zeros((n_cb, n_ant * n_sub, 4), np.complex128)
+    s[:, :, 0] = 1.0
+    s[:, :, 3] = 1.0
+    return s.reshape(n_cb, -1)
+
+
+def matrix_ref_relerr(sol_a, sol_b, n_ant, n_sub):
+    """Largest |A - B| / |B| (Frobenius, per matrix) after removing the phase of
+    antenna 0's J_00 per (block, sub-solution) from both: full-Jones
+    solutions are unique up to one phase per (block, direction solution)."""
+    def ref(s):
+        m = np.asarray(s).reshape(s.shape[0], n_ant, n_sub, 4)
+        p = m[:, :1, :, :1]
+        return m * (np.conj(p) / np.abs(p))
+    ra, rb = ref(sol_a), ref(sol_b)
+    return float(np.max(np.linalg.norm(ra - rb, axis=-1) / np.linalg.norm(rb, axis=-1)))
