@@ -175,7 +175,8 @@ SolverBase::SolveResult IterativeSolverHip::SolveImpl(
 }
 
 SolverBase::SolveResult IterativeSolverHip::Solve(
-    const DuoSolveData& data, std::vector<std::vector<DComplex>>& solutions, double) {
+    const DuoSolveData& data, std::vector<std::vector<DComplex>>& solutions, double time) {
+  if (n_solution_polarizations_ == 4) return SolverBase::Solve(data, solutions, time);
   return SolveImpl(data, solutions, 2);
 }
 

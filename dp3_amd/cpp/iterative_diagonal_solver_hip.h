@@ -1,9 +1,12 @@
-// IterativeDiagonalSolverHip and IterativeScalarSolverHip: DP3's
-// IterativeDiagonalSolver (ddecal/gain_solvers/IterativeDiagonalSolver.h:
-// 13-41) and IterativeScalarSolver (IterativeScalarSolver.h) behind the same
-// SolverBase interface, solving on an MI355X through libddecal_hip's C-ABI
-// (include/ddecal_hip.h). Duo (MC2x2FDiag) and Full (MC2x2F) data, two or
-// one solution polarizations, direction-dependent solution intervals.
+// IterativeDiagonalSolverHip, IterativeScalarSolverHip and
+// IterativeFullJonesSolverHip: DP3's IterativeDiagonalSolver
+// (ddecal/gain_solvers/IterativeDiagonalSolver.h:13-41),
+// IterativeScalarSolver (IterativeScalarSolver.h) and IterativeFullJonesSolver
+// (IterativeFullJonesSolver.h) behind the same SolverBase interface, solving
+// on an MI355X through libddecal_hip's C-ABI (include/ddecal_hip.h). Duo
+// (MC2x2FDiag) and Full (MC2x2F) data (Full only for the full-Jones solver),
+// two, one or four solution polarizations, direction-dependent solution
+// intervals.
 //
 // Behaviour as the reference: the iteration count is max+1 when the solve
 // does not converge (SolverBase.cc:197-200), NaN solutions are data, settings
@@ -25,8 +28,8 @@ struct ddh_solver;
 
 namespace dp3amd {
 
-// The shared implementation; n_solution_polarizations is 2 (diagonal) or 1
-// (scalar).
+// The shared implementation; n_solution_polarizations is 2 (diagonal), 1
+// (scalar) or 4 (full-Jones).
 class IterativeSolverHip : public SolverBase {
  public:
   ~IterativeSolverHip() override;
@@ -102,6 +105,16 @@ class IterativeDiagonalSolverHip final : public IterativeSolverHip {
 class IterativeScalarSolverHip final : public IterativeSolverHip {
  public:
   explicit IterativeScalarSolverHip(int model_storage = 0) : IterativeSolverHip(model_storage, 1) {}
+};
+
+// IterativeFullJonesSolver: a 2x2 gain matrix per antenna and solution
+// (NSolutionPolarizations() == 4, solutions [ant][sub][4] row-major per
+// channel block); Full data only, as the reference (Duo data raises
+// std::logic_error, SolverBase.h:213-219).
+class IterativeFullJonesSolverHip final : public IterativeSolverHip {
+ public:
+  explicit IterativeFullJonesSolverHip(int model_storage = 0)
+      : IterativeSolverHip(model_storage, 4) {}
 };
 
 }  // namespace dp3amd

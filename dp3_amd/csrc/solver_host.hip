@@ -150,8 +150,13 @@ void ValidateConfig(const ddh_config& c) {
   DDH_REQUIRE(c.n_channel_blocks >= 1, -1, "n_channel_blocks must be >= 1");
   DDH_REQUIRE(c.n_correlations == 2 || c.n_correlations == 4, -1,
               "n_correlations must be 2 (Duo) or 4 (Full)");
-  DDH_REQUIRE(c.n_solution_polarizations == 1 || c.n_solution_polarizations == 2, -1,
-              "n_solution_polarizations must be 1 (scalar) or 2 (diagonal)");
+  DDH_REQUIRE(c.n_solution_polarizations == 1 || c.n_solution_polarizations == 2 ||
+                  c.n_solution_polarizations == 4,
+              -1, "n_solution_polarizations must be 1 (scalar), 2 (diagonal) or 4 (full-Jones)");
+  DDH_REQUIRE(c.n_solution_polarizations != 4 || c.n_correlations == 4, -1,
+              "the full-Jones solver needs Full data (n_correlations 4)");
+  DDH_REQUIRE(c.n_solution_polarizations != 4 || c.iteration == DDH_ITERATION_STREAM, -4,
+              "the full-Jones solver uses the streaming iteration");
   DDH_REQUIRE(c.n_solutions_per_direction != nullptr, -1,
               "n_solutions_per_direction is required");
   DDH_REQUIRE(c.max_iterations >= 1 && c.max_iterations <= 100000, -1,
@@ -373,6 +378,10 @@ Structure DetectStructure(const ddh_cb_view* cbs, int n_cb, int n_dir) {
   return st;
 }
 
+// Complex values per (antenna, sub-solution) on the device: a 2x2 matrix for
+// the full-Jones solver, else a pair (equal for the scalar solver).
+int DevicePols(const ddh_config& c) { return c.n_solution_polarizations == 4 ? 4 : 2; }
+
 IterArgs MakeArgs(ddh_solver& s, ddh_problem& p) {
   const ddh_config& c = s.cfg.c;
   IterArgs a{};
@@ -422,6 +431,7 @@ IterArgs MakeArgs(ddh_solver& s, ddh_problem& p) {
   while (a.split < 8 && c.n_antennas * max_nsd * a.split * 2 <= threads) a.split *= 2;
   a.max_nsd = max_nsd;
   a.scalar = c.n_solution_polarizations == 1 ? 1 : 0;
+  a.full_jones = c.n_solution_polarizations == 4 ? 1 : 0;
   if (s.sm_on) {
     a.sm_freq = s.sm_freq.ptr;
     a.sm_corr = s.sm_corr.ptr;
@@ -498,7 +508,7 @@ void CheckMatch(const ddh_solver& s, const ddh_problem& p) {
 // Sizes the solver's per-solve device buffers for problem p.
 void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
   const ddh_config& c = s.cfg.c;
-  const size_t sol_per_cb = size_t(c.n_antennas) * s.cfg.n_sub * 2;
+  const size_t sol_per_cb = size_t(c.n_antennas) * s.cfg.n_sub * DevicePols(c);
   const size_t n_sol = sol_per_cb * c.n_channel_blocks;
   const size_t n_wg = size_t(c.n_channel_blocks) * c.n_directions;
   const void* before[3] = {s.sol0.ptr, s.partials.ptr, s.gram.ptr};
@@ -510,7 +520,9 @@ void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
   s.wg_mean.Resize(n_wg);
   s.wg_mean_n.Resize(n_wg);
   s.step_mags.Resize(size_t(std::max(c.max_iterations, c.min_iterations)) + 2);
-  s.partials.Resize(size_t(p.n_cb) * p.n_seg * p.n_dir * p.n_corr * 3 * p.bl_stride);
+  const size_t terms = c.n_solution_polarizations == 4 ? size_t(kFullJonesTerms)
+                                                      : size_t(p.n_corr) * 3;
+  s.partials.Resize(size_t(p.n_cb) * p.n_seg * p.n_dir * terms * p.bl_stride);
   s.state.Resize(1);
   if (c.iteration == DDH_ITERATION_GRAM)
     s.gram.Resize(size_t(p.n_cb) * p.n_seg * 2 * GramTerms(p.n_dir) * p.bl_stride);
@@ -519,13 +531,15 @@ void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
 }
 
 // Solutions cross the C-ABI in the caller's layout: per (antenna,
-// sub-solution) two values (diagonal solver) or one (scalar solver). The
-// device holds two per pair; a scalar solver keeps them equal.
+// sub-solution) two values (diagonal solver), one (scalar solver) or four
+// (full-Jones solver, a row-major 2x2 matrix). The device holds two per pair
+// (equal for a scalar solver) or the four of a matrix.
 void SolutionsIn(const ddh_solver& s, double2* dev, const double* host, size_t n_pairs,
                  hipStream_t stream) {
-  if (s.cfg.c.n_solution_polarizations == 2) {
-    DDH_HIP_CHECK(hipMemcpyAsync(dev, host, n_pairs * 2 * sizeof(double2), hipMemcpyHostToDevice,
-                                 stream));
+  const int np = s.cfg.c.n_solution_polarizations;
+  if (np == 2 || np == 4) {
+    DDH_HIP_CHECK(hipMemcpyAsync(dev, host, n_pairs * np * sizeof(double2),
+                                 hipMemcpyHostToDevice, stream));
     return;
   }
   std::vector<double2> tmp(n_pairs * 2);
@@ -541,8 +555,9 @@ void SolutionsIn(const ddh_solver& s, double2* dev, const double* host, size_t n
 void SolutionsOut(const ddh_solver& s, double* host, const void* dev, size_t n_pairs,
                   hipStream_t stream, bool real = false) {
   const size_t value = real ? sizeof(double) : sizeof(double2);
-  if (s.cfg.c.n_solution_polarizations == 2) {
-    DDH_HIP_CHECK(hipMemcpyAsync(host, dev, n_pairs * 2 * value, hipMemcpyDeviceToHost, stream));
+  const int np = s.cfg.c.n_solution_polarizations;
+  if (np == 2 || np == 4) {
+    DDH_HIP_CHECK(hipMemcpyAsync(host, dev, n_pairs * np * value, hipMemcpyDeviceToHost, stream));
     return;
   }
   std::vector<double> tmp(n_pairs * 2 * value / sizeof(double));
@@ -557,10 +572,11 @@ void SolutionsOut(const ddh_solver& s, double* host, const void* dev, size_t n_p
 void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* result) {
   const ddh_config& c = s.cfg.c;
   CheckMatch(s, p);
-  const size_t sol_per_cb = size_t(c.n_antennas) * s.cfg.n_sub * 2;
+  const size_t n_sols = size_t(c.n_antennas) * s.cfg.n_sub;
+  const size_t sol_per_cb = n_sols * DevicePols(c);
   PrepareBuffers(s, p);
   for (int cb = 0; cb < c.n_channel_blocks; ++cb)
-    SolutionsIn(s, s.sol0.ptr + cb * sol_per_cb, solutions[cb], sol_per_cb / 2, s.stream);
+    SolutionsIn(s, s.sol0.ptr + cb * sol_per_cb, solutions[cb], n_sols, s.stream);
   const IterArgs a = MakeArgs(s, p);
   const int limit = std::max(c.max_iterations, c.min_iterations);
   const bool graph = s.use_graph && !s.profile;
@@ -605,7 +621,7 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
   DDH_REQUIRE(st.done, -3, "solver did not reach its stopping criterion");
   const double2* final_sol = (st.iteration & 1) ? s.sol1.ptr : s.sol0.ptr;
   for (int cb = 0; cb < c.n_channel_blocks; ++cb)
-    SolutionsOut(s, solutions[cb], final_sol + cb * sol_per_cb, sol_per_cb / 2, s.stream);
+    SolutionsOut(s, solutions[cb], final_sol + cb * sol_per_cb, n_sols, s.stream);
   DDH_HIP_CHECK(hipStreamSynchronize(s.stream));
   float ms = 0.0f;
   DDH_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
@@ -1048,7 +1064,7 @@ int ddh_solver_set_smoothness(ddh_solver* s, const double* channel_block_frequen
       s->sm_weights.Upload(std::vector<double>(size_t(n_ant) * n_cb, 1.0), s->stream);
       s->sm_sub_weights = false;
     }
-    s->sm_out.Resize(size_t(n_cb) * n_ant * n_sub * 2);
+    s->sm_out.Resize(size_t(n_cb) * n_ant * n_sub * DevicePols(s->cfg.c));
     s->sm_bandwidth = bandwidth_hz;
     s->sm_truncate = kernel_truncation ? 1 : 0;
     DDH_HIP_CHECK(hipStreamSynchronize(s->stream));
@@ -1196,7 +1212,7 @@ void BuildBatch(ddh_batch& b) {
     if (!a.sm_freq) continue;
     b.smooth.n_intervals = int(b.args.size());
     b.smooth.max_elements =
-        std::max(b.smooth.max_elements, int64_t(a.n_cb) * a.n_ant * a.n_sub * 2);
+        std::max(b.smooth.max_elements, int64_t(a.n_cb) * a.n_ant * a.n_sub * (a.full_jones ? 4 : 2));
     b.smooth.finish_threads =
         std::max(b.smooth.finish_threads, SmoothFinishThreads(a.n_ant, a.max_nsd));
   }
@@ -1512,6 +1528,8 @@ ddh_batch* ddh_batch_create(ddh_solver* const* solvers, ddh_problem* const* prob
                   "all intervals of a batch share correlations and model storage");
       DDH_REQUIRE(GramMode(*b->solvers[i]) == b->gram, -4,
                   "all intervals of a batch share the iteration kind");
+      DDH_REQUIRE(b->solvers[i]->cfg.c.n_solution_polarizations != 4, -4,
+                  "batches take scalar and diagonal solvers; solve full-Jones intervals singly");
     }
     DDH_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     DDH_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b->host_states),
@@ -1680,7 +1698,8 @@ int ddh_autotune(ddh_solver* s, ddh_problem* p, int32_t* waves_out, int32_t* row
     DDH_REQUIRE(s && p, -1, "NULL argument");
     CheckMatch(*s, *p);
     PrepareBuffers(*s, *p);
-    const size_t n = size_t(s->cfg.c.n_antennas) * s->cfg.n_sub * 2 * s->cfg.c.n_channel_blocks;
+    const size_t n = size_t(s->cfg.c.n_antennas) * s->cfg.n_sub * DevicePols(s->cfg.c) *
+                     s->cfg.c.n_channel_blocks;
     std::vector<double2> ones(n, make_double2(1.0, 0.0));
     DDH_HIP_CHECK(hipMemcpyAsync(s->sol0.ptr, ones.data(), n * sizeof(double2),
                                  hipMemcpyHostToDevice, s->stream));
@@ -1719,6 +1738,8 @@ int ddh_one_step(ddh_solver* s, ddh_problem* p, double* const* solutions_in,
                  double* const* next_out, double* const* den_out) {
   return Guard([&] {
     DDH_REQUIRE(s && p && solutions_in && next_out, -1, "NULL argument");
+    DDH_REQUIRE(s->cfg.c.n_solution_polarizations != 4, -4,
+                "one_step serves the scalar and diagonal solvers");
     CheckMatch(*s, *p);
     PrepareBuffers(*s, *p);
     const ddh_config& c = s->cfg.c;
@@ -1748,9 +1769,10 @@ int ddh_chi2(ddh_solver* s, ddh_problem* p, double* const* solutions, double* ch
     CheckMatch(*s, *p);
     PrepareBuffers(*s, *p);
     const ddh_config& c = s->cfg.c;
-    const size_t per_cb = size_t(c.n_antennas) * s->cfg.n_sub * 2;
+    const size_t n_sols = size_t(c.n_antennas) * s->cfg.n_sub;
+    const size_t per_cb = n_sols * DevicePols(c);
     for (int cb = 0; cb < c.n_channel_blocks; ++cb)
-      SolutionsIn(*s, s->sol1.ptr + cb * per_cb, solutions[cb], per_cb / 2, s->stream);
+      SolutionsIn(*s, s->sol1.ptr + cb * per_cb, solutions[cb], n_sols, s->stream);
     const IterArgs a = MakeArgs(*s, *p);
     const int n_partial = Chi2Partials(a);
     s->chi2.Resize(size_t(n_partial) + 1);

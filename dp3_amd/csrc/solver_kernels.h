@@ -95,6 +95,9 @@ struct IterArgs {
   int32_t split;             // threads per (antenna, solution) in antenna_solve
   int32_t max_nsd;           // largest n_sol_dir entry
   int32_t scalar;            // 1: one gain for both polarizations (scalar solver)
+  int32_t full_jones;        // 1: 2x2 solutions [n_cb][n_ant][n_sub][4] (full-Jones
+                             // solver; Full data, streaming iteration, partials
+                             // [n_cb][n_seg][n_dir][kFullJonesTerms][bl_stride])
   // Smoothness constraint on the stepped solutions (null sm_freq: none).
   const double* sm_freq;        // [n_cb] channel-block frequencies (Hz)
   const double* sm_corr;        // [n_cb] frequency correction of the kernel size per block
@@ -110,6 +113,11 @@ struct IterArgs {
   // once per solve; [n_cb][n_seg][2][n_dir + n_dir(n_dir+1)/2][bl_stride].
   float2* gram;
 };
+
+// Per-baseline partials of the full-Jones correlate: X = sum R_all J_b M^H
+// (8 floats), Y = sum R_all^H J_a M (8), D_a = sum (J_b M^H)^H (J_b M^H) and
+// D_b = sum (J_a M)^H (J_a M) (Hermitian, 4 floats each: 00, 11, re 01, im 01).
+constexpr int kFullJonesTerms = 24;
 
 // Largest direction count of the Gram iteration (its per-lane accumulators
 // grow with n_dir^2).

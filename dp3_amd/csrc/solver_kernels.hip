@@ -868,6 +868,90 @@ __global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterAr
   AntennaSolveBlock<NC, F16>(a, blockIdx.x, blockIdx.y);
 }
 
+// The repaired value of 2x2 solution `m` (MakeSolutionsFinite4Pol,
+// SolverBase.cc:373-405): a matrix with a non-finite entry reads as the
+// block's mean |J_00| in all four entries, or as the identity when the block
+// had no finite matrix (mean < 0 marks that case).
+__device__ __forceinline__ void FixedMatrix(const double2* sol, size_t m, double mean,
+                                            double2 (&out)[4]) {
+  bool finite = true;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    out[k] = LoadGlobal(sol + m * 4 + k);
+    finite = finite && cfinite(out[k]);
+  }
+  if (finite) return;
+  if (mean < 0.0) {
+    out[0] = out[3] = make_double2(1.0, 0.0);
+    out[1] = out[2] = make_double2(0.0, 0.0);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = make_double2(mean, 0.0);
+  }
+}
+
+// acc += a * b and acc += conj(a) * b (complex64)
+__device__ __forceinline__ void CMac(float2& acc, float2 a, float2 b) {
+  acc.x += a.x * b.x - a.y * b.y;
+  acc.y += a.x * b.y + a.y * b.x;
+}
+__device__ __forceinline__ void CMacC(float2& acc, float2 a, float2 b) {
+  acc.x += a.x * b.x + a.y * b.y;
+  acc.y += a.x * b.y - a.y * b.x;
+}
+
+// 2x2 inverse in FP64 (adjugate over determinant); false when the
+// determinant is exactly zero.
+__device__ __forceinline__ bool Invert2x2(const double2 (&m)[4], double2 (&inv)[4]) {
+  const double2 p = cmul(m[0], m[3]), q = cmul(m[1], m[2]);
+  const double2 det = make_double2(p.x - q.x, p.y - q.y);
+  if (det.x == 0.0 && det.y == 0.0) return false;
+  const double n = cnorm(det);
+  const double2 r = make_double2(det.x / n, -det.y / n);
+  inv[0] = cmul(m[3], r);
+  inv[1] = cmul(make_double2(-m[1].x, -m[1].y), r);
+  inv[2] = cmul(make_double2(-m[2].x, -m[2].y), r);
+  inv[3] = cmul(m[0], r);
+  return true;
+}
+
+__device__ __forceinline__ void Mul2x2(const double2 (&a)[4], const double2 (&b)[4],
+                                       double2 (&c)[4]) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const double2 x = cmul(a[2 * i], b[j]), y = cmul(a[2 * i + 1], b[2 + j]);
+      c[2 * i + j] = make_double2(x.x + y.x, x.y + y.y);
+    }
+}
+
+// Convergence term of one 2x2 solution (SolverBase.cc:269-283): the entries
+// of (next - old) old^-1, skipped when old is singular; and |next_00| for the
+// next repair mean when next is finite.
+__device__ __forceinline__ void AddMatrixTerms(const double2 (&old)[4], const double2 (&next)[4],
+                                               double& conv_sum, int& conv_n, double& mean_sum,
+                                               int& mean_n) {
+  double2 inv[4];
+  if (Invert2x2(old, inv)) {
+    double2 diff[4], ns[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) diff[k] = make_double2(next[k].x - old[k].x, next[k].y - old[k].y);
+    Mul2x2(diff, inv, ns);
+    const double sumabs =
+        hypot(ns[0].x, ns[0].y) + hypot(ns[1].x, ns[1].y) + hypot(ns[2].x, ns[2].y) +
+        hypot(ns[3].x, ns[3].y);
+    if (isfinite(sumabs)) {
+      conv_sum += sumabs;
+      conv_n += 4;
+    }
+  }
+  if (cfinite(next[0]) && cfinite(next[1]) && cfinite(next[2]) && cfinite(next[3])) {
+    mean_sum += hypot(next[0].x, next[0].y);
+    mean_n += 1;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Smoothness constraint (SmoothnessConstraint.cc:50-97 with the Gaussian
 // kernel of KernelSmoother.h): each (antenna, sub-solution, polarization)
@@ -880,12 +964,18 @@ __global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterAr
 // the results go to sm_out, and a second pass per (direction, block) writes
 // them back with the convergence partials, which replace antenna_solve's.
 // ---------------------------------------------------------------------------
+// Solution values the smoothing pass covers (pairs, or 2x2 matrices).
+__host__ __device__ __forceinline__ int64_t SmoothElements(const IterArgs& a) {
+  return int64_t(a.n_cb) * a.n_ant * a.n_sub * (a.full_jones ? 4 : 2);
+}
+
 __device__ __forceinline__ void SmoothElement(const IterArgs& a, const int64_t e) {
-  const int n_series = a.n_ant * a.n_sub * 2;
+  const int n_pol = a.full_jones ? 4 : 2;
+  const int n_series = a.n_ant * a.n_sub * n_pol;
   const int i = int(e / n_series);
   const int idx = int(e - int64_t(i) * n_series);
-  const int sub = (idx >> 1) % a.n_sub;
-  const int ant = idx / (a.n_sub * 2);
+  const int sub = (idx / n_pol) % a.n_sub;
+  const int ant = idx / (a.n_sub * n_pol);
   const double2* nxt = (a.state->iteration & 1) ? a.sol0 : a.sol1;
   const double* w = a.sm_weights + (a.sm_sub_weights ? size_t(sub) * a.n_ant * a.n_cb : 0) +
                     size_t(ant) * a.n_cb;
@@ -921,6 +1011,17 @@ __device__ __forceinline__ void SmoothFinishBlock(const IterArgs& a, const int d
   for (int item = threadIdx.x; item < a.n_ant * nsd; item += blockDim.x) {
     const int ant = item / nsd;
     const size_t pair = gbase + size_t(ant) * a.n_sub + s0 + (item - ant * nsd);
+    if (a.full_jones) {
+      double2 next4[4], old4[4];
+      FixedMatrix(cur, pair, mean, old4);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        next4[k] = a.sm_out[pair * 4 + k];
+        nxt[pair * 4 + k] = next4[k];
+      }
+      AddMatrixTerms(old4, next4, conv_sum, conv_n, mean_sum, mean_n);
+      continue;
+    }
     double2 next[2], old[2];
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
@@ -944,7 +1045,7 @@ constexpr int kSmoothThreads = 256;
 
 __global__ void __launch_bounds__(kSmoothThreads) SmoothKernel(const IterArgs a) {
   if (a.state->done) return;
-  const int64_t n = int64_t(a.n_cb) * a.n_ant * a.n_sub * 2;
+  const int64_t n = SmoothElements(a);
   for (int64_t e = int64_t(blockIdx.x) * kSmoothThreads + threadIdx.x; e < n;
        e += int64_t(gridDim.x) * kSmoothThreads)
     SmoothElement(a, e);
@@ -961,7 +1062,7 @@ __global__ void __launch_bounds__(kSmoothThreads)
     BatchSmoothKernel(const IterArgs* __restrict__ args) {
   const IterArgs& a = args[blockIdx.y];
   if (!a.sm_freq || a.state->done) return;
-  const int64_t n = int64_t(a.n_cb) * a.n_ant * a.n_sub * 2;
+  const int64_t n = SmoothElements(a);
   for (int64_t e = int64_t(blockIdx.x) * kSmoothThreads + threadIdx.x; e < n;
        e += int64_t(gridDim.x) * kSmoothThreads)
     SmoothElement(a, e);
@@ -995,7 +1096,9 @@ __device__ __forceinline__ void DecideBlock(const IterArgs& a) {
       ms += a.wg_mean[c * a.n_dir + dd];
       mn += a.wg_mean_n[c * a.n_dir + dd];
     }
-    a.mean[c] = mn == 0 ? 1.0 : ms / mn;
+    // no finite solution: 1 for both polarizations (SolverBase.cc:356-358),
+    // the identity for 2x2 solutions (:390-394), marked by -1
+    a.mean[c] = mn == 0 ? (a.full_jones ? -1.0 : 1.0) : ms / mn;
   }
   if (threadIdx.x == 0) {
     DeviceState st = *a.state;
@@ -1047,17 +1150,20 @@ __device__ __forceinline__ void PrologueBlock(const IterArgs& a, const int cb) {
     *a.state = st;
   }
   const int n_pairs = a.n_ant * a.n_sub;
-  const double2* s = a.sol0 + size_t(cb) * n_pairs * 2;
+  const int n_pol = a.full_jones ? 4 : 2;
+  const double2* s = a.sol0 + size_t(cb) * n_pairs * n_pol;
   double sum = 0.0, unused = 0.0;
   int count = 0, unused_n = 0;
   for (int i = threadIdx.x; i < n_pairs; i += blockDim.x) {
-    if (cfinite(s[2 * i]) && cfinite(s[2 * i + 1])) {
-      sum += hypot(s[2 * i].x, s[2 * i].y);
+    bool finite = true;
+    for (int k = 0; k < n_pol; ++k) finite = finite && cfinite(s[n_pol * i + k]);
+    if (finite) {
+      sum += hypot(s[n_pol * i].x, s[n_pol * i].y);
       ++count;
     }
   }
   BlockSum2(sum, count, unused, unused_n, red, redc);
-  if (threadIdx.x == 0) a.mean[cb] = count == 0 ? 1.0 : sum / count;
+  if (threadIdx.x == 0) a.mean[cb] = count == 0 ? (a.full_jones ? -1.0 : 1.0) : sum / count;
 }
 
 __global__ void __launch_bounds__(kBlockThreads) PrologueKernel(const IterArgs a) {
@@ -1127,6 +1233,426 @@ __global__ void SumPartialsKernel(const double* partial, int n, double* out) {
     double t = 0.0;
     for (int i = 0; i < n; ++i) t += partial[i];
     *out = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Full-Jones solver (IterativeFullJonesSolver.cc:18-224): 2x2 solutions
+// [n_cb][n_ant][n_sub][4], row-major (00, 01, 10, 11), Full data. For
+// direction d of baseline (a, b), with A = J_a M, B = J_b M^H and R_all the
+// residual with every direction subtracted, the reference's per-visibility
+// sums (:143-176) over the residual R_d = R_all + A J_b^H become
+//   antenna a: numerator = sum R_d B   = X + J_a D_a,  denominator = D_a
+//   antenna b: numerator = sum R_d^H A = Y + J_b D_b,  denominator = D_b
+// with X = sum R_all B, Y = sum R_all^H A, D_a = sum B^H B, D_b = sum A^H A
+// (M J_b^H = B^H). The gains are constant over a baseline's rows within a
+// segment, so the correlate sums X, Y, D_a, D_b per baseline (24 floats) and
+// the antenna solve adds them over the antenna's baselines in FP64 and forms
+// J + X D^-1 = numerator * denominator^-1 (:178-185).
+// ---------------------------------------------------------------------------
+constexpr int kFullJonesGroup = 4;  // directions accumulated per workgroup
+
+// The four correlations of visibility `idx` of a Full plane, FP16 planes
+// scaled back.
+template <bool F16>
+__device__ __forceinline__ void LoadFull(const void* base, size_t idx, float sc, float2 (&v)[4]) {
+  LoadVis<4, F16>(base, idx, v);
+  if constexpr (F16) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k].x *= sc;
+      v[k].y *= sc;
+    }
+  }
+}
+
+// R -= (J_a M) J_b^H; returns A = J_a M.
+__device__ __forceinline__ void SubtractDirection(const float2* ja, const float2* jb,
+                                                  const float2 (&m)[4], float2 (&r)[4],
+                                                  float2 (&a)[4]) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      a[2 * i + j] = make_float2(0.0f, 0.0f);
+      CMac(a[2 * i + j], ja[2 * i], m[j]);
+      CMac(a[2 * i + j], ja[2 * i + 1], m[2 + j]);
+    }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      // (A J_b^H)_ij = A_i0 conj(Jb_j0) + A_i1 conj(Jb_j1) = sum conj(Jb_jk) A_ik
+      float2 t = make_float2(0.0f, 0.0f);
+      CMacC(t, jb[2 * j], a[2 * i]);
+      CMacC(t, jb[2 * j + 1], a[2 * i + 1]);
+      r[2 * i + j].x -= t.x;
+      r[2 * i + j].y -= t.y;
+    }
+}
+
+// correlate: grid (bl tiles x direction groups, segments, channel blocks),
+// block = waves x 64, lane = baseline. Every workgroup forms the full
+// residual over all directions and accumulates X, Y, D_a, D_b of its group's
+// (up to NG) directions, combined across waves through LDS in a fixed order.
+template <int NG, bool F16>
+__global__ void __launch_bounds__(kCorrelateMaxThreads) FullJonesCorrelateKernel(const IterArgs a) {
+  if (AsConst(a.state)->done) return;
+  const int n_tiles = a.bl_stride / kWave;
+  const int group = blockIdx.x / n_tiles;
+  const int tile = blockIdx.x - group * n_tiles;
+  const int seg = blockIdx.y;
+  const int cb = blockIdx.z;
+  const int g0 = group * NG;
+  const int ng = min(NG, a.n_dir - g0);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int n_waves = blockDim.x / kWave;
+  const CbInfo info = LoadConst(a.cbinfo + cb);
+  const size_t plane = size_t(a.n_t) * info.n_ch * a.bl_stride;
+  const int bl = tile * kWave + lane;
+
+  extern __shared__ float smem[];
+  float2* g_s = reinterpret_cast<float2*>(smem);  // [n_dir][64][8]: J_a, then J_b
+  float* red_s = smem + size_t(a.n_dir) * kWave * 16;  // [24 * NG][64]
+
+  // Stage the repaired, float-cast gains of every direction (SolveData.h
+  // casts the solutions to float); waves split the directions.
+  {
+    const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
+    const double2* sol = (AsConst(a.state)->iteration & 1) ? a.sol1 : a.sol0;
+    const double mean = AsConst(a.mean)[cb];
+    const bool valid = bl < a.n_bl;
+    const int a1 = valid ? a.ant1[bl] : 0;
+    const int a2 = valid ? a.ant2[bl] : 0;
+    for (int d = wave; d < a.n_dir; d += n_waves) {
+      const int s = AsConst(a.seg_sol)[seg * a.n_dir + d];
+      double2 j1[4], j2[4];
+      FixedMatrix(sol, gbase + size_t(a1) * a.n_sub + s, mean, j1);
+      FixedMatrix(sol, gbase + size_t(a2) * a.n_sub + s, mean, j2);
+      float2* dst = g_s + (size_t(d) * kWave + lane) * 8;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        dst[k] = valid ? make_float2(float(j1[k].x), float(j1[k].y)) : make_float2(0.0f, 0.0f);
+        dst[4 + k] = valid ? make_float2(float(j2[k].x), float(j2[k].y)) : make_float2(0.0f, 0.0f);
+      }
+    }
+  }
+  __syncthreads();
+
+  float2 x[NG][4], y[NG][4];
+  float da[NG][4], db[NG][4];
+#pragma unroll
+  for (int j = 0; j < NG; ++j)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      x[j][k] = y[j][k] = make_float2(0.0f, 0.0f);
+      da[j][k] = db[j][k] = 0.0f;
+    }
+  float sc[NG];
+#pragma unroll
+  for (int j = 0; j < NG; ++j)
+    sc[j] = (F16 && j < ng) ? AsConst(a.model_scale)[cb * a.n_dir + g0 + j] : 1.0f;
+
+  const int r0 = AsConst(a.seg_t0)[seg] * info.n_ch;
+  const int r1 = AsConst(a.seg_t0)[seg + 1] * info.n_ch;
+  for (int row = r0 + wave; row < r1; row += n_waves) {
+    const size_t e = size_t(row) * a.bl_stride + bl;
+    float2 r[4];
+    LoadVis<4, false>(a.data, info.data_off + e, r);
+    float2 m[NG][4];
+#pragma unroll
+    for (int j = 0; j < NG; ++j)
+      if (j < ng) LoadFull<F16>(a.model, info.model_off + (g0 + j) * plane + e, sc[j], m[j]);
+    // The group's directions, then the others (IterativeFullJonesSolver.cc:
+    // 88-89 subtracts every direction from the data).
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+      if (j < ng) {
+        const float2* gj = g_s + (size_t(g0 + j) * kWave + lane) * 8;
+        float2 av[4];
+        SubtractDirection(gj, gj + 4, m[j], r, av);
+      }
+    }
+    for (int d = 0; d < a.n_dir; ++d) {
+      if (d >= g0 && d < g0 + ng) continue;
+      const float s = F16 ? AsConst(a.model_scale)[cb * a.n_dir + d] : 1.0f;
+      float2 mo[4];
+      LoadFull<F16>(a.model, info.model_off + d * plane + e, s, mo);
+      const float2* gd = g_s + (size_t(d) * kWave + lane) * 8;
+      float2 av[4];
+      SubtractDirection(gd, gd + 4, mo, r, av);
+    }
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+      if (j >= ng) continue;
+      const float2* ja = g_s + (size_t(g0 + j) * kWave + lane) * 8;
+      const float2* jb = ja + 4;
+      float2 av[4], bv[4];
+      // A = J_a M
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          av[2 * i + k] = make_float2(0.0f, 0.0f);
+          CMac(av[2 * i + k], ja[2 * i], m[j][k]);
+          CMac(av[2 * i + k], ja[2 * i + 1], m[j][2 + k]);
+        }
+      // B = J_b M^H: B_ik = Jb_i0 conj(M_k0) + Jb_i1 conj(M_k1)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          float2 t = make_float2(0.0f, 0.0f);
+          CMacC(t, m[j][2 * k], jb[2 * i]);
+          CMacC(t, m[j][2 * k + 1], jb[2 * i + 1]);
+          bv[2 * i + k] = t;
+        }
+      // X += R B; Y += R^H A
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          CMac(x[j][2 * i + k], r[2 * i], bv[k]);
+          CMac(x[j][2 * i + k], r[2 * i + 1], bv[2 + k]);
+          CMacC(y[j][2 * i + k], r[i], av[k]);
+          CMacC(y[j][2 * i + k], r[2 + i], av[2 + k]);
+        }
+      // D_a += B^H B, D_b += A^H A (00, 11, 01)
+      da[j][0] += (bv[0].x * bv[0].x + bv[0].y * bv[0].y) + (bv[2].x * bv[2].x + bv[2].y * bv[2].y);
+      da[j][1] += (bv[1].x * bv[1].x + bv[1].y * bv[1].y) + (bv[3].x * bv[3].x + bv[3].y * bv[3].y);
+      {
+        float2 t = make_float2(da[j][2], da[j][3]);
+        CMacC(t, bv[0], bv[1]);
+        CMacC(t, bv[2], bv[3]);
+        da[j][2] = t.x;
+        da[j][3] = t.y;
+      }
+      db[j][0] += (av[0].x * av[0].x + av[0].y * av[0].y) + (av[2].x * av[2].x + av[2].y * av[2].y);
+      db[j][1] += (av[1].x * av[1].x + av[1].y * av[1].y) + (av[3].x * av[3].x + av[3].y * av[3].y);
+      {
+        float2 t = make_float2(db[j][2], db[j][3]);
+        CMacC(t, av[0], av[1]);
+        CMacC(t, av[2], av[3]);
+        db[j][2] = t.x;
+        db[j][3] = t.y;
+      }
+    }
+  }
+
+  auto term = [&](int j, int v) -> float& {
+    if (v < 8) return (v & 1) ? x[j][v >> 1].y : x[j][v >> 1].x;
+    if (v < 16) return (v & 1) ? y[j][(v - 8) >> 1].y : y[j][(v - 8) >> 1].x;
+    if (v < 20) return da[j][v - 16];
+    return db[j][v - 20];
+  };
+  // Combine the waves: wave w-1 += wave w, for w = n_waves-1 .. 1.
+  for (int w = n_waves - 1; w >= 1; --w) {
+    if (wave == w) {
+#pragma unroll
+      for (int j = 0; j < NG; ++j)
+#pragma unroll
+        for (int v = 0; v < kFullJonesTerms; ++v)
+          red_s[(j * kFullJonesTerms + v) * kWave + lane] = term(j, v);
+    }
+    __syncthreads();
+    if (wave == w - 1) {
+#pragma unroll
+      for (int j = 0; j < NG; ++j)
+#pragma unroll
+        for (int v = 0; v < kFullJonesTerms; ++v)
+          term(j, v) += red_s[(j * kFullJonesTerms + v) * kWave + lane];
+    }
+    __syncthreads();
+  }
+  if (wave == 0) {
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+      if (j >= ng) continue;
+      float* out = a.partials +
+                   ((size_t(cb) * a.n_seg + seg) * a.n_dir + g0 + j) * kFullJonesTerms * a.bl_stride;
+#pragma unroll
+      for (int v = 0; v < kFullJonesTerms; ++v) out[size_t(v) * a.bl_stride + bl] = term(j, v);
+    }
+  }
+}
+
+// antenna solve: grid (n_dir, n_cb); `split` threads per (antenna, solution
+// of d) add the antenna's baseline sums (X and D_a where it is the first
+// antenna, Y and D_b where it is the second) in FP64 and are combined by a
+// fixed shuffle tree; then numerator * denominator^-1 with NaN for a singular
+// denominator (IterativeFullJonesSolver.cc:178-185), the step
+// (SolverBase.cc:88-158, entry by entry) and the convergence partials.
+__device__ __forceinline__ void FullJonesSolveBlock(const IterArgs& a, const int d, const int cb) {
+  __shared__ double red[2 * kBlockThreads / 64];
+  __shared__ int redc[2 * kBlockThreads / 64];
+  const int iteration = a.state->iteration;
+  const double2* cur = (iteration & 1) ? a.sol1 : a.sol0;
+  double2* nxt = (iteration & 1) ? a.sol0 : a.sol1;
+  const int nsd = a.n_sol_dir[d];
+  const int s0 = a.sol_start[d];
+  const int n_items = a.n_ant * nsd;
+  const int split = a.split;
+  const int item = threadIdx.x / split;
+  const int q = threadIdx.x - item * split;
+  const bool active = item < n_items;
+  const int ant = active ? item / nsd : 0;
+  const int s_abs = s0 + (active ? item - ant * nsd : 0);
+  const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
+  const double mean = a.mean[cb];
+  const size_t bs = size_t(a.bl_stride);
+  int e_begin = 0, e_end = 0;
+  if (active) {
+    e_begin = AsGlobal(a.csr_off)[ant];
+    e_end = AsGlobal(a.csr_off)[ant + 1];
+  }
+  const GlobalT<int32_t>* csr = AsGlobal(a.csr);
+  double sx[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double sd[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int seg = 0; seg < a.n_seg; ++seg) {
+    if (!active || AsConst(a.seg_sol)[seg * a.n_dir + d] != s_abs) continue;
+    const GlobalT<float>* src = AsGlobal(a.partials) +
+                                ((size_t(cb) * a.n_seg + seg) * a.n_dir + d) * kFullJonesTerms * bs;
+    for (int e = e_begin + q; e < e_end; e += split) {
+      const int code = csr[2 * e];
+      const int b = code >> 1;
+      const int side = code & 1;
+      const GlobalT<float>* xs = src + (side ? 8 : 0) * bs + b;
+      const GlobalT<float>* ds = src + (side ? 20 : 16) * bs + b;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sx[k] += double(xs[k * bs]);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sd[k] += double(ds[k * bs]);
+    }
+  }
+  for (int m = 1; m < split; m <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sx[k] += __shfl_xor(sx[k], m);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sd[k] += __shfl_xor(sd[k], m);
+  }
+
+  double conv_sum = 0.0, mean_sum = 0.0;
+  int conv_n = 0, mean_n = 0;
+  if (active && q == 0) {
+    const size_t sidx = gbase + size_t(ant) * a.n_sub + s_abs;
+    double2 old[4];
+    FixedMatrix(cur, sidx, mean, old);
+    // the gain the correlate used (cast to float)
+    double2 jf[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) jf[k] = make_double2(double(float(old[k].x)), double(float(old[k].y)));
+    const double2 den[4] = {make_double2(sd[0], 0.0), make_double2(sd[2], sd[3]),
+                            make_double2(sd[2], -sd[3]), make_double2(sd[1], 0.0)};
+    double2 jd[4], num[4], inv[4], nx[4];
+    Mul2x2(jf, den, jd);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) num[k] = make_double2(sx[2 * k] + jd[k].x, sx[2 * k + 1] + jd[k].y);
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    if (Invert2x2(den, inv)) {
+      Mul2x2(num, inv, nx);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) nx[k] = make_double2(nan, nan);
+    }
+    double2 stepped[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (!a.phase_only) {
+        stepped[k] = make_double2(old[k].x * (1.0 - a.step_size) + nx[k].x * a.step_size,
+                                  old[k].y * (1.0 - a.step_size) + nx[k].y * a.step_size);
+      } else {
+        const double phase_from = atan2(old[k].y, old[k].x);
+        double distance = atan2(nx[k].y, nx[k].x) - phase_from;
+        if (distance > M_PI)
+          distance -= 2.0 * M_PI;
+        else if (distance < -M_PI)
+          distance += 2.0 * M_PI;
+        double sn, cs;
+        sincos(phase_from + a.step_size * distance, &sn, &cs);
+        stepped[k] = make_double2(cs, sn);
+      }
+      nxt[sidx * 4 + k] = stepped[k];
+    }
+    AddMatrixTerms(old, stepped, conv_sum, conv_n, mean_sum, mean_n);
+  }
+  BlockSum2(conv_sum, conv_n, mean_sum, mean_n, red, redc);
+  const int wg = cb * a.n_dir + d;
+  if (threadIdx.x == 0) {
+    a.wg_conv[wg] = conv_sum;
+    a.wg_conv_n[wg] = conv_n;
+    a.wg_mean[wg] = mean_sum;
+    a.wg_mean_n[wg] = mean_n;
+  }
+}
+
+__global__ void __launch_bounds__(kBlockThreads) FullJonesSolveKernel(const IterArgs a) {
+  if (a.state->done) return;
+  FullJonesSolveBlock(a, blockIdx.x, blockIdx.y);
+}
+
+// chi^2 with 2x2 solutions: |D - sum_d J_a M_d J_b^H|^2 in FP64 (gains
+// unrepaired). Grid (bl tiles, blocks); one partial per workgroup.
+template <bool F16>
+__global__ void __launch_bounds__(256) FullJonesChi2Kernel(const IterArgs a, const double2* sol,
+                                                           double* partial) {
+  __shared__ double red[256 / 64];
+  const int tile = blockIdx.x;
+  const int cb = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x / 64;
+  const int n_waves = blockDim.x / 64;
+  const int bl = tile * 64 + lane;
+  const CbInfo info = a.cbinfo[cb];
+  const size_t plane = size_t(a.n_t) * info.n_ch * a.bl_stride;
+  double acc = 0.0;
+  if (bl < a.n_bl) {
+    const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
+    const int a1 = a.ant1[bl], a2 = a.ant2[bl];
+    for (int seg = 0; seg < a.n_seg; ++seg) {
+      const int r0 = a.seg_t0[seg] * info.n_ch, r1 = a.seg_t0[seg + 1] * info.n_ch;
+      for (int row = r0 + wave; row < r1; row += n_waves) {
+        const size_t e = size_t(row) * a.bl_stride + bl;
+        float2 df[4];
+        LoadVis<4, false>(a.data, info.data_off + e, df);
+        double2 r[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r[k] = make_double2(df[k].x, df[k].y);
+        for (int d = 0; d < a.n_dir; ++d) {
+          const int s = a.seg_sol[seg * a.n_dir + d];
+          float2 mf[4];
+          LoadVis<4, F16>(a.model, info.model_off + d * plane + e, mf);
+          const double sc = F16 ? double(a.model_scale[cb * a.n_dir + d]) : 1.0;
+          double2 ja[4], jb[4], m[4], am[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            ja[k] = sol[(gbase + size_t(a1) * a.n_sub + s) * 4 + k];
+            jb[k] = sol[(gbase + size_t(a2) * a.n_sub + s) * 4 + k];
+            m[k] = make_double2(sc * mf[k].x, sc * mf[k].y);
+          }
+          Mul2x2(ja, m, am);
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+              const double2 p = cmulc(am[2 * i], jb[2 * j]);
+              const double2 t = cmulc(am[2 * i + 1], jb[2 * j + 1]);
+              r[2 * i + j].x -= p.x + t.x;
+              r[2 * i + j].y -= p.y + t.y;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc += cnorm(r[k]);
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (lane == 0) red[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < n_waves; ++w) t += red[w];
+    partial[size_t(cb) * gridDim.x + tile] = t;
   }
 }
 
@@ -1389,6 +1915,13 @@ size_t CorrelateSharedBytes(int n_dir, int n_corr, int waves) {
 }
 
 namespace {
+using FullJonesKernelFn = void (*)(IterArgs);
+const FullJonesKernelFn kFullJonesCorrelate[2][kFullJonesGroup] = {
+    {FullJonesCorrelateKernel<1, false>, FullJonesCorrelateKernel<2, false>,
+     FullJonesCorrelateKernel<3, false>, FullJonesCorrelateKernel<4, false>},
+    {FullJonesCorrelateKernel<1, true>, FullJonesCorrelateKernel<2, true>,
+     FullJonesCorrelateKernel<3, true>, FullJonesCorrelateKernel<4, true>}};
+
 int g_max_lds = 64 * 1024;
 int g_max_lds_solve = 160 * 1024;
 }
@@ -1423,6 +1956,10 @@ void PrepareKernelAttributes() {
                                         hipFuncAttributeMaxDynamicSharedMemorySize,
                                         int(kApplyGramMaxShared)));
     }
+  for (int h = 0; h < 2; ++h)
+    for (int g = 0; g < kFullJonesGroup; ++g)
+      DDH_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kFullJonesCorrelate[h][g]),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
   // antenna_solve also holds static LDS for its reductions
   for (const void* f : {reinterpret_cast<const void*>(&AntennaSolveKernel<2, false>),
                         reinterpret_cast<const void*>(&AntennaSolveKernel<2, true>),
@@ -1516,9 +2053,32 @@ void LaunchBatchPrologue(const IterArgs* d_args, const BatchTile* d_items, int n
   DDH_HIP_CHECK(hipGetLastError());
 }
 
+size_t FullJonesCorrelateSharedBytes(int n_dir) {
+  const int ng = std::min(n_dir, kFullJonesGroup);
+  return size_t(n_dir) * kWave * 8 * sizeof(float2) +
+         size_t(kFullJonesTerms) * ng * kWave * sizeof(float);
+}
+
+// Full-Jones correlate: directions in groups of up to kFullJonesGroup, one
+// workgroup per (tile, group, segment, block).
+void LaunchFullJonesCorrelate(const IterArgs& a, bool f16, int waves, hipStream_t stream) {
+  const int ng = std::min(a.n_dir, kFullJonesGroup);
+  const int n_groups = (a.n_dir + ng - 1) / ng;
+  const dim3 grid((a.bl_stride / kWave) * n_groups, a.n_seg, a.n_cb);
+  const size_t lds = FullJonesCorrelateSharedBytes(a.n_dir);
+  DDH_REQUIRE(lds <= MaxSharedBytes(), -4, "full-Jones correlate needs more LDS than the device has");
+  hipLaunchKernelGGL(kFullJonesCorrelate[f16][ng - 1], grid, dim3(waves * kWave), lds, stream, a);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
 void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, int rows,
                      hipStream_t stream) {
   DDH_REQUIRE(waves >= 1 && waves * kWave <= kCorrelateMaxThreads, -1, "bad waves per tile");
+  if (a.full_jones) {
+    DDH_REQUIRE(n_corr == 4, -1, "the full-Jones solver needs Full data");
+    LaunchFullJonesCorrelate(a, f16, waves, stream);
+    return;
+  }
   const KernelFn f = CorrelateFor(a.n_dir, n_corr, f16, rows);
   const dim3 grid(a.bl_stride / kWave, a.n_seg, a.n_cb);
   const size_t lds = CorrelateSharedBytes(a.n_dir, n_corr, waves);
@@ -1541,8 +2101,13 @@ int AntennaSolveThreads(int n_ant, int max_nsd) {
 }
 
 void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, hipStream_t stream) {
-  const size_t lds = AntennaSolveSharedBytes(a, n_corr);
   const dim3 block(AntennaSolveThreads(a.n_ant, a.max_nsd));
+  if (a.full_jones) {
+    hipLaunchKernelGGL(FullJonesSolveKernel, dim3(a.n_dir, a.n_cb), block, 0, stream, a);
+    DDH_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  const size_t lds = AntennaSolveSharedBytes(a, n_corr);
   DDH_REQUIRE(lds <= size_t(g_max_lds_solve), -4, "too many baselines for the antenna solve LDS");
   const dim3 grid(a.n_dir, a.n_cb);
   if (n_corr == 2) {
@@ -1565,7 +2130,13 @@ void LaunchChi2(const IterArgs& a, int n_corr, bool f16, const double2* sol, dou
                 int n_partial, double* out, hipStream_t stream) {
   DDH_REQUIRE(n_partial >= Chi2Partials(a), -1, "chi2 partial buffer too small");
   const dim3 grid(a.bl_stride / kWave, a.n_cb);
-  if (n_corr == 2) {
+  if (a.full_jones) {
+    if (f16)
+      hipLaunchKernelGGL((FullJonesChi2Kernel<true>), grid, dim3(256), 0, stream, a, sol, partial);
+    else
+      hipLaunchKernelGGL((FullJonesChi2Kernel<false>), grid, dim3(256), 0, stream, a, sol,
+                         partial);
+  } else if (n_corr == 2) {
     if (f16)
       hipLaunchKernelGGL((Chi2Kernel<2, true>), grid, dim3(256), 0, stream, a, sol, partial);
     else
@@ -1588,7 +2159,7 @@ int SmoothFinishThreads(int n_ant, int max_nsd) {
 
 void LaunchSmoothing(const IterArgs& a, hipStream_t stream) {
   if (!a.sm_freq) return;
-  const int64_t n = int64_t(a.n_cb) * a.n_ant * a.n_sub * 2;
+  const int64_t n = SmoothElements(a);
   const int64_t blocks = std::min<int64_t>((n + kSmoothThreads - 1) / kSmoothThreads, 1024);
   hipLaunchKernelGGL(SmoothKernel, dim3(unsigned(std::max<int64_t>(blocks, 1))),
                      dim3(kSmoothThreads), 0, stream, a);

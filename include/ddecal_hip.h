@@ -70,7 +70,11 @@ typedef struct {
   int32_t iteration;                         // DDH_ITERATION_*
   // 2: IterativeDiagonalSolver (default); 1: IterativeScalarSolver
   // (IterativeScalarSolver.cc:30-226), one gain per antenna and solution for
-  // both correlations, solutions [n_ant][n_sub] per channel block
+  // both correlations, solutions [n_ant][n_sub] per channel block; 4:
+  // IterativeFullJonesSolver (IterativeFullJonesSolver.cc:18-224), a 2x2
+  // matrix per antenna and solution, solutions [n_ant][n_sub][4] (row-major
+  // 00, 01, 10, 11) per channel block; Full data and the streaming iteration
+  // only, single solves (no batch, no one-step)
   int32_t n_solution_polarizations;
 } ddh_config;
 

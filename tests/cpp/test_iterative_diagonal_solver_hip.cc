@@ -22,6 +22,7 @@
 using dp3amd::DuoSolveData;
 using dp3amd::FullSolveData;
 using dp3amd::IterativeDiagonalSolverHip;
+using dp3amd::IterativeFullJonesSolverHip;
 using dp3amd::IterativeScalarSolverHip;
 using dp3amd::SolverBase;
 using DComplex = std::complex<double>;
@@ -60,8 +61,11 @@ struct Fixture {
 
   int n_pol = 2;  // solution polarizations of the solver under test
 
-  Fixture(std::vector<int32_t> n_solutions, int n_corr, bool scalar = false)
-      : n_sol(std::move(n_solutions)), n_pol(scalar ? 1 : 2) {
+  // n_solution_pols: 2 (diagonal), 1 (scalar fixture) or 4 (full-Jones
+  // solver on the diagonal fixture)
+  Fixture(std::vector<int32_t> n_solutions, int n_corr, int n_solution_pols = 2)
+      : n_sol(std::move(n_solutions)), n_pol(n_solution_pols) {
+    const bool scalar = n_pol == 1;
     orc_solvertester_shape(&n_ant, &n_dir, &n_ch, &n_t, &n_cb);
     const int n_bl = n_ant * (n_ant - 1) / 2;
     int n_sub = 0;
@@ -125,9 +129,14 @@ struct Fixture {
     return out;
   }
 
+  // Unit gains; identity matrices for the full-Jones solver (tSolvers.cc:374-381).
   std::vector<std::vector<DComplex>> UnitSolutions() const {
-    return std::vector<std::vector<DComplex>>(
+    std::vector<std::vector<DComplex>> out(
         n_cb, std::vector<DComplex>(size_t(n_ant) * NSub() * n_pol, 1.0));
+    if (n_pol == 4)
+      for (auto& v : out)
+        for (size_t i = 0; i < v.size(); i += 4) v[i + 1] = v[i + 2] = 0.0;
+    return out;
   }
 
   void InitializeSolver(SolverBase& solver) const {
@@ -189,7 +198,11 @@ struct Fixture {
     s.n_threads = 4;
     std::vector<double> flat(size_t(n_cb) * n_ant * NSub() * n_pol * 2);
     for (size_t i = 0; i < flat.size(); i += 2) flat[i] = 1.0;
-    if (n_pol == 1)
+    if (n_pol == 4) {
+      for (size_t i = 0; i < flat.size(); i += 8) flat[i + 2] = flat[i + 4] = 0.0;
+      orc_solve_full_jones(&s, smoothness, offsets.data(), data.data(), model.data(), ant1.data(),
+                           ant2.data(), sol_map.data(), flat.data(), result, nullptr);
+    } else if (n_pol == 1)
       orc_solve_scalar(&s, smoothness, offsets.data(), data.data(), model.data(), ant1.data(),
                        ant2.data(), sol_map.data(), flat.data(), result, nullptr);
     else if (smoothness)
@@ -304,7 +317,7 @@ void SmoothnessConstraint() {
 // at 1e-3; and parity with the oracle's scalar solve.
 template <typename SolveDataType>
 void ScalarDdIntervals(int n_corr) {
-  Fixture f({1, 2, 5}, n_corr, true);
+  Fixture f({1, 2, 5}, n_corr, 1);
   IterativeScalarSolverHip solver;
   f.InitializeSolver(solver);
   CHECK(solver.NSolutionPolarizations() == 1u);
@@ -327,6 +340,56 @@ void ScalarDdIntervals(int n_corr) {
               n_corr, result.iterations, static_cast<long long>(oracle_result.iterations), worst);
   CHECK(worst <= 1e-4);
   CHECK(int64_t(result.iterations) <= oracle_result.iterations);
+}
+
+// tSolvers.cc:409-450 (iterative_full_jones_dd_intervals): SetDiagonalSolutions,
+// unit matrices, the diagonals through CheckDualResults at 1e-2 %; parity with
+// the oracle's full-Jones solve (per matrix, after removing antenna 0's J_00
+// phase); Duo data raises std::logic_error.
+void FullJonesDdIntervals() {
+  Fixture f({1, 2, 5}, 4, 4);
+  IterativeFullJonesSolverHip solver;
+  f.InitializeSolver(solver);
+  CHECK(solver.NSolutionPolarizations() == 4u);
+  const FullSolveData data = f.Make<FullSolveData>(4);
+  auto solutions = f.UnitSolutions();
+  const SolverBase::SolveResult result = solver.Solve(data, solutions, 0.0);
+  std::vector<std::vector<DComplex>> diagonals(solutions.size());
+  for (size_t cb = 0; cb < solutions.size(); ++cb)
+    for (size_t i = 0; i < solutions[cb].size(); i += 4) {
+      diagonals[cb].push_back(solutions[cb][i]);
+      diagonals[cb].push_back(solutions[cb][i + 3]);
+    }
+  f.CheckDualResults(diagonals, 1.0e-2);
+  orc_result oracle_result{};
+  const auto reference = f.OracleSolve(4, &oracle_result);
+  const size_t n_sub = size_t(f.NSub());
+  double worst = 0.0;
+  for (size_t cb = 0; cb < solutions.size(); ++cb)
+    for (size_t m = 0; m < solutions[cb].size() / 4; ++m) {
+      const size_t r = m % n_sub;  // antenna 0, same sub-solution
+      const DComplex pa = std::conj(solutions[cb][r * 4]) / std::abs(solutions[cb][r * 4]);
+      const DComplex pb = std::conj(reference[cb][r * 4]) / std::abs(reference[cb][r * 4]);
+      double num = 0.0, den = 0.0;
+      for (int k = 0; k < 4; ++k) {
+        num += std::norm(solutions[cb][m * 4 + k] * pa - reference[cb][m * 4 + k] * pb);
+        den += std::norm(reference[cb][m * 4 + k] * pb);
+      }
+      worst = std::max(worst, std::sqrt(num / den));
+    }
+  std::printf("full-Jones dd intervals: iterations %zu (oracle %lld), gain error vs oracle %.2e\n",
+              result.iterations, static_cast<long long>(oracle_result.iterations), worst);
+  CHECK(worst <= 1e-4);
+  CHECK(int64_t(result.iterations) <= oracle_result.iterations);
+  bool threw = false;
+  try {
+    const DuoSolveData duo = f.Make<DuoSolveData>(2);
+    auto duo_solutions = f.UnitSolutions();
+    solver.Solve(duo, duo_solutions, 0.0);
+  } catch (const std::logic_error&) {
+    threw = true;
+  }
+  CHECK(threw);
 }
 
 void IterationSemantics() {
@@ -400,6 +463,7 @@ int main() {
       {"iterative_diagonal_dd_intervals", [] { DdIntervals<FullSolveData>(4); }},
       {"iterative_duo_scalar_dd_intervals", [] { ScalarDdIntervals<DuoSolveData>(2); }},
       {"iterative_scalar_dd_intervals", [] { ScalarDdIntervals<FullSolveData>(4); }},
+      {"iterative_full_jones_dd_intervals", FullJonesDdIntervals},
       {"iteration_semantics", IterationSemantics},
       {"smoothness_constraint", SmoothnessConstraint},
       {"errors", Errors},

@@ -139,4 +139,4 @@ def test_scalar_rejects_bad_polarizations(small):
     w, geom, sky, gains, P = small
     with pytest.raises(H.HipError, match="n_solution_polarizations"):
         H.Solver(P.n_antennas, P.n_solutions_per_direction, P.n_channel_blocks,
-                 n_solution_polarizations=4)
+                 n_solution_polarizations=3)
