@@ -355,6 +355,28 @@ def main():
     for _ in range(2):  # the second call is timed
         ing_ms = ing.assign_and_weight(sub_data, sub_model, a1, a2, flags=sub_f, weights=sub_w)
     ing.close()
+    # ---- the full-Jones solver (IterativeFullJonesSolver) on one interval of
+    # the same shape: a Full complex64 problem packed on the device from the
+    # raw buffers, a fixed-length solve from unit matrices with the correlate
+    # timed per iteration (HIP events)
+    fj = H.Solver(w.n_stations, w.sols_per_dir(), w.n_channel_blocks, n_correlations=4,
+                  n_solution_polarizations=4, max_iterations=ROOF_ITERATIONS, accuracy=0.0,
+                  detect_stalling=False)
+    fj_dp = H.DeviceProblem(fj, w.solint, a1[cross], a2[cross], np.diff(cbs), s16[0].sol_index)
+    fj_dp.assign_and_weight(sub_data, sub_model, a1, a2)
+    fj.set_schedule(waves_per_tile=4, rows_per_trip=1, use_graph=0, poll_interval=8)
+    fj.set_profiling(True)
+    fj_sub = int(np.sum(w.sols_per_dir()))
+    fj_g = np.zeros((w.n_channel_blocks, w.n_stations * fj_sub, 4), np.complex128)
+    fj_g[:, :, 0] = fj_g[:, :, 3] = 1.0
+    fj_g = fj_g.reshape(w.n_channel_blocks, -1)
+    fj.solve_problem(fj_dp, fj_g)
+    fj_cms, fj_n, fj_total = fj.last_timing()
+    fj_corr = fj_cms / max(fj_n, 1)
+    fj_iter = fj_total / max(fj_n, 1)
+    fj_dp.close()
+    fj.close()
+    log(f"full-Jones solver: {fj_iter * 1e3:.1f} us per iteration, correlate {fj_corr * 1e3:.1f} us")
     del sub_data, sub_model, sub_w, sub_f
     sub_vis = w.solint * n_bl_all * w.n_channels
     cross_vis = w.solint * int(cross.sum()) * w.n_channels
@@ -372,7 +394,15 @@ def main():
                   f"problem: the {cross_vis} raw cross-correlation visibilities (data, flags, "
                   "weights, every model) read once and written packed (Duo); the autocorrelations "
                   "are not read; two passes over the raw buffers (the first finds the FP16 "
-                  "scale)"))}
+                  "scale)")),
+        "full_jones_correlate": roof(
+            "full_jones_correlate", cross_vis * (32 + w.n_directions * 32), fj_corr,
+            iteration_ms=round(fj_iter, 5),
+            note=(f"IterativeFullJonesSolver correlate over one interval ({cross_vis} "
+                  "visibilities, 4 correlations, complex64 planes): data and every model plane "
+                  "read once per direction group of four (8 directions: two groups, so the "
+                  "planes are read twice), 24 per-baseline partials per direction written; "
+                  "iteration_ms adds the FP64 antenna solve and the decision"))}
     log(f"correct and subtract: {sub_ms * 1e3:.1f} us per interval; assign and weight "
         f"(FP16 problem): {ing_ms * 1e3:.1f} us per interval")
 
