@@ -400,9 +400,9 @@ def main():
             iteration_ms=round(fj_iter, 5),
             note=(f"IterativeFullJonesSolver correlate over one interval ({cross_vis} "
                   "visibilities, 4 correlations, complex64 planes): data and every model plane "
-                  "read once per direction group of four (8 directions: two groups, so the "
-                  "planes are read twice), 24 per-baseline partials per direction written; "
-                  "iteration_ms adds the FP64 antenna solve and the decision"))}
+                  "read once (all 8 directions in one group), X and Y (16 per-baseline partials "
+                  "per direction) written; the denominators come from model products made once "
+                  "per solve; iteration_ms adds the FP64 antenna solve and the decision"))}
     log(f"correct and subtract: {sub_ms * 1e3:.1f} us per interval; assign and weight "
         f"(FP16 problem): {ing_ms * 1e3:.1f} us per interval")
 

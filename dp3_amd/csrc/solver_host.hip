@@ -106,6 +106,7 @@ struct ddh_solver {
   ddh::DeviceBuffer<double> mean, wg_conv, wg_mean, step_mags;
   ddh::DeviceBuffer<int32_t> wg_conv_n, wg_mean_n;
   ddh::DeviceBuffer<float> partials;
+  ddh::DeviceBuffer<float> fj_products;  // full-Jones model products (once per solve)
   ddh::DeviceBuffer<ddh::DeviceState> state;
   ddh::DeviceBuffer<int32_t> n_sol_dir, sol_start;
   ddh::DeviceBuffer<double> den, chi2;  // one-step denominators; chi^2 partials + total
@@ -432,6 +433,7 @@ IterArgs MakeArgs(ddh_solver& s, ddh_problem& p) {
   a.max_nsd = max_nsd;
   a.scalar = c.n_solution_polarizations == 1 ? 1 : 0;
   a.full_jones = c.n_solution_polarizations == 4 ? 1 : 0;
+  a.fj_products = s.fj_products.ptr;
   if (s.sm_on) {
     a.sm_freq = s.sm_freq.ptr;
     a.sm_corr = s.sm_corr.ptr;
@@ -467,6 +469,7 @@ void LaunchCorrelation(const ddh_solver& s, const IterArgs& a, const ddh_problem
 void LaunchSolvePrologue(const ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
   LaunchPrologue(a, s.stream);
   if (GramMode(s)) LaunchGram(a, p.f16, s.waves, s.stream);
+  if (a.full_jones) LaunchFullJonesModelProducts(a, p.f16, s.waves, s.stream);
 }
 
 void LaunchIteration(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
@@ -480,7 +483,8 @@ void LaunchIteration(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
 // the solver buffers stay the same.
 void EnsureGraph(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
   const std::vector<const void*> key = {&p, p.data.ptr, p.model.ptr, s.sol0.ptr, s.partials.ptr,
-                                        s.gram.ptr, s.sm_out.ptr, s.sm_weights.ptr,
+                                        s.gram.ptr, s.fj_products.ptr, s.sm_out.ptr,
+                                        s.sm_weights.ptr,
                                         reinterpret_cast<const void*>(intptr_t(s.poll)),
                                         reinterpret_cast<const void*>(intptr_t(s.waves)),
                                         reinterpret_cast<const void*>(intptr_t(s.unroll))};
@@ -511,7 +515,7 @@ void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
   const size_t sol_per_cb = size_t(c.n_antennas) * s.cfg.n_sub * DevicePols(c);
   const size_t n_sol = sol_per_cb * c.n_channel_blocks;
   const size_t n_wg = size_t(c.n_channel_blocks) * c.n_directions;
-  const void* before[3] = {s.sol0.ptr, s.partials.ptr, s.gram.ptr};
+  const void* before[4] = {s.sol0.ptr, s.partials.ptr, s.gram.ptr, s.fj_products.ptr};
   s.sol0.Resize(n_sol);
   s.sol1.Resize(n_sol);
   s.mean.Resize(c.n_channel_blocks);
@@ -526,7 +530,10 @@ void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
   s.state.Resize(1);
   if (c.iteration == DDH_ITERATION_GRAM)
     s.gram.Resize(size_t(p.n_cb) * p.n_seg * 2 * GramTerms(p.n_dir) * p.bl_stride);
-  if (before[0] != s.sol0.ptr || before[1] != s.partials.ptr || before[2] != s.gram.ptr)
+  if (c.n_solution_polarizations == 4)
+    s.fj_products.Resize(size_t(p.n_cb) * p.n_seg * p.n_dir * kFullJonesModelTerms * p.bl_stride);
+  if (before[0] != s.sol0.ptr || before[1] != s.partials.ptr || before[2] != s.gram.ptr ||
+      before[3] != s.fj_products.ptr)
     DestroyGraph(s);
 }
 

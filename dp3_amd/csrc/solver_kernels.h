@@ -98,6 +98,8 @@ struct IterArgs {
   int32_t full_jones;        // 1: 2x2 solutions [n_cb][n_ant][n_sub][4] (full-Jones
                              // solver; Full data, streaming iteration, partials
                              // [n_cb][n_seg][n_dir][kFullJonesTerms][bl_stride])
+  float* fj_products;        // full-Jones model products, once per solve
+                             // [n_cb][n_seg][n_dir][kFullJonesModelTerms][bl_stride]
   // Smoothness constraint on the stepped solutions (null sm_freq: none).
   const double* sm_freq;        // [n_cb] channel-block frequencies (Hz)
   const double* sm_corr;        // [n_cb] frequency correction of the kernel size per block
@@ -115,9 +117,11 @@ struct IterArgs {
 };
 
 // Per-baseline partials of the full-Jones correlate: X = sum R_all J_b M^H
-// (8 floats), Y = sum R_all^H J_a M (8), D_a = sum (J_b M^H)^H (J_b M^H) and
-// D_b = sum (J_a M)^H (J_a M) (Hermitian, 4 floats each: 00, 11, re 01, im 01).
-constexpr int kFullJonesTerms = 24;
+// (8 floats) and Y = sum R_all^H J_a M (8); and the model products made once
+// per solve, Q[p][q] = sum M_p conj(M_q) over the four entries of M
+// (Hermitian: 4 real diagonal terms and 6 complex upper terms).
+constexpr int kFullJonesTerms = 16;
+constexpr int kFullJonesModelTerms = 16;
 
 // Largest direction count of the Gram iteration (its per-lane accumulators
 // grow with n_dir^2).
@@ -140,6 +144,8 @@ void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, int row
 void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, hipStream_t stream);
 void LaunchDecide(const IterArgs& a, hipStream_t stream);
 void LaunchPrologue(const IterArgs& a, hipStream_t stream);
+// Full-Jones solver: the model products of the problem, once per solve.
+void LaunchFullJonesModelProducts(const IterArgs& a, bool f16, int waves, hipStream_t stream);
 // FP64 chi^2 of the residual with solutions `sol` (current buffer of `a`):
 // per-workgroup partial sums into `partial`, then their fixed-order sum into
 // `out`.

@@ -1246,11 +1246,15 @@ __global__ void SumPartialsKernel(const double* partial, int n, double* out) {
 //   antenna b: numerator = sum R_d^H A = Y + J_b D_b,  denominator = D_b
 // with X = sum R_all B, Y = sum R_all^H A, D_a = sum B^H B, D_b = sum A^H A
 // (M J_b^H = B^H). The gains are constant over a baseline's rows within a
-// segment, so the correlate sums X, Y, D_a, D_b per baseline (24 floats) and
-// the antenna solve adds them over the antenna's baselines in FP64 and forms
-// J + X D^-1 = numerator * denominator^-1 (:178-185).
+// segment, so the correlate sums X and Y per baseline (16 floats), and the
+// denominators follow from model products Q made once per solve:
+//   D_a,ij = sum_kl (J_b^H J_b)_kl Q[2i+k][2j+l],
+//   D_b,ij = sum_kl (J_a^H J_a)_kl Q[2l+j][2k+i],   Q[p][q] = sum M_p conj(M_q)
+// (p, q index the four entries of M). The antenna solve adds them over the
+// antenna's baselines in FP64 and forms J + X D^-1 = numerator *
+// denominator^-1 (:178-185).
 // ---------------------------------------------------------------------------
-constexpr int kFullJonesGroup = 4;  // directions accumulated per workgroup
+constexpr int kFullJonesGroup = 8;  // directions accumulated per workgroup
 
 // The four correlations of visibility `idx` of a Full plane, FP16 planes
 // scaled back.
@@ -1293,8 +1297,8 @@ __device__ __forceinline__ void SubtractDirection(const float2* ja, const float2
 
 // correlate: grid (bl tiles x direction groups, segments, channel blocks),
 // block = waves x 64, lane = baseline. Every workgroup forms the full
-// residual over all directions and accumulates X, Y, D_a, D_b of its group's
-// (up to NG) directions, combined across waves through LDS in a fixed order.
+// residual over all directions and accumulates X and Y of its group's (up to
+// NG) directions, combined across waves through LDS in a fixed order.
 template <int NG, bool F16>
 __global__ void __launch_bounds__(kCorrelateMaxThreads) FullJonesCorrelateKernel(const IterArgs a) {
   if (AsConst(a.state)->done) return;
@@ -1314,7 +1318,7 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads) FullJonesCorrelateKernel
 
   extern __shared__ float smem[];
   float2* g_s = reinterpret_cast<float2*>(smem);  // [n_dir][64][8]: J_a, then J_b
-  float* red_s = smem + size_t(a.n_dir) * kWave * 16;  // [24 * NG][64]
+  float* red_s = smem + size_t(a.n_dir) * kWave * 16;  // [16 * NG][64]
 
   // Stage the repaired, float-cast gains of every direction (SolveData.h
   // casts the solutions to float); waves split the directions.
@@ -1341,14 +1345,10 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads) FullJonesCorrelateKernel
   __syncthreads();
 
   float2 x[NG][4], y[NG][4];
-  float da[NG][4], db[NG][4];
 #pragma unroll
   for (int j = 0; j < NG; ++j)
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      x[j][k] = y[j][k] = make_float2(0.0f, 0.0f);
-      da[j][k] = db[j][k] = 0.0f;
-    }
+    for (int k = 0; k < 4; ++k) x[j][k] = y[j][k] = make_float2(0.0f, 0.0f);
   float sc[NG];
 #pragma unroll
   for (int j = 0; j < NG; ++j)
@@ -1418,33 +1418,12 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads) FullJonesCorrelateKernel
           CMacC(y[j][2 * i + k], r[i], av[k]);
           CMacC(y[j][2 * i + k], r[2 + i], av[2 + k]);
         }
-      // D_a += B^H B, D_b += A^H A (00, 11, 01)
-      da[j][0] += (bv[0].x * bv[0].x + bv[0].y * bv[0].y) + (bv[2].x * bv[2].x + bv[2].y * bv[2].y);
-      da[j][1] += (bv[1].x * bv[1].x + bv[1].y * bv[1].y) + (bv[3].x * bv[3].x + bv[3].y * bv[3].y);
-      {
-        float2 t = make_float2(da[j][2], da[j][3]);
-        CMacC(t, bv[0], bv[1]);
-        CMacC(t, bv[2], bv[3]);
-        da[j][2] = t.x;
-        da[j][3] = t.y;
-      }
-      db[j][0] += (av[0].x * av[0].x + av[0].y * av[0].y) + (av[2].x * av[2].x + av[2].y * av[2].y);
-      db[j][1] += (av[1].x * av[1].x + av[1].y * av[1].y) + (av[3].x * av[3].x + av[3].y * av[3].y);
-      {
-        float2 t = make_float2(db[j][2], db[j][3]);
-        CMacC(t, av[0], av[1]);
-        CMacC(t, av[2], av[3]);
-        db[j][2] = t.x;
-        db[j][3] = t.y;
-      }
     }
   }
 
   auto term = [&](int j, int v) -> float& {
     if (v < 8) return (v & 1) ? x[j][v >> 1].y : x[j][v >> 1].x;
-    if (v < 16) return (v & 1) ? y[j][(v - 8) >> 1].y : y[j][(v - 8) >> 1].x;
-    if (v < 20) return da[j][v - 16];
-    return db[j][v - 20];
+    return (v & 1) ? y[j][(v - 8) >> 1].y : y[j][(v - 8) >> 1].x;
   };
   // Combine the waves: wave w-1 += wave w, for w = n_waves-1 .. 1.
   for (int w = n_waves - 1; w >= 1; --w) {
@@ -1475,6 +1454,107 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads) FullJonesCorrelateKernel
       for (int v = 0; v < kFullJonesTerms; ++v) out[size_t(v) * a.bl_stride + bl] = term(j, v);
     }
   }
+}
+
+// Model products, once per solve: Q[p][q] = sum over a segment's rows of
+// M_p conj(M_q) per (block, segment, direction, baseline), Hermitian in
+// (p, q): the four real diagonal terms, then (p < q) in the order (0,1),
+// (0,2), (0,3), (1,2), (1,3), (2,3) as (re, im). Grid (bl tiles, segments,
+// blocks x directions), lane = baseline; waves split the rows and are
+// combined through LDS in a fixed order.
+template <bool F16>
+__global__ void __launch_bounds__(kCorrelateMaxThreads)
+    FullJonesModelProductsKernel(const IterArgs a) {
+  const int tile = blockIdx.x;
+  const int seg = blockIdx.y;
+  const int cb = blockIdx.z / a.n_dir;
+  const int d = blockIdx.z - cb * a.n_dir;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int n_waves = blockDim.x / kWave;
+  const CbInfo info = LoadConst(a.cbinfo + cb);
+  const size_t plane = size_t(a.n_t) * info.n_ch * a.bl_stride;
+  const int bl = tile * kWave + lane;
+  const float sc = F16 ? AsConst(a.model_scale)[cb * a.n_dir + d] : 1.0f;
+  __shared__ float red_s[kFullJonesModelTerms * kWave];
+  float q[kFullJonesModelTerms];
+#pragma unroll
+  for (int v = 0; v < kFullJonesModelTerms; ++v) q[v] = 0.0f;
+  const int r0 = AsConst(a.seg_t0)[seg] * info.n_ch;
+  const int r1 = AsConst(a.seg_t0)[seg + 1] * info.n_ch;
+  for (int row = r0 + wave; row < r1; row += n_waves) {
+    float2 m[4];
+    LoadFull<F16>(a.model, info.model_off + d * plane + size_t(row) * a.bl_stride + bl, sc, m);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) q[p] += m[p].x * m[p].x + m[p].y * m[p].y;
+    int v = 4;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int r = p + 1; r < 4; ++r) {
+        // M_p conj(M_r)
+        q[v] += m[p].x * m[r].x + m[p].y * m[r].y;
+        q[v + 1] += m[p].y * m[r].x - m[p].x * m[r].y;
+        v += 2;
+      }
+  }
+  for (int w = n_waves - 1; w >= 1; --w) {
+    if (wave == w) {
+#pragma unroll
+      for (int v = 0; v < kFullJonesModelTerms; ++v) red_s[v * kWave + lane] = q[v];
+    }
+    __syncthreads();
+    if (wave == w - 1) {
+#pragma unroll
+      for (int v = 0; v < kFullJonesModelTerms; ++v) q[v] += red_s[v * kWave + lane];
+    }
+    __syncthreads();
+  }
+  if (wave == 0) {
+    float* out = a.fj_products +
+                 ((size_t(cb) * a.n_seg + seg) * a.n_dir + d) * kFullJonesModelTerms * a.bl_stride;
+#pragma unroll
+    for (int v = 0; v < kFullJonesModelTerms; ++v) out[size_t(v) * a.bl_stride + bl] = q[v];
+  }
+}
+
+// Q[p][q] of the model products of one baseline (FP64).
+struct ModelProducts {
+  double v[kFullJonesModelTerms];
+  __device__ __forceinline__ double2 at(int p, int q) const {
+    if (p == q) return make_double2(v[p], 0.0);
+    const int lo = p < q ? p : q, hi = p < q ? q : p;
+    // offset of pair (lo, hi) among (0,1), (0,2), (0,3), (1,2), (1,3), (2,3)
+    const int idx = 4 + 2 * (lo == 0 ? hi - 1 : lo == 1 ? hi + 1 : 5);
+    return p < q ? make_double2(v[idx], v[idx + 1]) : make_double2(v[idx], -v[idx + 1]);
+  }
+};
+
+// Adds the denominator of one baseline to sd (00, 11, re 01, im 01):
+// side 0 (this antenna first):  D_ij = sum_kl H_kl Q[2i+k][2j+l]
+// side 1 (this antenna second): D_ij = sum_kl H_kl Q[2l+j][2k+i]
+// with H = J^H J of the other antenna's (float-cast) gain.
+__device__ __forceinline__ void AddDenominator(const ModelProducts& q, const double2 (&h)[4],
+                                               int side, double (&sd)[4]) {
+  double2 dv[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  const int ij[3][2] = {{0, 0}, {1, 1}, {0, 1}};
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const int i = ij[e][0], j = ij[e][1];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int l = 0; l < 2; ++l) {
+        const double2 qq = side == 0 ? q.at(2 * i + k, 2 * j + l) : q.at(2 * l + j, 2 * k + i);
+        const double2 t = cmul(h[2 * k + l], qq);
+        dv[e].x += t.x;
+        dv[e].y += t.y;
+      }
+  }
+  sd[0] += dv[0].x;
+  sd[1] += dv[1].x;
+  sd[2] += dv[2].x;
+  sd[3] += dv[2].y;
 }
 
 // antenna solve: grid (n_dir, n_cb); `split` threads per (antenna, solution
@@ -1511,18 +1591,33 @@ __device__ __forceinline__ void FullJonesSolveBlock(const IterArgs& a, const int
   double sd[4] = {0.0, 0.0, 0.0, 0.0};
   for (int seg = 0; seg < a.n_seg; ++seg) {
     if (!active || AsConst(a.seg_sol)[seg * a.n_dir + d] != s_abs) continue;
-    const GlobalT<float>* src = AsGlobal(a.partials) +
-                                ((size_t(cb) * a.n_seg + seg) * a.n_dir + d) * kFullJonesTerms * bs;
+    const size_t off = (size_t(cb) * a.n_seg + seg) * a.n_dir + d;
+    const GlobalT<float>* src = AsGlobal(a.partials) + off * kFullJonesTerms * bs;
+    const GlobalT<float>* qsrc = AsGlobal(a.fj_products) + off * kFullJonesModelTerms * bs;
     for (int e = e_begin + q; e < e_end; e += split) {
       const int code = csr[2 * e];
+      const int other = csr[2 * e + 1];
       const int b = code >> 1;
       const int side = code & 1;
       const GlobalT<float>* xs = src + (side ? 8 : 0) * bs + b;
-      const GlobalT<float>* ds = src + (side ? 20 : 16) * bs + b;
 #pragma unroll
       for (int k = 0; k < 8; ++k) sx[k] += double(xs[k * bs]);
+      ModelProducts mp;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) sd[k] += double(ds[k * bs]);
+      for (int k = 0; k < kFullJonesModelTerms; ++k) mp.v[k] = double(qsrc[k * bs + b]);
+      // H = J^H J of the other antenna's gain as the correlate used it
+      double2 jo[4], h[4];
+      FixedMatrix(cur, gbase + size_t(other) * a.n_sub + s_abs, mean, jo);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) jo[k] = make_double2(double(float(jo[k].x)), double(float(jo[k].y)));
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int l = 0; l < 2; ++l) {
+          const double2 t0 = cmulc(jo[l], jo[k]), t1 = cmulc(jo[2 + l], jo[2 + k]);
+          h[2 * k + l] = make_double2(t0.x + t1.x, t0.y + t1.y);  // sum_m conj(J_mk) J_ml
+        }
+      AddDenominator(mp, h, side, sd);
     }
   }
   for (int m = 1; m < split; m <<= 1) {
@@ -1918,9 +2013,13 @@ namespace {
 using FullJonesKernelFn = void (*)(IterArgs);
 const FullJonesKernelFn kFullJonesCorrelate[2][kFullJonesGroup] = {
     {FullJonesCorrelateKernel<1, false>, FullJonesCorrelateKernel<2, false>,
-     FullJonesCorrelateKernel<3, false>, FullJonesCorrelateKernel<4, false>},
+     FullJonesCorrelateKernel<3, false>, FullJonesCorrelateKernel<4, false>,
+     FullJonesCorrelateKernel<5, false>, FullJonesCorrelateKernel<6, false>,
+     FullJonesCorrelateKernel<7, false>, FullJonesCorrelateKernel<8, false>},
     {FullJonesCorrelateKernel<1, true>, FullJonesCorrelateKernel<2, true>,
-     FullJonesCorrelateKernel<3, true>, FullJonesCorrelateKernel<4, true>}};
+     FullJonesCorrelateKernel<3, true>, FullJonesCorrelateKernel<4, true>,
+     FullJonesCorrelateKernel<5, true>, FullJonesCorrelateKernel<6, true>,
+     FullJonesCorrelateKernel<7, true>, FullJonesCorrelateKernel<8, true>}};
 
 int g_max_lds = 64 * 1024;
 int g_max_lds_solve = 160 * 1024;
@@ -2068,6 +2167,17 @@ void LaunchFullJonesCorrelate(const IterArgs& a, bool f16, int waves, hipStream_
   const size_t lds = FullJonesCorrelateSharedBytes(a.n_dir);
   DDH_REQUIRE(lds <= MaxSharedBytes(), -4, "full-Jones correlate needs more LDS than the device has");
   hipLaunchKernelGGL(kFullJonesCorrelate[f16][ng - 1], grid, dim3(waves * kWave), lds, stream, a);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchFullJonesModelProducts(const IterArgs& a, bool f16, int waves, hipStream_t stream) {
+  DDH_REQUIRE(waves >= 1 && waves * kWave <= kCorrelateMaxThreads, -1, "bad waves per tile");
+  const dim3 grid(a.bl_stride / kWave, a.n_seg, a.n_cb * a.n_dir);
+  if (f16)
+    hipLaunchKernelGGL(FullJonesModelProductsKernel<true>, grid, dim3(waves * kWave), 0, stream, a);
+  else
+    hipLaunchKernelGGL(FullJonesModelProductsKernel<false>, grid, dim3(waves * kWave), 0, stream,
+                       a);
   DDH_HIP_CHECK(hipGetLastError());
 }
 

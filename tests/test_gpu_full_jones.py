@@ -105,13 +105,13 @@ def test_full_jones_matches_oracle(small4, storage):
 
 
 def test_full_jones_direction_groups():
-    """Six directions: the correlate accumulates them in two groups of at most
-    four (each group forms the residual over all six)."""
+    """Ten directions: the correlate accumulates them in two groups of at most
+    eight (each group forms the residual over all ten)."""
     import dataclasses
-    w = dataclasses.replace(S.SMALL, name="24st-6dir", n_directions=6,
-                            n_solutions_per_direction=(1, 2, 1, 4, 1, 2), seed=5)
+    w = dataclasses.replace(S.SMALL, name="24st-10dir", n_directions=10,
+                            n_solutions_per_direction=(1, 2, 1, 4, 1, 2, 1, 1, 2, 1), seed=5)
     w_, geom, sky, gains, P = small_problem(w, n_corr=4)
-    print("case: full-Jones 6 directions", flush=True)
+    print("case: full-Jones 10 directions", flush=True)
     compare(P, max_iterations=w.max_iterations, accuracy=w.tolerance)
 
 
