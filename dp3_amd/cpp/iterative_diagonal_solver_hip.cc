@@ -185,4 +185,10 @@ SolverBase::SolveResult IterativeSolverHip::Solve(
   return SolveImpl(data, solutions, 4);
 }
 
+SolverBase::SolveResult IterativeSolverHip::Solve(
+    const UniSolveData& data, std::vector<std::vector<DComplex>>& solutions, double time) {
+  if (n_solution_polarizations_ != 1) return SolverBase::Solve(data, solutions, time);
+  return SolveImpl(data, solutions, 1);
+}
+
 }  // namespace dp3amd

@@ -44,6 +44,10 @@ class IterativeSolverHip : public SolverBase {
                     double time) override;
   SolveResult Solve(const FullSolveData& data, std::vector<std::vector<DComplex>>& solutions,
                     double time) override;
+  // Single-correlation data: the scalar solver only (as the reference, which
+  // instantiates IterativeScalarSolver<std::complex<float>> alone).
+  SolveResult Solve(const UniSolveData& data, std::vector<std::vector<DComplex>>& solutions,
+                    double time) override;
 
   // Whether the last Solve met its accuracy (the reference keeps this inside
   // MakeResult).

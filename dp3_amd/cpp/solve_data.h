@@ -22,6 +22,9 @@
 
 namespace dp3amd {
 
+struct UniMatrix {  // std::complex<float>: 0.5 (XX + YY) (SolveData.cc:29-31)
+  std::complex<float> values[1];
+};
 struct DuoMatrix {  // aocommon::MC2x2FDiag: (XX, YY)
   std::complex<float> values[2];
 };
@@ -90,6 +93,7 @@ class SolveData {
   std::vector<ChannelBlockData> channel_blocks_;
 };
 
+using UniSolveData = SolveData<UniMatrix>;
 using DuoSolveData = SolveData<DuoMatrix>;
 using FullSolveData = SolveData<FullMatrix>;
 

@@ -80,6 +80,11 @@ class SolverBase {
         "Duo-visibility (xx/yy) solver called for a solver that does not support "
         "duo-visibility solving");
   }
+  virtual SolveResult Solve(const UniSolveData&, std::vector<std::vector<DComplex>>&, double) {
+    throw std::logic_error(
+        "Single-visibility (xx/yy) solver called for a solver that does not support "
+        "single-visibility solving");
+  }
 
  private:
   size_t n_antennas_ = 0;

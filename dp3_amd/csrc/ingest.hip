@@ -63,12 +63,19 @@ __device__ __forceinline__ uint32_t HalfBits(float re, float im) {
 }
 
 // Visibility x (correlations 0..3) weighted by w (per correlation) and
-// zeroed when flagged.
-__device__ __forceinline__ void Weigh(const float4_v (&x)[2], float4_v w, bool flagged,
+// zeroed when flagged. For single-correlation problems (uni) correlations 0
+// and 3 then both hold 0.5f * (XX + YY) (SolveData.cc:29-31), which the
+// problem keeps as a Duo pair.
+__device__ __forceinline__ void Weigh(const float4_v (&x)[2], float4_v w, bool flagged, bool uni,
                                       float4_v (&out)[2]) {
   const float4_v z = {0.0f, 0.0f, 0.0f, 0.0f};
   out[0] = flagged ? z : x[0] * float4_v{w.x, w.x, w.y, w.y};
   out[1] = flagged ? z : x[1] * float4_v{w.z, w.z, w.w, w.w};
+  if (uni) {
+    const float re = 0.5f * (out[0].x + out[1].z), im = 0.5f * (out[0].y + out[1].w);
+    out[0].x = out[1].z = re;
+    out[0].y = out[1].w = im;
+  }
 }
 
 // Largest finite |re|, |im| over the correlations a plane keeps.
@@ -214,7 +221,7 @@ __global__ void __launch_bounds__(kIngestThreads) AssignWeighKernel(IngestArgs g
 #pragma unroll
         for (int d = 0; d < NDB; ++d) {
           if (d < g.n_dir) {
-            Weigh(m[d], w, flagged, o);
+            Weigh(m[d], w, flagged, g.uni != 0, o);
             mx[d] = MaxAbs<NC>(o, mx[d]);
           }
         }
@@ -225,7 +232,7 @@ __global__ void __launch_bounds__(kIngestThreads) AssignWeighKernel(IngestArgs g
       const int wbl = bl0 + w_bl;
       const int w_cross = (wbl < g.n_bl_all && w_ch_ok) ? g.cross[wbl] : -1;
       const size_t out = (size_t(t) * w_nch + (w_ch - w_c0)) * g.bl_stride + w_cross;
-      Weigh(x, w, flagged, o);
+      Weigh(x, w, flagged, g.uni != 0, o);
       s_data[s_idx] = Pack<NC, false>(o, 1.0f);
       __syncthreads();
       if (w_cross >= 0)
@@ -234,7 +241,7 @@ __global__ void __launch_bounds__(kIngestThreads) AssignWeighKernel(IngestArgs g
       for (int d = 0; d < NDB; ++d) {
         if (d < g.n_dir) {
           const float inv = F16 ? 1.0f / g.scale[r_cb * g.n_dir + d] : 1.0f;
-          Weigh(m[d], w, flagged, o);
+          Weigh(m[d], w, flagged, g.uni != 0, o);
           s_model[d & 1][s_idx] = Pack<NC, F16>(o, inv);
           __syncthreads();
           if (w_cross >= 0)

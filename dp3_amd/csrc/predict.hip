@@ -427,6 +427,8 @@ extern "C" int ddh_predict(ddh_problem* problem, double ref_ra, double ref_dec,
     DDH_REQUIRE(!correct_freq_smearing || chan_width, -1, "frequency smearing needs widths");
     ProblemView v = GetProblemView(problem);
     DDH_REQUIRE(n_stations >= v.n_ant, -1, "station count smaller than the antenna count");
+    DDH_REQUIRE(!v.uni, -4, "the predict writes XX and YY; single-correlation problems take "
+                            "their models through upload or ingest");
     std::vector<ddh_component> comps(components, components + n_components);
     // group components by direction (stable)
     std::stable_sort(comps.begin(), comps.end(),

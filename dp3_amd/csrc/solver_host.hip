@@ -68,6 +68,7 @@ struct SolverConfig {
 struct ddh_problem {
   // structure
   int n_t = 0, n_bl = 0, bl_stride = 0, n_cb = 0, n_dir = 0, n_corr = 0, n_ant = 0;
+  int in_corr = 0;  // correlations per visibility of host input (1: single, kept as Duo)
   int n_seg = 0;
   bool f16 = false;
   std::vector<int32_t> ant1, ant2, ch_per_cb, seg_t0, seg_sol;
@@ -136,11 +137,17 @@ struct ddh_solver {
 namespace ddh {
 namespace {
 
+// Correlations of the device planes: single-correlation data (one value,
+// 0.5 (XX + YY), per visibility) is kept as Duo pairs of equal values, which a
+// scalar solve treats exactly as the single value (both terms of its sums are
+// equal, so they double exactly and the quotient is unchanged).
+int DeviceCorrelations(const ddh_config& c) { return c.n_correlations == 1 ? 2 : c.n_correlations; }
+
 void ValidateConfig(const ddh_config& c) {
   DDH_REQUIRE(c.iteration == DDH_ITERATION_STREAM || c.iteration == DDH_ITERATION_GRAM, -1,
               "unknown iteration kind");
   DDH_REQUIRE(c.iteration != DDH_ITERATION_GRAM ||
-                  (c.n_correlations == 2 && c.n_directions <= kMaxGramDirections),
+                  (DeviceCorrelations(c) == 2 && c.n_directions <= kMaxGramDirections),
               -4, "the Gram iteration is built for Duo data with at most 8 directions");
   DDH_REQUIRE(c.iteration != DDH_ITERATION_GRAM ||
                   ApplyGramSharedBytes(c.n_antennas, c.n_directions) <= kApplyGramMaxShared,
@@ -149,8 +156,10 @@ void ValidateConfig(const ddh_config& c) {
   DDH_REQUIRE(c.n_directions >= 1 && c.n_directions <= 16, -4,
               "n_directions must be 1..16");
   DDH_REQUIRE(c.n_channel_blocks >= 1, -1, "n_channel_blocks must be >= 1");
-  DDH_REQUIRE(c.n_correlations == 2 || c.n_correlations == 4, -1,
-              "n_correlations must be 2 (Duo) or 4 (Full)");
+  DDH_REQUIRE(c.n_correlations == 1 || c.n_correlations == 2 || c.n_correlations == 4, -1,
+              "n_correlations must be 1 (single), 2 (Duo) or 4 (Full)");
+  DDH_REQUIRE(c.n_correlations != 1 || c.n_solution_polarizations == 1, -1,
+              "single-correlation data needs the scalar solver (n_solution_polarizations 1)");
   DDH_REQUIRE(c.n_solution_polarizations == 1 || c.n_solution_polarizations == 2 ||
                   c.n_solution_polarizations == 4,
               -1, "n_solution_polarizations must be 1 (scalar), 2 (diagonal) or 4 (full-Jones)");
@@ -208,7 +217,8 @@ ddh_problem* CreateProblem(const ddh_solver& s, int n_times, int n_baselines,
   p->bl_stride = (n_baselines + kWave - 1) / kWave * kWave;
   p->n_cb = c.n_channel_blocks;
   p->n_dir = c.n_directions;
-  p->n_corr = c.n_correlations;
+  p->n_corr = DeviceCorrelations(c);
+  p->in_corr = c.n_correlations;
   p->n_ant = c.n_antennas;
   p->f16 = c.model_storage == DDH_MODEL_F16;
   p->stream = s.stream;
@@ -284,14 +294,14 @@ ddh_problem* CreateProblem(const ddh_solver& s, int n_times, int n_baselines,
 void UploadBlock(ddh_problem& p, int cb, const float* data, const float* model) {
   const int n_ch = p.ch_per_cb[cb];
   const size_t n_vis = size_t(p.n_t) * p.n_bl * n_ch;
-  const size_t vis_floats = size_t(p.n_corr) * 2;
+  const size_t vis_floats = size_t(p.in_corr) * 2;  // host input
   const size_t need = std::max(n_vis, n_vis * p.n_dir) * vis_floats;
   p.staging.Resize(need);
   // data
   DDH_HIP_CHECK(hipMemcpyAsync(p.staging.ptr, data, n_vis * vis_floats * sizeof(float),
                                hipMemcpyHostToDevice, p.stream));
   LaunchRepack(p.staging.ptr, p.data.ptr + p.cbinfo[cb].data_off * p.DataVisBytes(), 1, p.n_t,
-               p.n_bl, n_ch, p.bl_stride, p.n_corr, false, nullptr, p.stream);
+               p.n_bl, n_ch, p.bl_stride, p.n_corr, false, nullptr, p.stream, p.in_corr == 1);
   // model
   DDH_HIP_CHECK(hipMemcpyAsync(p.staging.ptr, model,
                                n_vis * p.n_dir * vis_floats * sizeof(float),
@@ -306,7 +316,8 @@ void UploadBlock(ddh_problem& p, int cb, const float* data, const float* model) 
     LaunchScaleFromMax(p.max_bits.ptr, scale, p.n_dir, p.stream);
   }
   LaunchRepack(p.staging.ptr, p.model.ptr + p.cbinfo[cb].model_off * p.ModelVisBytes(),
-               p.n_dir, p.n_t, p.n_bl, n_ch, p.bl_stride, p.n_corr, p.f16, scale, p.stream);
+               p.n_dir, p.n_t, p.n_bl, n_ch, p.bl_stride, p.n_corr, p.f16, scale, p.stream,
+               p.in_corr == 1);
   // The staging buffer is reused by the next block.
   DDH_HIP_CHECK(hipStreamSynchronize(p.stream));
 }
@@ -503,7 +514,7 @@ void EnsureGraph(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
 void CheckMatch(const ddh_solver& s, const ddh_problem& p) {
   const ddh_config& c = s.cfg.c;
   DDH_REQUIRE(p.n_cb == c.n_channel_blocks && p.n_dir == c.n_directions &&
-                  p.n_corr == c.n_correlations && p.n_ant == c.n_antennas,
+                  p.in_corr == c.n_correlations && p.n_ant == c.n_antennas,
               -1, "problem does not match the solver configuration");
   DDH_REQUIRE(p.f16 == (c.model_storage == DDH_MODEL_F16), -1,
               "problem model storage does not match the solver");
@@ -722,6 +733,7 @@ void AssignAndWeighInto(ddh_problem& p, int n_bl_all, const int32_t* a1, const i
   g.n_dir = p.n_dir;
   g.bl_stride = p.bl_stride;
   g.linear = linear ? 1 : 0;
+  g.uni = p.in_corr == 1 ? 1 : 0;
   LaunchAssignAndWeigh(g, p.n_corr, p.f16, stream);
 }
 
@@ -739,6 +751,7 @@ ProblemView GetProblemView(ddh_problem* p) {
   v.n_cb = p->n_cb;
   v.n_dir = p->n_dir;
   v.n_corr = p->n_corr;
+  v.uni = p->in_corr == 1;
   v.n_ant = p->n_ant;
   v.f16 = p->f16;
   v.ch_per_cb = p->ch_per_cb;
@@ -1030,7 +1043,8 @@ int ddh_solver_set_smoothness(ddh_solver* s, const double* channel_block_frequen
     const ddh_config& c = s->cfg.c;
     const int n_cb = c.n_channel_blocks, n_ant = c.n_antennas, n_sub = s->cfg.n_sub;
     DDH_REQUIRE(bandwidth_hz > 0.0, -1, "the smoothing bandwidth must be > 0");
-    DDH_REQUIRE(c.n_correlations == 2 || c.n_correlations == 4, -1, "bad correlations");
+    DDH_REQUIRE(c.n_correlations == 1 || c.n_correlations == 2 || c.n_correlations == 4, -1,
+                "bad correlations");
     std::vector<double> f(channel_block_frequencies, channel_block_frequencies + n_cb);
     for (int i = 1; i < n_cb; ++i)
       DDH_REQUIRE(f[i] >= f[i - 1], -1, "channel-block frequencies must ascend");
@@ -1788,6 +1802,8 @@ int ddh_chi2(ddh_solver* s, ddh_problem* p, double* const* solutions, double* ch
     DDH_HIP_CHECK(hipMemcpyAsync(chi2, s->chi2.ptr + n_partial, sizeof(double),
                                  hipMemcpyDeviceToHost, s->stream));
     DDH_HIP_CHECK(hipStreamSynchronize(s->stream));
+    // single-correlation problems hold each value in both correlations
+    if (c.n_correlations == 1) *chi2 *= 0.5;
   });
 }
 

@@ -209,10 +209,12 @@ void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_ite
                      const SmoothLaunch& smooth, hipStream_t stream);
 
 // Layout conversion: reference order [plane][t][bl][ch][n_corr] complex64 ->
-// device order [plane][t][ch][bl_stride] (complex64 or FP16/scale).
+// device order [plane][t][ch][bl_stride] (complex64 or FP16/scale). With
+// `uni` the source holds one complex value per visibility and the Duo
+// (n_corr 2) destination gets it in both correlations.
 void LaunchRepack(const float* src, void* dst, int n_planes, int n_t, int n_bl,
                   int n_ch, int bl_stride, int n_corr, bool f16,
-                  const float* plane_scale, hipStream_t stream);
+                  const float* plane_scale, hipStream_t stream, bool uni = false);
 // Per-plane max |re|,|im| of reference-order planes (FP16 scale), written as
 // float bits into max_bits[n_planes] (which must be zeroed first).
 void LaunchPlaneMaxAbs(const float* src, int64_t plane_floats, int n_planes,
@@ -236,6 +238,7 @@ struct IngestArgs {
   const float* scale;      // [n_cb][n_dir] FP16 scale (written by the first pass)
   uint32_t* max_bits;      // [n_cb][n_dir] workspace of the FP16 maxima
   int32_t n_t, n_bl_all, n_ch, n_cb, n_dir, bl_stride, linear;
+  int32_t uni;             // single-correlation problem: both Duo slots 0.5 (XX + YY)
 };
 void LaunchAssignAndWeigh(const IngestArgs& g, int n_corr, bool f16, hipStream_t stream);
 
@@ -250,6 +253,7 @@ struct ProblemView {
   const CbInfo* cbinfo;
   int n_t, n_bl, bl_stride, n_cb, n_dir, n_corr, n_ant;
   bool f16;
+  bool uni;  // single-correlation problem kept as Duo pairs of equal values
   std::vector<int32_t> ch_per_cb;
   std::vector<CbInfo> cb_host;
   hipStream_t stream;

@@ -1802,7 +1802,7 @@ __global__ void SnapshotKernel(const IterArgs* __restrict__ args, int interval, 
 // ---------------------------------------------------------------------------
 // Layout conversion for host-provided SolveData.
 // ---------------------------------------------------------------------------
-template <int NC, bool F16>
+template <int NC, bool F16, bool UNI = false>
 __global__ void RepackKernel(const float* __restrict__ src, void* __restrict__ dst,
                              int n_planes, int n_t, int n_bl, int n_ch, int bl_stride,
                              const float* plane_scale) {
@@ -1816,7 +1816,12 @@ __global__ void RepackKernel(const float* __restrict__ src, void* __restrict__ d
     const int t = int(rest % n_t);
     const int plane = int(rest / n_t);
     float v[2 * NC];
-    if (bl < n_bl) {
+    if (bl < n_bl && UNI) {
+      // one value per visibility, into both correlations of the pair
+      const float* s = src + (((int64_t(plane) * n_t + t) * n_bl + bl) * n_ch + ch) * 2;
+      v[0] = v[2] = s[0];
+      v[1] = v[3] = s[1];
+    } else if (bl < n_bl) {
       const float* s = src + ((((int64_t(plane) * n_t + t) * n_bl + bl) * n_ch + ch) * NC) * 2;
 #pragma unroll
       for (int i = 0; i < 2 * NC; ++i) v[i] = s[i];
@@ -2290,13 +2295,21 @@ void LaunchPrologue(const IterArgs& a, hipStream_t stream) {
 
 void LaunchRepack(const float* src, void* dst, int n_planes, int n_t, int n_bl, int n_ch,
                   int bl_stride, int n_corr, bool f16, const float* plane_scale,
-                  hipStream_t stream) {
+                  hipStream_t stream, bool uni) {
   const int64_t total = int64_t(n_planes) * n_t * n_ch * bl_stride;
   if (total == 0) return;
   const int threads = 256;
   const int64_t blocks64 = std::min<int64_t>((total + threads - 1) / threads, 65536);
   const dim3 grid(static_cast<unsigned>(blocks64));
-  if (n_corr == 2) {
+  DDH_REQUIRE(!uni || n_corr == 2, -1, "single-correlation planes are kept as Duo pairs");
+  if (uni) {
+    if (f16)
+      hipLaunchKernelGGL((RepackKernel<2, true, true>), grid, dim3(threads), 0, stream, src, dst,
+                         n_planes, n_t, n_bl, n_ch, bl_stride, plane_scale);
+    else
+      hipLaunchKernelGGL((RepackKernel<2, false, true>), grid, dim3(threads), 0, stream, src,
+                         dst, n_planes, n_t, n_bl, n_ch, bl_stride, plane_scale);
+  } else if (n_corr == 2) {
     if (f16)
       hipLaunchKernelGGL((RepackKernel<2, true>), grid, dim3(threads), 0, stream, src, dst,
                          n_planes, n_t, n_bl, n_ch, bl_stride, plane_scale);

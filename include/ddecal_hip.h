@@ -56,7 +56,9 @@ typedef struct {
   int32_t n_antennas;
   int32_t n_directions;
   int32_t n_channel_blocks;
-  int32_t n_correlations;                    // 2 (Duo, XX/YY) or 4 (Full)
+  int32_t n_correlations;                    // 2 (Duo, XX/YY), 4 (Full) or 1 (single:
+                                             // 0.5 (XX + YY), SolveData.cc:29-31; the
+                                             // scalar solver only)
   const int32_t* n_solutions_per_direction;  // [n_directions]
   int32_t max_iterations;                    // SolverBase default 100
   int32_t min_iterations;                    // default 0

@@ -18,6 +18,22 @@ using CD = std::complex<double>;
 
 // Corr selection of ToSpecificMatrix (SolveData.cc:20-27): Duo takes XX, YY.
 inline int CorrIndex(int n_corr, int k) { return n_corr == 2 ? 3 * k : k; }
+
+// One packed visibility (ToSpecificMatrix, SolveData.cc:19-31): the selected
+// correlations, or for single-correlation data 0.5f * (XX + YY) in float.
+inline void PackVisibility(const float* in, int n_corr, float* out) {
+  if (n_corr == 1) {
+    const CF v = 0.5f * (CF(in[0], in[1]) + CF(in[6], in[7]));
+    out[0] = v.real();
+    out[1] = v.imag();
+    return;
+  }
+  for (int k = 0; k != n_corr; ++k) {
+    const size_t c = CorrIndex(n_corr, k);
+    out[2 * k] = in[c * 2];
+    out[2 * k + 1] = in[c * 2 + 1];
+  }
+}
 }  // namespace
 
 extern "C" int orc_pack_solve_data(
@@ -27,7 +43,7 @@ extern "C" int orc_pack_solve_data(
     const float* model, int64_t* vis_offsets, float* out_data,
     float* out_model, uint32_t* out_ant1, uint32_t* out_ant2,
     uint32_t* out_sol_map) {
-  if (n_corr != 2 && n_corr != 4) return -2;
+  if (n_corr != 1 && n_corr != 2 && n_corr != 4) return -2;
   // Cross-correlation count (SolveData.cc:95-100).
   size_t n_cross = 0;
   for (int32_t bl = 0; bl != n_bl; ++bl) {
@@ -67,12 +83,7 @@ extern "C" int orc_pack_solve_data(
         const size_t cb_size = ch_begin[cb + 1] - first;
         for (size_t i = 0; i != cb_size; ++i) {
           const size_t src = ((size_t(t) * n_bl + bl) * n_ch + first + i) * 4;
-          for (int k = 0; k != n_corr; ++k) {
-            const size_t c = CorrIndex(n_corr, k);
-            out_data[((v0 + vi + i) * n_corr + k) * 2] = data[(src + c) * 2];
-            out_data[((v0 + vi + i) * n_corr + k) * 2 + 1] =
-                data[(src + c) * 2 + 1];
-          }
+          PackVisibility(&data[src * 2], n_corr, &out_data[(v0 + vi + i) * n_corr * 2]);
           out_ant1[v0 + vi + i] = a1;
           out_ant2[v0 + vi + i] = a2;
         }
@@ -82,11 +93,7 @@ extern "C" int orc_pack_solve_data(
             const size_t src =
                 (((size_t(d) * n_t + t) * n_bl + bl) * n_ch + first + i) * 4;
             const size_t dst = v0 * n_dir + size_t(d) * nv + vi + i;
-            for (int k = 0; k != n_corr; ++k) {
-              const size_t c = CorrIndex(n_corr, k);
-              out_model[(dst * n_corr + k) * 2] = model[(src + c) * 2];
-              out_model[(dst * n_corr + k) * 2 + 1] = model[(src + c) * 2 + 1];
-            }
+            PackVisibility(&model[src * 2], n_corr, &out_model[dst * n_corr * 2]);
             out_sol_map[dst] = si;
           }
         }

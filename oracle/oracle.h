@@ -93,7 +93,8 @@ int orc_solve(const orc_settings* settings, const int64_t* vis_offsets,
               orc_result* result, double* step_magnitudes);
 // IterativeScalarSolver (IterativeScalarSolver.cc:30-226): one solution
 // polarization, solutions [n_cb][n_ant][n_sub] complex128; with a smoothness
-// constraint when `smoothness` is not NULL.
+// constraint when `smoothness` is not NULL. n_correlations 1 (single, the
+// std::complex<float> instantiation), 2 or 4.
 int orc_solve_scalar(const orc_settings* settings, const orc_smoothness* smoothness,
                      const int64_t* vis_offsets, const float* data, const float* model,
                      const uint32_t* ant1, const uint32_t* ant2, const uint32_t* sol_map,
@@ -138,6 +139,7 @@ double orc_chi2_full_jones(int32_t n_ant, int32_t n_dir, int32_t n_cb, int32_t n
 // Inputs per time step t: data [n_t][n_bl][n_ch][4] complex64 and
 // model [n_dir][n_t][n_bl][n_ch][4]. Outputs use the orc_solve layout.
 // vis_offsets must hold n_cb+1 entries; call with data==NULL to get offsets.
+// n_corr 4 (Full), 2 (Duo: XX, YY) or 1 (single: 0.5 (XX + YY)).
 int orc_pack_solve_data(int32_t n_t, int32_t n_bl, int32_t n_ch, int32_t n_dir,
                         int32_t n_cb, int32_t n_corr, int32_t n_ant,
                         const int32_t* n_solutions_per_direction,

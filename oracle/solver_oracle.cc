@@ -38,6 +38,11 @@ namespace {
 using DComplex = std::complex<double>;
 
 template <typename T>
+struct Uni {
+  static constexpr int kCorr = 1;
+  std::complex<T> v[1];
+};
+template <typename T>
 struct Duo {
   static constexpr int kCorr = 2;
   std::complex<T> v[2];
@@ -54,6 +59,14 @@ struct Full {
 //   (SolveData.h:230-237; row/column scaling as in the CUDA helper
 //   kernels/MatrixComplex2x2.h:104-108).
 // ---------------------------------------------------------------------------
+// model * solution_1 * solution_2_conj (IterativeScalarSolver.cc:218-219)
+template <typename T>
+inline Uni<T> Contribution(const std::complex<T> a[2], const Uni<T>& m,
+                           const std::complex<T> b[2]) {
+  Uni<T> r;
+  r.v[0] = (m.v[0] * a[0]) * std::conj(b[0]);
+  return r;
+}
 template <typename T>
 inline Duo<T> Contribution(const std::complex<T> a[2], const Duo<T>& m,
                            const std::complex<T> b[2]) {
@@ -178,6 +191,18 @@ inline void AccumulateAnt2(const Full<T>& r, const Full<T>& m,
 // with C1 = M^H g_b and C2 = M g_a, antenna 1 adds Trace(R C1) and Norm(C1),
 // antenna 2 adds Trace(R^H C2) and Norm(C2); the products are float, the
 // sums double.
+// Single correlation: Trace and Norm of 1x1 matrices (IterativeScalarSolver.cc:20-27).
+template <typename T>
+inline void AccumulateScalar(const Uni<T>& r, const Uni<T>& m, std::complex<T> g1,
+                             std::complex<T> g2, DComplex& num1, double& den1, DComplex& num2,
+                             double& den2) {
+  const std::complex<T> c1 = std::conj(m.v[0]) * g2;
+  const std::complex<T> c2 = m.v[0] * g1;
+  num1 += DComplex(r.v[0] * c1);
+  den1 += double(std::norm(c1));
+  num2 += DComplex(std::conj(r.v[0]) * c2);
+  den2 += double(std::norm(c2));
+}
 template <typename T>
 inline void AccumulateScalar(const Duo<T>& r, const Duo<T>& m, std::complex<T> g1,
                              std::complex<T> g2, DComplex& num1, double& den1, DComplex& num2,
@@ -448,6 +473,12 @@ class IterativeDiagonalOracle {
         return;
       }
     }
+    if constexpr (V::kCorr >= 2) SolveDirectionDiagonal(cb, residual, d, solutions, next);
+  }
+
+  // IterativeDiagonalSolver.cc:130-222
+  void SolveDirectionDiagonal(const ChannelBlock<V>& cb, const std::vector<V>& residual,
+                              size_t d, const DComplex* solutions, DComplex* next) const {
     const size_t n_dir_sol = n_sol_dir_[d];
     const uint32_t sol0 = cb.n_vis ? cb.sol_map[d * cb.n_vis] : sol_start_[d];
     const size_t n_parts =
@@ -881,6 +912,11 @@ extern "C" int orc_solve_scalar(const orc_settings* s, const orc_smoothness* c,
                                 double* solutions, orc_result* result, double* step_mags) {
   if (!s || s->n_directions <= 0 || s->n_channel_blocks <= 0 || s->n_antennas <= 0) return -1;
   if (c && (!c->frequencies || !(c->bandwidth_hz > 0.0))) return -1;
+  if (s->n_correlations == 1)
+    return s->fp64 ? SolveTyped<Uni<double>>(s, vo, data, model, a1, a2, sm, solutions, result,
+                                             step_mags, c, 1)
+                   : SolveTyped<Uni<float>>(s, vo, data, model, a1, a2, sm, solutions, result,
+                                            step_mags, c, 1);
   if (s->n_correlations == 2)
     return s->fp64 ? SolveTyped<Duo<double>>(s, vo, data, model, a1, a2, sm, solutions, result,
                                              step_mags, c, 1)

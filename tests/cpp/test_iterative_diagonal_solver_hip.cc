@@ -25,6 +25,7 @@ using dp3amd::IterativeDiagonalSolverHip;
 using dp3amd::IterativeFullJonesSolverHip;
 using dp3amd::IterativeScalarSolverHip;
 using dp3amd::SolverBase;
+using dp3amd::UniSolveData;
 using DComplex = std::complex<double>;
 
 namespace {
@@ -461,6 +462,7 @@ int main() {
   const std::vector<std::pair<const char*, std::function<void()>>> cases = {
       {"iterative_duo_diagonal_dd_intervals", [] { DdIntervals<DuoSolveData>(2); }},
       {"iterative_diagonal_dd_intervals", [] { DdIntervals<FullSolveData>(4); }},
+      {"iterative_uni_scalar_dd_intervals", [] { ScalarDdIntervals<UniSolveData>(1); }},
       {"iterative_duo_scalar_dd_intervals", [] { ScalarDdIntervals<DuoSolveData>(2); }},
       {"iterative_scalar_dd_intervals", [] { ScalarDdIntervals<FullSolveData>(4); }},
       {"iterative_full_jones_dd_intervals", FullJonesDdIntervals},

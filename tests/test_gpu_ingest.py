@@ -167,6 +167,50 @@ def test_assign_and_weight_direction_counts(n_dir, storage):
     s.close()
 
 
+@pytest.mark.parametrize("storage", [H.MODEL_C64, H.MODEL_F16])
+def test_assign_and_weight_single_correlation(storage):
+    """Single-correlation problems (scalar solver, n_correlations 1): both Duo
+    slots of every plane hold 0.5 (XX + YY) of the weighted visibility
+    (SolveData.cc:29-31), bit for bit as the oracle's one-correlation packing
+    (complex64) and as the upload of that packing (FP16)."""
+    print(f"case: ingest single correlation storage={storage}", flush=True)
+    n_ch, n_cb = 11, 3
+    data, model, flags, weights, a1, a2 = raw_interval(5)
+    n_t = data.shape[0]
+    n_ant = int(max(a1.max(), a2.max())) + 1
+    cross = a1 != a2
+
+    def problem():
+        s = H.Solver(n_ant, list(N_SOL), n_cb, n_correlations=1, n_solution_polarizations=1,
+                     model_storage=storage)
+        return s, H.DeviceProblem(s, n_t, a1[cross], a2[cross], channel_blocks(n_ch, n_cb),
+                                  sol_index(n_t))
+
+    s, p = problem()
+    p.assign_and_weight(data, model, a1, a2, flags=flags, weights=weights)
+    ref = oracle_packed(data, model, flags, weights, a1, a2, n_cb, 1)
+    f16 = storage == H.MODEL_F16
+    if f16:
+        s2, q = problem()
+        q.upload(ref)
+    n_bl = p.n_baselines
+    for cb, ch in enumerate(p.channels_per_block):
+        for plane in range(model.shape[0] + 1):
+            got = p.download_plane(cb, plane, n_corr=2, f16=f16 and plane > 0)
+            if f16 and plane > 0:
+                want = q.download_plane(cb, plane, n_corr=2, f16=True)
+                np.testing.assert_array_equal(got.view(np.uint16), want.view(np.uint16))
+            else:
+                one = device_layout(ref, cb, plane, n_t, n_bl, int(ch), p.bl_stride, 1)
+                np.testing.assert_array_equal(got, np.repeat(one, 2, axis=2),
+                                              err_msg=f"block {cb} plane {plane}")
+    if f16:
+        q.close()
+        s2.close()
+    p.close()
+    s.close()
+
+
 def test_assign_and_weight_defaults_and_linear_mode():
     """No flags and unit weights; linear weighting multiplies by w itself
     (SolverTools.cc:131-136)."""

@@ -42,8 +42,11 @@ def gpu_scalar(P, **kw):
                     n_correlations=P.n_correlations, n_solution_polarizations=1, **kw)
 
 
-@pytest.mark.parametrize("n_corr", [2, 4])
+@pytest.mark.parametrize("n_corr", [1, 2, 4])
 def test_scalar_solvertester_dd_intervals(n_corr):
+    """n_corr 1: single-correlation data 0.5 (XX + YY), the reference's
+    iterative_uni_scalar_dd_intervals (tSolvers.cc:171-185); kept on the
+    device as Duo pairs of equal values."""
     print(f"case: scalar solvertester n_corr={n_corr}", flush=True)
     P, inp = solvertester_problem((1, 2, 5), n_corr=n_corr, scalar=True)
     s = gpu_scalar(P, max_iterations=100, accuracy=1e-8, step_size=0.2)
@@ -140,3 +143,28 @@ def test_scalar_rejects_bad_polarizations(small):
     with pytest.raises(H.HipError, match="n_solution_polarizations"):
         H.Solver(P.n_antennas, P.n_solutions_per_direction, P.n_channel_blocks,
                  n_solution_polarizations=3)
+
+
+def test_scalar_single_correlation_chi2_and_device_problem():
+    """A single-correlation problem uploaded to the device: the solve equals
+    the drop-in solve, and chi^2 counts each visibility once (as the oracle's
+    one-correlation chi^2)."""
+    P, _ = solvertester_problem((1, 2, 5), n_corr=1, scalar=True)
+    print("case: scalar single correlation device problem", flush=True)
+    s = gpu_scalar(P, max_iterations=20, accuracy=0.0, detect_stalling=False)
+    a = scalar_units(P)
+    s.solve(P, a)
+    dp = H.DeviceProblem.from_problem(s, P)
+    dp.upload(P)
+    b = scalar_units(P)
+    s.solve_problem(dp, b)
+    np.testing.assert_array_equal(a, b)
+    got = H.chi2(s, dp, b)
+    want = O.chi2(P, as_diagonal(b))
+    print(f"chi2 gpu {got:.10g} oracle {want:.10g}")
+    assert abs(got - want) <= 1e-9 * want
+    with pytest.raises(H.HipError, match="scalar"):
+        H.Solver(P.n_antennas, P.n_solutions_per_direction, P.n_channel_blocks,
+                 n_correlations=1, n_solution_polarizations=2)
+    dp.close()
+    s.close()
