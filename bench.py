@@ -265,6 +265,9 @@ def main():
     n_vis = w.n_baselines * w.n_channels * w.solint
     bl_stride = -(-w.n_baselines // 64) * 64
     partial_bytes = w.n_channel_blocks * w.n_directions * 2 * 3 * bl_stride * 4
+    # the apply writes U (2 of the 3 partials per correlation); S is written
+    # once per solve by the cross-products pass
+    apply_partial_bytes = w.n_channel_blocks * w.n_directions * 2 * 2 * bl_stride * 4
     solution_bytes = w.n_channel_blocks * w.n_stations * w.n_sub * 2 * 16
     gram_bytes = w.n_channel_blocks * 2 * (w.n_directions + w.n_directions *
                                            (w.n_directions + 1) // 2) * bl_stride * 8
@@ -281,8 +284,9 @@ def main():
     g_apply, g_solve = opt["gram"]["kernels_ms"]
     s_corr, s_solve = opt["streaming"]["kernels_ms"]
     rooflines = {
-        "gram_apply": roof("gram_apply", B * (gram_bytes + partial_bytes), g_apply,
-                           note="per-baseline cross products read, correlate partials written"),
+        "gram_apply": roof("gram_apply", B * (gram_bytes + apply_partial_bytes), g_apply,
+                           note="per-baseline cross products read, correlate partials U written "
+                                "(S, constant over the solve, comes from the cross-products pass)"),
         "antenna_solve": roof("antenna_solve", B * (partial_bytes + 2 * solution_bytes), g_solve,
                               note="partials read, solutions read and written; with the "
                                    "decision kernel; bound by reduction latency, not HBM"),

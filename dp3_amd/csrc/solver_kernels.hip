@@ -460,6 +460,15 @@ __device__ __forceinline__ void GramTile(const IterArgs& a, const int tile, cons
         out[2 * size_t(k * NQ + q) * a.bl_stride + 0] = acc[q][k][0];
         out[2 * size_t(k * NQ + q) * a.bl_stride + 1] = acc[q][k][1];
       }
+    // S_d,k = G_d,d,k does not change over the solve: its slot of the
+    // correlate partials is written here, once, and the apply writes U only.
+    GlobalT<float>* part =
+        AsGlobal(a.partials) + (size_t(cb) * a.n_seg + seg) * ND * NC * 3 * a.bl_stride + bl;
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+      for (int k = 0; k < NC; ++k)
+        part[(size_t(d * NC + k) * 3 + 2) * a.bl_stride] = acc[GramPair<ND>(d, d)][k][0];
   }
 }
 
@@ -528,7 +537,7 @@ __device__ __forceinline__ void ApplyGramTile(const IterArgs& a, const int tile,
     const size_t v = size_t(d * NC + k) * 3;
     out[(v + 0) * a.bl_stride] = u.x;
     out[(v + 1) * a.bl_stride] = u.y;
-    out[(v + 2) * a.bl_stride] = q[GramPair<ND>(d, d)].x;
+    // S (slot v + 2) was written by the cross-products pass
   }
 }
 
