@@ -560,10 +560,23 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads)
   GramTile<ND, F16>(a, t.tile, t.seg, t.cb);
 }
 
+// Workgroups go to the 8 XCDs round-robin (workgroup w to XCD w % 8), and
+// each XCD has its own L2. Workgroup w takes tile-list entry
+// (w % 8) * per + w / 8 (per = n / 8), so every XCD works through one
+// contiguous run of the list, in which consecutive tiles of an interval's
+// block share that block's gains in its L2; the n % 8 tail keeps its order.
+// A permutation of the list: results do not depend on it.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int XcdContiguous(int w, int n) {
+  const int per = n / kXcds;
+  if (w >= per * kXcds) return w;
+  return (w % kXcds) * per + w / kXcds;
+}
+
 template <int ND, bool F16>
 __global__ void __launch_bounds__(kApplyThreads)
     BatchApplyGramKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ tiles) {
-  const BatchTile t = LoadConst(tiles + blockIdx.x);
+  const BatchTile t = LoadConst(tiles + XcdContiguous(blockIdx.x, gridDim.x));
   const IterArgs a = LoadConst(args + t.interval);
   if (AsConst(a.state)->done) return;
   ApplyGramTile<ND, F16>(a, t.tile, t.seg, t.cb);
