@@ -168,3 +168,22 @@ def test_scalar_single_correlation_chi2_and_device_problem():
                  n_correlations=1, n_solution_polarizations=2)
     dp.close()
     s.close()
+
+
+def test_scalar_phase_only(small):
+    """The scalarphase mode: the scalar solver with SetPhaseOnly(true)
+    (SolverFactory.cc:297-303), a unit-modulus step along the circle
+    (SolverBase.cc:128-155)."""
+    w, geom, sky, gains, P = small
+    print("case: scalar phase only", flush=True)
+    s = gpu_scalar(P, max_iterations=30, accuracy=w.tolerance, phase_only=True)
+    sol = scalar_units(P)
+    r = s.solve(P, sol)
+    s.close()
+    ref = scalar_units(P)
+    rr = O.solve(P, ref, max_iterations=30, accuracy=w.tolerance, phase_only=True, scalar=True)
+    err = scalar_relerr(sol, ref, P.n_antennas, P.n_sub)
+    print(f"iterations gpu {r.iterations} oracle {rr.iterations}, gain err {err:.2e}")
+    assert r.iterations == rr.iterations
+    np.testing.assert_allclose(np.abs(sol), 1.0, atol=1e-12)
+    assert err <= GAIN_TOL
