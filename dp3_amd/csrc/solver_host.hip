@@ -151,7 +151,8 @@ void ValidateConfig(const ddh_config& c) {
               -4, "the Gram iteration is built for Duo data with at most 8 directions");
   DDH_REQUIRE(c.iteration != DDH_ITERATION_GRAM ||
                   ApplyGramSharedBytes(c.n_antennas, c.n_directions) <= kApplyGramMaxShared,
-              -4, "the Gram iteration stages at most 4096 antenna x direction gains");
+              -4, "the Gram iteration stages every antenna's gains in 64 KB of LDS "
+                  "(n_antennas x (2 n_directions + 1) x 8 bytes)");
   DDH_REQUIRE(c.n_antennas > 1, -1, "n_antennas must be > 1");
   DDH_REQUIRE(c.n_directions >= 1 && c.n_directions <= 16, -4,
               "n_directions must be 1..16");

@@ -481,7 +481,12 @@ __device__ __forceinline__ void ApplyGramTile(const IterArgs& a, const int tile,
   constexpr int NC = 2;
   constexpr int NQ = ND + ND * (ND + 1) / 2;
   typedef float f2v __attribute__((ext_vector_type(2)));
-  extern __shared__ float2 s_gain[];  // [n_ant][ND][NC], repaired, cast to float
+  // [n_ant][ND * NC + 1], repaired, cast to float. Lanes of a tile read
+  // antennas a2 = a1 + 1, a1 + 2, ...: the one-value pad per antenna makes the
+  // row stride 2 * ND * NC + 2 dwords (an odd multiple of 2), so the 64-bit
+  // reads of 32 consecutive antennas fall in distinct bank pairs.
+  extern __shared__ float2 s_gain[];
+  constexpr int kGainRow = ND * NC + 1;
   const int lane = threadIdx.x & (kWave - 1);
   const int k = int(threadIdx.x >> 6);  // kApplyThreads = NC waves
   const int bl = tile * kWave + lane;
@@ -506,8 +511,8 @@ __device__ __forceinline__ void ApplyGramTile(const IterArgs& a, const int tile,
       const size_t p = gbase + size_t(an) * a.n_sub + AsConst(a.seg_sol)[seg * ND + d];
       double2 g[2] = {LoadGlobal(sol + p * 2), LoadGlobal(sol + p * 2 + 1)};
       if (!(cfinite(g[0]) && cfinite(g[1]))) g[0] = g[1] = make_double2(mean, 0.0);
-      s_gain[e * NC + 0] = make_float2(float(g[0].x), float(g[0].y));
-      s_gain[e * NC + 1] = make_float2(float(g[1].x), float(g[1].y));
+      s_gain[an * kGainRow + d * NC + 0] = make_float2(float(g[0].x), float(g[0].y));
+      s_gain[an * kGainRow + d * NC + 1] = make_float2(float(g[1].x), float(g[1].y));
     }
   }
   __syncthreads();
@@ -517,7 +522,7 @@ __device__ __forceinline__ void ApplyGramTile(const IterArgs& a, const int tile,
   for (int d = 0; d < ND; ++d) {
     float sc = 1.0f;
     if constexpr (F16) sc = AsConst(a.model_scale)[cb * ND + d];
-    const float2 g = cmulc(s_gain[(a1 * ND + d) * NC + k], s_gain[(a2 * ND + d) * NC + k]);
+    const float2 g = cmulc(s_gain[a1 * kGainRow + d * NC + k], s_gain[a2 * kGainRow + d * NC + k]);
     gg[d] = valid ? make_float2(g.x * sc, g.y * sc) : make_float2(0.0f, 0.0f);
   }
   GlobalT<float>* out =
@@ -1990,7 +1995,7 @@ void LaunchGram(const IterArgs& a, bool f16, int waves, hipStream_t stream) {
 }
 
 size_t ApplyGramSharedBytes(int n_ant, int n_dir) {
-  return size_t(n_ant) * n_dir * 2 * sizeof(float2);
+  return size_t(n_ant) * (n_dir * 2 + 1) * sizeof(float2);  // padded rows
 }
 
 void LaunchApplyGram(const IterArgs& a, bool f16, hipStream_t stream) {
