@@ -766,7 +766,7 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
   // entry holds (baseline << 1 | side, other antenna).
   extern __shared__ double smem_d[];
   double2* g_s = reinterpret_cast<double2*>(smem_d);             // [n_ant][nsd][2]
-  float* part_s = reinterpret_cast<float*>(g_s + size_t(a.n_ant) * nsd * 2);  // [NC][3][bs]
+  float* part_s = reinterpret_cast<float*>(g_s + size_t(a.n_ant) * nsd * 2);  // [3][bs], one k
   for (int i = threadIdx.x; i < a.n_ant * nsd; i += blockDim.x) {
     const int an = i / nsd;
     const size_t pair = gbase + size_t(an) * a.n_sub + s0 + (i - an * nsd);
@@ -788,43 +788,45 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
   __syncthreads();
   const double2 gant[2] = {active ? gain(ant, 0) : make_double2(0, 0),
                            active ? gain(ant, 1) : make_double2(0, 0)};
+  // The partials are staged one correlation at a time (3 x bl_stride floats),
+  // which keeps the LDS per workgroup small enough for six workgroups per CU.
   for (int seg = 0; seg < a.n_seg; ++seg) {
     const int s_seg = AsConst(a.seg_sol)[seg * a.n_dir + d];
     const float* src =
         a.partials + ((size_t(cb) * a.n_seg + seg) * a.n_dir + d) * NC * 3 * size_t(bs);
-    if (seg > 0) __syncthreads();
-    StageWords(src, part_s, NC * 3 * bs);
-    __syncthreads();
-    if (!active || s_seg != s_abs) continue;
-    for (int e = e_begin + q; e < e_end; e += split) {
-      const int code = csr[2 * e];
-      const int other = csr[2 * e + 1];
-      const int b = code >> 1;
-      const int side = code & 1;
-      const double2 go[2] = {gain(other, 0), gain(other, 1)};
 #pragma unroll
-      for (int k = 0; k < NC; ++k) {
-        const int i = PolI<NC>(k);
-        const int j = PolJ<NC>(k);
-        const double2 u = make_double2(sc * part_s[(k * 3 + 0) * bs + b],
-                                       sc * part_s[(k * 3 + 1) * bs + b]);
-        const double s = sc2 * part_s[(k * 3 + 2) * bs + b];
+    for (int k = 0; k < NC; ++k) {
+      if (seg > 0 || k > 0) __syncthreads();  // the previous stage has been read
+      StageWords(src + size_t(k) * 3 * bs, part_s, 3 * bs);
+      __syncthreads();
+      if (!active || s_seg != s_abs) continue;
+      const int i = PolI<NC>(k);
+      const int j = PolJ<NC>(k);
+      for (int e = e_begin + q; e < e_end; e += split) {
+        const int code = csr[2 * e];
+        const int other = csr[2 * e + 1];
+        const int b = code >> 1;
+        const int side = code & 1;
+        const double2 u = make_double2(sc * part_s[b], sc * part_s[bs + b]);
+        const double s = sc2 * part_s[2 * bs + b];
         if (side == 0) {
           // this antenna is a (first); the other is b
-          const double2 gg = cmulc(gant[i], go[j]);
+          const double2 go = gain(other, j);
+          const double2 gg = cmulc(gant[i], go);
           const double2 t = make_double2(u.x + gg.x * s, u.y + gg.y * s);
-          const double2 c = cmul(go[j], t);
+          const double2 c = cmul(go, t);
           nr[i] += c.x;
           ni[i] += c.y;
-          dn[i] += cnorm(go[j]) * s;
+          dn[i] += cnorm(go) * s;
         } else {
           // this antenna is b (second); the other is a
-          const double2 gg = cmulc(go[i], gant[j]);
+          const double2 go = gain(other, i);
+          const double2 gg = cmulc(go, gant[j]);
           const double2 t = make_double2(u.x + gg.x * s, u.y + gg.y * s);
-          const double2 c = cmulc(go[i], t);  // G_a,i * conj(T)
+          const double2 c = cmulc(go, t);  // G_a,i * conj(T)
           nr[j] += c.x;
           ni[j] += c.y;
-          dn[j] += cnorm(go[i]) * s;
+          dn[j] += cnorm(go) * s;
         }
       }
     }
@@ -2230,9 +2232,9 @@ void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, int row
 }
 
 size_t AntennaSolveSharedBytes(const IterArgs& a, int n_corr) {
+  (void)n_corr;  // one correlation's partials are staged at a time
   const int max_nsd = std::max(1, a.max_nsd);
-  return size_t(a.n_ant) * max_nsd * 2 * sizeof(double2) +
-         size_t(n_corr) * 3 * a.bl_stride * sizeof(float);
+  return size_t(a.n_ant) * max_nsd * 2 * sizeof(double2) + size_t(3) * a.bl_stride * sizeof(float);
 }
 
 int AntennaSolveThreads(int n_ant, int max_nsd) {
