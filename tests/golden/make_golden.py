@@ -4,7 +4,7 @@ a restatement of DP3's IterativeDiagonalSolver, SolveData packing and
 Simulator predict; the reference itself cannot be built here).
 
   golden_small_duo.npz / golden_small_full.npz
-      A reduced config[0]-style interval (8 stations, 2 directions, one point
+      A reduced interval of the 24-station parity workload's kind (8 stations, 2 directions, one point
       and one Gaussian component each, 8 channels in 2 blocks, solution
       interval 2, step 0.2, at most 200 iterations, tolerance 1e-5, seed 0;
       it converges in 170):

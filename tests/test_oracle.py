@@ -1,5 +1,5 @@
 """Pins the CPU oracle against the reference's own known answers and against
-what the mathematics fixes, so that the oracle can judge the HIP path.
+what the mathematics fixes, so that the oracle can check the HIP path.
 
   * predict:  tSimulator.cc known answers (|V| = 1, frequency smearing
               0.759154, time smearing 0.893578 / 0.891990, radec2lmn closed
