@@ -1142,6 +1142,7 @@ struct ddh_batch {
   int n_corr = 2;
   bool f16 = false;
   bool gram = false;  // Gram iteration
+  bool full_jones = false;  // full-Jones intervals (all or none)
   int waves = 2, unroll = 1, use_graph = 1, poll = 8;
   int limit = 0;  // largest iteration limit over the intervals
   size_t solve_lds = 0;
@@ -1150,8 +1151,9 @@ struct ddh_batch {
   std::vector<ddh::TileGroup> groups;  // tiles of one direction count each
   ddh::SmoothLaunch smooth{};          // smoothness constraint launches (none: 0 intervals)
   std::vector<ddh::BatchTile> tiles, items, pro_items;
+  std::vector<ddh::BatchTile> prod_tiles;  // full-Jones model products: plain tiles
   ddh::DeviceBuffer<ddh::IterArgs> d_args;
-  ddh::DeviceBuffer<ddh::BatchTile> d_tiles, d_items, d_pro_items;
+  ddh::DeviceBuffer<ddh::BatchTile> d_tiles, d_items, d_pro_items, d_prod_tiles;
   ddh::DeviceBuffer<ddh::DeviceState> d_states;
   ddh::DeviceState* host_states = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -1184,6 +1186,7 @@ void BuildBatch(ddh_batch& b) {
   b.tiles.clear();
   b.items.clear();
   b.pro_items.clear();
+  b.prod_tiles.clear();
   b.d_states.Resize(n);
   b.limit = 0;
   b.solve_lds = 0;
@@ -1209,8 +1212,15 @@ void BuildBatch(ddh_batch& b) {
     for (int cb = 0; cb < p.n_cb; ++cb) {
       for (int seg = 0; seg < p.n_seg; ++seg) {
         const double rows = double(p.seg_t0[seg + 1] - p.seg_t0[seg]) * p.ch_per_cb[cb];
-        for (int tile = 0; tile < p.bl_stride / kWave; ++tile)
-          costed.push_back({BatchTile{i, cb, seg, tile}, p.n_dir, rows * vis_bytes});
+        const int n_tiles = p.bl_stride / kWave;
+        // full-Jones: one entry per (tile, direction group), tile + group * n_tiles
+        const int n_groups = b.full_jones ? (p.n_dir + kFullJonesGroupSize - 1) / kFullJonesGroupSize
+                                          : 1;
+        for (int tile = 0; tile < n_tiles; ++tile) {
+          if (b.full_jones) b.prod_tiles.push_back(BatchTile{i, cb, seg, tile});
+          for (int g = 0; g < n_groups; ++g)
+            costed.push_back({BatchTile{i, cb, seg, tile + g * n_tiles}, p.n_dir, rows * vis_bytes});
+        }
       }
       for (int d = 0; d < p.n_dir; ++d) b.items.push_back(BatchTile{i, cb, d, 0});
       b.pro_items.push_back(BatchTile{i, cb, 0, 0});
@@ -1242,6 +1252,7 @@ void BuildBatch(ddh_batch& b) {
   b.d_tiles.Upload(b.tiles, b.stream);
   b.d_items.Upload(b.items, b.stream);
   b.d_pro_items.Upload(b.pro_items, b.stream);
+  if (b.full_jones) b.d_prod_tiles.Upload(b.prod_tiles, b.stream);
   DDH_HIP_CHECK(hipStreamSynchronize(b.stream));
   DestroyBatchGraph(b);
 }
@@ -1250,7 +1261,11 @@ void BuildBatch(ddh_batch& b) {
 // them): the correlate (Gram iteration: apply) of every tile group, then the
 // antenna solve over n_items items and the decision for n_intervals intervals.
 void LaunchBatchStep(ddh_batch& b, const IterArgs* args, int n_items, int n_intervals) {
-  if (b.gram) {
+  if (b.full_jones) {
+    LaunchBatchFullJones(args, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()),
+                         b.d_items.ptr, n_items, n_intervals, b.solve_threads, b.f16, b.waves,
+                         b.smooth, b.stream);
+  } else if (b.gram) {
     int apply_ant = 0;
     for (const IterArgs& a : b.args) apply_ant = std::max(apply_ant, a.n_ant);
     LaunchBatchApply(args, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), apply_ant,
@@ -1266,6 +1281,9 @@ void LaunchBatchStep(ddh_batch& b, const IterArgs* args, int n_items, int n_inte
 // Gram iteration: the cross products of every interval (shared by all
 // argument variants, which point at the same buffers).
 void LaunchBatchCrossProducts(ddh_batch& b) {
+  if (b.full_jones)
+    LaunchBatchFullJonesModelProducts(b.d_args.ptr, b.d_prod_tiles.ptr, int(b.prod_tiles.size()),
+                                      b.f16, b.waves, b.stream);
   if (b.gram)
     LaunchBatchGram(b.d_args.ptr, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), b.f16,
                     b.waves, b.stream);
@@ -1295,7 +1313,7 @@ void EnsureBatchGraph(ddh_batch& b) {
 // Complex solution values of interval i over all its channel blocks.
 size_t SolutionCount(const ddh_batch& b, int i) {
   const ddh_solver& s = *b.solvers[i];
-  return size_t(s.cfg.c.n_channel_blocks) * s.cfg.c.n_antennas * s.cfg.n_sub * 2;
+  return size_t(s.cfg.c.n_channel_blocks) * s.cfg.c.n_antennas * s.cfg.n_sub * DevicePols(s.cfg.c);
 }
 
 // Host solutions[i][cb] -> dst(i) on the batch stream.
@@ -1304,9 +1322,10 @@ void UploadSolutions(ddh_batch& b, double* const* const* solutions, Dst&& dst) {
   const int n = int(b.solvers.size());
   for (int i = 0; i < n; ++i) {
     const ddh_solver& s = *b.solvers[i];
-    const size_t per_cb = size_t(s.cfg.c.n_antennas) * s.cfg.n_sub * 2;
+    const size_t n_sols = size_t(s.cfg.c.n_antennas) * s.cfg.n_sub;
+    const size_t per_cb = n_sols * DevicePols(s.cfg.c);
     for (int cb = 0; cb < s.cfg.c.n_channel_blocks; ++cb)
-      SolutionsIn(s, dst(i) + cb * per_cb, solutions[i][cb], per_cb / 2, b.stream);
+      SolutionsIn(s, dst(i) + cb * per_cb, solutions[i][cb], n_sols, b.stream);
   }
 }
 
@@ -1332,6 +1351,7 @@ void SolveBatch(ddh_batch& b, double* const* const* solutions, ddh_result* resul
 // ddh_mlp_forward and ddh_warmstart_apply.
 void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* const* solutions,
                     ddh_result* results) {
+  DDH_REQUIRE(!b.full_jones, -4, "the learned warm start serves the scalar and diagonal solvers");
   CheckWarmNetwork(net);
   const int n = int(b.solvers.size());
   std::vector<size_t> off(n + 1, 0);
@@ -1443,8 +1463,14 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
           DDH_HIP_CHECK(hipEventRecord(b.events[3 * it], b.stream));
           LaunchBatchStep(b, b.d_args.ptr, 0, 0);
           DDH_HIP_CHECK(hipEventRecord(b.events[3 * it + 1], b.stream));
-          LaunchBatchTail(b.d_args.ptr, b.d_items.ptr, int(b.items.size()), int(b.solvers.size()),
-                          b.solve_lds, b.solve_threads, b.n_corr, b.f16, b.smooth, b.stream);
+          if (b.full_jones)
+            LaunchBatchFullJones(b.d_args.ptr, b.d_tiles.ptr, b.groups.data(), 0, b.d_items.ptr,
+                                 int(b.items.size()), int(b.solvers.size()), b.solve_threads, b.f16,
+                                 b.waves, b.smooth, b.stream);
+          else
+            LaunchBatchTail(b.d_args.ptr, b.d_items.ptr, int(b.items.size()),
+                            int(b.solvers.size()), b.solve_lds, b.solve_threads, b.n_corr, b.f16,
+                            b.smooth, b.stream);
           DDH_HIP_CHECK(hipEventRecord(b.events[3 * it + 2], b.stream));
         } else {
           LaunchBatchIteration(b);
@@ -1464,10 +1490,11 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
   for (int i = 0; i < n; ++i) {
     const ddh_solver& s = *b.solvers[i];
     const DeviceState st = b.host_states[i];
-    const size_t per_cb = size_t(s.cfg.c.n_antennas) * s.cfg.n_sub * 2;
+    const size_t n_sols = size_t(s.cfg.c.n_antennas) * s.cfg.n_sub;
+    const size_t per_cb = n_sols * DevicePols(s.cfg.c);
     const double2* fin = (st.iteration & 1) ? b.args[i].sol1 : b.args[i].sol0;
     for (int cb = 0; cb < s.cfg.c.n_channel_blocks; ++cb)
-      SolutionsOut(s, solutions[i][cb], fin + cb * per_cb, per_cb / 2, b.stream);
+      SolutionsOut(s, solutions[i][cb], fin + cb * per_cb, n_sols, b.stream);
     if (results) {
       results[i].iterations = st.has_converged ? st.iteration : st.iteration + 1;
       results[i].constraint_iterations = 0;
@@ -1545,13 +1572,15 @@ ddh_batch* ddh_batch_create(ddh_solver* const* solvers, ddh_problem* const* prob
     b->n_corr = b->problems[0]->n_corr;
     b->f16 = b->problems[0]->f16;
     b->gram = GramMode(*b->solvers[0]);
+    b->full_jones = b->solvers[0]->cfg.c.n_solution_polarizations == 4;
     for (int i = 0; i < n_intervals; ++i) {
       DDH_REQUIRE(b->problems[i]->n_corr == b->n_corr && b->problems[i]->f16 == b->f16, -4,
                   "all intervals of a batch share correlations and model storage");
       DDH_REQUIRE(GramMode(*b->solvers[i]) == b->gram, -4,
                   "all intervals of a batch share the iteration kind");
-      DDH_REQUIRE(b->solvers[i]->cfg.c.n_solution_polarizations != 4, -4,
-                  "batches take scalar and diagonal solvers; solve full-Jones intervals singly");
+      DDH_REQUIRE((b->solvers[i]->cfg.c.n_solution_polarizations == 4) ==
+                      (b->solvers[0]->cfg.c.n_solution_polarizations == 4),
+                  -4, "the intervals of a batch are all full-Jones or none");
     }
     DDH_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     DDH_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b->host_states),

@@ -122,6 +122,9 @@ struct IterArgs {
 // (Hermitian: 4 real diagonal terms and 6 complex upper terms).
 constexpr int kFullJonesTerms = 16;
 constexpr int kFullJonesModelTerms = 16;
+// Directions whose X and Y one full-Jones correlate workgroup accumulates;
+// more directions take several workgroups (direction groups) per tile.
+constexpr int kFullJonesGroupSize = 8;
 
 // Largest direction count of the Gram iteration (its per-lane accumulators
 // grow with n_dir^2).
@@ -207,6 +210,20 @@ void LaunchBatchPrologue(const IterArgs* d_args, const BatchTile* d_items, int n
 void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_items,
                      int n_intervals, size_t solve_lds, int solve_threads, int n_corr, bool f16,
                      const SmoothLaunch& smooth, hipStream_t stream);
+// The smoothing passes (when any interval has the constraint) and the decision.
+void LaunchBatchSmoothDecide(const IterArgs* d_args, const BatchTile* d_items, int n_items,
+                             int n_intervals, const SmoothLaunch& smooth, hipStream_t stream);
+// One batched full-Jones iteration: the correlate per direction-count group
+// (tile entries hold tile + group * bl_stride / 64), the antenna solve over
+// n_items (interval, block, direction) items, smoothing and the decision.
+void LaunchBatchFullJones(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
+                          int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
+                          int solve_threads, bool f16, int waves, const SmoothLaunch& smooth,
+                          hipStream_t stream);
+// The full-Jones model products of every interval, once per solve: one
+// workgroup per (interval, block, segment, tile) entry.
+void LaunchBatchFullJonesModelProducts(const IterArgs* d_args, const BatchTile* d_tiles,
+                                       int n_tiles, bool f16, int waves, hipStream_t stream);
 
 // Layout conversion: reference order [plane][t][bl][ch][n_corr] complex64 ->
 // device order [plane][t][ch][bl_stride] (complex64 or FP16/scale). With

@@ -1283,7 +1283,7 @@ __global__ void SumPartialsKernel(const double* partial, int n, double* out) {
 // antenna's baselines in FP64 and forms J + X D^-1 = numerator *
 // denominator^-1 (:178-185).
 // ---------------------------------------------------------------------------
-constexpr int kFullJonesGroup = 8;  // directions accumulated per workgroup
+constexpr int kFullJonesGroup = kFullJonesGroupSize;  // directions accumulated per workgroup
 
 // The four correlations of visibility `idx` of a Full plane, FP16 planes
 // scaled back.
@@ -1329,13 +1329,9 @@ __device__ __forceinline__ void SubtractDirection(const float2* ja, const float2
 // residual over all directions and accumulates X and Y of its group's (up to
 // NG) directions, combined across waves through LDS in a fixed order.
 template <int NG, bool F16>
-__global__ void __launch_bounds__(kCorrelateMaxThreads) FullJonesCorrelateKernel(const IterArgs a) {
-  if (AsConst(a.state)->done) return;
-  const int n_tiles = a.bl_stride / kWave;
-  const int group = blockIdx.x / n_tiles;
-  const int tile = blockIdx.x - group * n_tiles;
-  const int seg = blockIdx.y;
-  const int cb = blockIdx.z;
+__device__ __forceinline__ void FullJonesCorrelateTile(const IterArgs& a, const int tile,
+                                                       const int group, const int seg,
+                                                       const int cb) {
   const int g0 = group * NG;
   const int ng = min(NG, a.n_dir - g0);
   const int lane = threadIdx.x & (kWave - 1);
@@ -1485,6 +1481,29 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads) FullJonesCorrelateKernel
   }
 }
 
+// Single solve: grid (bl tiles x direction groups, segments, blocks).
+template <int NG, bool F16>
+__global__ void __launch_bounds__(kCorrelateMaxThreads) FullJonesCorrelateKernel(const IterArgs a) {
+  if (AsConst(a.state)->done) return;
+  const int n_tiles = a.bl_stride / kWave;
+  const int group = blockIdx.x / n_tiles;
+  FullJonesCorrelateTile<NG, F16>(a, blockIdx.x - group * n_tiles, group, blockIdx.y, blockIdx.z);
+}
+
+// Batch: one workgroup per tile-list entry, whose `tile` field holds
+// tile + group * (bl_stride / 64) of its interval.
+template <int NG, bool F16>
+__global__ void __launch_bounds__(kCorrelateMaxThreads)
+    BatchFullJonesCorrelateKernel(const IterArgs* __restrict__ args,
+                                  const BatchTile* __restrict__ tiles) {
+  const BatchTile t = LoadConst(tiles + blockIdx.x);
+  const IterArgs a = LoadConst(args + t.interval);
+  if (AsConst(a.state)->done) return;
+  const int n_tiles = a.bl_stride / kWave;
+  const int group = t.tile / n_tiles;
+  FullJonesCorrelateTile<NG, F16>(a, t.tile - group * n_tiles, group, t.seg, t.cb);
+}
+
 // Model products, once per solve: Q[p][q] = sum over a segment's rows of
 // M_p conj(M_q) per (block, segment, direction, baseline), Hermitian in
 // (p, q): the four real diagonal terms, then (p < q) in the order (0,1),
@@ -1492,12 +1511,9 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads) FullJonesCorrelateKernel
 // blocks x directions), lane = baseline; waves split the rows and are
 // combined through LDS in a fixed order.
 template <bool F16>
-__global__ void __launch_bounds__(kCorrelateMaxThreads)
-    FullJonesModelProductsKernel(const IterArgs a) {
-  const int tile = blockIdx.x;
-  const int seg = blockIdx.y;
-  const int cb = blockIdx.z / a.n_dir;
-  const int d = blockIdx.z - cb * a.n_dir;
+__device__ __forceinline__ void FullJonesModelProductsTile(const IterArgs& a, const int tile,
+                                                           const int seg, const int cb,
+                                                           const int d) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int n_waves = blockDim.x / kWave;
@@ -1544,6 +1560,27 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads)
                  ((size_t(cb) * a.n_seg + seg) * a.n_dir + d) * kFullJonesModelTerms * a.bl_stride;
 #pragma unroll
     for (int v = 0; v < kFullJonesModelTerms; ++v) out[size_t(v) * a.bl_stride + bl] = q[v];
+  }
+}
+
+template <bool F16>
+__global__ void __launch_bounds__(kCorrelateMaxThreads)
+    FullJonesModelProductsKernel(const IterArgs a) {
+  const int cb = blockIdx.z / a.n_dir;
+  FullJonesModelProductsTile<F16>(a, blockIdx.x, blockIdx.y, cb, blockIdx.z - cb * a.n_dir);
+}
+
+// Batch: one workgroup per (interval, block, segment, tile) entry, over every
+// direction of its interval in turn.
+template <bool F16>
+__global__ void __launch_bounds__(kCorrelateMaxThreads)
+    BatchFullJonesModelProductsKernel(const IterArgs* __restrict__ args,
+                                      const BatchTile* __restrict__ tiles) {
+  const BatchTile t = LoadConst(tiles + blockIdx.x);
+  const IterArgs a = LoadConst(args + t.interval);
+  for (int d = 0; d < a.n_dir; ++d) {
+    FullJonesModelProductsTile<F16>(a, t.tile, t.seg, t.cb, d);
+    __syncthreads();  // the next direction reuses the reduction LDS
   }
 }
 
@@ -1713,6 +1750,15 @@ __device__ __forceinline__ void FullJonesSolveBlock(const IterArgs& a, const int
 __global__ void __launch_bounds__(kBlockThreads) FullJonesSolveKernel(const IterArgs a) {
   if (a.state->done) return;
   FullJonesSolveBlock(a, blockIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(kBlockThreads)
+    BatchFullJonesSolveKernel(const IterArgs* __restrict__ args,
+                              const BatchTile* __restrict__ items) {
+  const BatchTile t = items[blockIdx.x];
+  const IterArgs& a = args[t.interval];
+  if (a.state->done) return;
+  FullJonesSolveBlock(a, t.seg, t.cb);  // seg holds the direction
 }
 
 // chi^2 with 2x2 solutions: |D - sum_d J_a M_d J_b^H|^2 in FP64 (gains
@@ -2045,6 +2091,16 @@ size_t CorrelateSharedBytes(int n_dir, int n_corr, int waves) {
 
 namespace {
 using FullJonesKernelFn = void (*)(IterArgs);
+using BatchFullJonesKernelFn = void (*)(const IterArgs*, const BatchTile*);
+const BatchFullJonesKernelFn kBatchFullJonesCorrelate[2][kFullJonesGroup] = {
+    {BatchFullJonesCorrelateKernel<1, false>, BatchFullJonesCorrelateKernel<2, false>,
+     BatchFullJonesCorrelateKernel<3, false>, BatchFullJonesCorrelateKernel<4, false>,
+     BatchFullJonesCorrelateKernel<5, false>, BatchFullJonesCorrelateKernel<6, false>,
+     BatchFullJonesCorrelateKernel<7, false>, BatchFullJonesCorrelateKernel<8, false>},
+    {BatchFullJonesCorrelateKernel<1, true>, BatchFullJonesCorrelateKernel<2, true>,
+     BatchFullJonesCorrelateKernel<3, true>, BatchFullJonesCorrelateKernel<4, true>,
+     BatchFullJonesCorrelateKernel<5, true>, BatchFullJonesCorrelateKernel<6, true>,
+     BatchFullJonesCorrelateKernel<7, true>, BatchFullJonesCorrelateKernel<8, true>}};
 const FullJonesKernelFn kFullJonesCorrelate[2][kFullJonesGroup] = {
     {FullJonesCorrelateKernel<1, false>, FullJonesCorrelateKernel<2, false>,
      FullJonesCorrelateKernel<3, false>, FullJonesCorrelateKernel<4, false>,
@@ -2090,9 +2146,13 @@ void PrepareKernelAttributes() {
                                         int(kApplyGramMaxShared)));
     }
   for (int h = 0; h < 2; ++h)
-    for (int g = 0; g < kFullJonesGroup; ++g)
+    for (int g = 0; g < kFullJonesGroup; ++g) {
       DDH_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kFullJonesCorrelate[h][g]),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+      DDH_HIP_CHECK(
+          hipFuncSetAttribute(reinterpret_cast<const void*>(kBatchFullJonesCorrelate[h][g]),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+    }
   // antenna_solve also holds static LDS for its reductions
   for (const void* f : {reinterpret_cast<const void*>(&AntennaSolveKernel<2, false>),
                         reinterpret_cast<const void*>(&AntennaSolveKernel<2, true>),
@@ -2155,6 +2215,11 @@ void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_ite
         hipLaunchKernelGGL((BatchAntennaSolveKernel<4, false>), dim3(n_items), dim3(solve_threads), solve_lds, stream, d_args, d_items);
     }
   }
+  LaunchBatchSmoothDecide(d_args, d_items, n_items, n_intervals, smooth, stream);
+}
+
+void LaunchBatchSmoothDecide(const IterArgs* d_args, const BatchTile* d_items, int n_items,
+                             int n_intervals, const SmoothLaunch& smooth, hipStream_t stream) {
   if (n_items > 0 && smooth.n_intervals > 0) {
     DDH_REQUIRE(smooth.finish_threads >= kWave && smooth.finish_threads <= kBlockThreads, -1,
                 "bad smoothing block size");
@@ -2212,6 +2277,42 @@ void LaunchFullJonesModelProducts(const IterArgs& a, bool f16, int waves, hipStr
   else
     hipLaunchKernelGGL(FullJonesModelProductsKernel<false>, grid, dim3(waves * kWave), 0, stream,
                        a);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchBatchFullJones(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
+                          int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
+                          int solve_threads, bool f16, int waves, const SmoothLaunch& smooth,
+                          hipStream_t stream) {
+  DDH_REQUIRE(waves >= 1 && waves * kWave <= kCorrelateMaxThreads, -1, "bad waves per tile");
+  for (int g = 0; g < n_groups; ++g) {
+    const TileGroup& grp = groups[g];
+    if (grp.count == 0) continue;
+    const int ng = std::min(grp.n_dir, kFullJonesGroup);
+    const size_t lds = FullJonesCorrelateSharedBytes(grp.n_dir);
+    DDH_REQUIRE(lds <= MaxSharedBytes(), -4, "full-Jones correlate needs more LDS than the device has");
+    hipLaunchKernelGGL(kBatchFullJonesCorrelate[f16][ng - 1], dim3(grp.count),
+                       dim3(waves * kWave), lds, stream, d_args, d_tiles + grp.first);
+  }
+  DDH_HIP_CHECK(hipGetLastError());
+  DDH_REQUIRE(solve_threads >= kWave && solve_threads <= kBlockThreads, -1,
+              "bad antenna solve block size");
+  if (n_items > 0)
+    hipLaunchKernelGGL(BatchFullJonesSolveKernel, dim3(n_items), dim3(solve_threads), 0, stream,
+                       d_args, d_items);
+  LaunchBatchSmoothDecide(d_args, d_items, n_items, n_intervals, smooth, stream);
+}
+
+void LaunchBatchFullJonesModelProducts(const IterArgs* d_args, const BatchTile* d_tiles,
+                                       int n_tiles, bool f16, int waves, hipStream_t stream) {
+  DDH_REQUIRE(waves >= 1 && waves * kWave <= kCorrelateMaxThreads, -1, "bad waves per tile");
+  if (n_tiles == 0) return;
+  if (f16)
+    hipLaunchKernelGGL(BatchFullJonesModelProductsKernel<true>, dim3(n_tiles), dim3(waves * kWave),
+                       0, stream, d_args, d_tiles);
+  else
+    hipLaunchKernelGGL(BatchFullJonesModelProductsKernel<false>, dim3(n_tiles),
+                       dim3(waves * kWave), 0, stream, d_args, d_tiles);
   DDH_HIP_CHECK(hipGetLastError());
 }
 

@@ -76,7 +76,8 @@ typedef struct {
   // IterativeFullJonesSolver (IterativeFullJonesSolver.cc:18-224), a 2x2
   // matrix per antenna and solution, solutions [n_ant][n_sub][4] (row-major
   // 00, 01, 10, 11) per channel block; Full data and the streaming iteration
-  // only, single solves (no batch, no one-step)
+  // only; single and batched solves (a batch holds full-Jones intervals only),
+  // no learned warm start, no one-step
   int32_t n_solution_polarizations;
 } ddh_config;
 

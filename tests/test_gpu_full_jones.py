@@ -197,9 +197,49 @@ def test_full_jones_rejections(small4):
     s = gpu_full_jones(P)
     dp = H.DeviceProblem.from_problem(s, P)
     dp.upload(P)
-    with pytest.raises(H.HipError, match="full-Jones"):
-        H.Batch([s], [dp])
     with pytest.raises(H.HipError, match="one_step"):
         H.one_step(s, dp, units(P))
-    dp.close()
-    s.close()
+    diag = H.Solver(P.n_antennas, P.n_solutions_per_direction, P.n_channel_blocks,
+                    n_correlations=4)
+    dq = H.DeviceProblem.from_problem(diag, P)
+    dq.upload(P)
+    with pytest.raises(H.HipError, match="full-Jones or none"):
+        H.Batch([s, diag], [dp, dq])
+    for x in (dq, diag, dp, s):
+        x.close()
+
+
+def test_full_jones_batch_equals_single_solves(small4):
+    """Batched full-Jones solves (one launch per kernel per iteration over
+    every interval, model products once per solve) equal single solves under
+    the same schedule: the same iteration counts, and solutions to FP64
+    rounding (the batched and single kernels are compiled separately, and
+    their FP64 contractions may differ in the last bit)."""
+    w, geom, sky, gains, P = small4
+    print("case: full-Jones batch", flush=True)
+    P2 = small_problem(n_corr=4, seed=11)[4]
+    problems = [P, P2]
+    kw = dict(max_iterations=w.max_iterations, accuracy=w.tolerance)
+    singles = []
+    for X in problems:
+        s = gpu_full_jones(X, **kw)
+        s.set_schedule(waves_per_tile=2, rows_per_trip=1, use_graph=0)
+        sol = units(X)
+        r = s.solve(X, sol)
+        singles.append((sol, r.iterations))
+        s.close()
+    solvers = [gpu_full_jones(X, **kw) for X in problems]
+    dps = [H.DeviceProblem.from_problem(s, X) for s, X in zip(solvers, problems)]
+    for dp, X in zip(dps, problems):
+        dp.upload(X)
+    b = H.Batch(solvers, dps)
+    b.set_schedule(waves_per_tile=2, rows_per_trip=1, use_graph=1, poll_interval=4)
+    sols = [units(X) for X in problems]
+    res = b.solve(sols)
+    for (sol, it), got, r in zip(singles, sols, res):
+        print(f"iterations batch {r.iterations} single {it}")
+        assert r.iterations == it
+        np.testing.assert_allclose(got, sol, rtol=1e-13, atol=1e-15)
+    b.close()
+    for x in dps + solvers:
+        x.close()
