@@ -209,17 +209,18 @@ def test_full_jones_rejections(small4):
         x.close()
 
 
-def test_full_jones_batch_equals_single_solves(small4):
+@pytest.mark.parametrize("storage", [H.MODEL_C64, H.MODEL_F16])
+def test_full_jones_batch_equals_single_solves(small4, storage):
     """Batched full-Jones solves (one launch per kernel per iteration over
     every interval, model products once per solve) equal single solves under
     the same schedule: the same iteration counts, and solutions to FP64
     rounding (the batched and single kernels are compiled separately, and
     their FP64 contractions may differ in the last bit)."""
     w, geom, sky, gains, P = small4
-    print("case: full-Jones batch", flush=True)
+    print(f"case: full-Jones batch storage={storage}", flush=True)
     P2 = small_problem(n_corr=4, seed=11)[4]
     problems = [P, P2]
-    kw = dict(max_iterations=w.max_iterations, accuracy=w.tolerance)
+    kw = dict(max_iterations=w.max_iterations, accuracy=w.tolerance, model_storage=storage)
     singles = []
     for X in problems:
         s = gpu_full_jones(X, **kw)
