@@ -187,3 +187,24 @@ def test_scalar_phase_only(small):
     assert r.iterations == rr.iterations
     np.testing.assert_allclose(np.abs(sol), 1.0, atol=1e-12)
     assert err <= GAIN_TOL
+
+
+def test_scalar_single_correlation_gram_iteration():
+    """Single-correlation data through the Gram iteration (its device planes
+    are Duo pairs): the same iteration count and solutions as the streaming
+    iteration's reference parity (tSolvers.cc:171-185)."""
+    P, inp = solvertester_problem((1, 2, 5), n_corr=1, scalar=True)
+    print("case: scalar single correlation gram", flush=True)
+    s = gpu_scalar(P, max_iterations=100, accuracy=1e-8, step_size=0.2,
+                   iteration=H.ITERATION_GRAM)
+    sol = scalar_units(P)
+    r = s.solve(P, sol)
+    s.close()
+    check_scalar_results(sol, inp, P.n_antennas, P.n_sub, P.n_channel_blocks, 1e-3)
+    ref = scalar_units(P)
+    rr = O.solve(P, ref, max_iterations=100, accuracy=1e-8, step_size=0.2, scalar=True,
+                 n_threads=8)
+    err = scalar_relerr(sol, ref, P.n_antennas, P.n_sub)
+    print(f"iterations gpu {r.iterations} oracle {rr.iterations}, gain err {err:.2e}")
+    assert r.iterations <= rr.iterations
+    assert err <= GAIN_TOL
