@@ -622,7 +622,8 @@ __device__ void StageWords(const T* __restrict__ src, T* dst, int n) {
 
 // Block-wide deterministic sums of two (double, int) pairs: a fixed xor
 // shuffle tree inside each wave, then the per-wave results in wave order.
-// `red` needs 2 * (blockDim.x / 64) doubles, `redc` as many ints.
+// `red` needs 2 * (blockDim.x / 64) doubles, `redc` as many ints, used by
+// nothing else in the workgroup (one barrier, between their writes and reads).
 __device__ void BlockSum2(double& v, int& c, double& w, int& e, double* red, int* redc) {
   for (int off = 32; off > 0; off >>= 1) {
     v += __shfl_xor(v, off);
@@ -632,7 +633,6 @@ __device__ void BlockSum2(double& v, int& c, double& w, int& e, double* red, int
   }
   const int n_waves = blockDim.x / 64;
   const int wave = threadIdx.x / 64;
-  __syncthreads();
   if ((threadIdx.x & 63) == 0) {
     red[wave] = v;
     redc[wave] = c;
@@ -650,7 +650,6 @@ __device__ void BlockSum2(double& v, int& c, double& w, int& e, double* red, int
     w += red[n_waves + i];
     e += redc[n_waves + i];
   }
-  __syncthreads();
 }
 
 // Block-wide deterministic sum (fixed tree) of one double and one int.
@@ -785,6 +784,10 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
   auto gain = [&](int an, int pol) { return g_s[(an * nsd + rel) * 2 + pol]; };
 
   double nr[2] = {0.0, 0.0}, ni[2] = {0.0, 0.0}, dn[2] = {0.0, 0.0};
+  // the first segment's first correlation is staged under the gains' barrier
+  StageWords(a.partials + size_t(cb) * a.n_seg * a.n_dir * NC * 3 * size_t(bs) +
+                 size_t(d) * NC * 3 * size_t(bs),
+             part_s, 3 * bs);
   __syncthreads();
   const double2 gant[2] = {active ? gain(ant, 0) : make_double2(0, 0),
                            active ? gain(ant, 1) : make_double2(0, 0)};
@@ -796,9 +799,11 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
         a.partials + ((size_t(cb) * a.n_seg + seg) * a.n_dir + d) * NC * 3 * size_t(bs);
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
-      if (seg > 0 || k > 0) __syncthreads();  // the previous stage has been read
-      StageWords(src + size_t(k) * 3 * bs, part_s, 3 * bs);
-      __syncthreads();
+      if (seg > 0 || k > 0) {
+        __syncthreads();  // the previous stage has been read
+        StageWords(src + size_t(k) * 3 * bs, part_s, 3 * bs);
+        __syncthreads();
+      }
       if (!active || s_seg != s_abs) continue;
       const int i = PolI<NC>(k);
       const int j = PolJ<NC>(k);
