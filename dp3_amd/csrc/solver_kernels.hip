@@ -652,21 +652,25 @@ __device__ void BlockSum2(double& v, int& c, double& w, int& e, double* red, int
   }
 }
 
-// Block-wide deterministic sum (fixed tree) of one double and one int.
+// Block-wide deterministic sum of one double and one int: a fixed shuffle
+// tree in each wave, then the wave sums in wave order after one barrier.
 __device__ void BlockSum(double& v, int& c, double* red, int* redc) {
-  red[threadIdx.x] = v;
-  redc[threadIdx.x] = c;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      red[threadIdx.x] += red[threadIdx.x + s];
-      redc[threadIdx.x] += redc[threadIdx.x + s];
-    }
-    __syncthreads();
+  for (int off = 32; off > 0; off >>= 1) {
+    v += __shfl_xor(v, off);
+    c += __shfl_xor(c, off);
   }
-  v = red[0];
-  c = redc[0];
+  const int n_waves = blockDim.x / 64;
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x / 64] = v;
+    redc[threadIdx.x / 64] = c;
+  }
   __syncthreads();
+  v = 0.0;
+  c = 0;
+  for (int i = 0; i < n_waves; ++i) {
+    v += red[i];
+    c += redc[i];
+  }
 }
 
 // a / b, scaled by the larger part of b (Smith's algorithm).
@@ -1113,8 +1117,8 @@ __global__ void __launch_bounds__(kBlockThreads)
 // One workgroup: the iteration's decision from the per-workgroup partials of
 // antenna_solve (the kernel boundary makes them visible without fences).
 __device__ __forceinline__ void DecideBlock(const IterArgs& a) {
-  __shared__ double red[kBlockThreads];
-  __shared__ int redc[kBlockThreads];
+  __shared__ double red[kBlockThreads / 64];
+  __shared__ int redc[kBlockThreads / 64];
   const int n_wg = a.n_cb * a.n_dir;
   double sum = 0.0;
   int count = 0;
