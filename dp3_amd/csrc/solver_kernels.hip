@@ -928,14 +928,26 @@ __device__ __forceinline__ void FixedMatrix(const double2* sol, size_t m, double
   }
 }
 
-// acc += a * b and acc += conj(a) * b (complex64)
+// acc += a * b, acc += conj(a) * b and acc -= conj(a) * b (complex64), each
+// component as two fused multiply-adds into the accumulator (four VALU
+// operations per complex product; the form acc += (p - q) costs six).
 __device__ __forceinline__ void CMac(float2& acc, float2 a, float2 b) {
-  acc.x += a.x * b.x - a.y * b.y;
-  acc.y += a.x * b.y + a.y * b.x;
+  acc.x = fmaf(a.x, b.x, acc.x);
+  acc.x = fmaf(-a.y, b.y, acc.x);
+  acc.y = fmaf(a.x, b.y, acc.y);
+  acc.y = fmaf(a.y, b.x, acc.y);
 }
 __device__ __forceinline__ void CMacC(float2& acc, float2 a, float2 b) {
-  acc.x += a.x * b.x + a.y * b.y;
-  acc.y += a.x * b.y - a.y * b.x;
+  acc.x = fmaf(a.x, b.x, acc.x);
+  acc.x = fmaf(a.y, b.y, acc.x);
+  acc.y = fmaf(a.x, b.y, acc.y);
+  acc.y = fmaf(-a.y, b.x, acc.y);
+}
+__device__ __forceinline__ void CMsubC(float2& acc, float2 a, float2 b) {
+  acc.x = fmaf(-a.x, b.x, acc.x);
+  acc.x = fmaf(-a.y, b.y, acc.x);
+  acc.y = fmaf(-a.x, b.y, acc.y);
+  acc.y = fmaf(a.y, b.x, acc.y);
 }
 
 // 2x2 inverse in FP64 (adjugate over determinant); false when the
@@ -1325,11 +1337,8 @@ __device__ __forceinline__ void SubtractDirection(const float2* ja, const float2
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       // (A J_b^H)_ij = A_i0 conj(Jb_j0) + A_i1 conj(Jb_j1) = sum conj(Jb_jk) A_ik
-      float2 t = make_float2(0.0f, 0.0f);
-      CMacC(t, jb[2 * j], a[2 * i]);
-      CMacC(t, jb[2 * j + 1], a[2 * i + 1]);
-      r[2 * i + j].x -= t.x;
-      r[2 * i + j].y -= t.y;
+      CMsubC(r[2 * i + j], jb[2 * j], a[2 * i]);
+      CMsubC(r[2 * i + j], jb[2 * j + 1], a[2 * i + 1]);
     }
 }
 
