@@ -1321,7 +1321,16 @@ __device__ __forceinline__ void LoadFull(const void* base, size_t idx, float sc,
 }
 
 // R -= (J_a M) J_b^H; returns A = J_a M.
-__device__ __forceinline__ void SubtractDirection(const float2* ja, const float2* jb,
+// One lane's 2x2 gain in the correlate's LDS, entries kWave words apart: the
+// layout [direction][J_a, J_b entry][lane] has the 64 lanes of a read on
+// consecutive 8-byte words (with the entries of a lane adjacent, lanes 64 B
+// apart, a read touched two bank groups).
+struct LaneGains {
+  const float2* p;
+  __device__ __forceinline__ float2 operator[](int k) const { return p[k * kWave]; }
+};
+
+__device__ __forceinline__ void SubtractDirection(const LaneGains ja, const LaneGains jb,
                                                   const float2 (&m)[4], float2 (&r)[4],
                                                   float2 (&a)[4]) {
 #pragma unroll
@@ -1360,7 +1369,7 @@ __device__ __forceinline__ void FullJonesCorrelateTile(const IterArgs& a, const 
   const int bl = tile * kWave + lane;
 
   extern __shared__ float smem[];
-  float2* g_s = reinterpret_cast<float2*>(smem);  // [n_dir][64][8]: J_a, then J_b
+  float2* g_s = reinterpret_cast<float2*>(smem);  // [n_dir][8][64]: J_a, then J_b
   float* red_s = smem + size_t(a.n_dir) * kWave * 16;  // [16 * NG][64]
 
   // Stage the repaired, float-cast gains of every direction (SolveData.h
@@ -1377,11 +1386,13 @@ __device__ __forceinline__ void FullJonesCorrelateTile(const IterArgs& a, const 
       double2 j1[4], j2[4];
       FixedMatrix(sol, gbase + size_t(a1) * a.n_sub + s, mean, j1);
       FixedMatrix(sol, gbase + size_t(a2) * a.n_sub + s, mean, j2);
-      float2* dst = g_s + (size_t(d) * kWave + lane) * 8;
+      float2* dst = g_s + size_t(d) * 8 * kWave + lane;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        dst[k] = valid ? make_float2(float(j1[k].x), float(j1[k].y)) : make_float2(0.0f, 0.0f);
-        dst[4 + k] = valid ? make_float2(float(j2[k].x), float(j2[k].y)) : make_float2(0.0f, 0.0f);
+        dst[k * kWave] =
+            valid ? make_float2(float(j1[k].x), float(j1[k].y)) : make_float2(0.0f, 0.0f);
+        dst[(4 + k) * kWave] =
+            valid ? make_float2(float(j2[k].x), float(j2[k].y)) : make_float2(0.0f, 0.0f);
       }
     }
   }
@@ -1412,9 +1423,9 @@ __device__ __forceinline__ void FullJonesCorrelateTile(const IterArgs& a, const 
 #pragma unroll
     for (int j = 0; j < NG; ++j) {
       if (j < ng) {
-        const float2* gj = g_s + (size_t(g0 + j) * kWave + lane) * 8;
+        const LaneGains gj{g_s + size_t(g0 + j) * 8 * kWave + lane};
         float2 av[4];
-        SubtractDirection(gj, gj + 4, m[j], r, av);
+        SubtractDirection(gj, LaneGains{gj.p + 4 * kWave}, m[j], r, av);
       }
     }
     for (int d = 0; d < a.n_dir; ++d) {
@@ -1422,15 +1433,15 @@ __device__ __forceinline__ void FullJonesCorrelateTile(const IterArgs& a, const 
       const float s = F16 ? AsConst(a.model_scale)[cb * a.n_dir + d] : 1.0f;
       float2 mo[4];
       LoadFull<F16>(a.model, info.model_off + d * plane + e, s, mo);
-      const float2* gd = g_s + (size_t(d) * kWave + lane) * 8;
+      const LaneGains gd{g_s + size_t(d) * 8 * kWave + lane};
       float2 av[4];
-      SubtractDirection(gd, gd + 4, mo, r, av);
+      SubtractDirection(gd, LaneGains{gd.p + 4 * kWave}, mo, r, av);
     }
 #pragma unroll
     for (int j = 0; j < NG; ++j) {
       if (j >= ng) continue;
-      const float2* ja = g_s + (size_t(g0 + j) * kWave + lane) * 8;
-      const float2* jb = ja + 4;
+      const LaneGains ja{g_s + size_t(g0 + j) * 8 * kWave + lane};
+      const LaneGains jb{ja.p + 4 * kWave};
       float2 av[4], bv[4];
       // A = J_a M
 #pragma unroll
