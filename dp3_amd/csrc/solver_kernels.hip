@@ -491,30 +491,64 @@ __device__ __forceinline__ void ApplyGramTile(const IterArgs& a, const int tile,
   const int k = int(threadIdx.x >> 6);  // kApplyThreads = NC waves
   const int bl = tile * kWave + lane;
   const bool valid = bl < a.n_bl;
-  const int a1 = valid ? AsGlobal(a.ant1)[bl] : 0;
-  const int a2 = valid ? AsGlobal(a.ant2)[bl] : 0;
+  // The segment's gains of every antenna and direction, repaired (a pair with
+  // a non-finite entry reads as the block mean) and cast to float, as
+  // correlate stages them. Vector loads complete in order, so the first
+  // kStageUnroll entries per thread are loaded before the cross products:
+  // writing them to LDS then waits for these loads only, while the cross
+  // products stream in. Further entries (more than kStageUnroll x
+  // kApplyThreads) are staged after.
+  constexpr int kStageUnroll = 4;
+  const int n_entries = a.n_ant * ND;
+  const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
+  const double2* sol = (AsConst(a.state)->iteration & 1) ? a.sol1 : a.sol0;
+  // entry e: antenna e / ND, direction e % ND
+  auto pair_of = [&](int e, int sub) { return gbase + size_t(e / ND) * a.n_sub + sub; };
+  // Loads without branches (an entry past the end loads entry 0 and is not
+  // written), so that they issue back to back.
+  int sub[kStageUnroll];
+#pragma unroll
+  for (int u = 0; u < kStageUnroll; ++u) {
+    const int e = threadIdx.x + u * kApplyThreads;
+    sub[u] = AsConst(a.seg_sol)[seg * ND + (e < n_entries ? e : 0) % ND];
+  }
+  double2 sg[kStageUnroll][2];
+#pragma unroll
+  for (int u = 0; u < kStageUnroll; ++u) {
+    const int e = threadIdx.x + u * kApplyThreads;
+    const size_t p = pair_of(e < n_entries ? e : 0, sub[u]);
+    sg[u][0] = LoadGlobal(sol + p * 2);
+    sg[u][1] = LoadGlobal(sol + p * 2 + 1);
+  }
+  // keep the scheduler from moving the cross-product loads ahead of these
+  __builtin_amdgcn_sched_barrier(0);
   const GlobalT<f2v>* in = (const GlobalT<f2v>*)AsGlobal(
       a.gram + (size_t(cb * a.n_seg + seg) * NC * NQ + size_t(k) * NQ) * a.bl_stride + bl);
   f2v q[NQ];
 #pragma unroll
   for (int i = 0; i < NQ; ++i) q[i] = in[size_t(i) * a.bl_stride];
-  // The segment's gains of every antenna and direction, repaired (a pair with
-  // a non-finite entry reads as the block mean) and cast to float, as
-  // correlate stages them.
+  const int bl_c = valid ? bl : 0;
+  const int a1v = AsGlobal(a.ant1)[bl_c];
+  const int a2v = AsGlobal(a.ant2)[bl_c];
   {
-    const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
-    const double2* sol = (AsConst(a.state)->iteration & 1) ? a.sol1 : a.sol0;
     const double mean = AsConst(a.mean)[cb];
-    for (int e = threadIdx.x; e < a.n_ant * ND; e += kApplyThreads) {
-      const int an = e / ND;
-      const int d = e - an * ND;
-      const size_t p = gbase + size_t(an) * a.n_sub + AsConst(a.seg_sol)[seg * ND + d];
-      double2 g[2] = {LoadGlobal(sol + p * 2), LoadGlobal(sol + p * 2 + 1)};
-      if (!(cfinite(g[0]) && cfinite(g[1]))) g[0] = g[1] = make_double2(mean, 0.0);
-      s_gain[an * kGainRow + d * NC + 0] = make_float2(float(g[0].x), float(g[0].y));
-      s_gain[an * kGainRow + d * NC + 1] = make_float2(float(g[1].x), float(g[1].y));
+    auto put = [&](int e, double2 g0, double2 g1) {
+      if (!(cfinite(g0) && cfinite(g1))) g0 = g1 = make_double2(mean, 0.0);
+      s_gain[e * NC + e / ND + 0] = make_float2(float(g0.x), float(g0.y));
+      s_gain[e * NC + e / ND + 1] = make_float2(float(g1.x), float(g1.y));
+    };
+#pragma unroll
+    for (int u = 0; u < kStageUnroll; ++u) {
+      const int e = threadIdx.x + u * kApplyThreads;
+      if (e < n_entries) put(e, sg[u][0], sg[u][1]);
+    }
+    for (int e = threadIdx.x + kStageUnroll * kApplyThreads; e < n_entries; e += kApplyThreads) {
+      const size_t p = pair_of(e, AsConst(a.seg_sol)[seg * ND + e % ND]);
+      put(e, LoadGlobal(sol + p * 2), LoadGlobal(sol + p * 2 + 1));
     }
   }
+  const int a1 = valid ? a1v : 0;
+  const int a2 = valid ? a2v : 0;
   __syncthreads();
   // gg_d,k with the FP16 scale folded in
   float2 gg[ND];
