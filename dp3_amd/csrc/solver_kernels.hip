@@ -1354,7 +1354,6 @@ __device__ __forceinline__ void LoadFull(const void* base, size_t idx, float sc,
   }
 }
 
-// R -= (J_a M) J_b^H; returns A = J_a M.
 // One lane's 2x2 gain in the correlate's LDS, entries kWave words apart: the
 // layout [direction][J_a, J_b entry][lane] has the 64 lanes of a read on
 // consecutive 8-byte words (with the entries of a lane adjacent, lanes 64 B
@@ -1364,6 +1363,7 @@ struct LaneGains {
   __device__ __forceinline__ float2 operator[](int k) const { return p[k * kWave]; }
 };
 
+// R -= (J_a M) J_b^H; returns A = J_a M.
 __device__ __forceinline__ void SubtractDirection(const LaneGains ja, const LaneGains jb,
                                                   const float2 (&m)[4], float2 (&r)[4],
                                                   float2 (&a)[4]) {
