@@ -35,16 +35,26 @@ __global__ void __launch_bounds__(256)
     CorrectSubtractKernel(float4_v* __restrict__ data, float4_v* __restrict__ model,
                           const double2* __restrict__ sol, const int32_t* __restrict__ ant1,
                           const int32_t* __restrict__ ant2, const int32_t* __restrict__ cb_start,
-                          int n_t, int n_bl, int n_ch, int n_dir, int n_ant, int subtract,
-                          int keep_model) {
+                          int n_t, int n_bl, int n_ch, int n_cb, int n_dir, int n_ant,
+                          int subtract, int keep_model) {
   const int64_t n_vis = int64_t(n_t) * n_bl * n_ch;
   for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n_vis;
        v += int64_t(gridDim.x) * blockDim.x) {
     const int ch = int(v % n_ch);
     const int bl = int((v / n_ch) % n_bl);
     const int a1 = ant1[bl], a2 = ant2[bl];
-    int cb = 0;
-    while (ch >= cb_start[cb + 1]) ++cb;  // block of this channel (DDECal.cc:1106-1108)
+    // The block of this channel (DDECal.cc:1106-1108): the first cb with
+    // cb_start[cb + 1] > ch, by bisection over the increasing starts
+    // (cb_start[n_cb] = n_ch ends it).
+    int lo = 0, hi = n_cb - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (cb_start[mid + 1] > ch)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    const int cb = lo;
     const double2* s = sol + size_t(cb) * n_ant * n_dir * NPOL;
     C2 sum[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     for (int d = 0; d < n_dir; ++d) {
@@ -121,7 +131,8 @@ void CheckShape(int32_t n_t, int32_t n_bl, int32_t n_ch, int32_t n_dir, int32_t 
               "solution polarizations must be 1, 2 or 4");
 }
 
-void Launch(int32_t n_t, int32_t n_bl, int32_t n_ch, int32_t n_dir, int32_t n_ant, int32_t n_pol,
+void Launch(int32_t n_t, int32_t n_bl, int32_t n_ch, int32_t n_cb, int32_t n_dir, int32_t n_ant,
+            int32_t n_pol,
             const int32_t* d_ant1, const int32_t* d_ant2, const int32_t* d_cb_start,
             const double* d_sol, void* d_data, void* d_model, int subtract, int keep_model,
             hipStream_t stream) {
@@ -132,15 +143,15 @@ void Launch(int32_t n_t, int32_t n_bl, int32_t n_ch, int32_t n_dir, int32_t n_an
   auto* sol = reinterpret_cast<const double2*>(d_sol);
   if (n_pol == 1)
     hipLaunchKernelGGL(CorrectSubtractKernel<1>, dim3(blocks), dim3(256), 0, stream, data, model,
-                       sol, d_ant1, d_ant2, d_cb_start, n_t, n_bl, n_ch, n_dir, n_ant, subtract,
+                       sol, d_ant1, d_ant2, d_cb_start, n_t, n_bl, n_ch, n_cb, n_dir, n_ant, subtract,
                        keep_model);
   else if (n_pol == 2)
     hipLaunchKernelGGL(CorrectSubtractKernel<2>, dim3(blocks), dim3(256), 0, stream, data, model,
-                       sol, d_ant1, d_ant2, d_cb_start, n_t, n_bl, n_ch, n_dir, n_ant, subtract,
+                       sol, d_ant1, d_ant2, d_cb_start, n_t, n_bl, n_ch, n_cb, n_dir, n_ant, subtract,
                        keep_model);
   else
     hipLaunchKernelGGL(CorrectSubtractKernel<4>, dim3(blocks), dim3(256), 0, stream, data, model,
-                       sol, d_ant1, d_ant2, d_cb_start, n_t, n_bl, n_ch, n_dir, n_ant, subtract,
+                       sol, d_ant1, d_ant2, d_cb_start, n_t, n_bl, n_ch, n_cb, n_dir, n_ant, subtract,
                        keep_model);
   DDH_HIP_CHECK(hipGetLastError());
 }
@@ -164,7 +175,8 @@ int ddh_correct_and_subtract_device(int32_t n_times, int32_t n_baselines, int32_
                 "NULL argument");
     CheckShape(n_times, n_baselines, n_channels, n_directions, n_antennas, n_channel_blocks,
                n_pol);
-    Launch(n_times, n_baselines, n_channels, n_directions, n_antennas, n_pol, antenna1, antenna2,
+    Launch(n_times, n_baselines, n_channels, n_channel_blocks, n_directions, n_antennas, n_pol,
+           antenna1, antenna2,
            channel_block_start, solutions, data, model, subtract, keep_model,
            reinterpret_cast<hipStream_t>(stream));
   });
@@ -182,8 +194,8 @@ int ddh_correct_and_subtract(int32_t n_times, int32_t n_baselines, int32_t n_cha
     CheckShape(n_times, n_baselines, n_channels, n_directions, n_antennas, n_channel_blocks,
                n_pol);
     for (int cb = 0; cb < n_channel_blocks; ++cb)
-      DDH_REQUIRE(channel_block_start[cb] <= channel_block_start[cb + 1], -1,
-                  "channel blocks must be ordered");
+      DDH_REQUIRE(channel_block_start[cb] < channel_block_start[cb + 1], -1,
+                  "channel blocks must be ordered and hold >= 1 channel");
     DDH_REQUIRE(channel_block_start[0] == 0 && channel_block_start[n_channel_blocks] == n_channels,
                 -1, "channel blocks must cover the channels");
     for (int b = 0; b < n_baselines; ++b)
@@ -208,7 +220,8 @@ int ddh_correct_and_subtract(int32_t n_times, int32_t n_baselines, int32_t n_cha
     DDH_HIP_CHECK(hipEventCreate(&e0));
     DDH_HIP_CHECK(hipEventCreate(&e1));
     DDH_HIP_CHECK(hipEventRecord(e0, stream));
-    Launch(n_times, n_baselines, n_channels, n_directions, n_antennas, n_pol, a1.ptr, a2.ptr,
+    Launch(n_times, n_baselines, n_channels, n_channel_blocks, n_directions, n_antennas, n_pol,
+           a1.ptr, a2.ptr,
            cbs.ptr, sol.ptr, d.ptr, m.ptr, subtract, keep_model, stream);
     DDH_HIP_CHECK(hipEventRecord(e1, stream));
     DDH_HIP_CHECK(hipEventSynchronize(e1));

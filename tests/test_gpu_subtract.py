@@ -10,7 +10,7 @@ from oracle import pyoracle as O
 pytestmark = pytest.mark.gpu
 
 
-def case(n_pol, seed=5, n_t=3, n_ant=7, n_ch=9, n_dir=4):
+def case(n_pol, seed=5, n_t=3, n_ant=7, n_ch=9, n_dir=4, starts=(0, 2, 6)):
     rng = np.random.default_rng(seed)
     bl = [(a, b) for a in range(n_ant) for b in range(a, n_ant)]
     a1 = np.array([x for x, _ in bl], np.int32)
@@ -22,7 +22,7 @@ def case(n_pol, seed=5, n_t=3, n_ant=7, n_ch=9, n_dir=4):
 
     data = cplx(n_t, n_bl, n_ch, 4).astype(np.complex64)
     model = cplx(n_dir, n_t, n_bl, n_ch, 4).astype(np.complex64)
-    cbs = np.array([0, 2, 6, n_ch], np.int32)
+    cbs = np.array(list(starts) + [n_ch], np.int32)
     sol = cplx(len(cbs) - 1, n_ant, n_dir, n_pol)
     return data, model, sol, a1, a2, cbs, n_ant
 
@@ -42,3 +42,20 @@ def test_correct_and_subtract_matches_oracle(n_pol, subtract, keep):
         np.testing.assert_array_equal(d, data)
     if not keep:
         np.testing.assert_array_equal(m, model)
+
+
+@pytest.mark.parametrize("n_pol", [1, 2])
+def test_correct_and_subtract_single_channel_blocks(n_pol):
+    """Blocks of one channel at both ends and inside: every channel finds its
+    block (DDECal.cc:1106-1108, one block step per channel); a block without
+    channels is rejected."""
+    print(f"case: correct and subtract single-channel blocks n_pol={n_pol}", flush=True)
+    data, model, sol, a1, a2, cbs, n_ant = case(n_pol, n_ch=11, starts=(0, 1, 3, 4, 10))
+    d_ref, m_ref = O.correct_and_subtract(data, model, sol, a1, a2, cbs, n_ant, 1, 1)
+    d, m, ms = H.correct_and_subtract(data, model, sol, a1, a2, cbs, n_ant, 1, 1)
+    scale = np.abs(model).max() * np.abs(sol).max() ** 2 * model.shape[0]
+    np.testing.assert_allclose(d, d_ref, rtol=0, atol=4e-7 * scale)
+    np.testing.assert_allclose(m, m_ref, rtol=0, atol=4e-7 * scale)
+    data, model, sol, a1, a2, cbs, n_ant = case(n_pol, n_ch=11, starts=(0, 3, 3, 10))
+    with pytest.raises(H.HipError, match="channel"):
+        H.correct_and_subtract(data, model, sol, a1, a2, cbs, n_ant, 1, 1)
