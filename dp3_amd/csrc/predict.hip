@@ -13,10 +13,15 @@
 //                         FWHM / (2 sqrt(2 ln 2)); absolute PA rotates the
 //                         station UVW with the Euler matrices of
 //                         steps/PhaseShift.cc:197-212
-//   frequency smearing    sinc(0.5 (phi_q - phi_p) width) in single precision
+//   frequency smearing    sinc(0.5 (phi_q - phi_p) width), Simulator.cc:190-199
+//   time smearing         sinc(0.5 (rot_q - rot_p) f), Simulator.cc:177-188, with the
+//                         station earth-rotation phase of Simulator.cc:488-503
+//                         (scaled_ncp_uvw[0] taken as zero, as there)
 //   Stokes-I copy         steps/OnePredict.cc:440-452 (XX = YY = I, XY = YX = 0)
-// Station phases and their sines/cosines are FP64; the per-baseline products
-// and the sum over components are FP32, rounded once to the model storage.
+// As in Simulator.cc, everything is FP64 (station phasors, Gaussian envelope,
+// per-baseline products and the sum over a direction's components) and the
+// sum is rounded once to the model storage; sinc takes a float argument, as
+// the reference's sinc(float) does (Simulator.cc:472-474).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,9 +41,9 @@ constexpr double kSpeedOfLight = 299792458.0;
 
 struct CompGeom {
   double l, m, nm1;                // lmn, n - 1
-  float cos_phi, sin_phi;          // Gaussian rotation
-  float u_scale, v_scale;          // FWHM -> sigma
-  float rot[9];                    // Euler rotation (identity unless absolute PA)
+  double cos_phi, sin_phi;         // Gaussian rotation
+  double u_scale, v_scale;         // FWHM -> sigma
+  double rot[9];                   // Euler rotation (identity unless absolute PA)
   int32_t is_gaussian;
   int32_t direction;
 };
@@ -82,16 +87,16 @@ __global__ void ComponentGeometryKernel(const ddh_component* comps, int n_comp, 
   g.nm1 = lmn[2] - 1.0;
   g.is_gaussian = s.is_gaussian;
   g.direction = s.direction;
-  for (int i = 0; i < 9; ++i) g.rot[i] = (i % 4 == 0) ? 1.0f : 0.0f;
+  for (int i = 0; i < 9; ++i) g.rot[i] = (i % 4 == 0) ? 1.0 : 0.0;
   if (s.is_gaussian) {
     const double phi = M_PI_2 + s.position_angle + M_PI;
     double sp, cp;
     sincos(phi, &sp, &cp);
-    g.cos_phi = float(cp);
-    g.sin_phi = float(sp);
+    g.cos_phi = cp;
+    g.sin_phi = sp;
     const double fwhm2sigma = 1.0 / (2.0 * sqrt(2.0 * log(2.0)));
-    g.u_scale = float(s.major_axis * fwhm2sigma);
-    g.v_scale = float(s.minor_axis * fwhm2sigma);
+    g.u_scale = s.major_axis * fwhm2sigma;
+    g.v_scale = s.minor_axis * fwhm2sigma;
     if (s.pa_is_absolute) {
       double ep[3][3], es[3][3];
       Euler(ra0, dec0, ep);
@@ -100,16 +105,19 @@ __global__ void ComponentGeometryKernel(const ddh_component* comps, int n_comp, 
         for (int j = 0; j < 3; ++j) {
           double acc = 0.0;
           for (int k = 0; k < 3; ++k) acc += es[k][i] * ep[k][j];
-          g.rot[i * 3 + j] = float(acc);
+          g.rot[i * 3 + j] = acc;
         }
     }
   }
   out[c] = g;
 }
 
-// Stokes (I, Q, U, V) per (component, channel) (PointSource.cc:51-99).
+// Stokes (I, Q, U, V) per (component, channel) (PointSource.cc:51-99), FP64.
+struct Stokes4 {
+  double I, Q, U, V;
+};
 __global__ void SpectrumKernel(const ddh_component* comps, int n_comp, const double* freq,
-                               int n_ch, float4* spec) {
+                               int n_ch, Stokes4* spec) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n_comp * n_ch) return;
   const int c = idx / n_ch;
@@ -140,15 +148,19 @@ __global__ void SpectrumKernel(const ddh_component* comps, int n_comp, const dou
     Q = qu * cc;
     U = qu * sc;
   }
-  spec[idx] = make_float4(float(I), float(Q), float(U), float(V));
+  spec[idx] = Stokes4{I, Q, U, V};
 }
 
 // Station phase phi = 2 pi / c (u l + v m + w (n - 1)) per (t, component,
 // station) and its per-channel phasor exp(i phi f) for the channels of one
-// block: table[(t * n_chb + ch) * n_comp + c][st] (complex64).
+// block: table[(t * n_chb + ch) * n_comp + c][st] (complex128). With
+// scaled_ncp_uvw (time smearing), also the station earth-rotation phase
+// 2 pi / c (l (v ncp_w - w ncp_v) - m u ncp_w + (n - 1) u ncp_v)
+// (Simulator.cc:488-503) per (t, component, station).
 __global__ void StationPhasorKernel(const double* uvw, int n_st, const CompGeom* geom,
                                     int n_comp, const double* freq, int ch0, int n_chb,
-                                    int n_t, float2* table, double* st_phase) {
+                                    int n_t, double2* table, double* st_phase, double* st_rot,
+                                    double ncp_v, double ncp_w) {
   const int64_t total = int64_t(n_t) * n_chb * n_comp * n_st;
   const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= total) return;
@@ -160,12 +172,17 @@ __global__ void StationPhasorKernel(const double* uvw, int n_st, const CompGeom*
   const int t = int(r / n_chb);
   const double* u = uvw + (int64_t(t) * n_st + st) * 3;
   const CompGeom& g = geom[c];
-  const double phi =
-      2.0 * M_PI / kSpeedOfLight * (u[0] * g.l + u[1] * g.m + u[2] * g.nm1);
+  constexpr double cinv = 2.0 * M_PI / kSpeedOfLight;
+  const double phi = cinv * (u[0] * g.l + u[1] * g.m + u[2] * g.nm1);
   double s, co;
   sincos(phi * freq[ch0 + ch], &s, &co);
-  table[idx] = make_float2(float(co), float(s));
-  if (ch == 0) st_phase[(int64_t(t) * n_comp + c) * n_st + st] = phi;
+  table[idx] = make_double2(co, s);
+  if (ch == 0) {
+    const int64_t k = (int64_t(t) * n_comp + c) * n_st + st;
+    st_phase[k] = phi;
+    st_rot[k] = cinv * (g.l * (u[1] * ncp_w - u[2] * ncp_v) + g.m * (-u[0] * ncp_w) +
+                        g.nm1 * (u[0] * ncp_v));
+  }
 }
 
 __device__ __forceinline__ float Sinc(float x) { return fabsf(x) < 1e-6f ? 1.0f : sinf(x) / x; }
@@ -179,12 +196,16 @@ __device__ __forceinline__ uint32_t HalfBits(float re, float im) {
 }
 
 // One thread per baseline of row (t, ch); loops over the components of
-// direction d. Writes the model plane of (cb, d).
+// direction d in FP64 (Simulator.cc:157-281 for points, :284-468 for
+// Gaussians: the envelope, then the time- and frequency-smearing factors, on
+// shift_q conj(shift_p) S) and rounds the direction's sum once. Writes the
+// model plane of (cb, d).
 template <int NC, bool F16>
-__global__ void BaselinePredictKernel(const float2* __restrict__ table,
+__global__ void BaselinePredictKernel(const double2* __restrict__ table,
                                       const double* __restrict__ st_phase,
+                                      const double* __restrict__ st_rot,
                                       const CompGeom* __restrict__ geom,
-                                      const float4* __restrict__ spec,
+                                      const Stokes4* __restrict__ spec,
                                       const double* __restrict__ uvw,
                                       const int32_t* __restrict__ ant1,
                                       const int32_t* __restrict__ ant2,
@@ -192,65 +213,73 @@ __global__ void BaselinePredictKernel(const float2* __restrict__ table,
                                       const double* __restrict__ chan_width, int n_bl,
                                       int bl_stride, int n_st, int n_comp, int c0, int c1,
                                       int ch0, int n_chb, int n_ch_total, int stokes_i_only,
-                                      int freq_smear, const float* scale, void* plane) {
+                                      int freq_smear, int time_smear, const float* scale,
+                                      void* plane) {
   const int bl = blockIdx.x * blockDim.x + threadIdx.x;
   const int ch = blockIdx.y;
   const int t = blockIdx.z;
   if (bl >= bl_stride) return;
   const int64_t out_idx = (int64_t(t) * n_chb + ch) * bl_stride + bl;
-  float2 v[4] = {make_float2(0, 0), make_float2(0, 0), make_float2(0, 0), make_float2(0, 0)};
+  double2 v[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0),
+                  make_double2(0, 0)};
   if (bl < n_bl) {
     const int p = ant1[bl], q = ant2[bl];
     const double* up = uvw + (int64_t(t) * n_st + p) * 3;
     const double* uq = uvw + (int64_t(t) * n_st + q) * 3;
-    const float du[3] = {float(uq[0] - up[0]), float(uq[1] - up[1]), float(uq[2] - up[2])};
     const double f = freq[ch0 + ch];
-    const float f2c = float(f * f / (kSpeedOfLight * kSpeedOfLight));
-    const float width = freq_smear ? float(chan_width[ch0 + ch]) : 0.0f;
-    const float2* row = table + (int64_t(t) * n_chb + ch) * n_comp * int64_t(n_st);
+    const double f2c = f * f * (1.0 / (kSpeedOfLight * kSpeedOfLight));
+    const double width = freq_smear ? chan_width[ch0 + ch] : 0.0;
+    const double2* row = table + (int64_t(t) * n_chb + ch) * n_comp * int64_t(n_st);
+    const double* phase_t = st_phase + int64_t(t) * n_comp * n_st;
+    const double* rot_t = st_rot + int64_t(t) * n_comp * n_st;
     for (int c = c0; c < c1; ++c) {
-      const float2 sp = row[int64_t(c) * n_st + p];
-      const float2 sq = row[int64_t(c) * n_st + q];
+      const double2 sp = row[int64_t(c) * n_st + p];
+      const double2 sq = row[int64_t(c) * n_st + q];
       // shift_q * conj(shift_p)
-      const float2 ph = make_float2(sp.x * sq.x + sp.y * sq.y, sp.x * sq.y - sq.x * sp.y);
+      const double2 ph = make_double2(sp.x * sq.x + sp.y * sq.y, sp.x * sq.y - sq.x * sp.y);
       const CompGeom& g = geom[c];
-      float amp = 1.0f;
+      double amp = 1.0;
       if (g.is_gaussian) {
-        const float u = g.rot[0] * du[0] + g.rot[1] * du[1] + g.rot[2] * du[2];
-        const float vv = g.rot[3] * du[0] + g.rot[4] * du[1] + g.rot[5] * du[2];
-        const float up_ = g.u_scale * (u * g.cos_phi - vv * g.sin_phi);
-        const float vp_ = g.v_scale * (u * g.sin_phi + vv * g.cos_phi);
-        const float uv = float(-2.0 * M_PI * M_PI) * (up_ * up_ + vp_ * vp_);
-        amp = expf(f2c * uv);
+        // station UVW rotated to the source (identity unless absolute PA), then
+        // the baseline's (u, v) difference (Simulator.cc:312-342)
+        double uvq[2], uvp[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          uvq[i] = g.rot[i * 3] * uq[0] + g.rot[i * 3 + 1] * uq[1] + g.rot[i * 3 + 2] * uq[2];
+          uvp[i] = g.rot[i * 3] * up[0] + g.rot[i * 3 + 1] * up[1] + g.rot[i * 3 + 2] * up[2];
+        }
+        const double u = uvq[0] - uvp[0];
+        const double vv = uvq[1] - uvp[1];
+        const double up_ = g.u_scale * (u * g.cos_phi - vv * g.sin_phi);
+        const double vp_ = g.v_scale * (u * g.sin_phi + vv * g.cos_phi);
+        const double uv = (-2.0 * M_PI * M_PI) * (up_ * up_ + vp_ * vp_);
+        amp = exp(f2c * uv);
       }
-      if (freq_smear) {
-        const double dphi = st_phase[(int64_t(t) * n_comp + c) * n_st + q] -
-                            st_phase[(int64_t(t) * n_comp + c) * n_st + p];
-        amp *= Sinc(float(0.5 * dphi * width));
-      }
-      const float4 s = spec[int64_t(c) * n_ch_total + ch0 + ch];
-      const float2 pa = make_float2(amp * ph.x, amp * ph.y);
+      if (time_smear)
+        amp *= double(Sinc(float(0.5 * (rot_t[int64_t(c) * n_st + q] -
+                                        rot_t[int64_t(c) * n_st + p]) * f)));
+      if (freq_smear)
+        amp *= double(Sinc(float(0.5 * (phase_t[int64_t(c) * n_st + q] -
+                                        phase_t[int64_t(c) * n_st + p]) * width)));
+      const Stokes4 s = spec[int64_t(c) * n_ch_total + ch0 + ch];
       if (stokes_i_only) {
-        v[0].x += pa.x * s.x;
-        v[0].y += pa.y * s.x;
+        v[0].x += amp * (ph.x * s.I);
+        v[0].y += amp * (ph.y * s.I);
       } else {
         // XX = I + Q, XY = U + iV, YX = U - iV, YY = I - Q
-        const float2 xx = make_float2(s.x + s.y, 0.0f);
-        const float2 xy = make_float2(s.z, s.w);
-        const float2 yx = make_float2(s.z, -s.w);
-        const float2 yy = make_float2(s.x - s.y, 0.0f);
-        const float2 sv[4] = {xx, xy, yx, yy};
+        const double2 sv[4] = {make_double2(s.I + s.Q, 0.0), make_double2(s.U, s.V),
+                               make_double2(s.U, -s.V), make_double2(s.I - s.Q, 0.0)};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          v[k].x += pa.x * sv[k].x - pa.y * sv[k].y;
-          v[k].y += pa.x * sv[k].y + pa.y * sv[k].x;
+          v[k].x += amp * (ph.x * sv[k].x - ph.y * sv[k].y);
+          v[k].y += amp * (ph.x * sv[k].y + ph.y * sv[k].x);
         }
       }
     }
     if (stokes_i_only) v[3] = v[0];
   }
   // correlations stored: Duo (XX, YY) or Full (XX, XY, YX, YY)
-  float2 out[NC];
+  double2 out[NC];
   if constexpr (NC == 2) {
     out[0] = v[0];
     out[1] = v[3];
@@ -262,26 +291,24 @@ __global__ void BaselinePredictKernel(const float2* __restrict__ table,
     float4* o = reinterpret_cast<float4*>(plane) + out_idx * (NC / 2);
 #pragma unroll
     for (int h = 0; h < NC / 2; ++h)
-      o[h] = make_float4(out[2 * h].x, out[2 * h].y, out[2 * h + 1].x, out[2 * h + 1].y);
+      o[h] = make_float4(float(out[2 * h].x), float(out[2 * h].y), float(out[2 * h + 1].x),
+                         float(out[2 * h + 1].y));
   } else {
-    const float inv = 1.0f / *scale;
+    const double inv = 1.0 / double(*scale);
+    uint32_t bits[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) bits[k] = HalfBits(float(out[k].x * inv), float(out[k].y * inv));
     if constexpr (NC == 2) {
-      reinterpret_cast<uint2*>(plane)[out_idx] =
-          make_uint2(HalfBits(out[0].x * inv, out[0].y * inv),
-                     HalfBits(out[1].x * inv, out[1].y * inv));
+      reinterpret_cast<uint2*>(plane)[out_idx] = make_uint2(bits[0], bits[1]);
     } else {
-      reinterpret_cast<uint4*>(plane)[out_idx] =
-          make_uint4(HalfBits(out[0].x * inv, out[0].y * inv),
-                     HalfBits(out[1].x * inv, out[1].y * inv),
-                     HalfBits(out[2].x * inv, out[2].y * inv),
-                     HalfBits(out[3].x * inv, out[3].y * inv));
+      reinterpret_cast<uint4*>(plane)[out_idx] = make_uint4(bits[0], bits[1], bits[2], bits[3]);
     }
   }
 }
 
 // FP16 scale bound per direction: max over channels of sum_c |S_c| (every
 // model value of the direction is bounded by it); largest maps to 2^14.
-__global__ void DirectionScaleKernel(const float4* spec, const CompGeom* geom, int n_comp,
+__global__ void DirectionScaleKernel(const Stokes4* spec, const CompGeom* geom, int n_comp,
                                      int n_ch_total, int n_dir, float* bound) {
   const int d = blockIdx.x;
   __shared__ float red[256];
@@ -290,8 +317,8 @@ __global__ void DirectionScaleKernel(const float4* spec, const CompGeom* geom, i
     float acc = 0.0f;
     for (int c = 0; c < n_comp; ++c) {
       if (geom[c].direction != d) continue;
-      const float4 s = spec[int64_t(c) * n_ch_total + ch];
-      acc += fabsf(s.x) + fabsf(s.y) + fabsf(s.z) + fabsf(s.w);
+      const Stokes4 s = spec[int64_t(c) * n_ch_total + ch];
+      acc += float(fabs(s.I) + fabs(s.Q) + fabs(s.U) + fabs(s.V));
     }
     m = fmaxf(m, acc);
   }
@@ -420,7 +447,7 @@ extern "C" int ddh_predict(ddh_problem* problem, double ref_ra, double ref_dec,
                            const double* station_uvw, int32_t n_stations, const double* freq,
                            const double* chan_width, const ddh_component* components,
                            int32_t n_components, int32_t stokes_i_only,
-                           int32_t correct_freq_smearing) {
+                           int32_t correct_freq_smearing, const double* scaled_ncp_uvw) {
   return Guard([&] {
     DDH_REQUIRE(problem && station_uvw && freq && components, -1, "NULL argument");
     DDH_REQUIRE(n_components >= 1 && n_stations >= 1, -1, "empty predict");
@@ -447,9 +474,9 @@ extern "C" int ddh_predict(ddh_problem* problem, double ref_ra, double ref_dec,
     hipStream_t s = v.stream;
     static thread_local Scratch<ddh_component> d_comps;
     static thread_local Scratch<CompGeom> d_geom;
-    static thread_local Scratch<float4> d_spec;
-    static thread_local Scratch<double> d_uvw, d_freq, d_width, d_phase;
-    static thread_local Scratch<float2> d_table;
+    static thread_local Scratch<Stokes4> d_spec;
+    static thread_local Scratch<double> d_uvw, d_freq, d_width, d_phase, d_rot;
+    static thread_local Scratch<double2> d_table;
     ddh_component* dc = d_comps.Get(comps.size());
     DDH_HIP_CHECK(hipMemcpyAsync(dc, comps.data(), comps.size() * sizeof(ddh_component),
                                  hipMemcpyHostToDevice, s));
@@ -465,7 +492,7 @@ extern "C" int ddh_predict(ddh_problem* problem, double ref_ra, double ref_dec,
     CompGeom* dg = d_geom.Get(comps.size());
     hipLaunchKernelGGL(ComponentGeometryKernel, dim3((n_components + 127) / 128), dim3(128), 0, s,
                        dc, n_components, ref_ra, ref_dec, dg);
-    float4* dspec = d_spec.Get(size_t(n_components) * n_ch_total);
+    Stokes4* dspec = d_spec.Get(size_t(n_components) * n_ch_total);
     const int n_spec = n_components * n_ch_total;
     hipLaunchKernelGGL(SpectrumKernel, dim3((n_spec + 255) / 256), dim3(256), 0, s, dc,
                        n_components, df, n_ch_total, dspec);
@@ -481,13 +508,18 @@ extern "C" int ddh_predict(ddh_problem* problem, double ref_ra, double ref_dec,
                                      hipMemcpyDeviceToDevice, s));
     }
     double* dphase = d_phase.Get(size_t(v.n_t) * n_components * n_stations);
+    double* drot = d_rot.Get(size_t(v.n_t) * n_components * n_stations);
+    const int time_smear = scaled_ncp_uvw ? 1 : 0;
+    const double ncp_v = scaled_ncp_uvw ? scaled_ncp_uvw[1] : 0.0;
+    const double ncp_w = scaled_ncp_uvw ? scaled_ncp_uvw[2] : 0.0;
     int ch0 = 0;
     for (int cb = 0; cb < v.n_cb; ++cb) {
       const int n_chb = v.ch_per_cb[cb];
       const int64_t n_tab = int64_t(v.n_t) * n_chb * n_components * n_stations;
-      float2* tab = d_table.Get(size_t(n_tab));
+      double2* tab = d_table.Get(size_t(n_tab));
       hipLaunchKernelGGL(StationPhasorKernel, dim3(unsigned((n_tab + 255) / 256)), dim3(256), 0, s,
-                         du, n_stations, dg, n_components, df, ch0, n_chb, v.n_t, tab, dphase);
+                         du, n_stations, dg, n_components, df, ch0, n_chb, v.n_t, tab, dphase,
+                         drot, ncp_v, ncp_w);
       DDH_HIP_CHECK(hipGetLastError());
       const size_t plane = size_t(v.n_t) * n_chb * v.bl_stride;
       const size_t vis_bytes = v.f16 ? size_t(v.n_corr) * 4 : size_t(v.n_corr) * 8;
@@ -498,10 +530,10 @@ extern "C" int ddh_predict(ddh_problem* problem, double ref_ra, double ref_dec,
         const dim3 grid((v.bl_stride + 255) / 256, n_chb, v.n_t);
         const int c0 = c_off[d], c1 = c_off[d + 1];
 #define DDH_PREDICT_LAUNCH(NC, F16)                                                            \
-  hipLaunchKernelGGL((BaselinePredictKernel<NC, F16>), grid, dim3(256), 0, s, tab, dphase, dg, \
-                     dspec, du, v.ant1, v.ant2, df, dw, v.n_bl, v.bl_stride, n_stations,       \
+  hipLaunchKernelGGL((BaselinePredictKernel<NC, F16>), grid, dim3(256), 0, s, tab, dphase, drot, \
+                     dg, dspec, du, v.ant1, v.ant2, df, dw, v.n_bl, v.bl_stride, n_stations,   \
                      n_components, c0, c1, ch0, n_chb, n_ch_total, stokes_i_only,              \
-                     correct_freq_smearing, sc, out)
+                     correct_freq_smearing, time_smear, sc, out)
         if (v.n_corr == 2) {
           if (v.f16) DDH_PREDICT_LAUNCH(2, true); else DDH_PREDICT_LAUNCH(2, false);
         } else {

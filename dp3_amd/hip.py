@@ -119,7 +119,8 @@ SIGNATURES = {
     "ddh_set_schedule": ([C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32], C.c_int),
     "ddh_set_profiling": ([C.c_void_p, C.c_int32], C.c_int),
     "ddh_predict": ([C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_int32, C.c_void_p,
-                     C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+                     C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p],
+                    C.c_int),
     "ddh_problem_synthesize_data": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
                                      C.c_float, C.c_uint32], C.c_int),
     "ddh_calibrate_bandwidth": ([C.c_int64, C.c_int32, C.POINTER(C.c_double),
@@ -615,16 +616,20 @@ class DeviceProblem:
         _check(lib().ddh_problem_upload(self.handle, views), "ddh_problem_upload")
 
     def predict(self, ref, station_uvw, freq, chan_width, components, *, stokes_i_only=True,
-                correct_freq_smearing=False):
+                correct_freq_smearing=False, scaled_ncp_uvw=None):
         """Fills every model plane. station_uvw [n_t, n_st, 3], freq/chan_width
-        over all channels, components a ddh_component array."""
+        over all channels, components a ddh_component array; scaled_ncp_uvw
+        (3 values) turns on the time-smearing correction."""
         uvw = np.ascontiguousarray(station_uvw, dtype=np.float64)
         f = np.ascontiguousarray(freq, dtype=np.float64)
         w = np.ascontiguousarray(chan_width, dtype=np.float64)
+        ncp = None if scaled_ncp_uvw is None else np.ascontiguousarray(scaled_ncp_uvw, np.float64)
+        assert ncp is None or ncp.shape == (3,)
         assert uvw.shape[0] == self.n_times
         _check(lib().ddh_predict(self.handle, ref[0], ref[1], _ptr(uvw), uvw.shape[1], _ptr(f),
                                  _ptr(w), C.cast(components, C.c_void_p), len(components),
-                                 int(stokes_i_only), int(correct_freq_smearing)), "ddh_predict")
+                                 int(stokes_i_only), int(correct_freq_smearing),
+                                 None if ncp is None else _ptr(ncp)), "ddh_predict")
 
     def synthesize_data(self, gains, sol_index, noise_sigma, seed=0):
         """data = sum_d G_a M_d G_b^H + noise, on the device. gains [n_cb, n_ant*n_sub*2]."""

@@ -111,8 +111,9 @@ void ddh_solver_destroy(ddh_solver* solver);
 
 // Drop-in solve from host SolveData arrays. solutions[cb] points at the
 // channel block's [n_ant][n_sub][2] complex128 vector: initial guess in,
-// result out. `time` is passed to constraints in the reference; no
-// constraints are supported here, so it is unused.
+// result out. `time` is passed to constraints in the reference; the only
+// constraint applied here is the smoothness constraint
+// (ddh_solver_set_smoothness), which does not depend on it, so it is unused.
 int ddh_solve(ddh_solver* solver, const ddh_cb_view* channel_blocks,
               int32_t n_channel_blocks, double* const* solutions, double time,
               ddh_result* result);
@@ -200,12 +201,20 @@ typedef struct {
 // directions): station_uvw [n_times][n_stations][3] metres, freq and
 // chan_width [total channels] Hz. Antenna indices of the problem's baselines
 // index the stations. stokes_i_only writes XX = YY = I (XY = YX = 0).
+// correct_freq_smearing multiplies by sinc(0.5 (phi_q - phi_p) width)
+// (Simulator.cc:190-199). scaled_ncp_uvw (3 doubles, the north celestial
+// pole in UVW scaled by the earth's angular speed over one integration, as
+// FastPredict.cc:447-462 derives it; element 0 taken as zero, as
+// Simulator.cc:493-503 does) turns on the time-smearing factor
+// sinc(0.5 (rot_q - rot_p) f) of Simulator.cc:177-188 (the constructor of
+// Simulator.h:142-150); NULL leaves it off (the constructor of :117-122).
+// FP64 throughout, each direction's sum rounded once.
 // FP16 problems get a per-direction scale bound from the spectra.
 int ddh_predict(ddh_problem* problem, double ref_ra, double ref_dec,
                 const double* station_uvw, int32_t n_stations, const double* freq,
                 const double* chan_width, const ddh_component* components,
                 int32_t n_components, int32_t stokes_i_only,
-                int32_t correct_freq_smearing);
+                int32_t correct_freq_smearing, const double* scaled_ncp_uvw);
 
 // Writes the data planes of `problem` as sum_d Diag(g_a) M_d Diag(g_b)^H plus
 // complex Gaussian noise (noise_sigma on each of re and im), with
@@ -268,7 +277,9 @@ int ddh_chi2(ddh_solver* solver, ddh_problem* problem, double* const* solutions,
 // model storage) advance together, one launch per kernel per iteration over
 // all intervals' tiles, until every interval met its stopping criterion.
 // Replaces DDECal's serial loop over buffered solution intervals
-// (steps/DDECal.cc:595-708). Solvers and problems must outlive the batch.
+// (steps/DDECal.cc:595-708). Solvers must outlive the batch; a problem must
+// live while the batch is bound to it (a rebind to other problems releases
+// it: the batch keeps copies of the shapes it checks, not the problems).
 typedef struct ddh_batch ddh_batch;
 ddh_batch* ddh_batch_create(ddh_solver* const* solvers, ddh_problem* const* problems,
                             int32_t n_intervals);
