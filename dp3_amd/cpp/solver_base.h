@@ -3,8 +3,9 @@
 // with their reference defaults (SolverBase.cc:27-39), Initialize
 // (SolverBase.cc:41-61), the Solve overloads per data kind (SolverBase.h:
 // 205-227, std::logic_error for unsupported kinds) and SolveResult.
-// Constraints are not supported: the device loop runs without the per-
-// iteration host round trip they would need.
+// Of DP3's constraints, the smoothness constraint is applied on the device
+// after every step (IterativeDiagonalSolverHip::SetSmoothness); the others
+// would need a per-iteration host round trip and are not offered here.
 #ifndef DP3AMD_SOLVER_BASE_H
 #define DP3AMD_SOLVER_BASE_H
 

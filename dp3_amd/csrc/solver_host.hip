@@ -97,6 +97,12 @@ struct ddh_problem {
 struct ddh_solver {
   ddh::SolverConfig cfg;
   hipStream_t stream = nullptr;
+  // Bumped whenever something a batch copies out of the solver at build time
+  // changes: a device buffer's address (reallocation for another problem
+  // shape, constraint weights of another size) or the smoothness settings.
+  // A batch compares it with the version it was built from and rebuilds its
+  // arguments, launch plan and graph when they differ.
+  uint64_t version = 0;
   int waves = 4;
   int unroll = 1;
   int use_graph = 0;
@@ -527,7 +533,13 @@ void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
   const size_t sol_per_cb = size_t(c.n_antennas) * s.cfg.n_sub * DevicePols(c);
   const size_t n_sol = sol_per_cb * c.n_channel_blocks;
   const size_t n_wg = size_t(c.n_channel_blocks) * c.n_directions;
-  const void* before[4] = {s.sol0.ptr, s.partials.ptr, s.gram.ptr, s.fj_products.ptr};
+  auto addresses = [&] {
+    return std::vector<const void*>{s.sol0.ptr,      s.sol1.ptr,      s.mean.ptr,
+                                    s.wg_conv.ptr,   s.wg_conv_n.ptr, s.wg_mean.ptr,
+                                    s.wg_mean_n.ptr, s.step_mags.ptr, s.partials.ptr,
+                                    s.state.ptr,     s.gram.ptr,      s.fj_products.ptr};
+  };
+  const std::vector<const void*> before = addresses();
   s.sol0.Resize(n_sol);
   s.sol1.Resize(n_sol);
   s.mean.Resize(c.n_channel_blocks);
@@ -544,9 +556,10 @@ void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
     s.gram.Resize(size_t(p.n_cb) * p.n_seg * 2 * GramTerms(p.n_dir) * p.bl_stride);
   if (c.n_solution_polarizations == 4)
     s.fj_products.Resize(size_t(p.n_cb) * p.n_seg * p.n_dir * kFullJonesModelTerms * p.bl_stride);
-  if (before[0] != s.sol0.ptr || before[1] != s.partials.ptr || before[2] != s.gram.ptr ||
-      before[3] != s.fj_products.ptr)
+  if (addresses() != before) {
     DestroyGraph(s);
+    ++s.version;
+  }
 }
 
 // Solutions cross the C-ABI in the caller's layout: per (antenna,
@@ -1025,7 +1038,10 @@ int ddh_solver_last_timing(const ddh_solver* s, double* correlate_ms, int64_t* l
 static void UploadConstraintWeights(ddh_solver& s, const std::vector<double>& w, bool per_sub) {
   const double* before = s.sm_weights.ptr;
   s.sm_weights.Upload(w, s.stream);
-  if (s.sm_weights.ptr != before || s.sm_sub_weights != per_sub) DestroyGraph(s);
+  if (s.sm_weights.ptr != before || s.sm_sub_weights != per_sub) {
+    DestroyGraph(s);
+    ++s.version;
+  }
   s.sm_sub_weights = per_sub;
   DDH_HIP_CHECK(hipStreamSynchronize(s.stream));
 }
@@ -1037,6 +1053,7 @@ int ddh_solver_set_smoothness(ddh_solver* s, const double* channel_block_frequen
   return Guard([&] {
     DDH_REQUIRE(s, -1, "NULL solver");
     DestroyGraph(*s);
+    ++s->version;
     if (!channel_block_frequencies) {
       s->sm_on = false;
       return;
@@ -1138,6 +1155,21 @@ int ddh_set_profiling(ddh_solver* s, int32_t enable) {
 struct ddh_batch {
   std::vector<ddh_solver*> solvers;
   std::vector<ddh_problem*> problems;
+  // Shape of each interval's problem at build time: a rebind checks new
+  // problems against these copies, so the batch never reads a problem it has
+  // been moved away from (the caller may release it).
+  struct Shape {
+    int n_t, n_bl, n_cb, n_dir, n_corr, bl_stride;
+    bool f16;
+    std::vector<int32_t> ch_per_cb, seg_t0, seg_sol, ant1, ant2;
+    bool operator==(const Shape& o) const {
+      return n_t == o.n_t && n_bl == o.n_bl && n_cb == o.n_cb && n_dir == o.n_dir &&
+             n_corr == o.n_corr && bl_stride == o.bl_stride && f16 == o.f16 &&
+             ch_per_cb == o.ch_per_cb && seg_t0 == o.seg_t0 && seg_sol == o.seg_sol &&
+             ant1 == o.ant1 && ant2 == o.ant2;
+    }
+  };
+  std::vector<Shape> shapes;
   hipStream_t stream = nullptr;
   int n_corr = 2;
   bool f16 = false;
@@ -1148,6 +1180,7 @@ struct ddh_batch {
   size_t solve_lds = 0;
   int solve_threads = 0;
   std::vector<ddh::IterArgs> args;
+  std::vector<uint64_t> versions;      // member solver versions the arguments were built from
   std::vector<ddh::TileGroup> groups;  // tiles of one direction count each
   ddh::SmoothLaunch smooth{};          // smoothness constraint launches (none: 0 intervals)
   std::vector<ddh::BatchTile> tiles, items, pro_items;
@@ -1180,8 +1213,15 @@ void DestroyBatchGraph(ddh_batch& b) {
   b.graph_exec = nullptr;
 }
 
+ddh_batch::Shape ShapeOf(const ddh_problem& p) {
+  return ddh_batch::Shape{p.n_t,   p.n_bl,     p.n_cb,   p.n_dir,   p.n_corr, p.bl_stride,
+                          p.f16,   p.ch_per_cb, p.seg_t0, p.seg_sol, p.ant1,   p.ant2};
+}
+
 void BuildBatch(ddh_batch& b) {
   const int n = int(b.solvers.size());
+  b.shapes.clear();
+  for (const ddh_problem* p : b.problems) b.shapes.push_back(ShapeOf(*p));
   b.args.clear();
   b.tiles.clear();
   b.items.clear();
@@ -1248,6 +1288,8 @@ void BuildBatch(ddh_batch& b) {
     b.smooth.finish_threads =
         std::max(b.smooth.finish_threads, SmoothFinishThreads(a.n_ant, a.max_nsd));
   }
+  b.versions.clear();
+  for (const ddh_solver* s : b.solvers) b.versions.push_back(s->version);
   b.d_args.Upload(b.args, b.stream);
   b.d_tiles.Upload(b.tiles, b.stream);
   b.d_items.Upload(b.items, b.stream);
@@ -1255,6 +1297,17 @@ void BuildBatch(ddh_batch& b) {
   if (b.full_jones) b.d_prod_tiles.Upload(b.prod_tiles, b.stream);
   DDH_HIP_CHECK(hipStreamSynchronize(b.stream));
   DestroyBatchGraph(b);
+}
+
+// Rebuilds the batch when a member solver changed since the build (new
+// buffers, constraint settings or weights of another size): the arguments,
+// launch plan and captured graph hold those by value.
+void RefreshBatch(ddh_batch& b) {
+  for (size_t i = 0; i < b.solvers.size(); ++i)
+    if (b.solvers[i]->version != b.versions[i]) {
+      BuildBatch(b);
+      return;
+    }
 }
 
 // One batched iteration over `args` (the batch's arguments or a variant of
@@ -1335,6 +1388,7 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
               bool cross_products);
 
 void SolveBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results) {
+  RefreshBatch(b);
   UploadSolutions(b, solutions, [&](int i) { return b.args[i].sol0; });
   b.last_warm_ms = 0.0;
   RunBatch(b, solutions, results, true);
@@ -1353,6 +1407,7 @@ void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* cons
                     ddh_result* results) {
   DDH_REQUIRE(!b.full_jones, -4, "the learned warm start serves the scalar and diagonal solvers");
   CheckWarmNetwork(net);
+  RefreshBatch(b);
   const int n = int(b.solvers.size());
   std::vector<size_t> off(n + 1, 0);
   for (int i = 0; i < n; ++i) {
@@ -1630,20 +1685,26 @@ int ddh_batch_rebind(ddh_batch* b, ddh_problem* const* problems) {
     DDH_REQUIRE(b && problems, -1, "NULL argument");
     const int n = int(b->solvers.size());
     for (int i = 0; i < n; ++i) {
-      ddh_problem* p = problems[i];
-      const ddh_problem* o = b->problems[i];
-      DDH_REQUIRE(p, -1, "NULL problem");
-      DDH_REQUIRE(p->n_t == o->n_t && p->n_bl == o->n_bl && p->n_cb == o->n_cb &&
-                      p->n_dir == o->n_dir && p->n_corr == o->n_corr && p->f16 == o->f16 &&
-                      p->bl_stride == o->bl_stride && p->ch_per_cb == o->ch_per_cb &&
-                      p->seg_t0 == o->seg_t0 && p->seg_sol == o->seg_sol,
-                  -2, "rebinding needs problems of the same shapes");
+      DDH_REQUIRE(problems[i], -1, "NULL problem");
+      DDH_REQUIRE(ShapeOf(*problems[i]) == b->shapes[i], -2,
+                  "rebinding needs problems of the same shapes (baselines, channels per "
+                  "block, time steps, directions, solution intervals, storage)");
+    }
+    bool changed = false;
+    for (int i = 0; i < n; ++i) {
+      CheckMatch(*b->solvers[i], *problems[i]);
+      PrepareBuffers(*b->solvers[i], *problems[i]);
+      changed = changed || b->solvers[i]->version != b->versions[i];
+    }
+    if (changed) {
+      // a member solver changed since the build: full rebuild (new plan and graph)
+      for (int i = 0; i < n; ++i) b->problems[i] = problems[i];
+      BuildBatch(*b);
+      return;
     }
     std::vector<IterArgs> args(n);
     for (int i = 0; i < n; ++i) {
       ddh_solver& s = *b->solvers[i];
-      CheckMatch(s, *problems[i]);
-      PrepareBuffers(s, *problems[i]);
       args[i] = MakeArgs(s, *problems[i]);
       args[i].state = b->args[i].state;
       DDH_REQUIRE(AntennaSolveSharedBytes(args[i], b->n_corr) <= b->solve_lds, -2,
@@ -1704,6 +1765,7 @@ int ddh_batch_last_timing(const ddh_batch* b, double* correlate_ms, double* solv
 int ddh_batch_autotune(ddh_batch* b, int32_t* waves_out, int32_t* rows_out, double* best_ms) {
   return Guard([&] {
     DDH_REQUIRE(b, -1, "NULL batch");
+    RefreshBatch(*b);
     // unit gains and a reset state make every interval live for the timing
     for (size_t i = 0; i < b->solvers.size(); ++i) {
       const ddh_solver& s = *b->solvers[i];

@@ -1187,8 +1187,10 @@ __device__ __forceinline__ void DecideBlock(const IterArgs& a) {
   if (threadIdx.x == 0) {
     DeviceState st = *a.state;
     const int it = st.iteration + 1;
-    // AssignSolutions (SolverBase.cc:296-306); no constraints are attached, so
-    // constraints are satisfied and the plain accuracy applies.
+    // AssignSolutions (SolverBase.cc:296-306). The only constraint the device
+    // applies is the smoothness constraint, which is always satisfied
+    // (Constraint::Satisfied), so the plain accuracy applies; the solve time
+    // is unused because smoothing does not depend on it.
     const double step_magnitude = count == 0 ? 0.0 : sum / a.step_size / count;
     a.step_mags[it - 1] = step_magnitude;
     const bool converged = step_magnitude <= a.accuracy;

@@ -97,14 +97,16 @@ __global__ void __launch_bounds__(256)
     if (subtract) {
       float4_v* dp = data + size_t(v) * 2;
       float4_v d0 = dp[0], d1 = dp[1];
-      d0.x -= float(sum[0].re);
-      d0.y -= float(sum[0].im);
-      d0.z -= float(sum[1].re);
-      d0.w -= float(sum[1].im);
-      d1.x -= float(sum[2].re);
-      d1.y -= float(sum[2].im);
-      d1.z -= float(sum[3].re);
-      d1.w -= float(sum[3].im);
+      // complex<float> -= complex<double> (DDECal.cc:1070): the difference is
+      // taken in double and rounded once
+      d0.x = float(double(d0.x) - sum[0].re);
+      d0.y = float(double(d0.y) - sum[0].im);
+      d0.z = float(double(d0.z) - sum[1].re);
+      d0.w = float(double(d0.w) - sum[1].im);
+      d1.x = float(double(d1.x) - sum[2].re);
+      d1.y = float(double(d1.y) - sum[2].im);
+      d1.z = float(double(d1.z) - sum[3].re);
+      d1.w = float(double(d1.w) - sum[3].im);
       dp[0] = d0;
       dp[1] = d1;
     }

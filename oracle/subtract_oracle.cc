@@ -81,7 +81,9 @@ extern "C" void orc_correct_and_subtract(int32_t n_t, int32_t n_bl, int32_t n_ch
         }
         if (subtract) {
           CF* dv = d + (((size_t(t) * n_bl + bl) * n_ch + ch) * 4);
-          for (int c = 0; c < 4; ++c) dv[c] -= CF(sum[c]);
+          // complex<float> -= complex<double> (DDECal.cc:1070): promoted to
+          // double, subtracted, rounded once
+          for (int c = 0; c < 4; ++c) dv[c] = CF(CD(dv[c]) - sum[c]);
         }
       }
     }

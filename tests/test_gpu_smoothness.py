@@ -168,6 +168,54 @@ def test_smoothness_batch_equals_single_solves(small):
         s.close()
 
 
+def test_batch_sees_constraint_changes_made_after_creation(small):
+    """A batch built from unconstrained solvers whose solvers then get the
+    smoothness constraint, first with plain weights and then with
+    per-sub-solution weights (a larger weights buffer), gives the single
+    constrained solves' results bit for bit, through solve and rebind."""
+    w, geom, sky, gains, P = small
+    print("case: batch after constraint changes", flush=True)
+    rng = np.random.default_rng(8)
+    cs = [constraint(P, w.n_channels, rng, sub_weights=True) for _ in range(2)]
+    singles = []
+    for c in cs:
+        s = gpu_solver(P, c, max_iterations=w.max_iterations, accuracy=w.tolerance)
+        s.set_schedule(waves_per_tile=2, rows_per_trip=1, use_graph=0)
+        sol = P.unit_solutions()
+        r = s.solve(P, sol)
+        singles.append((sol, r.iterations))
+        s.close()
+    solvers = [H.Solver(P.n_antennas, P.n_solutions_per_direction, P.n_channel_blocks,
+                        max_iterations=w.max_iterations, accuracy=w.tolerance) for _ in cs]
+    problems = [H.DeviceProblem.from_problem(s, P) for s in solvers]
+    for dp in problems:
+        dp.upload(P)
+    b = H.Batch(solvers, problems)
+    b.set_schedule(waves_per_tile=2, rows_per_trip=1, use_graph=1, poll_interval=4)
+    plain = [P.unit_solutions() for _ in cs]
+    b.solve(plain)  # builds and captures the unconstrained graph
+    for s, c in zip(solvers, cs):
+        s.set_smoothness(c.frequencies, c.bandwidth_hz, c.bandwidth_ref_frequency_hz,
+                         c.spectral_exponent, c.kernel_truncation, c.antenna_factors,
+                         c.dd_factors)
+        s.set_constraint_weights(np.ones((P.n_antennas, P.n_channel_blocks)))
+        s.set_sub_solution_weights(c.sub_solution_weights)
+    for step in ("solve", "rebind"):
+        if step == "rebind":
+            b.rebind(problems)
+        sols = [P.unit_solutions() for _ in cs]
+        res = b.solve(sols)
+        for (sol, it), got, r in zip(singles, sols, res):
+            assert r.iterations == it, step
+            np.testing.assert_array_equal(got, sol)
+    assert not np.array_equal(plain[0], singles[0][0])  # the constraint acted
+    b.close()
+    for dp in problems:
+        dp.close()
+    for s in solvers:
+        s.close()
+
+
 def test_smoothness_off_restores_plain_solve(small):
     w, geom, sky, gains, P = small
     print("case: smoothness off", flush=True)

@@ -364,6 +364,13 @@ def _subtract_case(n_pol, seed=3):
     return data, model, sol, a1, a2, cbs, n_ant
 
 
+def one_rounding(got, ref, slack=1e-12):
+    """complex64 `got` is `ref` (complex128) rounded once: within half an ulp
+    (2^-24 relative) per real and imaginary part, plus FP64-level slack."""
+    for g, r in ((got.real, ref.real), (got.imag, ref.imag)):
+        assert np.all(np.abs(g.astype(np.float64) - r) <= 2.0 ** -24 * np.abs(r) + slack)
+
+
 @pytest.mark.parametrize("n_pol", [1, 2, 4])
 def test_correct_and_subtract_against_matrix_products(n_pol):
     """The oracle equals sum_d S1 M_d S2^H per visibility, with S the scalar,
@@ -391,11 +398,11 @@ def test_correct_and_subtract_against_matrix_products(n_pol):
                     s1 = mat(sol[cb_of[c], a1[b], d])
                     s2 = mat(sol[cb_of[c], a2[b], d])
                     corr = s1 @ model[d, t, b, c].astype(np.complex128).reshape(2, 2) @ s2.conj().T
-                    np.testing.assert_allclose(m_out[d, t, b, c], corr.ravel(), rtol=2e-6,
-                                               atol=2e-6)
+                    one_rounding(m_out[d, t, b, c], corr.ravel())
                     tot += corr
                 ref_d[t, b, c] -= tot.ravel()
-    np.testing.assert_allclose(d_out, ref_d, rtol=0, atol=2e-5)
+    # FP64 products, sum and difference, rounded to complex64 once
+    one_rounding(d_out, ref_d)
 
 
 def test_correct_and_subtract_flags():
