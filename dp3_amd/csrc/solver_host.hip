@@ -79,6 +79,13 @@ struct ddh_problem {
   ddh::DeviceBuffer<float> scale;
   ddh::DeviceBuffer<int32_t> d_ant1, d_ant2, d_csr_off, d_csr, d_seg_t0, d_seg_sol;
   ddh::DeviceBuffer<ddh::CbInfo> d_cbinfo;
+  // Walk order of the fused Gram iteration (at most 64 antennas, each
+  // antenna pair at most once): walk_rows = floor(n_ant / 2) rows of 64
+  // slots, slot (row, lane) = the baseline {lane, (lane + row + 1) mod n_ant};
+  // walk_rows = 0 when the baselines admit no walk.
+  int walk_rows = 0;
+  std::vector<int32_t> walk_code, walk_bl;
+  ddh::DeviceBuffer<int32_t> d_walk_code, d_walk_bl;
   ddh::DeviceBuffer<float> staging;
   ddh::DeviceBuffer<uint32_t> max_bits;
   // raw-buffer ingest: the DPBuffer baseline list the map was built for,
@@ -107,9 +114,13 @@ struct ddh_solver {
   int unroll = 1;
   int use_graph = 0;
   int poll = 8;
+  int fuse = 1;       // fused Gram iteration where the problem allows it
+  int gs_groups = 2;  // its row groups per polarization (ddh_set_gram_solve_groups)
   // per-solve device buffers
   ddh::DeviceBuffer<double2> sol0, sol1;
-  ddh::DeviceBuffer<float2> gram;  // Gram iteration cross products
+  ddh::DeviceBuffer<float2> gram;       // Gram iteration cross products
+  ddh::DeviceBuffer<float2> gram_walk;  // the same in walk order (fused iteration)
+  ddh::DeviceBuffer<double> gs_part;     // fused iteration: per-workgroup sums
   ddh::DeviceBuffer<double> mean, wg_conv, wg_mean, step_mags;
   ddh::DeviceBuffer<int32_t> wg_conv_n, wg_mean_n;
   ddh::DeviceBuffer<float> partials;
@@ -213,6 +224,42 @@ void BuildSegments(ddh_problem& p, const uint32_t* sol_index, const SolverConfig
   p.n_seg = int(p.seg_t0.size()) - 1;
 }
 
+// The ring walk of the fused Gram iteration over the problem's baselines:
+// step s = 1 .. n/2 pairs lane a with antenna (a + s) mod n (for even n at
+// s = n/2 only lanes a < n/2), which visits every antenna pair once. A slot
+// records the baseline of its pair and whether the lane's antenna is the
+// baseline's first (1) or second (2). Left empty (walk_rows 0) for more than
+// 64 antennas or when a pair has two baselines.
+void BuildWalk(ddh_problem& p) {
+  p.walk_rows = 0;
+  p.walk_code.clear();
+  p.walk_bl.clear();
+  const int n = p.n_ant;
+  if (n < 2 || n > kWave) return;
+  std::vector<int32_t> pair_bl(size_t(n) * n, -1);
+  for (int b = 0; b < p.n_bl; ++b) {
+    const int lo = std::min(p.ant1[b], p.ant2[b]), hi = std::max(p.ant1[b], p.ant2[b]);
+    int32_t& e = pair_bl[size_t(lo) * n + hi];
+    if (e >= 0) return;  // two baselines of one pair
+    e = b;
+  }
+  const int rows = n / 2;
+  p.walk_code.assign(size_t(rows) * kWave, 0);
+  p.walk_bl.assign(size_t(rows) * kWave, -1);
+  for (int r = 0; r < rows; ++r) {
+    const int step = r + 1;
+    for (int a = 0; a < n; ++a) {
+      if (2 * step == n && a >= n / 2) continue;
+      const int q = (a + step) % n;
+      const int b = pair_bl[size_t(std::min(a, q)) * n + std::max(a, q)];
+      if (b < 0) continue;
+      p.walk_bl[size_t(r) * kWave + a] = b;
+      p.walk_code[size_t(r) * kWave + a] = p.ant1[b] == a ? 1 : 2;
+    }
+  }
+  p.walk_rows = rows;
+}
+
 ddh_problem* CreateProblem(const ddh_solver& s, int n_times, int n_baselines,
                            const int32_t* antenna1, const int32_t* antenna2,
                            const int32_t* channels_per_block, const uint32_t* sol_index) {
@@ -289,6 +336,11 @@ ddh_problem* CreateProblem(const ddh_solver& s, int n_times, int n_baselines,
   }
   p->d_csr_off.Upload(off, s.stream);
   p->d_csr.Upload(csr, s.stream);
+  BuildWalk(*p);
+  if (p->walk_rows > 0) {
+    p->d_walk_code.Upload(p->walk_code, s.stream);
+    p->d_walk_bl.Upload(p->walk_bl, s.stream);
+  }
   p->d_seg_t0.Upload(p->seg_t0, s.stream);
   p->d_seg_sol.Upload(p->seg_sol, s.stream);
   p->d_cbinfo.Upload(p->cbinfo, s.stream);
@@ -401,9 +453,33 @@ Structure DetectStructure(const ddh_cb_view* cbs, int n_cb, int n_dir) {
 // the full-Jones solver, else a pair (equal for the scalar solver).
 int DevicePols(const ddh_config& c) { return c.n_solution_polarizations == 4 ? 4 : 2; }
 
-IterArgs MakeArgs(ddh_solver& s, ddh_problem& p) {
+// The fused Gram iteration (apply and antenna solve in one kernel) serves
+// Gram solves of Duo data whose problem has a walk (at most 64 antennas, each
+// pair once) and one solution per direction in one time segment.
+bool FusedGram(const ddh_solver& s, const ddh_problem& p) {
+  const ddh_config& c = s.cfg.c;
+  if (!s.fuse || c.iteration != DDH_ITERATION_GRAM || c.n_solution_polarizations == 4) return false;
+  if (p.n_corr != 2 || p.walk_rows == 0 || p.n_seg != 1 || p.n_dir > kMaxGramDirections)
+    return false;
+  for (int v : s.cfg.n_sol_dir)
+    if (v != 1) return false;
+  return true;
+}
+
+// fused: -1 decides by FusedGram; 0 / 1 forces (a batch uses one path for all
+// of its intervals).
+IterArgs MakeArgs(ddh_solver& s, ddh_problem& p, int fused = -1) {
   const ddh_config& c = s.cfg.c;
   IterArgs a{};
+  if (fused < 0) fused = FusedGram(s, p) ? 1 : 0;
+  if (fused) {
+    a.gram_walk = s.gram_walk.ptr;
+    a.walk_code = p.d_walk_code.ptr;
+    a.walk_bl = p.d_walk_bl.ptr;
+    a.walk_rows = p.walk_rows;
+    a.gs_groups = s.gs_groups;
+    a.gs_part = s.gs_part.ptr;
+  }
   a.data = p.data.ptr;
   a.model = p.model.ptr;
   a.model_scale = p.scale.ptr;
@@ -487,12 +563,23 @@ void LaunchCorrelation(const ddh_solver& s, const IterArgs& a, const ddh_problem
 void LaunchSolvePrologue(const ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
   LaunchPrologue(a, s.stream);
   if (GramMode(s)) LaunchGram(a, p.f16, s.waves, s.stream);
+  if (a.walk_rows > 0) LaunchGramPermute(a, s.stream);
   if (a.full_jones) LaunchFullJonesModelProducts(a, p.f16, s.waves, s.stream);
 }
 
-void LaunchIteration(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
+// Correlate + antenna solve of one iteration (the fused Gram iteration: one
+// kernel for both).
+void LaunchCorrelateAndSolve(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
+  if (a.walk_rows > 0) {
+    LaunchGramSolve(a, p.f16, s.stream);
+    return;
+  }
   LaunchCorrelation(s, a, p);
   LaunchAntennaSolve(a, p.n_corr, p.f16, s.stream);
+}
+
+void LaunchIteration(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
+  LaunchCorrelateAndSolve(s, a, p);
   LaunchSmoothing(a, s.stream);
   LaunchDecide(a, s.stream);
 }
@@ -501,7 +588,10 @@ void LaunchIteration(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
 // the solver buffers stay the same.
 void EnsureGraph(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
   const std::vector<const void*> key = {&p, p.data.ptr, p.model.ptr, s.sol0.ptr, s.partials.ptr,
-                                        s.gram.ptr, s.fj_products.ptr, s.sm_out.ptr,
+                                        s.gram.ptr, s.gram_walk.ptr,
+                                        reinterpret_cast<const void*>(intptr_t(a.walk_rows)),
+                                        reinterpret_cast<const void*>(intptr_t(a.gs_groups)),
+                                        s.fj_products.ptr, s.sm_out.ptr,
                                         s.sm_weights.ptr,
                                         reinterpret_cast<const void*>(intptr_t(s.poll)),
                                         reinterpret_cast<const void*>(intptr_t(s.waves)),
@@ -537,7 +627,8 @@ void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
     return std::vector<const void*>{s.sol0.ptr,      s.sol1.ptr,      s.mean.ptr,
                                     s.wg_conv.ptr,   s.wg_conv_n.ptr, s.wg_mean.ptr,
                                     s.wg_mean_n.ptr, s.step_mags.ptr, s.partials.ptr,
-                                    s.state.ptr,     s.gram.ptr,      s.fj_products.ptr};
+                                    s.state.ptr,     s.gram.ptr,      s.fj_products.ptr,
+                                    s.gram_walk.ptr, s.gs_part.ptr};
   };
   const std::vector<const void*> before = addresses();
   s.sol0.Resize(n_sol);
@@ -554,6 +645,10 @@ void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
   s.state.Resize(1);
   if (c.iteration == DDH_ITERATION_GRAM)
     s.gram.Resize(size_t(p.n_cb) * p.n_seg * 2 * GramTerms(p.n_dir) * p.bl_stride);
+  if (FusedGram(s, p)) {
+    s.gram_walk.Resize(size_t(p.n_cb) * 2 * GramTerms(p.n_dir) * p.walk_rows * kWave);
+    s.gs_part.Resize(size_t(p.n_cb) * 2 * kGramSolveMaxGroups * p.n_dir * 3 * kWave);
+  }
   if (c.n_solution_polarizations == 4)
     s.fj_products.Resize(size_t(p.n_cb) * p.n_seg * p.n_dir * kFullJonesModelTerms * p.bl_stride);
   if (addresses() != before) {
@@ -633,10 +728,17 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
       const int k = std::min(s.poll, limit - launched);
       for (int i = 0; i < k; ++i) {
         const int it = launched + i;
+        // events bracket the correlate (fused Gram iteration: apply and
+        // antenna solve in one kernel)
         if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it], s.stream));
-        LaunchCorrelation(s, a, p);
-        if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it + 1], s.stream));
-        LaunchAntennaSolve(a, p.n_corr, p.f16, s.stream);
+        if (a.walk_rows > 0) {
+          LaunchGramSolve(a, p.f16, s.stream);
+          if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it + 1], s.stream));
+        } else {
+          LaunchCorrelation(s, a, p);
+          if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it + 1], s.stream));
+          LaunchAntennaSolve(a, p.n_corr, p.f16, s.stream);
+        }
         LaunchSmoothing(a, s.stream);
         LaunchDecide(a, s.stream);
       }
@@ -1141,6 +1243,28 @@ int ddh_set_schedule(ddh_solver* s, int32_t waves_per_tile, int32_t rows_per_tri
   });
 }
 
+int ddh_set_gram_fusion(ddh_solver* s, int32_t enable) {
+  return Guard([&] {
+    DDH_REQUIRE(s, -1, "NULL solver");
+    if ((enable != 0) != (s->fuse != 0)) {
+      s->fuse = enable != 0;
+      DestroyGraph(*s);
+      ++s->version;
+    }
+  });
+}
+
+int ddh_set_gram_solve_groups(ddh_solver* s, int32_t groups) {
+  return Guard([&] {
+    DDH_REQUIRE(s, -1, "NULL solver");
+    DDH_REQUIRE(groups >= 1 && groups <= kGramSolveMaxGroups, -1, "row groups must be 1..8");
+    if (groups != s->gs_groups) {
+      s->gs_groups = groups;
+      DestroyGraph(*s);
+    }
+  });
+}
+
 int ddh_set_profiling(ddh_solver* s, int32_t enable) {
   if (!s) return -1;
   s->profile = enable;
@@ -1175,6 +1299,13 @@ struct ddh_batch {
   bool f16 = false;
   bool gram = false;  // Gram iteration
   bool full_jones = false;  // full-Jones intervals (all or none)
+  bool fused = false;       // fused Gram iteration for every interval
+  int gs_groups = 2;        // its row groups per polarization
+  int max_dir = 0;
+  // fused Gram iteration: items (interval, cb) grouped by direction count
+  std::vector<ddh::BatchTile> gs_items;
+  std::vector<ddh::TileGroup> gs_launch;  // gs_items split by direction count
+  ddh::DeviceBuffer<ddh::BatchTile> d_gs_items;
   int waves = 2, unroll = 1, use_graph = 1, poll = 8;
   int limit = 0;  // largest iteration limit over the intervals
   size_t solve_lds = 0;
@@ -1237,12 +1368,21 @@ void BuildBatch(ddh_batch& b) {
     double cost;
   };
   std::vector<Costed> costed;
+  b.fused = b.gram;
+  b.max_dir = 0;
+  for (int i = 0; i < n; ++i) {
+    CheckMatch(*b.solvers[i], *b.problems[i]);
+    PrepareBuffers(*b.solvers[i], *b.problems[i]);
+    b.fused = b.fused && FusedGram(*b.solvers[i], *b.problems[i]);
+    b.max_dir = std::max(b.max_dir, b.problems[i]->n_dir);
+  }
+  b.gs_items.clear();
+  b.gs_launch.clear();
   for (int i = 0; i < n; ++i) {
     ddh_solver& s = *b.solvers[i];
     ddh_problem& p = *b.problems[i];
-    CheckMatch(s, p);
-    PrepareBuffers(s, p);
-    IterArgs a = MakeArgs(s, p);
+    IterArgs a = MakeArgs(s, p, b.fused ? 1 : 0);
+    a.gs_groups = b.gs_groups;
     a.state = b.d_states.ptr + i;
     b.args.push_back(a);
     b.limit = std::max(b.limit, std::max(s.cfg.c.max_iterations, s.cfg.c.min_iterations));
@@ -1278,6 +1418,24 @@ void BuildBatch(ddh_batch& b) {
       b.groups.push_back(TileGroup{c.n_dir, int32_t(b.tiles.size()), 0});
     ++b.groups.back().count;
     b.tiles.push_back(c.t);
+  }
+  if (b.fused) {
+    // fused workgroups (interval, block, polarization x row group), one launch
+    // per direction count; a block's workgroups are adjacent in the list
+    std::vector<std::pair<int, BatchTile>> gs;
+    for (int i = 0; i < n; ++i)
+      for (int cb = 0; cb < b.problems[i]->n_cb; ++cb)
+        for (int part = 0; part < 2 * b.gs_groups; ++part)
+          gs.push_back({b.problems[i]->n_dir, BatchTile{i, cb, part, 0}});
+    std::stable_sort(gs.begin(), gs.end(),
+                     [](const auto& x, const auto& y) { return x.first > y.first; });
+    for (const auto& e : gs) {
+      if (b.gs_launch.empty() || b.gs_launch.back().n_dir != e.first)
+        b.gs_launch.push_back(TileGroup{e.first, int32_t(b.gs_items.size()), 0});
+      ++b.gs_launch.back().count;
+      b.gs_items.push_back(e.second);
+    }
+    b.d_gs_items.Upload(b.gs_items, b.stream);
   }
   b.smooth = SmoothLaunch{};
   for (const IterArgs& a : b.args) {
@@ -1318,6 +1476,11 @@ void LaunchBatchStep(ddh_batch& b, const IterArgs* args, int n_items, int n_inte
     LaunchBatchFullJones(args, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()),
                          b.d_items.ptr, n_items, n_intervals, b.solve_threads, b.f16, b.waves,
                          b.smooth, b.stream);
+  } else if (b.fused) {
+    if (n_items > 0)
+      LaunchBatchGramSolve(args, b.d_gs_items.ptr, b.gs_launch.data(), int(b.gs_launch.size()),
+                           b.f16, b.d_pro_items.ptr, int(b.pro_items.size()), b.stream);
+    LaunchBatchSmoothDecide(args, b.d_items.ptr, n_items, n_intervals, b.smooth, b.stream);
   } else if (b.gram) {
     int apply_ant = 0;
     for (const IterArgs& a : b.args) apply_ant = std::max(apply_ant, a.n_ant);
@@ -1340,6 +1503,9 @@ void LaunchBatchCrossProducts(ddh_batch& b) {
   if (b.gram)
     LaunchBatchGram(b.d_args.ptr, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), b.f16,
                     b.waves, b.stream);
+  if (b.fused)
+    LaunchBatchGramPermute(b.d_args.ptr, b.d_gs_items.ptr, int(b.gs_items.size()), b.max_dir,
+                           b.stream);
 }
 
 void LaunchBatchIteration(ddh_batch& b) {
@@ -1516,9 +1682,17 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
           // correlate alone between the first two events, antenna solve and
           // decision between the last two
           DDH_HIP_CHECK(hipEventRecord(b.events[3 * it], b.stream));
-          LaunchBatchStep(b, b.d_args.ptr, 0, 0);
+          if (b.fused)  // apply and antenna solve in one kernel
+            LaunchBatchGramSolve(b.d_args.ptr, b.d_gs_items.ptr, b.gs_launch.data(),
+                                 int(b.gs_launch.size()), b.f16, b.d_pro_items.ptr,
+                                 int(b.pro_items.size()), b.stream);
+          else
+            LaunchBatchStep(b, b.d_args.ptr, 0, 0);
           DDH_HIP_CHECK(hipEventRecord(b.events[3 * it + 1], b.stream));
-          if (b.full_jones)
+          if (b.fused)
+            LaunchBatchSmoothDecide(b.d_args.ptr, b.d_items.ptr, int(b.items.size()),
+                                    int(b.solvers.size()), b.smooth, b.stream);
+          else if (b.full_jones)
             LaunchBatchFullJones(b.d_args.ptr, b.d_tiles.ptr, b.groups.data(), 0, b.d_items.ptr,
                                  int(b.items.size()), int(b.solvers.size()), b.solve_threads, b.f16,
                                  b.waves, b.smooth, b.stream);
@@ -1680,6 +1854,17 @@ int ddh_batch_set_schedule(ddh_batch* b, int32_t waves_per_tile, int32_t rows_pe
   });
 }
 
+int ddh_batch_set_gram_solve_groups(ddh_batch* b, int32_t groups) {
+  return Guard([&] {
+    DDH_REQUIRE(b, -1, "NULL batch");
+    DDH_REQUIRE(groups >= 1 && groups <= kGramSolveMaxGroups, -1, "row groups must be 1..8");
+    b->gs_groups = groups;
+    BuildBatch(*b);
+  });
+}
+
+int ddh_batch_gram_solve_groups(const ddh_batch* b) { return b ? b->gs_groups : -1; }
+
 int ddh_batch_rebind(ddh_batch* b, ddh_problem* const* problems) {
   return Guard([&] {
     DDH_REQUIRE(b && problems, -1, "NULL argument");
@@ -1705,7 +1890,8 @@ int ddh_batch_rebind(ddh_batch* b, ddh_problem* const* problems) {
     std::vector<IterArgs> args(n);
     for (int i = 0; i < n; ++i) {
       ddh_solver& s = *b->solvers[i];
-      args[i] = MakeArgs(s, *problems[i]);
+      args[i] = MakeArgs(s, *problems[i], b->fused ? 1 : 0);
+      args[i].gs_groups = b->gs_groups;
       args[i].state = b->args[i].state;
       DDH_REQUIRE(AntennaSolveSharedBytes(args[i], b->n_corr) <= b->solve_lds, -2,
                   "rebinding needs problems of the same shapes");
@@ -1799,6 +1985,30 @@ int ddh_batch_autotune(ddh_batch* b, int32_t* waves_out, int32_t* rows_out, doub
     b->waves = bw;
     b->unroll = bu;
     BuildBatch(*b);
+    if (b->fused) {
+      // Row groups of the fused iteration: the walk rows of a block and
+      // polarization split over 1..8 workgroups. Timed on the cross products of
+      // the unit-gain state; a launch only rewrites the next solutions and the
+      // partials of the same iteration, so repeats leave the state unchanged.
+      LaunchBatchCrossProducts(*b);
+      double best_gs = 1e30;
+      int bg = b->gs_groups;
+      for (int groups : {1, 2, 4, 8}) {
+        b->gs_groups = groups;
+        BuildBatch(*b);
+        const double ms = TimeLaunches(b->stream, 5, [&] {
+          LaunchBatchGramSolve(b->d_args.ptr, b->d_gs_items.ptr, b->gs_launch.data(),
+                               int(b->gs_launch.size()), b->f16, b->d_pro_items.ptr,
+                               int(b->pro_items.size()), b->stream);
+        });
+        if (ms < best_gs) {
+          best_gs = ms;
+          bg = groups;
+        }
+      }
+      b->gs_groups = bg;
+      BuildBatch(*b);
+    }
     if (waves_out) *waves_out = bw;
     if (rows_out) *rows_out = bu;
     if (best_ms) *best_ms = best;
@@ -1865,8 +2075,7 @@ int ddh_one_step(ddh_solver* s, ddh_problem* p, double* const* solutions_in,
     a.phase_only = 0;
     a.den_out = s->den.ptr;
     LaunchSolvePrologue(*s, a, *p);
-    LaunchCorrelation(*s, a, *p);
-    LaunchAntennaSolve(a, p->n_corr, p->f16, s->stream);
+    LaunchCorrelateAndSolve(*s, a, *p);
     for (int cb = 0; cb < c.n_channel_blocks; ++cb) {
       SolutionsOut(*s, next_out[cb], s->sol1.ptr + cb * per_cb, per_cb / 2, s->stream);
       if (den_out)

@@ -114,7 +114,22 @@ struct IterArgs {
   // products C_d and G_e,d per (block, segment, correlation, baseline), made
   // once per solve; [n_cb][n_seg][2][n_dir + n_dir(n_dir+1)/2][bl_stride].
   float2* gram;
+  // Fused Gram iteration (GramSolve: apply and antenna solve in one kernel;
+  // Duo data, at most 64 antennas, one solution per direction in one
+  // segment): the cross products in walk order [cb][k][q][walk_rows * 64],
+  // and per walk slot (step row * 64 + lane) the lane's role for its baseline
+  // (0: none, 1: the lane's antenna is the first, 2: the second) and the
+  // baseline (-1: none). walk_rows = 0: no fused iteration.
+  float2* gram_walk;
+  const int32_t* walk_code;
+  const int32_t* walk_bl;
+  int32_t walk_rows;
+  int32_t gs_groups;  // row groups per polarization of the fused iteration (1..8)
+  double* gs_part;    // [n_cb][2 x gs_groups][n_dir][3][64] per-workgroup sums
 };
+
+// Largest row-group count of the fused Gram iteration.
+constexpr int kGramSolveMaxGroups = 8;
 
 // Per-baseline partials of the full-Jones correlate: X = sum R_all J_b M^H
 // (8 floats) and Y = sum R_all^H J_a M (8); and the model products made once
@@ -129,7 +144,7 @@ constexpr int kFullJonesGroupSize = 8;
 // Largest direction count of the Gram iteration (its per-lane accumulators
 // grow with n_dir^2).
 constexpr int kMaxGramDirections = 8;
-inline int GramTerms(int n_dir) { return n_dir + n_dir * (n_dir + 1) / 2; }
+__host__ __device__ inline int GramTerms(int n_dir) { return n_dir + n_dir * (n_dir + 1) / 2; }
 
 // One unit of batched work: correlate tiles use all four fields; antenna
 // solve items use (interval, cb) with the direction in `seg`; prologue items
@@ -189,6 +204,19 @@ void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, const TileGro
 // correlate partials from them and the current gains.
 void LaunchGram(const IterArgs& a, bool f16, int waves, hipStream_t stream);
 void LaunchApplyGram(const IterArgs& a, bool f16, hipStream_t stream);
+// Fused Gram iteration: the cross products into walk order once per solve,
+// then per iteration one workgroup per block for apply + antenna solve (the
+// decision follows as before).
+void LaunchGramPermute(const IterArgs& a, hipStream_t stream);
+void LaunchGramSolve(const IterArgs& a, bool f16, hipStream_t stream);
+// Batched: items (interval, cb); groups split them by direction count.
+void LaunchBatchGramPermute(const IterArgs* d_args, const BatchTile* d_items, int n_items,
+                            int max_dir, hipStream_t stream);
+// d_items: (interval, cb, part) workgroups; d_blocks: (interval, cb) for the
+// finishing kernel (n_blocks 0: the sums only).
+void LaunchBatchGramSolve(const IterArgs* d_args, const BatchTile* d_items,
+                          const TileGroup* groups, int n_groups, bool f16,
+                          const BatchTile* d_blocks, int n_blocks, hipStream_t stream);
 // LDS of the Gram apply: the segment's gains of every antenna and direction.
 size_t ApplyGramSharedBytes(int n_ant, int n_dir);
 constexpr size_t kApplyGramMaxShared = 64 * 1024;

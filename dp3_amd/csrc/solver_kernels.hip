@@ -760,6 +760,52 @@ __device__ __forceinline__ void AddPairTerms(const double2 (&old)[2], const doub
   }
 }
 
+// The end of SolveDirection for one solution pair (antenna, sub-solution) of
+// a block, from its FP64 numerator and denominator sums per polarization: the
+// division with NaN for a zero denominator (IterativeDiagonalSolver.cc:
+// 206-221), the step (SolverBase.cc:88-158, the polar step for phase-only
+// solves), the optional denominators, and the pair's convergence and repair-
+// mean terms added to the running sums. The scalar solver fits one gain to
+// both polarizations: the sums of both (IterativeScalarSolver.cc:142-173).
+__device__ __forceinline__ void FinishSolution(const IterArgs& a, const double2* cur,
+                                               double2* nxt, const size_t pair, const double mean,
+                                               double (&nr)[2], double (&ni)[2], double (&dn)[2],
+                                               double& conv_sum, int& conv_n, double& mean_sum,
+                                               int& mean_n) {
+  if (a.scalar) {
+    nr[0] = nr[1] = nr[0] + nr[1];
+    ni[0] = ni[1] = ni[0] + ni[1];
+    dn[0] = dn[1] = dn[0] + dn[1];
+  }
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  double2 stepped[2];
+  double2 old[2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const double2 nx = dn[p] == 0.0 ? make_double2(nan, 0.0)
+                                    : make_double2(nr[p] / dn[p], ni[p] / dn[p]);
+    old[p] = FixedSolution(cur, pair, p, mean);
+    if (!a.phase_only) {
+      stepped[p] = make_double2(old[p].x * (1.0 - a.step_size) + nx.x * a.step_size,
+                                old[p].y * (1.0 - a.step_size) + nx.y * a.step_size);
+    } else {
+      // Step along the complex circle (SolverBase.cc:147-155).
+      const double phase_from = atan2(old[p].y, old[p].x);
+      double distance = atan2(nx.y, nx.x) - phase_from;
+      if (distance > M_PI)
+        distance -= 2.0 * M_PI;
+      else if (distance < -M_PI)
+        distance += 2.0 * M_PI;
+      double sn, cs;
+      sincos(phase_from + a.step_size * distance, &sn, &cs);
+      stepped[p] = make_double2(cs, sn);
+    }
+    nxt[pair * 2 + p] = stepped[p];
+    if (a.den_out) a.den_out[pair * 2 + p] = dn[p];
+  }
+  AddPairTerms(old, stepped, a.scalar != 0, conv_sum, conv_n, mean_sum, mean_n);
+}
+
 // ---------------------------------------------------------------------------
 // antenna_solve: grid (n_dir, n_cb); `split` threads per (antenna, solution
 // of d) share that antenna's baseline list and are combined by a fixed
@@ -884,45 +930,11 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
     }
   }
 
-  // Scalar solver: one gain for both polarizations, the sums of both
-  // (IterativeScalarSolver.cc:142-173: Trace and Norm over all correlations).
-  if (a.scalar) {
-    nr[0] = nr[1] = nr[0] + nr[1];
-    ni[0] = ni[1] = ni[0] + ni[1];
-    dn[0] = dn[1] = dn[0] + dn[1];
-  }
   double conv_sum = 0.0, mean_sum = 0.0;
   int conv_n = 0, mean_n = 0;
-  if (active && q == 0) {
-    const size_t pair = gbase + size_t(ant) * a.n_sub + s_abs;
-    const double nan = std::numeric_limits<double>::quiet_NaN();
-    double2 stepped[2];
-    double2 old[2];
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      const double2 nx = dn[p] == 0.0 ? make_double2(nan, 0.0)
-                                      : make_double2(nr[p] / dn[p], ni[p] / dn[p]);
-      old[p] = FixedSolution(cur, pair, p, mean);
-      if (!a.phase_only) {
-        stepped[p] = make_double2(old[p].x * (1.0 - a.step_size) + nx.x * a.step_size,
-                                  old[p].y * (1.0 - a.step_size) + nx.y * a.step_size);
-      } else {
-        // Step along the complex circle (SolverBase.cc:147-155).
-        const double phase_from = atan2(old[p].y, old[p].x);
-        double distance = atan2(nx.y, nx.x) - phase_from;
-        if (distance > M_PI)
-          distance -= 2.0 * M_PI;
-        else if (distance < -M_PI)
-          distance += 2.0 * M_PI;
-        double sn, cs;
-        sincos(phase_from + a.step_size * distance, &sn, &cs);
-        stepped[p] = make_double2(cs, sn);
-      }
-      nxt[pair * 2 + p] = stepped[p];
-      if (a.den_out) a.den_out[pair * 2 + p] = dn[p];
-    }
-    AddPairTerms(old, stepped, a.scalar != 0, conv_sum, conv_n, mean_sum, mean_n);
-  }
+  if (active && q == 0)
+    FinishSolution(a, cur, nxt, gbase + size_t(ant) * a.n_sub + s_abs, mean, nr, ni, dn,
+                   conv_sum, conv_n, mean_sum, mean_n);
   BlockSum2(conv_sum, conv_n, mean_sum, mean_n, red, redc);
   const int wg = cb * a.n_dir + d;
   if (threadIdx.x == 0) {
@@ -938,6 +950,260 @@ template <int NC, bool F16>
 __global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterArgs a) {
   if (a.state->done) return;
   AntennaSolveBlock<NC, F16>(a, blockIdx.x, blockIdx.y);
+}
+
+// ---------------------------------------------------------------------------
+// Fused Gram iteration: the apply and the antenna solve of a channel block
+// with the per-antenna sums formed on chip (no per-baseline partials in HBM).
+// For problems of Duo data with at most 64 antennas and one solution per
+// direction in one time segment.
+//
+// Every wave is a ring of the block's n antennas, lane a = antenna a. At
+// walk step s (1 <= s <= n/2) lane a takes the baseline {a, (a + s) mod n}
+// (for even n at s = n/2 only lanes a < n/2, which covers each such pair
+// once), so the steps visit every baseline exactly once. The cross products
+// are stored in this walk order, [cb][k][q][step row * 64 + lane], zero where
+// no baseline is, so each wave's loads are contiguous. For its baseline
+// (A, B) and every direction d the lane forms
+//   U_d = C_d - sum_e gg_e G_e,d,  S_d = G_d,d,  T_d = U_d + gg_d S_d
+// (gg_e = g_A conj(g_B), the FP16 scale folded in) and the two terms of the
+// reference's SolveDirection (IterativeDiagonalSolver.cc:172-197):
+//   antenna A: num += g_B T, den += |g_B|^2 S;  B: num += g_A conj(T), den += |g_A|^2 S.
+// U is FP32 as in the apply; T, the two terms and the sums are FP64, as the
+// antenna solve forms them. The lane's own antenna's term goes to its
+// accumulators; the partner's term is pulled by the partner's lane (lane b
+// reads lane (b - s) mod n, whose partner is b).
+//
+// Work split: a block's walk rows of one polarization are cut into `groups`
+// row groups; one workgroup of four waves per (block, polarization, group),
+// its waves taking every fourth row. The workgroup adds its waves' sums in
+// LDS in a fixed order and stores them (per antenna and direction, FP64) in
+// the scratch `gs_part`; a second kernel (GramFinishBlock, one workgroup per
+// block) adds the block's 2 x groups sums in a fixed order and finishes every
+// solution. (Finishing in the last workgroup to arrive needs device-scope
+// fences, which write back and invalidate an XCD's L2 per workgroup; the
+// kernel boundary is cheaper.) Results differ from apply + antenna solve by
+// the FP64 summation order only, and do not depend on the order in which
+// workgroups run.
+// ---------------------------------------------------------------------------
+constexpr int kGramSolveThreads = 256;
+constexpr int kGramSolveWaves = kGramSolveThreads / kWave;
+
+template <int ND, bool F16>
+__device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, const int part) {
+  constexpr int NC = 2;
+  constexpr int NQ = ND + ND * (ND + 1) / 2;
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  __shared__ float2 s_gain[ND][kWave];
+  __shared__ double s_red[kGramSolveWaves][ND][3][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kWave);
+  const int groups = a.gs_groups;
+  const int k = part / groups;  // polarization
+  const int group = part - k * groups;
+  const int n = a.n_ant;
+  const int iteration = AsConst(a.state)->iteration;
+  const double2* cur = (iteration & 1) ? a.sol1 : a.sol0;
+  double2* nxt = (iteration & 1) ? a.sol0 : a.sol1;
+  const size_t gbase = size_t(cb) * n * a.n_sub;
+  const double mean = AsConst(a.mean)[cb];
+
+  const int rows = a.walk_rows;
+  const int per = (rows + groups - 1) / groups;
+  const int r0 = group * per;
+  const int r1 = min(rows, r0 + per);
+  const uint32_t slots = uint32_t(rows) * kWave;
+  // the block's terms of polarization k through one buffer resource: a lane's
+  // byte offset in a VGPR, the term's offset in an SGPR
+  const __amdgpu_buffer_rsrc_t grsrc =
+      PlaneResource(a.gram_walk + (size_t(cb) * NC + k) * NQ * slots, NQ * slots * 8u);
+  const GlobalT<int32_t>* codes = AsGlobal(a.walk_code);
+
+  // Gains of polarization k of every antenna and direction (one solution
+  // each), repaired (a pair with a non-finite entry reads as the block mean)
+  // and cast to float, as the correlate uses them (SolveData.h:230-235).
+  for (int e = threadIdx.x; e < ND * kWave; e += kGramSolveThreads) {
+    const int ant = e & (kWave - 1);
+    const int d = e / kWave;
+    float2 gf = make_float2(0.0f, 0.0f);
+    if (ant < n) {
+      const size_t pair = gbase + size_t(ant) * a.n_sub + AsConst(a.seg_sol)[d];
+      double2 g0 = LoadGlobal(cur + pair * 2), g1 = LoadGlobal(cur + pair * 2 + 1);
+      if (!(cfinite(g0) && cfinite(g1))) g0 = g1 = make_double2(mean, 0.0);
+      const double2 g = k ? g1 : g0;
+      gf = make_float2(float(g.x), float(g.y));
+    }
+    s_gain[d][ant] = gf;
+  }
+  float sc[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) sc[d] = F16 ? AsConst(a.model_scale)[cb * ND + d] : 1.0f;
+  __syncthreads();
+
+  double acc[ND][3];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) acc[d][0] = acc[d][1] = acc[d][2] = 0.0;
+  for (int r = r0 + wave; r < r1; r += kGramSolveWaves) {
+    const int step = r + 1;
+    const uint32_t slot = uint32_t(r) * kWave + uint32_t(lane);
+    f2v q[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i)
+      q[i] = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(
+                                         grsrc, slot * 8u, uint32_t(i) * slots * 8u, 0));
+    const bool first = codes[slot] != 2;  // 1: lane's antenna is A; 0 (no baseline): zeros
+    const int pa = lane < n ? (lane + step < n ? lane + step : lane + step - n) : lane;
+    const int src = lane < n ? (lane >= step ? lane - step : lane - step + n) : lane;
+    float2 go[ND], gp[ND], gg[ND], ggs[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      go[d] = s_gain[d][lane];
+      gp[d] = s_gain[d][pa];
+      gg[d] = first ? cmulc(go[d], gp[d]) : cmulc(gp[d], go[d]);  // g_A conj(g_B)
+      ggs[d] = make_float2(gg[d].x * sc[d], gg[d].y * sc[d]);
+    }
+    // U_d = C_d - sum_e ggs_e G_e,d (FP32, as the apply; G_e,d = conj(G_d,e)
+    // for e > d), four FMAs per complex term; U and S first, so that the
+    // loaded terms are dead before the FP64 part
+    float2 uu[ND];
+    float ss[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      float ux = q[d].x, uy = q[d].y;
+#pragma unroll
+      for (int e = 0; e < ND; ++e) {
+        const f2v ge = e <= d ? q[GramPair<ND>(e, d)] : q[GramPair<ND>(d, e)];
+        const float gi = e <= d ? ge.y : -ge.y;
+        ux = fmaf(-ggs[e].x, ge.x, ux);
+        ux = fmaf(ggs[e].y, gi, ux);
+        uy = fmaf(-ggs[e].x, gi, uy);
+        uy = fmaf(-ggs[e].y, ge.x, uy);
+      }
+      uu[d] = make_float2(ux, uy);
+      ss[d] = q[GramPair<ND>(d, d)].x;
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      const double sd = double(sc[d]) * double(sc[d]) * double(ss[d]);
+      double2 t = make_double2(double(sc[d]) * double(uu[d].x) + double(gg[d].x) * sd,
+                               double(sc[d]) * double(uu[d].y) + double(gg[d].y) * sd);
+      // T with the lane's orientation: T for antenna A's lane, conj(T) for B's
+      if (!first) t.y = -t.y;
+      const double2 o = make_double2(go[d].x, go[d].y);
+      const double2 p = make_double2(gp[d].x, gp[d].y);
+      const double2 own = cmul(p, t);    // A: g_B T; B: g_A conj(T)
+      const double2 prt = cmulc(o, t);   // partner B: g_A conj(T); partner A: g_B T
+      acc[d][0] += own.x + __shfl(prt.x, src);
+      acc[d][1] += own.y + __shfl(prt.y, src);
+      acc[d][2] += cnorm(p) * sd + __shfl(cnorm(o) * sd, src);
+    }
+  }
+
+  // the four waves' sums, added in wave order; wave w stores directions w, w + 4
+#pragma unroll
+  for (int d = 0; d < ND; ++d)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s_red[wave][d][c][lane] = acc[d][c];
+  __syncthreads();
+  const size_t part_stride = size_t(ND) * 3 * kWave;  // doubles per workgroup's sums
+  double* mine = a.gs_part + (size_t(cb) * 2 * groups + part) * part_stride;
+  for (int d = wave; d < ND; d += kGramSolveWaves)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double v = 0.0;
+#pragma unroll
+      for (int w = 0; w < kGramSolveWaves; ++w) v += s_red[w][d][c][lane];
+      mine[(d * 3 + c) * kWave + lane] = v;
+    }
+}
+
+// The end of the fused iteration for one block, after every workgroup of the
+// block stored its sums (the kernel boundary makes them visible): per
+// direction (one wave each, lane = antenna) the 2 x groups sums are added in
+// a fixed order and every solution is finished (FinishSolution), with the
+// direction's convergence partials written as the antenna solve does.
+__device__ __forceinline__ void GramFinishBlock(const IterArgs& a, const int cb) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = int(threadIdx.x) / kWave;
+  const int n_waves = int(blockDim.x) / kWave;
+  const int groups = a.gs_groups;
+  const int n = a.n_ant;
+  const int iteration = AsConst(a.state)->iteration;
+  const double2* cur = (iteration & 1) ? a.sol1 : a.sol0;
+  double2* nxt = (iteration & 1) ? a.sol0 : a.sol1;
+  const size_t gbase = size_t(cb) * n * a.n_sub;
+  const double mean = AsConst(a.mean)[cb];
+  const size_t part_stride = size_t(a.n_dir) * 3 * kWave;
+  const GlobalT<double>* all = AsGlobal(a.gs_part) + size_t(cb) * 2 * groups * part_stride;
+  for (int d = wave; d < a.n_dir; d += n_waves) {
+    double nr[2] = {0.0, 0.0}, ni[2] = {0.0, 0.0}, dn[2] = {0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+      for (int g = 0; g < groups; ++g) {
+        const GlobalT<double>* x =
+            all + size_t(p * groups + g) * part_stride + size_t(d) * 3 * kWave + lane;
+        nr[p] += x[0];
+        ni[p] += x[kWave];
+        dn[p] += x[2 * kWave];
+      }
+    double conv_sum = 0.0, mean_sum = 0.0;
+    int conv_n = 0, mean_n = 0;
+    if (lane < n)
+      FinishSolution(a, cur, nxt, gbase + size_t(lane) * a.n_sub + AsConst(a.seg_sol)[d], mean,
+                     nr, ni, dn, conv_sum, conv_n, mean_sum, mean_n);
+    for (int off = 32; off > 0; off >>= 1) {
+      conv_sum += __shfl_xor(conv_sum, off);
+      conv_n += __shfl_xor(conv_n, off);
+      mean_sum += __shfl_xor(mean_sum, off);
+      mean_n += __shfl_xor(mean_n, off);
+    }
+    if (lane == 0) {
+      const int wg = cb * a.n_dir + d;
+      a.wg_conv[wg] = conv_sum;
+      a.wg_conv_n[wg] = conv_n;
+      a.wg_mean[wg] = mean_sum;
+      a.wg_mean_n[wg] = mean_n;
+    }
+  }
+}
+
+constexpr int kGramFinishThreads = 512;
+
+__global__ void __launch_bounds__(kGramFinishThreads) GramFinishKernel(const IterArgs a) {
+  if (AsConst(a.state)->done) return;
+  GramFinishBlock(a, blockIdx.x);
+}
+
+// grid (2 x groups, n_cb)
+template <int ND, bool F16>
+__global__ void __launch_bounds__(kGramSolveThreads, 2) GramSolveKernel(const IterArgs a) {
+  if (AsConst(a.state)->done) return;
+  GramSolveBlock<ND, F16>(a, blockIdx.y, blockIdx.x);
+}
+
+// Cross products from baseline order [cb][seg 0][k][q][bl_stride] into walk
+// order [cb][k][q][walk_rows * 64] (zeros where the walk has no baseline);
+// one workgroup per (block, correlation, term), so a term's baseline row is
+// read into one XCD's L2 once. Once per solve.
+__device__ __forceinline__ void GramPermuteRow(const IterArgs& a, const int cb, const int k,
+                                               const int q, const int nq) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  const size_t slots = size_t(a.walk_rows) * kWave;
+  const GlobalT<f2v>* src = (const GlobalT<f2v>*)AsGlobal(
+      a.gram + (size_t(cb) * a.n_seg * 2 + k) * nq * a.bl_stride + size_t(q) * a.bl_stride);
+  GlobalT<f2v>* dst =
+      (GlobalT<f2v>*)AsGlobal(a.gram_walk + ((size_t(cb) * 2 + k) * nq + q) * slots);
+  const GlobalT<int32_t>* bl = AsGlobal(a.walk_bl);
+  for (size_t i = threadIdx.x; i < slots; i += blockDim.x) {
+    const int b = bl[i];
+    f2v v = {0.0f, 0.0f};
+    if (b >= 0) v = src[b];
+    dst[i] = v;
+  }
+}
+
+__global__ void __launch_bounds__(256) GramPermuteKernel(const IterArgs a) {
+  GramPermuteRow(a, blockIdx.z, blockIdx.y, blockIdx.x, GramTerms(a.n_dir));
 }
 
 // The repaired value of 2x2 solution `m` (MakeSolutionsFinite4Pol,
@@ -1916,6 +2182,35 @@ __global__ void __launch_bounds__(kBlockThreads)
   AntennaSolveBlock<NC, F16>(a, t.seg, t.cb);  // seg holds the direction
 }
 
+// items (interval, cb, part); one workgroup per item
+template <int ND, bool F16>
+__global__ void __launch_bounds__(kGramSolveThreads, 2)
+    BatchGramSolveKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ items) {
+  const BatchTile t = LoadConst(items + XcdContiguous(blockIdx.x, gridDim.x));
+  const IterArgs a = LoadConst(args + t.interval);
+  if (AsConst(a.state)->done) return;
+  GramSolveBlock<ND, F16>(a, t.cb, t.seg);
+}
+
+// items (interval, cb): the end of the fused iteration of each block
+__global__ void __launch_bounds__(kGramFinishThreads)
+    BatchGramFinishKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ items) {
+  const BatchTile t = LoadConst(items + blockIdx.x);
+  const IterArgs a = LoadConst(args + t.interval);
+  if (AsConst(a.state)->done) return;
+  GramFinishBlock(a, t.cb);
+}
+
+// items: (interval, cb); grid (terms, 2, n_items)
+__global__ void __launch_bounds__(256)
+    BatchGramPermuteKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ items) {
+  const BatchTile t = LoadConst(items + blockIdx.z);
+  const IterArgs a = LoadConst(args + t.interval);
+  const int nq = GramTerms(a.n_dir);
+  if (int(blockIdx.x) >= nq) return;
+  GramPermuteRow(a, t.cb, blockIdx.y, blockIdx.x, nq);
+}
+
 __global__ void __launch_bounds__(256) BatchDecideKernel(const IterArgs* __restrict__ args) {
   const IterArgs& a = args[blockIdx.x];
   if (a.state->done) return;
@@ -2146,6 +2441,61 @@ void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const Ti
   DDH_HIP_CHECK(hipGetLastError());
   LaunchBatchTail(d_args, d_items, n_items, n_intervals, solve_lds, solve_threads, 2, f16, smooth,
                   stream);
+}
+
+namespace {
+template <bool F16, int... ND>
+constexpr auto MakeGramSolveTable(std::integer_sequence<int, ND...>) {
+  return std::array<KernelFn, sizeof...(ND)>{&GramSolveKernel<ND + 1, F16>...};
+}
+template <bool F16, int... ND>
+constexpr auto MakeBatchGramSolveTable(std::integer_sequence<int, ND...>) {
+  return std::array<BatchKernelFn, sizeof...(ND)>{&BatchGramSolveKernel<ND + 1, F16>...};
+}
+const std::array<KernelFn, kMaxGramDirections> kGramSolve[2] = {
+    MakeGramSolveTable<false>(GramDirs{}), MakeGramSolveTable<true>(GramDirs{})};
+const std::array<BatchKernelFn, kMaxGramDirections> kBatchGramSolve[2] = {
+    MakeBatchGramSolveTable<false>(GramDirs{}), MakeBatchGramSolveTable<true>(GramDirs{})};
+}  // namespace
+
+void LaunchGramPermute(const IterArgs& a, hipStream_t stream) {
+  DDH_REQUIRE(a.walk_rows > 0 && a.n_seg == 1, -1, "no walk order for this problem");
+  hipLaunchKernelGGL(GramPermuteKernel, dim3(GramTerms(a.n_dir), 2, a.n_cb), dim3(256), 0, stream,
+                     a);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchGramSolve(const IterArgs& a, bool f16, hipStream_t stream) {
+  CheckGramDirections(a.n_dir);
+  DDH_REQUIRE(a.walk_rows > 0 && a.n_ant <= kWave, -1, "no walk order for this problem");
+  DDH_REQUIRE(a.gs_groups >= 1 && a.gs_groups <= kGramSolveMaxGroups, -1, "bad row groups");
+  hipLaunchKernelGGL(kGramSolve[f16][a.n_dir - 1], dim3(2 * a.gs_groups, a.n_cb),
+                     dim3(kGramSolveThreads), 0, stream, a);
+  hipLaunchKernelGGL(GramFinishKernel, dim3(a.n_cb), dim3(kGramFinishThreads), 0, stream, a);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchBatchGramPermute(const IterArgs* d_args, const BatchTile* d_items, int n_items,
+                            int max_dir, hipStream_t stream) {
+  if (n_items == 0) return;
+  hipLaunchKernelGGL(BatchGramPermuteKernel, dim3(GramTerms(max_dir), 2, n_items), dim3(256), 0,
+                     stream, d_args, d_items);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchBatchGramSolve(const IterArgs* d_args, const BatchTile* d_items,
+                          const TileGroup* groups, int n_groups, bool f16,
+                          const BatchTile* d_blocks, int n_blocks, hipStream_t stream) {
+  for (int g = 0; g < n_groups; ++g) {
+    if (groups[g].count == 0) continue;
+    CheckGramDirections(groups[g].n_dir);
+    hipLaunchKernelGGL(kBatchGramSolve[f16][groups[g].n_dir - 1], dim3(groups[g].count),
+                       dim3(kGramSolveThreads), 0, stream, d_args, d_items + groups[g].first);
+  }
+  if (n_blocks > 0)
+    hipLaunchKernelGGL(BatchGramFinishKernel, dim3(n_blocks), dim3(kGramFinishThreads), 0, stream,
+                       d_args, d_blocks);
+  DDH_HIP_CHECK(hipGetLastError());
 }
 
 size_t CorrelateSharedBytes(int n_dir, int n_corr, int waves) {

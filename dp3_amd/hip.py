@@ -118,6 +118,10 @@ SIGNATURES = {
                                 C.POINTER(C.c_double)], C.c_int),
     "ddh_set_schedule": ([C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32], C.c_int),
     "ddh_set_profiling": ([C.c_void_p, C.c_int32], C.c_int),
+    "ddh_set_gram_fusion": ([C.c_void_p, C.c_int32], C.c_int),
+    "ddh_set_gram_solve_groups": ([C.c_void_p, C.c_int32], C.c_int),
+    "ddh_batch_set_gram_solve_groups": ([C.c_void_p, C.c_int32], C.c_int),
+    "ddh_batch_gram_solve_groups": ([C.c_void_p], C.c_int),
     "ddh_predict": ([C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_int32, C.c_void_p,
                      C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p],
                     C.c_int),
@@ -391,6 +395,15 @@ class Solver:
     def set_profiling(self, enable=True):
         _check(lib().ddh_set_profiling(self.handle, int(enable)), "ddh_set_profiling")
 
+    def set_gram_fusion(self, enable=True):
+        """Fused Gram iteration (apply + antenna solve in one kernel) where the
+        problem allows it; on by default."""
+        _check(lib().ddh_set_gram_fusion(self.handle, int(enable)), "ddh_set_gram_fusion")
+
+    def set_gram_solve_groups(self, groups):
+        _check(lib().ddh_set_gram_solve_groups(self.handle, int(groups)),
+               "ddh_set_gram_solve_groups")
+
     def last_timing(self):
         ms = C.c_double()
         n = C.c_int64()
@@ -504,6 +517,13 @@ class Batch:
         _check(lib().ddh_batch_autotune(self.handle, C.byref(w), C.byref(r), C.byref(ms)),
                "ddh_batch_autotune")
         return w.value, r.value, ms.value
+
+    def set_gram_solve_groups(self, groups):
+        _check(lib().ddh_batch_set_gram_solve_groups(self.handle, int(groups)),
+               "ddh_batch_set_gram_solve_groups")
+
+    def gram_solve_groups(self):
+        return int(lib().ddh_batch_gram_solve_groups(self.handle))
 
     def _pointers(self, solutions):
         n = len(self.solvers)

@@ -255,6 +255,18 @@ int ddh_solver_last_timing(const ddh_solver* solver, double* correlate_ms,
 int ddh_set_schedule(ddh_solver* solver, int32_t waves_per_tile,
                      int32_t rows_per_trip, int32_t use_graph,
                      int32_t poll_interval);
+// Fused Gram iteration (on by default): for Gram solves of Duo data with at
+// most 64 antennas, each antenna pair at most once and one solution per
+// direction in one time segment, one kernel per channel block forms the
+// correlate sums and each antenna's numerator and denominator on chip,
+// without per-baseline partials in device memory. Off: the apply and antenna
+// solve kernels. Results agree to FP32 summation order.
+int ddh_set_gram_fusion(ddh_solver* solver, int32_t enable);
+// Schedule of the fused Gram iteration: each block's walk over the antenna
+// pairs is split, per polarization, into `groups` (1..8) workgroups whose
+// per-antenna sums the last one to finish adds up (default 2; set by the
+// autotuners).
+int ddh_set_gram_solve_groups(ddh_solver* solver, int32_t groups);
 // Records HIP events around every correlate launch of the next solves
 // (ddh_solver_last_timing then reports their summed duration).
 int ddh_set_profiling(ddh_solver* solver, int32_t enable);
@@ -286,6 +298,10 @@ ddh_batch* ddh_batch_create(ddh_solver* const* solvers, ddh_problem* const* prob
 void ddh_batch_destroy(ddh_batch* batch);
 int ddh_batch_set_schedule(ddh_batch* batch, int32_t waves_per_tile, int32_t rows_per_trip,
                            int32_t use_graph, int32_t poll_interval);
+// Row groups of the batch's fused Gram iteration (as ddh_set_gram_solve_groups)
+// and the current value.
+int ddh_batch_set_gram_solve_groups(ddh_batch* batch, int32_t groups);
+int ddh_batch_gram_solve_groups(const ddh_batch* batch);
 // Points the batch at other problems of the same shapes (the next intervals
 // of the same sequences), keeping buffers, schedule and captured graph.
 int ddh_batch_rebind(ddh_batch* batch, ddh_problem* const* problems);
@@ -318,8 +334,11 @@ int ddh_batch_last_timing(const ddh_batch* batch, double* correlate_ms, double* 
                           int64_t* launches);
 
 // Schedule autotuner: times the correlate kernel under every candidate
-// schedule (waves per tile x rows per trip) on the actual shapes and keeps
-// the fastest. Leaves the solve state reset (unit gains).
+// schedule (waves per tile x rows per trip; Gram iteration: the cross
+// products) on the actual shapes and keeps the fastest; for a fused Gram
+// iteration it then times the fused kernel under 1, 2, 4 and 8 row groups and
+// keeps the fastest (ddh_batch_gram_solve_groups reads it back). Leaves the
+// solve state reset (unit gains).
 int ddh_autotune(ddh_solver* solver, ddh_problem* problem, int32_t* waves_per_tile,
                  int32_t* rows_per_trip, double* best_ms);
 int ddh_batch_autotune(ddh_batch* batch, int32_t* waves_per_tile, int32_t* rows_per_trip,
