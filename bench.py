@@ -7,10 +7,11 @@ components, 240 channels in 24 channel blocks, solution interval 10, step 0.2,
 tolerance 1e-6. `--streams` independent interval sequences (one per stream,
 e.g. sub-bands of one observation) of `--train` + `--warmup` + `--steps`
 consecutive intervals with gains drifting by a phase random walk
-(0.05 rad / interval) and an amplitude random walk (0.005). The warm-start
+(0.05 rad / interval) and an amplitude random walk (0.005): the warm-start
 network is trained on the solver's converged outputs of each stream's first
-`--train` intervals; the following intervals are timed. Inputs of the timed
-intervals are resident in HBM before the timed region starts.
+`--train` (150) intervals and evaluated on the following `--steps` (50). A
+step is one interval of every stream. Inputs of the evaluated intervals are
+resident in HBM before the timed region starts.
 
 Every form stops on the reference's own criterion (tolerance 1e-6 on the
 step magnitude, stall detection on), with the iteration limit raised from 100
@@ -18,42 +19,46 @@ to 1000: on this workload no start reaches the tolerance within 100
 iterations, and the chi^2 after 100 iterations from unit gains is ~100x the
 converged chi^2. "Same chi^2" is checked, not assumed: every final chi^2 is
 evaluated in FP64 on the complex64 model and compared with the interval's
-converged chi^2 (a 1000-iteration solve at tolerance 1e-9).
+converged chi^2 (a 1000-iteration solve at tolerance 1e-9 along the chain).
 
   * plain         the HIP solver with the reference's settings: unit gains,
                   complex64 model planes, one interval per solve, streaming
                   iteration (data and model planes read every iteration).
+                  Solved one interval at a time, so its rate does not depend
+                  on the number of streams; it runs on `--plain-streams`.
   * optimized     batched (one launch per kernel per iteration over all
                   streams), FP16 model planes (per block/direction scale),
                   learned warm start from the previous interval's solutions
                   computed on the device (one undamped step + 2 x 4 damped
                   iterations -> features -> 3-layer 256-hidden bf16 MLP),
-                  inside the timed region, and the Gram iteration (per-baseline
-                  data/model cross products made once per solve, applied to
-                  the gains every iteration). `value` is this form.
-Also reported: the optimized form with the streaming iteration (the three
-changes alone), the optimized form without the network (previous
-solutions), the plain form at the 100-iteration limit and previous-solution
-starts (DP3's propagatesolutions) unbatched in complex64.
+                  inside the timed region, and the fused Gram iteration
+                  (per-baseline data/model cross products made once per
+                  solve; each iteration one kernel forms every antenna's
+                  numerator and denominator from them and the gains). `value`
+                  is this form.
+Also reported: the optimized form with the streaming iteration, the
+optimized form without the network (previous solutions), the plain form at
+the 100-iteration limit and previous-solution starts (DP3's
+propagatesolutions) unbatched in complex64.
 
-`roofline` is for the dominant kernel of the optimized form, by time per
-batched iteration: the Gram apply (algorithmic bytes: the cross products read
-and the correlate partials written) or the antenna solve + decision (partials
-read, solutions read and written). Times come from HIP events on the batch
-stream around each launch of a fixed-length (50-iteration, accuracy 0) solve
-in which every interval is live in every launch. `roofline_other` holds the
-others, including the batched FP16 correlate of the streaming form
-(streams x n_vis x (16 + n_dir x 8) bytes); `roofline_plain` is the complex64
-correlate of the plain form (n_vis x (16 + n_dir x 16)). `traffic` is read
-from the committed rocprofv3 --pmc summary of the same command, when present.
-`cpu_baseline` times the CPU oracle (a restatement of DP3's
-IterativeDiagonalSolver, threads over channel blocks) on a bounded sample of
-plain-form iterations and scales it to solves/s at the plain form's mean
-iteration count.
+`roofline` is for the dominant kernel of the optimized form, the fused Gram
+iteration kernel (algorithmic bytes: the per-baseline cross products it
+reads); its mean duration comes from HIP events on the batch stream around
+each of its launches in a fixed-length (50-iteration, accuracy 0) batched
+solve in which every interval is live in every launch. `roofline_other`
+holds the rest of that iteration (finishing kernel and decision), the
+unfused apply + antenna solve it replaces, and the batched FP16 correlate of
+the streaming form; `roofline_plain` the complex64 correlate of the plain
+form. `traffic` is read from the committed rocprofv3 --pmc summary of the
+same command, when present. `cpu_baseline` runs the CPU oracle (a
+restatement of DP3's IterativeDiagonalSolver, threads over channel blocks)
+on one full plain solve of the first evaluated interval; the same interval
+gives the gain errors against the oracle and the FP64 oracle.
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -68,6 +73,7 @@ from dp3_amd import warmstart as WS  # noqa: E402
 
 METRIC = "DDECal solves/s (62 st, 8 dir, 240 ch): plain vs batched+warm-start, same χ²"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_GFLOPS = 78600.0
 MAX_ITERATIONS = 1000
 TOLERANCE = 1e-6
 ROOF_ITERATIONS = 50
@@ -89,33 +95,19 @@ def measured_traffic(kernel):
     return None
 
 
-def algorithmic_bytes(w, model_bytes):
-    n_vis = w.n_baselines * w.n_channels * w.solint
-    return n_vis * (16 + w.n_directions * model_bytes)
-
-
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(dp, w, iterations_per_solve, sample_iterations, threads):
-    """Oracle (reference loop structure) on `sample_iterations` iterations."""
-    from oracle import pyoracle as O
-    P = S.download_problem(dp, w)
-    sol = P.unit_solutions()
-    t0 = time.perf_counter()
-    O.solve(P, sol, max_iterations=sample_iterations, accuracy=0.0, detect_stalling=False,
-            n_threads=threads)
-    per_iter = (time.perf_counter() - t0) / sample_iterations
-    return {
-        "value": round(1.0 / (per_iter * iterations_per_solve), 5),
-        "unit": "solves/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": (f"{sample_iterations} iterations of one {w.name} interval "
-                   f"({per_iter:.3f} s/iteration, {threads} threads over channel blocks), "
-                   f"scaled to the plain form's {iterations_per_solve:.1f} iterations per solve"),
-    }
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
 
 
 def summary(chis, its):
@@ -124,18 +116,36 @@ def summary(chis, its):
             "iterations_range": [int(np.min(its)), int(np.max(its))]}
 
 
+def drop_in_views(problem):
+    """ddh_cb_view array (and the arrays it points to) of a SolveData-layout
+    problem, as DDECal hands its SolveData to the solver."""
+    views = (H.CbView * problem.n_channel_blocks)()
+    keep = []
+    n_dir = problem.n_directions
+    for cb in range(problem.n_channel_blocks):
+        v0, v1 = int(problem.vis_offsets[cb]), int(problem.vis_offsets[cb + 1])
+        d = np.ascontiguousarray(problem.data[v0:v1])
+        m = np.ascontiguousarray(problem.model[v0 * n_dir:v1 * n_dir])
+        pairs = np.ascontiguousarray(np.stack([problem.ant1[v0:v1], problem.ant2[v0:v1]],
+                                              axis=1).astype(np.uint32))
+        sm = np.ascontiguousarray(problem.sol_map[v0 * n_dir:v1 * n_dir])
+        keep += [d, m, pairs, sm]
+        views[cb] = H.CbView(v1 - v0, d.ctypes.data, m.ctypes.data, pairs.ctypes.data,
+                             sm.ctypes.data)
+    return views, keep
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20, help="timed intervals per stream")
+    ap.add_argument("--steps", type=int, default=50, help="timed intervals per stream")
     ap.add_argument("--warmup", type=int, default=2, help="untimed intervals per stream")
-    # 8 streams: the batched launches carry 8 intervals (about 8% more solves/s
-    # than 4 on this workload; resident inputs of the timed intervals ~180 GB)
     ap.add_argument("--streams", type=int, default=8, help="interval sequences batched together")
+    ap.add_argument("--plain-streams", type=int, default=2,
+                    help="streams the unbatched plain forms are timed on")
     ap.add_argument("--train", type=int, default=150, help="training intervals per stream")
     ap.add_argument("--stretch", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-iterations", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
@@ -154,66 +164,45 @@ def main():
     GRAPH = 0 if args.no_graph else 1
     w = S.LOFAR
     B, T, TR, WU = args.streams, args.steps, args.train, args.warmup
+    PB = min(args.plain_streams, B)
     n_total = TR + WU + T
     first = TR  # first interval after training (warmup intervals, then timed ones)
     timed = range(TR + WU, n_total)
     t_setup = time.perf_counter()
+    kw = dict(max_iterations=MAX_ITERATIONS, accuracy=TOLERANCE)
     s16 = [WS.IntervalSequence(w, n_total, seed=args.seed + 11 * b, storage=H.MODEL_F16,
-                               solver_kwargs=dict(max_iterations=MAX_ITERATIONS,
-                                                  accuracy=TOLERANCE))
-           for b in range(B)]
+                               solver_kwargs=kw) for b in range(B)]
     s64 = [WS.IntervalSequence(w, n_total, seed=args.seed + 11 * b, storage=H.MODEL_C64,
-                               solver_kwargs=dict(max_iterations=MAX_ITERATIONS,
-                                                  accuracy=TOLERANCE))
-           for b in range(B)]
-
+                               solver_kwargs=kw) for b in range(B)]
     # inputs of the evaluated intervals go resident first (contiguous device
-    # memory); training intervals then pass through one scratch problem each
+    # memory): FP16 for every stream, complex64 for the plain forms' streams
     for i in range(first, n_total):
         for b in range(B):
             _ = s16[b].problems[i]
+        for b in range(PB):
             _ = s64[b].problems[i]
-    log(f"built {2 * B * (n_total - first)} resident intervals "
+    log(f"built {(B + PB) * (n_total - first)} resident intervals "
         f"({time.perf_counter() - t_setup:.1f} s)")
 
     # ---- learned warm start: trained on the solver's own converged outputs
     ws = WS.WarmStart(hidden=256, seed=args.seed, stretch=args.stretch)
-    xs, ts, prevs = [], [], []
-    for q in s16:
-        x, t, prev = WS.training_set(q, ws, 0, TR, converge_iterations=1000, transient=True)
-        xs.append(x)
-        ts.append(t)
-        prevs.append(prev)
-        log(f"stream {len(prevs)}/{B}: {TR} training intervals solved "
-            f"({time.perf_counter() - t_setup:.1f} s)")
-    x, t = np.concatenate(xs), np.concatenate(ts)
+    sets = WS.training_sets_batched(s16, ws, 0, TR, converge_iterations=1000)
+    x = np.concatenate([s[0] for s in sets])
+    t = np.concatenate([s[1] for s in sets])
+    prevs = [s[2] for s in sets]
+    log(f"{B} x {TR} training intervals solved ({time.perf_counter() - t_setup:.1f} s)")
     t_fit = time.perf_counter()
     loss = ws.fit(x, t, epochs=10, batch=4096, lr=1e-3, seed=args.seed)
     t_fit = time.perf_counter() - t_fit
-    log(f"warm start: {len(x)} samples from {B} x {TR} intervals, loss {loss:.3e}, "
-        f"target variance {t.var():.3e}, fit {t_fit:.1f} s")
-
-    # ---- resident inputs of the evaluated intervals, converged chi^2 along the chain
-    conv = {}
-    long = [WS.sibling(q.solver, 1000, 1e-9, False) for q in s64]
-    chain = [p.copy() for p in prevs]
-    for i in range(first, n_total):
-        for b in range(B):
-            g = chain[b].copy()
-            long[b].solve_problem(s64[b].problems[i], g)
-            conv[(b, i)] = H.chi2(s64[b].solver, s64[b].problems[i], g)
-            chain[b] = g
+    log(f"warm start: {len(x)} samples, loss {loss:.3e}, target variance {t.var():.3e}, "
+        f"fit {t_fit:.1f} s")
     t_setup = time.perf_counter() - t_setup
-    log(f"setup (build, train, converged references) {t_setup:.1f} s")
-
-    def excess(b, i, g):
-        return H.chi2(s64[b].solver, s64[b].problems[i], g) / conv[(b, i)] - 1.0
 
     # ---- optimized forms: batched + FP16 + learned warm start on the device,
-    # with the streaming and the Gram iteration
+    # with the fused Gram and the streaming iteration
     def run_batched(batch, use_net):
         prev = [p.copy() for p in prevs]
-        t_all, warm_ms, chis, its = 0.0, 0.0, [], []
+        t_all, warm_ms, its, sols = 0.0, 0.0, [], {}
         for i in range(first, n_total):
             t0 = time.perf_counter()
             batch.rebind([q.problems[i] for q in s16])
@@ -226,18 +215,24 @@ def main():
                 t_all += dt
                 warm_ms += batch.last_warm_ms() if use_net else 0.0
                 for b in range(B):
-                    chis.append(excess(b, i, prev[b]))
-                    its.append(res[b].iterations)
-        return {"value": B * T / t_all, "seconds": t_all, "warm_ms": warm_ms, "chi2": chis,
-                "iterations": its}
+                    its.append(res[b].iterations + (1 + 2 * ws.stretch if use_net else 0))
+                    sols[(b, i)] = prev[b].copy()
+        return {"value": B * T / t_all, "seconds": t_all, "warm_ms": warm_ms, "iterations": its,
+                "solutions": sols}
 
-    def kernel_times(iteration, tuned):
-        """Mean correlate (or apply) and antenna solve + decision per batched
-        iteration over a fixed-length profiled solve in which every interval is
-        live in every launch (the last launches of these kernels in the run)."""
-        rb = H.Batch([WS.sibling(q.solver, ROOF_ITERATIONS, 0.0, False, iteration=iteration)
-                      for q in s16], [q.problems[n_total - 1] for q in s16])
+    def kernel_times(iteration, tuned, fused=True):
+        """Mean of the first and second timed part of a batched iteration over
+        a fixed-length profiled solve in which every interval is live in every
+        launch: (fused kernel, finishing kernel + decision) for the fused Gram
+        iteration, (apply or correlate, antenna solve + decision) otherwise."""
+        solvers = [WS.sibling(q.solver, ROOF_ITERATIONS, 0.0, False, iteration=iteration)
+                   for q in s16]
+        for s in solvers:
+            s.set_gram_fusion(fused)
+        rb = H.Batch(solvers, [q.problems[n_total - 1] for q in s16])
         rb.set_schedule(tuned[0], tuned[1], 0, 8)
+        if fused and iteration == H.ITERATION_GRAM:
+            rb.set_gram_solve_groups(tuned[2])
         rb.set_profiling(True)
         rb.solve([q.unit() for q in s16])
         corr_ms, solve_ms, n_it = rb.last_timing()
@@ -250,52 +245,22 @@ def main():
                    for q in s16]
         batch = H.Batch(solvers, [q.problems[first] for q in s16])
         tuned = batch.autotune()
+        groups = batch.gram_solve_groups()
         batch.set_schedule(tuned[0], tuned[1], GRAPH, 8)
         r = run_batched(batch, True)
         r["without_network"] = run_batched(batch, False)
-        r["schedule"] = tuned
-        r["kernels_ms"] = kernel_times(iteration, tuned)
+        r["schedule"] = (tuned[0], tuned[1], groups)
+        r["kernels_ms"] = kernel_times(iteration, r["schedule"])
         opt[name] = r
         log(f"optimized ({name} iteration): {r['value']:.2f} solves/s, without network "
-            f"{r['without_network']['value']:.2f}; per batched iteration: correlate "
-            f"{r['kernels_ms'][0] * 1e3:.1f} us, antenna solve + decision "
-            f"{r['kernels_ms'][1] * 1e3:.1f} us")
+            f"{r['without_network']['value']:.2f}; per batched iteration: "
+            f"{r['kernels_ms'][0] * 1e3:.1f} us + {r['kernels_ms'][1] * 1e3:.1f} us")
         batch.close()
+    unfused = kernel_times(H.ITERATION_GRAM, opt["gram"]["schedule"], fused=False)
+    log(f"unfused Gram iteration: apply {unfused[0] * 1e3:.1f} us, antenna solve + decision "
+        f"{unfused[1] * 1e3:.1f} us")
     best = opt["gram"]
     value, t_opt, warm_ms = best["value"], best["seconds"], best["warm_ms"]
-
-    n_vis = w.n_baselines * w.n_channels * w.solint
-    bl_stride = -(-w.n_baselines // 64) * 64
-    partial_bytes = w.n_channel_blocks * w.n_directions * 2 * 3 * bl_stride * 4
-    # the apply writes U (2 of the 3 partials per correlation); S is written
-    # once per solve by the cross-products pass
-    apply_partial_bytes = w.n_channel_blocks * w.n_directions * 2 * 2 * bl_stride * 4
-    solution_bytes = w.n_channel_blocks * w.n_stations * w.n_sub * 2 * 16
-    gram_bytes = w.n_channel_blocks * 2 * (w.n_directions + w.n_directions *
-                                           (w.n_directions + 1) // 2) * bl_stride * 8
-
-    def roof(name, nbytes, ms, **extra):
-        achieved = nbytes / (ms * 1e-3) / 1e9
-        out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-               "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-               "traffic": measured_traffic(name), "kernel": name,
-               "algorithmic_bytes_per_launch": int(nbytes), "mean_launch_ms": round(ms, 5)}
-        out.update(extra)
-        return out
-
-    g_apply, g_solve = opt["gram"]["kernels_ms"]
-    s_corr, s_solve = opt["streaming"]["kernels_ms"]
-    rooflines = {
-        "gram_apply": roof("gram_apply", B * (gram_bytes + apply_partial_bytes), g_apply,
-                           note="per-baseline cross products read, correlate partials U written "
-                                "(S, constant over the solve, comes from the cross-products pass)"),
-        "antenna_solve": roof("antenna_solve", B * (partial_bytes + 2 * solution_bytes), g_solve,
-                              note="partials read, solutions read and written; with the "
-                                   "decision kernel; bound by reduction latency, not HBM"),
-        "batched_correlate_f16": roof("batched_correlate_f16", B * n_vis * (16 + w.n_directions * 8),
-                                      s_corr),
-    }
-    dominant = "gram_apply" if g_apply >= g_solve else "antenna_solve"
 
     # ---- plain forms: complex64, one interval at a time
     plain = s64[0].solver
@@ -307,23 +272,101 @@ def main():
         s = WS.sibling(plain, limit, TOLERANCE)
         s.set_schedule(waves_per_tile=p_tuned[0], rows_per_trip=p_tuned[1], use_graph=GRAPH,
                        poll_interval=8)
-        prev = [p.copy() for p in prevs]
-        t_all, chis, its = 0.0, [], []
+        prev = [p.copy() for p in prevs[:PB]]
+        t_all, its, sols = 0.0, [], {}
         for i in range(first, n_total):
-            for b, q in enumerate(s64):
-                g = q.unit() if start == "unit" else prev[b].copy()
+            for b in range(PB):
+                g = s64[b].unit() if start == "unit" else prev[b].copy()
                 t0 = time.perf_counter()
-                r = s.solve_problem(q.problems[i], g)
+                r = s.solve_problem(s64[b].problems[i], g)
                 dt = time.perf_counter() - t0
                 prev[b] = g
                 if i in timed:
                     t_all += dt
-                    chis.append(excess(b, i, g))
                     its.append(r.iterations)
-        results[name] = (B * T / t_all, chis, its)
-        log(f"{name}: {results[name][0]:.2f} solves/s, mean iterations {np.mean(its):.1f}, "
-            f"chi2 excess max {np.max(chis):.2e}")
-    # roofline of the complex64 correlate (plain form), fixed length as above
+                    sols[(b, i)] = g.copy()
+        results[name] = {"value": PB * T / t_all, "iterations": its, "solutions": sols}
+        log(f"{name}: {results[name]['value']:.2f} solves/s, mean iterations {np.mean(its):.1f}")
+
+    # ---- equal chi^2: every form's final chi^2 against the interval's
+    # converged chi^2, on complex64 model planes (rebuilt for the streams the
+    # plain forms did not keep resident), converged references batched
+    long = [WS.sibling(q.solver, 1000, 1e-9, False, iteration=H.ITERATION_GRAM) for q in s64]
+    chain = [p.copy() for p in prevs]
+    excess = {k: [] for k in ("optimized", "optimized_without_network", "optimized_streaming",
+                              "optimized_streaming_without_network", "plain", "plain_100",
+                              "plain_previous")}
+    sources = {"optimized": best["solutions"],
+               "optimized_without_network": best["without_network"]["solutions"],
+               "optimized_streaming": opt["streaming"]["solutions"],
+               "optimized_streaming_without_network":
+                   opt["streaming"]["without_network"]["solutions"],
+               "plain": results["plain"]["solutions"], "plain_100": results["plain_100"]["solutions"],
+               "plain_previous": results["plain_previous"]["solutions"]}
+    lb = None
+    for i in range(first, n_total):
+        dps = [s64[b].problems[i] if b < PB else s64[b].transient(i) for b in range(B)]
+        if lb is None:
+            lb = H.Batch(long, dps)
+        else:
+            lb.rebind(dps)
+        g = [c.copy() for c in chain]
+        lb.solve(g)
+        chain = g
+        if i not in timed:
+            continue
+        for b in range(B):
+            conv = H.chi2(s64[b].solver, dps[b], g[b])
+            for k, src in sources.items():
+                if (b, i) in src:
+                    excess[k].append(H.chi2(s64[b].solver, dps[b], src[(b, i)]) / conv - 1.0)
+    lb.close()
+    for k in excess:
+        log(f"{k}: chi2 excess max {np.max(excess[k]):.2e}")
+
+    # ---- rooflines of the optimized form's batched iteration
+    n_vis = w.n_baselines * w.n_channels * w.solint
+    bl_stride = -(-w.n_baselines // 64) * 64
+    nq = w.n_directions + w.n_directions * (w.n_directions + 1) // 2
+    cross_bytes = w.n_channel_blocks * 2 * nq * w.n_baselines * 8  # per interval
+    solution_bytes = w.n_channel_blocks * w.n_stations * w.n_sub * 2 * 16
+    partial_bytes = w.n_channel_blocks * w.n_directions * 2 * 3 * bl_stride * 4
+    apply_partial_bytes = w.n_channel_blocks * w.n_directions * 2 * 2 * bl_stride * 4
+
+    def roof(name, nbytes, ms, **extra):
+        achieved = nbytes / (ms * 1e-3) / 1e9
+        out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+               "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+               "traffic": measured_traffic(name), "kernel": name,
+               "algorithmic_bytes_per_launch": int(nbytes), "mean_launch_ms": round(ms, 5)}
+        out.update(extra)
+        return out
+
+    g_solve, g_rest = opt["gram"]["kernels_ms"]
+    s_corr, s_solve = opt["streaming"]["kernels_ms"]
+    rooflines = {
+        "gram_solve": roof(
+            "gram_solve", B * cross_bytes, g_solve,
+            note=("fused Gram iteration kernel over the batch: the per-baseline cross products "
+                  "C_d and G_e,d (44 complex64 per baseline and correlation) read once; per "
+                  f"antenna FP64 sums per row group written ({opt['gram']['schedule'][2]} row "
+                  "groups per polarization)")),
+        "gram_finish_decide": roof(
+            "gram_finish_decide", B * 2 * solution_bytes, g_rest,
+            note="finishing kernel (row-group sums added, FP64 divide, step, convergence "
+                 "terms) and the decision kernel; solutions read and written; latency-bound"),
+        "unfused_gram_apply": roof(
+            "gram_apply", B * (cross_bytes * bl_stride // w.n_baselines + apply_partial_bytes),
+            unfused[0], note="the apply kernel the fused iteration replaces: cross products "
+                             "read, correlate partials U written"),
+        "unfused_antenna_solve": roof(
+            "antenna_solve", B * (partial_bytes + 2 * solution_bytes), unfused[1],
+            note="the antenna solve + decision the fused iteration replaces"),
+        "batched_correlate_f16": roof("batched_correlate_f16",
+                                      B * n_vis * (16 + w.n_directions * 8), s_corr),
+    }
+
+    # ---- plain form's complex64 correlate, fixed length as above
     s = WS.sibling(plain, ROOF_ITERATIONS, 0.0, False)
     s.set_schedule(waves_per_tile=p_tuned[0], rows_per_trip=p_tuned[1], use_graph=0,
                    poll_interval=8)
@@ -332,7 +375,74 @@ def main():
     s.solve_problem(s64[0].problems[first], g)
     pms, pn, _ = s.last_timing()
     p_avg = pms / max(pn, 1)
-    p_achieved = algorithmic_bytes(w, 16) / (p_avg * 1e-3) / 1e9
+    p_roof = roof("correlate_c64", n_vis * (16 + w.n_directions * 16), p_avg)
+
+    # ---- the predict feeding the solver: one interval's model planes
+    # (8 directions x 20 components, complex64 and FP16), FP64 arithmetic
+    q = s64[0]
+    geom = S.make_geometry(w, None, start_step=first * w.solint, station_enu=q.geom0.station_enu)
+    comps = H.components_from_sky(q.sky)
+    pdp = q.problems[first]
+    pdp.predict(geom.ref, geom.uvw, geom.freq, geom.chan_width, comps)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(3):
+        pdp.predict(geom.ref, geom.uvw, geom.freq, geom.chan_width, comps)
+    predict_ms = (time.perf_counter() - t0) / 3 * 1e3
+    n_gauss = int(np.sum(q.sky.is_gaussian))
+    n_point = len(q.sky.is_gaussian) - n_gauss
+    bl_ch_t = w.n_baselines * w.n_channels * w.solint
+    # FP64 operations per component and visibility: the baseline phasor
+    # shift_q conj(shift_p) (6), times the Stokes-I spectrum (2), the
+    # amplitude (2) and the accumulation (2); a Gaussian adds its envelope
+    # exp(f^2 k uv') (2 + the exponential, counted as one)
+    flops = bl_ch_t * (12 * n_point + 15 * n_gauss)
+    pred_gflops = flops / (predict_ms * 1e-3) / 1e9
+    predict_line = {
+        "bound": "valu", "achieved": round(pred_gflops, 1), "peak": FP64_PEAK_GFLOPS,
+        "unit": "GFLOP/s (FP64)", "frac": round(pred_gflops / FP64_PEAK_GFLOPS, 4),
+        "traffic": None, "kernel": "predict", "mean_launch_ms": round(predict_ms, 4),
+        "note": (f"ddh_predict of one interval: {len(q.sky.is_gaussian)} components "
+                 f"({n_point} points, {n_gauss} Gaussians) x {bl_ch_t} visibilities, FP64 "
+                 "phasors, envelope and sums rounded once to complex64; wall time of the call "
+                 "(host staging of the component list and UVW included); 12 FP64 operations "
+                 "per point and 15 per Gaussian component-visibility counted; the station "
+                 "phasor sincos and the Gaussian exp cost more than their count")}
+
+    # ---- the drop-in: ddh_solve from host SolveData (what DDECal calls),
+    # including the repack upload over PCIe, plain settings
+    P0 = S.download_problem(s64[0].problems[timed[0]], w)
+    views, keep = drop_in_views(P0)
+    di = WS.sibling(plain, MAX_ITERATIONS, TOLERANCE)
+    di.set_schedule(waves_per_tile=p_tuned[0], rows_per_trip=p_tuned[1], use_graph=GRAPH,
+                    poll_interval=8)
+    dres = H.Result()
+    for rep in range(2):  # the second call is timed
+        gd = s64[0].unit()
+        ptrs = di._sol_ptrs(gd)
+        t0 = time.perf_counter()
+        rc = H.lib().ddh_solve(di.handle, views, P0.n_channel_blocks, H.C.cast(ptrs, H.C.c_void_p),
+                               0.0, H.C.byref(dres))
+        drop_s = time.perf_counter() - t0
+        assert rc == 0, H.last_error()
+    dev_solve = WS.sibling(plain, MAX_ITERATIONS, TOLERANCE)
+    dev_solve.set_schedule(waves_per_tile=p_tuned[0], rows_per_trip=p_tuned[1],
+                           use_graph=GRAPH, poll_interval=8)
+    gr = s64[0].unit()
+    t0 = time.perf_counter()
+    dev_solve.solve_problem(s64[0].problems[timed[0]], gr)
+    resident_s = time.perf_counter() - t0
+    drop_in = {"solves_per_s_pcie_inclusive": round(1.0 / drop_s, 3),
+               "seconds": round(drop_s, 4), "resident_solve_seconds": round(resident_s, 4),
+               "iterations": int(dres.iterations),
+               "upload_bytes": int(P0.data.nbytes + P0.model.nbytes + P0.ant1.nbytes * 2 +
+                                   P0.sol_map.nbytes),
+               "note": ("ddh_solve on host SolveData arrays of one LOFAR-scale interval "
+                        "(complex64), plain settings from unit gains: structure check, repack "
+                        "upload over PCIe, solve, solutions back; resident_solve_seconds is the "
+                        "same solve on the device-resident interval")}
+    del keep
+    log(f"drop-in ddh_solve {drop_s * 1e3:.1f} ms (resident {resident_s * 1e3:.1f} ms); "
+        f"predict {predict_ms:.2f} ms per interval")
 
     # ---- the step after the solve, DDECal::CorrectAndSubtractModels, on one
     # interval of the same shape in DPBuffer order (autocorrelations included)
@@ -387,6 +497,8 @@ def main():
     sub_vis = w.solint * n_bl_all * w.n_channels
     cross_vis = w.solint * int(cross.sum()) * w.n_channels
     components = {
+        "predict": predict_line,
+        "drop_in_ddh_solve": drop_in,
         "correct_and_subtract": roof(
             "correct_and_subtract", sub_vis * (64 + w.n_directions * 32), sub_ms,
             note=(f"DDECal::CorrectAndSubtractModels over one interval: {sub_vis} visibilities "
@@ -399,8 +511,7 @@ def main():
             note=(f"AssignAndWeight + SolveData packing of one interval into an FP16 device "
                   f"problem: the {cross_vis} raw cross-correlation visibilities (data, flags, "
                   "weights, every model) read once and written packed (Duo); the autocorrelations "
-                  "are not read; two passes over the raw buffers (the first finds the FP16 "
-                  "scale)")),
+                  "are not read")),
         "full_jones_correlate": roof(
             "full_jones_correlate", cross_vis * (32 + w.n_directions * 32), fj_corr,
             iteration_ms=round(fj_iter, 5),
@@ -412,8 +523,57 @@ def main():
     log(f"correct and subtract: {sub_ms * 1e3:.1f} us per interval; assign and weight "
         f"(FP16 problem): {ing_ms * 1e3:.1f} us per interval")
 
-    v_plain = results["plain"][0]
-    p_roof = roof("correlate_c64", algorithmic_bytes(w, 16), p_avg)
+    # ---- gains against the oracle (reference precision) and the FP64 oracle
+    # on the first evaluated interval of stream 0: one full CPU solve of the
+    # plain form from unit gains (the CPU baseline) and one FP64 solve
+    b0, i0 = 0, timed[0]
+    accuracy_line = None
+    cpu = None
+    if not args.no_cpu:
+        from oracle import pyoracle as O
+        ref = P0.unit_solutions()
+        t0 = time.perf_counter()
+        rr = O.solve(P0, ref, max_iterations=MAX_ITERATIONS, accuracy=TOLERANCE,
+                     n_threads=args.cpu_threads)
+        cpu_s = time.perf_counter() - t0
+        ref64 = P0.unit_solutions()
+        rr64 = O.solve(P0, ref64, max_iterations=MAX_ITERATIONS, accuracy=TOLERANCE, fp64=True,
+                       n_threads=args.cpu_threads)
+
+        def err(a, b):
+            ra = S.phase_referenced(a, w.n_stations, w.n_sub)
+            rb = S.phase_referenced(b, w.n_stations, w.n_sub)
+            return float(np.max(np.abs(ra - rb) / np.abs(rb)))
+
+        plain_sol = results["plain"]["solutions"][(b0, i0)]
+        opt_sol = best["solutions"][(b0, i0)]
+        c_ref = O.chi2(P0, ref64)
+        accuracy_line = {
+            "interval": f"stream {b0}, interval {i0}",
+            "plain_vs_oracle": err(plain_sol, ref),
+            "plain_vs_fp64_oracle": err(plain_sol, ref64),
+            "optimized_vs_fp64_oracle": err(opt_sol, ref64),
+            "oracle_vs_fp64_oracle": err(ref, ref64),
+            "iterations": {"plain_gpu": int(results["plain"]["iterations"][0]),
+                           "oracle": int(rr.iterations), "fp64_oracle": int(rr64.iterations)},
+            "chi2_excess_vs_fp64_oracle": {
+                "plain": O.chi2(P0, plain_sol) / c_ref - 1.0,
+                "optimized": O.chi2(P0, opt_sol) / c_ref - 1.0,
+                "oracle": O.chi2(P0, ref) / c_ref - 1.0},
+            "note": ("gains after referencing to antenna 0, max relative difference; the plain "
+                     "GPU form and the oracles start from unit gains, the optimized form from "
+                     "its learned warm start; all stop at tolerance 1e-6")}
+        cpu = {"value": round(1.0 / cpu_s, 5), "unit": "solves/s", "cores": args.cpu_threads,
+               "kind": "port",
+               "sample": (f"one full plain solve of one {w.name} interval on the host: "
+                          f"{rr.iterations} iterations in {cpu_s:.1f} s with "
+                          f"{args.cpu_threads} threads over channel blocks"),
+               "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+        log(f"CPU oracle: {cpu_s:.1f} s for {rr.iterations} iterations; gains vs oracle "
+            f"{accuracy_line['plain_vs_oracle']:.2e}")
+
+    v_plain = results["plain"]["value"]
+    dominant = "gram_solve"
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -425,7 +585,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32 arithmetic, fp16 model storage (f64 convergence, step, sums, chi2)",
         "data": ("synthetic: seeded station layouts, HIP-predicted point+Gaussian sky, drifting "
                  "gains, noise 0.01 Jy; data synthesized on the device"),
         "config": {
@@ -434,34 +594,46 @@ def main():
                          f"step 0.2, tolerance 1e-6, iteration limit {MAX_ITERATIONS}"),
             "form": (f"optimized: {B} interval streams batched, FP16 model planes with FP32 "
                      "accumulation and FP64 convergence test, learned warm start (3-layer "
-                     "256-hidden bf16 MLP) on the device, Gram iteration, runs to tolerance"),
+                     "256-hidden bf16 MLP) on the device, fused Gram iteration, runs to "
+                     "tolerance"),
             "step": f"one interval of each of {B} streams (a batch of {B} solves)",
             "streams": B,
+            "plain_streams": PB,
             "train_intervals_per_stream": TR,
+            "evaluated_intervals_per_stream": T,
             "train_samples": int(len(x)),
             "warm_start_stretch": ws.stretch,
             "warm_start_device_ms_per_step": round(warm_ms / T, 3),
-            "optimized": summary(best["chi2"], best["iterations"]),
+            "optimized": summary(excess["optimized"], best["iterations"]),
             "speedup_vs_plain": round(value / v_plain, 3),
             "plain_solves_per_s": round(v_plain, 3),
-            "plain": summary(results["plain"][1], results["plain"][2]),
+            "plain": summary(excess["plain"], results["plain"]["iterations"]),
             "optimized_streaming_solves_per_s": round(opt["streaming"]["value"], 3),
-            "optimized_streaming": summary(opt["streaming"]["chi2"], opt["streaming"]["iterations"]),
+            "optimized_streaming": summary(excess["optimized_streaming"],
+                                           opt["streaming"]["iterations"]),
             "speedup_streaming_vs_plain": round(opt["streaming"]["value"] / v_plain, 3),
             "optimized_without_network_solves_per_s": round(best["without_network"]["value"], 3),
-            "optimized_without_network": summary(best["without_network"]["chi2"],
+            "optimized_without_network": summary(excess["optimized_without_network"],
                                                  best["without_network"]["iterations"]),
-            "plain_100_iterations_solves_per_s": round(results["plain_100"][0], 3),
-            "plain_100_iterations": summary(results["plain_100"][1], results["plain_100"][2]),
-            "plain_previous_start_solves_per_s": round(results["plain_previous"][0], 3),
-            "plain_previous_start": summary(results["plain_previous"][1],
-                                            results["plain_previous"][2]),
+            "plain_100_iterations_solves_per_s": round(results["plain_100"]["value"], 3),
+            "plain_100_iterations": summary(excess["plain_100"],
+                                            results["plain_100"]["iterations"]),
+            "plain_previous_start_solves_per_s": round(results["plain_previous"]["value"], 3),
+            "plain_previous_start": summary(excess["plain_previous"],
+                                            results["plain_previous"]["iterations"]),
+            "iterations_note": ("iterations of the forms with the learned warm start include "
+                                f"its {1 + 2 * ws.stretch} iterations (one undamped update and "
+                                "two stretches)"),
             "per_batched_iteration_us": {
-                "gram_apply": round(g_apply * 1e3, 1), "gram_antenna_solve": round(g_solve * 1e3, 1),
+                "gram_solve": round(g_solve * 1e3, 1),
+                "gram_finish_decide": round(g_rest * 1e3, 1),
+                "unfused_gram_apply": round(unfused[0] * 1e3, 1),
+                "unfused_gram_antenna_solve_decide": round(unfused[1] * 1e3, 1),
                 "streaming_correlate": round(s_corr * 1e3, 1),
-                "streaming_antenna_solve": round(s_solve * 1e3, 1)},
+                "streaming_antenna_solve_decide": round(s_solve * 1e3, 1)},
             "batch_schedule": {"waves_per_tile": opt["gram"]["schedule"][0],
-                               "rows_per_trip": opt["gram"]["schedule"][1]},
+                               "rows_per_trip": opt["gram"]["schedule"][1],
+                               "gram_solve_row_groups": opt["gram"]["schedule"][2]},
             "batch_schedule_streaming": {"waves_per_tile": opt["streaming"]["schedule"][0],
                                          "rows_per_trip": opt["streaming"]["schedule"][1]},
             "plain_schedule": {"waves_per_tile": p_tuned[0], "rows_per_trip": p_tuned[1]},
@@ -472,14 +644,14 @@ def main():
         "roofline_other": {k: v for k, v in rooflines.items() if k != dominant},
         "components": components,
     }
+    if accuracy_line:
+        line["config"]["gain_error"] = accuracy_line
     if args.calibrate:
         copy_gbs, read_gbs = H.calibrate_bandwidth(1 << 30, 10)
         line["calibration"] = {"copy_GBps": round(copy_gbs, 1), "read_GBps": round(read_gbs, 1),
                                "bytes": 1 << 30}
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(s64[0].problems[first], w,
-                                            float(np.mean(results["plain"][2])),
-                                            args.cpu_iterations, args.cpu_threads)
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
 
 

@@ -55,6 +55,32 @@ struct DeviceBuffer {
   }
 };
 
+// Pinned host memory for staging solution transfers: one DMA per interval
+// instead of one pageable copy per channel block.
+struct PinnedBuffer {
+  unsigned char* ptr = nullptr;
+  size_t bytes = 0;
+  PinnedBuffer() = default;
+  PinnedBuffer(const PinnedBuffer&) = delete;
+  PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+  ~PinnedBuffer() {
+    if (ptr) (void)hipHostFree(ptr);
+  }
+  template <typename T>
+  T* Get(size_t count) {
+    const size_t n = count * sizeof(T);
+    if (n > bytes || !ptr) {
+      if (ptr) (void)hipHostFree(ptr);
+      ptr = nullptr;
+      bytes = 0;
+      DDH_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&ptr), std::max<size_t>(n, 1),
+                                  hipHostMallocDefault));
+      bytes = n;
+    }
+    return reinterpret_cast<T*>(ptr);
+  }
+};
+
 struct SolverConfig {
   ddh_config c;
   std::vector<int32_t> n_sol_dir;
@@ -132,6 +158,7 @@ struct ddh_solver {
   hipGraphExec_t graph_exec = nullptr;
   std::vector<const void*> graph_key;
   ddh::DeviceState* host_state = nullptr;  // pinned
+  ddh::PinnedBuffer stage;                 // pinned staging of solutions
   // timing of the last solve
   std::vector<hipEvent_t> events;
   int profile = 0;
@@ -696,14 +723,47 @@ void SolutionsOut(const ddh_solver& s, double* host, const void* dev, size_t n_p
     for (size_t w = 0; w < words; ++w) host[i * words + w] = tmp[2 * i * words + w];
 }
 
+// All channel blocks' solutions of one solver between the caller's per-block
+// arrays and the device buffer `dev` ([n_cb][n_ant][n_sub][n_pol]), through
+// pinned staging: the blocks are gathered (or scattered) on the host and move
+// in one DMA. Diagonal and full-Jones layouts only (the scalar layout is
+// converted per block by SolutionsIn / SolutionsOut).
+void BulkIn(const ddh_solver& s, PinnedBuffer& stage, double2* dev, double* const* host,
+            hipStream_t stream) {
+  const ddh_config& c = s.cfg.c;
+  const size_t per_cb = size_t(c.n_antennas) * s.cfg.n_sub * DevicePols(c);
+  double2* st = stage.Get<double2>(per_cb * c.n_channel_blocks);
+  DDH_HIP_CHECK(hipStreamSynchronize(stream));  // the stage may still feed an earlier copy
+  for (int cb = 0; cb < c.n_channel_blocks; ++cb)
+    std::memcpy(st + cb * per_cb, host[cb], per_cb * sizeof(double2));
+  DDH_HIP_CHECK(hipMemcpyAsync(dev, st, per_cb * c.n_channel_blocks * sizeof(double2),
+                               hipMemcpyHostToDevice, stream));
+}
+
+void BulkOut(const ddh_solver& s, PinnedBuffer& stage, double* const* host, const double2* dev,
+             hipStream_t stream) {
+  const ddh_config& c = s.cfg.c;
+  const size_t per_cb = size_t(c.n_antennas) * s.cfg.n_sub * DevicePols(c);
+  double2* st = stage.Get<double2>(per_cb * c.n_channel_blocks);
+  DDH_HIP_CHECK(hipMemcpyAsync(st, dev, per_cb * c.n_channel_blocks * sizeof(double2),
+                               hipMemcpyDeviceToHost, stream));
+  DDH_HIP_CHECK(hipStreamSynchronize(stream));
+  for (int cb = 0; cb < c.n_channel_blocks; ++cb)
+    std::memcpy(host[cb], st + cb * per_cb, per_cb * sizeof(double2));
+}
+
 void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* result) {
   const ddh_config& c = s.cfg.c;
   CheckMatch(s, p);
   const size_t n_sols = size_t(c.n_antennas) * s.cfg.n_sub;
   const size_t sol_per_cb = n_sols * DevicePols(c);
   PrepareBuffers(s, p);
-  for (int cb = 0; cb < c.n_channel_blocks; ++cb)
-    SolutionsIn(s, s.sol0.ptr + cb * sol_per_cb, solutions[cb], n_sols, s.stream);
+  if (c.n_solution_polarizations == 1) {
+    for (int cb = 0; cb < c.n_channel_blocks; ++cb)
+      SolutionsIn(s, s.sol0.ptr + cb * sol_per_cb, solutions[cb], n_sols, s.stream);
+  } else {
+    BulkIn(s, s.stage, s.sol0.ptr, solutions, s.stream);
+  }
   const IterArgs a = MakeArgs(s, p);
   const int limit = std::max(c.max_iterations, c.min_iterations);
   const bool graph = s.use_graph && !s.profile;
@@ -754,9 +814,13 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
   const DeviceState st = *s.host_state;
   DDH_REQUIRE(st.done, -3, "solver did not reach its stopping criterion");
   const double2* final_sol = (st.iteration & 1) ? s.sol1.ptr : s.sol0.ptr;
-  for (int cb = 0; cb < c.n_channel_blocks; ++cb)
-    SolutionsOut(s, solutions[cb], final_sol + cb * sol_per_cb, n_sols, s.stream);
-  DDH_HIP_CHECK(hipStreamSynchronize(s.stream));
+  if (c.n_solution_polarizations == 1) {
+    for (int cb = 0; cb < c.n_channel_blocks; ++cb)
+      SolutionsOut(s, solutions[cb], final_sol + cb * sol_per_cb, n_sols, s.stream);
+    DDH_HIP_CHECK(hipStreamSynchronize(s.stream));
+  } else {
+    BulkOut(s, s.stage, solutions, final_sol, s.stream);
+  }
   float ms = 0.0f;
   DDH_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
   s.last_solve_ms = ms;
@@ -1320,6 +1384,7 @@ struct ddh_batch {
   ddh::DeviceBuffer<ddh::BatchTile> d_tiles, d_items, d_pro_items, d_prod_tiles;
   ddh::DeviceBuffer<ddh::DeviceState> d_states;
   ddh::DeviceState* host_states = nullptr;
+  ddh::PinnedBuffer stage;  // pinned staging of every interval's solutions
   hipGraphExec_t graph_exec = nullptr;
   int graph_poll = 0, graph_waves = 0, graph_unroll = 0;
   double last_ms = 0.0;
@@ -1536,15 +1601,31 @@ size_t SolutionCount(const ddh_batch& b, int i) {
 }
 
 // Host solutions[i][cb] -> dst(i) on the batch stream.
+// Host solutions[i][cb] -> dst(i) on the batch stream: every interval's blocks
+// gathered into one pinned stage and copied with one DMA per interval
+// (scalar layouts: per block, converted).
 template <typename Dst>
 void UploadSolutions(ddh_batch& b, double* const* const* solutions, Dst&& dst) {
   const int n = int(b.solvers.size());
+  size_t total = 0;
+  for (int i = 0; i < n; ++i) total += SolutionCount(b, i);
+  double2* st = b.stage.Get<double2>(total);
+  DDH_HIP_CHECK(hipStreamSynchronize(b.stream));  // the stage may still feed an earlier copy
+  size_t off = 0;
   for (int i = 0; i < n; ++i) {
     const ddh_solver& s = *b.solvers[i];
     const size_t n_sols = size_t(s.cfg.c.n_antennas) * s.cfg.n_sub;
     const size_t per_cb = n_sols * DevicePols(s.cfg.c);
-    for (int cb = 0; cb < s.cfg.c.n_channel_blocks; ++cb)
-      SolutionsIn(s, dst(i) + cb * per_cb, solutions[i][cb], n_sols, b.stream);
+    if (s.cfg.c.n_solution_polarizations == 1) {
+      for (int cb = 0; cb < s.cfg.c.n_channel_blocks; ++cb)
+        SolutionsIn(s, dst(i) + cb * per_cb, solutions[i][cb], n_sols, b.stream);
+    } else {
+      for (int cb = 0; cb < s.cfg.c.n_channel_blocks; ++cb)
+        std::memcpy(st + off + cb * per_cb, solutions[i][cb], per_cb * sizeof(double2));
+      DDH_HIP_CHECK(hipMemcpyAsync(dst(i), st + off, SolutionCount(b, i) * sizeof(double2),
+                                   hipMemcpyHostToDevice, b.stream));
+    }
+    off += SolutionCount(b, i);
   }
 }
 
@@ -1682,16 +1763,19 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
           // correlate alone between the first two events, antenna solve and
           // decision between the last two
           DDH_HIP_CHECK(hipEventRecord(b.events[3 * it], b.stream));
-          if (b.fused)  // apply and antenna solve in one kernel
+          if (b.fused)  // apply and antenna solve in one kernel (its sums)
             LaunchBatchGramSolve(b.d_args.ptr, b.d_gs_items.ptr, b.gs_launch.data(),
-                                 int(b.gs_launch.size()), b.f16, b.d_pro_items.ptr,
-                                 int(b.pro_items.size()), b.stream);
+                                 int(b.gs_launch.size()), b.f16, b.d_pro_items.ptr, 0, b.stream);
           else
             LaunchBatchStep(b, b.d_args.ptr, 0, 0);
           DDH_HIP_CHECK(hipEventRecord(b.events[3 * it + 1], b.stream));
-          if (b.fused)
+          if (b.fused) {
+            // the finishing kernel alone (no sums groups), then the decision
+            LaunchBatchGramSolve(b.d_args.ptr, b.d_gs_items.ptr, b.gs_launch.data(), 0, b.f16,
+                                 b.d_pro_items.ptr, int(b.pro_items.size()), b.stream);
             LaunchBatchSmoothDecide(b.d_args.ptr, b.d_items.ptr, int(b.items.size()),
                                     int(b.solvers.size()), b.smooth, b.stream);
+          }
           else if (b.full_jones)
             LaunchBatchFullJones(b.d_args.ptr, b.d_tiles.ptr, b.groups.data(), 0, b.d_items.ptr,
                                  int(b.items.size()), int(b.solvers.size()), b.solve_threads, b.f16,
@@ -1716,14 +1800,37 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
   DDH_HIP_CHECK(hipEventRecord(t1, b.stream));
   DDH_HIP_CHECK(hipStreamSynchronize(b.stream));
   DDH_REQUIRE(all_done, -3, "a batched interval did not reach its stopping criterion");
+  // final solutions: one DMA per interval into the pinned stage, then
+  // scattered to the caller's blocks
+  size_t total = 0;
+  for (int i = 0; i < n; ++i) total += SolutionCount(b, i);
+  double2* stage = b.stage.Get<double2>(total);
+  size_t off = 0;
   for (int i = 0; i < n; ++i) {
     const ddh_solver& s = *b.solvers[i];
     const DeviceState st = b.host_states[i];
     const size_t n_sols = size_t(s.cfg.c.n_antennas) * s.cfg.n_sub;
     const size_t per_cb = n_sols * DevicePols(s.cfg.c);
     const double2* fin = (st.iteration & 1) ? b.args[i].sol1 : b.args[i].sol0;
-    for (int cb = 0; cb < s.cfg.c.n_channel_blocks; ++cb)
-      SolutionsOut(s, solutions[i][cb], fin + cb * per_cb, n_sols, b.stream);
+    if (s.cfg.c.n_solution_polarizations == 1) {
+      for (int cb = 0; cb < s.cfg.c.n_channel_blocks; ++cb)
+        SolutionsOut(s, solutions[i][cb], fin + cb * per_cb, n_sols, b.stream);
+    } else {
+      DDH_HIP_CHECK(hipMemcpyAsync(stage + off, fin, SolutionCount(b, i) * sizeof(double2),
+                                   hipMemcpyDeviceToHost, b.stream));
+    }
+    off += SolutionCount(b, i);
+  }
+  DDH_HIP_CHECK(hipStreamSynchronize(b.stream));
+  off = 0;
+  for (int i = 0; i < n; ++i) {
+    const ddh_solver& s = *b.solvers[i];
+    const DeviceState st = b.host_states[i];
+    const size_t per_cb = size_t(s.cfg.c.n_antennas) * s.cfg.n_sub * DevicePols(s.cfg.c);
+    if (s.cfg.c.n_solution_polarizations != 1)
+      for (int cb = 0; cb < s.cfg.c.n_channel_blocks; ++cb)
+        std::memcpy(solutions[i][cb], stage + off + cb * per_cb, per_cb * sizeof(double2));
+    off += SolutionCount(b, i);
     if (results) {
       results[i].iterations = st.has_converged ? st.iteration : st.iteration + 1;
       results[i].constraint_iterations = 0;
@@ -2107,5 +2214,6 @@ int ddh_chi2(ddh_solver* s, ddh_problem* p, double* const* solutions, double* ch
     if (c.n_correlations == 1) *chi2 *= 0.5;
   });
 }
+
 
 }  // extern "C"

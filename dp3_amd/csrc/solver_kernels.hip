@@ -1043,30 +1043,42 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
   double acc[ND][3];
 #pragma unroll
   for (int d = 0; d < ND; ++d) acc[d][0] = acc[d][1] = acc[d][2] = 0.0;
-  for (int r = r0 + wave; r < r1; r += kGramSolveWaves) {
-    const int step = r + 1;
-    const uint32_t slot = uint32_t(r) * kWave + uint32_t(lane);
-    f2v q[NQ];
+  // Software pipelined: the next row's terms are loaded as soon as this
+  // row's U is formed, so they arrive while the FP64 part runs.
+  f2v q[NQ];
+  auto load_row = [&](int r) {
+    const uint32_t off = (uint32_t(r) * kWave + uint32_t(lane)) * 8u;
 #pragma unroll
     for (int i = 0; i < NQ; ++i)
-      q[i] = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(
-                                         grsrc, slot * 8u, uint32_t(i) * slots * 8u, 0));
+      q[i] = __builtin_bit_cast(
+          f2v, __builtin_amdgcn_raw_buffer_load_b64(grsrc, off, uint32_t(i) * slots * 8u, 0));
+  };
+  int r = r0 + wave;
+  if (r < r1) load_row(r);
+  for (; r < r1; r += kGramSolveWaves) {
+    const int step = r + 1;
+    const uint32_t slot = uint32_t(r) * kWave + uint32_t(lane);
     const bool first = codes[slot] != 2;  // 1: lane's antenna is A; 0 (no baseline): zeros
+    // The own gains are the same every row; reading them through an index the
+    // compiler cannot see through keeps them (and their FP64 norms) from being
+    // hoisted out of the loop into registers the next row's loads need.
+    int own_lane = lane;
+    asm volatile("" : "+v"(own_lane));
     const int pa = lane < n ? (lane + step < n ? lane + step : lane + step - n) : lane;
     const int src = lane < n ? (lane >= step ? lane - step : lane - step + n) : lane;
-    float2 go[ND], gp[ND], gg[ND], ggs[ND];
+    float2 ggs[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
-      go[d] = s_gain[d][lane];
-      gp[d] = s_gain[d][pa];
-      gg[d] = first ? cmulc(go[d], gp[d]) : cmulc(gp[d], go[d]);  // g_A conj(g_B)
-      ggs[d] = make_float2(gg[d].x * sc[d], gg[d].y * sc[d]);
+      const float2 go = s_gain[d][own_lane];
+      const float2 gp = s_gain[d][pa];
+      // g_A conj(g_B) with the FP16 scale of d
+      const float2 gg = first ? cmulc(go, gp) : cmulc(gp, go);
+      ggs[d] = make_float2(gg.x * sc[d], gg.y * sc[d]);
     }
     // U_d = C_d - sum_e ggs_e G_e,d (FP32, as the apply; G_e,d = conj(G_d,e)
-    // for e > d), four FMAs per complex term; U and S first, so that the
-    // loaded terms are dead before the FP64 part
-    float2 uu[ND];
-    float ss[ND];
+    // for e > d), four FMAs per complex term, and with it
+    // T_d / sc_d = U_d + ggs_d G_d,d
+    float2 tt[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
       float ux = q[d].x, uy = q[d].y;
@@ -1079,18 +1091,27 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
         uy = fmaf(-ggs[e].x, gi, uy);
         uy = fmaf(-ggs[e].y, ge.x, uy);
       }
-      uu[d] = make_float2(ux, uy);
-      ss[d] = q[GramPair<ND>(d, d)].x;
+      tt[d] = make_float2(ux, uy);
     }
+    float ss[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) ss[d] = q[GramPair<ND>(d, d)].x;
+    if (r + kGramSolveWaves < r1) load_row(r + kGramSolveWaves);
+    // In units of the FP16 scale: T = sc_d T' with T' = U_d + ggs_d G_d,d and
+    // S = sc_d^2 G_d,d, so the numerator sums carry sc_d and the denominator
+    // sums sc_d^2, applied once when the sums are stored.
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
-      const double sd = double(sc[d]) * double(sc[d]) * double(ss[d]);
-      double2 t = make_double2(double(sc[d]) * double(uu[d].x) + double(gg[d].x) * sd,
-                               double(sc[d]) * double(uu[d].y) + double(gg[d].y) * sd);
+      const double sd = double(ss[d]);
+      double2 t = make_double2(double(tt[d].x) + double(ggs[d].x) * sd,
+                               double(tt[d].y) + double(ggs[d].y) * sd);
       // T with the lane's orientation: T for antenna A's lane, conj(T) for B's
       if (!first) t.y = -t.y;
-      const double2 o = make_double2(go[d].x, go[d].y);
-      const double2 p = make_double2(gp[d].x, gp[d].y);
+      // the gains again from LDS (kept out of registers while the next row loads)
+      const float2 go = s_gain[d][own_lane];
+      const float2 gp = s_gain[d][pa];
+      const double2 o = make_double2(go.x, go.y);
+      const double2 p = make_double2(gp.x, gp.y);
       const double2 own = cmul(p, t);    // A: g_B T; B: g_A conj(T)
       const double2 prt = cmulc(o, t);   // partner B: g_A conj(T); partner A: g_B T
       acc[d][0] += own.x + __shfl(prt.x, src);
@@ -1107,14 +1128,16 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
   __syncthreads();
   const size_t part_stride = size_t(ND) * 3 * kWave;  // doubles per workgroup's sums
   double* mine = a.gs_part + (size_t(cb) * 2 * groups + part) * part_stride;
-  for (int d = wave; d < ND; d += kGramSolveWaves)
+  for (int d = wave; d < ND; d += kGramSolveWaves) {
+    const double scd = F16 ? double(AsConst(a.model_scale)[cb * ND + d]) : 1.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       double v = 0.0;
 #pragma unroll
       for (int w = 0; w < kGramSolveWaves; ++w) v += s_red[w][d][c][lane];
-      mine[(d * 3 + c) * kWave + lane] = v;
+      mine[(d * 3 + c) * kWave + lane] = v * (c == 2 ? scd * scd : scd);
     }
+  }
 }
 
 // The end of the fused iteration for one block, after every workgroup of the

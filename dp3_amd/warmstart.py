@@ -204,3 +204,32 @@ def training_set(seq: IntervalSequence, ws: WarmStart, first, last, converge_ite
         ts.append(ws.targets(prev, sol))
         prev = sol
     return np.concatenate(xs), np.concatenate(ts), prev
+
+
+def training_sets_batched(seqs, ws: WarmStart, first, last, converge_iterations=1000):
+    """training_set for several interval sequences of one shape at once: each
+    step builds interval i of every sequence into its scratch problem, takes
+    the features from the previous converged solutions, and solves all of
+    them to convergence (tolerance 1e-9) in one batch with the Gram
+    iteration. Returns per sequence (x, t, last converged solutions)."""
+    dps = [q.transient(first) for q in seqs]
+    longs = [sibling(q.solver, converge_iterations, 1e-9, False, iteration=H.ITERATION_GRAM)
+             for q in seqs]
+    batch = H.Batch(longs, dps)
+    prev = [q.unit() for q in seqs]
+    batch.solve(prev)  # the first interval of each sequence from unit gains
+    xs = [[] for _ in seqs]
+    ts = [[] for _ in seqs]
+    for i in range(first + 1, last):
+        dps = [q.transient(i) for q in seqs]  # same scratch problems, refilled
+        for k, q in enumerate(seqs):
+            xs[k].append(ws.features(q.solver, dps[k], prev[k]))
+        sol = [p.copy() for p in prev]
+        batch.solve(sol)
+        for k in range(len(seqs)):
+            ts[k].append(ws.targets(prev[k], sol[k]))
+        prev = sol
+    batch.close()
+    for s in longs:
+        s.close()
+    return [(np.concatenate(x), np.concatenate(t), p) for x, t, p in zip(xs, ts, prev)]

@@ -2,11 +2,12 @@
 block, baselines visited in a ring walk of the antennas) against the oracle
 and against the unfused apply + antenna solve kernels.
 
-The fused kernel sums each antenna's numerator and denominator in FP32 over
-its wave's walk steps and adds the eight waves' sums in FP64; the unfused
-antenna solve adds per-baseline FP32 partials in FP64. Both follow the
-reference's rules (IterativeDiagonalSolver.cc:130-222), so they agree with
-the oracle to the parity bars and with each other to FP32 summation order.
+The fused kernel forms each baseline's two SolveDirection terms in FP64 from
+its FP32 U and sums them per antenna in FP64 over the walk steps, its waves
+and its row groups; the unfused antenna solve adds per-baseline FP32 partials
+in FP64. Both follow the reference's rules (IterativeDiagonalSolver.cc:
+130-222), so they agree with the oracle to the parity bars and with each
+other to summation order.
 Baseline orientation and order are free in SolveData: reversed baselines
 (data and model conjugated) and reordered or missing ones must give the same
 solutions."""
@@ -25,7 +26,7 @@ pytestmark = pytest.mark.gpu
 
 GAIN_TOL = 1e-4
 CHI2_TOL = 1e-3
-# fused against unfused: the same iteration with a different FP32 summation order
+# fused against unfused: the same iteration with a different summation order
 ORDER_TOL = 1e-5
 
 
