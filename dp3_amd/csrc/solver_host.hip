@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -20,7 +21,38 @@ namespace ddh {
 
 namespace {
 thread_local std::string g_last_error;
+
+// Scratch of ddh_chi2_device, the entry point that allocates nothing: the
+// per-workgroup partial sums of the chi^2 kernels (one double per
+// 64-baseline tile and channel block), at most kNoAllocChi2Partials of them
+// (512 KiB). Each GPU holds its own copy of the array for the life of the
+// process. The array's name is a host-side stand-in, not a device address:
+// host code uses only the per-device address that hipGetSymbolAddress
+// returns (NoAllocScratch).
+constexpr int kNoAllocChi2Partials = 1 << 16;
+constexpr int kNoAllocMaxDevices = 64;
+double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+std::mutex g_scratch_mutex;
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocChi2Partials];
+
+// The address of the current device's copy of `scratch`, looked up once per
+// device (at solver creation, or the first ddh_chi2_device call on a device).
+double* NoAllocScratch() {
+  int dev = 0;
+  DDH_HIP_CHECK(hipGetDevice(&dev));
+  DDH_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices, -3, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    DDH_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    DDH_REQUIRE(ptr != nullptr, -3, "no device address for the chi^2 scratch");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  return g_scratch_ptr[dev];
 }
+
 void SetLastError(const std::string& message) { g_last_error = message; }
 const char* LastError() { return g_last_error.c_str(); }
 
@@ -993,6 +1025,7 @@ ddh_solver* ddh_solver_create(const ddh_config* config) {
     }
     s->cfg.n_sub = acc;
     PrepareKernelAttributes();
+    (void)NoAllocScratch();  // this device's scratch address, before any caller needs it
     DDH_HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     DDH_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&s->host_state), sizeof(DeviceState),
                                 hipHostMallocDefault));
@@ -2215,5 +2248,27 @@ int ddh_chi2(ddh_solver* s, ddh_problem* p, double* const* solutions, double* ch
   });
 }
 
+int ddh_chi2_device(ddh_solver* s, ddh_problem* p, const double* solutions, double* chi2,
+                    void* stream) {
+  return Guard([&] {
+    DDH_REQUIRE(s && p && solutions && chi2, -1, "NULL argument");
+    CheckMatch(*s, *p);
+    const ddh_config& c = s->cfg.c;
+    DDH_REQUIRE(c.n_solution_polarizations != 1, -4,
+                "ddh_chi2_device takes pairs or 2x2 matrices (device layout); the scalar "
+                "layout needs ddh_chi2");
+    const IterArgs a = MakeArgs(*s, *p);  // reads the problem's buffers, allocates nothing
+    const int n_partial = Chi2Partials(a);
+    DDH_REQUIRE(n_partial <= kNoAllocChi2Partials, -4,
+                "ddh_chi2_device: the problem needs " + std::to_string(n_partial) +
+                    " partial sums (64-baseline tiles x channel blocks); its fixed scratch holds " +
+                    std::to_string(kNoAllocChi2Partials));
+    double* partial = NoAllocScratch();
+    // every partial is written before it is read: nothing to clear
+    LaunchChi2(a, p->n_corr, p->f16, reinterpret_cast<const double2*>(solutions), partial,
+               n_partial, chi2, reinterpret_cast<hipStream_t>(stream),
+               c.n_correlations == 1 ? 0.5 : 1.0);
+  });
+}
 
 }  // extern "C"

@@ -165,10 +165,10 @@ void LaunchPrologue(const IterArgs& a, hipStream_t stream);
 // Full-Jones solver: the model products of the problem, once per solve.
 void LaunchFullJonesModelProducts(const IterArgs& a, bool f16, int waves, hipStream_t stream);
 // FP64 chi^2 of the residual with solutions `sol` (current buffer of `a`):
-// per-workgroup partial sums into `partial`, then their fixed-order sum into
-// `out`.
+// per-workgroup partial sums into `partial`, then their fixed-order sum,
+// times `factor`, into `out`.
 void LaunchChi2(const IterArgs& a, int n_corr, bool f16, const double2* sol, double* partial,
-                int n_partial, double* out, hipStream_t stream);
+                int n_partial, double* out, hipStream_t stream, double factor = 1.0);
 int Chi2Partials(const IterArgs& a);
 size_t CorrelateSharedBytes(int n_dir, int n_corr, int waves);
 size_t MaxSharedBytes();

@@ -1603,11 +1603,11 @@ __global__ void __launch_bounds__(256) Chi2Kernel(const IterArgs a, const double
   }
 }
 
-__global__ void SumPartialsKernel(const double* partial, int n, double* out) {
+__global__ void SumPartialsKernel(const double* partial, int n, double factor, double* out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     double t = 0.0;
     for (int i = 0; i < n; ++i) t += partial[i];
-    *out = t;
+    *out = t * factor;
   }
 }
 
@@ -2810,7 +2810,7 @@ void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, hipStream_t str
 int Chi2Partials(const IterArgs& a) { return (a.bl_stride / kWave) * a.n_cb; }
 
 void LaunchChi2(const IterArgs& a, int n_corr, bool f16, const double2* sol, double* partial,
-                int n_partial, double* out, hipStream_t stream) {
+                int n_partial, double* out, hipStream_t stream, double factor) {
   DDH_REQUIRE(n_partial >= Chi2Partials(a), -1, "chi2 partial buffer too small");
   const dim3 grid(a.bl_stride / kWave, a.n_cb);
   if (a.full_jones) {
@@ -2831,7 +2831,7 @@ void LaunchChi2(const IterArgs& a, int n_corr, bool f16, const double2* sol, dou
       hipLaunchKernelGGL((Chi2Kernel<4, false>), grid, dim3(256), 0, stream, a, sol, partial);
   }
   hipLaunchKernelGGL(SumPartialsKernel, dim3(1), dim3(64), 0, stream, partial,
-                     Chi2Partials(a), out);
+                     Chi2Partials(a), factor, out);
   DDH_HIP_CHECK(hipGetLastError());
 }
 

@@ -161,6 +161,7 @@ SIGNATURES = {
                             C.POINTER(C.c_double)], C.c_int),
     "ddh_one_step": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "ddh_chi2": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "ddh_chi2_device": ([C.c_void_p] * 5, C.c_int),
     "ddh_mlp_create": ([C.c_int32, C.c_int32, C.c_int32, C.c_uint32], C.c_void_p),
     "ddh_mlp_destroy": ([C.c_void_p], None),
     "ddh_mlp_num_params": ([C.c_void_p], C.c_int64),

@@ -284,6 +284,20 @@ int ddh_one_step(ddh_solver* solver, ddh_problem* problem, double* const* soluti
 int ddh_chi2(ddh_solver* solver, ddh_problem* problem, double* const* solutions,
              double* chi2);
 
+// ddh_chi2 without allocation, for callers that cannot allocate device
+// memory (code under graph capture, tight serving loops); no counterpart in
+// the reference. Enqueues on `stream` (a hipStream_t; NULL: the null stream)
+// and returns without waiting. solutions: device memory, [n_cb][n_ant][n_sub]
+// pairs (diagonal solver) or row-major 2x2 matrices (full-Jones solver) of
+// complex128, the layout ddh_solve takes per block, blocks consecutive;
+// chi2: one double of device memory. The per-workgroup partial sums go to a
+// fixed device array of 65536 doubles (512 KiB, held by each GPU until the
+// process ends), so the problem may have at most 65536 64-baseline tiles x
+// channel blocks; larger problems and the scalar solver are rejected. Calls
+// on one GPU share that array: calls that overlap in time on one GPU race.
+int ddh_chi2_device(ddh_solver* solver, ddh_problem* problem, const double* solutions,
+                    double* chi2, void* stream);
+
 // Batched solves: many solution intervals (each its own solver
 // configuration and device problem; all with the same correlation count and
 // model storage) advance together, one launch per kernel per iteration over
