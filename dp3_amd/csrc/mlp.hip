@@ -11,10 +11,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <random>
+#include <utility>
 #include <vector>
 
 #include "../../include/ddecal_hip.h"
@@ -89,6 +91,112 @@ __global__ void __launch_bounds__(256) GemmBf16Kernel(const uint16_t* __restrict
       if (relu) v = fmaxf(v, 0.0f);
       float* out = C + size_t(row) * ldc + col;
       *out = accumulate ? *out + v : v;
+    }
+  }
+}
+
+// W [K x N] (bf16, row-major, leading dimension N) in the B-operand fragment
+// order of v_mfma_f32_32x32x16_bf16: fragment (tn, ks) holds, for lane l, the
+// 8 values W[16 ks + 8 (l / 32) + j][32 tn + l % 32], j = 0..7 (zero outside
+// K x N), so a wave loads a fragment as one contiguous 1 KiB read.
+__global__ void PackFragmentsKernel(const uint16_t* __restrict__ w, int K, int N, int n_tn,
+                                    int n_ks, uint16_t* __restrict__ packed) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= int64_t(n_tn) * n_ks * 64 * 8) return;
+  const int j = int(i % 8);
+  const int lane = int((i / 8) % 64);
+  const int frag = int(i / (8 * 64));
+  const int tn = frag / n_ks, ks = frag % n_ks;
+  const int k = 16 * ks + 8 * (lane / 32) + j;
+  const int n = 32 * tn + lane % 32;
+  packed[i] = (k < K && n < N) ? w[size_t(k) * N + n] : uint16_t(0);
+}
+
+// The warm-start network's forward pass in one kernel, for kWarmFeatures (16)
+// inputs, kWarmOutputs (4) outputs and 32 * NT hidden units: one wave per 32
+// rows runs the three layers, its hidden activations (rounded to bf16, as the
+// next layer's operand) staged in its own LDS rows, the weights read in
+// fragment order. Every output is the same sequence of
+// v_mfma_f32_32x32x16_bf16 steps (k ascending) over the same bf16 operands,
+// with the same bias and ReLU, as the layer-by-layer GemmBf16Kernel path, so
+// the two give the same bits; only the inputs and outputs touch HBM.
+constexpr int kForwardRows = 32;
+
+template <int NT>
+__device__ __forceinline__ void StoreHidden(uint16_t* hs, int stride, const float16_v& acc,
+                                            int tn, const float* __restrict__ bias, int lane) {
+  const int col = 32 * tn + (lane & 31);
+  const float b = bias[col];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int row = 8 * (i / 4) + 4 * (lane >> 5) + (i % 4);
+    hs[row * stride + col] = ToBf16(fmaxf(acc[i] + b, 0.0f));
+  }
+}
+
+template <int NT>
+__global__ void __launch_bounds__(64) MlpForwardKernel(
+    const float* __restrict__ x, int64_t n, const bf16x8* __restrict__ w1p,
+    const float* __restrict__ b1, const bf16x8* __restrict__ w2p, const float* __restrict__ b2,
+    const bf16x8* __restrict__ w3p, const float* __restrict__ b3, float* __restrict__ y) {
+  constexpr int kHidden = 32 * NT;
+  constexpr int kSteps = kHidden / 16;  // k steps of layers 2 and 3
+  constexpr int kStride = kHidden + 8;  // bf16 per LDS row: 16 bytes of padding
+  __shared__ __attribute__((aligned(16))) uint16_t hs[kForwardRows * kStride];
+  const int lane = threadIdx.x;
+  const int64_t row0 = int64_t(blockIdx.x) * kForwardRows;
+  const int r = lane & 31;
+  const int kg = 8 * (lane >> 5);
+  // layer 1: the row's 16 features as the A operand (row r, k = kg..kg+7)
+  bf16x8 a;
+  {
+    const int64_t m = row0 + r;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint16_t v = m < n ? ToBf16(x[m * kWarmFeatures + kg + j]) : uint16_t(0);
+      a[j] = __builtin_bit_cast(__bf16, v);
+    }
+  }
+#pragma unroll
+  for (int tn = 0; tn < NT; ++tn) {
+    float16_v acc;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, w1p[tn * 64 + lane], acc, 0, 0, 0);
+    StoreHidden<NT>(hs, kStride, acc, tn, b1, lane);
+  }
+  __syncthreads();
+  // layer 2: every output tile advances over k together; H1 read once
+  float16_v acc2[NT];
+#pragma unroll
+  for (int tn = 0; tn < NT; ++tn)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc2[tn][i] = 0.0f;
+  for (int ks = 0; ks < kSteps; ++ks) {
+    const bf16x8 a2 = *reinterpret_cast<const bf16x8*>(hs + r * kStride + 16 * ks + kg);
+#pragma unroll
+    for (int tn = 0; tn < NT; ++tn)
+      acc2[tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, w2p[(tn * kSteps + ks) * 64 + lane],
+                                                         acc2[tn], 0, 0, 0);
+  }
+  __syncthreads();  // every read of H1 is done before H2 overwrites it
+#pragma unroll
+  for (int tn = 0; tn < NT; ++tn) StoreHidden<NT>(hs, kStride, acc2[tn], tn, b2, lane);
+  __syncthreads();
+  // layer 3: outputs in columns 0..3 of one tile
+  float16_v acc3;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc3[i] = 0.0f;
+  for (int ks = 0; ks < kSteps; ++ks) {
+    const bf16x8 a3 = *reinterpret_cast<const bf16x8*>(hs + r * kStride + 16 * ks + kg);
+    acc3 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a3, w3p[ks * 64 + lane], acc3, 0, 0, 0);
+  }
+  if (r < kWarmOutputs) {
+    const float b = b3[r];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int64_t m = row0 + 8 * (i / 4) + 4 * (lane >> 5) + (i % 4);
+      if (m < n) y[m * kWarmOutputs + r] = acc3[i] + b;
     }
   }
 }
@@ -195,54 +303,73 @@ __device__ inline void LogRatio(double2 a, double2 b, float& lmag, float& ang) {
   ang = float(atan2(im, re));
 }
 
-__global__ void WarmFeaturesKernel(const double2* __restrict__ g0,
-                                   const double2* __restrict__ next1,
-                                   const double2* __restrict__ ga,
-                                   const double2* __restrict__ gb,
-                                   const double* __restrict__ den, int n_cb, int n_ant,
-                                   int n_sub, float* __restrict__ feat) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t n_pairs = int64_t(n_cb) * n_ant * n_sub;
-  if (i >= n_pairs) return;
-  const int s = int(i % n_sub);
-  const int cb = int(i / (int64_t(n_ant) * n_sub));
-  float f[kWarmFeatures] = {};
-  for (int p = 0; p < 2; ++p) {
-    float* fp = f + 7 * p;
-    const double2 base = g0[i * 2 + p];
-    LogRatio(next1[i * 2 + p], base, fp[0], fp[1]);
-    LogRatio(ga[i * 2 + p], base, fp[2], fp[3]);
-    LogRatio(gb[i * 2 + p], base, fp[4], fp[5]);
-    double mean = 0.0;
-    int cnt = 0;
-    for (int a = 0; a < n_ant; ++a) {
-      const double dv = den[((int64_t(cb) * n_ant + a) * n_sub + s) * 2 + p];
-      if (dv > 0.0 && isfinite(dv)) {
-        mean += dv;
-        ++cnt;
+// One launch serves many intervals: blockIdx.y selects the interval's span
+// (its first solution pair in the concatenated buffers and its shape).
+__global__ void WarmFeaturesKernel(const double2* __restrict__ g0_all,
+                                   const double2* __restrict__ next1_all,
+                                   const double2* __restrict__ ga_all,
+                                   const double2* __restrict__ gb_all,
+                                   const double* __restrict__ den_all,
+                                   const WarmSpan* __restrict__ spans, float* __restrict__ feat_all) {
+  const WarmSpan sp = spans[blockIdx.y];
+  const int n_ant = sp.n_ant, n_sub = sp.n_sub;
+  const int64_t n_pairs = int64_t(sp.n_cb) * n_ant * n_sub;
+  const double2* g0 = g0_all + 2 * sp.pair0;
+  const double2* next1 = next1_all + 2 * sp.pair0;
+  const double2* ga = ga_all + 2 * sp.pair0;
+  const double2* gb = gb_all + 2 * sp.pair0;
+  const double* den = den_all + 2 * sp.pair0;
+  float* feat = feat_all + sp.pair0 * kWarmFeatures;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n_pairs;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int s = int(i % n_sub);
+    const int cb = int(i / (int64_t(n_ant) * n_sub));
+    float f[kWarmFeatures] = {};
+    for (int p = 0; p < 2; ++p) {
+      float* fp = f + 7 * p;
+      const double2 base = g0[i * 2 + p];
+      LogRatio(next1[i * 2 + p], base, fp[0], fp[1]);
+      LogRatio(ga[i * 2 + p], base, fp[2], fp[3]);
+      LogRatio(gb[i * 2 + p], base, fp[4], fp[5]);
+      double mean = 0.0;
+      int cnt = 0;
+      for (int a = 0; a < n_ant; ++a) {
+        const double dv = den[((int64_t(cb) * n_ant + a) * n_sub + s) * 2 + p];
+        if (dv > 0.0 && isfinite(dv)) {
+          mean += dv;
+          ++cnt;
+        }
       }
+      const double dv = den[i * 2 + p];
+      fp[6] = (cnt > 0 && dv > 0.0 && isfinite(dv)) ? float(log10(dv / (mean / cnt))) : -3.0f;
     }
-    const double dv = den[i * 2 + p];
-    fp[6] = (cnt > 0 && dv > 0.0 && isfinite(dv)) ? float(log10(dv / (mean / cnt))) : -3.0f;
+    for (int k = 0; k < kWarmFeatures; ++k)
+      feat[i * kWarmFeatures + k] = isfinite(f[k]) ? f[k] : 0.0f;
   }
-  for (int k = 0; k < kWarmFeatures; ++k) feat[i * kWarmFeatures + k] = isfinite(f[k]) ? f[k] : 0.0f;
 }
 
 // init = prev * exp(d_log_mag + i d_arg) per polarization; non-finite prev
-// falls back to 1.
-__global__ void WarmApplyKernel(const double2* __restrict__ prev, const float* __restrict__ pred,
-                                int64_t n_pairs, double2* __restrict__ init) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n_pairs) return;
-  for (int p = 0; p < 2; ++p) {
-    double2 g = prev[i * 2 + p];
-    if (!(isfinite(g.x) && isfinite(g.y))) g = make_double2(1.0, 0.0);
-    const double lm = pred[i * kWarmOutputs + 2 * p];
-    const double an = pred[i * kWarmOutputs + 2 * p + 1];
-    double sn, cs;
-    sincos(an, &sn, &cs);
-    const double sc = exp(lm);
-    init[i * 2 + p] = make_double2(sc * (g.x * cs - g.y * sn), sc * (g.x * sn + g.y * cs));
+// falls back to 1. Spans as for the features.
+__global__ void WarmApplyKernel(const double2* __restrict__ prev_all,
+                                const float* __restrict__ pred_all,
+                                const WarmSpan* __restrict__ spans, double2* __restrict__ init_all) {
+  const WarmSpan sp = spans[blockIdx.y];
+  const int64_t n_pairs = int64_t(sp.n_cb) * sp.n_ant * sp.n_sub;
+  const double2* prev = prev_all + 2 * sp.pair0;
+  const float* pred = pred_all + sp.pair0 * kWarmOutputs;
+  double2* init = init_all + 2 * sp.pair0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n_pairs;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    for (int p = 0; p < 2; ++p) {
+      double2 g = prev[i * 2 + p];
+      if (!(isfinite(g.x) && isfinite(g.y))) g = make_double2(1.0, 0.0);
+      const double lm = pred[i * kWarmOutputs + 2 * p];
+      const double an = pred[i * kWarmOutputs + 2 * p + 1];
+      double sn, cs;
+      sincos(an, &sn, &cs);
+      const double sc = exp(lm);
+      init[i * 2 + p] = make_double2(sc * (g.x * cs - g.y * sn), sc * (g.x * sn + g.y * cs));
+    }
   }
 }
 
@@ -296,6 +423,11 @@ struct ddh_mlp {
   ddh::Buf<uint16_t> x16, h1_16, h2_16, dy16, dh1_16, dh2_16;
   ddh::Buf<float> data_x, data_t;
   ddh::Buf<int32_t> perm;
+  // bf16 weights in MFMA fragment order for the one-kernel forward pass,
+  // repacked when the weights change (weights_version)
+  ddh::Buf<uint16_t> packed;
+  int64_t weights_version = 0, packed_version = -1;
+  bool fused_forward = true;
 
   int64_t n_params() const { return off[6]; }
 };
@@ -315,6 +447,68 @@ void Forward(ddh_mlp& net, int64_t n, hipStream_t s) {
   ToBf16(net.h2.ptr, net.h2_16.ptr, n * h, s);
   Gemm(net.h2_16.ptr, h, false, net.w16.ptr + net.off[4], net.n_out, false, net.y.ptr, net.n_out,
        int(n), net.n_out, h, net.w.ptr + net.off[5], false, false, s);
+}
+
+// The one-kernel forward pass serves 16 -> 32 * NT -> 32 * NT -> 4 networks,
+// NT = 1..8; other shapes run layer by layer.
+constexpr int kForwardMaxTiles = 8;
+
+bool UsesFusedForward(const ddh_mlp& net) {
+  return net.fused_forward && net.n_in == kWarmFeatures && net.n_out == kWarmOutputs &&
+         net.n_hidden % 32 == 0 && net.n_hidden / 32 <= kForwardMaxTiles;
+}
+
+// Offsets (in bf16 values) of the packed W1, W2 and W3 fragments.
+void PackedOffsets(const ddh_mlp& net, int64_t (&o)[4]) {
+  const int nt = net.n_hidden / 32, ks = net.n_hidden / 16;
+  o[0] = 0;
+  o[1] = o[0] + int64_t(nt) * 1 * 64 * 8;
+  o[2] = o[1] + int64_t(nt) * ks * 64 * 8;
+  o[3] = o[2] + int64_t(1) * ks * 64 * 8;
+}
+
+void PackWeights(ddh_mlp& net, hipStream_t s) {
+  if (net.packed_version == net.weights_version) return;
+  const int h = net.n_hidden, nt = h / 32, ks = h / 16;
+  int64_t o[4];
+  PackedOffsets(net, o);
+  uint16_t* p = net.packed.Get(o[3]);
+  hipLaunchKernelGGL(PackFragmentsKernel, dim3(Blocks(o[1] - o[0])), dim3(256), 0, s,
+                     net.w16.ptr + net.off[0], kWarmFeatures, h, nt, 1, p + o[0]);
+  hipLaunchKernelGGL(PackFragmentsKernel, dim3(Blocks(o[2] - o[1])), dim3(256), 0, s,
+                     net.w16.ptr + net.off[2], h, h, nt, ks, p + o[1]);
+  hipLaunchKernelGGL(PackFragmentsKernel, dim3(Blocks(o[3] - o[2])), dim3(256), 0, s,
+                     net.w16.ptr + net.off[4], h, kWarmOutputs, 1, ks, p + o[2]);
+  DDH_HIP_CHECK(hipGetLastError());
+  net.packed_version = net.weights_version;
+}
+
+typedef void (*ForwardKernelFn)(const float*, int64_t, const bf16x8*, const float*, const bf16x8*,
+                                const float*, const bf16x8*, const float*, float*);
+template <int... NT>
+constexpr std::array<ForwardKernelFn, sizeof...(NT)> MakeForwardTable(
+    std::integer_sequence<int, NT...>) {
+  return {&MlpForwardKernel<NT + 1>...};
+}
+constexpr auto kForwardKernels =
+    MakeForwardTable(std::make_integer_sequence<int, kForwardMaxTiles>{});
+
+// Network outputs of the n rows of net.x into net.y on stream s.
+void Infer(ddh_mlp& net, int64_t n, hipStream_t s) {
+  if (!UsesFusedForward(net)) {
+    Forward(net, n, s);
+    return;
+  }
+  PackWeights(net, s);
+  int64_t o[4];
+  PackedOffsets(net, o);
+  const bf16x8* p = reinterpret_cast<const bf16x8*>(net.packed.ptr);
+  const int64_t blocks = (n + kForwardRows - 1) / kForwardRows;
+  DDH_REQUIRE(blocks <= INT32_MAX, -1, "too many rows for one forward pass");
+  hipLaunchKernelGGL(kForwardKernels[net.n_hidden / 32 - 1], dim3(unsigned(blocks)), dim3(64), 0,
+                     s, net.x.ptr, n, p + o[0] / 8, net.w.ptr + net.off[1], p + o[1] / 8,
+                     net.w.ptr + net.off[3], p + o[2] / 8, net.w.ptr + net.off[5], net.y.ptr);
+  DDH_HIP_CHECK(hipGetLastError());
 }
 
 // Gradients of the mean squared error over n rows into net.g; loss into net.loss[0].
@@ -388,24 +582,28 @@ float* MlpInputBuffer(ddh_mlp& net, int64_t n) {
 }
 
 const float* MlpForwardDevice(ddh_mlp& net, int64_t n, hipStream_t stream) {
-  Forward(net, n, stream);
+  Infer(net, n, stream);
   DDH_HIP_CHECK(hipGetLastError());
   return net.y.ptr;
 }
 
+dim3 WarmGrid(int64_t max_pairs, int n_spans) {
+  return dim3(unsigned(std::min<int64_t>(std::max<int64_t>(Blocks(max_pairs), 1), 4096)),
+              unsigned(n_spans));
+}
+
 void LaunchWarmFeatures(const double2* g0, const double2* next1, const double2* ga,
-                        const double2* gb, const double* den, int n_cb, int n_ant, int n_sub,
-                        float* features, hipStream_t stream) {
-  const int64_t n_pairs = int64_t(n_cb) * n_ant * n_sub;
-  hipLaunchKernelGGL(WarmFeaturesKernel, dim3(Blocks(n_pairs)), dim3(256), 0, stream, g0, next1,
-                     ga, gb, den, n_cb, n_ant, n_sub, features);
+                        const double2* gb, const double* den, const WarmSpan* spans, int n_spans,
+                        int64_t max_pairs, float* features, hipStream_t stream) {
+  hipLaunchKernelGGL(WarmFeaturesKernel, WarmGrid(max_pairs, n_spans), dim3(256), 0, stream, g0,
+                     next1, ga, gb, den, spans, features);
   DDH_HIP_CHECK(hipGetLastError());
 }
 
-void LaunchWarmApply(const double2* g0, const float* pred, int64_t n_pairs, double2* init,
-                     hipStream_t stream) {
-  hipLaunchKernelGGL(WarmApplyKernel, dim3(Blocks(n_pairs)), dim3(256), 0, stream, g0, pred,
-                     n_pairs, init);
+void LaunchWarmApply(const double2* g0, const float* pred, const WarmSpan* spans, int n_spans,
+                     int64_t max_pairs, double2* init, hipStream_t stream) {
+  hipLaunchKernelGGL(WarmApplyKernel, WarmGrid(max_pairs, n_spans), dim3(256), 0, stream, g0, pred,
+                     spans, init);
   DDH_HIP_CHECK(hipGetLastError());
 }
 
@@ -477,7 +675,15 @@ int ddh_mlp_set_params(ddh_mlp* net, const float* params) {
     DDH_HIP_CHECK(hipMemcpy(net->w.ptr, params, net->n_params() * sizeof(float),
                             hipMemcpyHostToDevice));
     ToBf16(net->w.ptr, net->w16.ptr, net->n_params(), net->stream);
+    ++net->weights_version;
     DDH_HIP_CHECK(hipStreamSynchronize(net->stream));
+  });
+}
+
+int ddh_mlp_set_fused_forward(ddh_mlp* net, int32_t enable) {
+  return Guard([&] {
+    DDH_REQUIRE(net, -1, "NULL argument");
+    net->fused_forward = enable != 0;
   });
 }
 
@@ -487,7 +693,7 @@ int ddh_mlp_forward(ddh_mlp* net, const float* x, int64_t n, float* y) {
     Reserve(*net, n);
     DDH_HIP_CHECK(hipMemcpyAsync(net->x.ptr, x, n * net->n_in * sizeof(float),
                                  hipMemcpyHostToDevice, net->stream));
-    Forward(*net, n, net->stream);
+    Infer(*net, n, net->stream);
     DDH_HIP_CHECK(hipMemcpyAsync(y, net->y.ptr, n * net->n_out * sizeof(float),
                                  hipMemcpyDeviceToHost, net->stream));
     DDH_HIP_CHECK(hipStreamSynchronize(net->stream));
@@ -553,6 +759,7 @@ int ddh_mlp_train(ddh_mlp* net, const float* x, const float* t, int64_t n, int32
         hipLaunchKernelGGL(AdamKernel, dim3(Blocks(net->n_params())), dim3(256), 0, s, net->w.ptr,
                            net->g.ptr, net->m.ptr, net->v.ptr, net->w16.ptr, net->n_params(), lr,
                            beta1, beta2, eps, bias1, bias2);
+        ++net->weights_version;
         float sq = 0.0f;
         if (e == epochs - 1) {
           DDH_HIP_CHECK(hipMemcpyAsync(&sq, net->loss.ptr, sizeof(float), hipMemcpyDeviceToHost, s));
@@ -605,10 +812,11 @@ int ddh_warmstart_features(const double* g0, const double* next1, const double* 
     DDH_HIP_CHECK(hipMemcpy(da.Get(n_pairs * 2), ga, bytes, hipMemcpyHostToDevice));
     DDH_HIP_CHECK(hipMemcpy(db.Get(n_pairs * 2), gb, bytes, hipMemcpyHostToDevice));
     DDH_HIP_CHECK(hipMemcpy(dd.Get(n_pairs * 2), den, n_pairs * 2 * sizeof(double), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(WarmFeaturesKernel, dim3(Blocks(n_pairs)), dim3(256), 0, nullptr, d0.ptr,
-                       d1.ptr, da.ptr, db.ptr, dd.ptr, n_cb, n_ant, n_sub,
-                       df.Get(n_pairs * kWarmFeatures));
-    DDH_HIP_CHECK(hipGetLastError());
+    Buf<WarmSpan> span;
+    const WarmSpan one{0, n_cb, n_ant, n_sub, 0};
+    DDH_HIP_CHECK(hipMemcpy(span.Get(1), &one, sizeof(one), hipMemcpyHostToDevice));
+    LaunchWarmFeatures(d0.ptr, d1.ptr, da.ptr, db.ptr, dd.ptr, span.ptr, 1, n_pairs,
+                       df.Get(n_pairs * kWarmFeatures), nullptr);
     DDH_HIP_CHECK(hipMemcpy(features, df.ptr, n_pairs * kWarmFeatures * sizeof(float),
                             hipMemcpyDeviceToHost));
   });
@@ -621,9 +829,11 @@ int ddh_warmstart_apply(const double* prev, const float* pred, int64_t n_pairs, 
     Buf<float> dpred;
     DDH_HIP_CHECK(hipMemcpy(dp.Get(n_pairs * 2), prev, n_pairs * 2 * sizeof(double2), hipMemcpyHostToDevice));
     DDH_HIP_CHECK(hipMemcpy(dpred.Get(n_pairs * kWarmOutputs), pred, n_pairs * kWarmOutputs * sizeof(float), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(WarmApplyKernel, dim3(Blocks(n_pairs)), dim3(256), 0, nullptr, dp.ptr,
-                       dpred.ptr, n_pairs, di.Get(n_pairs * 2));
-    DDH_HIP_CHECK(hipGetLastError());
+    DDH_REQUIRE(n_pairs >= 1 && n_pairs <= INT32_MAX, -1, "bad pair count");
+    Buf<WarmSpan> span;
+    const WarmSpan one{0, 1, 1, int32_t(n_pairs), 0};
+    DDH_HIP_CHECK(hipMemcpy(span.Get(1), &one, sizeof(one), hipMemcpyHostToDevice));
+    LaunchWarmApply(dp.ptr, dpred.ptr, span.ptr, 1, n_pairs, di.Get(n_pairs * 2), nullptr);
     DDH_HIP_CHECK(hipMemcpy(init, di.ptr, n_pairs * 2 * sizeof(double2), hipMemcpyDeviceToHost));
   });
 }

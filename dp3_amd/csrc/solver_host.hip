@@ -1430,6 +1430,11 @@ struct ddh_batch {
   ddh::DeviceBuffer<ddh::IterArgs> d_args_step, d_args_aux;
   ddh::DeviceBuffer<double2> warm_g0, warm_next, warm_a, warm_b;
   ddh::DeviceBuffer<double> warm_den;
+  // every interval's place in the warm buffers (as last uploaded)
+  std::vector<int64_t> warm_off;
+  std::vector<ddh::WarmSpan> warm_spans;
+  ddh::DeviceBuffer<int64_t> d_warm_off;
+  ddh::DeviceBuffer<ddh::WarmSpan> d_warm_spans;
   hipEvent_t ev_warm0 = nullptr, ev_warm1 = nullptr;
   double last_warm_ms = 0.0;
 };
@@ -1601,8 +1606,8 @@ void LaunchBatchCrossProducts(ddh_batch& b) {
   if (b.gram)
     LaunchBatchGram(b.d_args.ptr, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), b.f16,
                     b.waves, b.stream);
-  if (b.fused)
-    LaunchBatchGramPermute(b.d_args.ptr, b.d_gs_items.ptr, int(b.gs_items.size()), b.max_dir,
+  if (b.fused)  // once per (interval, block)
+    LaunchBatchGramPermute(b.d_args.ptr, b.d_pro_items.ptr, int(b.pro_items.size()), b.max_dir,
                            b.stream);
 }
 
@@ -1702,6 +1707,29 @@ void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* cons
   b.warm_a.Resize(total);
   b.warm_b.Resize(total);
   b.warm_den.Resize(total);
+  // every interval's place in the concatenated buffers, for one launch each
+  std::vector<int64_t> off64(off.begin(), off.end());
+  std::vector<WarmSpan> spans(n);
+  int64_t max_count = 0;
+  for (int i = 0; i < n; ++i) {
+    const ddh_config& c = b.solvers[i]->cfg.c;
+    spans[i] = WarmSpan{int64_t(off[i] / 2), c.n_channel_blocks, c.n_antennas,
+                        b.solvers[i]->cfg.n_sub, 0};
+    max_count = std::max<int64_t>(max_count, int64_t(off[i + 1] - off[i]));
+  }
+  const bool same_spans =
+      off64 == b.warm_off &&
+      std::equal(spans.begin(), spans.end(), b.warm_spans.begin(), b.warm_spans.end(),
+                 [](const WarmSpan& x, const WarmSpan& y) {
+                   return x.pair0 == y.pair0 && x.n_cb == y.n_cb && x.n_ant == y.n_ant &&
+                          x.n_sub == y.n_sub;
+                 });
+  if (!same_spans) {
+    b.d_warm_off.Upload(off64, b.stream);
+    b.d_warm_spans.Upload(spans, b.stream);
+    b.warm_off = off64;
+    b.warm_spans = spans;
+  }
   // undamped single step with denominators; fixed-length damped stretches
   std::vector<IterArgs> step(b.args), aux(b.args);
   for (int i = 0; i < n; ++i) {
@@ -1721,44 +1749,36 @@ void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* cons
   DDH_HIP_CHECK(hipEventRecord(b.ev_warm0, b.stream));
   UploadSolutions(b, solutions, [&](int i) { return b.warm_g0.ptr + off[i]; });
   const int n_pro = int(b.pro_items.size());
-  auto copy_into_sol0 = [&](const double2* src) {
-    for (int i = 0; i < n; ++i)
-      DDH_HIP_CHECK(hipMemcpyAsync(b.args[i].sol0, src + off[i], SolutionCount(b, i) * sizeof(double2),
-                                   hipMemcpyDeviceToDevice, b.stream));
+  // the solver buffers of every interval <-> the concatenated buffers; the
+  // argument variants share the solution buffers and states
+  auto move = [&](BatchMove mode, double2* buf) {
+    LaunchBatchMove(b.d_args.ptr, b.d_warm_off.ptr, n, max_count, mode, buf, b.stream);
   };
   auto run_aux = [&](double2* out) {
     LaunchBatchPrologue(b.d_args_aux.ptr, b.d_pro_items.ptr, n_pro, b.stream);
     for (int k = 0; k < stretch; ++k)
       LaunchBatchStep(b, b.d_args_aux.ptr, int(b.items.size()), n);
-    for (int i = 0; i < n; ++i)
-      LaunchSnapshot(b.d_args_aux.ptr, i, int64_t(SolutionCount(b, i)), out + off[i], b.stream);
+    move(kMoveFromCurrent, out);
   };
   // next1: iteration 0 reads sol0 and writes sol1; no decision is taken
-  copy_into_sol0(b.warm_g0.ptr);
+  move(kMoveToSol0, b.warm_g0.ptr);
   LaunchBatchPrologue(b.d_args_step.ptr, b.d_pro_items.ptr, n_pro, b.stream);
   LaunchBatchCrossProducts(b);  // once for the whole warm start and solve
   LaunchBatchStep(b, b.d_args_step.ptr, int(b.items.size()), 0);
-  for (int i = 0; i < n; ++i)
-    DDH_HIP_CHECK(hipMemcpyAsync(b.warm_next.ptr + off[i], b.args[i].sol1,
-                                 SolutionCount(b, i) * sizeof(double2), hipMemcpyDeviceToDevice,
-                                 b.stream));
+  move(kMoveFromSol1, b.warm_next.ptr);
   run_aux(b.warm_a.ptr);  // sol0 still holds g0
-  copy_into_sol0(b.warm_a.ptr);
+  move(kMoveToSol0, b.warm_a.ptr);
   run_aux(b.warm_b.ptr);
   // features of all intervals as consecutive rows, one forward pass
   const int64_t rows = int64_t(total / 2);
   float* x = MlpInputBuffer(net, rows);
-  for (int i = 0; i < n; ++i) {
-    const ddh_config& c = b.solvers[i]->cfg.c;
-    LaunchWarmFeatures(b.warm_g0.ptr + off[i], b.warm_next.ptr + off[i], b.warm_a.ptr + off[i],
-                       b.warm_b.ptr + off[i], b.warm_den.ptr + off[i], c.n_channel_blocks,
-                       c.n_antennas, b.solvers[i]->cfg.n_sub, x + int64_t(off[i] / 2) * kWarmFeatures,
-                       b.stream);
-  }
+  LaunchWarmFeatures(b.warm_g0.ptr, b.warm_next.ptr, b.warm_a.ptr, b.warm_b.ptr, b.warm_den.ptr,
+                     b.d_warm_spans.ptr, n, max_count / 2, x, b.stream);
   const float* pred = MlpForwardDevice(net, rows, b.stream);
-  for (int i = 0; i < n; ++i)
-    LaunchWarmApply(b.warm_g0.ptr + off[i], pred + int64_t(off[i] / 2) * kWarmOutputs,
-                    int64_t(SolutionCount(b, i) / 2), b.args[i].sol0, b.stream);
+  // the starts, through warm_a (no longer needed), into every sol0
+  LaunchWarmApply(b.warm_g0.ptr, pred, b.d_warm_spans.ptr, n, max_count / 2, b.warm_a.ptr,
+                  b.stream);
+  move(kMoveToSol0, b.warm_a.ptr);
   DDH_HIP_CHECK(hipEventRecord(b.ev_warm1, b.stream));
   RunBatch(b, solutions, results, false);
   float ms = 0.0f;

@@ -229,10 +229,15 @@ void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const Ti
                       int n_groups, int apply_ant, const BatchTile* d_items, int n_items,
                       int n_intervals, size_t solve_lds, int solve_threads, bool f16,
                       const SmoothLaunch& smooth, hipStream_t stream);
-// Current solutions (by iteration parity) of batched interval `interval`
-// into dst, `count` complex values.
-void LaunchSnapshot(const IterArgs* d_args, int interval, int64_t count, double2* dst,
-                    hipStream_t stream);
+// Solution values of every batched interval between its solver buffers and
+// one concatenated buffer `buf` (interval i at d_off[i] .. d_off[i + 1],
+// complex values; max_count the largest), one launch:
+//   kMoveToSol0        buf -> sol0
+//   kMoveFromSol1      sol1 -> buf
+//   kMoveFromCurrent   the current solutions (by iteration parity) -> buf
+enum BatchMove { kMoveToSol0 = 0, kMoveFromSol1 = 1, kMoveFromCurrent = 2 };
+void LaunchBatchMove(const IterArgs* d_args, const int64_t* d_off, int n_intervals,
+                     int64_t max_count, BatchMove mode, double2* buf, hipStream_t stream);
 void LaunchBatchPrologue(const IterArgs* d_args, const BatchTile* d_items, int n_items,
                          hipStream_t stream);
 void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_items,

@@ -2248,13 +2248,22 @@ __global__ void __launch_bounds__(kBlockThreads)
 
 // Copies the current solutions of one batched interval (the buffer its
 // iteration count selects) to dst.
-__global__ void SnapshotKernel(const IterArgs* __restrict__ args, int interval, int64_t count,
-                               double2* __restrict__ dst) {
-  const IterArgs& a = args[interval];
-  const double2* src = (a.state->iteration & 1) ? a.sol1 : a.sol0;
+// grid (blocks, intervals)
+__global__ void BatchMoveKernel(const IterArgs* __restrict__ args, const int64_t* __restrict__ off,
+                                int mode, double2* __restrict__ buf) {
+  const IterArgs& a = args[blockIdx.y];
+  const int64_t o = off[blockIdx.y];
+  const int64_t count = off[blockIdx.y + 1] - o;
+  double2* sol = mode == kMoveToSol0     ? a.sol0
+                 : mode == kMoveFromSol1 ? a.sol1
+                                         : ((a.state->iteration & 1) ? a.sol1 : a.sol0);
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
-       i += int64_t(gridDim.x) * blockDim.x)
-    dst[i] = src[i];
+       i += int64_t(gridDim.x) * blockDim.x) {
+    if (mode == kMoveToSol0)
+      sol[i] = buf[o + i];
+    else
+      buf[o + i] = sol[i];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2674,11 +2683,12 @@ void LaunchBatchSmoothDecide(const IterArgs* d_args, const BatchTile* d_items, i
   DDH_HIP_CHECK(hipGetLastError());
 }
 
-void LaunchSnapshot(const IterArgs* d_args, int interval, int64_t count, double2* dst,
-                    hipStream_t stream) {
-  const int64_t blocks = std::min<int64_t>((count + 255) / 256, 1024);
-  hipLaunchKernelGGL(SnapshotKernel, dim3(unsigned(std::max<int64_t>(blocks, 1))), dim3(256), 0,
-                     stream, d_args, interval, count, dst);
+void LaunchBatchMove(const IterArgs* d_args, const int64_t* d_off, int n_intervals,
+                     int64_t max_count, BatchMove mode, double2* buf, hipStream_t stream) {
+  if (n_intervals == 0) return;
+  const int64_t blocks = std::min<int64_t>((max_count + 255) / 256, 1024);
+  hipLaunchKernelGGL(BatchMoveKernel, dim3(unsigned(std::max<int64_t>(blocks, 1)), n_intervals),
+                     dim3(256), 0, stream, d_args, d_off, int(mode), buf);
   DDH_HIP_CHECK(hipGetLastError());
 }
 

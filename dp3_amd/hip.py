@@ -168,6 +168,7 @@ SIGNATURES = {
     "ddh_mlp_get_params": ([C.c_void_p, C.c_void_p], C.c_int),
     "ddh_mlp_set_params": ([C.c_void_p, C.c_void_p], C.c_int),
     "ddh_mlp_forward": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
+    "ddh_mlp_set_fused_forward": ([C.c_void_p, C.c_int32], C.c_int),
     "ddh_mlp_loss_and_grads": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                 C.POINTER(C.c_float), C.c_void_p], C.c_int),
     "ddh_mlp_train": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
@@ -258,6 +259,10 @@ class MLP:
         p = np.ascontiguousarray(p, np.float32)
         assert p.size == self.num_params()
         _check(lib().ddh_mlp_set_params(self.handle, _ptr(p)), "ddh_mlp_set_params")
+
+    def set_fused_forward(self, enable=True):
+        _check(lib().ddh_mlp_set_fused_forward(self.handle, int(enable)),
+               "ddh_mlp_set_fused_forward")
 
     def forward(self, x):
         x = np.ascontiguousarray(x, np.float32)

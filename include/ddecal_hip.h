@@ -399,6 +399,12 @@ int64_t ddh_mlp_num_params(const ddh_mlp* net);
 int ddh_mlp_get_params(const ddh_mlp* net, float* params);
 int ddh_mlp_set_params(ddh_mlp* net, const float* params);
 int ddh_mlp_forward(ddh_mlp* net, const float* x, int64_t n, float* y);
+// Forward passes (ddh_mlp_forward and the batched warm start) of a
+// 16 -> 32k -> 32k -> 4 network (k = 1..8) run as one kernel by default
+// (enable = 1): the same MFMA steps over the same bf16 operands as the
+// layer-by-layer GEMMs (enable = 0), hence the same outputs, without the
+// hidden activations' HBM traffic. Other shapes always run layer by layer.
+int ddh_mlp_set_fused_forward(ddh_mlp* net, int32_t enable);
 // Mean squared error over n rows and its gradient w.r.t. the parameters.
 int ddh_mlp_loss_and_grads(ddh_mlp* net, const float* x, const float* t, int64_t n,
                            float* loss, float* grads);

@@ -52,6 +52,31 @@ def test_mlp_gradients_match_fp64_oracle():
         o += sz
 
 
+@pytest.mark.parametrize("h,n", [(32, 1), (96, 45), (256, 4099), (256, 32), (40, 70)])
+def test_fused_forward_equals_layered(h, n):
+    """The one-kernel forward pass gives the layer-by-layer GEMM path's bits
+    (same MFMA steps over the same bf16 operands); 40 hidden units run layer
+    by layer either way. Partial last row blocks (n not a multiple of 32)
+    included. After set_params the packed weights follow the new values."""
+    print(f"case: fused forward h={h} n={n}", flush=True)
+    net = H.MLP(16, h, 4, seed=7)
+    rng = np.random.default_rng(h + n)
+    x = (2.0 * rng.normal(size=(n, 16))).astype(np.float32)
+    for k in range(2):
+        params = (0.1 * rng.normal(size=net.num_params())).astype(np.float32)
+        net.set_params(params)
+        net.set_fused_forward(True)
+        fused = net.forward(x)
+        net.set_fused_forward(False)
+        layered = net.forward(x)
+        np.testing.assert_array_equal(fused, layered)
+        # FP64 oracle with the same bf16 rounding points: FP32 sums in another
+        # order can move a hidden activation across a bf16 rounding boundary
+        _, _, ref = MO.loss_and_grads(params, x, np.zeros((n, 4), np.float32), 16, h, 4)
+        assert np.abs(fused - ref).max() <= 1e-3 * np.abs(ref).max()
+    net.close()
+
+
 def test_mlp_training_fits_a_smooth_map():
     print("case: mlp training", flush=True)
     rng = np.random.default_rng(1)
