@@ -174,6 +174,9 @@ struct ddh_solver {
   int poll = 8;
   int fuse = 1;       // fused Gram iteration where the problem allows it
   int gs_groups = 2;  // its row groups per polarization (ddh_set_gram_solve_groups)
+  // antenna solve schedule (ddh_set_antenna_solve_schedule; 0: the default)
+  int solve_split = 0;
+  int solve_threads = 0;
   // per-solve device buffers
   ddh::DeviceBuffer<double2> sol0, sol1;
   ddh::DeviceBuffer<float2> gram;       // Gram iteration cross products
@@ -580,9 +583,8 @@ IterArgs MakeArgs(ddh_solver& s, ddh_problem& p, int fused = -1) {
   a.gram = s.gram.ptr;
   int max_nsd = 1;
   for (int v : s.cfg.n_sol_dir) max_nsd = std::max(max_nsd, v);
-  const int threads = AntennaSolveThreads(c.n_antennas, max_nsd);
-  a.split = 1;
-  while (a.split < 8 && c.n_antennas * max_nsd * a.split * 2 <= threads) a.split *= 2;
+  a.split = DefaultSolveSplit(c.n_antennas, max_nsd);
+  if (s.solve_split > 0 && c.n_solution_polarizations != 4) a.split = s.solve_split;
   a.max_nsd = max_nsd;
   a.scalar = c.n_solution_polarizations == 1 ? 1 : 0;
   a.full_jones = c.n_solution_polarizations == 4 ? 1 : 0;
@@ -610,6 +612,19 @@ void DestroyGraph(ddh_solver& s) {
 
 bool GramMode(const ddh_solver& s) { return s.cfg.c.iteration == DDH_ITERATION_GRAM; }
 
+void CheckSolveSchedule(int split, int threads) {
+  DDH_REQUIRE(split >= 0 && split <= kWave && (split & (split - 1)) == 0, -1,
+              "antenna solve split must be 0 (default) or a power of two up to 64");
+  DDH_REQUIRE(threads >= 0 && threads <= kSolveMaxThreads && threads % kWave == 0, -1,
+              "antenna solve block must be 0 (default) or a multiple of 64 up to 1024");
+}
+
+int ConfigMaxNsd(const ddh_solver& s) {
+  int m = 1;
+  for (int v : s.cfg.n_sol_dir) m = std::max(m, v);
+  return m;
+}
+
 // The correlate partials of one iteration, by the configured iteration kind.
 void LaunchCorrelation(const ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
   if (GramMode(s))
@@ -634,7 +649,7 @@ void LaunchCorrelateAndSolve(ddh_solver& s, const IterArgs& a, const ddh_problem
     return;
   }
   LaunchCorrelation(s, a, p);
-  LaunchAntennaSolve(a, p.n_corr, p.f16, s.stream);
+  LaunchAntennaSolve(a, p.n_corr, p.f16, s.solve_threads, s.stream);
 }
 
 void LaunchIteration(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
@@ -654,7 +669,9 @@ void EnsureGraph(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
                                         s.sm_weights.ptr,
                                         reinterpret_cast<const void*>(intptr_t(s.poll)),
                                         reinterpret_cast<const void*>(intptr_t(s.waves)),
-                                        reinterpret_cast<const void*>(intptr_t(s.unroll))};
+                                        reinterpret_cast<const void*>(intptr_t(s.unroll)),
+                                        reinterpret_cast<const void*>(intptr_t(a.split)),
+                                        reinterpret_cast<const void*>(intptr_t(s.solve_threads))};
   if (s.graph_exec && key == s.graph_key) return;
   DestroyGraph(s);
   hipGraph_t graph = nullptr;
@@ -829,7 +846,7 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
         } else {
           LaunchCorrelation(s, a, p);
           if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it + 1], s.stream));
-          LaunchAntennaSolve(a, p.n_corr, p.f16, s.stream);
+          LaunchAntennaSolve(a, p.n_corr, p.f16, s.solve_threads, s.stream);
         }
         LaunchSmoothing(a, s.stream);
         LaunchDecide(a, s.stream);
@@ -1362,6 +1379,34 @@ int ddh_set_gram_solve_groups(ddh_solver* s, int32_t groups) {
   });
 }
 
+int ddh_set_antenna_solve_schedule(ddh_solver* s, int32_t split, int32_t threads) {
+  return Guard([&] {
+    DDH_REQUIRE(s, -1, "NULL solver");
+    CheckSolveSchedule(split, threads);
+    if (split != s->solve_split || threads != s->solve_threads) {
+      s->solve_split = split;
+      s->solve_threads = threads;
+      DestroyGraph(*s);
+      ++s->version;  // batches built from this solver copy its split
+    }
+  });
+}
+
+int ddh_antenna_solve_schedule(const ddh_solver* s, int32_t* split, int32_t* threads) {
+  return Guard([&] {
+    DDH_REQUIRE(s && split && threads, -1, "NULL argument");
+    const ddh_config& c = s->cfg.c;
+    IterArgs a{};
+    a.n_ant = c.n_antennas;
+    a.max_nsd = ConfigMaxNsd(*s);
+    a.full_jones = c.n_solution_polarizations == 4 ? 1 : 0;
+    a.split = DefaultSolveSplit(a.n_ant, a.max_nsd);
+    if (s->solve_split > 0 && !a.full_jones) a.split = s->solve_split;
+    *split = a.split;
+    *threads = SolveThreads(a, s->solve_threads);
+  });
+}
+
 int ddh_set_profiling(ddh_solver* s, int32_t enable) {
   if (!s) return -1;
   s->profile = enable;
@@ -1407,6 +1452,8 @@ struct ddh_batch {
   int limit = 0;  // largest iteration limit over the intervals
   size_t solve_lds = 0;
   int solve_threads = 0;
+  // antenna solve schedule requested for every interval (0: the default)
+  int solve_split_req = 0, solve_threads_req = 0;
   std::vector<ddh::IterArgs> args;
   std::vector<uint64_t> versions;      // member solver versions the arguments were built from
   std::vector<ddh::TileGroup> groups;  // tiles of one direction count each
@@ -1487,10 +1534,11 @@ void BuildBatch(ddh_batch& b) {
     IterArgs a = MakeArgs(s, p, b.fused ? 1 : 0);
     a.gs_groups = b.gs_groups;
     a.state = b.d_states.ptr + i;
+    if (b.solve_split_req > 0 && !a.full_jones) a.split = b.solve_split_req;
     b.args.push_back(a);
     b.limit = std::max(b.limit, std::max(s.cfg.c.max_iterations, s.cfg.c.min_iterations));
     b.solve_lds = std::max(b.solve_lds, AntennaSolveSharedBytes(a, p.n_corr));
-    b.solve_threads = std::max(b.solve_threads, AntennaSolveThreads(a.n_ant, a.max_nsd));
+    b.solve_threads = std::max(b.solve_threads, SolveThreads(a, b.solve_threads_req));
     const double vis_bytes = double(p.DataVisBytes() + p.n_dir * p.ModelVisBytes());
     for (int cb = 0; cb < p.n_cb; ++cb) {
       for (int seg = 0; seg < p.n_seg; ++seg) {
@@ -1943,6 +1991,19 @@ std::vector<std::pair<int, int>> ScheduleCandidates(int n_corr) {
   return c;
 }
 
+// Antenna solve schedules (threads per (antenna, solution), block threads)
+// for `items` (antenna, solution) items: each split with blocks of one, two
+// and four times the threads it needs; (0, 0) is the default.
+std::vector<std::pair<int, int>> SolveScheduleCandidates(int items) {
+  std::vector<std::pair<int, int>> c{{0, 0}};
+  for (int split : {1, 2, 4, 8, 16}) {
+    const int need = std::max(kWave, ((items * split + kWave - 1) / kWave) * kWave);
+    if (need > kSolveMaxThreads) break;
+    for (int t = need; t <= std::min(kSolveMaxThreads, 4 * need); t *= 2) c.push_back({split, t});
+  }
+  return c;
+}
+
 }  // namespace
 }  // namespace ddh
 
@@ -2024,6 +2085,33 @@ int ddh_batch_set_gram_solve_groups(ddh_batch* b, int32_t groups) {
 }
 
 int ddh_batch_gram_solve_groups(const ddh_batch* b) { return b ? b->gs_groups : -1; }
+
+int ddh_batch_set_antenna_solve_schedule(ddh_batch* b, int32_t split, int32_t threads) {
+  return Guard([&] {
+    DDH_REQUIRE(b, -1, "NULL batch");
+    CheckSolveSchedule(split, threads);
+    const int old_split = b->solve_split_req, old_threads = b->solve_threads_req;
+    b->solve_split_req = split;
+    b->solve_threads_req = threads;
+    try {
+      BuildBatch(*b);
+    } catch (...) {
+      b->solve_split_req = old_split;
+      b->solve_threads_req = old_threads;
+      BuildBatch(*b);
+      throw;
+    }
+  });
+}
+
+int ddh_batch_antenna_solve_schedule(const ddh_batch* b, int32_t* split, int32_t* threads) {
+  return Guard([&] {
+    DDH_REQUIRE(b && split && threads, -1, "NULL argument");
+    DDH_REQUIRE(!b->args.empty(), -1, "empty batch");
+    *split = b->args[0].split;
+    *threads = b->solve_threads;
+  });
+}
 
 int ddh_batch_rebind(ddh_batch* b, ddh_problem* const* problems) {
   return Guard([&] {
@@ -2168,6 +2256,32 @@ int ddh_batch_autotune(ddh_batch* b, int32_t* waves_out, int32_t* rows_out, doub
       }
       b->gs_groups = bg;
       BuildBatch(*b);
+    } else if (!b->full_jones) {
+      // Antenna solve schedule, timed on the partials of one correlate (or
+      // apply) of the unit-gain state; the solve alone, without the decision,
+      // leaves the state unchanged.
+      if (b->gram) LaunchBatchCrossProducts(*b);
+      LaunchBatchStep(*b, b->d_args.ptr, 0, 0);
+      int items = 0;
+      for (const IterArgs& a : b->args) items = std::max(items, a.n_ant * std::max(1, a.max_nsd));
+      double best_s = 1e30;
+      std::pair<int, int> bs{b->solve_split_req, b->solve_threads_req};
+      for (const auto& cand : SolveScheduleCandidates(items)) {
+        b->solve_split_req = cand.first;
+        b->solve_threads_req = cand.second;
+        BuildBatch(*b);
+        const double ms = TimeLaunches(b->stream, 5, [&] {
+          LaunchBatchTail(b->d_args.ptr, b->d_items.ptr, int(b->items.size()), 0, b->solve_lds,
+                          b->solve_threads, b->n_corr, b->f16, SmoothLaunch{}, b->stream);
+        });
+        if (ms < best_s) {
+          best_s = ms;
+          bs = cand;
+        }
+      }
+      b->solve_split_req = bs.first;
+      b->solve_threads_req = bs.second;
+      BuildBatch(*b);
     }
     if (waves_out) *waves_out = bw;
     if (rows_out) *rows_out = bu;
@@ -2206,6 +2320,49 @@ int ddh_autotune(ddh_solver* s, ddh_problem* p, int32_t* waves_out, int32_t* row
     }
     s->waves = bw;
     s->unroll = bu;
+    DestroyGraph(*s);
+    // The fused iteration's row groups, or the antenna solve schedule, timed
+    // on the cross products (partials) of the unit-gain state: a launch
+    // rewrites only the next solutions and the per-workgroup sums of the same
+    // iteration, so repeats leave the state unchanged.
+    IterArgs t = MakeArgs(*s, *p);
+    if (t.walk_rows > 0) {
+      LaunchSolvePrologue(*s, t, *p);
+      double best_g = 1e30;
+      int bg = s->gs_groups;
+      for (int groups : {1, 2, 4, 8}) {
+        IterArgs u = t;
+        u.gs_groups = groups;
+        const double ms = TimeLaunches(s->stream, 5, [&] { LaunchGramSolve(u, p->f16, s->stream); });
+        if (ms < best_g) {
+          best_g = ms;
+          bg = groups;
+        }
+      }
+      s->gs_groups = bg;
+    } else if (!t.full_jones) {
+      LaunchSolvePrologue(*s, t, *p);
+      LaunchCorrelation(*s, t, *p);
+      double best_s = 1e30;
+      std::pair<int, int> bs{s->solve_split, s->solve_threads};
+      for (const auto& cand : SolveScheduleCandidates(t.n_ant * std::max(1, t.max_nsd))) {
+        IterArgs u = t;
+        if (cand.first > 0) u.split = cand.first;
+        else u.split = DefaultSolveSplit(t.n_ant, t.max_nsd);
+        const double ms = TimeLaunches(s->stream, 5, [&] {
+          LaunchAntennaSolve(u, p->n_corr, p->f16, cand.second, s->stream);
+        });
+        if (ms < best_s) {
+          best_s = ms;
+          bs = cand;
+        }
+      }
+      if (bs.first != s->solve_split || bs.second != s->solve_threads) {
+        s->solve_split = bs.first;
+        s->solve_threads = bs.second;
+        ++s->version;
+      }
+    }
     DestroyGraph(*s);
     if (waves_out) *waves_out = bw;
     if (rows_out) *rows_out = bu;

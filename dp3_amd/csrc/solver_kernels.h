@@ -159,7 +159,8 @@ struct BatchTile {
 // Host-side launchers (defined in solver_kernels.hip).
 void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, int rows,
                      hipStream_t stream);
-void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, hipStream_t stream);
+// threads: the antenna solve's block size (0: the default for a.split).
+void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, int threads, hipStream_t stream);
 void LaunchDecide(const IterArgs& a, hipStream_t stream);
 void LaunchPrologue(const IterArgs& a, hipStream_t stream);
 // Full-Jones solver: the model products of the problem, once per solve.
@@ -175,6 +176,14 @@ size_t MaxSharedBytes();
 size_t AntennaSolveSharedBytes(const IterArgs& a, int n_corr);
 // Block size of antenna_solve: about four threads per (antenna, solution).
 int AntennaSolveThreads(int n_ant, int max_nsd);
+// Antenna solve schedule: `split` threads per (antenna, solution) (a power of
+// two, combined by a shuffle tree) in blocks of SolveThreads(a, requested)
+// threads (requested 0: the default, at least four threads per item). The
+// default split is the largest power of two up to 8 that fits twice into the
+// default block. The full-Jones solve keeps its default block.
+constexpr int kSolveMaxThreads = 1024;
+int DefaultSolveSplit(int n_ant, int max_nsd);
+int SolveThreads(const IterArgs& a, int requested);
 void PrepareKernelAttributes();
 // Tiles of one direction count: entries [first, first + count) of a tile list.
 struct TileGroup {

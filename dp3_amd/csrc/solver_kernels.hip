@@ -2793,8 +2793,30 @@ int AntennaSolveThreads(int n_ant, int max_nsd) {
   return std::min(kBlockThreads, std::max(kWave, t));
 }
 
-void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, hipStream_t stream) {
-  const dim3 block(AntennaSolveThreads(a.n_ant, a.max_nsd));
+int DefaultSolveSplit(int n_ant, int max_nsd) {
+  const int threads = AntennaSolveThreads(n_ant, max_nsd);
+  const int items = n_ant * std::max(1, max_nsd);
+  int split = 1;
+  while (split < 8 && items * split * 2 <= threads) split *= 2;
+  return split;
+}
+
+int SolveThreads(const IterArgs& a, int requested) {
+  if (a.full_jones) return AntennaSolveThreads(a.n_ant, a.max_nsd);
+  const int items = a.n_ant * std::max(1, a.max_nsd);
+  const int need = std::max(kWave, ((items * a.split + kWave - 1) / kWave) * kWave);
+  DDH_REQUIRE(need <= kBlockThreads, -4,
+              "antenna solve: threads per (antenna, solution) x solutions exceed 1024");
+  if (requested <= 0) return std::max(need, AntennaSolveThreads(a.n_ant, a.max_nsd));
+  DDH_REQUIRE(requested % kWave == 0 && requested <= kBlockThreads && requested >= need, -1,
+              "antenna solve block: a multiple of 64 up to 1024 that covers every (antenna, "
+              "solution) x split");
+  return requested;
+}
+
+void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, int threads,
+                        hipStream_t stream) {
+  const dim3 block(SolveThreads(a, threads));
   if (a.full_jones) {
     hipLaunchKernelGGL(FullJonesSolveKernel, dim3(a.n_dir, a.n_cb), block, 0, stream, a);
     DDH_HIP_CHECK(hipGetLastError());

@@ -120,6 +120,12 @@ SIGNATURES = {
     "ddh_set_profiling": ([C.c_void_p, C.c_int32], C.c_int),
     "ddh_set_gram_fusion": ([C.c_void_p, C.c_int32], C.c_int),
     "ddh_set_gram_solve_groups": ([C.c_void_p, C.c_int32], C.c_int),
+    "ddh_set_antenna_solve_schedule": ([C.c_void_p, C.c_int32, C.c_int32], C.c_int),
+    "ddh_antenna_solve_schedule": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+                                   C.c_int),
+    "ddh_batch_set_antenna_solve_schedule": ([C.c_void_p, C.c_int32, C.c_int32], C.c_int),
+    "ddh_batch_antenna_solve_schedule": ([C.c_void_p, C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_int32)], C.c_int),
     "ddh_batch_set_gram_solve_groups": ([C.c_void_p, C.c_int32], C.c_int),
     "ddh_batch_gram_solve_groups": ([C.c_void_p], C.c_int),
     "ddh_predict": ([C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_int32, C.c_void_p,
@@ -410,6 +416,16 @@ class Solver:
         _check(lib().ddh_set_gram_solve_groups(self.handle, int(groups)),
                "ddh_set_gram_solve_groups")
 
+    def set_antenna_solve_schedule(self, split=0, threads=0):
+        _check(lib().ddh_set_antenna_solve_schedule(self.handle, int(split), int(threads)),
+               "ddh_set_antenna_solve_schedule")
+
+    def antenna_solve_schedule(self):
+        sp, th = C.c_int32(), C.c_int32()
+        _check(lib().ddh_antenna_solve_schedule(self.handle, C.byref(sp), C.byref(th)),
+               "ddh_antenna_solve_schedule")
+        return sp.value, th.value
+
     def last_timing(self):
         ms = C.c_double()
         n = C.c_int64()
@@ -530,6 +546,16 @@ class Batch:
 
     def gram_solve_groups(self):
         return int(lib().ddh_batch_gram_solve_groups(self.handle))
+
+    def set_antenna_solve_schedule(self, split=0, threads=0):
+        _check(lib().ddh_batch_set_antenna_solve_schedule(self.handle, int(split), int(threads)),
+               "ddh_batch_set_antenna_solve_schedule")
+
+    def antenna_solve_schedule(self):
+        sp, th = C.c_int32(), C.c_int32()
+        _check(lib().ddh_batch_antenna_solve_schedule(self.handle, C.byref(sp), C.byref(th)),
+               "ddh_batch_antenna_solve_schedule")
+        return sp.value, th.value
 
     def _pointers(self, solutions):
         n = len(self.solvers)

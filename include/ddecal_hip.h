@@ -264,9 +264,21 @@ int ddh_set_schedule(ddh_solver* solver, int32_t waves_per_tile,
 int ddh_set_gram_fusion(ddh_solver* solver, int32_t enable);
 // Schedule of the fused Gram iteration: each block's walk over the antenna
 // pairs is split, per polarization, into `groups` (1..8) workgroups whose
-// per-antenna sums the last one to finish adds up (default 2; set by the
+// per-antenna sums a finishing kernel adds up (default 2; set by the
 // autotuners).
 int ddh_set_gram_solve_groups(ddh_solver* solver, int32_t groups);
+// Schedule of the antenna solve (streaming and unfused Gram iterations of
+// the scalar and diagonal solvers; IterativeDiagonalSolver.cc:130-222 per
+// (block, direction)): `split` threads (a power of two up to 64) share each
+// (antenna, solution)'s baseline list, in workgroups of `threads` (a multiple
+// of 64 up to 1024, at least split x antennas x solutions of the largest
+// direction). 0 selects the default for either (about four threads per
+// (antenna, solution), split 2 for LOFAR-sized arrays); the autotuners pick
+// them. The split changes the FP64 summation order of each antenna's
+// numerator and denominator only.
+int ddh_set_antenna_solve_schedule(ddh_solver* solver, int32_t split, int32_t threads);
+// The split and block size the solver's antenna solve uses.
+int ddh_antenna_solve_schedule(const ddh_solver* solver, int32_t* split, int32_t* threads);
 // Records HIP events around every correlate launch of the next solves
 // (ddh_solver_last_timing then reports their summed duration).
 int ddh_set_profiling(ddh_solver* solver, int32_t enable);
@@ -316,6 +328,11 @@ int ddh_batch_set_schedule(ddh_batch* batch, int32_t waves_per_tile, int32_t row
 // and the current value.
 int ddh_batch_set_gram_solve_groups(ddh_batch* batch, int32_t groups);
 int ddh_batch_gram_solve_groups(const ddh_batch* batch);
+// Antenna solve schedule of every interval of the batch (as
+// ddh_set_antenna_solve_schedule; the block covers the largest interval) and
+// the values in use.
+int ddh_batch_set_antenna_solve_schedule(ddh_batch* batch, int32_t split, int32_t threads);
+int ddh_batch_antenna_solve_schedule(const ddh_batch* batch, int32_t* split, int32_t* threads);
 // Points the batch at other problems of the same shapes (the next intervals
 // of the same sequences), keeping buffers, schedule and captured graph.
 int ddh_batch_rebind(ddh_batch* batch, ddh_problem* const* problems);
@@ -351,8 +368,12 @@ int ddh_batch_last_timing(const ddh_batch* batch, double* correlate_ms, double* 
 // schedule (waves per tile x rows per trip; Gram iteration: the cross
 // products) on the actual shapes and keeps the fastest; for a fused Gram
 // iteration it then times the fused kernel under 1, 2, 4 and 8 row groups and
-// keeps the fastest (ddh_batch_gram_solve_groups reads it back). Leaves the
-// solve state reset (unit gains).
+// keeps the fastest (ddh_batch_gram_solve_groups reads it back); otherwise
+// (scalar and diagonal solvers) it times the antenna solve under every split
+// of 1..16 threads per (antenna, solution) with one, two and four times the
+// threads that split needs, and the default, and keeps the fastest
+// (ddh_antenna_solve_schedule / ddh_batch_antenna_solve_schedule read it
+// back). Leaves the solve state reset (unit gains).
 int ddh_autotune(ddh_solver* solver, ddh_problem* problem, int32_t* waves_per_tile,
                  int32_t* rows_per_trip, double* best_ms);
 int ddh_batch_autotune(ddh_batch* batch, int32_t* waves_per_tile, int32_t* rows_per_trip,
