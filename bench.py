@@ -220,7 +220,7 @@ def main():
         return {"value": B * T / t_all, "seconds": t_all, "warm_ms": warm_ms, "iterations": its,
                 "solutions": sols}
 
-    def kernel_times(iteration, tuned, fused=True):
+    def kernel_times(iteration, tuned, fused=True, solve_schedule=(0, 0)):
         """Mean of the first and second timed part of a batched iteration over
         a fixed-length profiled solve in which every interval is live in every
         launch: (fused kernel, finishing kernel + decision) for the fused Gram
@@ -231,6 +231,7 @@ def main():
             s.set_gram_fusion(fused)
         rb = H.Batch(solvers, [q.problems[n_total - 1] for q in s16])
         rb.set_schedule(tuned[0], tuned[1], 0, 8)
+        rb.set_antenna_solve_schedule(*solve_schedule)
         if fused and iteration == H.ITERATION_GRAM:
             rb.set_gram_solve_groups(tuned[2])
         rb.set_profiling(True)
@@ -250,13 +251,15 @@ def main():
         r = run_batched(batch, True)
         r["without_network"] = run_batched(batch, False)
         r["schedule"] = (tuned[0], tuned[1], groups)
-        r["kernels_ms"] = kernel_times(iteration, r["schedule"])
+        r["solve_schedule"] = batch.antenna_solve_schedule()
+        r["kernels_ms"] = kernel_times(iteration, r["schedule"], solve_schedule=r["solve_schedule"])
         opt[name] = r
         log(f"optimized ({name} iteration): {r['value']:.2f} solves/s, without network "
             f"{r['without_network']['value']:.2f}; per batched iteration: "
             f"{r['kernels_ms'][0] * 1e3:.1f} us + {r['kernels_ms'][1] * 1e3:.1f} us")
         batch.close()
-    unfused = kernel_times(H.ITERATION_GRAM, opt["gram"]["schedule"], fused=False)
+    unfused = kernel_times(H.ITERATION_GRAM, opt["gram"]["schedule"], fused=False,
+                           solve_schedule=opt["streaming"]["solve_schedule"])
     log(f"unfused Gram iteration: apply {unfused[0] * 1e3:.1f} us, antenna solve + decision "
         f"{unfused[1] * 1e3:.1f} us")
     best = opt["gram"]
@@ -265,6 +268,7 @@ def main():
     # ---- plain forms: complex64, one interval at a time
     plain = s64[0].solver
     p_tuned = H.autotune(plain, s64[0].problems[first])
+    p_solve = plain.antenna_solve_schedule()
     forms = {"plain": (MAX_ITERATIONS, "unit"), "plain_100": (w.max_iterations, "unit"),
              "plain_previous": (MAX_ITERATIONS, "previous")}
     results = {}
@@ -272,6 +276,7 @@ def main():
         s = WS.sibling(plain, limit, TOLERANCE)
         s.set_schedule(waves_per_tile=p_tuned[0], rows_per_trip=p_tuned[1], use_graph=GRAPH,
                        poll_interval=8)
+        s.set_antenna_solve_schedule(*p_solve)
         prev = [p.copy() for p in prevs[:PB]]
         t_all, its, sols = 0.0, [], {}
         for i in range(first, n_total):
@@ -370,6 +375,7 @@ def main():
     s = WS.sibling(plain, ROOF_ITERATIONS, 0.0, False)
     s.set_schedule(waves_per_tile=p_tuned[0], rows_per_trip=p_tuned[1], use_graph=0,
                    poll_interval=8)
+    s.set_antenna_solve_schedule(*p_solve)
     s.set_profiling(True)
     g = s64[0].unit()
     s.solve_problem(s64[0].problems[first], g)
@@ -415,6 +421,7 @@ def main():
     di = WS.sibling(plain, MAX_ITERATIONS, TOLERANCE)
     di.set_schedule(waves_per_tile=p_tuned[0], rows_per_trip=p_tuned[1], use_graph=GRAPH,
                     poll_interval=8)
+    di.set_antenna_solve_schedule(*p_solve)
     dres = H.Result()
     for rep in range(2):  # the second call is timed
         gd = s64[0].unit()
@@ -427,6 +434,7 @@ def main():
     dev_solve = WS.sibling(plain, MAX_ITERATIONS, TOLERANCE)
     dev_solve.set_schedule(waves_per_tile=p_tuned[0], rows_per_trip=p_tuned[1],
                            use_graph=GRAPH, poll_interval=8)
+    dev_solve.set_antenna_solve_schedule(*p_solve)
     gr = s64[0].unit()
     t0 = time.perf_counter()
     dev_solve.solve_problem(s64[0].problems[timed[0]], gr)
@@ -635,8 +643,14 @@ def main():
                                "rows_per_trip": opt["gram"]["schedule"][1],
                                "gram_solve_row_groups": opt["gram"]["schedule"][2]},
             "batch_schedule_streaming": {"waves_per_tile": opt["streaming"]["schedule"][0],
-                                         "rows_per_trip": opt["streaming"]["schedule"][1]},
-            "plain_schedule": {"waves_per_tile": p_tuned[0], "rows_per_trip": p_tuned[1]},
+                                         "rows_per_trip": opt["streaming"]["schedule"][1],
+                                         "antenna_solve_split":
+                                             opt["streaming"]["solve_schedule"][0],
+                                         "antenna_solve_threads":
+                                             opt["streaming"]["solve_schedule"][1]},
+            "plain_schedule": {"waves_per_tile": p_tuned[0], "rows_per_trip": p_tuned[1],
+                               "antenna_solve_split": p_solve[0],
+                               "antenna_solve_threads": p_solve[1]},
             "setup_s": round(t_setup, 1),
         },
         "roofline": rooflines[dominant],

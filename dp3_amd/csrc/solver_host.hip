@@ -192,7 +192,8 @@ struct ddh_solver {
   // captured iteration body (poll iterations), valid for graph_key
   hipGraphExec_t graph_exec = nullptr;
   std::vector<const void*> graph_key;
-  ddh::DeviceState* host_state = nullptr;  // pinned
+  ddh::DeviceState* host_state = nullptr;  // pinned, two slots (pipelined polling)
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};
   ddh::PinnedBuffer stage;                 // pinned staging of solutions
   // timing of the last solve
   std::vector<hipEvent_t> events;
@@ -828,8 +829,14 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
   if (graph) EnsureGraph(s, a, p);
   DDH_HIP_CHECK(hipEventRecord(t0, s.stream));
   LaunchSolvePrologue(s, a, p);
+  for (hipEvent_t& e : s.poll_ev)
+    if (!e) DDH_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  // Pipelined polling: the next chunk of `poll` iterations is queued before
+  // the host waits for the state after the current one, so the device does
+  // not idle while the host reads it; a chunk queued after the solve stopped
+  // returns at once (every kernel reads the done flag first).
   int launched = 0;
-  while (launched < limit) {
+  auto enqueue = [&](int slot) {
     if (graph) {
       DDH_HIP_CHECK(hipGraphLaunch(s.graph_exec, s.stream));
       launched += s.poll;
@@ -853,14 +860,22 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
       }
       launched += k;
     }
-    DDH_HIP_CHECK(hipMemcpyAsync(s.host_state, s.state.ptr, sizeof(DeviceState),
+    DDH_HIP_CHECK(hipMemcpyAsync(s.host_state + slot, s.state.ptr, sizeof(DeviceState),
                                  hipMemcpyDeviceToHost, s.stream));
-    DDH_HIP_CHECK(hipStreamSynchronize(s.stream));
-    if (s.host_state->done) break;
+    DDH_HIP_CHECK(hipEventRecord(s.poll_ev[slot], s.stream));
+  };
+  int slot = 0;
+  enqueue(slot);
+  for (;;) {
+    const bool more = launched < limit;
+    if (more) enqueue(slot ^ 1);
+    DDH_HIP_CHECK(hipEventSynchronize(s.poll_ev[slot]));
+    if (s.host_state[slot].done || !more) break;
+    slot ^= 1;
   }
   DDH_HIP_CHECK(hipEventRecord(t1, s.stream));
   DDH_HIP_CHECK(hipStreamSynchronize(s.stream));
-  const DeviceState st = *s.host_state;
+  const DeviceState st = s.host_state[slot];
   DDH_REQUIRE(st.done, -3, "solver did not reach its stopping criterion");
   const double2* final_sol = (st.iteration & 1) ? s.sol1.ptr : s.sol0.ptr;
   if (c.n_solution_polarizations == 1) {
@@ -1044,7 +1059,7 @@ ddh_solver* ddh_solver_create(const ddh_config* config) {
     PrepareKernelAttributes();
     (void)NoAllocScratch();  // this device's scratch address, before any caller needs it
     DDH_HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-    DDH_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&s->host_state), sizeof(DeviceState),
+    DDH_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&s->host_state), 2 * sizeof(DeviceState),
                                 hipHostMallocDefault));
     s->n_sol_dir.Upload(s->cfg.n_sol_dir, s->stream);
     s->sol_start.Upload(s->cfg.sol_start, s->stream);
@@ -1066,6 +1081,8 @@ void ddh_solver_destroy(ddh_solver* s) {
   DestroyGraph(*s);
   for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
   if (s->host_state) (void)hipHostFree(s->host_state);
+  for (hipEvent_t e : s->poll_ev)
+    if (e) (void)hipEventDestroy(e);
   s->sol0.Free();
   s->sol1.Free();
   s->mean.Free();
@@ -1463,7 +1480,8 @@ struct ddh_batch {
   ddh::DeviceBuffer<ddh::IterArgs> d_args;
   ddh::DeviceBuffer<ddh::BatchTile> d_tiles, d_items, d_pro_items, d_prod_tiles;
   ddh::DeviceBuffer<ddh::DeviceState> d_states;
-  ddh::DeviceState* host_states = nullptr;
+  ddh::DeviceState* host_states = nullptr;  // pinned, two slots of n (pipelined polling)
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};
   ddh::PinnedBuffer stage;  // pinned staging of every interval's solutions
   hipGraphExec_t graph_exec = nullptr;
   int graph_poll = 0, graph_waves = 0, graph_unroll = 0;
@@ -1850,9 +1868,14 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
   DDH_HIP_CHECK(hipEventRecord(t0, b.stream));
   LaunchBatchPrologue(b.d_args.ptr, b.d_pro_items.ptr, int(b.pro_items.size()), b.stream);
   if (cross_products) LaunchBatchCrossProducts(b);
+  for (hipEvent_t& e : b.poll_ev)
+    if (!e) DDH_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  // Pipelined polling: the next chunk of `poll` iterations is queued before
+  // the host waits for the states after the current one, so the device does
+  // not idle while the host reads them; a chunk queued after every interval
+  // stopped returns at once (every kernel reads the done flags first).
   int launched = 0;
-  bool all_done = false;
-  while (launched < b.limit && !all_done) {
+  auto enqueue = [&](int slot) {
     if (graph) {
       DDH_HIP_CHECK(hipGraphLaunch(b.graph_exec, b.stream));
       launched += b.poll;
@@ -1892,15 +1915,26 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
       }
       launched += k;
     }
-    DDH_HIP_CHECK(hipMemcpyAsync(b.host_states, b.d_states.ptr, n * sizeof(DeviceState),
+    DDH_HIP_CHECK(hipMemcpyAsync(b.host_states + slot * n, b.d_states.ptr, n * sizeof(DeviceState),
                                  hipMemcpyDeviceToHost, b.stream));
-    DDH_HIP_CHECK(hipStreamSynchronize(b.stream));
+    DDH_HIP_CHECK(hipEventRecord(b.poll_ev[slot], b.stream));
+  };
+  int slot = 0;
+  bool all_done = false;
+  enqueue(slot);
+  for (;;) {
+    const bool more = launched < b.limit;
+    if (more) enqueue(slot ^ 1);
+    DDH_HIP_CHECK(hipEventSynchronize(b.poll_ev[slot]));
     all_done = true;
-    for (int i = 0; i < n; ++i) all_done = all_done && b.host_states[i].done;
+    for (int i = 0; i < n; ++i) all_done = all_done && b.host_states[slot * n + i].done;
+    if (all_done || !more) break;
+    slot ^= 1;
   }
   DDH_HIP_CHECK(hipEventRecord(t1, b.stream));
   DDH_HIP_CHECK(hipStreamSynchronize(b.stream));
   DDH_REQUIRE(all_done, -3, "a batched interval did not reach its stopping criterion");
+  const DeviceState* final_states = b.host_states + slot * n;
   // final solutions: one DMA per interval into the pinned stage, then
   // scattered to the caller's blocks
   size_t total = 0;
@@ -1909,7 +1943,7 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
   size_t off = 0;
   for (int i = 0; i < n; ++i) {
     const ddh_solver& s = *b.solvers[i];
-    const DeviceState st = b.host_states[i];
+    const DeviceState st = final_states[i];
     const size_t n_sols = size_t(s.cfg.c.n_antennas) * s.cfg.n_sub;
     const size_t per_cb = n_sols * DevicePols(s.cfg.c);
     const double2* fin = (st.iteration & 1) ? b.args[i].sol1 : b.args[i].sol0;
@@ -1926,7 +1960,7 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
   off = 0;
   for (int i = 0; i < n; ++i) {
     const ddh_solver& s = *b.solvers[i];
-    const DeviceState st = b.host_states[i];
+    const DeviceState st = final_states[i];
     const size_t per_cb = size_t(s.cfg.c.n_antennas) * s.cfg.n_sub * DevicePols(s.cfg.c);
     if (s.cfg.c.n_solution_polarizations != 1)
       for (int cb = 0; cb < s.cfg.c.n_channel_blocks; ++cb)
@@ -1950,7 +1984,7 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
   if (b.profile) {
     // launches after the last interval stopped do no work: count those that did
     int worked = 0;
-    for (int i = 0; i < n; ++i) worked = std::max(worked, b.host_states[i].iteration);
+    for (int i = 0; i < n; ++i) worked = std::max(worked, final_states[i].iteration);
     for (int it = 0; it < worked; ++it) {
       float e = 0.0f, f = 0.0f;
       DDH_HIP_CHECK(hipEventElapsedTime(&e, b.events[3 * it], b.events[3 * it + 1]));
@@ -2034,7 +2068,7 @@ ddh_batch* ddh_batch_create(ddh_solver* const* solvers, ddh_problem* const* prob
     }
     DDH_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     DDH_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b->host_states),
-                                sizeof(DeviceState) * n_intervals, hipHostMallocDefault));
+                                2 * sizeof(DeviceState) * n_intervals, hipHostMallocDefault));
     // solvers' buffers are used from the batch stream
     for (ddh_solver* s : b->solvers) DDH_HIP_CHECK(hipStreamSynchronize(s->stream));
     BuildBatch(*b);
@@ -2051,6 +2085,8 @@ void ddh_batch_destroy(ddh_batch* b) {
   if (b->ev_warm0) (void)hipEventDestroy(b->ev_warm0);
   if (b->ev_warm1) (void)hipEventDestroy(b->ev_warm1);
   if (b->host_states) (void)hipHostFree(b->host_states);
+  for (hipEvent_t e : b->poll_ev)
+    if (e) (void)hipEventDestroy(e);
   b->d_args.Free();
   b->d_tiles.Free();
   b->d_items.Free();
