@@ -186,7 +186,7 @@ def main():
 
     # ---- learned warm start: trained on the solver's own converged outputs
     ws = WS.WarmStart(hidden=256, seed=args.seed, stretch=args.stretch)
-    sets = WS.training_sets_batched(s16, ws, 0, TR, converge_iterations=1000)
+    sets = WS.training_sets_batched(s16, ws, 0, TR, converge_iterations=1000, use_graph=GRAPH)
     x = np.concatenate([s[0] for s in sets])
     t = np.concatenate([s[1] for s in sets])
     prevs = [s[2] for s in sets]
@@ -313,6 +313,7 @@ def main():
         dps = [s64[b].problems[i] if b < PB else s64[b].transient(i) for b in range(B)]
         if lb is None:
             lb = H.Batch(long, dps)
+            lb.set_schedule(use_graph=GRAPH)
         else:
             lb.rebind(dps)
         g = [c.copy() for c in chain]
@@ -334,6 +335,11 @@ def main():
     bl_stride = -(-w.n_baselines // 64) * 64
     nq = w.n_directions + w.n_directions * (w.n_directions + 1) // 2
     cross_bytes = w.n_channel_blocks * 2 * nq * w.n_baselines * 8  # per interval
+    # the fused kernel's walk-order terms: C_d, G_e,d (e < d) and the real
+    # diagonals G_d,d two per 8 bytes (40 slots of 8 bytes at 8 directions)
+    nq_walk = (w.n_directions + w.n_directions * (w.n_directions - 1) // 2 +
+               (w.n_directions + 1) // 2)
+    walk_bytes = w.n_channel_blocks * 2 * nq_walk * w.n_baselines * 8  # per interval
     solution_bytes = w.n_channel_blocks * w.n_stations * w.n_sub * 2 * 16
     partial_bytes = w.n_channel_blocks * w.n_directions * 2 * 3 * bl_stride * 4
     apply_partial_bytes = w.n_channel_blocks * w.n_directions * 2 * 2 * bl_stride * 4
@@ -351,9 +357,10 @@ def main():
     s_corr, s_solve = opt["streaming"]["kernels_ms"]
     rooflines = {
         "gram_solve": roof(
-            "gram_solve", B * cross_bytes, g_solve,
+            "gram_solve", B * walk_bytes, g_solve,
             note=("fused Gram iteration kernel over the batch: the per-baseline cross products "
-                  "C_d and G_e,d (44 complex64 per baseline and correlation) read once; per "
+                  "in walk order read once (C_d, G_e,d for e < d, and the real diagonals G_d,d "
+                  f"two per slot: {nq_walk} x 8 bytes per baseline and correlation); per "
                   f"antenna FP64 sums per row group written ({opt['gram']['schedule'][2]} row "
                   "groups per polarization)")),
         "gram_finish_decide": roof(

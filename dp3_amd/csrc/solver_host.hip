@@ -723,7 +723,7 @@ void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
   if (c.iteration == DDH_ITERATION_GRAM)
     s.gram.Resize(size_t(p.n_cb) * p.n_seg * 2 * GramTerms(p.n_dir) * p.bl_stride);
   if (FusedGram(s, p)) {
-    s.gram_walk.Resize(size_t(p.n_cb) * 2 * GramTerms(p.n_dir) * p.walk_rows * kWave);
+    s.gram_walk.Resize(size_t(p.n_cb) * 2 * GramWalkTerms(p.n_dir) * p.walk_rows * p.n_ant);
     s.gs_part.Resize(size_t(p.n_cb) * 2 * kGramSolveMaxGroups * p.n_dir * 3 * kWave);
   }
   if (c.n_solution_polarizations == 4)

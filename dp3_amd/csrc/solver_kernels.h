@@ -116,10 +116,11 @@ struct IterArgs {
   float2* gram;
   // Fused Gram iteration (GramSolve: apply and antenna solve in one kernel;
   // Duo data, at most 64 antennas, one solution per direction in one
-  // segment): the cross products in walk order [cb][k][q][walk_rows * 64],
-  // and per walk slot (step row * 64 + lane) the lane's role for its baseline
-  // (0: none, 1: the lane's antenna is the first, 2: the second) and the
-  // baseline (-1: none). walk_rows = 0: no fused iteration.
+  // segment): the cross products in walk order
+  // [cb][k][walk_rows][GramWalkTerms(n_dir)][n_ant] (a row's terms of every
+  // lane contiguous), and per walk slot (step row * 64 + lane) the lane's role
+  // for its baseline (0: none, 1: the lane's antenna is the first, 2: the
+  // second) and the baseline (-1: none). walk_rows = 0: no fused iteration.
   float2* gram_walk;
   const int32_t* walk_code;
   const int32_t* walk_bl;
@@ -145,6 +146,11 @@ constexpr int kFullJonesGroupSize = 8;
 // grow with n_dir^2).
 constexpr int kMaxGramDirections = 8;
 __host__ __device__ inline int GramTerms(int n_dir) { return n_dir + n_dir * (n_dir + 1) / 2; }
+// Terms per walk slot of the fused iteration: C_d (n_dir), the off-diagonal
+// G_e,d (e < d), and the real diagonals G_d,d two per slot.
+__host__ __device__ inline int GramWalkTerms(int n_dir) {
+  return n_dir + n_dir * (n_dir - 1) / 2 + (n_dir + 1) / 2;
+}
 
 // One unit of batched work: correlate tiles use all four fields; antenna
 // solve items use (interval, cb) with the direction in `seg`; prologue items
@@ -214,8 +220,8 @@ void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, const TileGro
 void LaunchGram(const IterArgs& a, bool f16, int waves, hipStream_t stream);
 void LaunchApplyGram(const IterArgs& a, bool f16, hipStream_t stream);
 // Fused Gram iteration: the cross products into walk order once per solve,
-// then per iteration one workgroup per block for apply + antenna solve (the
-// decision follows as before).
+// then per iteration workgroups per block for apply + antenna solve and a
+// finishing kernel (the decision follows as before).
 void LaunchGramPermute(const IterArgs& a, hipStream_t stream);
 void LaunchGramSolve(const IterArgs& a, bool f16, hipStream_t stream);
 // Batched: items (interval, cb); groups split them by direction count.

@@ -962,9 +962,16 @@ __global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterAr
 // walk step s (1 <= s <= n/2) lane a takes the baseline {a, (a + s) mod n}
 // (for even n at s = n/2 only lanes a < n/2, which covers each such pair
 // once), so the steps visit every baseline exactly once. The cross products
-// are stored in this walk order, [cb][k][q][step row * 64 + lane], zero where
-// no baseline is, so each wave's loads are contiguous. For its baseline
-// (A, B) and every direction d the lane forms
+// are stored in this walk order, row by row: [cb][k][step row][term][n]
+// (zero where a pair has no baseline), so a wave's row is one contiguous
+// run of NQ * n * 8 bytes and a workgroup's rows follow each other. Lanes
+// without a pair (a >= n, and a >= n/2 in the last row for even n) load
+// nothing: their offset is past the buffer resource's range, which returns
+// zeros without a memory access.
+// The terms are C_d (d < ND), the off-diagonal G_e,d (e < d) and the real
+// diagonals G_d,d two per 8-byte slot (they are sums of |M_d|^2).
+//
+// For its baseline (A, B) and every direction d the lane forms
 //   U_d = C_d - sum_e gg_e G_e,d,  S_d = G_d,d,  T_d = U_d + gg_d S_d
 // (gg_e = g_A conj(g_B), the FP16 scale folded in) and the two terms of the
 // reference's SolveDirection (IterativeDiagonalSolver.cc:172-197):
@@ -977,25 +984,45 @@ __global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterAr
 // Work split: a block's walk rows of one polarization are cut into `groups`
 // row groups; one workgroup of four waves per (block, polarization, group),
 // its waves taking every fourth row. The workgroup adds its waves' sums in
-// LDS in a fixed order and stores them (per antenna and direction, FP64) in
+// LDS in wave order and stores them (per antenna and direction, FP64) in
 // the scratch `gs_part`; a second kernel (GramFinishBlock, one workgroup per
 // block) adds the block's 2 x groups sums in a fixed order and finishes every
 // solution. (Finishing in the last workgroup to arrive needs device-scope
 // fences, which write back and invalidate an XCD's L2 per workgroup; the
 // kernel boundary is cheaper.) Results differ from apply + antenna solve by
-// the FP64 summation order only, and do not depend on the order in which
+// the summation order only, and do not depend on the order in which
 // workgroups run.
 // ---------------------------------------------------------------------------
 constexpr int kGramSolveThreads = 256;
 constexpr int kGramSolveWaves = kGramSolveThreads / kWave;
 
+// Walk-order term slots: C_d at d, G_e,d (e < d) after them, then the
+// diagonal pairs (G_2j,2j, G_2j+1,2j+1).
+template <int ND>
+constexpr int WalkOffPair(int e, int d) {
+  return ND + d * (d - 1) / 2 + e;
+}
+template <int ND>
+constexpr int WalkDiag(int d) {
+  return ND + ND * (ND - 1) / 2 + d / 2;
+}
+
+// FP64 add to an LDS word (ds_add_f64, no value returned).
+__device__ __forceinline__ void LdsAdd(double* p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 template <int ND, bool F16>
 __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, const int part) {
   constexpr int NC = 2;
-  constexpr int NQ = ND + ND * (ND + 1) / 2;
+  constexpr int NQ = ND + ND * (ND - 1) / 2 + (ND + 1) / 2;
+  constexpr int kGainEntries = (ND * kWave + kGramSolveThreads - 1) / kGramSolveThreads;
   typedef float f2v __attribute__((ext_vector_type(2)));
   __shared__ float2 s_gain[ND][kWave];
-  __shared__ double s_red[kGramSolveWaves][ND][3][kWave];
+  // each wave's per-antenna sums (numerator re, im, denominator per
+  // direction): FP64 adds in LDS, which need no registers (a wave's region
+  // is its own, so every address is added to in program order)
+  __shared__ double s_acc[kGramSolveWaves][ND][3][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kWave);
   const int groups = a.gs_groups;
@@ -1004,58 +1031,81 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
   const int n = a.n_ant;
   const int iteration = AsConst(a.state)->iteration;
   const double2* cur = (iteration & 1) ? a.sol1 : a.sol0;
-  double2* nxt = (iteration & 1) ? a.sol0 : a.sol1;
   const size_t gbase = size_t(cb) * n * a.n_sub;
-  const double mean = AsConst(a.mean)[cb];
 
   const int rows = a.walk_rows;
   const int per = (rows + groups - 1) / groups;
   const int r0 = group * per;
   const int r1 = min(rows, r0 + per);
-  const uint32_t slots = uint32_t(rows) * kWave;
+  const uint32_t row_bytes = uint32_t(NQ) * uint32_t(n) * 8u;
   // the block's terms of polarization k through one buffer resource: a lane's
   // byte offset in a VGPR, the term's offset in an SGPR
-  const __amdgpu_buffer_rsrc_t grsrc =
-      PlaneResource(a.gram_walk + (size_t(cb) * NC + k) * NQ * slots, NQ * slots * 8u);
+  const __amdgpu_buffer_rsrc_t grsrc = PlaneResource(
+      a.gram_walk + (size_t(cb) * NC + k) * size_t(rows) * NQ * n, uint32_t(rows) * row_bytes);
   const GlobalT<int32_t>* codes = AsGlobal(a.walk_code);
 
   // Gains of polarization k of every antenna and direction (one solution
-  // each), repaired (a pair with a non-finite entry reads as the block mean)
-  // and cast to float, as the correlate uses them (SolveData.h:230-235).
-  for (int e = threadIdx.x; e < ND * kWave; e += kGramSolveThreads) {
+  // each). Their loads are issued first and the first row's cross products
+  // right after them: vector loads complete in order, so writing the gains
+  // to LDS waits for the gain loads only while the row streams in.
+  double2 gl[kGainEntries][2];
+#pragma unroll
+  for (int u = 0; u < kGainEntries; ++u) {
+    const int e = threadIdx.x + u * kGramSolveThreads;
     const int ant = e & (kWave - 1);
-    const int d = e / kWave;
-    float2 gf = make_float2(0.0f, 0.0f);
-    if (ant < n) {
-      const size_t pair = gbase + size_t(ant) * a.n_sub + AsConst(a.seg_sol)[d];
-      double2 g0 = LoadGlobal(cur + pair * 2), g1 = LoadGlobal(cur + pair * 2 + 1);
-      if (!(cfinite(g0) && cfinite(g1))) g0 = g1 = make_double2(mean, 0.0);
-      const double2 g = k ? g1 : g0;
-      gf = make_float2(float(g.x), float(g.y));
+    const int d = (e < ND * kWave ? e : 0) / kWave;
+    const size_t pair =
+        gbase + size_t(ant < n ? ant : 0) * a.n_sub + AsConst(a.seg_sol)[d];
+    gl[u][0] = LoadGlobal(cur + pair * 2);
+    gl[u][1] = LoadGlobal(cur + pair * 2 + 1);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  f2v q[NQ];
+  // lanes with a pair in row r: n, or n/2 in the last row for even n
+  auto load_row = [&](int r) {
+    const int lanes = (2 * (r + 1) == n) ? n / 2 : n;
+    const uint32_t off =
+        lane < lanes ? uint32_t(r) * row_bytes + uint32_t(lane) * 8u : 0x40000000u;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i)
+      q[i] = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(
+                                         grsrc, off, uint32_t(i) * uint32_t(n) * 8u, 0));
+  };
+  int r = r0 + wave;
+  if (r < r1) load_row(r);
+  __builtin_amdgcn_sched_barrier(0);
+  // repaired (a pair with a non-finite entry reads as the block mean) and
+  // cast to float, as the correlate uses them (SolveData.h:230-235)
+  {
+    const double mean = AsConst(a.mean)[cb];
+#pragma unroll
+    for (int u = 0; u < kGainEntries; ++u) {
+      const int e = threadIdx.x + u * kGramSolveThreads;
+      if (e < ND * kWave) {
+        const int ant = e & (kWave - 1);
+        double2 g0 = gl[u][0], g1 = gl[u][1];
+        if (!(cfinite(g0) && cfinite(g1))) g0 = g1 = make_double2(mean, 0.0);
+        const double2 g = k ? g1 : g0;
+        s_gain[e / kWave][ant] =
+            ant < n ? make_float2(float(g.x), float(g.y)) : make_float2(0.0f, 0.0f);
+      }
     }
-    s_gain[d][ant] = gf;
   }
   float sc[ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) sc[d] = F16 ? AsConst(a.model_scale)[cb * ND + d] : 1.0f;
   __syncthreads();
 
-  double acc[ND][3];
 #pragma unroll
-  for (int d = 0; d < ND; ++d) acc[d][0] = acc[d][1] = acc[d][2] = 0.0;
-  // Software pipelined: the next row's terms are loaded as soon as this
-  // row's U is formed, so they arrive while the FP64 part runs.
-  f2v q[NQ];
-  auto load_row = [&](int r) {
-    const uint32_t off = (uint32_t(r) * kWave + uint32_t(lane)) * 8u;
+  for (int d = 0; d < ND; ++d)
 #pragma unroll
-    for (int i = 0; i < NQ; ++i)
-      q[i] = __builtin_bit_cast(
-          f2v, __builtin_amdgcn_raw_buffer_load_b64(grsrc, off, uint32_t(i) * slots * 8u, 0));
-  };
-  int r = r0 + wave;
-  if (r < r1) load_row(r);
+    for (int c = 0; c < 3; ++c) s_acc[wave][d][c][lane] = 0.0;
+  // One row at a time (its terms loaded at the top of the loop, the first
+  // while the gains were staged): without a second row in flight a wave
+  // needs at most 168 registers, so three workgroups share a compute unit
+  // and the other waves cover a row's load latency.
   for (; r < r1; r += kGramSolveWaves) {
+    if (r != r0 + wave) load_row(r);
     const int step = r + 1;
     const uint32_t slot = uint32_t(r) * kWave + uint32_t(lane);
     const bool first = codes[slot] != 2;  // 1: lane's antenna is A; 0 (no baseline): zeros
@@ -1071,21 +1121,29 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
     for (int d = 0; d < ND; ++d) {
       const float2 go = s_gain[d][own_lane];
       const float2 gp = s_gain[d][pa];
-      // g_A conj(g_B) with the FP16 scale of d
-      const float2 gg = first ? cmulc(go, gp) : cmulc(gp, go);
-      ggs[d] = make_float2(gg.x * sc[d], gg.y * sc[d]);
+      // g_A conj(g_B) with the FP16 scale of d: g_own conj(g_partner) for
+      // antenna A's lane, its conjugate for B's (selects, no branch)
+      const float2 x = cmulc(go, gp);
+      ggs[d] = make_float2(x.x * sc[d], (first ? x.y : -x.y) * sc[d]);
     }
+    float ss[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) ss[d] = (d & 1) ? q[WalkDiag<ND>(d)].y : q[WalkDiag<ND>(d)].x;
     // U_d = C_d - sum_e ggs_e G_e,d (FP32, as the apply; G_e,d = conj(G_d,e)
-    // for e > d), four FMAs per complex term, and with it
-    // T_d / sc_d = U_d + ggs_d G_d,d
+    // for e > d, real for e = d), four FMAs per complex term
     float2 tt[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
       float ux = q[d].x, uy = q[d].y;
 #pragma unroll
       for (int e = 0; e < ND; ++e) {
-        const f2v ge = e <= d ? q[GramPair<ND>(e, d)] : q[GramPair<ND>(d, e)];
-        const float gi = e <= d ? ge.y : -ge.y;
+        if (e == d) {
+          ux = fmaf(-ggs[e].x, ss[d], ux);
+          uy = fmaf(-ggs[e].y, ss[d], uy);
+          continue;
+        }
+        const f2v ge = e < d ? q[WalkOffPair<ND>(e, d)] : q[WalkOffPair<ND>(d, e)];
+        const float gi = e < d ? ge.y : -ge.y;
         ux = fmaf(-ggs[e].x, ge.x, ux);
         ux = fmaf(ggs[e].y, gi, ux);
         uy = fmaf(-ggs[e].x, gi, uy);
@@ -1093,10 +1151,6 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
       }
       tt[d] = make_float2(ux, uy);
     }
-    float ss[ND];
-#pragma unroll
-    for (int d = 0; d < ND; ++d) ss[d] = q[GramPair<ND>(d, d)].x;
-    if (r + kGramSolveWaves < r1) load_row(r + kGramSolveWaves);
     // In units of the FP16 scale: T = sc_d T' with T' = U_d + ggs_d G_d,d and
     // S = sc_d^2 G_d,d, so the numerator sums carry sc_d and the denominator
     // sums sc_d^2, applied once when the sums are stored.
@@ -1106,7 +1160,7 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
       double2 t = make_double2(double(tt[d].x) + double(ggs[d].x) * sd,
                                double(tt[d].y) + double(ggs[d].y) * sd);
       // T with the lane's orientation: T for antenna A's lane, conj(T) for B's
-      if (!first) t.y = -t.y;
+      t.y = first ? t.y : -t.y;
       // the gains again from LDS (kept out of registers while the next row loads)
       const float2 go = s_gain[d][own_lane];
       const float2 gp = s_gain[d][pa];
@@ -1114,17 +1168,13 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
       const double2 p = make_double2(gp.x, gp.y);
       const double2 own = cmul(p, t);    // A: g_B T; B: g_A conj(T)
       const double2 prt = cmulc(o, t);   // partner B: g_A conj(T); partner A: g_B T
-      acc[d][0] += own.x + __shfl(prt.x, src);
-      acc[d][1] += own.y + __shfl(prt.y, src);
-      acc[d][2] += cnorm(p) * sd + __shfl(cnorm(o) * sd, src);
+      LdsAdd(&s_acc[wave][d][0][lane], own.x + __shfl(prt.x, src));
+      LdsAdd(&s_acc[wave][d][1][lane], own.y + __shfl(prt.y, src));
+      LdsAdd(&s_acc[wave][d][2][lane], cnorm(p) * sd + __shfl(cnorm(o) * sd, src));
     }
   }
 
-  // the four waves' sums, added in wave order; wave w stores directions w, w + 4
-#pragma unroll
-  for (int d = 0; d < ND; ++d)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) s_red[wave][d][c][lane] = acc[d][c];
+  // the four waves' sums, added in wave order
   __syncthreads();
   const size_t part_stride = size_t(ND) * 3 * kWave;  // doubles per workgroup's sums
   double* mine = a.gs_part + (size_t(cb) * 2 * groups + part) * part_stride;
@@ -1132,9 +1182,9 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
     const double scd = F16 ? double(AsConst(a.model_scale)[cb * ND + d]) : 1.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      double v = 0.0;
+      double v = s_acc[0][d][c][lane];
 #pragma unroll
-      for (int w = 0; w < kGramSolveWaves; ++w) v += s_red[w][d][c][lane];
+      for (int w = 1; w < kGramSolveWaves; ++w) v += s_acc[w][d][c][lane];
       mine[(d * 3 + c) * kWave + lane] = v * (c == 2 ? scd * scd : scd);
     }
   }
@@ -1159,16 +1209,30 @@ __device__ __forceinline__ void GramFinishBlock(const IterArgs& a, const int cb)
   const size_t part_stride = size_t(a.n_dir) * 3 * kWave;
   const GlobalT<double>* all = AsGlobal(a.gs_part) + size_t(cb) * 2 * groups * part_stride;
   for (int d = wave; d < a.n_dir; d += n_waves) {
+    // every sum loaded before the first is added (one round trip, not one per
+    // row group), then added in row-group order
+    double v[2][kGramSolveMaxGroups][3];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int g = 0; g < kGramSolveMaxGroups; ++g)
+        if (g < groups) {
+          const GlobalT<double>* x =
+              all + size_t(p * groups + g) * part_stride + size_t(d) * 3 * kWave + lane;
+          v[p][g][0] = x[0];
+          v[p][g][1] = x[kWave];
+          v[p][g][2] = x[2 * kWave];
+        }
     double nr[2] = {0.0, 0.0}, ni[2] = {0.0, 0.0}, dn[2] = {0.0, 0.0};
 #pragma unroll
     for (int p = 0; p < 2; ++p)
-      for (int g = 0; g < groups; ++g) {
-        const GlobalT<double>* x =
-            all + size_t(p * groups + g) * part_stride + size_t(d) * 3 * kWave + lane;
-        nr[p] += x[0];
-        ni[p] += x[kWave];
-        dn[p] += x[2 * kWave];
-      }
+#pragma unroll
+      for (int g = 0; g < kGramSolveMaxGroups; ++g)
+        if (g < groups) {
+          nr[p] += v[p][g][0];
+          ni[p] += v[p][g][1];
+          dn[p] += v[p][g][2];
+        }
     double conv_sum = 0.0, mean_sum = 0.0;
     int conv_n = 0, mean_n = 0;
     if (lane < n)
@@ -1199,34 +1263,61 @@ __global__ void __launch_bounds__(kGramFinishThreads) GramFinishKernel(const Ite
 
 // grid (2 x groups, n_cb)
 template <int ND, bool F16>
-__global__ void __launch_bounds__(kGramSolveThreads, 2) GramSolveKernel(const IterArgs a) {
+__global__ void __launch_bounds__(kGramSolveThreads, 3) GramSolveKernel(const IterArgs a) {
   if (AsConst(a.state)->done) return;
   GramSolveBlock<ND, F16>(a, blockIdx.y, blockIdx.x);
 }
 
-// Cross products from baseline order [cb][seg 0][k][q][bl_stride] into walk
-// order [cb][k][q][walk_rows * 64] (zeros where the walk has no baseline);
-// one workgroup per (block, correlation, term), so a term's baseline row is
-// read into one XCD's L2 once. Once per solve.
+// Cross products from baseline order [cb][seg 0][k][q][bl_stride] into the
+// walk order [cb][k][step row][qw][n] of the fused iteration (zeros where a
+// pair has no baseline): walk term qw < n_dir is C_qw, then the
+// off-diagonal G_e,d (e < d), then the real parts of the diagonal G_d,d two
+// per slot. One workgroup per (block, correlation, walk term), so a term's
+// baseline row is read into one XCD's L2 once. Once per solve.
 __device__ __forceinline__ void GramPermuteRow(const IterArgs& a, const int cb, const int k,
-                                               const int q, const int nq) {
+                                               const int qw) {
   typedef float f2v __attribute__((ext_vector_type(2)));
-  const size_t slots = size_t(a.walk_rows) * kWave;
+  const int nd = a.n_dir;
+  const int nq = GramTerms(nd);
+  const int n = a.n_ant;
+  const int nqw = GramWalkTerms(nd);
+  const int rows = a.walk_rows;
+  const int slots = rows * n;
   const GlobalT<f2v>* src = (const GlobalT<f2v>*)AsGlobal(
-      a.gram + (size_t(cb) * a.n_seg * 2 + k) * nq * a.bl_stride + size_t(q) * a.bl_stride);
-  GlobalT<f2v>* dst =
-      (GlobalT<f2v>*)AsGlobal(a.gram_walk + ((size_t(cb) * 2 + k) * nq + q) * slots);
+      a.gram + (size_t(cb) * a.n_seg * 2 + k) * nq * a.bl_stride);
+  GlobalT<f2v>* dst = (GlobalT<f2v>*)AsGlobal(
+      a.gram_walk + (size_t(cb) * 2 + k) * size_t(rows) * nqw * n + size_t(qw) * n);
+  // source terms (baseline-order index q = nd + d(d+1)/2 + e for G_e,d)
+  int q0 = qw, q1 = -1;  // q1 >= 0: a diagonal pair
+  const int n_off = nd * (nd - 1) / 2;
+  if (qw >= nd + n_off) {
+    const int d = 2 * (qw - nd - n_off);
+    q0 = nd + d * (d + 1) / 2 + d;
+    q1 = d + 1 < nd ? nd + (d + 1) * (d + 2) / 2 + d + 1 : -2;
+  } else if (qw >= nd) {
+    int d = 1, j = qw - nd;
+    while (j >= d) j -= d++;
+    q0 = nd + d * (d + 1) / 2 + j;  // G_j,d
+  }
   const GlobalT<int32_t>* bl = AsGlobal(a.walk_bl);
-  for (size_t i = threadIdx.x; i < slots; i += blockDim.x) {
-    const int b = bl[i];
+  for (int i = threadIdx.x; i < slots; i += blockDim.x) {
+    const int r = i / n;
+    const int b = bl[r * kWave + (i - r * n)];  // -1 past the last row's pairs
     f2v v = {0.0f, 0.0f};
-    if (b >= 0) v = src[b];
-    dst[i] = v;
+    if (b >= 0) {
+      if (q1 == -1) {
+        v = src[size_t(q0) * a.bl_stride + b];
+      } else {
+        v.x = src[size_t(q0) * a.bl_stride + b].x;
+        if (q1 >= 0) v.y = src[size_t(q1) * a.bl_stride + b].x;
+      }
+    }
+    dst[size_t(r) * nqw * n + (i - r * n)] = v;
   }
 }
 
 __global__ void __launch_bounds__(256) GramPermuteKernel(const IterArgs a) {
-  GramPermuteRow(a, blockIdx.z, blockIdx.y, blockIdx.x, GramTerms(a.n_dir));
+  GramPermuteRow(a, blockIdx.z, blockIdx.y, blockIdx.x);
 }
 
 // The repaired value of 2x2 solution `m` (MakeSolutionsFinite4Pol,
@@ -1455,6 +1546,9 @@ __device__ __forceinline__ void DecideBlock(const IterArgs& a) {
   __shared__ double red[kBlockThreads / 64];
   __shared__ int redc[kBlockThreads / 64];
   const int n_wg = a.n_cb * a.n_dir;
+  // the state is loaded first, so that its latency overlaps the sums'
+  DeviceState st;
+  if (threadIdx.x == 0) st = *a.state;
   double sum = 0.0;
   int count = 0;
   for (int i = threadIdx.x; i < n_wg; i += blockDim.x) {
@@ -1462,19 +1556,32 @@ __device__ __forceinline__ void DecideBlock(const IterArgs& a) {
     count += a.wg_conv_n[i];
   }
   BlockSum(sum, count, red, redc);
+  constexpr int kMeanBatch = 8;
   for (int c = threadIdx.x; c < a.n_cb; c += blockDim.x) {
     double ms = 0.0;
     int mn = 0;
-    for (int dd = 0; dd < a.n_dir; ++dd) {
-      ms += a.wg_mean[c * a.n_dir + dd];
-      mn += a.wg_mean_n[c * a.n_dir + dd];
+    // the block's terms loaded kMeanBatch at a time, added in direction order
+    for (int d0 = 0; d0 < a.n_dir; d0 += kMeanBatch) {
+      double mv[kMeanBatch];
+      int nv[kMeanBatch];
+#pragma unroll
+      for (int j = 0; j < kMeanBatch; ++j)
+        if (d0 + j < a.n_dir) {
+          mv[j] = a.wg_mean[c * a.n_dir + d0 + j];
+          nv[j] = a.wg_mean_n[c * a.n_dir + d0 + j];
+        }
+#pragma unroll
+      for (int j = 0; j < kMeanBatch; ++j)
+        if (d0 + j < a.n_dir) {
+          ms += mv[j];
+          mn += nv[j];
+        }
     }
     // no finite solution: 1 for both polarizations (SolverBase.cc:356-358),
     // the identity for 2x2 solutions (:390-394), marked by -1
     a.mean[c] = mn == 0 ? (a.full_jones ? -1.0 : 1.0) : ms / mn;
   }
   if (threadIdx.x == 0) {
-    DeviceState st = *a.state;
     const int it = st.iteration + 1;
     // AssignSolutions (SolverBase.cc:296-306). The only constraint the device
     // applies is the smoothness constraint, which is always satisfied
@@ -2207,7 +2314,7 @@ __global__ void __launch_bounds__(kBlockThreads)
 
 // items (interval, cb, part); one workgroup per item
 template <int ND, bool F16>
-__global__ void __launch_bounds__(kGramSolveThreads, 2)
+__global__ void __launch_bounds__(kGramSolveThreads, 3)
     BatchGramSolveKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ items) {
   const BatchTile t = LoadConst(items + XcdContiguous(blockIdx.x, gridDim.x));
   const IterArgs a = LoadConst(args + t.interval);
@@ -2229,9 +2336,8 @@ __global__ void __launch_bounds__(256)
     BatchGramPermuteKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ items) {
   const BatchTile t = LoadConst(items + blockIdx.z);
   const IterArgs a = LoadConst(args + t.interval);
-  const int nq = GramTerms(a.n_dir);
-  if (int(blockIdx.x) >= nq) return;
-  GramPermuteRow(a, t.cb, blockIdx.y, blockIdx.x, nq);
+  if (int(blockIdx.x) >= GramWalkTerms(a.n_dir)) return;
+  GramPermuteRow(a, t.cb, blockIdx.y, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(256) BatchDecideKernel(const IterArgs* __restrict__ args) {
@@ -2492,7 +2598,7 @@ const std::array<BatchKernelFn, kMaxGramDirections> kBatchGramSolve[2] = {
 
 void LaunchGramPermute(const IterArgs& a, hipStream_t stream) {
   DDH_REQUIRE(a.walk_rows > 0 && a.n_seg == 1, -1, "no walk order for this problem");
-  hipLaunchKernelGGL(GramPermuteKernel, dim3(GramTerms(a.n_dir), 2, a.n_cb), dim3(256), 0, stream,
+  hipLaunchKernelGGL(GramPermuteKernel, dim3(GramWalkTerms(a.n_dir), 2, a.n_cb), dim3(256), 0, stream,
                      a);
   DDH_HIP_CHECK(hipGetLastError());
 }
@@ -2510,7 +2616,7 @@ void LaunchGramSolve(const IterArgs& a, bool f16, hipStream_t stream) {
 void LaunchBatchGramPermute(const IterArgs* d_args, const BatchTile* d_items, int n_items,
                             int max_dir, hipStream_t stream) {
   if (n_items == 0) return;
-  hipLaunchKernelGGL(BatchGramPermuteKernel, dim3(GramTerms(max_dir), 2, n_items), dim3(256), 0,
+  hipLaunchKernelGGL(BatchGramPermuteKernel, dim3(GramWalkTerms(max_dir), 2, n_items), dim3(256), 0,
                      stream, d_args, d_items);
   DDH_HIP_CHECK(hipGetLastError());
 }

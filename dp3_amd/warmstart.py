@@ -206,16 +206,19 @@ def training_set(seq: IntervalSequence, ws: WarmStart, first, last, converge_ite
     return np.concatenate(xs), np.concatenate(ts), prev
 
 
-def training_sets_batched(seqs, ws: WarmStart, first, last, converge_iterations=1000):
+def training_sets_batched(seqs, ws: WarmStart, first, last, converge_iterations=1000,
+                          use_graph=1):
     """training_set for several interval sequences of one shape at once: each
     step builds interval i of every sequence into its scratch problem, takes
     the features from the previous converged solutions, and solves all of
     them to convergence (tolerance 1e-9) in one batch with the Gram
-    iteration. Returns per sequence (x, t, last converged solutions)."""
+    iteration (its iterations replayed as HIP graphs unless use_graph is 0).
+    Returns per sequence (x, t, last converged solutions)."""
     dps = [q.transient(first) for q in seqs]
     longs = [sibling(q.solver, converge_iterations, 1e-9, False, iteration=H.ITERATION_GRAM)
              for q in seqs]
     batch = H.Batch(longs, dps)
+    batch.set_schedule(use_graph=use_graph)
     prev = [q.unit() for q in seqs]
     batch.solve(prev)  # the first interval of each sequence from unit gains
     xs = [[] for _ in seqs]

@@ -44,10 +44,16 @@ def test_uneven_batch_of_64_intervals():
     waves, rows, ms = batch.autotune()
     print(f"autotuned waves {waves} rows {rows}: correlate {ms:.3f} ms over 64 intervals")
     batch.set_schedule(waves, rows, 1, 8)
+    # The antenna solve's threads per (antenna, solution) set its FP64
+    # summation order: the batch and the single solves get the same one (the
+    # autotuner may keep each interval's default, which differs by size).
+    split, _ = batch.antenna_solve_schedule()
+    batch.set_antenna_solve_schedule(split, 0)
     starts = [q.unit() for q in seqs]
     res = batch.solve(starts)
     for i, (q, s) in enumerate(zip(seqs, solvers)):
         s.set_schedule(waves_per_tile=waves, rows_per_trip=rows, use_graph=0, poll_interval=8)
+        s.set_antenna_solve_schedule(split, 0)
         one = q.unit()
         r = s.solve_problem(q.problems[0], one)
         assert r.iterations == res[i].iterations, i
