@@ -413,7 +413,8 @@ def main():
     predict_line = {
         "bound": "valu", "achieved": round(pred_gflops, 1), "peak": FP64_PEAK_GFLOPS,
         "unit": "GFLOP/s (FP64)", "frac": round(pred_gflops / FP64_PEAK_GFLOPS, 4),
-        "traffic": None, "kernel": "predict", "mean_launch_ms": round(predict_ms, 4),
+        "traffic": measured_traffic("predict_c64"), "kernel": "predict",
+        "mean_launch_ms": round(predict_ms, 4),
         "note": (f"ddh_predict of one interval: {len(q.sky.is_gaussian)} components "
                  f"({n_point} points, {n_gauss} Gaussians) x {bl_ch_t} visibilities, FP64 "
                  "phasors, envelope and sums rounded once to complex64; wall time of the call "

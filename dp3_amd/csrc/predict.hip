@@ -151,40 +151,6 @@ __global__ void SpectrumKernel(const ddh_component* comps, int n_comp, const dou
   spec[idx] = Stokes4{I, Q, U, V};
 }
 
-// Station phase phi = 2 pi / c (u l + v m + w (n - 1)) per (t, component,
-// station) and its per-channel phasor exp(i phi f) for the channels of one
-// block: table[(t * n_chb + ch) * n_comp + c][st] (complex128). With
-// scaled_ncp_uvw (time smearing), also the station earth-rotation phase
-// 2 pi / c (l (v ncp_w - w ncp_v) - m u ncp_w + (n - 1) u ncp_v)
-// (Simulator.cc:488-503) per (t, component, station).
-__global__ void StationPhasorKernel(const double* uvw, int n_st, const CompGeom* geom,
-                                    int n_comp, const double* freq, int ch0, int n_chb,
-                                    int n_t, double2* table, double* st_phase, double* st_rot,
-                                    double ncp_v, double ncp_w) {
-  const int64_t total = int64_t(n_t) * n_chb * n_comp * n_st;
-  const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= total) return;
-  const int st = int(idx % n_st);
-  int64_t r = idx / n_st;
-  const int c = int(r % n_comp);
-  r /= n_comp;
-  const int ch = int(r % n_chb);
-  const int t = int(r / n_chb);
-  const double* u = uvw + (int64_t(t) * n_st + st) * 3;
-  const CompGeom& g = geom[c];
-  constexpr double cinv = 2.0 * M_PI / kSpeedOfLight;
-  const double phi = cinv * (u[0] * g.l + u[1] * g.m + u[2] * g.nm1);
-  double s, co;
-  sincos(phi * freq[ch0 + ch], &s, &co);
-  table[idx] = make_double2(co, s);
-  if (ch == 0) {
-    const int64_t k = (int64_t(t) * n_comp + c) * n_st + st;
-    st_phase[k] = phi;
-    st_rot[k] = cinv * (g.l * (u[1] * ncp_w - u[2] * ncp_v) + g.m * (-u[0] * ncp_w) +
-                        g.nm1 * (u[0] * ncp_v));
-  }
-}
-
 __device__ __forceinline__ float Sinc(float x) { return fabsf(x) < 1e-6f ? 1.0f : sinf(x) / x; }
 
 typedef _Float16 half2_v __attribute__((ext_vector_type(2)));
@@ -195,113 +161,213 @@ __device__ __forceinline__ uint32_t HalfBits(float re, float im) {
   return __builtin_bit_cast(uint32_t, h);
 }
 
-// One thread per baseline of row (t, ch); loops over the components of
-// direction d in FP64 (Simulator.cc:157-281 for points, :284-468 for
-// Gaussians: the envelope, then the time- and frequency-smearing factors, on
-// shift_q conj(shift_p) S) and rounds the direction's sum once. Writes the
-// model plane of (cb, d).
-template <int NC, bool F16>
-__global__ void BaselinePredictKernel(const double2* __restrict__ table,
-                                      const double* __restrict__ st_phase,
-                                      const double* __restrict__ st_rot,
-                                      const CompGeom* __restrict__ geom,
-                                      const Stokes4* __restrict__ spec,
-                                      const double* __restrict__ uvw,
-                                      const int32_t* __restrict__ ant1,
-                                      const int32_t* __restrict__ ant2,
-                                      const double* __restrict__ freq,
-                                      const double* __restrict__ chan_width, int n_bl,
-                                      int bl_stride, int n_st, int n_comp, int c0, int c1,
-                                      int ch0, int n_chb, int n_ch_total, int stokes_i_only,
-                                      int freq_smear, int time_smear, const float* scale,
-                                      void* plane) {
-  const int bl = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ch = blockIdx.y;
-  const int t = blockIdx.z;
-  if (bl >= bl_stride) return;
-  const int64_t out_idx = (int64_t(t) * n_chb + ch) * bl_stride + bl;
-  double2 v[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0),
-                  make_double2(0, 0)};
-  if (bl < n_bl) {
-    const int p = ant1[bl], q = ant2[bl];
-    const double* up = uvw + (int64_t(t) * n_st + p) * 3;
-    const double* uq = uvw + (int64_t(t) * n_st + q) * 3;
-    const double f = freq[ch0 + ch];
-    const double f2c = f * f * (1.0 / (kSpeedOfLight * kSpeedOfLight));
-    const double width = freq_smear ? chan_width[ch0 + ch] : 0.0;
-    const double2* row = table + (int64_t(t) * n_chb + ch) * n_comp * int64_t(n_st);
-    const double* phase_t = st_phase + int64_t(t) * n_comp * n_st;
-    const double* rot_t = st_rot + int64_t(t) * n_comp * n_st;
-    for (int c = c0; c < c1; ++c) {
-      const double2 sp = row[int64_t(c) * n_st + p];
-      const double2 sq = row[int64_t(c) * n_st + q];
-      // shift_q * conj(shift_p)
-      const double2 ph = make_double2(sp.x * sq.x + sp.y * sq.y, sp.x * sq.y - sq.x * sp.y);
-      const CompGeom& g = geom[c];
-      double amp = 1.0;
-      if (g.is_gaussian) {
-        // station UVW rotated to the source (identity unless absolute PA), then
-        // the baseline's (u, v) difference (Simulator.cc:312-342)
-        double uvq[2], uvp[2];
+// ---- the per-visibility predict ------------------------------------------
+// One workgroup per (channel, time step, direction, baseline chunk of
+// kPredictThreads * kPredictBaselines): each thread sums the direction's
+// components for up to kPredictBaselines baselines in FP64 registers and
+// rounds the sum once into the model plane (Simulator.cc accumulates in
+// double, OnePredict casts to complex64).
+//
+// A direction's components come points first, then Gaussians (the host
+// orders them so; the sum is the same up to FP64 rounding), and are taken in
+// chunks that fit the LDS. For a chunk the workgroup first forms, per
+// (component, station), the terms every baseline of the station shares:
+//   phase   phi_s = 2 pi / c (u l + v m + w (n - 1))           (Simulator.cc:487-488)
+//   shift   exp(i phi_s f) (cos, sin)                           (:510-521)
+//   rot_s   the earth-rotation phase of time smearing           (:496-503)
+//   uv'_s   the station's (u, v) rotated to the source (absolute position
+//           angle) and by the Gaussian's position angle, scaled by the axes
+//           (:312-342, :360-366)
+// and the components' spectra at the channel, so a baseline's Gaussian
+// (u', v') is uv'_q - uv'_p (the rotation and scaling are linear; the
+// reference forms u_q - u_p first, which differs by FP64 rounding only).
+// Then every thread runs over the chunk's points and Gaussians for each of
+// its baselines: shift_q conj(shift_p) times the spectrum, times the
+// envelope exp(f^2 / c^2 (-2 pi^2 (u'^2 + v'^2))) and the smearing factors,
+// with no per-component branch.
+// ---------------------------------------------------------------------------
+constexpr int kPredictThreads = 512;
+constexpr int kPredictBaselines = 4;  // baselines per thread
+constexpr size_t kPredictLds = 48 * 1024;
+
+// Per global channel: where its rows start in the model planes of its block.
+struct PredictChannel {
+  int64_t base;      // model_off(cb) + ch_in_block * bl_stride (visibilities)
+  int64_t plane;     // visibilities of one (block, direction) plane
+  int64_t t_stride;  // n_chb * bl_stride
+  int32_t cb;
+  int32_t pad;
+};
+
+struct PredictArgs {
+  const double* uvw;           // [n_t][n_st][3]
+  const CompGeom* geom;        // [n_comp], by direction, points before Gaussians
+  const Stokes4* spec;         // [n_comp][n_ch_total]
+  const int32_t* c_off;        // [n_dir + 1] component ranges per direction
+  const int32_t* g_off;        // [n_dir] first Gaussian of each direction
+  const double* freq;          // [n_ch_total]
+  const double* width;         // [n_ch_total]
+  const int32_t* ant1;         // [n_bl]
+  const int32_t* ant2;
+  const PredictChannel* chan;  // [n_ch_total]
+  const float* scale;          // [n_cb][n_dir] (FP16 planes)
+  void* model;
+  int32_t n_st, n_bl, bl_stride, n_t, n_dir, n_ch_total, chunk;
+  int32_t freq_smear, time_smear;
+  double ncp_v, ncp_w;
+};
+
+// LDS doubles per (component, station) of a chunk: shift (2) and Gaussian
+// uv' (2), with smearing also rot and phase; then 4 per component (Stokes).
+constexpr int PredictWords(bool smear) { return smear ? 6 : 4; }
+
+// SI: Stokes I only (XX = YY = I, XY = YX = 0); SMEAR: time or frequency
+// smearing on.
+template <int NC, bool F16, bool SI, bool SMEAR>
+__device__ __forceinline__ void PredictBody(const PredictArgs& a) {
+  extern __shared__ double s_st[];
+  constexpr int W = PredictWords(SMEAR);
+  const int ch = blockIdx.x;
+  const int t = blockIdx.y / a.n_dir;
+  const int d = blockIdx.y - t * a.n_dir;
+  const int bl0 = blockIdx.z * (kPredictThreads * kPredictBaselines);
+  const PredictChannel pc = a.chan[ch];
+  const int c_begin = a.c_off[d], c_end = a.c_off[d + 1], g_begin = a.g_off[d];
+  const double f = a.freq[ch];
+  const double f2c = f * f * (1.0 / (kSpeedOfLight * kSpeedOfLight));
+  const double width = (SMEAR && a.freq_smear) ? a.width[ch] : 0.0;
+  const int n_st = a.n_st;
+  constexpr double cinv = 2.0 * M_PI / kSpeedOfLight;
+  constexpr int NV = SI ? 1 : (NC == 2 ? 2 : 4);  // sums kept: I, (XX, YY) or all four
+  double2 v[kPredictBaselines][NV];
+  int pq[kPredictBaselines][2];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          uvq[i] = g.rot[i * 3] * uq[0] + g.rot[i * 3 + 1] * uq[1] + g.rot[i * 3 + 2] * uq[2];
-          uvp[i] = g.rot[i * 3] * up[0] + g.rot[i * 3 + 1] * up[1] + g.rot[i * 3 + 2] * up[2];
-        }
-        const double u = uvq[0] - uvp[0];
-        const double vv = uvq[1] - uvp[1];
-        const double up_ = g.u_scale * (u * g.cos_phi - vv * g.sin_phi);
-        const double vp_ = g.v_scale * (u * g.sin_phi + vv * g.cos_phi);
-        const double uv = (-2.0 * M_PI * M_PI) * (up_ * up_ + vp_ * vp_);
-        amp = exp(f2c * uv);
+  for (int j = 0; j < kPredictBaselines; ++j) {
+    const int bl = bl0 + int(threadIdx.x) + j * kPredictThreads;
+    const bool ok = bl < a.n_bl;
+    pq[j][0] = ok ? a.ant1[bl] : 0;
+    pq[j][1] = ok ? a.ant2[bl] : 0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[j][k] = make_double2(0.0, 0.0);
+  }
+  const double* uvw_t = a.uvw + size_t(t) * n_st * 3;
+  for (int c0 = c_begin; c0 < c_end; c0 += a.chunk) {
+    const int nc = min(a.chunk, c_end - c0);
+    const int np = min(max(g_begin - c0, 0), nc);  // the chunk's points
+    double* s_spec = s_st + size_t(nc) * W * n_st;
+    __syncthreads();  // the previous chunk's terms are no longer read
+    for (int e = threadIdx.x; e < nc * n_st; e += kPredictThreads) {
+      const int ci = e / n_st;
+      const int st = e - ci * n_st;
+      const CompGeom& g = a.geom[c0 + ci];
+      const double u = uvw_t[st * 3], vv = uvw_t[st * 3 + 1], w = uvw_t[st * 3 + 2];
+      const double phi = cinv * (u * g.l + vv * g.m + w * g.nm1);
+      double sn, cs;
+      sincos(phi * f, &sn, &cs);
+      double* row = s_st + size_t(ci) * W * n_st + st;
+      row[0] = cs;
+      row[n_st] = sn;
+      if (ci >= np) {
+        const double ur = g.rot[0] * u + g.rot[1] * vv + g.rot[2] * w;
+        const double vr = g.rot[3] * u + g.rot[4] * vv + g.rot[5] * w;
+        row[2 * n_st] = g.u_scale * (ur * g.cos_phi - vr * g.sin_phi);
+        row[3 * n_st] = g.v_scale * (ur * g.sin_phi + vr * g.cos_phi);
       }
-      if (time_smear)
-        amp *= double(Sinc(float(0.5 * (rot_t[int64_t(c) * n_st + q] -
-                                        rot_t[int64_t(c) * n_st + p]) * f)));
-      if (freq_smear)
-        amp *= double(Sinc(float(0.5 * (phase_t[int64_t(c) * n_st + q] -
-                                        phase_t[int64_t(c) * n_st + p]) * width)));
-      const Stokes4 s = spec[int64_t(c) * n_ch_total + ch0 + ch];
-      if (stokes_i_only) {
-        v[0].x += amp * (ph.x * s.I);
-        v[0].y += amp * (ph.y * s.I);
-      } else {
-        // XX = I + Q, XY = U + iV, YX = U - iV, YY = I - Q
-        const double2 sv[4] = {make_double2(s.I + s.Q, 0.0), make_double2(s.U, s.V),
-                               make_double2(s.U, -s.V), make_double2(s.I - s.Q, 0.0)};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          v[k].x += amp * (ph.x * sv[k].x - ph.y * sv[k].y);
-          v[k].y += amp * (ph.x * sv[k].y + ph.y * sv[k].x);
-        }
+      if constexpr (SMEAR) {
+        row[4 * n_st] = cinv * (g.l * (vv * a.ncp_w - w * a.ncp_v) + g.m * (-u * a.ncp_w) +
+                                g.nm1 * (u * a.ncp_v));
+        row[5 * n_st] = phi;
       }
     }
-    if (stokes_i_only) v[3] = v[0];
+    for (int e = threadIdx.x; e < nc; e += kPredictThreads) {
+      const Stokes4 s = a.spec[size_t(c0 + e) * a.n_ch_total + ch];
+      s_spec[4 * e] = s.I;
+      s_spec[4 * e + 1] = s.Q;
+      s_spec[4 * e + 2] = s.U;
+      s_spec[4 * e + 3] = s.V;
+    }
+    __syncthreads();
+    // one component's contribution to baseline j: shift_q conj(shift_p)
+    // (Simulator.cc:212-213) times the amplitude and the spectrum
+    auto add = [&](const int j, const int ci, const double amp_in, const bool gauss) {
+      const int p = pq[j][0], q = pq[j][1];
+      const double* row = s_st + size_t(ci) * W * n_st;
+      const double xp = row[p], yp = row[n_st + p], xq = row[q], yq = row[n_st + q];
+      const double re = xp * xq + yp * yq;
+      const double im = xp * yq - xq * yp;
+      double amp = amp_in;
+      if (gauss) {
+        const double du = row[2 * n_st + q] - row[2 * n_st + p];
+        const double dv = row[3 * n_st + q] - row[3 * n_st + p];
+        amp = exp(f2c * ((-2.0 * M_PI * M_PI) * (du * du + dv * dv)));
+      }
+      if constexpr (SMEAR) {
+        if (a.time_smear)
+          amp *= double(Sinc(float(0.5 * (row[4 * n_st + q] - row[4 * n_st + p]) * f)));
+        if (a.freq_smear)
+          amp *= double(Sinc(float(0.5 * (row[5 * n_st + q] - row[5 * n_st + p]) * width)));
+      }
+      const double* sp = s_spec + 4 * ci;
+      if constexpr (SI) {
+        const double I = sp[0];
+        v[j][0].x += amp * (re * I);
+        v[j][0].y += amp * (im * I);
+      } else {
+        // XX = I + Q, XY = U + iV, YX = U - iV, YY = I - Q
+        const double I = sp[0], Q = sp[1], U = sp[2], V = sp[3];
+        const double2 sv[4] = {make_double2(I + Q, 0.0), make_double2(U, V),
+                               make_double2(U, -V), make_double2(I - Q, 0.0)};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int kk = NC == 2 ? (k == 0 ? 0 : (k == 3 ? 1 : -1)) : k;
+          if (kk < 0) continue;
+          v[j][kk].x += amp * (re * sv[k].x - im * sv[k].y);
+          v[j][kk].y += amp * (re * sv[k].y + im * sv[k].x);
+        }
+      }
+    };
+#pragma unroll
+    for (int j = 0; j < kPredictBaselines; ++j) {
+#pragma unroll 2
+      for (int ci = 0; ci < np; ++ci) add(j, ci, 1.0, false);
+#pragma unroll 2
+      for (int ci = np; ci < nc; ++ci) add(j, ci, 1.0, true);
+    }
   }
-  // correlations stored: Duo (XX, YY) or Full (XX, XY, YX, YY)
-  double2 out[NC];
-  if constexpr (NC == 2) {
-    out[0] = v[0];
-    out[1] = v[3];
-  } else {
+  // correlations stored: Duo (XX, YY) or Full (XX, XY, YX, YY); Stokes-I
+  // only: XX = YY = I, XY = YX = 0
+  const double inv = F16 ? 1.0 / double(a.scale[size_t(pc.cb) * a.n_dir + d]) : 1.0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) out[k] = v[k];
-  }
-  if constexpr (!F16) {
-    float4* o = reinterpret_cast<float4*>(plane) + out_idx * (NC / 2);
+  for (int j = 0; j < kPredictBaselines; ++j) {
+    const int bl = bl0 + int(threadIdx.x) + j * kPredictThreads;
+    if (bl >= a.bl_stride) break;
+    double2 out[NC];
 #pragma unroll
-    for (int h = 0; h < NC / 2; ++h)
-      o[h] = make_float4(float(out[2 * h].x), float(out[2 * h].y), float(out[2 * h + 1].x),
-                         float(out[2 * h + 1].y));
-  } else {
-    const double inv = 1.0 / double(*scale);
-    uint32_t bits[NC];
+    for (int k = 0; k < NC; ++k) {
+      if (bl >= a.n_bl)  // padding lanes hold zeros
+        out[k] = make_double2(0.0, 0.0);
+      else if constexpr (SI)
+        out[k] = (NC == 2 || k == 0 || k == 3) ? v[j][0] : make_double2(0.0, 0.0);
+      else
+        out[k] = v[j][k];
+    }
+    const int64_t idx = pc.base + d * pc.plane + int64_t(t) * pc.t_stride + bl;
+    if constexpr (!F16) {
+      float4* o = reinterpret_cast<float4*>(a.model) + idx * (NC / 2);
 #pragma unroll
-    for (int k = 0; k < NC; ++k) bits[k] = HalfBits(float(out[k].x * inv), float(out[k].y * inv));
-    if constexpr (NC == 2) {
-      reinterpret_cast<uint2*>(plane)[out_idx] = make_uint2(bits[0], bits[1]);
+      for (int h = 0; h < NC / 2; ++h)
+        o[h] = make_float4(float(out[2 * h].x), float(out[2 * h].y), float(out[2 * h + 1].x),
+                           float(out[2 * h + 1].y));
     } else {
-      reinterpret_cast<uint4*>(plane)[out_idx] = make_uint4(bits[0], bits[1], bits[2], bits[3]);
+      uint32_t bits[NC];
+#pragma unroll
+      for (int k = 0; k < NC; ++k)
+        bits[k] = HalfBits(float(out[k].x * inv), float(out[k].y * inv));
+      if constexpr (NC == 2) {
+        reinterpret_cast<uint2*>(a.model)[idx] = make_uint2(bits[0], bits[1]);
+      } else {
+        reinterpret_cast<uint4*>(a.model)[idx] = make_uint4(bits[0], bits[1], bits[2], bits[3]);
+      }
     }
   }
 }
@@ -309,7 +375,7 @@ __global__ void BaselinePredictKernel(const double2* __restrict__ table,
 // FP16 scale bound per direction: max over channels of sum_c |S_c| (every
 // model value of the direction is bounded by it); largest maps to 2^14.
 __global__ void DirectionScaleKernel(const Stokes4* spec, const CompGeom* geom, int n_comp,
-                                     int n_ch_total, int n_dir, float* bound) {
+                                     int n_ch_total, int n_dir, int n_cb, float* scale) {
   const int d = blockIdx.x;
   __shared__ float red[256];
   float m = 0.0f;
@@ -328,7 +394,9 @@ __global__ void DirectionScaleKernel(const Stokes4* spec, const CompGeom* geom, 
     if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) bound[d] = red[0] > 0.0f ? red[0] / 16384.0f : 1.0f;
+  // the same scale for every block: [n_cb][n_dir]
+  const float b = red[0] > 0.0f ? red[0] / 16384.0f : 1.0f;
+  for (int cb = threadIdx.x; cb < n_cb; cb += blockDim.x) scale[size_t(cb) * n_dir + d] = b;
 }
 
 // ---- data synthesis -------------------------------------------------------
@@ -438,6 +506,36 @@ struct Scratch {
   }
 };
 
+template <int NC, bool F16, bool SI, bool SMEAR>
+__global__ void __launch_bounds__(kPredictThreads) PredictKernel(const PredictArgs a) {
+  PredictBody<NC, F16, SI, SMEAR>(a);
+}
+// Stokes I without smearing (the common case) at four waves per SIMD: two
+// workgroups per compute unit instead of one; the registers it gives up cost
+// a spill or two in the station terms' sincos, once per (component, station).
+template <int NC, bool F16>
+__global__ void __launch_bounds__(kPredictThreads) __attribute__((amdgpu_waves_per_eu(4)))
+    PredictStokesIKernel(const PredictArgs a) {
+  PredictBody<NC, F16, true, false>(a);
+}
+
+using PredictFn = void (*)(PredictArgs);
+template <int NC, bool F16>
+PredictFn PredictVariant(bool si, bool smear) {
+  if (si) return smear ? PredictKernel<NC, F16, true, true> : PredictStokesIKernel<NC, F16>;
+  return smear ? PredictKernel<NC, F16, false, true> : PredictKernel<NC, F16, false, false>;
+}
+
+void LaunchPredict(bool duo, bool f16, bool si, bool smear, dim3 grid, size_t lds,
+                   hipStream_t s, const PredictArgs& pa) {
+  const PredictFn fn = duo ? (f16 ? PredictVariant<2, true>(si, smear)
+                                  : PredictVariant<2, false>(si, smear))
+                           : (f16 ? PredictVariant<4, true>(si, smear)
+                                  : PredictVariant<4, false>(si, smear));
+  hipLaunchKernelGGL(fn, grid, dim3(kPredictThreads), lds, s, pa);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace
 }  // namespace ddh
 
@@ -457,26 +555,30 @@ extern "C" int ddh_predict(ddh_problem* problem, double ref_ra, double ref_dec,
     DDH_REQUIRE(!v.uni, -4, "the predict writes XX and YY; single-correlation problems take "
                             "their models through upload or ingest");
     std::vector<ddh_component> comps(components, components + n_components);
-    // group components by direction (stable)
-    std::stable_sort(comps.begin(), comps.end(),
-                     [](const ddh_component& a, const ddh_component& b) {
-                       return a.direction < b.direction;
-                     });
-    std::vector<int> c_off(v.n_dir + 1, 0);
     for (const ddh_component& c : comps) {
       DDH_REQUIRE(c.direction >= 0 && c.direction < v.n_dir, -1, "component direction out of range");
       DDH_REQUIRE(c.n_spectral_terms <= 8, -1, "at most 8 spectral terms");
-      c_off[c.direction + 1]++;
     }
+    // group components by direction, points before Gaussians (stable)
+    std::stable_sort(comps.begin(), comps.end(),
+                     [](const ddh_component& a, const ddh_component& b) {
+                       return a.direction != b.direction ? a.direction < b.direction
+                                                         : (a.is_gaussian != 0) < (b.is_gaussian != 0);
+                     });
+    std::vector<int32_t> c_off(v.n_dir + 1, 0), g_off(v.n_dir, 0);
+    for (const ddh_component& c : comps) c_off[c.direction + 1]++;
     for (int d = 0; d < v.n_dir; ++d) c_off[d + 1] += c_off[d];
+    for (int d = 0; d < v.n_dir; ++d) {
+      g_off[d] = c_off[d];
+      while (g_off[d] < c_off[d + 1] && !comps[g_off[d]].is_gaussian) ++g_off[d];
+    }
     int n_ch_total = 0;
     for (int x : v.ch_per_cb) n_ch_total += x;
     hipStream_t s = v.stream;
     static thread_local Scratch<ddh_component> d_comps;
     static thread_local Scratch<CompGeom> d_geom;
     static thread_local Scratch<Stokes4> d_spec;
-    static thread_local Scratch<double> d_uvw, d_freq, d_width, d_phase, d_rot;
-    static thread_local Scratch<double2> d_table;
+    static thread_local Scratch<double> d_uvw, d_freq, d_width;
     ddh_component* dc = d_comps.Get(comps.size());
     DDH_HIP_CHECK(hipMemcpyAsync(dc, comps.data(), comps.size() * sizeof(ddh_component),
                                  hipMemcpyHostToDevice, s));
@@ -497,53 +599,68 @@ extern "C" int ddh_predict(ddh_problem* problem, double ref_ra, double ref_dec,
     hipLaunchKernelGGL(SpectrumKernel, dim3((n_spec + 255) / 256), dim3(256), 0, s, dc,
                        n_components, df, n_ch_total, dspec);
     DDH_HIP_CHECK(hipGetLastError());
-    if (v.f16) {
-      // one scale per direction, shared by every block
-      static thread_local Scratch<float> d_bound;
-      float* bound = d_bound.Get(v.n_dir);
+    if (v.f16)  // one scale per direction, shared by every block
       hipLaunchKernelGGL(DirectionScaleKernel, dim3(v.n_dir), dim3(256), 0, s, dspec, dg,
-                         n_components, n_ch_total, v.n_dir, bound);
-      for (int cb = 0; cb < v.n_cb; ++cb)
-        DDH_HIP_CHECK(hipMemcpyAsync(v.scale + size_t(cb) * v.n_dir, bound, v.n_dir * sizeof(float),
-                                     hipMemcpyDeviceToDevice, s));
-    }
-    double* dphase = d_phase.Get(size_t(v.n_t) * n_components * n_stations);
-    double* drot = d_rot.Get(size_t(v.n_t) * n_components * n_stations);
-    const int time_smear = scaled_ncp_uvw ? 1 : 0;
-    const double ncp_v = scaled_ncp_uvw ? scaled_ncp_uvw[1] : 0.0;
-    const double ncp_w = scaled_ncp_uvw ? scaled_ncp_uvw[2] : 0.0;
-    int ch0 = 0;
-    for (int cb = 0; cb < v.n_cb; ++cb) {
-      const int n_chb = v.ch_per_cb[cb];
-      const int64_t n_tab = int64_t(v.n_t) * n_chb * n_components * n_stations;
-      double2* tab = d_table.Get(size_t(n_tab));
-      hipLaunchKernelGGL(StationPhasorKernel, dim3(unsigned((n_tab + 255) / 256)), dim3(256), 0, s,
-                         du, n_stations, dg, n_components, df, ch0, n_chb, v.n_t, tab, dphase,
-                         drot, ncp_v, ncp_w);
-      DDH_HIP_CHECK(hipGetLastError());
-      const size_t plane = size_t(v.n_t) * n_chb * v.bl_stride;
-      const size_t vis_bytes = v.f16 ? size_t(v.n_corr) * 4 : size_t(v.n_corr) * 8;
-      for (int d = 0; d < v.n_dir; ++d) {
-        char* out = static_cast<char*>(v.model) +
-                    (size_t(v.cb_host[cb].model_off) + d * plane) * vis_bytes;
-        const float* sc = v.scale + size_t(cb) * v.n_dir + d;
-        const dim3 grid((v.bl_stride + 255) / 256, n_chb, v.n_t);
-        const int c0 = c_off[d], c1 = c_off[d + 1];
-#define DDH_PREDICT_LAUNCH(NC, F16)                                                            \
-  hipLaunchKernelGGL((BaselinePredictKernel<NC, F16>), grid, dim3(256), 0, s, tab, dphase, drot, \
-                     dg, dspec, du, v.ant1, v.ant2, df, dw, v.n_bl, v.bl_stride, n_stations,   \
-                     n_components, c0, c1, ch0, n_chb, n_ch_total, stokes_i_only,              \
-                     correct_freq_smearing, time_smear, sc, out)
-        if (v.n_corr == 2) {
-          if (v.f16) DDH_PREDICT_LAUNCH(2, true); else DDH_PREDICT_LAUNCH(2, false);
-        } else {
-          if (v.f16) DDH_PREDICT_LAUNCH(4, true); else DDH_PREDICT_LAUNCH(4, false);
+                         n_components, n_ch_total, v.n_dir, v.n_cb, v.scale);
+    // where each channel's rows start in its block's model planes
+    std::vector<PredictChannel> chan(n_ch_total);
+    {
+      int ch = 0;
+      for (int cb = 0; cb < v.n_cb; ++cb) {
+        const int n_chb = v.ch_per_cb[cb];
+        for (int k = 0; k < n_chb; ++k, ++ch) {
+          PredictChannel& c = chan[ch];
+          c.base = v.cb_host[cb].model_off + int64_t(k) * v.bl_stride;
+          c.plane = int64_t(v.n_t) * n_chb * v.bl_stride;
+          c.t_stride = int64_t(n_chb) * v.bl_stride;
+          c.cb = cb;
+          c.pad = 0;
         }
-#undef DDH_PREDICT_LAUNCH
-        DDH_HIP_CHECK(hipGetLastError());
       }
-      ch0 += n_chb;
     }
+    static thread_local Scratch<PredictChannel> d_chan;
+    static thread_local Scratch<int32_t> d_coff;
+    PredictChannel* dch = d_chan.Get(chan.size());
+    DDH_HIP_CHECK(hipMemcpyAsync(dch, chan.data(), chan.size() * sizeof(PredictChannel),
+                                 hipMemcpyHostToDevice, s));
+    // c_off then g_off in one buffer
+    std::vector<int32_t> offs(c_off);
+    offs.insert(offs.end(), g_off.begin(), g_off.end());
+    int32_t* dco = d_coff.Get(offs.size());
+    DDH_HIP_CHECK(hipMemcpyAsync(dco, offs.data(), offs.size() * sizeof(int32_t),
+                                 hipMemcpyHostToDevice, s));
+    PredictArgs pa{};
+    pa.uvw = du;
+    pa.geom = dg;
+    pa.spec = dspec;
+    pa.c_off = dco;
+    pa.g_off = dco + c_off.size();
+    pa.freq = df;
+    pa.width = dw;
+    pa.ant1 = v.ant1;
+    pa.ant2 = v.ant2;
+    pa.chan = dch;
+    pa.scale = v.scale;
+    pa.model = v.model;
+    pa.n_st = n_stations;
+    pa.n_bl = v.n_bl;
+    pa.bl_stride = v.bl_stride;
+    pa.n_t = v.n_t;
+    pa.n_dir = v.n_dir;
+    pa.n_ch_total = n_ch_total;
+    pa.freq_smear = correct_freq_smearing ? 1 : 0;
+    pa.time_smear = scaled_ncp_uvw ? 1 : 0;
+    pa.ncp_v = scaled_ncp_uvw ? scaled_ncp_uvw[1] : 0.0;
+    pa.ncp_w = scaled_ncp_uvw ? scaled_ncp_uvw[2] : 0.0;
+    const bool smear = pa.freq_smear || pa.time_smear;
+    // LDS per component: its station terms and its Stokes values
+    const size_t per_comp = (size_t(PredictWords(smear)) * n_stations + 4) * sizeof(double);
+    DDH_REQUIRE(per_comp <= kPredictLds, -1, "too many stations for the predict's LDS");
+    pa.chunk = int(kPredictLds / per_comp);
+    const int per_wg = kPredictThreads * kPredictBaselines;
+    const dim3 grid(n_ch_total, v.n_t * v.n_dir, (v.bl_stride + per_wg - 1) / per_wg);
+    const size_t lds = size_t(pa.chunk) * per_comp;
+    LaunchPredict(v.n_corr == 2, v.f16, stokes_i_only != 0, smear, grid, lds, s, pa);
     DDH_HIP_CHECK(hipStreamSynchronize(s));
   });
 }
