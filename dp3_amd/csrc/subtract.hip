@@ -30,6 +30,9 @@ __device__ __forceinline__ C2 Mul(C2 a, C2 b) { return {a.re * b.re - a.im * b.i
 __device__ __forceinline__ C2 MulC(C2 a, C2 b) { return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im}; }
 __device__ __forceinline__ C2 Add(C2 a, C2 b) { return {a.re + b.re, a.im + b.im}; }
 
+constexpr int kDirBatch = 4;
+constexpr int kMaxSubtractBlocks = 1024;
+
 template <int NPOL>
 __global__ void __launch_bounds__(256)
     CorrectSubtractKernel(float4_v* __restrict__ data, float4_v* __restrict__ model,
@@ -38,10 +41,22 @@ __global__ void __launch_bounds__(256)
                           int n_t, int n_bl, int n_ch, int n_cb, int n_dir, int n_ant,
                           int subtract, int keep_model) {
   const int64_t n_vis = int64_t(n_t) * n_bl * n_ch;
+  // the block starts in LDS for the bisection below (the host checks the size)
+  __shared__ int32_t s_start[kMaxSubtractBlocks + 1];
+  for (int i = threadIdx.x; i <= n_cb; i += blockDim.x) s_start[i] = cb_start[i];
+  __syncthreads();
+  const bool small = n_vis < (int64_t(1) << 32);
   for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n_vis;
        v += int64_t(gridDim.x) * blockDim.x) {
-    const int ch = int(v % n_ch);
-    const int bl = int((v / n_ch) % n_bl);
+    int ch, bl;
+    if (small) {  // 32-bit division where the index fits
+      const uint32_t u = uint32_t(v), q = u / uint32_t(n_ch);
+      ch = int(u - q * uint32_t(n_ch));
+      bl = int(q % uint32_t(n_bl));
+    } else {
+      ch = int(v % n_ch);
+      bl = int((v / n_ch) % n_bl);
+    }
     const int a1 = ant1[bl], a2 = ant2[bl];
     // The block of this channel (DDECal.cc:1106-1108): the first cb with
     // cb_start[cb + 1] > ch, by bisection over the increasing starts
@@ -49,54 +64,76 @@ __global__ void __launch_bounds__(256)
     int lo = 0, hi = n_cb - 1;
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
-      if (cb_start[mid + 1] > ch)
+      if (s_start[mid + 1] > ch)
         hi = mid;
       else
         lo = mid + 1;
     }
     const int cb = lo;
     const double2* s = sol + size_t(cb) * n_ant * n_dir * NPOL;
+    // the data first (read before the models, so its latency overlaps theirs)
+    float4_v* dp = data + size_t(v) * 2;
+    float4_v d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};
+    if (subtract) {
+      d0 = __builtin_nontemporal_load(dp);
+      d1 = __builtin_nontemporal_load(dp + 1);
+    }
     C2 sum[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    for (int d = 0; d < n_dir; ++d) {
-      float4_v* mp = model + (size_t(d) * n_vis + size_t(v)) * 2;
-      const float4_v m0 = __builtin_nontemporal_load(mp), m1 = __builtin_nontemporal_load(mp + 1);
-      const C2 m[4] = {{m0.x, m0.y}, {m0.z, m0.w}, {m1.x, m1.y}, {m1.z, m1.w}};
-      const double2* p1 = s + (size_t(a1) * n_dir + d) * NPOL;
-      const double2* p2 = s + (size_t(a2) * n_dir + d) * NPOL;
-      C2 out[4];
-      if constexpr (NPOL == 1) {
-        const C2 f = MulC({p1[0].x, p1[0].y}, {p2[0].x, p2[0].y});
-        for (int c = 0; c < 4; ++c) out[c] = Mul(m[c], f);
-      } else if constexpr (NPOL == 2) {
-        const C2 s10 = {p1[0].x, p1[0].y}, s11 = {p1[1].x, p1[1].y};
-        const C2 s20 = {p2[0].x, p2[0].y}, s21 = {p2[1].x, p2[1].y};
-        out[0] = MulC(Mul(s10, m[0]), s20);
-        out[1] = MulC(Mul(s10, m[1]), s21);
-        out[2] = MulC(Mul(s11, m[2]), s20);
-        out[3] = MulC(Mul(s11, m[3]), s21);
-      } else {
-        const C2 a[4] = {{p1[0].x, p1[0].y}, {p1[1].x, p1[1].y}, {p1[2].x, p1[2].y},
-                         {p1[3].x, p1[3].y}};
-        const C2 b[4] = {{p2[0].x, p2[0].y}, {p2[1].x, p2[1].y}, {p2[2].x, p2[2].y},
-                         {p2[3].x, p2[3].y}};
-        // t = S1 M, out = t S2^H
-        const C2 t[4] = {Add(Mul(a[0], m[0]), Mul(a[1], m[2])), Add(Mul(a[0], m[1]), Mul(a[1], m[3])),
-                         Add(Mul(a[2], m[0]), Mul(a[3], m[2])), Add(Mul(a[2], m[1]), Mul(a[3], m[3]))};
-        out[0] = Add(MulC(t[0], b[0]), MulC(t[1], b[1]));
-        out[1] = Add(MulC(t[0], b[2]), MulC(t[1], b[3]));
-        out[2] = Add(MulC(t[2], b[0]), MulC(t[3], b[1]));
-        out[3] = Add(MulC(t[2], b[2]), MulC(t[3], b[3]));
+    // directions kDirBatch at a time: their model loads issued together, then
+    // the products added in direction order
+    for (int d0i = 0; d0i < n_dir; d0i += kDirBatch) {
+      float4_v mv[kDirBatch][2];
+#pragma unroll
+      for (int j = 0; j < kDirBatch; ++j) {
+        const int d = d0i + j < n_dir ? d0i + j : d0i;
+        const float4_v* mp = model + (size_t(d) * n_vis + size_t(v)) * 2;
+        mv[j][0] = __builtin_nontemporal_load(mp);
+        mv[j][1] = __builtin_nontemporal_load(mp + 1);
       }
 #pragma unroll
-      for (int c = 0; c < 4; ++c) sum[c] = Add(sum[c], out[c]);
-      if (keep_model) {
-        mp[0] = float4_v{float(out[0].re), float(out[0].im), float(out[1].re), float(out[1].im)};
-        mp[1] = float4_v{float(out[2].re), float(out[2].im), float(out[3].re), float(out[3].im)};
+      for (int j = 0; j < kDirBatch; ++j) {
+        const int d = d0i + j;
+        if (d >= n_dir) break;
+        float4_v* mp = model + (size_t(d) * n_vis + size_t(v)) * 2;
+        const float4_v m0 = mv[j][0], m1 = mv[j][1];
+        const C2 m[4] = {{m0.x, m0.y}, {m0.z, m0.w}, {m1.x, m1.y}, {m1.z, m1.w}};
+        const double2* p1 = s + (size_t(a1) * n_dir + d) * NPOL;
+        const double2* p2 = s + (size_t(a2) * n_dir + d) * NPOL;
+        C2 out[4];
+        if constexpr (NPOL == 1) {
+          const C2 f = MulC({p1[0].x, p1[0].y}, {p2[0].x, p2[0].y});
+          for (int c = 0; c < 4; ++c) out[c] = Mul(m[c], f);
+        } else if constexpr (NPOL == 2) {
+          const C2 s10 = {p1[0].x, p1[0].y}, s11 = {p1[1].x, p1[1].y};
+          const C2 s20 = {p2[0].x, p2[0].y}, s21 = {p2[1].x, p2[1].y};
+          out[0] = MulC(Mul(s10, m[0]), s20);
+          out[1] = MulC(Mul(s10, m[1]), s21);
+          out[2] = MulC(Mul(s11, m[2]), s20);
+          out[3] = MulC(Mul(s11, m[3]), s21);
+        } else {
+          const C2 a[4] = {{p1[0].x, p1[0].y}, {p1[1].x, p1[1].y}, {p1[2].x, p1[2].y},
+                           {p1[3].x, p1[3].y}};
+          const C2 b[4] = {{p2[0].x, p2[0].y}, {p2[1].x, p2[1].y}, {p2[2].x, p2[2].y},
+                           {p2[3].x, p2[3].y}};
+          // t = S1 M, out = t S2^H
+          const C2 t[4] = {Add(Mul(a[0], m[0]), Mul(a[1], m[2])),
+                           Add(Mul(a[0], m[1]), Mul(a[1], m[3])),
+                           Add(Mul(a[2], m[0]), Mul(a[3], m[2])),
+                           Add(Mul(a[2], m[1]), Mul(a[3], m[3]))};
+          out[0] = Add(MulC(t[0], b[0]), MulC(t[1], b[1]));
+          out[1] = Add(MulC(t[0], b[2]), MulC(t[1], b[3]));
+          out[2] = Add(MulC(t[2], b[0]), MulC(t[3], b[1]));
+          out[3] = Add(MulC(t[2], b[2]), MulC(t[3], b[3]));
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sum[c] = Add(sum[c], out[c]);
+        if (keep_model) {
+          mp[0] = float4_v{float(out[0].re), float(out[0].im), float(out[1].re), float(out[1].im)};
+          mp[1] = float4_v{float(out[2].re), float(out[2].im), float(out[3].re), float(out[3].im)};
+        }
       }
     }
     if (subtract) {
-      float4_v* dp = data + size_t(v) * 2;
-      float4_v d0 = dp[0], d1 = dp[1];
       // complex<float> -= complex<double> (DDECal.cc:1070): the difference is
       // taken in double and rounded once
       d0.x = float(double(d0.x) - sum[0].re);
@@ -131,6 +168,7 @@ void CheckShape(int32_t n_t, int32_t n_bl, int32_t n_ch, int32_t n_dir, int32_t 
               "empty dimension");
   DDH_REQUIRE(n_pol == 1 || n_pol == 2 || n_pol == 4, -1,
               "solution polarizations must be 1, 2 or 4");
+  DDH_REQUIRE(n_cb <= kMaxSubtractBlocks, -1, "at most 1024 channel blocks");
 }
 
 void Launch(int32_t n_t, int32_t n_bl, int32_t n_ch, int32_t n_cb, int32_t n_dir, int32_t n_ant,

@@ -392,7 +392,8 @@ int ddh_batch_autotune(ddh_batch* batch, int32_t* waves_per_tile, int32_t* rows_
 //         n_pol 1 (scalar), 2 (diagonal) or 4 (full Jones)
 //   channel_block_start [n_channel_blocks + 1], antenna1/2 [n_baselines];
 //         starts from 0 to n_channels, increasing (every block holds a
-//         channel, as DDECal's blocks do; the host form rejects others)
+//         channel, as DDECal's blocks do; the host form rejects others);
+//         at most 1024 channel blocks
 // The _device form takes device pointers for everything and enqueues on
 // `stream` (a hipStream_t, NULL for the default stream); the host form copies
 // host arrays in and out and reports the kernel's device time.
