@@ -26,8 +26,12 @@ converged chi^2 (a 1000-iteration solve at tolerance 1e-9 along the chain).
                   iteration (data and model planes read every iteration).
                   Solved one interval at a time, so its rate does not depend
                   on the number of streams; it runs on `--plain-streams`.
-  * optimized     batched (one launch per kernel per iteration over all
-                  streams), FP16 model planes (per block/direction scale),
+  * optimized     batched (one launch per kernel per iteration over the
+                  streams of a batch; the streams are split over `--batches`
+                  (2) batches driven concurrently, each by its own host
+                  thread on its own HIP stream, so one batch's latency-bound
+                  kernels overlap the other's bandwidth-bound ones), FP16
+                  model planes (per block/direction scale),
                   learned warm start from the previous interval's solutions
                   computed on the device (one undamped step + 2 x 4 damped
                   iterations -> features -> 3-layer 256-hidden bf16 MLP),
@@ -44,8 +48,9 @@ propagatesolutions) unbatched in complex64.
 `roofline` is for the dominant kernel of the optimized form, the fused Gram
 iteration kernel (algorithmic bytes: the per-baseline cross products it
 reads); its mean duration comes from HIP events on the batch stream around
-each of its launches in a fixed-length (50-iteration, accuracy 0) batched
-solve in which every interval is live in every launch. `roofline_other`
+each of its launches in a fixed-length (50-iteration, accuracy 0) solve of
+one batch's intervals, alone on the GPU, in which every interval is live in
+every launch. `roofline_other`
 holds the rest of that iteration (finishing kernel and decision), the
 unfused apply + antenna solve it replaces, and the batched FP16 correlate of
 the streaming form; `roofline_plain` the complex64 correlate of the plain
@@ -60,6 +65,7 @@ import json
 import os
 import platform
 import sys
+import threading
 import time
 
 import numpy as np
@@ -141,6 +147,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50, help="timed intervals per stream")
     ap.add_argument("--warmup", type=int, default=2, help="untimed intervals per stream")
     ap.add_argument("--streams", type=int, default=8, help="interval sequences batched together")
+    ap.add_argument("--batches", type=int, default=2,
+                    help="concurrent batches the streams are split over (one host thread and "
+                         "HIP stream each)")
     ap.add_argument("--plain-streams", type=int, default=2,
                     help="streams the unbatched plain forms are timed on")
     ap.add_argument("--train", type=int, default=150, help="training intervals per stream")
@@ -199,65 +208,101 @@ def main():
     t_setup = time.perf_counter() - t_setup
 
     # ---- optimized forms: batched + FP16 + learned warm start on the device,
-    # with the fused Gram and the streaming iteration
-    def run_batched(batch, use_net):
+    # with the fused Gram and the streaming iteration. The streams are split
+    # over `--batches` batches, each driven by its own host thread on its own
+    # HIP stream, so one batch's latency-bound kernels (finish, decision,
+    # warm-start steps) overlap another's bandwidth-bound ones.
+    K = max(1, min(args.batches, B))
+    split = [list(range(B))[k::K] for k in range(K)]
+
+    def run_batched(batches, use_net):
         prev = [p.copy() for p in prevs]
-        t_all, warm_ms, its, sols = 0.0, 0.0, [], {}
-        for i in range(first, n_total):
-            t0 = time.perf_counter()
-            batch.rebind([q.problems[i] for q in s16])
-            if use_net:
-                res = batch.solve_warm(ws.net, prev, stretch=ws.stretch)
-            else:
-                res = batch.solve(prev)
-            dt = time.perf_counter() - t0
-            if i in timed:
-                t_all += dt
-                warm_ms += batch.last_warm_ms() if use_net else 0.0
-                for b in range(B):
-                    its.append(res[b].iterations + (1 + 2 * ws.stretch if use_net else 0))
-                    sols[(b, i)] = prev[b].copy()
-        return {"value": B * T / t_all, "seconds": t_all, "warm_ms": warm_ms, "iterations": its,
-                "solutions": sols}
+        its, sols, warm_ms = {}, {}, [0.0] * K
+        start, ends, errors = [0.0], [0.0] * K, []
+        barrier = threading.Barrier(K, action=lambda: start.__setitem__(0, time.perf_counter()))
+
+        def worker(k):
+            try:
+                b, idx = batches[k], split[k]
+                for i in range(first, n_total):
+                    if i == timed[0]:
+                        barrier.wait()  # the timed region starts when every batch is ready
+                    b.rebind([s16[j].problems[i] for j in idx])
+                    mine = [prev[j] for j in idx]  # updated in place
+                    if use_net:
+                        res = b.solve_warm(ws.net, mine, stretch=ws.stretch)
+                    else:
+                        res = b.solve(mine)
+                    if i in timed:
+                        warm_ms[k] += b.last_warm_ms() if use_net else 0.0
+                        for m, j in enumerate(idx):
+                            its[(j, i)] = res[m].iterations + (1 + 2 * ws.stretch if use_net else 0)
+                            sols[(j, i)] = prev[j].copy()
+                ends[k] = time.perf_counter()
+            except BaseException as e:  # re-raised by the main thread
+                errors.append(e)
+                barrier.abort()
+
+        threads = [threading.Thread(target=worker, args=(k,)) for k in range(K)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+        t_all = max(ends) - start[0]
+        return {"value": B * T / t_all, "seconds": t_all, "warm_ms": sum(warm_ms) / K,
+                "iterations": [its[key] for key in sorted(its)], "solutions": sols}
 
     def kernel_times(iteration, tuned, fused=True, solve_schedule=(0, 0)):
         """Mean of the first and second timed part of a batched iteration over
-        a fixed-length profiled solve in which every interval is live in every
-        launch: (fused kernel, finishing kernel + decision) for the fused Gram
-        iteration, (apply or correlate, antenna solve + decision) otherwise."""
-        solvers = [WS.sibling(q.solver, ROOF_ITERATIONS, 0.0, False, iteration=iteration)
-                   for q in s16]
+        a fixed-length profiled solve of one batch's intervals in which every
+        interval is live in every launch: (fused kernel, finishing kernel +
+        decision) for the fused Gram iteration, (apply or correlate, antenna
+        solve + decision) otherwise."""
+        idx = split[0]
+        solvers = [WS.sibling(s16[j].solver, ROOF_ITERATIONS, 0.0, False, iteration=iteration)
+                   for j in idx]
         for s in solvers:
             s.set_gram_fusion(fused)
-        rb = H.Batch(solvers, [q.problems[n_total - 1] for q in s16])
+        rb = H.Batch(solvers, [s16[j].problems[n_total - 1] for j in idx])
         rb.set_schedule(tuned[0], tuned[1], 0, 8)
         rb.set_antenna_solve_schedule(*solve_schedule)
         if fused and iteration == H.ITERATION_GRAM:
             rb.set_gram_solve_groups(tuned[2])
         rb.set_profiling(True)
-        rb.solve([q.unit() for q in s16])
+        rb.solve([s16[j].unit() for j in idx])
         corr_ms, solve_ms, n_it = rb.last_timing()
         rb.close()
         return corr_ms / max(n_it, 1), solve_ms / max(n_it, 1)
 
     opt = {}
     for name, iteration in (("gram", H.ITERATION_GRAM), ("streaming", H.ITERATION_STREAM)):
-        solvers = [WS.sibling(q.solver, MAX_ITERATIONS, TOLERANCE, iteration=iteration)
-                   for q in s16]
-        batch = H.Batch(solvers, [q.problems[first] for q in s16])
-        tuned = batch.autotune()
-        groups = batch.gram_solve_groups()
-        batch.set_schedule(tuned[0], tuned[1], GRAPH, 8)
-        r = run_batched(batch, True)
-        r["without_network"] = run_batched(batch, False)
+        batches = []
+        for k in range(K):
+            solvers = [WS.sibling(s16[j].solver, MAX_ITERATIONS, TOLERANCE, iteration=iteration)
+                       for j in split[k]]
+            batches.append(H.Batch(solvers, [s16[j].problems[first] for j in split[k]]))
+        # one schedule for all batches (they have the same shapes): tuned on the first
+        tuned = batches[0].autotune()
+        groups = batches[0].gram_solve_groups()
+        solve_schedule = batches[0].antenna_solve_schedule()
+        for b in batches:
+            b.set_schedule(tuned[0], tuned[1], GRAPH, 8)
+            b.set_antenna_solve_schedule(*solve_schedule)
+            if iteration == H.ITERATION_GRAM:
+                b.set_gram_solve_groups(groups)
+        r = run_batched(batches, True)
+        r["without_network"] = run_batched(batches, False)
         r["schedule"] = (tuned[0], tuned[1], groups)
-        r["solve_schedule"] = batch.antenna_solve_schedule()
+        r["solve_schedule"] = solve_schedule
         r["kernels_ms"] = kernel_times(iteration, r["schedule"], solve_schedule=r["solve_schedule"])
         opt[name] = r
         log(f"optimized ({name} iteration): {r['value']:.2f} solves/s, without network "
             f"{r['without_network']['value']:.2f}; per batched iteration: "
             f"{r['kernels_ms'][0] * 1e3:.1f} us + {r['kernels_ms'][1] * 1e3:.1f} us")
-        batch.close()
+        for b in batches:
+            b.close()
     unfused = kernel_times(H.ITERATION_GRAM, opt["gram"]["schedule"], fused=False,
                            solve_schedule=opt["streaming"]["solve_schedule"])
     log(f"unfused Gram iteration: apply {unfused[0] * 1e3:.1f} us, antenna solve + decision "
@@ -355,27 +400,28 @@ def main():
 
     g_solve, g_rest = opt["gram"]["kernels_ms"]
     s_corr, s_solve = opt["streaming"]["kernels_ms"]
+    NB = len(split[0])  # intervals per batched launch
     rooflines = {
         "gram_solve": roof(
-            "gram_solve", B * walk_bytes, g_solve,
+            "gram_solve", NB * walk_bytes, g_solve,
             note=("fused Gram iteration kernel over the batch: the per-baseline cross products "
                   "in walk order read once (C_d, G_e,d for e < d, and the real diagonals G_d,d "
                   f"two per slot: {nq_walk} x 8 bytes per baseline and correlation); per "
                   f"antenna FP64 sums per row group written ({opt['gram']['schedule'][2]} row "
                   "groups per polarization)")),
         "gram_finish_decide": roof(
-            "gram_finish_decide", B * 2 * solution_bytes, g_rest,
+            "gram_finish_decide", NB * 2 * solution_bytes, g_rest,
             note="finishing kernel (row-group sums added, FP64 divide, step, convergence "
                  "terms) and the decision kernel; solutions read and written; latency-bound"),
         "unfused_gram_apply": roof(
-            "gram_apply", B * (cross_bytes * bl_stride // w.n_baselines + apply_partial_bytes),
+            "gram_apply", NB * (cross_bytes * bl_stride // w.n_baselines + apply_partial_bytes),
             unfused[0], note="the apply kernel the fused iteration replaces: cross products "
                              "read, correlate partials U written"),
         "unfused_antenna_solve": roof(
-            "antenna_solve", B * (partial_bytes + 2 * solution_bytes), unfused[1],
+            "antenna_solve", NB * (partial_bytes + 2 * solution_bytes), unfused[1],
             note="the antenna solve + decision the fused iteration replaces"),
         "batched_correlate_f16": roof("batched_correlate_f16",
-                                      B * n_vis * (16 + w.n_directions * 8), s_corr),
+                                      NB * n_vis * (16 + w.n_directions * 8), s_corr),
     }
 
     # ---- plain form's complex64 correlate, fixed length as above
@@ -608,18 +654,21 @@ def main():
             "workload": (f"{w.name}: 62 stations/1891 baselines, 8 directions x 20 "
                          "point+Gaussian components, 240 channels in 24 blocks, solint 10, "
                          f"step 0.2, tolerance 1e-6, iteration limit {MAX_ITERATIONS}"),
-            "form": (f"optimized: {B} interval streams batched, FP16 model planes with FP32 "
-                     "accumulation and FP64 convergence test, learned warm start (3-layer "
-                     "256-hidden bf16 MLP) on the device, fused Gram iteration, runs to "
+            "form": (f"optimized: {B} interval streams in {K} concurrent batches of "
+                     f"{len(split[0])} (one host thread and HIP stream each), FP16 model planes "
+                     "with FP32 accumulation and FP64 convergence test, learned warm start "
+                     "(3-layer 256-hidden bf16 MLP) on the device, fused Gram iteration, runs to "
                      "tolerance"),
-            "step": f"one interval of each of {B} streams (a batch of {B} solves)",
+            "step": f"one interval of each of {B} streams ({B} solves)",
             "streams": B,
+            "batches": K,
+            "streams_per_batch": len(split[0]),
             "plain_streams": PB,
             "train_intervals_per_stream": TR,
             "evaluated_intervals_per_stream": T,
             "train_samples": int(len(x)),
             "warm_start_stretch": ws.stretch,
-            "warm_start_device_ms_per_step": round(warm_ms / T, 3),
+            "warm_start_device_ms_per_batch_step": round(warm_ms / T, 3),
             "optimized": summary(excess["optimized"], best["iterations"]),
             "speedup_vs_plain": round(value / v_plain, 3),
             "plain_solves_per_s": round(v_plain, 3),

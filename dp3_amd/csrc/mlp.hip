@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <utility>
 #include <vector>
@@ -428,6 +429,10 @@ struct ddh_mlp {
   ddh::Buf<uint16_t> packed;
   int64_t weights_version = 0, packed_version = -1;
   bool fused_forward = true;
+  // serializes the packing of the weights (and the layered forward pass,
+  // which uses the activation buffers above) between batches that share the
+  // network from different host threads
+  std::mutex mu;
 
   int64_t n_params() const { return off[6]; }
 };
@@ -576,15 +581,38 @@ void CheckWarmNetwork(const ddh_mlp& net) {
 
 void WaitForNetwork(ddh_mlp& net) { DDH_HIP_CHECK(hipStreamSynchronize(net.stream)); }
 
-float* MlpInputBuffer(ddh_mlp& net, int64_t n) {
+void MlpForwardRows(ddh_mlp& net, const float* x, int64_t n, float* y, hipStream_t stream) {
+  if (UsesFusedForward(net)) {
+    {
+      // the fragment-ordered weights, packed once per weight update on the
+      // network's stream
+      std::lock_guard<std::mutex> lock(net.mu);
+      if (net.packed_version != net.weights_version) {
+        PackWeights(net, net.stream);
+        DDH_HIP_CHECK(hipStreamSynchronize(net.stream));
+      }
+    }
+    int64_t o[4];
+    PackedOffsets(net, o);
+    const bf16x8* p = reinterpret_cast<const bf16x8*>(net.packed.ptr);
+    const int64_t blocks = (n + kForwardRows - 1) / kForwardRows;
+    DDH_REQUIRE(blocks <= INT32_MAX, -1, "too many rows for one forward pass");
+    hipLaunchKernelGGL(kForwardKernels[net.n_hidden / 32 - 1], dim3(unsigned(blocks)), dim3(64),
+                       0, stream, x, n, p + o[0] / 8, net.w.ptr + net.off[1], p + o[1] / 8,
+                       net.w.ptr + net.off[3], p + o[2] / 8, net.w.ptr + net.off[5], y);
+    DDH_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  // layered forward through the network's own activation buffers: one
+  // caller at a time
+  std::lock_guard<std::mutex> lock(net.mu);
   Reserve(net, n);
-  return net.x.ptr;
-}
-
-const float* MlpForwardDevice(ddh_mlp& net, int64_t n, hipStream_t stream) {
-  Infer(net, n, stream);
-  DDH_HIP_CHECK(hipGetLastError());
-  return net.y.ptr;
+  DDH_HIP_CHECK(hipMemcpyAsync(net.x.ptr, x, size_t(n) * net.n_in * sizeof(float),
+                               hipMemcpyDeviceToDevice, stream));
+  Forward(net, n, stream);
+  DDH_HIP_CHECK(hipMemcpyAsync(y, net.y.ptr, size_t(n) * net.n_out * sizeof(float),
+                               hipMemcpyDeviceToDevice, stream));
+  DDH_HIP_CHECK(hipStreamSynchronize(stream));
 }
 
 dim3 WarmGrid(int64_t max_pairs, int n_spans) {

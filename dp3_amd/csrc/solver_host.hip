@@ -1495,6 +1495,7 @@ struct ddh_batch {
   ddh::DeviceBuffer<ddh::IterArgs> d_args_step, d_args_aux;
   ddh::DeviceBuffer<double2> warm_g0, warm_next, warm_a, warm_b;
   ddh::DeviceBuffer<double> warm_den;
+  ddh::DeviceBuffer<float> warm_x, warm_y;  // the network's input and output rows
   // every interval's place in the warm buffers (as last uploaded)
   std::vector<int64_t> warm_off;
   std::vector<ddh::WarmSpan> warm_spans;
@@ -1837,10 +1838,13 @@ void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* cons
   run_aux(b.warm_b.ptr);
   // features of all intervals as consecutive rows, one forward pass
   const int64_t rows = int64_t(total / 2);
-  float* x = MlpInputBuffer(net, rows);
+  b.warm_x.Resize(size_t(rows) * kWarmFeatures);
+  b.warm_y.Resize(size_t(rows) * kWarmOutputs);
+  float* x = b.warm_x.ptr;
   LaunchWarmFeatures(b.warm_g0.ptr, b.warm_next.ptr, b.warm_a.ptr, b.warm_b.ptr, b.warm_den.ptr,
                      b.d_warm_spans.ptr, n, max_count / 2, x, b.stream);
-  const float* pred = MlpForwardDevice(net, rows, b.stream);
+  MlpForwardRows(net, x, rows, b.warm_y.ptr, b.stream);
+  const float* pred = b.warm_y.ptr;
   // the starts, through warm_a (no longer needed), into every sol0
   LaunchWarmApply(b.warm_g0.ptr, pred, b.d_warm_spans.ptr, n, max_count / 2, b.warm_a.ptr,
                   b.stream);

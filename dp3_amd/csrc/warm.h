@@ -18,12 +18,11 @@ constexpr int kWarmOutputs = 4;
 void CheckWarmNetwork(const ddh_mlp& net);
 // Waits for work queued on the network's own stream (training).
 void WaitForNetwork(ddh_mlp& net);
-// Device input rows [n][n_in] of the next MlpForwardDevice; valid until the
-// network is used again.
-float* MlpInputBuffer(ddh_mlp& net, int64_t n);
-// Forward pass over the n rows of the input buffer on `stream`; returns the
-// device outputs [n][n_out].
-const float* MlpForwardDevice(ddh_mlp& net, int64_t n, hipStream_t stream);
+// Forward pass over the device rows x [n][n_in] into y [n][n_out] on
+// `stream`. Batches may share a network from different host threads (the
+// weights are read only; their packing and the layered fallback are
+// serialized), as long as no training runs meanwhile.
+void MlpForwardRows(ddh_mlp& net, const float* x, int64_t n, float* y, hipStream_t stream);
 // One interval's solutions in the warm start's concatenated buffers: its
 // first solution pair (two complex values per pair) and its shape
 // (n_cb * n_ant * n_sub pairs).
