@@ -28,7 +28,7 @@ converged chi^2 (a 1000-iteration solve at tolerance 1e-9 along the chain).
                   on the number of streams; it runs on `--plain-streams`.
   * optimized     batched (one launch per kernel per iteration over the
                   streams of a batch; the streams are split over `--batches`
-                  (2) batches driven concurrently, each by its own host
+                  (4) batches driven concurrently, each by its own host
                   thread on its own HIP stream, so one batch's latency-bound
                   kernels overlap the other's bandwidth-bound ones), FP16
                   model planes (per block/direction scale),
@@ -49,8 +49,9 @@ propagatesolutions) unbatched in complex64.
 iteration kernel (algorithmic bytes: the per-baseline cross products it
 reads); its mean duration comes from HIP events on the batch stream around
 each of its launches in a fixed-length (50-iteration, accuracy 0) solve of
-one batch's intervals, alone on the GPU, in which every interval is live in
-every launch. `roofline_other`
+every stream's interval in one batch, alone on the GPU and autotuned for
+that size, in which every interval is live in every launch (the timed forms
+run the same kernels over each concurrent batch's intervals). `roofline_other`
 holds the rest of that iteration (finishing kernel and decision), the
 unfused apply + antenna solve it replaces, and the batched FP16 correlate of
 the streaming form; `roofline_plain` the complex64 correlate of the plain
@@ -147,7 +148,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50, help="timed intervals per stream")
     ap.add_argument("--warmup", type=int, default=2, help="untimed intervals per stream")
     ap.add_argument("--streams", type=int, default=8, help="interval sequences batched together")
-    ap.add_argument("--batches", type=int, default=2,
+    ap.add_argument("--batches", type=int, default=4,
                     help="concurrent batches the streams are split over (one host thread and "
                          "HIP stream each)")
     ap.add_argument("--plain-streams", type=int, default=2,
@@ -254,27 +255,27 @@ def main():
         return {"value": B * T / t_all, "seconds": t_all, "warm_ms": sum(warm_ms) / K,
                 "iterations": [its[key] for key in sorted(its)], "solutions": sols}
 
-    def kernel_times(iteration, tuned, fused=True, solve_schedule=(0, 0)):
+    def kernel_times(iteration, fused=True):
         """Mean of the first and second timed part of a batched iteration over
-        a fixed-length profiled solve of one batch's intervals in which every
-        interval is live in every launch: (fused kernel, finishing kernel +
-        decision) for the fused Gram iteration, (apply or correlate, antenna
-        solve + decision) otherwise."""
-        idx = split[0]
-        solvers = [WS.sibling(s16[j].solver, ROOF_ITERATIONS, 0.0, False, iteration=iteration)
-                   for j in idx]
+        a fixed-length profiled solve of all streams' intervals in one batch,
+        alone on the GPU and autotuned for that size, in which every interval
+        is live in every launch: (fused kernel, finishing kernel + decision)
+        for the fused Gram iteration, (apply or correlate, antenna solve +
+        decision) otherwise. The timed forms run the same kernels over the
+        intervals of each of their concurrent batches."""
+        solvers = [WS.sibling(q.solver, ROOF_ITERATIONS, 0.0, False, iteration=iteration)
+                   for q in s16]
         for s in solvers:
             s.set_gram_fusion(fused)
-        rb = H.Batch(solvers, [s16[j].problems[n_total - 1] for j in idx])
-        rb.set_schedule(tuned[0], tuned[1], 0, 8)
-        rb.set_antenna_solve_schedule(*solve_schedule)
-        if fused and iteration == H.ITERATION_GRAM:
-            rb.set_gram_solve_groups(tuned[2])
+        rb = H.Batch(solvers, [q.problems[n_total - 1] for q in s16])
+        tuned_all = rb.autotune()
+        rb.set_schedule(tuned_all[0], tuned_all[1], 0, 8)
         rb.set_profiling(True)
-        rb.solve([s16[j].unit() for j in idx])
+        rb.solve([q.unit() for q in s16])
         corr_ms, solve_ms, n_it = rb.last_timing()
+        groups_all = rb.gram_solve_groups()
         rb.close()
-        return corr_ms / max(n_it, 1), solve_ms / max(n_it, 1)
+        return corr_ms / max(n_it, 1), solve_ms / max(n_it, 1), groups_all
 
     opt = {}
     for name, iteration in (("gram", H.ITERATION_GRAM), ("streaming", H.ITERATION_STREAM)):
@@ -296,15 +297,14 @@ def main():
         r["without_network"] = run_batched(batches, False)
         r["schedule"] = (tuned[0], tuned[1], groups)
         r["solve_schedule"] = solve_schedule
-        r["kernels_ms"] = kernel_times(iteration, r["schedule"], solve_schedule=r["solve_schedule"])
+        r["kernels_ms"] = kernel_times(iteration)
         opt[name] = r
         log(f"optimized ({name} iteration): {r['value']:.2f} solves/s, without network "
             f"{r['without_network']['value']:.2f}; per batched iteration: "
             f"{r['kernels_ms'][0] * 1e3:.1f} us + {r['kernels_ms'][1] * 1e3:.1f} us")
         for b in batches:
             b.close()
-    unfused = kernel_times(H.ITERATION_GRAM, opt["gram"]["schedule"], fused=False,
-                           solve_schedule=opt["streaming"]["solve_schedule"])
+    unfused = kernel_times(H.ITERATION_GRAM, fused=False)
     log(f"unfused Gram iteration: apply {unfused[0] * 1e3:.1f} us, antenna solve + decision "
         f"{unfused[1] * 1e3:.1f} us")
     best = opt["gram"]
@@ -398,17 +398,18 @@ def main():
         out.update(extra)
         return out
 
-    g_solve, g_rest = opt["gram"]["kernels_ms"]
-    s_corr, s_solve = opt["streaming"]["kernels_ms"]
-    NB = len(split[0])  # intervals per batched launch
+    g_solve, g_rest, roof_groups = opt["gram"]["kernels_ms"]
+    s_corr, s_solve, _ = opt["streaming"]["kernels_ms"]
+    NB = B  # the roofline solves batch every stream's interval
     rooflines = {
         "gram_solve": roof(
             "gram_solve", NB * walk_bytes, g_solve,
             note=("fused Gram iteration kernel over the batch: the per-baseline cross products "
                   "in walk order read once (C_d, G_e,d for e < d, and the real diagonals G_d,d "
                   f"two per slot: {nq_walk} x 8 bytes per baseline and correlation); per "
-                  f"antenna FP64 sums per row group written ({opt['gram']['schedule'][2]} row "
-                  "groups per polarization)")),
+                  f"antenna FP64 sums per row group written ({roof_groups} row groups per "
+                  f"polarization); one launch over all {B} streams' intervals, alone on the "
+                  "GPU")),
         "gram_finish_decide": roof(
             "gram_finish_decide", NB * 2 * solution_bytes, g_rest,
             note="finishing kernel (row-group sums added, FP64 divide, step, convergence "
@@ -689,6 +690,8 @@ def main():
             "iterations_note": ("iterations of the forms with the learned warm start include "
                                 f"its {1 + 2 * ws.stretch} iterations (one undamped update and "
                                 "two stretches)"),
+            "per_batched_iteration_note": (f"one batch of all {B} streams' intervals alone on "
+                                           "the GPU, 50 fixed iterations, HIP events"),
             "per_batched_iteration_us": {
                 "gram_solve": round(g_solve * 1e3, 1),
                 "gram_finish_decide": round(g_rest * 1e3, 1),

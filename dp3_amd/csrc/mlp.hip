@@ -45,48 +45,103 @@ __host__ __device__ inline float FromBf16(uint16_t b) {
 // (ReLU) for bf16 operands with arbitrary strides:
 //   opA(A)(m, k) = trans_a ? A[k * lda + m] : A[m * lda + k]
 //   opB(B)(k, n) = trans_b ? B[n * ldb + k] : B[k * ldb + n]
-// One wave per 32 x 32 tile of C; K advances 16 per v_mfma_f32_32x32x16_bf16.
-// Operand layout of that instruction (wave64): lane l supplies row (A) or
-// column (B) l % 32 and the 8 k values 8 * (l / 32) .. + 7; the 16 results of
-// lane l are C(8 * (i / 4) + 4 * (l / 32) + i % 4, l % 32), i = 0..15.
+// One 256-thread workgroup per 64 x 64 tile of C, one wave per 32 x 32
+// quarter; K in chunks of kGemmK staged in LDS as [row][k] (k contiguous) for
+// both operands, read by each lane as one 16-byte fragment per
+// v_mfma_f32_32x32x16_bf16 step. Operand rows that are 16-byte aligned and
+// whole are loaded as 16-byte vectors; the rest element by element, zero
+// outside the matrices. Operand layout of the instruction (wave64): lane l
+// supplies row (A) or column (B) l % 32 and the 8 k values 8 * (l / 32) .. + 7;
+// the 16 results of lane l are C(8 * (i / 4) + 4 * (l / 32) + i % 4, l % 32).
+// K ascends 16 per step, so an output's sum is the same sequence of MFMA steps
+// as the one-kernel forward pass makes. With `partial` set, the workgroups of
+// grid.y split K into chunks of k_len and store raw sums to
+// partial[blockIdx.y][M][N], which GemmReduceKernel adds in split order.
+constexpr int kGemmTile = 64;
+constexpr int kGemmK = 32;
+constexpr int kGemmRow = kGemmK + 8;  // LDS row in bf16 (80 bytes: 16-byte aligned rows)
+
+// The 8 bf16 at columns c .. c + 7 (c a multiple of 8) of row r of a matrix
+// with `rows` rows, `cols` columns and leading dimension ld; zero outside.
+__device__ __forceinline__ void LoadRow8(const uint16_t* __restrict__ base, int ld, int r,
+                                         int rows, int c, int cols, uint16_t (&v)[8]) {
+  if (r < rows && c + 7 < cols && (ld & 7) == 0 &&
+      (reinterpret_cast<uintptr_t>(base) & 15) == 0) {  // c is a multiple of 8
+    const uint4 q = *reinterpret_cast<const uint4*>(base + size_t(r) * ld + c);
+    v[0] = q.x & 0xffff; v[1] = q.x >> 16; v[2] = q.y & 0xffff; v[3] = q.y >> 16;
+    v[4] = q.z & 0xffff; v[5] = q.z >> 16; v[6] = q.w & 0xffff; v[7] = q.w >> 16;
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = (r < rows && c + j < cols) ? base[size_t(r) * ld + c + j] : 0;
+}
+
+// One operand's 64 x kGemmK chunk into LDS rows [64][kGemmRow] (row = m for A,
+// n for B). `kmajor`: the matrix is stored with k along its rows' contiguous
+// axis ([row][k], ld between rows); otherwise [k][row] and the chunk is
+// transposed on the way into LDS.
+__device__ __forceinline__ void StageChunk(uint16_t* __restrict__ s, const uint16_t* __restrict__ g,
+                                           int ld, bool kmajor, int r0, int rows, int k0,
+                                           int K) {
+  const int t = threadIdx.x;
+  uint16_t v[8];
+  if (kmajor) {
+    const int r = t >> 2, kk = (t & 3) * 8;  // 64 rows x 4 groups of 8 k
+    LoadRow8(g, ld, r0 + r, rows, k0 + kk, K, v);
+    uint4 q;
+    q.x = v[0] | (uint32_t(v[1]) << 16); q.y = v[2] | (uint32_t(v[3]) << 16);
+    q.z = v[4] | (uint32_t(v[5]) << 16); q.w = v[6] | (uint32_t(v[7]) << 16);
+    *reinterpret_cast<uint4*>(s + r * kGemmRow + kk) = q;
+  } else {
+    const int k = t >> 3, rr = (t & 7) * 8;  // 32 k x 8 groups of 8 rows
+    LoadRow8(g, ld, k0 + k, K, r0 + rr, rows, v);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[(rr + j) * kGemmRow + k] = v[j];
+  }
+}
+
 __global__ void __launch_bounds__(256) GemmBf16Kernel(const uint16_t* __restrict__ A, int lda,
                                                       int trans_a,
                                                       const uint16_t* __restrict__ B, int ldb,
                                                       int trans_b, float* __restrict__ C,
                                                       int ldc, int M, int N, int K,
                                                       const float* __restrict__ bias, int relu,
-                                                      int accumulate) {
+                                                      int accumulate, int k_len,
+                                                      float* __restrict__ partial) {
+  __shared__ __align__(16) uint16_t sA[kGemmTile * kGemmRow];
+  __shared__ __align__(16) uint16_t sB[kGemmTile * kGemmRow];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int tiles_n = (N + 31) / 32;
-  const int tile = blockIdx.x * 4 + wave;
-  const int tm = tile / tiles_n;
-  const int tn = tile - tm * tiles_n;
-  if (tm * 32 >= M) return;
-  const int m = tm * 32 + (lane & 31);
-  const int n = tn * 32 + (lane & 31);
-  const int kg = (lane >> 5) * 8;
+  const int tiles_n = (N + kGemmTile - 1) / kGemmTile;
+  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x - tm * tiles_n;
+  const int m0 = tm * kGemmTile, n0 = tn * kGemmTile;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int k_begin = blockIdx.y * k_len, k_end = min(K, k_begin + k_len);
   float16_v acc;
 #pragma unroll
   for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    bf16x8 a, b;
+  for (int k0 = k_begin; k0 < k_end; k0 += kGemmK) {
+    __syncthreads();  // the previous chunk's fragments are read
+    StageChunk(sA, A, lda, !trans_a, m0, M, k0, k_end);
+    StageChunk(sB, B, ldb, trans_b != 0, n0, N, k0, k_end);
+    __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int k = k0 + kg + j;
-      uint16_t av = 0, bv = 0;
-      if (m < M && k < K) av = trans_a ? A[size_t(k) * lda + m] : A[size_t(m) * lda + k];
-      if (n < N && k < K) bv = trans_b ? B[size_t(n) * ldb + k] : B[size_t(k) * ldb + n];
-      a[j] = __builtin_bit_cast(__bf16, av);
-      b[j] = __builtin_bit_cast(__bf16, bv);
+    for (int ks = 0; ks < kGemmK / 16; ++ks) {
+      const int kk = ks * 16 + (lane >> 5) * 8;
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(sA + (wm * 32 + (lane & 31)) * kGemmRow + kk);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(sB + (wn * 32 + (lane & 31)) * kGemmRow + kk);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
   }
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
-    const int row = tm * 32 + 8 * (i / 4) + 4 * (lane >> 5) + (i % 4);
-    const int col = tn * 32 + (lane & 31);
+    const int row = m0 + wm * 32 + 8 * (i / 4) + 4 * (lane >> 5) + (i % 4);
+    const int col = n0 + wn * 32 + (lane & 31);
     if (row < M && col < N) {
+      if (partial) {
+        partial[(size_t(blockIdx.y) * M + row) * N + col] = acc[i];
+        continue;
+      }
       float v = acc[i];
       if (bias) v += bias[col];
       if (relu) v = fmaxf(v, 0.0f);
@@ -94,6 +149,22 @@ __global__ void __launch_bounds__(256) GemmBf16Kernel(const uint16_t* __restrict
       *out = accumulate ? *out + v : v;
     }
   }
+}
+
+// C = sum over the splits (in split order) of the partial sums, then the
+// epilogue of GemmBf16Kernel.
+__global__ void GemmReduceKernel(const float* __restrict__ partial, int splits, int M, int N,
+                                 float* __restrict__ C, int ldc, const float* __restrict__ bias,
+                                 int relu, int accumulate) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= int64_t(M) * N) return;
+  const int row = int(i / N), col = int(i - int64_t(row) * N);
+  float v = partial[i];
+  for (int s = 1; s < splits; ++s) v += partial[size_t(s) * M * N + i];
+  if (bias) v += bias[col];
+  if (relu) v = fmaxf(v, 0.0f);
+  float* out = C + size_t(row) * ldc + col;
+  *out = accumulate ? *out + v : v;
 }
 
 // W [K x N] (bf16, row-major, leading dimension N) in the B-operand fragment
@@ -394,12 +465,33 @@ struct Buf {
 
 inline unsigned Blocks(int64_t n, int t = 256) { return unsigned((n + t - 1) / t); }
 
+// Split-K partial sums of GEMMs issued from this host thread (one stream at a
+// time per thread).
+Buf<float>& GemmWorkspace() {
+  static thread_local Buf<float> ws;
+  return ws;
+}
+
 void Gemm(const uint16_t* A, int lda, bool ta, const uint16_t* B, int ldb, bool tb, float* C,
           int ldc, int M, int N, int K, const float* bias, bool relu, bool accumulate,
           hipStream_t s) {
-  const int tiles = ((M + 31) / 32) * ((N + 31) / 32);
-  hipLaunchKernelGGL(GemmBf16Kernel, dim3((tiles + 3) / 4), dim3(256), 0, s, A, lda, int(ta), B,
-                     ldb, int(tb), C, ldc, M, N, K, bias, int(relu), int(accumulate));
+  const int tiles = ((M + kGemmTile - 1) / kGemmTile) * ((N + kGemmTile - 1) / kGemmTile);
+  // Long sums over few tiles (the weight gradients, K = rows of the batch)
+  // are split over workgroups: about 256 workgroups in all, chunks of at
+  // least 512 k.
+  int splits = 1;
+  if (K >= 1024 && tiles < 128)
+    splits = std::max(1, std::min({(256 + tiles - 1) / tiles, K / 512, 64}));
+  const int k_len = ((K + splits - 1) / splits + kGemmK - 1) / kGemmK * kGemmK;
+  splits = (K + k_len - 1) / k_len;
+  float* partial = nullptr;
+  if (splits > 1) partial = GemmWorkspace().Get(size_t(splits) * M * N);
+  hipLaunchKernelGGL(GemmBf16Kernel, dim3(tiles, splits), dim3(256), 0, s, A, lda, int(ta), B,
+                     ldb, int(tb), C, ldc, M, N, K, bias, int(relu), int(accumulate), k_len,
+                     partial);
+  if (splits > 1)
+    hipLaunchKernelGGL(GemmReduceKernel, dim3(Blocks(int64_t(M) * N)), dim3(256), 0, s, partial,
+                       splits, M, N, C, ldc, bias, int(relu), int(accumulate));
   DDH_HIP_CHECK(hipGetLastError());
 }
 

@@ -17,7 +17,9 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("M,N,K,ta,tb", [(64, 64, 16, 0, 0), (100, 37, 53, 0, 0),
                                          (33, 256, 256, 1, 0), (256, 4, 300, 0, 1),
-                                         (5, 7, 3, 1, 1)])
+                                         (5, 7, 3, 1, 1), (256, 256, 4096, 1, 0),
+                                         (16, 256, 2000, 1, 0), (256, 4, 1100, 1, 0),
+                                         (4096, 256, 256, 0, 1)])
 def test_gemm_bf16_matches_numpy(M, N, K, ta, tb):
     print(f"case: gemm {M}x{N}x{K} ta={ta} tb={tb}", flush=True)
     rng = np.random.default_rng(M * 1000 + N)
@@ -183,3 +185,51 @@ def test_batched_warm_start_matches_host_sequence():
     b2.solve_warm(ws.net, again, stretch=ws.stretch)
     for k in range(len(seqs)):
         np.testing.assert_array_equal(again[k], dev[k])
+
+
+def test_concurrent_batches_share_a_network():
+    """Two batches (each its own HIP stream) driven from two host threads at
+    once, sharing one warm-start network, give the same solutions and
+    iteration counts bit for bit as the same solves run one after the other:
+    the batches share nothing but the network's weights, and the forward pass
+    reads and writes each batch's own rows."""
+    import threading
+    print("case: concurrent batches, one network", flush=True)
+    w = S.Workload("wsc", 12, 3, 2, 1, 12, 3, 4, 60, 1e-5, seed=11)
+    seqs = [WS.IntervalSequence(w, 2, seed=40 + k, solver_kwargs=dict(iteration=H.ITERATION_GRAM))
+            for k in range(4)]
+    ws = WS.WarmStart(hidden=64, seed=2, stretch=3)
+    prevs = []
+    for q in seqs:
+        g = q.unit()
+        q.solver.solve_problem(q.problems[0], g)
+        prevs.append(g)
+    halves = [[0, 1], [2, 3]]
+    batches = [H.Batch([seqs[i].solver for i in h], [seqs[i].problems[1] for i in h])
+               for h in halves]
+    for b in batches:
+        b.set_schedule(4, 1, 1, 8)
+
+    def run(k, out, res):
+        sols = [prevs[i].copy() for i in halves[k]]
+        for rep in range(3):  # repeated, so the two threads overlap
+            batches[k].rebind([seqs[i].problems[1] for i in halves[k]])
+            sols = [prevs[i].copy() for i in halves[k]]
+            res[k] = batches[k].solve_warm(ws.net, sols, stretch=ws.stretch)
+        out[k] = sols
+
+    seq_out, seq_res = [None, None], [None, None]
+    for k in range(2):
+        run(k, seq_out, seq_res)
+    con_out, con_res = [None, None], [None, None]
+    threads = [threading.Thread(target=run, args=(k, con_out, con_res)) for k in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for k in range(2):
+        for m in range(2):
+            assert con_res[k][m].iterations == seq_res[k][m].iterations
+            np.testing.assert_array_equal(con_out[k][m], seq_out[k][m])
+    for b in batches:
+        b.close()
