@@ -33,6 +33,12 @@ __device__ __forceinline__ C2 Add(C2 a, C2 b) { return {a.re + b.re, a.im + b.im
 constexpr int kDirBatch = 4;
 constexpr int kMaxSubtractBlocks = 1024;
 
+// Two lanes per visibility: lane h (0, 1) of a pair holds row h of the 2x2
+// matrices (correlations (h, 0) and (h, 1), one 16-byte word), so each load
+// and store instruction of a wave covers 1 KiB of consecutive memory. A
+// full-Jones product needs the other row of the model, which the partner lane
+// passes over (S1 M is formed row by row, then (S1 M) S2^H, the order of the
+// 2x2 products above).
 template <int NPOL>
 __global__ void __launch_bounds__(256)
     CorrectSubtractKernel(float4_v* __restrict__ data, float4_v* __restrict__ model,
@@ -45,9 +51,12 @@ __global__ void __launch_bounds__(256)
   __shared__ int32_t s_start[kMaxSubtractBlocks + 1];
   for (int i = threadIdx.x; i <= n_cb; i += blockDim.x) s_start[i] = cb_start[i];
   __syncthreads();
-  const bool small = n_vis < (int64_t(1) << 32);
-  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n_vis;
-       v += int64_t(gridDim.x) * blockDim.x) {
+  const bool small = n_vis < (int64_t(1) << 31);
+  const int h = threadIdx.x & 1;
+  // the stride is even, so a pair's lanes stay together
+  for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < 2 * n_vis;
+       g += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t v = g >> 1;
     int ch, bl;
     if (small) {  // 32-bit division where the index fits
       const uint32_t u = uint32_t(v), q = u / uint32_t(n_ch);
@@ -72,80 +81,72 @@ __global__ void __launch_bounds__(256)
     const int cb = lo;
     const double2* s = sol + size_t(cb) * n_ant * n_dir * NPOL;
     // the data first (read before the models, so its latency overlaps theirs)
-    float4_v* dp = data + size_t(v) * 2;
-    float4_v d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};
-    if (subtract) {
-      d0 = __builtin_nontemporal_load(dp);
-      d1 = __builtin_nontemporal_load(dp + 1);
-    }
-    C2 sum[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    float4_v* dp = data + size_t(g);
+    float4_v dv = {0, 0, 0, 0};
+    if (subtract) dv = __builtin_nontemporal_load(dp);
+    C2 sum[2] = {{0, 0}, {0, 0}};
     // directions kDirBatch at a time: their model loads issued together, then
     // the products added in direction order
     for (int d0i = 0; d0i < n_dir; d0i += kDirBatch) {
-      float4_v mv[kDirBatch][2];
+      float4_v mv[kDirBatch];
 #pragma unroll
       for (int j = 0; j < kDirBatch; ++j) {
         const int d = d0i + j < n_dir ? d0i + j : d0i;
-        const float4_v* mp = model + (size_t(d) * n_vis + size_t(v)) * 2;
-        mv[j][0] = __builtin_nontemporal_load(mp);
-        mv[j][1] = __builtin_nontemporal_load(mp + 1);
+        mv[j] = __builtin_nontemporal_load(model + size_t(d) * n_vis * 2 + size_t(g));
       }
 #pragma unroll
       for (int j = 0; j < kDirBatch; ++j) {
         const int d = d0i + j;
         if (d >= n_dir) break;
-        float4_v* mp = model + (size_t(d) * n_vis + size_t(v)) * 2;
-        const float4_v m0 = mv[j][0], m1 = mv[j][1];
-        const C2 m[4] = {{m0.x, m0.y}, {m0.z, m0.w}, {m1.x, m1.y}, {m1.z, m1.w}};
+        const float4_v m = mv[j];
+        // this lane's row of M: (h, 0), (h, 1)
+        const C2 mr[2] = {{m.x, m.y}, {m.z, m.w}};
         const double2* p1 = s + (size_t(a1) * n_dir + d) * NPOL;
         const double2* p2 = s + (size_t(a2) * n_dir + d) * NPOL;
-        C2 out[4];
+        C2 out[2];
         if constexpr (NPOL == 1) {
           const C2 f = MulC({p1[0].x, p1[0].y}, {p2[0].x, p2[0].y});
-          for (int c = 0; c < 4; ++c) out[c] = Mul(m[c], f);
+          out[0] = Mul(mr[0], f);
+          out[1] = Mul(mr[1], f);
         } else if constexpr (NPOL == 2) {
-          const C2 s10 = {p1[0].x, p1[0].y}, s11 = {p1[1].x, p1[1].y};
+          const C2 s1 = {p1[h].x, p1[h].y};
           const C2 s20 = {p2[0].x, p2[0].y}, s21 = {p2[1].x, p2[1].y};
-          out[0] = MulC(Mul(s10, m[0]), s20);
-          out[1] = MulC(Mul(s10, m[1]), s21);
-          out[2] = MulC(Mul(s11, m[2]), s20);
-          out[3] = MulC(Mul(s11, m[3]), s21);
+          out[0] = MulC(Mul(s1, mr[0]), s20);
+          out[1] = MulC(Mul(s1, mr[1]), s21);
         } else {
-          const C2 a[4] = {{p1[0].x, p1[0].y}, {p1[1].x, p1[1].y}, {p1[2].x, p1[2].y},
-                           {p1[3].x, p1[3].y}};
+          // the partner's row of M
+          float4_v o;
+          o.x = __shfl_xor(m.x, 1);
+          o.y = __shfl_xor(m.y, 1);
+          o.z = __shfl_xor(m.z, 1);
+          o.w = __shfl_xor(m.w, 1);
+          const C2 orow[2] = {{o.x, o.y}, {o.z, o.w}};
+          const C2* r0 = h == 0 ? mr : orow;  // row 0 of M
+          const C2* r1 = h == 0 ? orow : mr;  // row 1
+          const C2 a0 = {p1[2 * h].x, p1[2 * h].y}, a1v = {p1[2 * h + 1].x, p1[2 * h + 1].y};
           const C2 b[4] = {{p2[0].x, p2[0].y}, {p2[1].x, p2[1].y}, {p2[2].x, p2[2].y},
                            {p2[3].x, p2[3].y}};
-          // t = S1 M, out = t S2^H
-          const C2 t[4] = {Add(Mul(a[0], m[0]), Mul(a[1], m[2])),
-                           Add(Mul(a[0], m[1]), Mul(a[1], m[3])),
-                           Add(Mul(a[2], m[0]), Mul(a[3], m[2])),
-                           Add(Mul(a[2], m[1]), Mul(a[3], m[3]))};
-          out[0] = Add(MulC(t[0], b[0]), MulC(t[1], b[1]));
-          out[1] = Add(MulC(t[0], b[2]), MulC(t[1], b[3]));
-          out[2] = Add(MulC(t[2], b[0]), MulC(t[3], b[1]));
-          out[3] = Add(MulC(t[2], b[2]), MulC(t[3], b[3]));
+          // row h of t = S1 M, then row h of t S2^H
+          const C2 t0 = Add(Mul(a0, r0[0]), Mul(a1v, r1[0]));
+          const C2 t1 = Add(Mul(a0, r0[1]), Mul(a1v, r1[1]));
+          out[0] = Add(MulC(t0, b[0]), MulC(t1, b[1]));
+          out[1] = Add(MulC(t0, b[2]), MulC(t1, b[3]));
         }
-#pragma unroll
-        for (int c = 0; c < 4; ++c) sum[c] = Add(sum[c], out[c]);
-        if (keep_model) {
-          mp[0] = float4_v{float(out[0].re), float(out[0].im), float(out[1].re), float(out[1].im)};
-          mp[1] = float4_v{float(out[2].re), float(out[2].im), float(out[3].re), float(out[3].im)};
-        }
+        sum[0] = Add(sum[0], out[0]);
+        sum[1] = Add(sum[1], out[1]);
+        if (keep_model)
+          model[size_t(d) * n_vis * 2 + size_t(g)] =
+              float4_v{float(out[0].re), float(out[0].im), float(out[1].re), float(out[1].im)};
       }
     }
     if (subtract) {
       // complex<float> -= complex<double> (DDECal.cc:1070): the difference is
       // taken in double and rounded once
-      d0.x = float(double(d0.x) - sum[0].re);
-      d0.y = float(double(d0.y) - sum[0].im);
-      d0.z = float(double(d0.z) - sum[1].re);
-      d0.w = float(double(d0.w) - sum[1].im);
-      d1.x = float(double(d1.x) - sum[2].re);
-      d1.y = float(double(d1.y) - sum[2].im);
-      d1.z = float(double(d1.z) - sum[3].re);
-      d1.w = float(double(d1.w) - sum[3].im);
-      dp[0] = d0;
-      dp[1] = d1;
+      dv.x = float(double(dv.x) - sum[0].re);
+      dv.y = float(double(dv.y) - sum[0].im);
+      dv.z = float(double(dv.z) - sum[1].re);
+      dv.w = float(double(dv.w) - sum[1].im);
+      *dp = dv;
     }
   }
 }
@@ -177,7 +178,7 @@ void Launch(int32_t n_t, int32_t n_bl, int32_t n_ch, int32_t n_cb, int32_t n_dir
             const double* d_sol, void* d_data, void* d_model, int subtract, int keep_model,
             hipStream_t stream) {
   const int64_t n_vis = int64_t(n_t) * n_bl * n_ch;
-  const unsigned blocks = unsigned(std::min<int64_t>((n_vis + 255) / 256, 1 << 20));
+  const unsigned blocks = unsigned(std::min<int64_t>((2 * n_vis + 255) / 256, 1 << 20));
   auto* data = reinterpret_cast<float4_v*>(d_data);
   auto* model = reinterpret_cast<float4_v*>(d_model);
   auto* sol = reinterpret_cast<const double2*>(d_sol);
