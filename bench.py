@@ -37,9 +37,10 @@ converged chi^2 (a 1000-iteration solve at tolerance 1e-9 along the chain).
                   iterations -> features -> 3-layer 256-hidden bf16 MLP),
                   inside the timed region, and the fused Gram iteration
                   (per-baseline data/model cross products made once per
-                  solve; each iteration one kernel forms every antenna's
-                  numerator and denominator from them and the gains). `value`
-                  is this form.
+                  solve; each iteration one kernel finishes the previous
+                  iteration's solutions and forms every antenna's numerator
+                  and denominator from the cross products and the new gains,
+                  then the decision). `value` is this form.
 Also reported: the optimized form with the streaming iteration, the
 optimized form without the network (previous solutions), the plain form at
 the 100-iteration limit and previous-solution starts (DP3's
@@ -47,12 +48,13 @@ propagatesolutions) unbatched in complex64.
 
 `roofline` is for the dominant kernel of the optimized form, the fused Gram
 iteration kernel (algorithmic bytes: the per-baseline cross products it
-reads); its mean duration comes from HIP events on the batch stream around
+reads, the previous iteration's per-workgroup sums read once per block and
+this iteration's written, the solutions read and written); its mean duration comes from HIP events on the batch stream around
 each of its launches in a fixed-length (50-iteration, accuracy 0) solve of
 every stream's interval in one batch, alone on the GPU and autotuned for
 that size, in which every interval is live in every launch (the timed forms
 run the same kernels over each concurrent batch's intervals). `roofline_other`
-holds the rest of that iteration (finishing kernel and decision), the
+holds the rest of that iteration (the decision), the
 unfused apply + antenna solve it replaces, and the batched FP16 correlate of
 the streaming form; `roofline_plain` the complex64 correlate of the plain
 form. `traffic` is read from the committed rocprofv3 --pmc summary of the
@@ -211,8 +213,8 @@ def main():
     # ---- optimized forms: batched + FP16 + learned warm start on the device,
     # with the fused Gram and the streaming iteration. The streams are split
     # over `--batches` batches, each driven by its own host thread on its own
-    # HIP stream, so one batch's latency-bound kernels (finish, decision,
-    # warm-start steps) overlap another's bandwidth-bound ones.
+    # HIP stream, so one batch's latency-bound kernels (decision, warm-start
+    # steps) overlap another's bandwidth-bound ones.
     K = max(1, min(args.batches, B))
     split = [list(range(B))[k::K] for k in range(K)]
 
@@ -259,8 +261,8 @@ def main():
         """Mean of the first and second timed part of a batched iteration over
         a fixed-length profiled solve of all streams' intervals in one batch,
         alone on the GPU and autotuned for that size, in which every interval
-        is live in every launch: (fused kernel, finishing kernel + decision)
-        for the fused Gram iteration, (apply or correlate, antenna solve +
+        is live in every launch: (fused kernel, decision) for the fused Gram
+        iteration, (apply or correlate, antenna solve +
         decision) otherwise. The timed forms run the same kernels over the
         intervals of each of their concurrent batches."""
         solvers = [WS.sibling(q.solver, ROOF_ITERATIONS, 0.0, False, iteration=iteration)
@@ -386,6 +388,11 @@ def main():
                (w.n_directions + 1) // 2)
     walk_bytes = w.n_channel_blocks * 2 * nq_walk * w.n_baselines * 8  # per interval
     solution_bytes = w.n_channel_blocks * w.n_stations * w.n_sub * 2 * 16
+
+    def sums_bytes(groups):
+        """The fused kernel's FP64 sums (numerator re, im, denominator) per
+        (block, polarization x row group, direction, antenna) of one interval."""
+        return w.n_channel_blocks * 2 * groups * w.n_directions * 3 * w.n_stations * 8
     partial_bytes = w.n_channel_blocks * w.n_directions * 2 * 3 * bl_stride * 4
     apply_partial_bytes = w.n_channel_blocks * w.n_directions * 2 * 2 * bl_stride * 4
 
@@ -403,17 +410,20 @@ def main():
     NB = B  # the roofline solves batch every stream's interval
     rooflines = {
         "gram_solve": roof(
-            "gram_solve", NB * walk_bytes, g_solve,
-            note=("fused Gram iteration kernel over the batch: the per-baseline cross products "
-                  "in walk order read once (C_d, G_e,d for e < d, and the real diagonals G_d,d "
-                  f"two per slot: {nq_walk} x 8 bytes per baseline and correlation); per "
-                  f"antenna FP64 sums per row group written ({roof_groups} row groups per "
-                  f"polarization); one launch over all {B} streams' intervals, alone on the "
-                  "GPU")),
-        "gram_finish_decide": roof(
-            "gram_finish_decide", NB * 2 * solution_bytes, g_rest,
-            note="finishing kernel (row-group sums added, FP64 divide, step, convergence "
-                 "terms) and the decision kernel; solutions read and written; latency-bound"),
+            "gram_solve",
+            NB * (walk_bytes + 2 * sums_bytes(roof_groups) + 2 * solution_bytes), g_solve,
+            note=("fused Gram iteration kernel over the batch: the previous iteration finished "
+                  "(its per-antenna FP64 sums read once per block, solutions read and written; "
+                  "every workgroup of a block repeats this step from cache), then the "
+                  "per-baseline cross products in walk order read once (C_d, G_e,d for e < d, "
+                  f"and the real diagonals G_d,d two per slot: {nq_walk} x 8 bytes per "
+                  "baseline and correlation) and the per-antenna FP64 sums written "
+                  f"({roof_groups} row groups per polarization); one launch over all {B} "
+                  "streams' intervals, alone on the GPU")),
+        "gram_decide": roof(
+            "gram_decide", NB * w.n_channel_blocks * w.n_directions * 24, g_rest,
+            note="decision kernel (convergence sums and repair means over the blocks, "
+                 "stopping rule); latency-bound"),
         "unfused_gram_apply": roof(
             "gram_apply", NB * (cross_bytes * bl_stride // w.n_baselines + apply_partial_bytes),
             unfused[0], note="the apply kernel the fused iteration replaces: cross products "
@@ -694,7 +704,7 @@ def main():
                                            "the GPU, 50 fixed iterations, HIP events"),
             "per_batched_iteration_us": {
                 "gram_solve": round(g_solve * 1e3, 1),
-                "gram_finish_decide": round(g_rest * 1e3, 1),
+                "gram_decide": round(g_rest * 1e3, 1),
                 "unfused_gram_apply": round(unfused[0] * 1e3, 1),
                 "unfused_gram_antenna_solve_decide": round(unfused[1] * 1e3, 1),
                 "streaming_correlate": round(s_corr * 1e3, 1),

@@ -172,7 +172,8 @@ struct ddh_solver {
   int unroll = 1;
   int use_graph = 0;
   int poll = 8;
-  int fuse = 1;       // fused Gram iteration where the problem allows it
+  int fuse = 1;       // fused Gram iteration where the problem allows it (1: merged
+                      // finishing step, 2: a finishing kernel per iteration)
   int gs_groups = 2;  // its row groups per polarization (ddh_set_gram_solve_groups)
   // antenna solve schedule (ddh_set_antenna_solve_schedule; 0: the default)
   int solve_split = 0;
@@ -529,6 +530,14 @@ bool FusedGram(const ddh_solver& s, const ddh_problem& p) {
   return true;
 }
 
+// The merged fused iteration (the finishing step of one iteration in the
+// kernel of the next) needs the stepped solutions unchanged until the next
+// iteration reads them: not with the smoothness constraint, which replaces
+// them between the finishing step and the decision.
+bool MergedGram(const ddh_solver& s, const IterArgs& a) {
+  return a.walk_rows > 0 && s.fuse == 1 && a.sm_freq == nullptr;
+}
+
 // fused: -1 decides by FusedGram; 0 / 1 forces (a batch uses one path for all
 // of its intervals).
 IterArgs MakeArgs(ddh_solver& s, ddh_problem& p, int fused = -1) {
@@ -639,14 +648,17 @@ void LaunchSolvePrologue(const ddh_solver& s, const IterArgs& a, const ddh_probl
   LaunchPrologue(a, s.stream);
   if (GramMode(s)) LaunchGram(a, p.f16, s.waves, s.stream);
   if (a.walk_rows > 0) LaunchGramPermute(a, s.stream);
+  if (MergedGram(s, a)) LaunchGramSolve(a, p.f16, s.stream, kGramSolveOnly);
   if (a.full_jones) LaunchFullJonesModelProducts(a, p.f16, s.waves, s.stream);
 }
 
 // Correlate + antenna solve of one iteration (the fused Gram iteration: one
-// kernel for both).
+// kernel for both; merged: the one kernel finishes the previous iteration and
+// forms this one's sums, after the solve prologue formed the first sums).
 void LaunchCorrelateAndSolve(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
   if (a.walk_rows > 0) {
-    LaunchGramSolve(a, p.f16, s.stream);
+    LaunchGramSolve(a, p.f16, s.stream,
+                    MergedGram(s, a) ? kGramFinishThenSolve : kGramSolveAndFinish);
     return;
   }
   LaunchCorrelation(s, a, p);
@@ -724,7 +736,7 @@ void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
     s.gram.Resize(size_t(p.n_cb) * p.n_seg * 2 * GramTerms(p.n_dir) * p.bl_stride);
   if (FusedGram(s, p)) {
     s.gram_walk.Resize(size_t(p.n_cb) * 2 * GramWalkTerms(p.n_dir) * p.walk_rows * p.n_ant);
-    s.gs_part.Resize(size_t(p.n_cb) * 2 * kGramSolveMaxGroups * p.n_dir * 3 * kWave);
+    s.gs_part.Resize(size_t(2) * p.n_cb * 2 * kGramSolveMaxGroups * p.n_dir * 3 * kWave);
   }
   if (c.n_solution_polarizations == 4)
     s.fj_products.Resize(size_t(p.n_cb) * p.n_seg * p.n_dir * kFullJonesModelTerms * p.bl_stride);
@@ -848,7 +860,8 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
         // antenna solve in one kernel)
         if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it], s.stream));
         if (a.walk_rows > 0) {
-          LaunchGramSolve(a, p.f16, s.stream);
+          LaunchGramSolve(a, p.f16, s.stream,
+                          MergedGram(s, a) ? kGramFinishThenSolve : kGramSolveAndFinish);
           if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it + 1], s.stream));
         } else {
           LaunchCorrelation(s, a, p);
@@ -1377,8 +1390,9 @@ int ddh_set_schedule(ddh_solver* s, int32_t waves_per_tile, int32_t rows_per_tri
 int ddh_set_gram_fusion(ddh_solver* s, int32_t enable) {
   return Guard([&] {
     DDH_REQUIRE(s, -1, "NULL solver");
-    if ((enable != 0) != (s->fuse != 0)) {
-      s->fuse = enable != 0;
+    DDH_REQUIRE(enable >= 0 && enable <= 2, -1, "fusion must be 0, 1 or 2");
+    if (enable != s->fuse) {
+      s->fuse = enable;
       DestroyGraph(*s);
       ++s->version;
     }
@@ -1459,6 +1473,7 @@ struct ddh_batch {
   bool gram = false;  // Gram iteration
   bool full_jones = false;  // full-Jones intervals (all or none)
   bool fused = false;       // fused Gram iteration for every interval
+  bool merged = false;      // its finishing step merged into the next iteration's kernel
   int gs_groups = 2;        // its row groups per polarization
   int max_dir = 0;
   // fused Gram iteration: items (interval, cb) grouped by direction count
@@ -1616,6 +1631,10 @@ void BuildBatch(ddh_batch& b) {
     b.smooth.finish_threads =
         std::max(b.smooth.finish_threads, SmoothFinishThreads(a.n_ant, a.max_nsd));
   }
+  // merged fused iteration: every interval asks for it and none smooths (the
+  // smoothing replaces the stepped solutions before the decision)
+  b.merged = b.fused && b.smooth.n_intervals == 0;
+  for (const ddh_solver* s : b.solvers) b.merged = b.merged && s->fuse == 1;
   b.versions.clear();
   for (const ddh_solver* s : b.solvers) b.versions.push_back(s->version);
   b.d_args.Upload(b.args, b.stream);
@@ -1678,8 +1697,26 @@ void LaunchBatchCrossProducts(ddh_batch& b) {
                            b.stream);
 }
 
+// One iteration of the solve loop. Merged fused iteration: one kernel
+// finishes the previous iteration and forms this one's sums (the loop's first
+// sums come from LaunchBatchFirstSums), then the decision.
 void LaunchBatchIteration(ddh_batch& b) {
+  if (b.merged) {
+    LaunchBatchGramSolve(b.d_args.ptr, b.d_gs_items.ptr, b.gs_launch.data(),
+                         int(b.gs_launch.size()), b.f16, b.d_pro_items.ptr, 0, b.stream, true);
+    LaunchBatchSmoothDecide(b.d_args.ptr, b.d_items.ptr, int(b.items.size()),
+                            int(b.solvers.size()), b.smooth, b.stream);
+    return;
+  }
   LaunchBatchStep(b, b.d_args.ptr, int(b.items.size()), int(b.solvers.size()));
+}
+
+// Merged fused iteration: the sums of the state's iteration (the first of
+// the loop), without finishing it.
+void LaunchBatchFirstSums(ddh_batch& b) {
+  if (b.merged)
+    LaunchBatchGramSolve(b.d_args.ptr, b.d_gs_items.ptr, b.gs_launch.data(),
+                         int(b.gs_launch.size()), b.f16, b.d_pro_items.ptr, 0, b.stream);
 }
 
 void EnsureBatchGraph(ddh_batch& b) {
@@ -1872,6 +1909,7 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
   DDH_HIP_CHECK(hipEventRecord(t0, b.stream));
   LaunchBatchPrologue(b.d_args.ptr, b.d_pro_items.ptr, int(b.pro_items.size()), b.stream);
   if (cross_products) LaunchBatchCrossProducts(b);
+  LaunchBatchFirstSums(b);
   for (hipEvent_t& e : b.poll_ev)
     if (!e) DDH_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   // Pipelined polling: the next chunk of `poll` iterations is queued before
@@ -1891,13 +1929,18 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
           // correlate alone between the first two events, antenna solve and
           // decision between the last two
           DDH_HIP_CHECK(hipEventRecord(b.events[3 * it], b.stream));
-          if (b.fused)  // apply and antenna solve in one kernel (its sums)
+          if (b.fused)  // apply and antenna solve in one kernel (its sums; merged:
+                        // after finishing the previous iteration)
             LaunchBatchGramSolve(b.d_args.ptr, b.d_gs_items.ptr, b.gs_launch.data(),
-                                 int(b.gs_launch.size()), b.f16, b.d_pro_items.ptr, 0, b.stream);
+                                 int(b.gs_launch.size()), b.f16, b.d_pro_items.ptr, 0, b.stream,
+                                 b.merged);
           else
             LaunchBatchStep(b, b.d_args.ptr, 0, 0);
           DDH_HIP_CHECK(hipEventRecord(b.events[3 * it + 1], b.stream));
-          if (b.fused) {
+          if (b.merged) {
+            LaunchBatchSmoothDecide(b.d_args.ptr, b.d_items.ptr, int(b.items.size()),
+                                    int(b.solvers.size()), b.smooth, b.stream);
+          } else if (b.fused) {
             // the finishing kernel alone (no sums groups), then the decision
             LaunchBatchGramSolve(b.d_args.ptr, b.d_gs_items.ptr, b.gs_launch.data(), 0, b.f16,
                                  b.d_pro_items.ptr, int(b.pro_items.size()), b.stream);
@@ -2284,10 +2327,18 @@ int ddh_batch_autotune(ddh_batch* b, int32_t* waves_out, int32_t* rows_out, doub
       for (int groups : {1, 2, 4, 8}) {
         b->gs_groups = groups;
         BuildBatch(*b);
+        // merged: the first sums in this layout, then the kernel that finishes
+        // them and forms the next iteration's (into the other parity's sums)
+        LaunchBatchFirstSums(*b);
         const double ms = TimeLaunches(b->stream, 5, [&] {
-          LaunchBatchGramSolve(b->d_args.ptr, b->d_gs_items.ptr, b->gs_launch.data(),
-                               int(b->gs_launch.size()), b->f16, b->d_pro_items.ptr,
-                               int(b->pro_items.size()), b->stream);
+          if (b->merged)
+            LaunchBatchGramSolve(b->d_args.ptr, b->d_gs_items.ptr, b->gs_launch.data(),
+                                 int(b->gs_launch.size()), b->f16, b->d_pro_items.ptr, 0,
+                                 b->stream, true);
+          else
+            LaunchBatchGramSolve(b->d_args.ptr, b->d_gs_items.ptr, b->gs_launch.data(),
+                                 int(b->gs_launch.size()), b->f16, b->d_pro_items.ptr,
+                                 int(b->pro_items.size()), b->stream);
         });
         if (ms < best_gs) {
           best_gs = ms;
@@ -2368,12 +2419,20 @@ int ddh_autotune(ddh_solver* s, ddh_problem* p, int32_t* waves_out, int32_t* row
     IterArgs t = MakeArgs(*s, *p);
     if (t.walk_rows > 0) {
       LaunchSolvePrologue(*s, t, *p);
+      const bool merged = MergedGram(*s, t);
       double best_g = 1e30;
       int bg = s->gs_groups;
       for (int groups : {1, 2, 4, 8}) {
         IterArgs u = t;
         u.gs_groups = groups;
-        const double ms = TimeLaunches(s->stream, 5, [&] { LaunchGramSolve(u, p->f16, s->stream); });
+        // merged: the first iteration's sums in this row-group layout, then
+        // the kernel that finishes it and forms the next sums (into the other
+        // parity's sums)
+        if (merged) LaunchGramSolve(u, p->f16, s->stream, kGramSolveOnly);
+        const double ms = TimeLaunches(s->stream, 5, [&] {
+          LaunchGramSolve(u, p->f16, s->stream,
+                          merged ? kGramFinishThenSolve : kGramSolveAndFinish);
+        });
         if (ms < best_g) {
           best_g = ms;
           bg = groups;

@@ -126,7 +126,9 @@ struct IterArgs {
   const int32_t* walk_bl;
   int32_t walk_rows;
   int32_t gs_groups;  // row groups per polarization of the fused iteration (1..8)
-  double* gs_part;    // [n_cb][2 x gs_groups][n_dir][3][64] per-workgroup sums
+  double* gs_part;    // per-workgroup sums by iteration parity: two halves of
+                      // n_cb x 2 x kGramSolveMaxGroups x n_dir x 3 x 64 doubles, each
+                      // [n_cb][2 x gs_groups][n_dir][3][64] from its start
 };
 
 // Largest row-group count of the fused Gram iteration.
@@ -221,17 +223,28 @@ void LaunchGram(const IterArgs& a, bool f16, int waves, hipStream_t stream);
 void LaunchApplyGram(const IterArgs& a, bool f16, hipStream_t stream);
 // Fused Gram iteration: the cross products into walk order once per solve,
 // then per iteration workgroups per block for apply + antenna solve and a
-// finishing kernel (the decision follows as before).
+// finishing kernel (the decision follows as before). Merged form: the sums
+// of the first iteration alone, then per iteration one kernel that finishes
+// the previous iteration and forms the sums of the next, and the decision.
+enum GramSolveMode {
+  kGramSolveAndFinish = 0,   // the state's iteration: sums, then the finishing kernel
+  kGramSolveOnly = 1,        // the state's iteration: its sums only
+  kGramFinishThenSolve = 2,  // the end of the state's iteration, then the next one's sums
+};
 void LaunchGramPermute(const IterArgs& a, hipStream_t stream);
-void LaunchGramSolve(const IterArgs& a, bool f16, hipStream_t stream);
+void LaunchGramSolve(const IterArgs& a, bool f16, hipStream_t stream,
+                     GramSolveMode mode = kGramSolveAndFinish);
 // Batched: items (interval, cb); groups split them by direction count.
 void LaunchBatchGramPermute(const IterArgs* d_args, const BatchTile* d_items, int n_items,
                             int max_dir, hipStream_t stream);
 // d_items: (interval, cb, part) workgroups; d_blocks: (interval, cb) for the
-// finishing kernel (n_blocks 0: the sums only).
+// finishing kernel (n_blocks 0: the sums only). finish_first: each workgroup
+// finishes the state's iteration and forms the next one's sums (merged form;
+// n_blocks 0).
 void LaunchBatchGramSolve(const IterArgs* d_args, const BatchTile* d_items,
                           const TileGroup* groups, int n_groups, bool f16,
-                          const BatchTile* d_blocks, int n_blocks, hipStream_t stream);
+                          const BatchTile* d_blocks, int n_blocks, hipStream_t stream,
+                          bool finish_first = false);
 // LDS of the Gram apply: the segment's gains of every antenna and direction.
 size_t ApplyGramSharedBytes(int n_ant, int n_dir);
 constexpr size_t kApplyGramMaxShared = 64 * 1024;

@@ -50,10 +50,14 @@ __device__ __forceinline__ double2 FixedSolution(const double2* sol, size_t pair
 template <int NC, bool F16>
 __device__ __forceinline__ void LoadVis(const void* base, size_t idx, float2 (&v)[NC]) {
   if constexpr (!F16) {
+    // A Full visibility is two 16-byte halves, so each half's load covers
+    // every other 16 bytes of the wave's lines: the first is a plain load,
+    // which keeps the lines in L2 for the second (non-temporal; as the first
+    // it would let them go and the second would fetch them again).
     const float4_v* p = reinterpret_cast<const float4_v*>(base) + idx * (NC / 2);
 #pragma unroll
     for (int h = 0; h < NC / 2; ++h) {
-      const float4_v x = __builtin_nontemporal_load(p + h);
+      const float4_v x = (NC == 4 && h == 0) ? p[h] : __builtin_nontemporal_load(p + h);
       v[2 * h] = make_float2(x.x, x.y);
       v[2 * h + 1] = make_float2(x.z, x.w);
     }
@@ -771,7 +775,7 @@ __device__ __forceinline__ void FinishSolution(const IterArgs& a, const double2*
                                                double2* nxt, const size_t pair, const double mean,
                                                double (&nr)[2], double (&ni)[2], double (&dn)[2],
                                                double& conv_sum, int& conv_n, double& mean_sum,
-                                               int& mean_n) {
+                                               int& mean_n, double2 (&out)[2]) {
   if (a.scalar) {
     nr[0] = nr[1] = nr[0] + nr[1];
     ni[0] = ni[1] = ni[0] + ni[1];
@@ -800,8 +804,11 @@ __device__ __forceinline__ void FinishSolution(const IterArgs& a, const double2*
       sincos(phase_from + a.step_size * distance, &sn, &cs);
       stepped[p] = make_double2(cs, sn);
     }
-    nxt[pair * 2 + p] = stepped[p];
-    if (a.den_out) a.den_out[pair * 2 + p] = dn[p];
+    if (nxt) {  // null: the caller keeps the values (the fused iteration's other workgroups)
+      nxt[pair * 2 + p] = stepped[p];
+      if (a.den_out) a.den_out[pair * 2 + p] = dn[p];
+    }
+    out[p] = stepped[p];
   }
   AddPairTerms(old, stepped, a.scalar != 0, conv_sum, conv_n, mean_sum, mean_n);
 }
@@ -932,9 +939,10 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
 
   double conv_sum = 0.0, mean_sum = 0.0;
   int conv_n = 0, mean_n = 0;
+  double2 unused[2];
   if (active && q == 0)
     FinishSolution(a, cur, nxt, gbase + size_t(ant) * a.n_sub + s_abs, mean, nr, ni, dn,
-                   conv_sum, conv_n, mean_sum, mean_n);
+                   conv_sum, conv_n, mean_sum, mean_n, unused);
   BlockSum2(conv_sum, conv_n, mean_sum, mean_n, red, redc);
   const int wg = cb * a.n_dir + d;
   if (threadIdx.x == 0) {
@@ -1012,7 +1020,67 @@ __device__ __forceinline__ void LdsAdd(double* p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-template <int ND, bool F16>
+// The fused iteration's per-workgroup sums of iteration parity `par`: two
+// sets, so that a kernel can finish one iteration from its sums while it
+// stores those of the next.
+__device__ __forceinline__ double* GramParts(const IterArgs& a, int par) {
+  return a.gs_part + size_t(par) * a.n_cb * 2 * kGramSolveMaxGroups * a.n_dir * 3 * kWave;
+}
+
+// Sums of row groups g0 and g0 + 1 (those below `groups`) of both
+// polarizations for the thread's (direction, antenna) items: direction
+// wave + u * waves, antenna = lane.
+template <int ND, int NE>
+__device__ __forceinline__ void LoadGroupSums(const GlobalT<double>* all, const int groups,
+                                              const int g0, const int wave, const int lane,
+                                              double (&v)[NE][2][2][3]) {
+  const size_t part_stride = size_t(ND) * 3 * kWave;
+#pragma unroll
+  for (int u = 0; u < NE; ++u) {
+    const int d = wave + u * kGramSolveWaves;
+    if (d >= ND) continue;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        if (g0 + j < groups) {
+          const GlobalT<double>* x =
+              all + size_t(p * groups + g0 + j) * part_stride + size_t(d) * 3 * kWave + lane;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) v[u][p][j][c] = x[c * kWave];
+        }
+  }
+}
+
+template <int NE>
+__device__ __forceinline__ void AddGroupSums(const double (&v)[NE][2][2][3], const int groups,
+                                             const int g0, double (&nr)[NE][2],
+                                             double (&ni)[NE][2], double (&dn)[NE][2]) {
+#pragma unroll
+  for (int u = 0; u < NE; ++u)
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        if (g0 + j < groups) {
+          nr[u][p] += v[u][p][j][0];
+          ni[u][p] += v[u][p][j][1];
+          dn[u][p] += v[u][p][j][2];
+        }
+}
+
+// Fused Gram iteration kernel body: the iteration the state names (FIN
+// false), or (FIN true) the end of that iteration followed by the next one.
+// With FIN every workgroup of a block first finishes all of the block's
+// solutions from the stored sums (the same sums in the same order, and the
+// same FinishSolution, as GramFinishBlock: the same values) and keeps its
+// share in registers; the block's first workgroup stores them with the
+// convergence and repair-mean partials the decision reads. The repair mean
+// of the new solutions is formed as the decision forms it (the directions'
+// |pol 0| sums in direction order), so the next iteration's gains are those
+// the unmerged sequence would stage. The decision then follows the kernel:
+// an interval it stops ran one more iteration's sums, which nothing reads.
+template <int ND, bool F16, bool FIN>
 __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, const int part) {
   constexpr int NC = 2;
   constexpr int NQ = ND + ND * (ND - 1) / 2 + (ND + 1) / 2;
@@ -1029,7 +1097,8 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
   const int k = part / groups;  // polarization
   const int group = part - k * groups;
   const int n = a.n_ant;
-  const int iteration = AsConst(a.state)->iteration;
+  const int it_state = AsConst(a.state)->iteration;
+  const int iteration = FIN ? it_state + 1 : it_state;  // the iteration solved here
   const double2* cur = (iteration & 1) ? a.sol1 : a.sol0;
   const size_t gbase = size_t(cb) * n * a.n_sub;
 
@@ -1047,17 +1116,31 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
   // Gains of polarization k of every antenna and direction (one solution
   // each). Their loads are issued first and the first row's cross products
   // right after them: vector loads complete in order, so writing the gains
-  // to LDS waits for the gain loads only while the row streams in.
+  // to LDS waits for the gain loads only while the row streams in. With FIN
+  // the stored sums the gains are finished from are loaded and added first
+  // (the row then streams in during the finishing step).
   double2 gl[kGainEntries][2];
+  [[maybe_unused]] double nr[kGainEntries][2] = {}, ni[kGainEntries][2] = {},
+                          dn[kGainEntries][2] = {};
+  if constexpr (FIN) {
+    const GlobalT<double>* fsums =
+        AsGlobal(GramParts(a, it_state & 1)) + size_t(cb) * 2 * groups * ND * 3 * kWave;
+    double fv[kGainEntries][2][2][3];
+    for (int g0 = 0; g0 < groups; g0 += 2) {
+      LoadGroupSums<ND, kGainEntries>(fsums, groups, g0, wave, lane, fv);
+      AddGroupSums<kGainEntries>(fv, groups, g0, nr, ni, dn);
+    }
+  } else {
 #pragma unroll
-  for (int u = 0; u < kGainEntries; ++u) {
-    const int e = threadIdx.x + u * kGramSolveThreads;
-    const int ant = e & (kWave - 1);
-    const int d = (e < ND * kWave ? e : 0) / kWave;
-    const size_t pair =
-        gbase + size_t(ant < n ? ant : 0) * a.n_sub + AsConst(a.seg_sol)[d];
-    gl[u][0] = LoadGlobal(cur + pair * 2);
-    gl[u][1] = LoadGlobal(cur + pair * 2 + 1);
+    for (int u = 0; u < kGainEntries; ++u) {
+      const int e = threadIdx.x + u * kGramSolveThreads;
+      const int ant = e & (kWave - 1);
+      const int d = (e < ND * kWave ? e : 0) / kWave;
+      const size_t pair =
+          gbase + size_t(ant < n ? ant : 0) * a.n_sub + AsConst(a.seg_sol)[d];
+      gl[u][0] = LoadGlobal(cur + pair * 2);
+      gl[u][1] = LoadGlobal(cur + pair * 2 + 1);
+    }
   }
   __builtin_amdgcn_sched_barrier(0);
   f2v q[NQ];
@@ -1074,10 +1157,57 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
   int r = r0 + wave;
   if (r < r1) load_row(r);
   __builtin_amdgcn_sched_barrier(0);
+  double mean;
+  if constexpr (FIN) {
+    __shared__ double s_msum[ND];
+    __shared__ int s_mn[ND];
+    const double2* fcur = (it_state & 1) ? a.sol1 : a.sol0;
+    double2* fnxt = part == 0 ? ((it_state & 1) ? a.sol0 : a.sol1) : nullptr;
+    const double old_mean = AsConst(a.mean)[cb];
+#pragma unroll
+    for (int u = 0; u < kGainEntries; ++u) {
+      const int d = wave + u * kGramSolveWaves;
+      if (d >= ND) continue;
+      double conv_sum = 0.0, mean_sum = 0.0;
+      int conv_n = 0, mean_n = 0;
+      gl[u][0] = gl[u][1] = make_double2(0.0, 0.0);
+      if (lane < n)
+        FinishSolution(a, fcur, fnxt, gbase + size_t(lane) * a.n_sub + AsConst(a.seg_sol)[d],
+                       old_mean, nr[u], ni[u], dn[u], conv_sum, conv_n, mean_sum, mean_n, gl[u]);
+      for (int off = 32; off > 0; off >>= 1) {
+        conv_sum += __shfl_xor(conv_sum, off);
+        conv_n += __shfl_xor(conv_n, off);
+        mean_sum += __shfl_xor(mean_sum, off);
+        mean_n += __shfl_xor(mean_n, off);
+      }
+      if (lane == 0) {
+        if (part == 0) {
+          const int wg = cb * ND + d;
+          a.wg_conv[wg] = conv_sum;
+          a.wg_conv_n[wg] = conv_n;
+          a.wg_mean[wg] = mean_sum;
+          a.wg_mean_n[wg] = mean_n;
+        }
+        s_msum[d] = mean_sum;
+        s_mn[d] = mean_n;
+      }
+    }
+    __syncthreads();
+    // the block's repair mean, as the decision forms it (DecideBlock)
+    double ms = 0.0;
+    int mn = 0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      ms += s_msum[d];
+      mn += s_mn[d];
+    }
+    mean = mn == 0 ? 1.0 : ms / mn;
+  } else {
+    mean = AsConst(a.mean)[cb];
+  }
   // repaired (a pair with a non-finite entry reads as the block mean) and
   // cast to float, as the correlate uses them (SolveData.h:230-235)
   {
-    const double mean = AsConst(a.mean)[cb];
 #pragma unroll
     for (int u = 0; u < kGainEntries; ++u) {
       const int e = threadIdx.x + u * kGramSolveThreads;
@@ -1177,7 +1307,7 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
   // the four waves' sums, added in wave order
   __syncthreads();
   const size_t part_stride = size_t(ND) * 3 * kWave;  // doubles per workgroup's sums
-  double* mine = a.gs_part + (size_t(cb) * 2 * groups + part) * part_stride;
+  double* mine = GramParts(a, iteration & 1) + (size_t(cb) * 2 * groups + part) * part_stride;
   for (int d = wave; d < ND; d += kGramSolveWaves) {
     const double scd = F16 ? double(AsConst(a.model_scale)[cb * ND + d]) : 1.0;
 #pragma unroll
@@ -1207,7 +1337,8 @@ __device__ __forceinline__ void GramFinishBlock(const IterArgs& a, const int cb)
   const size_t gbase = size_t(cb) * n * a.n_sub;
   const double mean = AsConst(a.mean)[cb];
   const size_t part_stride = size_t(a.n_dir) * 3 * kWave;
-  const GlobalT<double>* all = AsGlobal(a.gs_part) + size_t(cb) * 2 * groups * part_stride;
+  const GlobalT<double>* all =
+      AsGlobal(GramParts(a, iteration & 1)) + size_t(cb) * 2 * groups * part_stride;
   for (int d = wave; d < a.n_dir; d += n_waves) {
     // every sum loaded before the first is added (one round trip, not one per
     // row group), then added in row-group order
@@ -1235,9 +1366,10 @@ __device__ __forceinline__ void GramFinishBlock(const IterArgs& a, const int cb)
         }
     double conv_sum = 0.0, mean_sum = 0.0;
     int conv_n = 0, mean_n = 0;
+    double2 unused[2];
     if (lane < n)
       FinishSolution(a, cur, nxt, gbase + size_t(lane) * a.n_sub + AsConst(a.seg_sol)[d], mean,
-                     nr, ni, dn, conv_sum, conv_n, mean_sum, mean_n);
+                     nr, ni, dn, conv_sum, conv_n, mean_sum, mean_n, unused);
     for (int off = 32; off > 0; off >>= 1) {
       conv_sum += __shfl_xor(conv_sum, off);
       conv_n += __shfl_xor(conv_n, off);
@@ -1261,11 +1393,12 @@ __global__ void __launch_bounds__(kGramFinishThreads) GramFinishKernel(const Ite
   GramFinishBlock(a, blockIdx.x);
 }
 
-// grid (2 x groups, n_cb)
-template <int ND, bool F16>
+// grid (2 x groups, n_cb); FIN: the end of the state's iteration, then the
+// next iteration's sums
+template <int ND, bool F16, bool FIN>
 __global__ void __launch_bounds__(kGramSolveThreads, 3) GramSolveKernel(const IterArgs a) {
   if (AsConst(a.state)->done) return;
-  GramSolveBlock<ND, F16>(a, blockIdx.y, blockIdx.x);
+  GramSolveBlock<ND, F16, FIN>(a, blockIdx.y, blockIdx.x);
 }
 
 // Cross products from baseline order [cb][seg 0][k][q][bl_stride] into the
@@ -2313,13 +2446,13 @@ __global__ void __launch_bounds__(kBlockThreads)
 }
 
 // items (interval, cb, part); one workgroup per item
-template <int ND, bool F16>
+template <int ND, bool F16, bool FIN>
 __global__ void __launch_bounds__(kGramSolveThreads, 3)
     BatchGramSolveKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ items) {
   const BatchTile t = LoadConst(items + XcdContiguous(blockIdx.x, gridDim.x));
   const IterArgs a = LoadConst(args + t.interval);
   if (AsConst(a.state)->done) return;
-  GramSolveBlock<ND, F16>(a, t.cb, t.seg);
+  GramSolveBlock<ND, F16, FIN>(a, t.cb, t.seg);
 }
 
 // items (interval, cb): the end of the fused iteration of each block
@@ -2582,18 +2715,23 @@ void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const Ti
 }
 
 namespace {
-template <bool F16, int... ND>
+template <bool F16, bool FIN, int... ND>
 constexpr auto MakeGramSolveTable(std::integer_sequence<int, ND...>) {
-  return std::array<KernelFn, sizeof...(ND)>{&GramSolveKernel<ND + 1, F16>...};
+  return std::array<KernelFn, sizeof...(ND)>{&GramSolveKernel<ND + 1, F16, FIN>...};
 }
-template <bool F16, int... ND>
+template <bool F16, bool FIN, int... ND>
 constexpr auto MakeBatchGramSolveTable(std::integer_sequence<int, ND...>) {
-  return std::array<BatchKernelFn, sizeof...(ND)>{&BatchGramSolveKernel<ND + 1, F16>...};
+  return std::array<BatchKernelFn, sizeof...(ND)>{&BatchGramSolveKernel<ND + 1, F16, FIN>...};
 }
-const std::array<KernelFn, kMaxGramDirections> kGramSolve[2] = {
-    MakeGramSolveTable<false>(GramDirs{}), MakeGramSolveTable<true>(GramDirs{})};
-const std::array<BatchKernelFn, kMaxGramDirections> kBatchGramSolve[2] = {
-    MakeBatchGramSolveTable<false>(GramDirs{}), MakeBatchGramSolveTable<true>(GramDirs{})};
+// [FIN][F16]
+const std::array<KernelFn, kMaxGramDirections> kGramSolve[2][2] = {
+    {MakeGramSolveTable<false, false>(GramDirs{}), MakeGramSolveTable<true, false>(GramDirs{})},
+    {MakeGramSolveTable<false, true>(GramDirs{}), MakeGramSolveTable<true, true>(GramDirs{})}};
+const std::array<BatchKernelFn, kMaxGramDirections> kBatchGramSolve[2][2] = {
+    {MakeBatchGramSolveTable<false, false>(GramDirs{}),
+     MakeBatchGramSolveTable<true, false>(GramDirs{})},
+    {MakeBatchGramSolveTable<false, true>(GramDirs{}),
+     MakeBatchGramSolveTable<true, true>(GramDirs{})}};
 }  // namespace
 
 void LaunchGramPermute(const IterArgs& a, hipStream_t stream) {
@@ -2603,13 +2741,14 @@ void LaunchGramPermute(const IterArgs& a, hipStream_t stream) {
   DDH_HIP_CHECK(hipGetLastError());
 }
 
-void LaunchGramSolve(const IterArgs& a, bool f16, hipStream_t stream) {
+void LaunchGramSolve(const IterArgs& a, bool f16, hipStream_t stream, GramSolveMode mode) {
   CheckGramDirections(a.n_dir);
   DDH_REQUIRE(a.walk_rows > 0 && a.n_ant <= kWave, -1, "no walk order for this problem");
   DDH_REQUIRE(a.gs_groups >= 1 && a.gs_groups <= kGramSolveMaxGroups, -1, "bad row groups");
-  hipLaunchKernelGGL(kGramSolve[f16][a.n_dir - 1], dim3(2 * a.gs_groups, a.n_cb),
-                     dim3(kGramSolveThreads), 0, stream, a);
-  hipLaunchKernelGGL(GramFinishKernel, dim3(a.n_cb), dim3(kGramFinishThreads), 0, stream, a);
+  hipLaunchKernelGGL(kGramSolve[mode == kGramFinishThenSolve][f16][a.n_dir - 1],
+                     dim3(2 * a.gs_groups, a.n_cb), dim3(kGramSolveThreads), 0, stream, a);
+  if (mode == kGramSolveAndFinish)
+    hipLaunchKernelGGL(GramFinishKernel, dim3(a.n_cb), dim3(kGramFinishThreads), 0, stream, a);
   DDH_HIP_CHECK(hipGetLastError());
 }
 
@@ -2623,12 +2762,14 @@ void LaunchBatchGramPermute(const IterArgs* d_args, const BatchTile* d_items, in
 
 void LaunchBatchGramSolve(const IterArgs* d_args, const BatchTile* d_items,
                           const TileGroup* groups, int n_groups, bool f16,
-                          const BatchTile* d_blocks, int n_blocks, hipStream_t stream) {
+                          const BatchTile* d_blocks, int n_blocks, hipStream_t stream,
+                          bool finish_first) {
   for (int g = 0; g < n_groups; ++g) {
     if (groups[g].count == 0) continue;
     CheckGramDirections(groups[g].n_dir);
-    hipLaunchKernelGGL(kBatchGramSolve[f16][groups[g].n_dir - 1], dim3(groups[g].count),
-                       dim3(kGramSolveThreads), 0, stream, d_args, d_items + groups[g].first);
+    hipLaunchKernelGGL(kBatchGramSolve[finish_first][f16][groups[g].n_dir - 1],
+                       dim3(groups[g].count), dim3(kGramSolveThreads), 0, stream, d_args,
+                       d_items + groups[g].first);
   }
   if (n_blocks > 0)
     hipLaunchKernelGGL(BatchGramFinishKernel, dim3(n_blocks), dim3(kGramFinishThreads), 0, stream,

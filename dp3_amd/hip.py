@@ -409,7 +409,9 @@ class Solver:
 
     def set_gram_fusion(self, enable=True):
         """Fused Gram iteration (apply + antenna solve in one kernel) where the
-        problem allows it; on by default."""
+        problem allows it; on by default. True (1): each iteration's finishing
+        step runs inside the next iteration's kernel; 2: a separate finishing
+        kernel per iteration (the same values); False (0): off."""
         _check(lib().ddh_set_gram_fusion(self.handle, int(enable)), "ddh_set_gram_fusion")
 
     def set_gram_solve_groups(self, groups):

@@ -261,6 +261,10 @@ int ddh_set_schedule(ddh_solver* solver, int32_t waves_per_tile,
 // correlate sums and each antenna's numerator and denominator on chip,
 // without per-baseline partials in device memory. Off: the apply and antenna
 // solve kernels. Results agree to FP32 summation order.
+// enable: 0 off; 1 on, each iteration's finishing step (divide, step,
+// convergence terms) done inside the next iteration's kernel where no
+// smoothness constraint is set; 2 on, with a finishing kernel per iteration.
+// 1 and 2 give the same values.
 int ddh_set_gram_fusion(ddh_solver* solver, int32_t enable);
 // Schedule of the fused Gram iteration: each block's walk over the antenna
 // pairs is split, per polarization, into `groups` (1..8) workgroups whose

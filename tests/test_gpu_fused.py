@@ -208,3 +208,58 @@ def test_fused_batch_equals_single_and_warm_start():
         q.solver.solve_problem(q.problems[1], ref)
         assert phase_ref_relerr(x, ref, q.w.n_stations, q.w.n_sub) <= ORDER_TOL
     b.close()
+
+
+@pytest.mark.parametrize("storage,phase_only,graph",
+                         [(H.MODEL_C64, False, 1), (H.MODEL_F16, False, 0),
+                          (H.MODEL_F16, True, 1)])
+def test_merged_finishing_step_equals_finishing_kernel(small, storage, phase_only, graph):
+    """The finishing step inside the next iteration's kernel (fusion 1) gives
+    the solutions, iteration count and step magnitudes of the separate
+    finishing kernel (fusion 2) bit for bit, in single and batched solves."""
+    w, geom, sky, gains, P = small
+    print(f"case: merged finishing step storage={storage} phase_only={phase_only} "
+          f"graph={graph}", flush=True)
+    kw = dict(max_iterations=w.max_iterations, accuracy=w.tolerance, model_storage=storage,
+              phase_only=phase_only)
+    out = {}
+    for mode in (1, 2):
+        s = solver(P, mode, **kw)
+        s.set_schedule(waves_per_tile=2, rows_per_trip=1, use_graph=graph, poll_interval=8)
+        g = P.unit_solutions()
+        r = s.solve(P, g)
+        out[mode] = (g, r.iterations)
+    assert out[1][1] == out[2][1]
+    np.testing.assert_array_equal(out[1][0], out[2][0])
+    print(f"single: {out[1][1]} iterations in both forms")
+
+
+def test_merged_finishing_step_batch_and_row_groups():
+    """Batches of intervals of different shapes: the merged form equals the
+    separate finishing kernel bit for bit at every row-group count (one to
+    eight, the sums of groups beyond two added in a second pass)."""
+    print("case: merged finishing step, batch", flush=True)
+    ws = [S.Workload(f"mb{i}", 12 + 7 * i, 3 + i, 1, 1, 8, 2, 2 + 2 * i, 60, 1e-5, seed=20 + i)
+          for i in range(3)]
+    seqs = [WS.IntervalSequence(w, 1, seed=80 + i,
+                                solver_kwargs=dict(iteration=H.ITERATION_GRAM))
+            for i, w in enumerate(ws)]
+    for groups in (1, 3, 8):
+        res = {}
+        for mode in (1, 2):
+            for q in seqs:
+                q.solver.set_gram_fusion(mode)
+                q.solver.set_gram_solve_groups(groups)
+            b = H.Batch([q.solver for q in seqs], [q.problems[0] for q in seqs])
+            b.set_schedule(2, 1, 1, 8)
+            b.set_gram_solve_groups(groups)
+            got = [q.unit() for q in seqs]
+            rs = b.solve(got)
+            res[mode] = (got, [r.iterations for r in rs])
+            b.close()
+        assert res[1][1] == res[2][1]
+        for x, y in zip(res[1][0], res[2][0]):
+            np.testing.assert_array_equal(x, y)
+        print(f"groups {groups}: iterations {res[1][1]}")
+    for q in seqs:
+        q.solver.set_gram_fusion(1)
