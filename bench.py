@@ -559,13 +559,27 @@ def main():
     fj_g = np.zeros((w.n_channel_blocks, w.n_stations * fj_sub, 4), np.complex128)
     fj_g[:, :, 0] = fj_g[:, :, 3] = 1.0
     fj_g = fj_g.reshape(w.n_channel_blocks, -1)
-    fj.solve_problem(fj_dp, fj_g)
+    fj.solve_problem(fj_dp, fj_g.copy())
     fj_cms, fj_n, fj_total = fj.last_timing()
     fj_corr = fj_cms / max(fj_n, 1)
     fj_iter = fj_total / max(fj_n, 1)
-    fj_dp.close()
     fj.close()
-    log(f"full-Jones solver: {fj_iter * 1e3:.1f} us per iteration, correlate {fj_corr * 1e3:.1f} us")
+    # the same solve with the Gram iteration: the cross products once, then
+    # per iteration the apply (timed per iteration) and the antenna solve
+    fjg = H.Solver(w.n_stations, w.sols_per_dir(), w.n_channel_blocks, n_correlations=4,
+                   n_solution_polarizations=4, max_iterations=ROOF_ITERATIONS, accuracy=0.0,
+                   detect_stalling=False, iteration=H.ITERATION_GRAM)
+    fjg.set_schedule(waves_per_tile=4, rows_per_trip=1, use_graph=0, poll_interval=8)
+    fjg.set_profiling(True)
+    fjg.solve_problem(fj_dp, fj_g.copy())
+    fjg_cms, fjg_n, fjg_total = fjg.last_timing()
+    fjg_apply = fjg_cms / max(fjg_n, 1)
+    fjg_iter = fjg_total / max(fjg_n, 1)
+    fjg.close()
+    fj_dp.close()
+    log(f"full-Jones solver: {fj_iter * 1e3:.1f} us per iteration, correlate {fj_corr * 1e3:.1f} us; "
+        f"Gram iteration: {fjg_iter * 1e3:.1f} us per iteration (cross products included), "
+        f"apply {fjg_apply * 1e3:.1f} us")
     del sub_data, sub_model, sub_w, sub_f
     sub_vis = w.solint * n_bl_all * w.n_channels
     cross_vis = w.solint * int(cross.sum()) * w.n_channels
@@ -592,7 +606,19 @@ def main():
                   "visibilities, 4 correlations, complex64 planes): data and every model plane "
                   "read once (all 8 directions in one group), X and Y (16 per-baseline partials "
                   "per direction) written; the denominators come from model products made once "
-                  "per solve; iteration_ms adds the FP64 antenna solve and the decision"))}
+                  "per solve; iteration_ms adds the FP64 antenna solve and the decision")),
+        "full_jones_gram_apply": roof(
+            "full_jones_gram_apply",
+            w.n_channel_blocks * w.n_baselines * (
+                (16 * (w.n_directions + w.n_directions * (w.n_directions - 1) // 2)) * 8 +
+                w.n_directions * 16 * 4 * 2), fjg_apply,
+            iteration_ms=round(fjg_iter, 5),
+            note=("the full-Jones Gram iteration's apply over the same interval: the per-baseline "
+                  "cross products (C_d and G_e,d for e < d, 16 complex64 terms each) and the "
+                  "model products read, X and Y written (the correlate's partials); about 17,400 "
+                  "FP32 FMAs per baseline at 8 directions; iteration_ms adds the antenna solve, "
+                  "the decision and the once-per-solve cross products over the "
+                  f"{ROOF_ITERATIONS}-iteration solve"))}
     log(f"correct and subtract: {sub_ms * 1e3:.1f} us per interval; assign and weight "
         f"(FP16 problem): {ing_ms * 1e3:.1f} us per interval")
 

@@ -228,9 +228,11 @@ void ValidateConfig(const ddh_config& c) {
   DDH_REQUIRE(c.iteration == DDH_ITERATION_STREAM || c.iteration == DDH_ITERATION_GRAM, -1,
               "unknown iteration kind");
   DDH_REQUIRE(c.iteration != DDH_ITERATION_GRAM ||
-                  (DeviceCorrelations(c) == 2 && c.n_directions <= kMaxGramDirections),
-              -4, "the Gram iteration is built for Duo data with at most 8 directions");
-  DDH_REQUIRE(c.iteration != DDH_ITERATION_GRAM ||
+                  ((DeviceCorrelations(c) == 2 || c.n_solution_polarizations == 4) &&
+                   c.n_directions <= kMaxGramDirections),
+              -4, "the Gram iteration is built for Duo data (diagonal and scalar solvers) and the "
+                  "full-Jones solver, with at most 8 directions");
+  DDH_REQUIRE(c.iteration != DDH_ITERATION_GRAM || c.n_solution_polarizations == 4 ||
                   ApplyGramSharedBytes(c.n_antennas, c.n_directions) <= kApplyGramMaxShared,
               -4, "the Gram iteration stages every antenna's gains in 64 KB of LDS "
                   "(n_antennas x (2 n_directions + 1) x 8 bytes)");
@@ -247,8 +249,6 @@ void ValidateConfig(const ddh_config& c) {
               -1, "n_solution_polarizations must be 1 (scalar), 2 (diagonal) or 4 (full-Jones)");
   DDH_REQUIRE(c.n_solution_polarizations != 4 || c.n_correlations == 4, -1,
               "the full-Jones solver needs Full data (n_correlations 4)");
-  DDH_REQUIRE(c.n_solution_polarizations != 4 || c.iteration == DDH_ITERATION_STREAM, -4,
-              "the full-Jones solver uses the streaming iteration");
   DDH_REQUIRE(c.n_solutions_per_direction != nullptr, -1,
               "n_solutions_per_direction is required");
   DDH_REQUIRE(c.max_iterations >= 1 && c.max_iterations <= 100000, -1,
@@ -637,7 +637,9 @@ int ConfigMaxNsd(const ddh_solver& s) {
 
 // The correlate partials of one iteration, by the configured iteration kind.
 void LaunchCorrelation(const ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
-  if (GramMode(s))
+  if (GramMode(s) && a.full_jones)
+    LaunchFullJonesApplyGram(a, s.stream);
+  else if (GramMode(s))
     LaunchApplyGram(a, p.f16, s.stream);
   else
     LaunchCorrelate(a, p.n_corr, p.f16, s.waves, s.unroll, s.stream);
@@ -646,7 +648,10 @@ void LaunchCorrelation(const ddh_solver& s, const IterArgs& a, const ddh_problem
 // Gram iteration: the cross products of the problem, before the first iteration.
 void LaunchSolvePrologue(const ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
   LaunchPrologue(a, s.stream);
-  if (GramMode(s)) LaunchGram(a, p.f16, s.waves, s.stream);
+  if (GramMode(s) && a.full_jones)
+    LaunchFullJonesGram(a, p.f16, s.stream);
+  else if (GramMode(s))
+    LaunchGram(a, p.f16, s.waves, s.stream);
   if (a.walk_rows > 0) LaunchGramPermute(a, s.stream);
   if (MergedGram(s, a)) LaunchGramSolve(a, p.f16, s.stream, kGramSolveOnly);
   if (a.full_jones) LaunchFullJonesModelProducts(a, p.f16, s.waves, s.stream);
@@ -733,7 +738,9 @@ void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
   s.partials.Resize(size_t(p.n_cb) * p.n_seg * p.n_dir * terms * p.bl_stride);
   s.state.Resize(1);
   if (c.iteration == DDH_ITERATION_GRAM)
-    s.gram.Resize(size_t(p.n_cb) * p.n_seg * 2 * GramTerms(p.n_dir) * p.bl_stride);
+    s.gram.Resize(size_t(p.n_cb) * p.n_seg * p.bl_stride *
+                  (c.n_solution_polarizations == 4 ? FullJonesGramTerms(p.n_dir)
+                                                   : 2 * GramTerms(p.n_dir)));
   if (FusedGram(s, p)) {
     s.gram_walk.Resize(size_t(p.n_cb) * 2 * GramWalkTerms(p.n_dir) * p.walk_rows * p.n_ant);
     s.gs_part.Resize(size_t(2) * p.n_cb * 2 * kGramSolveMaxGroups * p.n_dir * 3 * kWave);
@@ -1662,9 +1669,11 @@ void RefreshBatch(ddh_batch& b) {
 // antenna solve over n_items items and the decision for n_intervals intervals.
 void LaunchBatchStep(ddh_batch& b, const IterArgs* args, int n_items, int n_intervals) {
   if (b.full_jones) {
+    // Gram iteration: the apply over (interval, block, segment, tile) entries
     LaunchBatchFullJones(args, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()),
                          b.d_items.ptr, n_items, n_intervals, b.solve_threads, b.f16, b.waves,
-                         b.smooth, b.stream);
+                         b.smooth, b.stream, b.gram ? b.d_prod_tiles.ptr : nullptr,
+                         b.gram ? int(b.prod_tiles.size()) : 0, b.max_dir);
   } else if (b.fused) {
     if (n_items > 0)
       LaunchBatchGramSolve(args, b.d_gs_items.ptr, b.gs_launch.data(), int(b.gs_launch.size()),
@@ -1689,7 +1698,10 @@ void LaunchBatchCrossProducts(ddh_batch& b) {
   if (b.full_jones)
     LaunchBatchFullJonesModelProducts(b.d_args.ptr, b.d_prod_tiles.ptr, int(b.prod_tiles.size()),
                                       b.f16, b.waves, b.stream);
-  if (b.gram)
+  if (b.gram && b.full_jones)
+    LaunchBatchFullJonesGram(b.d_args.ptr, b.d_prod_tiles.ptr, int(b.prod_tiles.size()),
+                             b.max_dir, b.f16, b.stream);
+  else if (b.gram)
     LaunchBatchGram(b.d_args.ptr, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), b.f16,
                     b.waves, b.stream);
   if (b.fused)  // once per (interval, block)
@@ -2398,7 +2410,9 @@ int ddh_autotune(ddh_solver* s, ddh_problem* p, int32_t* waves_out, int32_t* row
     for (const auto& cand : ScheduleCandidates(p->n_corr)) {
       // Gram iteration: the schedule shapes the once-per-solve cross products
       const double ms = TimeLaunches(s->stream, 5, [&] {
-        if (GramMode(*s))
+        if (GramMode(*s) && a.full_jones)
+          LaunchFullJonesGram(a, p->f16, s->stream);
+        else if (GramMode(*s))
           LaunchGram(a, p->f16, cand.first, s->stream);
         else
           LaunchCorrelate(a, p->n_corr, p->f16, cand.first, cand.second, s->stream);

@@ -147,6 +147,16 @@ constexpr int kFullJonesGroupSize = 8;
 // Largest direction count of the Gram iteration (its per-lane accumulators
 // grow with n_dir^2).
 constexpr int kMaxGramDirections = 8;
+// Full-Jones Gram iteration: cross products per (block, segment, baseline),
+// C_d and G_e,d (e < d), 16 complex terms each; the cross-products kernel
+// deals out three blocks of 16 terms per wave.
+constexpr int kFjGramBlocksPerWave = 3;
+__host__ __device__ inline int FullJonesGramTerms(int n_dir) {
+  return 16 * (n_dir + n_dir * (n_dir - 1) / 2);
+}
+__host__ __device__ inline int FjGramWaves(int n_dir) {
+  return (n_dir * (n_dir + 1) / 2 + kFjGramBlocksPerWave - 1) / kFjGramBlocksPerWave;
+}
 __host__ __device__ inline int GramTerms(int n_dir) { return n_dir + n_dir * (n_dir + 1) / 2; }
 // Terms per walk slot of the fused iteration: C_d (n_dir), the off-diagonal
 // G_e,d (e < d), and the real diagonals G_d,d two per slot.
@@ -173,6 +183,13 @@ void LaunchDecide(const IterArgs& a, hipStream_t stream);
 void LaunchPrologue(const IterArgs& a, hipStream_t stream);
 // Full-Jones solver: the model products of the problem, once per solve.
 void LaunchFullJonesModelProducts(const IterArgs& a, bool f16, int waves, hipStream_t stream);
+// Full-Jones Gram iteration: the cross products once per solve (after or
+// before the model products, which the apply also reads), and per iteration
+// the apply, which writes the correlate's X and Y partials.
+void LaunchFullJonesGram(const IterArgs& a, bool f16, hipStream_t stream);
+void LaunchFullJonesApplyGram(const IterArgs& a, hipStream_t stream);
+void LaunchBatchFullJonesGram(const IterArgs* d_args, const BatchTile* d_tiles, int n_tiles,
+                              int max_dir, bool f16, hipStream_t stream);
 // FP64 chi^2 of the residual with solutions `sol` (current buffer of `a`):
 // per-workgroup partial sums into `partial`, then their fixed-order sum,
 // times `factor`, into `out`.
@@ -277,10 +294,13 @@ void LaunchBatchSmoothDecide(const IterArgs* d_args, const BatchTile* d_items, i
 // One batched full-Jones iteration: the correlate per direction-count group
 // (tile entries hold tile + group * bl_stride / 64), the antenna solve over
 // n_items (interval, block, direction) items, smoothing and the decision.
+// Gram iteration: d_apply_tiles (interval, block, segment, tile) for the
+// apply over at most apply_dir directions, in place of the correlate groups.
 void LaunchBatchFullJones(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
                           int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
                           int solve_threads, bool f16, int waves, const SmoothLaunch& smooth,
-                          hipStream_t stream);
+                          hipStream_t stream, const BatchTile* d_apply_tiles = nullptr,
+                          int n_apply_tiles = 0, int apply_dir = 0);
 // The full-Jones model products of every interval, once per solve: one
 // workgroup per (interval, block, segment, tile) entry.
 void LaunchBatchFullJonesModelProducts(const IterArgs* d_args, const BatchTile* d_tiles,

@@ -2178,6 +2178,313 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Full-Jones Gram iteration. The gains are constant over a baseline's rows in
+// a segment, so with P_d[ij][km] = sum_v R_all,ij conj(M_d,km) (R_all the
+// residual with every direction subtracted; entry index 2i+j of the 2x2
+// matrices) the correlate's sums are
+//   X_d,ik = sum_jm Jb_d,jm P_d[ij][km],   Y_d,ik = sum_jm Ja_d,jm conj(P_d[ji][mk])
+// and P_d follows from cross products made once per solve:
+//   P_d[ij][km] = C_d[ij][km] - sum_e sum_no Ja_e,in conj(Jb_e,jo) G_e,d[no][km],
+//   C_d[p][q] = sum_v D_p conj(M_d,q),   G_e,d[p][q] = sum_v M_e,p conj(M_d,q)
+// (G_d,d is the model product Q_d; G_e,d = conj(G_d,e) transposed). With the
+// gains contracted first, per pair (e, d):
+//   X_d,ik -= sum_n Ja_e,in V[n][k],   V[n][k] = sum_om (Jb_e^H Jb_d)_om G_e,d[no][km]
+//   Y_d,ik -= sum_o Jb_e,io W[o][k],   W[o][k] = sum_nm (Ja_e^H Ja_d)_nm conj(G_e,d[no][mk])
+// and the data term adds sum_jm Jb_d,jm C_d[ij][km] to X and
+// sum_jm Ja_d,jm conj(C_d[ji][mk]) to Y. The apply writes X and Y in the
+// correlate's partials layout, so the full-Jones antenna solve and the
+// decision are those of the streaming iteration.
+// Cross products [n_cb][n_seg][FullJonesGramTerms(n_dir)][bl_stride]
+// complex64 in blocks of 16 terms (term 4p + q: entry p of the first factor,
+// entry q of the conjugated model): C_d at block d, then G_e,d (e < d) at
+// block n_dir + d(d-1)/2 + e. FP16 planes are scaled back as they are read.
+// ---------------------------------------------------------------------------
+// One visibility of a Full plane through L2 (plain loads: several waves of the
+// workgroup read the same rows), FP16 planes scaled back.
+template <bool F16>
+__device__ __forceinline__ void LoadFullShared(const void* base, size_t idx, float sc,
+                                               float2 (&v)[4]) {
+  if constexpr (!F16) {
+    const float4_v* p = reinterpret_cast<const float4_v*>(base) + idx * 2;
+    const float4_v x0 = p[0], x1 = p[1];
+    v[0] = make_float2(x0.x, x0.y);
+    v[1] = make_float2(x0.z, x0.w);
+    v[2] = make_float2(x1.x, x1.y);
+    v[3] = make_float2(x1.z, x1.w);
+  } else {
+    const uint4_v u = reinterpret_cast<const uint4_v*>(base)[idx];
+    v[0] = HalfBitsToFloat2(u.x);
+    v[1] = HalfBitsToFloat2(u.y);
+    v[2] = HalfBitsToFloat2(u.z);
+    v[3] = HalfBitsToFloat2(u.w);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k].x *= sc;
+      v[k].y *= sc;
+    }
+  }
+}
+
+// Cross products of one (block, segment, 64-baseline tile), lane = baseline.
+// The n_dir(n_dir+1)/2 product blocks in the order (target d, source) =
+// (0, D), (1, D), (1, M_0), (2, D), (2, M_0), (2, M_1), ... are dealt out
+// three per wave; every wave runs over all of the segment's rows (the rows of
+// the planes it shares with other waves come from L2).
+template <bool F16>
+__device__ __forceinline__ void FullJonesGramTile(const IterArgs& a, const int tile, const int seg,
+                                                  const int cb) {
+  constexpr int NB = kFjGramBlocksPerWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kWave);
+  const int nd = a.n_dir;
+  const int n_blocks = nd * (nd + 1) / 2;
+  const CbInfo info = LoadConst(a.cbinfo + cb);
+  const size_t plane = size_t(a.n_t) * info.n_ch * a.bl_stride;
+  const int bl = tile * kWave + lane;
+  int tgt[NB], src[NB], pos[NB];
+  float sct[NB], scs[NB];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    const int b = min(wave * NB + k, n_blocks - 1);  // past the last block: unused
+    int d = 0;
+    while ((d + 1) * (d + 2) / 2 <= b) ++d;
+    tgt[k] = d;
+    src[k] = b - d * (d + 1) / 2 - 1;  // -1: the data
+    pos[k] = src[k] < 0 ? d : nd + d * (d - 1) / 2 + src[k];
+    sct[k] = F16 ? AsConst(a.model_scale)[cb * nd + d] : 1.0f;
+    scs[k] = (F16 && src[k] >= 0) ? AsConst(a.model_scale)[cb * nd + src[k]] : 1.0f;
+  }
+  float2 acc[NB][16];
+#pragma unroll
+  for (int k = 0; k < NB; ++k)
+#pragma unroll
+    for (int t = 0; t < 16; ++t) acc[k][t] = make_float2(0.0f, 0.0f);
+  const int r0 = AsConst(a.seg_t0)[seg] * info.n_ch;
+  const int r1 = AsConst(a.seg_t0)[seg + 1] * info.n_ch;
+  for (int row = r0; row < r1; ++row) {
+    const size_t e = size_t(row) * a.bl_stride + bl;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      if (wave * NB + k >= n_blocks) continue;
+      float2 mt[4], ms[4];
+      LoadFullShared<F16>(a.model, info.model_off + tgt[k] * plane + e, sct[k], mt);
+      if (src[k] < 0)
+        LoadFullShared<false>(a.data, info.data_off + e, 1.0f, ms);
+      else
+        LoadFullShared<F16>(a.model, info.model_off + src[k] * plane + e, scs[k], ms);
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) CMacC(acc[k][4 * p + q], mt[q], ms[p]);
+    }
+  }
+  const size_t base = (size_t(cb) * a.n_seg + seg) * FullJonesGramTerms(nd);
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    if (wave * NB + k >= n_blocks) continue;
+    float2* out = a.gram + (base + size_t(pos[k]) * 16) * a.bl_stride + bl;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) out[size_t(t) * a.bl_stride] = acc[k][t];
+  }
+}
+
+// At most FjGramWaves(8) = 12 waves per workgroup.
+constexpr int kFjGramMaxThreads = 12 * kWave;
+
+// Single solve: grid (bl tiles, segments, blocks), FjGramWaves(n_dir) waves.
+template <bool F16>
+__global__ void __launch_bounds__(kFjGramMaxThreads) FullJonesGramKernel(const IterArgs a) {
+  FullJonesGramTile<F16>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Batch: one workgroup per (interval, block, segment, tile) entry.
+template <bool F16>
+__global__ void __launch_bounds__(kFjGramMaxThreads)
+    BatchFullJonesGramKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ tiles) {
+  const BatchTile t = LoadConst(tiles + blockIdx.x);
+  const IterArgs a = LoadConst(args + t.interval);
+  if (int(threadIdx.x) >= FjGramWaves(a.n_dir) * kWave) return;
+  FullJonesGramTile<F16>(a, t.tile, t.seg, t.cb);
+}
+
+// A complex64 value through a global load (8 bytes).
+__device__ __forceinline__ float2 LoadGlobalF2(const float2* p) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  const f2v v = *(const GlobalT<f2v>*)p;
+  return make_float2(v.x, v.y);
+}
+
+// The apply: X and Y of one (block, segment, 64-baseline tile), lane =
+// baseline; wave w takes the target directions d = w, w + waves, ...
+// The gains are staged in LDS as the correlate stages them.
+__device__ __forceinline__ void FullJonesApplyGramTile(const IterArgs& a, const int tile,
+                                                       const int seg, const int cb) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int n_waves = blockDim.x / kWave;
+  const int nd = a.n_dir;
+  const int bl = tile * kWave + lane;
+  extern __shared__ float smem[];
+  float2* g_s = reinterpret_cast<float2*>(smem);  // [n_dir][8][64]: J_a, then J_b
+  {
+    const size_t gbase = size_t(cb) * a.n_ant * a.n_sub;
+    const double2* sol = (AsConst(a.state)->iteration & 1) ? a.sol1 : a.sol0;
+    const double mean = AsConst(a.mean)[cb];
+    const bool valid = bl < a.n_bl;
+    const int a1 = valid ? a.ant1[bl] : 0;
+    const int a2 = valid ? a.ant2[bl] : 0;
+    for (int d = wave; d < nd; d += n_waves) {
+      const int sidx = AsConst(a.seg_sol)[seg * nd + d];
+      double2 j1[4], j2[4];
+      FixedMatrix(sol, gbase + size_t(a1) * a.n_sub + sidx, mean, j1);
+      FixedMatrix(sol, gbase + size_t(a2) * a.n_sub + sidx, mean, j2);
+      float2* dst = g_s + size_t(d) * 8 * kWave + lane;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        dst[k * kWave] =
+            valid ? make_float2(float(j1[k].x), float(j1[k].y)) : make_float2(0.0f, 0.0f);
+        dst[(4 + k) * kWave] =
+            valid ? make_float2(float(j2[k].x), float(j2[k].y)) : make_float2(0.0f, 0.0f);
+      }
+    }
+  }
+  __syncthreads();
+  const size_t base = (size_t(cb) * a.n_seg + seg) * FullJonesGramTerms(nd);
+  const float2* gram = a.gram + bl;
+  const GlobalT<float>* prod = AsGlobal(static_cast<const float*>(a.fj_products)) + bl;
+  for (int d = wave; d < nd; d += n_waves) {
+    const LaneGains ja{g_s + size_t(d) * 8 * kWave + lane};
+    const LaneGains jb{ja.p + 4 * kWave};
+    float2 x[4], y[4];
+    {
+      float2 c[16];
+      const float2* cd = gram + (base + size_t(d) * 16) * a.bl_stride;
+#pragma unroll
+      for (int t = 0; t < 16; ++t) c[t] = LoadGlobalF2(cd + size_t(t) * a.bl_stride);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          float2 xv = make_float2(0.0f, 0.0f), yv = make_float2(0.0f, 0.0f);
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+              // X += Jb_d,jm C[ij][km];  Y += Ja_d,jm conj(C[ji][mk])
+              CMac(xv, jb[2 * j + m], c[4 * (2 * i + j) + 2 * k + m]);
+              const float2 cc = c[4 * (2 * j + i) + 2 * m + k];
+              CMac(yv, ja[2 * j + m], make_float2(cc.x, -cc.y));
+            }
+          x[2 * i + k] = xv;
+          y[2 * i + k] = yv;
+        }
+    }
+    for (int e = 0; e < nd; ++e) {
+      // G_e,d[p][q] for p, q = 0..3
+      float2 g[16];
+      if (e < d) {
+        const float2* src = gram + (base + size_t(nd + d * (d - 1) / 2 + e) * 16) * a.bl_stride;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) g[t] = LoadGlobalF2(src + size_t(t) * a.bl_stride);
+      } else if (e > d) {
+        // G_e,d[p][q] = conj(G_d,e[q][p])
+        const float2* src = gram + (base + size_t(nd + e * (e - 1) / 2 + d) * 16) * a.bl_stride;
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const float2 v = LoadGlobalF2(src + size_t(4 * q + p) * a.bl_stride);
+            g[4 * p + q] = make_float2(v.x, -v.y);
+          }
+      } else {
+        // Q_d: the real diagonal, then (p < q) in the order (0,1), (0,2),
+        // (0,3), (1,2), (1,3), (2,3) as (re, im)
+        const GlobalT<float>* q16 =
+            prod + ((size_t(cb) * a.n_seg + seg) * nd + d) * kFullJonesModelTerms * a.bl_stride;
+        float qv[kFullJonesModelTerms];
+#pragma unroll
+        for (int t = 0; t < kFullJonesModelTerms; ++t) qv[t] = q16[size_t(t) * a.bl_stride];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) g[5 * p] = make_float2(qv[p], 0.0f);
+        int v = 4;
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+          for (int q = p + 1; q < 4; ++q) {
+            g[4 * p + q] = make_float2(qv[v], qv[v + 1]);
+            g[4 * q + p] = make_float2(qv[v], -qv[v + 1]);
+            v += 2;
+          }
+      }
+      const LaneGains jae{g_s + size_t(e) * 8 * kWave + lane};
+      const LaneGains jbe{jae.p + 4 * kWave};
+      // H = Jb_e^H Jb_d, H2 = Ja_e^H Ja_d: H[o][m] = sum_j conj(Jb_e,jo) Jb_d,jm
+      float2 h[4], h2[4];
+#pragma unroll
+      for (int o = 0; o < 2; ++o)
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+          float2 t = make_float2(0.0f, 0.0f), t2 = make_float2(0.0f, 0.0f);
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            CMacC(t, jbe[2 * j + o], jb[2 * j + m]);
+            CMacC(t2, jae[2 * j + o], ja[2 * j + m]);
+          }
+          h[2 * o + m] = t;
+          h2[2 * o + m] = t2;
+        }
+#pragma unroll
+      for (int n = 0; n < 2; ++n)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          // V[n][k] = sum_om H[o][m] G[no][km];  W[n][k] = sum_om H2[o][m] conj(G[on][mk])
+          float2 vv = make_float2(0.0f, 0.0f), ww = make_float2(0.0f, 0.0f);
+#pragma unroll
+          for (int o = 0; o < 2; ++o)
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+              CMac(vv, h[2 * o + m], g[4 * (2 * n + o) + 2 * k + m]);
+              const float2 gc = g[4 * (2 * o + n) + 2 * m + k];
+              CMac(ww, h2[2 * o + m], make_float2(gc.x, -gc.y));
+            }
+          // X_ik -= Ja_e,in V[n][k];  Y_ik -= Jb_e,in W[n][k]
+#pragma unroll
+          for (int i = 0; i < 2; ++i) {
+            const float2 ta = jae[2 * i + n];
+            const float2 tb = jbe[2 * i + n];
+            CMac(x[2 * i + k], make_float2(-ta.x, -ta.y), vv);
+            CMac(y[2 * i + k], make_float2(-tb.x, -tb.y), ww);
+          }
+        }
+    }
+    float* out = a.partials + ((size_t(cb) * a.n_seg + seg) * nd + d) * kFullJonesTerms * a.bl_stride;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      out[size_t(2 * v) * a.bl_stride + bl] = x[v].x;
+      out[size_t(2 * v + 1) * a.bl_stride + bl] = x[v].y;
+      out[size_t(8 + 2 * v) * a.bl_stride + bl] = y[v].x;
+      out[size_t(9 + 2 * v) * a.bl_stride + bl] = y[v].y;
+    }
+  }
+}
+
+// Single solve: grid (bl tiles, segments, blocks).
+__global__ void __launch_bounds__(256) FullJonesApplyGramKernel(const IterArgs a) {
+  if (AsConst(a.state)->done) return;
+  FullJonesApplyGramTile(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Batch: one workgroup per (interval, block, segment, tile) entry.
+__global__ void __launch_bounds__(256)
+    BatchFullJonesApplyGramKernel(const IterArgs* __restrict__ args,
+                                  const BatchTile* __restrict__ tiles) {
+  const BatchTile t = LoadConst(tiles + blockIdx.x);
+  const IterArgs a = LoadConst(args + t.interval);
+  if (AsConst(a.state)->done) return;
+  FullJonesApplyGramTile(a, t.tile, t.seg, t.cb);
+}
+
 // Q[p][q] of the model products of one baseline (FP64).
 struct ModelProducts {
   double v[kFullJonesModelTerms];
@@ -2975,11 +3282,63 @@ void LaunchFullJonesModelProducts(const IterArgs& a, bool f16, int waves, hipStr
   DDH_HIP_CHECK(hipGetLastError());
 }
 
+namespace {
+// The full-Jones Gram apply's workgroup: up to four waves (target directions
+// dealt out over them), the gains of every direction in LDS.
+int FjApplyWaves(int n_dir) { return std::min(n_dir, 4); }
+size_t FjApplySharedBytes(int n_dir) { return size_t(n_dir) * 8 * kWave * sizeof(float2); }
+}  // namespace
+
+void LaunchFullJonesGram(const IterArgs& a, bool f16, hipStream_t stream) {
+  CheckGramDirections(a.n_dir);
+  DDH_REQUIRE(a.full_jones && a.gram && a.fj_products, -1, "no full-Jones Gram buffers");
+  const dim3 grid(a.bl_stride / kWave, a.n_seg, a.n_cb);
+  const dim3 block(FjGramWaves(a.n_dir) * kWave);
+  DDH_REQUIRE(int(block.x) <= kFjGramMaxThreads, -1, "too many full-Jones Gram waves");
+  if (f16)
+    hipLaunchKernelGGL(FullJonesGramKernel<true>, grid, block, 0, stream, a);
+  else
+    hipLaunchKernelGGL(FullJonesGramKernel<false>, grid, block, 0, stream, a);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchFullJonesApplyGram(const IterArgs& a, hipStream_t stream) {
+  CheckGramDirections(a.n_dir);
+  hipLaunchKernelGGL(FullJonesApplyGramKernel, dim3(a.bl_stride / kWave, a.n_seg, a.n_cb),
+                     dim3(FjApplyWaves(a.n_dir) * kWave), FjApplySharedBytes(a.n_dir), stream, a);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchBatchFullJonesGram(const IterArgs* d_args, const BatchTile* d_tiles, int n_tiles,
+                              int max_dir, bool f16, hipStream_t stream) {
+  if (n_tiles == 0) return;
+  CheckGramDirections(max_dir);
+  const dim3 block(FjGramWaves(max_dir) * kWave);
+  DDH_REQUIRE(int(block.x) <= kFjGramMaxThreads, -1, "too many full-Jones Gram waves");
+  if (f16)
+    hipLaunchKernelGGL(BatchFullJonesGramKernel<true>, dim3(n_tiles), block, 0, stream, d_args,
+                       d_tiles);
+  else
+    hipLaunchKernelGGL(BatchFullJonesGramKernel<false>, dim3(n_tiles), block, 0, stream, d_args,
+                       d_tiles);
+  DDH_HIP_CHECK(hipGetLastError());
+}
+
 void LaunchBatchFullJones(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
                           int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
                           int solve_threads, bool f16, int waves, const SmoothLaunch& smooth,
-                          hipStream_t stream) {
+                          hipStream_t stream, const BatchTile* d_apply_tiles, int n_apply_tiles,
+                          int apply_dir) {
   DDH_REQUIRE(waves >= 1 && waves * kWave <= kCorrelateMaxThreads, -1, "bad waves per tile");
+  if (d_apply_tiles && n_apply_tiles > 0) {
+    // Gram iteration: X and Y from the cross products (every direction of a
+    // tile in one workgroup) instead of the correlate
+    CheckGramDirections(apply_dir);
+    hipLaunchKernelGGL(BatchFullJonesApplyGramKernel, dim3(n_apply_tiles),
+                       dim3(FjApplyWaves(apply_dir) * kWave), FjApplySharedBytes(apply_dir),
+                       stream, d_args, d_apply_tiles);
+    n_groups = 0;
+  }
   for (int g = 0; g < n_groups; ++g) {
     const TileGroup& grp = groups[g];
     if (grp.count == 0) continue;

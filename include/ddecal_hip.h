@@ -47,7 +47,9 @@ extern "C" {
 #define DDH_ITERATION_STREAM 0  // stream data and model planes every iteration
 #define DDH_ITERATION_GRAM 1    // per-baseline data/model cross products made once
                                 // per solve, applied to the gains every
-                                // iteration (Duo, up to 8 directions)
+                                // iteration (up to 8 directions; Duo data for
+                                // the diagonal and scalar solvers, Full data
+                                // for the full-Jones solver)
 
 typedef struct ddh_solver ddh_solver;
 typedef struct ddh_problem ddh_problem;

@@ -192,8 +192,6 @@ def test_full_jones_rejections(small4):
     with pytest.raises(H.HipError, match="Full data"):
         H.Solver(P.n_antennas, P.n_solutions_per_direction, P.n_channel_blocks,
                  n_correlations=2, n_solution_polarizations=4)
-    with pytest.raises(H.HipError, match="Gram|streaming"):
-        gpu_full_jones(P, iteration=H.ITERATION_GRAM)
     s = gpu_full_jones(P)
     dp = H.DeviceProblem.from_problem(s, P)
     dp.upload(P)
