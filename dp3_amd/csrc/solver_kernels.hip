@@ -765,13 +765,14 @@ __device__ __forceinline__ void AddPairTerms(const double2 (&old)[2], const doub
 }
 
 // The end of SolveDirection for one solution pair (antenna, sub-solution) of
-// a block, from its FP64 numerator and denominator sums per polarization: the
+// a block, from its stored current values cv and its FP64 numerator and
+// denominator sums per polarization: the
 // division with NaN for a zero denominator (IterativeDiagonalSolver.cc:
 // 206-221), the step (SolverBase.cc:88-158, the polar step for phase-only
 // solves), the optional denominators, and the pair's convergence and repair-
 // mean terms added to the running sums. The scalar solver fits one gain to
 // both polarizations: the sums of both (IterativeScalarSolver.cc:142-173).
-__device__ __forceinline__ void FinishSolution(const IterArgs& a, const double2* cur,
+__device__ __forceinline__ void FinishSolution(const IterArgs& a, const double2 (&cv)[2],
                                                double2* nxt, const size_t pair, const double mean,
                                                double (&nr)[2], double (&ni)[2], double (&dn)[2],
                                                double& conv_sum, int& conv_n, double& mean_sum,
@@ -788,7 +789,8 @@ __device__ __forceinline__ void FinishSolution(const IterArgs& a, const double2*
   for (int p = 0; p < 2; ++p) {
     const double2 nx = dn[p] == 0.0 ? make_double2(nan, 0.0)
                                     : make_double2(nr[p] / dn[p], ni[p] / dn[p]);
-    old[p] = FixedSolution(cur, pair, p, mean);
+    // the repaired current value (FixedSolution on the pair's stored values)
+    old[p] = (cfinite(cv[0]) && cfinite(cv[1])) ? cv[p] : make_double2(mean, 0.0);
     if (!a.phase_only) {
       stepped[p] = make_double2(old[p].x * (1.0 - a.step_size) + nx.x * a.step_size,
                                 old[p].y * (1.0 - a.step_size) + nx.y * a.step_size);
@@ -939,10 +941,13 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
 
   double conv_sum = 0.0, mean_sum = 0.0;
   int conv_n = 0, mean_n = 0;
-  double2 unused[2];
-  if (active && q == 0)
-    FinishSolution(a, cur, nxt, gbase + size_t(ant) * a.n_sub + s_abs, mean, nr, ni, dn,
-                   conv_sum, conv_n, mean_sum, mean_n, unused);
+  if (active && q == 0) {
+    const size_t pair = gbase + size_t(ant) * a.n_sub + s_abs;
+    const double2 cv[2] = {LoadGlobal(cur + pair * 2), LoadGlobal(cur + pair * 2 + 1)};
+    double2 unused[2];
+    FinishSolution(a, cv, nxt, pair, mean, nr, ni, dn, conv_sum, conv_n, mean_sum, mean_n,
+                   unused);
+  }
   BlockSum2(conv_sum, conv_n, mean_sum, mean_n, red, redc);
   const int wg = cb * a.n_dir + d;
   if (threadIdx.x == 0) {
@@ -1122,11 +1127,25 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
   double2 gl[kGainEntries][2];
   [[maybe_unused]] double nr[kGainEntries][2] = {}, ni[kGainEntries][2] = {},
                           dn[kGainEntries][2] = {};
+  // FIN: the stored current values of the thread's (direction, antenna) pairs
+  [[maybe_unused]] double2 fcv[kGainEntries][2];
   if constexpr (FIN) {
+    const double2* fcur = (it_state & 1) ? a.sol1 : a.sol0;
+#pragma unroll
+    for (int u = 0; u < kGainEntries; ++u) {
+      const int d = min(wave + u * kGramSolveWaves, ND - 1);
+      const size_t pair = gbase + size_t(lane < n ? lane : 0) * a.n_sub + AsConst(a.seg_sol)[d];
+      fcv[u][0] = LoadGlobal(fcur + pair * 2);
+      fcv[u][1] = LoadGlobal(fcur + pair * 2 + 1);
+    }
     const GlobalT<double>* fsums =
         AsGlobal(GramParts(a, it_state & 1)) + size_t(cb) * 2 * groups * ND * 3 * kWave;
     double fv[kGainEntries][2][2][3];
-    for (int g0 = 0; g0 < groups; g0 += 2) {
+    // groups >= 1: the first pass unconditional, so that no path reaches the
+    // row loads with these loads (or the current values) still in flight
+    LoadGroupSums<ND, kGainEntries>(fsums, groups, 0, wave, lane, fv);
+    AddGroupSums<kGainEntries>(fv, groups, 0, nr, ni, dn);
+    for (int g0 = 2; g0 < groups; g0 += 2) {
       LoadGroupSums<ND, kGainEntries>(fsums, groups, g0, wave, lane, fv);
       AddGroupSums<kGainEntries>(fv, groups, g0, nr, ni, dn);
     }
@@ -1161,7 +1180,6 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
   if constexpr (FIN) {
     __shared__ double s_msum[ND];
     __shared__ int s_mn[ND];
-    const double2* fcur = (it_state & 1) ? a.sol1 : a.sol0;
     double2* fnxt = part == 0 ? ((it_state & 1) ? a.sol0 : a.sol1) : nullptr;
     const double old_mean = AsConst(a.mean)[cb];
 #pragma unroll
@@ -1172,7 +1190,7 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
       int conv_n = 0, mean_n = 0;
       gl[u][0] = gl[u][1] = make_double2(0.0, 0.0);
       if (lane < n)
-        FinishSolution(a, fcur, fnxt, gbase + size_t(lane) * a.n_sub + AsConst(a.seg_sol)[d],
+        FinishSolution(a, fcv[u], fnxt, gbase + size_t(lane) * a.n_sub + AsConst(a.seg_sol)[d],
                        old_mean, nr[u], ni[u], dn[u], conv_sum, conv_n, mean_sum, mean_n, gl[u]);
       for (int off = 32; off > 0; off >>= 1) {
         conv_sum += __shfl_xor(conv_sum, off);
@@ -1366,10 +1384,13 @@ __device__ __forceinline__ void GramFinishBlock(const IterArgs& a, const int cb)
         }
     double conv_sum = 0.0, mean_sum = 0.0;
     int conv_n = 0, mean_n = 0;
-    double2 unused[2];
-    if (lane < n)
-      FinishSolution(a, cur, nxt, gbase + size_t(lane) * a.n_sub + AsConst(a.seg_sol)[d], mean,
-                     nr, ni, dn, conv_sum, conv_n, mean_sum, mean_n, unused);
+    if (lane < n) {
+      const size_t pair = gbase + size_t(lane) * a.n_sub + AsConst(a.seg_sol)[d];
+      const double2 cv[2] = {LoadGlobal(cur + pair * 2), LoadGlobal(cur + pair * 2 + 1)};
+      double2 unused[2];
+      FinishSolution(a, cv, nxt, pair, mean, nr, ni, dn, conv_sum, conv_n, mean_sum, mean_n,
+                     unused);
+    }
     for (int off = 32; off > 0; off >>= 1) {
       conv_sum += __shfl_xor(conv_sum, off);
       conv_n += __shfl_xor(conv_n, off);
@@ -1679,9 +1700,24 @@ __device__ __forceinline__ void DecideBlock(const IterArgs& a) {
   __shared__ double red[kBlockThreads / 64];
   __shared__ int redc[kBlockThreads / 64];
   const int n_wg = a.n_cb * a.n_dir;
-  // the state is loaded first, so that its latency overlaps the sums'
+  // The state, the first convergence terms and the first block's mean terms
+  // are loaded together (one round trip for the common case of one block per
+  // thread and at most 8 directions); the terms are added in the same order
+  // in any case.
   DeviceState st;
   if (threadIdx.x == 0) st = *a.state;
+  constexpr int kMeanBatch = 8;
+  const int c0 = threadIdx.x;
+  double mv0[kMeanBatch];
+  int nv0[kMeanBatch];
+  if (c0 < a.n_cb) {
+#pragma unroll
+    for (int j = 0; j < kMeanBatch; ++j)
+      if (j < a.n_dir) {
+        mv0[j] = a.wg_mean[c0 * a.n_dir + j];
+        nv0[j] = a.wg_mean_n[c0 * a.n_dir + j];
+      }
+  }
   double sum = 0.0;
   int count = 0;
   for (int i = threadIdx.x; i < n_wg; i += blockDim.x) {
@@ -1689,8 +1725,7 @@ __device__ __forceinline__ void DecideBlock(const IterArgs& a) {
     count += a.wg_conv_n[i];
   }
   BlockSum(sum, count, red, redc);
-  constexpr int kMeanBatch = 8;
-  for (int c = threadIdx.x; c < a.n_cb; c += blockDim.x) {
+  for (int c = c0; c < a.n_cb; c += blockDim.x) {
     double ms = 0.0;
     int mn = 0;
     // the block's terms loaded kMeanBatch at a time, added in direction order
@@ -1700,8 +1735,13 @@ __device__ __forceinline__ void DecideBlock(const IterArgs& a) {
 #pragma unroll
       for (int j = 0; j < kMeanBatch; ++j)
         if (d0 + j < a.n_dir) {
-          mv[j] = a.wg_mean[c * a.n_dir + d0 + j];
-          nv[j] = a.wg_mean_n[c * a.n_dir + d0 + j];
+          if (c == c0 && d0 == 0) {
+            mv[j] = mv0[j];
+            nv[j] = nv0[j];
+          } else {
+            mv[j] = a.wg_mean[c * a.n_dir + d0 + j];
+            nv[j] = a.wg_mean_n[c * a.n_dir + d0 + j];
+          }
         }
 #pragma unroll
       for (int j = 0; j < kMeanBatch; ++j)
