@@ -2356,8 +2356,13 @@ __device__ __forceinline__ float2 LoadGlobalF2(const float2* p) {
 }
 
 // The apply: X and Y of one (block, segment, 64-baseline tile), lane =
-// baseline; wave w takes the target directions d = w, w + waves, ...
-// The gains are staged in LDS as the correlate stages them.
+// baseline; one wave per target direction d (n_dir waves). Each stored block
+// G_e,d (e < d) serves two waves, d's and e's: the waves take their sources e
+// in the rounds of a round-robin pairing (round r pairs a and b with
+// a + b = 2r mod (m - 1) for the m = n_dir rounded up to even players, the
+// last player with r), so both readers of a block load it in the same round
+// and the second load hits in cache. The gains are staged in LDS as the
+// correlate stages them.
 __device__ __forceinline__ void FullJonesApplyGramTile(const IterArgs& a, const int tile,
                                                        const int seg, const int cb) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -2393,7 +2398,7 @@ __device__ __forceinline__ void FullJonesApplyGramTile(const IterArgs& a, const 
   const size_t base = (size_t(cb) * a.n_seg + seg) * FullJonesGramTerms(nd);
   const float2* gram = a.gram + bl;
   const GlobalT<float>* prod = AsGlobal(static_cast<const float*>(a.fj_products)) + bl;
-  for (int d = wave; d < nd; d += n_waves) {
+  for (int d = wave; d < nd; d += n_waves) {  // one pass (n_waves = n_dir)
     const LaneGains ja{g_s + size_t(d) * 8 * kWave + lane};
     const LaneGains jb{ja.p + 4 * kWave};
     float2 x[4], y[4];
@@ -2420,7 +2425,19 @@ __device__ __forceinline__ void FullJonesApplyGramTile(const IterArgs& a, const 
           y[2 * i + k] = yv;
         }
     }
-    for (int e = 0; e < nd; ++e) {
+    const int m = (nd + 1) & ~1;  // players of the pairing (a bye for odd n_dir)
+    for (int round = -1; round < m - 1; ++round) {
+      // round -1: e = d (Q_d); then d's partner of the round
+      int e = d;
+      if (round >= 0) {
+        if (d == m - 1)
+          e = round;
+        else if (d == round)
+          e = m - 1;
+        else
+          e = (2 * round - d + 2 * (m - 1)) % (m - 1);
+        if (e >= nd) continue;  // the bye
+      }
       // G_e,d[p][q] for p, q = 0..3
       float2 g[16];
       if (e < d) {
@@ -2509,14 +2526,17 @@ __device__ __forceinline__ void FullJonesApplyGramTile(const IterArgs& a, const 
   }
 }
 
-// Single solve: grid (bl tiles, segments, blocks).
-__global__ void __launch_bounds__(256) FullJonesApplyGramKernel(const IterArgs a) {
+// Single solve: grid (bl tiles, segments, blocks), n_dir waves.
+__global__ void __launch_bounds__(kMaxGramDirections * kWave)
+    FullJonesApplyGramKernel(const IterArgs a) {
   if (AsConst(a.state)->done) return;
   FullJonesApplyGramTile(a, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-// Batch: one workgroup per (interval, block, segment, tile) entry.
-__global__ void __launch_bounds__(256)
+// Batch: one workgroup per (interval, block, segment, tile) entry, launched
+// with the batch's largest n_dir waves; the waves past an interval's
+// direction count stage gains and wait at the barrier only.
+__global__ void __launch_bounds__(kMaxGramDirections * kWave)
     BatchFullJonesApplyGramKernel(const IterArgs* __restrict__ args,
                                   const BatchTile* __restrict__ tiles) {
   const BatchTile t = LoadConst(tiles + blockIdx.x);
@@ -3323,9 +3343,9 @@ void LaunchFullJonesModelProducts(const IterArgs& a, bool f16, int waves, hipStr
 }
 
 namespace {
-// The full-Jones Gram apply's workgroup: up to four waves (target directions
-// dealt out over them), the gains of every direction in LDS.
-int FjApplyWaves(int n_dir) { return std::min(n_dir, 4); }
+// The full-Jones Gram apply's workgroup: one wave per target direction, the
+// gains of every direction in LDS.
+int FjApplyWaves(int n_dir) { return n_dir; }
 size_t FjApplySharedBytes(int n_dir) { return size_t(n_dir) * 8 * kWave * sizeof(float2); }
 }  // namespace
 
