@@ -1799,7 +1799,8 @@ void SolveBatch(ddh_batch& b, double* const* const* solutions, ddh_result* resul
 //   next1 = one undamped update of g0 (with its denominators),
 //   ga    = `stretch` damped iterations from g0, gb = `stretch` more from ga,
 //   x     = features(g0, next1, ga, gb, den), pred = network(x),
-//   start = g0 * exp(pred),
+//   start = gb * exp(pred) (the network predicts the correction left after
+//           the stretches, so their progress is kept),
 // then the solve from `start`. Matches the per-interval host sequence
 // ddh_one_step, two ddh_solve_problem calls of `stretch` iterations at
 // accuracy 0 without stall detection, ddh_warmstart_features,
@@ -1894,8 +1895,9 @@ void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* cons
                      b.d_warm_spans.ptr, n, max_count / 2, x, b.stream);
   MlpForwardRows(net, x, rows, b.warm_y.ptr, b.stream);
   const float* pred = b.warm_y.ptr;
-  // the starts, through warm_a (no longer needed), into every sol0
-  LaunchWarmApply(b.warm_g0.ptr, pred, b.d_warm_spans.ptr, n, max_count / 2, b.warm_a.ptr,
+  // the starts from the second stretch's iterates, through warm_a (no longer
+  // needed), into every sol0
+  LaunchWarmApply(b.warm_b.ptr, pred, b.d_warm_spans.ptr, n, max_count / 2, b.warm_a.ptr,
                   b.stream);
   move(kMoveToSol0, b.warm_a.ptr);
   DDH_HIP_CHECK(hipEventRecord(b.ev_warm1, b.stream));

@@ -37,8 +37,10 @@ def sibling(solver: H.Solver, max_iterations, accuracy, detect_stalling=True, it
 class WarmStart:
     """Features: the undamped one-step update of the previous solutions g0 on
     the current interval and two iterates of the damped solver (after
-    `stretch` and 2 * `stretch` iterations), all relative to g0, plus the
-    update's denominators. Output: log(g_start / g0) per polarization."""
+    `stretch` and 2 * `stretch` iterations: ga, gb), all relative to g0, plus
+    the update's denominators. Output: log(g_start / gb) per polarization,
+    the correction left after the two stretches (the start keeps their
+    progress)."""
 
     N_IN = H.N_WARM_FEATURES
     N_OUT = H.N_WARM_OUTPUTS
@@ -55,21 +57,24 @@ class WarmStart:
             self._aux[key] = (solver, sibling(solver, self.stretch, 0.0, False))
         return self._aux[key][1]
 
-    def features(self, solver, dproblem, g0):
-        """Features [n_pairs, 16] of the current interval."""
+    def features(self, solver, dproblem, g0, return_stretch=False):
+        """Features [n_pairs, 16] of the current interval (and gb, the
+        iterate the start is formed from, with return_stretch)."""
         nxt, den = H.one_step(solver, dproblem, g0)
         aux = self._aux_solver(solver)
         ga = np.array(g0, np.complex128, copy=True)
         aux.solve_problem(dproblem, ga)
         gb = ga.copy()
         aux.solve_problem(dproblem, gb)
-        return H.warmstart_features(g0, nxt, ga, gb, den, solver.n_channel_blocks,
-                                    solver.n_antennas, solver.n_sub)
+        x = H.warmstart_features(g0, nxt, ga, gb, den, solver.n_channel_blocks,
+                                 solver.n_antennas, solver.n_sub)
+        return (x, gb) if return_stretch else x
 
     @staticmethod
-    def targets(g0, final):
-        """log(final / g0) per polarization as (log-magnitude, phase) pairs."""
-        p = np.asarray(g0, np.complex128).reshape(-1, 2)
+    def targets(base, final):
+        """log(final / base) per polarization as (log-magnitude, phase) pairs
+        (base: the second stretch's iterate gb)."""
+        p = np.asarray(base, np.complex128).reshape(-1, 2)
         f = np.asarray(final, np.complex128).reshape(-1, 2)
         with np.errstate(all="ignore"):
             r = f / p
@@ -84,9 +89,9 @@ class WarmStart:
         return loss
 
     def initial(self, solver, dproblem, g0):
-        x = self.features(solver, dproblem, g0)
+        x, gb = self.features(solver, dproblem, g0, return_stretch=True)
         pred = self.net.forward(x)
-        return H.warmstart_apply(g0, pred).reshape(np.asarray(g0).shape)
+        return H.warmstart_apply(gb, pred).reshape(np.asarray(g0).shape)
 
 
 def drifting_gains(w: S.Workload, rng, n_intervals, sigma_phase=0.05, sigma_amp=0.005):
@@ -198,10 +203,11 @@ def training_set(seq: IntervalSequence, ws: WarmStart, first, last, converge_ite
             long.solve_problem(dp, sol)
             prev = sol
             continue
-        xs.append(ws.features(seq.solver, dp, prev))
+        x, gb = ws.features(seq.solver, dp, prev, return_stretch=True)
+        xs.append(x)
         sol = prev.copy()
         long.solve_problem(dp, sol)
-        ts.append(ws.targets(prev, sol))
+        ts.append(ws.targets(gb, sol))
         prev = sol
     return np.concatenate(xs), np.concatenate(ts), prev
 
@@ -225,12 +231,15 @@ def training_sets_batched(seqs, ws: WarmStart, first, last, converge_iterations=
     ts = [[] for _ in seqs]
     for i in range(first + 1, last):
         dps = [q.transient(i) for q in seqs]  # same scratch problems, refilled
+        gbs = []
         for k, q in enumerate(seqs):
-            xs[k].append(ws.features(q.solver, dps[k], prev[k]))
+            x, gb = ws.features(q.solver, dps[k], prev[k], return_stretch=True)
+            xs[k].append(x)
+            gbs.append(gb)
         sol = [p.copy() for p in prev]
         batch.solve(sol)
         for k in range(len(seqs)):
-            ts[k].append(ws.targets(prev[k], sol[k]))
+            ts[k].append(ws.targets(gbs[k], sol[k]))
         prev = sol
     batch.close()
     for s in longs:

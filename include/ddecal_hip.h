@@ -350,7 +350,9 @@ int ddh_batch_last_time(const ddh_batch* batch, double* ms);
 // final solutions on return. From g0 the batch runs one undamped update
 // (with its denominators) and two stretches of `stretch` damped iterations,
 // turns them into features, runs the network (ddh_mlp, 16 -> 4) and starts
-// the solve from g0 * exp(output). Same arithmetic as the per-interval
+// the solve from gb * exp(output), gb the iterate after the second stretch
+// (the network predicts the remaining correction, so the stretches' progress
+// is kept). Same arithmetic as the per-interval
 // sequence ddh_one_step, 2 x ddh_solve_problem(max `stretch`, accuracy 0,
 // no stall detection), ddh_warmstart_features, ddh_mlp_forward and
 // ddh_warmstart_apply. The network must not be in use on another stream.
@@ -446,8 +448,8 @@ int ddh_gemm_bf16(const float* a, const float* b, float* c, int32_t M, int32_t N
 // Features per solution pair (16 floats) on the current interval: the
 // undamped one-step update of the previous solutions g0 (next1, with its
 // denominators den) and two later iterates ga, gb of the damped solver, each
-// relative to g0; and the initial solutions g0 * exp(pred) from the
-// network's 4 outputs per pair.
+// relative to g0; and the initial solutions prev * exp(pred) from the
+// network's 4 outputs per pair (the batched warm start passes prev = gb).
 int ddh_warmstart_features(const double* g0, const double* next1, const double* ga,
                            const double* gb, const double* den, int32_t n_cb, int32_t n_ant,
                            int32_t n_sub, float* features);
