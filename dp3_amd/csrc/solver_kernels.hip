@@ -711,8 +711,13 @@ __device__ void BlockSum(double& v, int& c, double* red, int* redc) {
   }
 }
 
+// The FP64 finishing arithmetic below is written without fused multiply-adds
+// (contraction off), so every kernel that inlines it (single, batched, fused)
+// rounds the same way and their solutions agree bit for bit.
+
 // a / b, scaled by the larger part of b (Smith's algorithm).
 __device__ __forceinline__ double2 CDiv(double2 a, double2 b) {
+#pragma clang fp contract(off)
   if (fabs(b.y) <= fabs(b.x)) {
     const double r = b.y / b.x, den = b.x + b.y * r;
     return make_double2((a.x + a.y * r) / den, (a.y - a.x * r) / den);
@@ -730,6 +735,7 @@ __device__ __forceinline__ double2 CDiv(double2 a, double2 b) {
 __device__ __forceinline__ void AddPairTerms(const double2 (&old)[2], const double2 (&next)[2],
                                              bool scalar, double& conv_sum, int& conv_n,
                                              double& mean_sum, int& mean_n) {
+#pragma clang fp contract(off)
   if (scalar) {
     if (old[0].x != 0.0 || old[0].y != 0.0) {
       const double2 q =
@@ -746,12 +752,16 @@ __device__ __forceinline__ void AddPairTerms(const double2 (&old)[2], const doub
     }
     return;
   }
-  const double n0 = cnorm(old[0]), n1 = cnorm(old[1]);
+  // (the complex products written out here, so that they are not contracted)
+  const double n0 = old[0].x * old[0].x + old[0].y * old[0].y;
+  const double n1 = old[1].x * old[1].x + old[1].y * old[1].y;
   const double2 inv0 = make_double2(old[0].x / n0, -old[0].y / n0);
   const double2 inv1 = make_double2(old[1].x / n1, -old[1].y / n1);
   if (cfinite(inv0) && cfinite(inv1)) {
-    const double2 d0 = cmul(make_double2(next[0].x - old[0].x, next[0].y - old[0].y), inv0);
-    const double2 d1 = cmul(make_double2(next[1].x - old[1].x, next[1].y - old[1].y), inv1);
+    const double2 e0 = make_double2(next[0].x - old[0].x, next[0].y - old[0].y);
+    const double2 e1 = make_double2(next[1].x - old[1].x, next[1].y - old[1].y);
+    const double2 d0 = make_double2(e0.x * inv0.x - e0.y * inv0.y, e0.x * inv0.y + e0.y * inv0.x);
+    const double2 d1 = make_double2(e1.x * inv1.x - e1.y * inv1.y, e1.x * inv1.y + e1.y * inv1.x);
     const double sumabs = hypot(d0.x, d0.y) + hypot(d1.x, d1.y);
     if (isfinite(sumabs)) {
       conv_sum += sumabs;
@@ -777,6 +787,7 @@ __device__ __forceinline__ void FinishSolution(const IterArgs& a, const double2 
                                                double (&nr)[2], double (&ni)[2], double (&dn)[2],
                                                double& conv_sum, int& conv_n, double& mean_sum,
                                                int& mean_n, double2 (&out)[2]) {
+#pragma clang fp contract(off)
   if (a.scalar) {
     nr[0] = nr[1] = nr[0] + nr[1];
     ni[0] = ni[1] = ni[0] + ni[1];
