@@ -37,10 +37,11 @@ converged chi^2 (a 1000-iteration solve at tolerance 1e-9 along the chain).
                   iterations -> features -> 3-layer 256-hidden bf16 MLP),
                   inside the timed region, and the fused Gram iteration
                   (per-baseline data/model cross products made once per
-                  solve; each iteration one kernel finishes the previous
-                  iteration's solutions and forms every antenna's numerator
-                  and denominator from the cross products and the new gains,
-                  then the decision). `value` is this form.
+                  solve; each iteration is one kernel that takes the
+                  convergence decision on the previous iteration, finishes its
+                  solutions and forms every antenna's numerator and
+                  denominator from the cross products and the new gains).
+                  `value` is this form.
 Also reported: the optimized form with the streaming iteration, the
 optimized form without the network (previous solutions), the plain form at
 the 100-iteration limit and previous-solution starts (DP3's
@@ -49,14 +50,15 @@ propagatesolutions) unbatched in complex64.
 `roofline` is for the dominant kernel of the optimized form, the fused Gram
 iteration kernel (algorithmic bytes: the per-baseline cross products it
 reads, the previous iteration's per-workgroup sums read once per block and
-this iteration's written, the solutions read and written); its mean duration comes from HIP events on the batch stream around
-each of its launches in a fixed-length (50-iteration, accuracy 0) solve of
-every stream's interval in one batch, alone on the GPU and autotuned for
-that size, in which every interval is live in every launch (the timed forms
-run the same kernels over each concurrent batch's intervals). `roofline_other`
-holds the rest of that iteration (the decision), the
-unfused apply + antenna solve it replaces, and the batched FP16 correlate of
-the streaming form; `roofline_plain` the complex64 correlate of the plain
+this iteration's written, the solutions read and written, the convergence
+partials the decision reads); its mean duration comes from HIP events on
+the batch stream around each of its launches in a fixed-length
+(50-iteration, accuracy 0) solve of every stream's interval in one batch,
+alone on the GPU and autotuned for that size, in which every interval is
+live in every launch (the timed forms run the same kernels over each
+concurrent batch's intervals). `roofline_other` holds the unfused apply +
+antenna solve it replaces, and the batched FP16 correlate of the streaming
+form; `roofline_plain` the complex64 correlate of the plain
 form. `traffic` is read from the committed rocprofv3 --pmc summary of the
 same command, when present. `cpu_baseline` runs the CPU oracle (a
 restatement of DP3's IterativeDiagonalSolver, threads over channel blocks)
@@ -261,8 +263,8 @@ def main():
         """Mean of the first and second timed part of a batched iteration over
         a fixed-length profiled solve of all streams' intervals in one batch,
         alone on the GPU and autotuned for that size, in which every interval
-        is live in every launch: (fused kernel, decision) for the fused Gram
-        iteration, (apply or correlate, antenna solve +
+        is live in every launch: (fused kernel, 0: the kernel decides) for
+        the fused Gram iteration, (apply or correlate, antenna solve +
         decision) otherwise. The timed forms run the same kernels over the
         intervals of each of their concurrent batches."""
         solvers = [WS.sibling(q.solver, ROOF_ITERATIONS, 0.0, False, iteration=iteration)
@@ -286,10 +288,20 @@ def main():
             solvers = [WS.sibling(s16[j].solver, MAX_ITERATIONS, TOLERANCE, iteration=iteration)
                        for j in split[k]]
             batches.append(H.Batch(solvers, [s16[j].problems[first] for j in split[k]]))
-        # one schedule for all batches (they have the same shapes): tuned on the first
-        tuned = batches[0].autotune()
-        groups = batches[0].gram_solve_groups()
-        solve_schedule = batches[0].antenna_solve_schedule()
+        # One schedule for all batches (they have the same shapes), tuned on one
+        # batch of every stream's interval: the concurrent batches' launches
+        # share the GPU, so a launch over all intervals is the load the
+        # schedule meets (a batch tuned alone picks more row groups, which
+        # are slower when the batches run together).
+        tsolvers = [WS.sibling(s16[j].solver, MAX_ITERATIONS, TOLERANCE, iteration=iteration)
+                    for j in range(B)]
+        tb = H.Batch(tsolvers, [s16[j].problems[first] for j in range(B)])
+        tuned = tb.autotune()
+        groups = tb.gram_solve_groups()
+        solve_schedule = tb.antenna_solve_schedule()
+        tb.close()
+        for ts in tsolvers:
+            ts.close()
         for b in batches:
             b.set_schedule(tuned[0], tuned[1], GRAPH, 8)
             b.set_antenna_solve_schedule(*solve_schedule)
@@ -405,25 +417,24 @@ def main():
         out.update(extra)
         return out
 
-    g_solve, g_rest, roof_groups = opt["gram"]["kernels_ms"]
+    g_solve, _, roof_groups = opt["gram"]["kernels_ms"]  # the merged kernel decides
     s_corr, s_solve, _ = opt["streaming"]["kernels_ms"]
     NB = B  # the roofline solves batch every stream's interval
     rooflines = {
         "gram_solve": roof(
             "gram_solve",
-            NB * (walk_bytes + 2 * sums_bytes(roof_groups) + 2 * solution_bytes), g_solve,
-            note=("fused Gram iteration kernel over the batch: the previous iteration finished "
-                  "(its per-antenna FP64 sums read once per block, solutions read and written; "
-                  "every workgroup of a block repeats this step from cache), then the "
+            NB * (walk_bytes + 2 * sums_bytes(roof_groups) + 2 * solution_bytes +
+                  w.n_channel_blocks * w.n_directions * 24), g_solve,
+            note=("fused Gram iteration kernel over the batch, one launch per iteration: the "
+                  "convergence decision on the previous iteration (its partials read; every "
+                  "workgroup decides, the last to arrive stores the state), that iteration "
+                  "finished (its per-antenna FP64 sums read once per block, solutions read and "
+                  "written; every workgroup of a block repeats this step from cache), then the "
                   "per-baseline cross products in walk order read once (C_d, G_e,d for e < d, "
                   f"and the real diagonals G_d,d two per slot: {nq_walk} x 8 bytes per "
                   "baseline and correlation) and the per-antenna FP64 sums written "
                   f"({roof_groups} row groups per polarization); one launch over all {B} "
                   "streams' intervals, alone on the GPU")),
-        "gram_decide": roof(
-            "gram_decide", NB * w.n_channel_blocks * w.n_directions * 24, g_rest,
-            note="decision kernel (convergence sums and repair means over the blocks, "
-                 "stopping rule); latency-bound"),
         "unfused_gram_apply": roof(
             "gram_apply", NB * (cross_bytes * bl_stride // w.n_baselines + apply_partial_bytes),
             unfused[0], note="the apply kernel the fused iteration replaces: cross products "
@@ -730,7 +741,6 @@ def main():
                                            "the GPU, 50 fixed iterations, HIP events"),
             "per_batched_iteration_us": {
                 "gram_solve": round(g_solve * 1e3, 1),
-                "gram_decide": round(g_rest * 1e3, 1),
                 "unfused_gram_apply": round(unfused[0] * 1e3, 1),
                 "unfused_gram_antenna_solve_decide": round(unfused[1] * 1e3, 1),
                 "streaming_correlate": round(s_corr * 1e3, 1),

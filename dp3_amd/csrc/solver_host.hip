@@ -185,6 +185,7 @@ struct ddh_solver {
   ddh::DeviceBuffer<double> gs_part;     // fused iteration: per-workgroup sums
   ddh::DeviceBuffer<double> mean, wg_conv, wg_mean, step_mags;
   ddh::DeviceBuffer<int32_t> wg_conv_n, wg_mean_n;
+  ddh::DeviceBuffer<int32_t> arrive;  // merged fused iteration: arrival count per launch
   ddh::DeviceBuffer<float> partials;
   ddh::DeviceBuffer<float> fj_products;  // full-Jones model products (once per solve)
   ddh::DeviceBuffer<ddh::DeviceState> state;
@@ -571,6 +572,7 @@ IterArgs MakeArgs(ddh_solver& s, ddh_problem& p, int fused = -1) {
   a.wg_conv_n = s.wg_conv_n.ptr;
   a.wg_mean = s.wg_mean.ptr;
   a.wg_mean_n = s.wg_mean_n.ptr;
+  a.arrive = s.arrive.ptr;
   a.partials = s.partials.ptr;
   a.state = s.state.ptr;
   a.step_mags = s.step_mags.ptr;
@@ -658,8 +660,9 @@ void LaunchSolvePrologue(const ddh_solver& s, const IterArgs& a, const ddh_probl
 }
 
 // Correlate + antenna solve of one iteration (the fused Gram iteration: one
-// kernel for both; merged: the one kernel finishes the previous iteration and
-// forms this one's sums, after the solve prologue formed the first sums).
+// kernel for both; merged: the one kernel decides the previous iteration,
+// finishes the next one and forms the following sums, after the solve
+// prologue formed the first sums).
 void LaunchCorrelateAndSolve(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
   if (a.walk_rows > 0) {
     LaunchGramSolve(a, p.f16, s.stream,
@@ -672,8 +675,16 @@ void LaunchCorrelateAndSolve(ddh_solver& s, const IterArgs& a, const ddh_problem
 
 void LaunchIteration(ddh_solver& s, const IterArgs& a, const ddh_problem& p) {
   LaunchCorrelateAndSolve(s, a, p);
+  if (MergedGram(s, a)) return;  // the merged kernel decides
   LaunchSmoothing(a, s.stream);
   LaunchDecide(a, s.stream);
+}
+
+// Launches of the solve loop: the iteration limit, plus one for the merged
+// fused iteration, whose launch decides the iteration the previous one
+// finished (the last decision needs a launch of its own).
+int LoopLaunches(const ddh_solver& s, const IterArgs& a, int limit) {
+  return limit + (MergedGram(s, a) ? 1 : 0);
 }
 
 // Captures `poll` iterations into one graph; reused while the problem and
@@ -722,16 +733,19 @@ void PrepareBuffers(ddh_solver& s, const ddh_problem& p) {
                                     s.wg_conv.ptr,   s.wg_conv_n.ptr, s.wg_mean.ptr,
                                     s.wg_mean_n.ptr, s.step_mags.ptr, s.partials.ptr,
                                     s.state.ptr,     s.gram.ptr,      s.fj_products.ptr,
-                                    s.gram_walk.ptr, s.gs_part.ptr};
+                                    s.gram_walk.ptr, s.gs_part.ptr,  s.arrive.ptr};
   };
   const std::vector<const void*> before = addresses();
   s.sol0.Resize(n_sol);
   s.sol1.Resize(n_sol);
   s.mean.Resize(c.n_channel_blocks);
-  s.wg_conv.Resize(n_wg);
-  s.wg_conv_n.Resize(n_wg);
-  s.wg_mean.Resize(n_wg);
-  s.wg_mean_n.Resize(n_wg);
+  // two halves by iteration parity (the merged fused iteration decides one
+  // iteration while it finishes the next)
+  s.wg_conv.Resize(2 * n_wg);
+  s.wg_conv_n.Resize(2 * n_wg);
+  s.wg_mean.Resize(2 * n_wg);
+  s.wg_mean_n.Resize(2 * n_wg);
+  s.arrive.Resize(1);
   s.step_mags.Resize(size_t(std::max(c.max_iterations, c.min_iterations)) + 2);
   const size_t terms = c.n_solution_polarizations == 4 ? size_t(kFullJonesTerms)
                                                       : size_t(p.n_corr) * 3;
@@ -834,7 +848,7 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
     BulkIn(s, s.stage, s.sol0.ptr, solutions, s.stream);
   }
   const IterArgs a = MakeArgs(s, p);
-  const int limit = std::max(c.max_iterations, c.min_iterations);
+  const int limit = LoopLaunches(s, a, std::max(c.max_iterations, c.min_iterations));
   const bool graph = s.use_graph && !s.profile;
 
   hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -875,8 +889,10 @@ void Solve(ddh_solver& s, ddh_problem& p, double* const* solutions, ddh_result* 
           if (s.profile) DDH_HIP_CHECK(hipEventRecord(s.events[2 * it + 1], s.stream));
           LaunchAntennaSolve(a, p.n_corr, p.f16, s.solve_threads, s.stream);
         }
-        LaunchSmoothing(a, s.stream);
-        LaunchDecide(a, s.stream);
+        if (!MergedGram(s, a)) {  // the merged kernel decides
+          LaunchSmoothing(a, s.stream);
+          LaunchDecide(a, s.stream);
+        }
       }
       launched += k;
     }
@@ -1709,15 +1725,14 @@ void LaunchBatchCrossProducts(ddh_batch& b) {
                            b.stream);
 }
 
-// One iteration of the solve loop. Merged fused iteration: one kernel
-// finishes the previous iteration and forms this one's sums (the loop's first
-// sums come from LaunchBatchFirstSums), then the decision.
+// One iteration of the solve loop. Merged fused iteration: one kernel that
+// decides the iteration the previous launch finished, finishes the next one
+// and forms the following sums (the loop's first sums come from
+// LaunchBatchFirstSums).
 void LaunchBatchIteration(ddh_batch& b) {
   if (b.merged) {
     LaunchBatchGramSolve(b.d_args.ptr, b.d_gs_items.ptr, b.gs_launch.data(),
                          int(b.gs_launch.size()), b.f16, b.d_pro_items.ptr, 0, b.stream, true);
-    LaunchBatchSmoothDecide(b.d_args.ptr, b.d_items.ptr, int(b.items.size()),
-                            int(b.solvers.size()), b.smooth, b.stream);
     return;
   }
   LaunchBatchStep(b, b.d_args.ptr, int(b.items.size()), int(b.solvers.size()));
@@ -1911,10 +1926,13 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
               bool cross_products) {
   const int n = int(b.solvers.size());
   const bool graph = b.use_graph != 0 && !b.profile;
+  // merged fused iteration: one more launch, whose decision ends the solve at
+  // the iteration limit
+  const int limit = b.limit + (b.merged ? 1 : 0);
   if (graph) EnsureBatchGraph(b);
-  if (b.profile && int(b.events.size()) < 3 * b.limit) {
+  if (b.profile && int(b.events.size()) < 3 * limit) {
     for (hipEvent_t e : b.events) (void)hipEventDestroy(e);
-    b.events.assign(3 * b.limit, nullptr);
+    b.events.assign(3 * limit, nullptr);
     for (hipEvent_t& e : b.events) DDH_HIP_CHECK(hipEventCreate(&e));
   }
   hipEvent_t t0, t1;
@@ -1936,7 +1954,7 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
       DDH_HIP_CHECK(hipGraphLaunch(b.graph_exec, b.stream));
       launched += b.poll;
     } else {
-      const int k = std::min(b.poll, b.limit - launched);
+      const int k = std::min(b.poll, limit - launched);
       for (int i = 0; i < k; ++i) {
         const int it = launched + i;
         if (b.profile) {
@@ -1952,8 +1970,7 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
             LaunchBatchStep(b, b.d_args.ptr, 0, 0);
           DDH_HIP_CHECK(hipEventRecord(b.events[3 * it + 1], b.stream));
           if (b.merged) {
-            LaunchBatchSmoothDecide(b.d_args.ptr, b.d_items.ptr, int(b.items.size()),
-                                    int(b.solvers.size()), b.smooth, b.stream);
+            // nothing: the merged kernel decides
           } else if (b.fused) {
             // the finishing kernel alone (no sums groups), then the decision
             LaunchBatchGramSolve(b.d_args.ptr, b.d_gs_items.ptr, b.gs_launch.data(), 0, b.f16,
@@ -1984,7 +2001,7 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
   bool all_done = false;
   enqueue(slot);
   for (;;) {
-    const bool more = launched < b.limit;
+    const bool more = launched < limit;
     if (more) enqueue(slot ^ 1);
     DDH_HIP_CHECK(hipEventSynchronize(b.poll_ev[slot]));
     all_done = true;
@@ -2333,17 +2350,24 @@ int ddh_batch_autotune(ddh_batch* b, int32_t* waves_out, int32_t* rows_out, doub
     if (b->fused) {
       // Row groups of the fused iteration: the walk rows of a block and
       // polarization split over 1..8 workgroups. Timed on the cross products of
-      // the unit-gain state; a launch only rewrites the next solutions and the
-      // partials of the same iteration, so repeats leave the state unchanged.
+      // the unit-gain state; with separate finishing kernels a launch only
+      // rewrites the next solutions and the partials of the same iteration, so
+      // repeats leave the state unchanged (the merged kernel advances it from a
+      // fresh state per candidate).
       LaunchBatchCrossProducts(*b);
       double best_gs = 1e30;
       int bg = b->gs_groups;
       for (int groups : {1, 2, 4, 8}) {
         b->gs_groups = groups;
         BuildBatch(*b);
-        // merged: the first sums in this layout, then the kernel that finishes
-        // them and forms the next iteration's (into the other parity's sums)
-        LaunchBatchFirstSums(*b);
+        // merged: a fresh state and the first sums in this layout, then the
+        // kernel that decides, finishes and forms the next iteration's sums
+        // (repeats run a few iterations of the unit-gain solve)
+        if (b->merged) {
+          LaunchBatchPrologue(b->d_args.ptr, b->d_pro_items.ptr, int(b->pro_items.size()),
+                              b->stream);
+          LaunchBatchFirstSums(*b);
+        }
         const double ms = TimeLaunches(b->stream, 5, [&] {
           if (b->merged)
             LaunchBatchGramSolve(b->d_args.ptr, b->d_gs_items.ptr, b->gs_launch.data(),
@@ -2431,7 +2455,8 @@ int ddh_autotune(ddh_solver* s, ddh_problem* p, int32_t* waves_out, int32_t* row
     // The fused iteration's row groups, or the antenna solve schedule, timed
     // on the cross products (partials) of the unit-gain state: a launch
     // rewrites only the next solutions and the per-workgroup sums of the same
-    // iteration, so repeats leave the state unchanged.
+    // iteration, so repeats leave the state unchanged (the merged fused
+    // kernel advances it from a fresh state per candidate).
     IterArgs t = MakeArgs(*s, *p);
     if (t.walk_rows > 0) {
       LaunchSolvePrologue(*s, t, *p);
@@ -2441,10 +2466,13 @@ int ddh_autotune(ddh_solver* s, ddh_problem* p, int32_t* waves_out, int32_t* row
       for (int groups : {1, 2, 4, 8}) {
         IterArgs u = t;
         u.gs_groups = groups;
-        // merged: the first iteration's sums in this row-group layout, then
-        // the kernel that finishes it and forms the next sums (into the other
-        // parity's sums)
-        if (merged) LaunchGramSolve(u, p->f16, s->stream, kGramSolveOnly);
+        // merged: a fresh state and the first iteration's sums in this
+        // row-group layout, then the kernel that decides, finishes and forms
+        // the next sums (repeats run a few iterations of the unit-gain solve)
+        if (merged) {
+          LaunchPrologue(u, s->stream);
+          LaunchGramSolve(u, p->f16, s->stream, kGramSolveOnly);
+        }
         const double ms = TimeLaunches(s->stream, 5, [&] {
           LaunchGramSolve(u, p->f16, s->stream,
                           merged ? kGramFinishThenSolve : kGramSolveAndFinish);

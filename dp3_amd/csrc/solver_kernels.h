@@ -51,10 +51,11 @@ struct CbInfo {
 };
 
 struct DeviceState {
-  int32_t iteration;
+  int32_t iteration;  // iterations decided (the current solutions are sol[iteration & 1])
   int32_t done;
   int32_t has_converged;
-  int32_t pad;
+  int32_t pending;    // merged fused iteration: 1 when iteration `iteration` has been
+                      // finished (its solutions stored) but not yet decided
   double last_step;
   double n_var_count;
   double step_mean;
@@ -80,7 +81,9 @@ struct IterArgs {
   double2* sol0;             // [n_cb][n_ant][n_sub][2], current at even iterations
   double2* sol1;             // current at odd iterations
   double* mean;              // [n_cb] repair value of the current solutions
-  double* wg_conv;           // [n_cb][n_dir] per-workgroup convergence sums
+  double* wg_conv;           // [n_cb][n_dir] per-workgroup convergence sums (the merged
+                             // fused iteration uses two such halves by iteration parity,
+                             // likewise for the three arrays below)
   int32_t* wg_conv_n;
   double* wg_mean;           // [n_cb][n_dir] per-workgroup sums of |pol 0|
   int32_t* wg_mean_n;
@@ -126,6 +129,8 @@ struct IterArgs {
   const int32_t* walk_bl;
   int32_t walk_rows;
   int32_t gs_groups;  // row groups per polarization of the fused iteration (1..8)
+  int32_t* arrive;    // merged fused iteration: workgroups of the current launch that
+                      // have read the state (the last one stores the next state)
   double* gs_part;    // per-workgroup sums by iteration parity: two halves of
                       // n_cb x 2 x kGramSolveMaxGroups x n_dir x 3 x 64 doubles, each
                       // [n_cb][2 x gs_groups][n_dir][3][64] from its start

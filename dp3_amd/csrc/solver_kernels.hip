@@ -1031,6 +1031,45 @@ constexpr int WalkDiag(int d) {
   return ND + ND * (ND - 1) / 2 + d / 2;
 }
 
+// The decision after an iteration from its convergence sums (sum, count over
+// every (block, direction) in a fixed order) and the state before it: the
+// next state and the iteration's step magnitude. AssignSolutions
+// (SolverBase.cc:296-306); stall detection (:63-86); the stopping rule
+// (SolverBase.h:274-285). The only constraint the device applies is the
+// smoothness constraint, which is always satisfied (Constraint::Satisfied),
+// so the plain accuracy applies; the solve time is unused because smoothing
+// does not depend on it. Shared by the decision kernel and the merged fused
+// iteration, so both decide with the same arithmetic.
+__device__ __forceinline__ DeviceState DecideState(const IterArgs& a, DeviceState st, double sum,
+                                                   int count, double& step_magnitude) {
+#pragma clang fp contract(off)
+  const int it = st.iteration + 1;
+  step_magnitude = count == 0 ? 0.0 : sum / a.step_size / count;
+  const bool converged = step_magnitude <= a.accuracy;
+  bool stalled = false;
+  if (a.detect_stalling) {
+    if (it == 1) {
+      st.n_var_count = 1.0;
+      st.step_mean = step_magnitude;
+      st.step_var = 0.0;
+    } else {
+      st.n_var_count += 1.0;
+      const double d1 = step_magnitude - st.step_mean;
+      st.step_mean += d1 / st.n_var_count;
+      const double d2 = step_magnitude - st.step_mean;
+      st.step_var += d1 * d2;
+    }
+    if (it >= 30) stalled = st.step_mean < a.step_diff_sigma * sqrt(st.step_var / st.n_var_count);
+  }
+  const bool ready = it >= a.max_iterations || converged || stalled;
+  const bool stop = it >= a.min_iterations && ready;
+  st.iteration = it;
+  st.has_converged = converged ? 1 : 0;
+  st.last_step = step_magnitude;
+  st.done = stop ? 1 : 0;
+  return st;
+}
+
 // FP64 add to an LDS word (ds_add_f64, no value returned).
 __device__ __forceinline__ void LdsAdd(double* p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1085,17 +1124,27 @@ __device__ __forceinline__ void AddGroupSums(const double (&v)[NE][2][2][3], con
         }
 }
 
-// Fused Gram iteration kernel body: the iteration the state names (FIN
-// false), or (FIN true) the end of that iteration followed by the next one.
-// With FIN every workgroup of a block first finishes all of the block's
-// solutions from the stored sums (the same sums in the same order, and the
-// same FinishSolution, as GramFinishBlock: the same values) and keeps its
-// share in registers; the block's first workgroup stores them with the
-// convergence and repair-mean partials the decision reads. The repair mean
-// of the new solutions is formed as the decision forms it (the directions'
-// |pol 0| sums in direction order), so the next iteration's gains are those
-// the unmerged sequence would stage. The decision then follows the kernel:
-// an interval it stops ran one more iteration's sums, which nothing reads.
+// Fused Gram iteration kernel body: the sums of the iteration the state
+// names (FIN false; a finishing kernel and the decision follow), or (FIN
+// true) one launch per iteration that decides, finishes and forms sums:
+//  1. the decision of the iteration finished by the previous launch (state
+//     `pending`), from its convergence partials (the same sums in the same
+//     order, and the same DecideState, as DecideBlock); every workgroup of
+//     the interval takes it, and the last workgroup to have read the state
+//     (an arrival count; each arrives after its own read) stores the next
+//     state. A stopping decision ends the launch: the current solutions are
+//     those the previous launch stored.
+//  2. the end of the next iteration f: every workgroup of a block finishes
+//     all of the block's solutions from the stored sums (the same sums in the
+//     same order, and the same FinishSolution, as GramFinishBlock) and keeps
+//     its share in registers; the block's first workgroup stores them with
+//     the convergence and repair-mean partials of f (their own parity half).
+//     The repair means of f's current and new solutions are formed as the
+//     decision forms them (the directions' |pol 0| sums in direction order).
+//  3. the sums of iteration f + 1 from the new gains.
+// So the values are those of sums, finishing kernel and decision launched in
+// turn, bit for bit; an interval that stops has formed one more iteration's
+// sums, which nothing reads.
 template <int ND, bool F16, bool FIN>
 __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, const int part) {
   constexpr int NC = 2;
@@ -1113,10 +1162,13 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
   const int k = part / groups;  // polarization
   const int group = part - k * groups;
   const int n = a.n_ant;
-  const int it_state = AsConst(a.state)->iteration;
-  const int iteration = FIN ? it_state + 1 : it_state;  // the iteration solved here
+  const DeviceState st = LoadConst(a.state);
+  // FIN: the iteration finished here (the decided count, plus the pending one)
+  const int f = st.iteration + (FIN ? st.pending : 0);
+  const int iteration = FIN ? f + 1 : st.iteration;  // the iteration whose sums are formed
   const double2* cur = (iteration & 1) ? a.sol1 : a.sol0;
   const size_t gbase = size_t(cb) * n * a.n_sub;
+  [[maybe_unused]] const int n_wg = a.n_cb * ND;  // (block, direction) partials per half
 
   const int rows = a.walk_rows;
   const int per = (rows + groups - 1) / groups;
@@ -1140,8 +1192,10 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
                           dn[kGainEntries][2] = {};
   // FIN: the stored current values of the thread's (direction, antenna) pairs
   [[maybe_unused]] double2 fcv[kGainEntries][2];
+  [[maybe_unused]] double dsum = 0.0;
+  [[maybe_unused]] int dcount = 0;
   if constexpr (FIN) {
-    const double2* fcur = (it_state & 1) ? a.sol1 : a.sol0;
+    const double2* fcur = (f & 1) ? a.sol1 : a.sol0;
 #pragma unroll
     for (int u = 0; u < kGainEntries; ++u) {
       const int d = min(wave + u * kGramSolveWaves, ND - 1);
@@ -1150,7 +1204,7 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
       fcv[u][1] = LoadGlobal(fcur + pair * 2 + 1);
     }
     const GlobalT<double>* fsums =
-        AsGlobal(GramParts(a, it_state & 1)) + size_t(cb) * 2 * groups * ND * 3 * kWave;
+        AsGlobal(GramParts(a, f & 1)) + size_t(cb) * 2 * groups * ND * 3 * kWave;
     double fv[kGainEntries][2][2][3];
     // groups >= 1: the first pass unconditional, so that no path reaches the
     // row loads with these loads (or the current values) still in flight
@@ -1159,6 +1213,15 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
     for (int g0 = 2; g0 < groups; g0 += 2) {
       LoadGroupSums<ND, kGainEntries>(fsums, groups, g0, wave, lane, fv);
       AddGroupSums<kGainEntries>(fv, groups, g0, nr, ni, dn);
+    }
+    // the pending iteration's convergence partials (half st.iteration & 1),
+    // strided over the threads as the decision kernel's 256 threads take them
+    if (st.pending) {
+      const size_t h = size_t(st.iteration & 1) * n_wg;
+      for (int i = threadIdx.x; i < n_wg; i += kGramSolveThreads) {
+        dsum += AsGlobal(a.wg_conv)[h + i];
+        dcount += AsGlobal(a.wg_conv_n)[h + i];
+      }
     }
   } else {
 #pragma unroll
@@ -1191,8 +1254,48 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
   if constexpr (FIN) {
     __shared__ double s_msum[ND];
     __shared__ int s_mn[ND];
-    double2* fnxt = part == 0 ? ((it_state & 1) ? a.sol0 : a.sol1) : nullptr;
-    const double old_mean = AsConst(a.mean)[cb];
+    __shared__ double s_red[kGramSolveWaves];
+    __shared__ int s_redc[kGramSolveWaves];
+    if (st.pending) {
+      BlockSum(dsum, dcount, s_red, s_redc);
+      double step_magnitude;
+      DeviceState next = DecideState(a, st, dsum, dcount, step_magnitude);
+      const bool stop = next.done != 0;
+      next.pending = stop ? 0 : 1;  // f is finished below unless the solve stops
+      if (threadIdx.x == 0) {
+        const int arrived = atomicAdd(a.arrive, 1);
+        if (arrived == a.n_cb * 2 * groups - 1) {
+          *a.arrive = 0;
+          a.step_mags[st.iteration] = step_magnitude;
+          *a.state = next;
+        }
+      }
+      if (stop) return;
+    } else if (threadIdx.x == 0) {
+      // the first launch after the sums of iteration 0: nothing to decide
+      const int arrived = atomicAdd(a.arrive, 1);
+      if (arrived == a.n_cb * 2 * groups - 1) {
+        *a.arrive = 0;
+        DeviceState next = st;
+        next.pending = 1;
+        *a.state = next;
+      }
+    }
+    double2* fnxt = part == 0 ? ((f & 1) ? a.sol0 : a.sol1) : nullptr;
+    // the repair mean of f's current solutions: the prologue's for f = 0, else
+    // from iteration f - 1's partials as the decision forms it
+    double old_mean = AsConst(a.mean)[cb];
+    if (f > 0) {
+      const size_t h = size_t((f - 1) & 1) * n_wg + size_t(cb) * ND;
+      double ms = 0.0;
+      int mn = 0;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        ms += AsConst(a.wg_mean)[h + d];
+        mn += AsConst(a.wg_mean_n)[h + d];
+      }
+      old_mean = mn == 0 ? 1.0 : ms / mn;
+    }
 #pragma unroll
     for (int u = 0; u < kGainEntries; ++u) {
       const int d = wave + u * kGramSolveWaves;
@@ -1211,7 +1314,7 @@ __device__ __forceinline__ void GramSolveBlock(const IterArgs& a, const int cb, 
       }
       if (lane == 0) {
         if (part == 0) {
-          const int wg = cb * ND + d;
+          const size_t wg = size_t(f & 1) * n_wg + size_t(cb) * ND + d;
           a.wg_conv[wg] = conv_sum;
           a.wg_conv_n[wg] = conv_n;
           a.wg_mean[wg] = mean_sum;
@@ -1766,36 +1869,9 @@ __device__ __forceinline__ void DecideBlock(const IterArgs& a) {
     a.mean[c] = mn == 0 ? (a.full_jones ? -1.0 : 1.0) : ms / mn;
   }
   if (threadIdx.x == 0) {
-    const int it = st.iteration + 1;
-    // AssignSolutions (SolverBase.cc:296-306). The only constraint the device
-    // applies is the smoothness constraint, which is always satisfied
-    // (Constraint::Satisfied), so the plain accuracy applies; the solve time
-    // is unused because smoothing does not depend on it.
-    const double step_magnitude = count == 0 ? 0.0 : sum / a.step_size / count;
-    a.step_mags[it - 1] = step_magnitude;
-    const bool converged = step_magnitude <= a.accuracy;
-    bool stalled = false;
-    if (a.detect_stalling) {
-      if (it == 1) {
-        st.n_var_count = 1.0;
-        st.step_mean = step_magnitude;
-        st.step_var = 0.0;
-      } else {
-        st.n_var_count += 1.0;
-        const double d1 = step_magnitude - st.step_mean;
-        st.step_mean += d1 / st.n_var_count;
-        const double d2 = step_magnitude - st.step_mean;
-        st.step_var += d1 * d2;
-      }
-      if (it >= 30)
-        stalled = st.step_mean < a.step_diff_sigma * sqrt(st.step_var / st.n_var_count);
-    }
-    const bool ready = it >= a.max_iterations || converged || stalled;
-    const bool stop = it >= a.min_iterations && ready;
-    st.iteration = it;
-    st.has_converged = converged ? 1 : 0;
-    st.last_step = step_magnitude;
-    st.done = stop ? 1 : 0;
+    double step_magnitude;
+    st = DecideState(a, st, sum, count, step_magnitude);
+    a.step_mags[st.iteration - 1] = step_magnitude;
     *a.state = st;
   }
 }
@@ -1814,6 +1890,7 @@ __device__ __forceinline__ void PrologueBlock(const IterArgs& a, const int cb) {
   if (cb == 0 && threadIdx.x == 0) {
     DeviceState st = {};
     *a.state = st;
+    if (a.arrive) *a.arrive = 0;
   }
   const int n_pairs = a.n_ant * a.n_sub;
   const int n_pol = a.full_jones ? 4 : 2;
