@@ -263,3 +263,36 @@ def test_merged_finishing_step_batch_and_row_groups():
         print(f"groups {groups}: iterations {res[1][1]}")
     for q in seqs:
         q.solver.set_gram_fusion(1)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(max_iterations=7, accuracy=0.0, detect_stalling=False),      # the iteration limit
+    dict(max_iterations=40, min_iterations=25, accuracy=1e-2),         # converged before min
+    dict(max_iterations=300, accuracy=1e-12, detect_stalling=True),    # stall detection
+    dict(max_iterations=1, accuracy=0.0, detect_stalling=False),       # a single iteration
+], ids=["limit", "min_iterations", "stall", "one"])
+def test_merged_decision_stopping_rules(small, kw):
+    """The decision inside the merged kernel (fusion 1) against the decision
+    kernel (fusion 2) under each stopping rule: the same iteration count,
+    convergence flag, step magnitude and solutions, bit for bit."""
+    w, geom, sky, gains, P = small
+    print(f"case: merged decision {kw}", flush=True)
+    out = {}
+    for mode in (1, 2):
+        s = solver(P, mode, **kw)
+        s.set_schedule(waves_per_tile=2, rows_per_trip=1, use_graph=1, poll_interval=4)
+        g = P.unit_solutions()
+        r = s.solve(P, g)
+        out[mode] = (g, r)
+        s.close()
+    (g1, r1), (g2, r2) = out[1], out[2]
+    print(f"iterations {r1.iterations} / {r2.iterations}, converged {r1.has_converged}, "
+          f"last step {r1.last_step_magnitude:.3e}")
+    assert (r1.iterations, r1.has_converged, r1.n_step_magnitudes) == \
+        (r2.iterations, r2.has_converged, r2.n_step_magnitudes)
+    assert r1.last_step_magnitude == r2.last_step_magnitude
+    np.testing.assert_array_equal(g1, g2)
+    if kw["max_iterations"] in (1, 7) and kw["accuracy"] == 0.0:
+        assert r1.iterations == kw["max_iterations"] + 1  # max+1 when not converged
+    if "min_iterations" in kw:
+        assert r1.iterations >= kw["min_iterations"]
