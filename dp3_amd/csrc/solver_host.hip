@@ -600,6 +600,7 @@ IterArgs MakeArgs(ddh_solver& s, ddh_problem& p, int fused = -1) {
   a.max_nsd = max_nsd;
   a.scalar = c.n_solution_polarizations == 1 ? 1 : 0;
   a.full_jones = c.n_solution_polarizations == 4 ? 1 : 0;
+  a.solve_chunk = a.full_jones ? 0 : AntennaSolveChunk(a);
   a.fj_products = s.fj_products.ptr;
   if (s.sm_on) {
     a.sm_freq = s.sm_freq.ptr;

@@ -96,6 +96,8 @@ struct IterArgs {
   int32_t detect_stalling, phase_only;
   double* den_out;           // optional [n_cb][n_ant][n_sub][2] denominators
   int32_t split;             // threads per (antenna, solution) in antenna_solve
+  int32_t solve_chunk;       // baselines of partials per LDS stage in antenna_solve
+                             // (0: all of a block's; AntennaSolveChunk)
   int32_t max_nsd;           // largest n_sol_dir entry
   int32_t scalar;            // 1: one gain for both polarizations (scalar solver)
   int32_t full_jones;        // 1: 2x2 solutions [n_cb][n_ant][n_sub][4] (full-Jones
@@ -204,6 +206,10 @@ int Chi2Partials(const IterArgs& a);
 size_t CorrelateSharedBytes(int n_dir, int n_corr, int waves);
 size_t MaxSharedBytes();
 size_t AntennaSolveSharedBytes(const IterArgs& a, int n_corr);
+// Baselines per LDS stage of the antenna solve's partials: 0 (all) where a
+// block's fit the workgroup's LDS with the gains, else the largest multiple
+// of 64 that does (no limit on the station count beyond the gains' LDS).
+int AntennaSolveChunk(const IterArgs& a);
 // Block size of antenna_solve: about four threads per (antenna, solution).
 int AntennaSolveThreads(int n_ant, int max_nsd);
 // Antenna solve schedule: `split` threads per (antenna, solution) (a power of

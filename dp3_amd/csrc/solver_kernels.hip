@@ -864,12 +864,16 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
   const double sc2 = sc * sc;
   const int bs = a.bl_stride;
   // LDS: the gains of this direction's solutions as the correlate pass used
-  // them (repaired, then cast to float) and the partials of one segment. The
-  // per-antenna baseline lists are read from global memory (cached): each
-  // entry holds (baseline << 1 | side, other antenna).
+  // them (repaired, then cast to float) and the partials of one segment and
+  // correlation, `chunk` baselines at a time (all of them where they fit).
+  // The per-antenna baseline lists are read from global memory (cached):
+  // each entry holds (baseline << 1 | side, other antenna), in baseline
+  // order, so a thread's walk over its list is cut into chunks by a cursor
+  // and visits its entries in the same order whatever the chunk size.
   extern __shared__ double smem_d[];
   double2* g_s = reinterpret_cast<double2*>(smem_d);             // [n_ant][nsd][2]
-  float* part_s = reinterpret_cast<float*>(g_s + size_t(a.n_ant) * nsd * 2);  // [3][bs], one k
+  float* part_s = reinterpret_cast<float*>(g_s + size_t(a.n_ant) * nsd * 2);  // [3][chunk]
+  const int chunk = (a.solve_chunk > 0 && a.solve_chunk < bs) ? a.solve_chunk : bs;
   for (int i = threadIdx.x; i < a.n_ant * nsd; i += blockDim.x) {
     const int an = i / nsd;
     const size_t pair = gbase + size_t(an) * a.n_sub + s0 + (i - an * nsd);
@@ -888,10 +892,21 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
   auto gain = [&](int an, int pol) { return g_s[(an * nsd + rel) * 2 + pol]; };
 
   double nr[2] = {0.0, 0.0}, ni[2] = {0.0, 0.0}, dn[2] = {0.0, 0.0};
-  // the first segment's first correlation is staged under the gains' barrier
-  StageWords(a.partials + size_t(cb) * a.n_seg * a.n_dir * NC * 3 * size_t(bs) +
-                 size_t(d) * NC * 3 * size_t(bs),
-             part_s, 3 * bs);
+  // one correlation's partials [3][bs] (from src), baselines [c0, c0 + chunk)
+  auto stage = [&](const float* src, int c0) {
+    if (chunk == bs) {
+      StageWords(src, part_s, 3 * bs);
+    } else {
+      const int len = min(chunk, bs - c0);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) StageWords(src + size_t(r) * bs + c0, part_s + r * chunk, len);
+    }
+  };
+  // the first segment's first correlation (first chunk) is staged under the
+  // gains' barrier
+  stage(a.partials + size_t(cb) * a.n_seg * a.n_dir * NC * 3 * size_t(bs) +
+            size_t(d) * NC * 3 * size_t(bs),
+        0);
   __syncthreads();
   const double2 gant[2] = {active ? gain(ant, 0) : make_double2(0, 0),
                            active ? gain(ant, 1) : make_double2(0, 0)};
@@ -903,39 +918,43 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
         a.partials + ((size_t(cb) * a.n_seg + seg) * a.n_dir + d) * NC * 3 * size_t(bs);
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
-      if (seg > 0 || k > 0) {
-        __syncthreads();  // the previous stage has been read
-        StageWords(src + size_t(k) * 3 * bs, part_s, 3 * bs);
-        __syncthreads();
-      }
-      if (!active || s_seg != s_abs) continue;
-      const int i = PolI<NC>(k);
-      const int j = PolJ<NC>(k);
-      for (int e = e_begin + q; e < e_end; e += split) {
-        const int code = csr[2 * e];
-        const int other = csr[2 * e + 1];
-        const int b = code >> 1;
-        const int side = code & 1;
-        const double2 u = make_double2(sc * part_s[b], sc * part_s[bs + b]);
-        const double s = sc2 * part_s[2 * bs + b];
-        if (side == 0) {
-          // this antenna is a (first); the other is b
-          const double2 go = gain(other, j);
-          const double2 gg = cmulc(gant[i], go);
-          const double2 t = make_double2(u.x + gg.x * s, u.y + gg.y * s);
-          const double2 c = cmul(go, t);
-          nr[i] += c.x;
-          ni[i] += c.y;
-          dn[i] += cnorm(go) * s;
-        } else {
-          // this antenna is b (second); the other is a
-          const double2 go = gain(other, i);
-          const double2 gg = cmulc(go, gant[j]);
-          const double2 t = make_double2(u.x + gg.x * s, u.y + gg.y * s);
-          const double2 c = cmulc(go, t);  // G_a,i * conj(T)
-          nr[j] += c.x;
-          ni[j] += c.y;
-          dn[j] += cnorm(go) * s;
+      int e = e_begin + q;  // the thread's cursor in its baseline list
+      for (int c0 = 0; c0 < bs; c0 += chunk) {
+        if (seg > 0 || k > 0 || c0 > 0) {
+          __syncthreads();  // the previous stage has been read
+          stage(src + size_t(k) * 3 * bs, c0);
+          __syncthreads();
+        }
+        if (!active || s_seg != s_abs) continue;
+        const int i = PolI<NC>(k);
+        const int j = PolJ<NC>(k);
+        for (; e < e_end; e += split) {
+          const int code = csr[2 * e];
+          const int b = (code >> 1) - c0;
+          if (b >= chunk) break;  // in a later chunk
+          const int other = csr[2 * e + 1];
+          const int side = code & 1;
+          const double2 u = make_double2(sc * part_s[b], sc * part_s[chunk + b]);
+          const double s = sc2 * part_s[2 * chunk + b];
+          if (side == 0) {
+            // this antenna is a (first); the other is b
+            const double2 go = gain(other, j);
+            const double2 gg = cmulc(gant[i], go);
+            const double2 t = make_double2(u.x + gg.x * s, u.y + gg.y * s);
+            const double2 c = cmul(go, t);
+            nr[i] += c.x;
+            ni[i] += c.y;
+            dn[i] += cnorm(go) * s;
+          } else {
+            // this antenna is b (second); the other is a
+            const double2 go = gain(other, i);
+            const double2 gg = cmulc(go, gant[j]);
+            const double2 t = make_double2(u.x + gg.x * s, u.y + gg.y * s);
+            const double2 c = cmulc(go, t);  // G_a,i * conj(T)
+            nr[j] += c.x;
+            ni[j] += c.y;
+            dn[j] += cnorm(go) * s;
+          }
         }
       }
     }
@@ -3537,7 +3556,24 @@ void LaunchCorrelate(const IterArgs& a, int n_corr, bool f16, int waves, int row
 size_t AntennaSolveSharedBytes(const IterArgs& a, int n_corr) {
   (void)n_corr;  // one correlation's partials are staged at a time
   const int max_nsd = std::max(1, a.max_nsd);
-  return size_t(a.n_ant) * max_nsd * 2 * sizeof(double2) + size_t(3) * a.bl_stride * sizeof(float);
+  const int chunk = (a.solve_chunk > 0 && a.solve_chunk < a.bl_stride) ? a.solve_chunk
+                                                                      : a.bl_stride;
+  return size_t(a.n_ant) * max_nsd * 2 * sizeof(double2) + size_t(3) * chunk * sizeof(float);
+}
+
+int AntennaSolveChunk(const IterArgs& a) {
+  // every baseline of a block in one stage where that fits the workgroup's
+  // LDS; else stages of the largest multiple of 64 baselines that fits in
+  // 64 KB with the gains (two or more workgroups per compute unit), or in
+  // the whole LDS where the gains alone take most of that
+  const size_t gains = size_t(a.n_ant) * std::max(1, a.max_nsd) * 2 * sizeof(double2);
+  const size_t limit = size_t(g_max_lds_solve);
+  const size_t row = 3 * sizeof(float);  // one baseline's three partials
+  if (gains + a.bl_stride * row <= limit) return 0;
+  DDH_REQUIRE(gains + kWave * row <= limit, -4,
+              "too many antennas x solutions for the antenna solve LDS");
+  const size_t budget = gains + 1024 * row <= size_t(64) * 1024 ? size_t(64) * 1024 : limit;
+  return int((budget - gains) / row / kWave * kWave);
 }
 
 int AntennaSolveThreads(int n_ant, int max_nsd) {
