@@ -1498,6 +1498,7 @@ struct ddh_batch {
   bool full_jones = false;  // full-Jones intervals (all or none)
   bool fused = false;       // fused Gram iteration for every interval
   bool merged = false;      // its finishing step merged into the next iteration's kernel
+  bool solve_chunked = false;  // an interval's antenna solve stages its partials in chunks
   int gs_groups = 2;        // its row groups per polarization
   int max_dir = 0;
   // fused Gram iteration: items (interval, cb) grouped by direction count
@@ -1570,6 +1571,7 @@ void BuildBatch(ddh_batch& b) {
   b.limit = 0;
   b.solve_lds = 0;
   b.solve_threads = 0;
+  b.solve_chunked = false;
   struct Costed {
     BatchTile t;
     int n_dir;
@@ -1596,6 +1598,7 @@ void BuildBatch(ddh_batch& b) {
     b.args.push_back(a);
     b.limit = std::max(b.limit, std::max(s.cfg.c.max_iterations, s.cfg.c.min_iterations));
     b.solve_lds = std::max(b.solve_lds, AntennaSolveSharedBytes(a, p.n_corr));
+    if (a.solve_chunk > 0) b.solve_chunked = true;
     b.solve_threads = std::max(b.solve_threads, SolveThreads(a, b.solve_threads_req));
     const double vis_bytes = double(p.DataVisBytes() + p.n_dir * p.ModelVisBytes());
     for (int cb = 0; cb < p.n_cb; ++cb) {
@@ -1701,11 +1704,11 @@ void LaunchBatchStep(ddh_batch& b, const IterArgs* args, int n_items, int n_inte
     for (const IterArgs& a : b.args) apply_ant = std::max(apply_ant, a.n_ant);
     LaunchBatchApply(args, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), apply_ant,
                      b.d_items.ptr, n_items, n_intervals, b.solve_lds, b.solve_threads, b.f16,
-                     b.smooth, b.stream);
+                     b.smooth, b.stream, b.solve_chunked);
   } else {
     LaunchBatch(args, b.d_tiles.ptr, b.groups.data(), int(b.groups.size()), b.d_items.ptr,
                 n_items, n_intervals, b.solve_lds, b.solve_threads, b.n_corr, b.f16, b.waves,
-                b.unroll, b.smooth, b.stream);
+                b.unroll, b.smooth, b.stream, b.solve_chunked);
   }
 }
 
@@ -1986,7 +1989,7 @@ void RunBatch(ddh_batch& b, double* const* const* solutions, ddh_result* results
           else
             LaunchBatchTail(b.d_args.ptr, b.d_items.ptr, int(b.items.size()),
                             int(b.solvers.size()), b.solve_lds, b.solve_threads, b.n_corr, b.f16,
-                            b.smooth, b.stream);
+                            b.smooth, b.stream, b.solve_chunked);
           DDH_HIP_CHECK(hipEventRecord(b.events[3 * it + 2], b.stream));
         } else {
           LaunchBatchIteration(b);
@@ -2402,7 +2405,8 @@ int ddh_batch_autotune(ddh_batch* b, int32_t* waves_out, int32_t* rows_out, doub
         BuildBatch(*b);
         const double ms = TimeLaunches(b->stream, 5, [&] {
           LaunchBatchTail(b->d_args.ptr, b->d_items.ptr, int(b->items.size()), 0, b->solve_lds,
-                          b->solve_threads, b->n_corr, b->f16, SmoothLaunch{}, b->stream);
+                          b->solve_threads, b->n_corr, b->f16, SmoothLaunch{}, b->stream,
+                          b->solve_chunked);
         });
         if (ms < best_s) {
           best_s = ms;

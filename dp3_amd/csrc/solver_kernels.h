@@ -244,7 +244,7 @@ int SmoothFinishThreads(int n_ant, int max_nsd);
 void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
                  int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
                  size_t solve_lds, int solve_threads, int n_corr, bool f16, int waves, int rows,
-                 const SmoothLaunch& smooth, hipStream_t stream);
+                 const SmoothLaunch& smooth, hipStream_t stream, bool solve_chunked = false);
 // Gram iteration: the cross products once per solve, then per iteration the
 // correlate partials from them and the current gains.
 void LaunchGram(const IterArgs& a, bool f16, int waves, hipStream_t stream);
@@ -284,7 +284,7 @@ void LaunchBatchGram(const IterArgs* d_args, const BatchTile* d_tiles, const Til
 void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
                       int n_groups, int apply_ant, const BatchTile* d_items, int n_items,
                       int n_intervals, size_t solve_lds, int solve_threads, bool f16,
-                      const SmoothLaunch& smooth, hipStream_t stream);
+                      const SmoothLaunch& smooth, hipStream_t stream, bool solve_chunked = false);
 // Solution values of every batched interval between its solver buffers and
 // one concatenated buffer `buf` (interval i at d_off[i] .. d_off[i + 1],
 // complex values; max_count the largest), one launch:
@@ -298,7 +298,7 @@ void LaunchBatchPrologue(const IterArgs* d_args, const BatchTile* d_items, int n
                          hipStream_t stream);
 void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_items,
                      int n_intervals, size_t solve_lds, int solve_threads, int n_corr, bool f16,
-                     const SmoothLaunch& smooth, hipStream_t stream);
+                     const SmoothLaunch& smooth, hipStream_t stream, bool solve_chunked = false);
 // The smoothing passes (when any interval has the constraint) and the decision.
 void LaunchBatchSmoothDecide(const IterArgs* d_args, const BatchTile* d_items, int n_items,
                              int n_intervals, const SmoothLaunch& smooth, hipStream_t stream);

@@ -840,7 +840,9 @@ __device__ __forceinline__ void FinishSolution(const IterArgs& a, const double2 
 // convergence term (:254-268).
 // Solutions ping-pong between sol0 and sol1 by iteration parity.
 // ---------------------------------------------------------------------------
-template <int NC, bool F16>
+// CHUNKED: the partials staged `solve_chunk` baselines at a time (arrays
+// whose block does not fit one stage); otherwise all of a block's at once.
+template <int NC, bool F16, bool CHUNKED>
 __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d, const int cb) {
   __shared__ double red[2 * kBlockThreads / 64];
   __shared__ int redc[2 * kBlockThreads / 64];
@@ -873,7 +875,8 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
   extern __shared__ double smem_d[];
   double2* g_s = reinterpret_cast<double2*>(smem_d);             // [n_ant][nsd][2]
   float* part_s = reinterpret_cast<float*>(g_s + size_t(a.n_ant) * nsd * 2);  // [3][chunk]
-  const int chunk = (a.solve_chunk > 0 && a.solve_chunk < bs) ? a.solve_chunk : bs;
+  [[maybe_unused]] const int chunk =
+      (CHUNKED && a.solve_chunk > 0 && a.solve_chunk < bs) ? a.solve_chunk : bs;
   for (int i = threadIdx.x; i < a.n_ant * nsd; i += blockDim.x) {
     const int an = i / nsd;
     const size_t pair = gbase + size_t(an) * a.n_sub + s0 + (i - an * nsd);
@@ -892,50 +895,37 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
   auto gain = [&](int an, int pol) { return g_s[(an * nsd + rel) * 2 + pol]; };
 
   double nr[2] = {0.0, 0.0}, ni[2] = {0.0, 0.0}, dn[2] = {0.0, 0.0};
-  // one correlation's partials [3][bs] (from src), baselines [c0, c0 + chunk)
-  auto stage = [&](const float* src, int c0) {
-    if (chunk == bs) {
-      StageWords(src, part_s, 3 * bs);
-    } else {
-      const int len = min(chunk, bs - c0);
-#pragma unroll
-      for (int r = 0; r < 3; ++r) StageWords(src + size_t(r) * bs + c0, part_s + r * chunk, len);
-    }
-  };
-  // the first segment's first correlation (first chunk) is staged under the
-  // gains' barrier
-  stage(a.partials + size_t(cb) * a.n_seg * a.n_dir * NC * 3 * size_t(bs) +
-            size_t(d) * NC * 3 * size_t(bs),
-        0);
-  __syncthreads();
-  const double2 gant[2] = {active ? gain(ant, 0) : make_double2(0, 0),
-                           active ? gain(ant, 1) : make_double2(0, 0)};
-  // The partials are staged one correlation at a time (3 x bl_stride floats),
-  // which keeps the LDS per workgroup small enough for six workgroups per CU.
-  for (int seg = 0; seg < a.n_seg; ++seg) {
-    const int s_seg = AsConst(a.seg_sol)[seg * a.n_dir + d];
-    const float* src =
-        a.partials + ((size_t(cb) * a.n_seg + seg) * a.n_dir + d) * NC * 3 * size_t(bs);
-#pragma unroll
-    for (int k = 0; k < NC; ++k) {
-      int e = e_begin + q;  // the thread's cursor in its baseline list
-      for (int c0 = 0; c0 < bs; c0 += chunk) {
-        if (seg > 0 || k > 0 || c0 > 0) {
+  if constexpr (!CHUNKED) {
+    // the first segment's first correlation is staged under the gains' barrier
+    StageWords(a.partials + size_t(cb) * a.n_seg * a.n_dir * NC * 3 * size_t(bs) +
+                   size_t(d) * NC * 3 * size_t(bs),
+               part_s, 3 * bs);
+    __syncthreads();
+    const double2 gant[2] = {active ? gain(ant, 0) : make_double2(0, 0),
+                             active ? gain(ant, 1) : make_double2(0, 0)};
+    // The partials are staged one correlation at a time (3 x bl_stride floats),
+    // which keeps the LDS per workgroup small enough for six workgroups per CU.
+    for (int seg = 0; seg < a.n_seg; ++seg) {
+      const int s_seg = AsConst(a.seg_sol)[seg * a.n_dir + d];
+      const float* src =
+          a.partials + ((size_t(cb) * a.n_seg + seg) * a.n_dir + d) * NC * 3 * size_t(bs);
+  #pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        if (seg > 0 || k > 0) {
           __syncthreads();  // the previous stage has been read
-          stage(src + size_t(k) * 3 * bs, c0);
+          StageWords(src + size_t(k) * 3 * bs, part_s, 3 * bs);
           __syncthreads();
         }
         if (!active || s_seg != s_abs) continue;
         const int i = PolI<NC>(k);
         const int j = PolJ<NC>(k);
-        for (; e < e_end; e += split) {
+        for (int e = e_begin + q; e < e_end; e += split) {
           const int code = csr[2 * e];
-          const int b = (code >> 1) - c0;
-          if (b >= chunk) break;  // in a later chunk
           const int other = csr[2 * e + 1];
+          const int b = code >> 1;
           const int side = code & 1;
-          const double2 u = make_double2(sc * part_s[b], sc * part_s[chunk + b]);
-          const double s = sc2 * part_s[2 * chunk + b];
+          const double2 u = make_double2(sc * part_s[b], sc * part_s[bs + b]);
+          const double s = sc2 * part_s[2 * bs + b];
           if (side == 0) {
             // this antenna is a (first); the other is b
             const double2 go = gain(other, j);
@@ -954,6 +944,83 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
             nr[j] += c.x;
             ni[j] += c.y;
             dn[j] += cnorm(go) * s;
+          }
+        }
+      }
+    }
+  } else {
+    // one correlation's partials [3][bs] (from src), baselines [c0, c0 + chunk)
+    auto stage = [&](const float* src, int c0) {
+      if (chunk == bs) {
+        StageWords(src, part_s, 3 * bs);
+      } else {
+        const int len = min(chunk, bs - c0);
+  #pragma unroll
+        for (int r = 0; r < 3; ++r) StageWords(src + size_t(r) * bs + c0, part_s + r * chunk, len);
+      }
+    };
+    // the first segment's first correlation (first chunk) is staged under the
+    // gains' barrier
+    stage(a.partials + size_t(cb) * a.n_seg * a.n_dir * NC * 3 * size_t(bs) +
+              size_t(d) * NC * 3 * size_t(bs),
+          0);
+    __syncthreads();
+    const double2 gant[2] = {active ? gain(ant, 0) : make_double2(0, 0),
+                             active ? gain(ant, 1) : make_double2(0, 0)};
+    // The partials are staged one correlation at a time (3 x bl_stride floats),
+    // which keeps the LDS per workgroup small enough for six workgroups per CU.
+    for (int seg = 0; seg < a.n_seg; ++seg) {
+      const int s_seg = AsConst(a.seg_sol)[seg * a.n_dir + d];
+      const float* src =
+          a.partials + ((size_t(cb) * a.n_seg + seg) * a.n_dir + d) * NC * 3 * size_t(bs);
+  #pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        int e = e_begin + q;  // the thread's cursor in its baseline list
+        for (int c0 = 0; c0 < bs; c0 += chunk) {
+          if (seg > 0 || k > 0 || c0 > 0) {
+            __syncthreads();  // the previous stage has been read
+            stage(src + size_t(k) * 3 * bs, c0);
+            __syncthreads();
+          }
+          if (!active || s_seg != s_abs) continue;
+          const int i = PolI<NC>(k);
+          const int j = PolJ<NC>(k);
+          // one entry of the list: its baseline b (relative to the chunk)
+          auto add_edge = [&](int ei, int code) {
+            const int b = (code >> 1) - c0;
+            const int other = csr[2 * ei + 1];
+            const int side = code & 1;
+            const double2 u = make_double2(sc * part_s[b], sc * part_s[chunk + b]);
+            const double s = sc2 * part_s[2 * chunk + b];
+            if (side == 0) {
+              // this antenna is a (first); the other is b
+              const double2 go = gain(other, j);
+              const double2 gg = cmulc(gant[i], go);
+              const double2 t = make_double2(u.x + gg.x * s, u.y + gg.y * s);
+              const double2 c = cmul(go, t);
+              nr[i] += c.x;
+              ni[i] += c.y;
+              dn[i] += cnorm(go) * s;
+            } else {
+              // this antenna is b (second); the other is a
+              const double2 go = gain(other, i);
+              const double2 gg = cmulc(go, gant[j]);
+              const double2 t = make_double2(u.x + gg.x * s, u.y + gg.y * s);
+              const double2 c = cmulc(go, t);  // G_a,i * conj(T)
+              nr[j] += c.x;
+              ni[j] += c.y;
+              dn[j] += cnorm(go) * s;
+            }
+          };
+          if (chunk == bs) {
+            // one stage: the whole list, a loop without a data-dependent exit
+            for (; e < e_end; e += split) add_edge(e, csr[2 * e]);
+          } else {
+            for (; e < e_end; e += split) {
+              const int code = csr[2 * e];
+              if ((code >> 1) - c0 >= chunk) break;  // in a later chunk
+              add_edge(e, code);
+            }
           }
         }
       }
@@ -989,10 +1056,10 @@ __device__ __forceinline__ void AntennaSolveBlock(const IterArgs& a, const int d
 
 }
 
-template <int NC, bool F16>
+template <int NC, bool F16, bool CHUNKED>
 __global__ void __launch_bounds__(kBlockThreads) AntennaSolveKernel(const IterArgs a) {
   if (a.state->done) return;
-  AntennaSolveBlock<NC, F16>(a, blockIdx.x, blockIdx.y);
+  AntennaSolveBlock<NC, F16, CHUNKED>(a, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -2910,14 +2977,28 @@ __global__ void __launch_bounds__(kCorrelateMaxThreads)
   CorrelateTile<ND, NC, F16, ROWS>(a, t.tile, t.seg, t.cb);
 }
 
-template <int NC, bool F16>
+template <int NC, bool F16, bool CHUNKED>
 __global__ void __launch_bounds__(kBlockThreads)
     BatchAntennaSolveKernel(const IterArgs* __restrict__ args, const BatchTile* __restrict__ items) {
   const BatchTile t = items[blockIdx.x];
   const IterArgs& a = args[t.interval];
   if (a.state->done) return;
-  AntennaSolveBlock<NC, F16>(a, t.seg, t.cb);  // seg holds the direction
+  AntennaSolveBlock<NC, F16, CHUNKED>(a, t.seg, t.cb);  // seg holds the direction
 }
+
+using SolveKernelFn = void (*)(IterArgs);
+using BatchSolveKernelFn = void (*)(const IterArgs*, const BatchTile*);
+// [NC == 4][F16][CHUNKED]
+const SolveKernelFn kAntennaSolve[2][2][2] = {
+    {{AntennaSolveKernel<2, false, false>, AntennaSolveKernel<2, false, true>},
+     {AntennaSolveKernel<2, true, false>, AntennaSolveKernel<2, true, true>}},
+    {{AntennaSolveKernel<4, false, false>, AntennaSolveKernel<4, false, true>},
+     {AntennaSolveKernel<4, true, false>, AntennaSolveKernel<4, true, true>}}};
+const BatchSolveKernelFn kBatchAntennaSolve[2][2][2] = {
+    {{BatchAntennaSolveKernel<2, false, false>, BatchAntennaSolveKernel<2, false, true>},
+     {BatchAntennaSolveKernel<2, true, false>, BatchAntennaSolveKernel<2, true, true>}},
+    {{BatchAntennaSolveKernel<4, false, false>, BatchAntennaSolveKernel<4, false, true>},
+     {BatchAntennaSolveKernel<4, true, false>, BatchAntennaSolveKernel<4, true, true>}}};
 
 // items (interval, cb, part); one workgroup per item
 template <int ND, bool F16, bool FIN>
@@ -3175,7 +3256,7 @@ void LaunchBatchGram(const IterArgs* d_args, const BatchTile* d_tiles, const Til
 void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
                       int n_groups, int apply_ant, const BatchTile* d_items, int n_items,
                       int n_intervals, size_t solve_lds, int solve_threads, bool f16,
-                      const SmoothLaunch& smooth, hipStream_t stream) {
+                      const SmoothLaunch& smooth, hipStream_t stream, bool solve_chunked) {
   for (int g = 0; g < n_groups; ++g) {
     if (groups[g].count == 0) continue;
     CheckGramDirections(groups[g].n_dir);
@@ -3185,7 +3266,7 @@ void LaunchBatchApply(const IterArgs* d_args, const BatchTile* d_tiles, const Ti
   }
   DDH_HIP_CHECK(hipGetLastError());
   LaunchBatchTail(d_args, d_items, n_items, n_intervals, solve_lds, solve_threads, 2, f16, smooth,
-                  stream);
+                  stream, solve_chunked);
 }
 
 namespace {
@@ -3322,33 +3403,24 @@ void PrepareKernelAttributes() {
                               hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
     }
   // antenna_solve also holds static LDS for its reductions
-  for (const void* f : {reinterpret_cast<const void*>(&AntennaSolveKernel<2, false>),
-                        reinterpret_cast<const void*>(&AntennaSolveKernel<2, true>),
-                        reinterpret_cast<const void*>(&AntennaSolveKernel<4, false>),
-                        reinterpret_cast<const void*>(&AntennaSolveKernel<4, true>)}) {
-    hipFuncAttributes attr{};
-    DDH_HIP_CHECK(hipFuncGetAttributes(&attr, f));
-    const int dyn = max_lds - int(attr.sharedSizeBytes);
-    g_max_lds_solve = std::min(g_max_lds_solve, dyn);
-    DDH_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
-  }
-  for (const void* f : {reinterpret_cast<const void*>(&BatchAntennaSolveKernel<2, false>),
-                        reinterpret_cast<const void*>(&BatchAntennaSolveKernel<2, true>),
-                        reinterpret_cast<const void*>(&BatchAntennaSolveKernel<4, false>),
-                        reinterpret_cast<const void*>(&BatchAntennaSolveKernel<4, true>)}) {
-    hipFuncAttributes attr{};
-    DDH_HIP_CHECK(hipFuncGetAttributes(&attr, f));
-    const int dyn = max_lds - int(attr.sharedSizeBytes);
-    g_max_lds_solve = std::min(g_max_lds_solve, dyn);
-    DDH_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
-  }
+  for (int c = 0; c < 2; ++c)
+    for (int h = 0; h < 2; ++h)
+      for (int k = 0; k < 2; ++k)
+        for (const void* f : {reinterpret_cast<const void*>(kAntennaSolve[c][h][k]),
+                              reinterpret_cast<const void*>(kBatchAntennaSolve[c][h][k])}) {
+          hipFuncAttributes attr{};
+          DDH_HIP_CHECK(hipFuncGetAttributes(&attr, f));
+          const int dyn = max_lds - int(attr.sharedSizeBytes);
+          g_max_lds_solve = std::min(g_max_lds_solve, dyn);
+          DDH_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+        }
   done = true;
 }
 
 void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, const TileGroup* groups,
                  int n_groups, const BatchTile* d_items, int n_items, int n_intervals,
                  size_t solve_lds, int solve_threads, int n_corr, bool f16, int waves, int rows,
-                 const SmoothLaunch& smooth, hipStream_t stream) {
+                 const SmoothLaunch& smooth, hipStream_t stream, bool solve_chunked) {
   DDH_REQUIRE(waves >= 1 && waves * kWave <= kCorrelateMaxThreads, -1, "bad waves per tile");
   for (int g = 0; g < n_groups; ++g) {
     const TileGroup& grp = groups[g];
@@ -3360,29 +3432,19 @@ void LaunchBatch(const IterArgs* d_args, const BatchTile* d_tiles, const TileGro
   }
   DDH_HIP_CHECK(hipGetLastError());
   LaunchBatchTail(d_args, d_items, n_items, n_intervals, solve_lds, solve_threads, n_corr, f16,
-                  smooth, stream);
+                  smooth, stream, solve_chunked);
 }
 
 void LaunchBatchTail(const IterArgs* d_args, const BatchTile* d_items, int n_items,
                      int n_intervals, size_t solve_lds, int solve_threads, int n_corr, bool f16,
-                     const SmoothLaunch& smooth, hipStream_t stream) {
+                     const SmoothLaunch& smooth, hipStream_t stream, bool solve_chunked) {
   DDH_REQUIRE(solve_threads >= kWave && solve_threads <= kBlockThreads, -1,
               "bad antenna solve block size");
   // antenna solve + decide only
   DDH_REQUIRE(solve_lds <= size_t(g_max_lds_solve), -4, "batch antenna solve LDS exceeds the device");
-  if (n_items > 0) {
-    if (n_corr == 2) {
-      if (f16)
-        hipLaunchKernelGGL((BatchAntennaSolveKernel<2, true>), dim3(n_items), dim3(solve_threads), solve_lds, stream, d_args, d_items);
-      else
-        hipLaunchKernelGGL((BatchAntennaSolveKernel<2, false>), dim3(n_items), dim3(solve_threads), solve_lds, stream, d_args, d_items);
-    } else {
-      if (f16)
-        hipLaunchKernelGGL((BatchAntennaSolveKernel<4, true>), dim3(n_items), dim3(solve_threads), solve_lds, stream, d_args, d_items);
-      else
-        hipLaunchKernelGGL((BatchAntennaSolveKernel<4, false>), dim3(n_items), dim3(solve_threads), solve_lds, stream, d_args, d_items);
-    }
-  }
+  if (n_items > 0)
+    hipLaunchKernelGGL(kBatchAntennaSolve[n_corr == 4][f16][solve_chunked], dim3(n_items),
+                       dim3(solve_threads), solve_lds, stream, d_args, d_items);
   LaunchBatchSmoothDecide(d_args, d_items, n_items, n_intervals, smooth, stream);
 }
 
@@ -3615,17 +3677,8 @@ void LaunchAntennaSolve(const IterArgs& a, int n_corr, bool f16, int threads,
   const size_t lds = AntennaSolveSharedBytes(a, n_corr);
   DDH_REQUIRE(lds <= size_t(g_max_lds_solve), -4, "too many baselines for the antenna solve LDS");
   const dim3 grid(a.n_dir, a.n_cb);
-  if (n_corr == 2) {
-    if (f16)
-      hipLaunchKernelGGL((AntennaSolveKernel<2, true>), grid, block, lds, stream, a);
-    else
-      hipLaunchKernelGGL((AntennaSolveKernel<2, false>), grid, block, lds, stream, a);
-  } else {
-    if (f16)
-      hipLaunchKernelGGL((AntennaSolveKernel<4, true>), grid, block, lds, stream, a);
-    else
-      hipLaunchKernelGGL((AntennaSolveKernel<4, false>), grid, block, lds, stream, a);
-  }
+  hipLaunchKernelGGL(kAntennaSolve[n_corr == 4][f16][a.solve_chunk > 0], grid, block, lds, stream,
+                     a);
   DDH_HIP_CHECK(hipGetLastError());
 }
 
