@@ -1,6 +1,7 @@
 // Device kernels of the iterative diagonal solver.
 //
-// One solver iteration (IterativeDiagonalSolver.cc:67-94) is three launches:
+// One solver iteration (IterativeDiagonalSolver.cc:67-94) of the streaming
+// iteration is three launches:
 //
 //   correlate      streams the data plane and every model plane of a block
 //                  once. Per visibility it forms the full residual
@@ -24,6 +25,11 @@
 //                  (SolverBase.h:274-285) and the NaN-repair values of the next
 //                  iteration (SolverBase.cc:342-371), which consumers apply
 //                  when they read a solution.
+//
+// The Gram iteration replaces the correlate by an apply of per-baseline cross
+// products made once per solve; its fused form (at most 64 antennas) runs the
+// apply, the antenna solve, the finishing step and the decision of an
+// iteration as one launch (GramSolveBlock).
 //
 // Solutions ping-pong between two buffers by iteration parity, so no copy
 // pass is needed.
