@@ -360,6 +360,9 @@ __global__ void GatherRowsKernel(const float* __restrict__ src, const int32_t* _
 //   s  = log10(den / mean den over the antennas of this (cb, sub))
 // Features (kWarmFeatures, zero padded): for p = 0, 1:
 //   [log|r1|, arg r1, log|ra|, arg ra, log|rb|, arg rb, s].
+// A 2x2 solution (span ne = 4) takes its diagonal entries J_00, J_11 as the
+// two polarizations; its denominators are the diagonal of the 2x2
+// denominator, two per solution as for a pair.
 
 __device__ inline void LogRatio(double2 a, double2 b, float& lmag, float& ang) {
   // log(a / b) = log|a| - log|b| + i (arg a - arg b), wrapped to (-pi, pi]
@@ -386,10 +389,11 @@ __global__ void WarmFeaturesKernel(const double2* __restrict__ g0_all,
   const WarmSpan sp = spans[blockIdx.y];
   const int n_ant = sp.n_ant, n_sub = sp.n_sub;
   const int64_t n_pairs = int64_t(sp.n_cb) * n_ant * n_sub;
-  const double2* g0 = g0_all + 2 * sp.pair0;
-  const double2* next1 = next1_all + 2 * sp.pair0;
-  const double2* ga = ga_all + 2 * sp.pair0;
-  const double2* gb = gb_all + 2 * sp.pair0;
+  const int ne = sp.ne;  // complex values per solution
+  const double2* g0 = g0_all + ne * sp.pair0;
+  const double2* next1 = next1_all + ne * sp.pair0;
+  const double2* ga = ga_all + ne * sp.pair0;
+  const double2* gb = gb_all + ne * sp.pair0;
   const double* den = den_all + 2 * sp.pair0;
   float* feat = feat_all + sp.pair0 * kWarmFeatures;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n_pairs;
@@ -399,10 +403,11 @@ __global__ void WarmFeaturesKernel(const double2* __restrict__ g0_all,
     float f[kWarmFeatures] = {};
     for (int p = 0; p < 2; ++p) {
       float* fp = f + 7 * p;
-      const double2 base = g0[i * 2 + p];
-      LogRatio(next1[i * 2 + p], base, fp[0], fp[1]);
-      LogRatio(ga[i * 2 + p], base, fp[2], fp[3]);
-      LogRatio(gb[i * 2 + p], base, fp[4], fp[5]);
+      const int64_t v = i * ne + p * (ne - 1);  // value p of the pair, entry pp of a matrix
+      const double2 base = g0[v];
+      LogRatio(next1[v], base, fp[0], fp[1]);
+      LogRatio(ga[v], base, fp[2], fp[3]);
+      LogRatio(gb[v], base, fp[4], fp[5]);
       double mean = 0.0;
       int cnt = 0;
       for (int a = 0; a < n_ant; ++a) {
@@ -421,17 +426,46 @@ __global__ void WarmFeaturesKernel(const double2* __restrict__ g0_all,
 }
 
 // init = prev * exp(d_log_mag + i d_arg) per polarization; non-finite prev
-// falls back to 1. Spans as for the features.
+// falls back to 1. A 2x2 solution (ne = 4): row p of prev times the factor
+// of polarization p (the diagonal entries take the pair's rule, the leakage
+// terms scale with their row); a matrix with a non-finite entry falls back
+// to the identity. Spans as for the features.
 __global__ void WarmApplyKernel(const double2* __restrict__ prev_all,
                                 const float* __restrict__ pred_all,
                                 const WarmSpan* __restrict__ spans, double2* __restrict__ init_all) {
   const WarmSpan sp = spans[blockIdx.y];
   const int64_t n_pairs = int64_t(sp.n_cb) * sp.n_ant * sp.n_sub;
-  const double2* prev = prev_all + 2 * sp.pair0;
+  const int ne = sp.ne;
+  const double2* prev = prev_all + ne * sp.pair0;
   const float* pred = pred_all + sp.pair0 * kWarmOutputs;
-  double2* init = init_all + 2 * sp.pair0;
+  double2* init = init_all + ne * sp.pair0;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n_pairs;
        i += int64_t(gridDim.x) * blockDim.x) {
+    if (ne == 4) {
+      double2 m[4];
+      bool finite = true;
+      for (int k = 0; k < 4; ++k) {
+        m[k] = prev[i * 4 + k];
+        finite = finite && isfinite(m[k].x) && isfinite(m[k].y);
+      }
+      if (!finite) {
+        m[0] = m[3] = make_double2(1.0, 0.0);
+        m[1] = m[2] = make_double2(0.0, 0.0);
+      }
+      for (int p = 0; p < 2; ++p) {
+        const double lm = pred[i * kWarmOutputs + 2 * p];
+        const double an = pred[i * kWarmOutputs + 2 * p + 1];
+        double sn, cs;
+        sincos(an, &sn, &cs);
+        const double sc = exp(lm);
+        for (int q = 0; q < 2; ++q) {
+          const double2 g = m[2 * p + q];
+          init[i * 4 + 2 * p + q] =
+              make_double2(sc * (g.x * cs - g.y * sn), sc * (g.x * sn + g.y * cs));
+        }
+      }
+      continue;
+    }
     for (int p = 0; p < 2; ++p) {
       double2 g = prev[i * 2 + p];
       if (!(isfinite(g.x) && isfinite(g.y))) g = make_double2(1.0, 0.0);
@@ -729,6 +763,53 @@ void LaunchWarmApply(const double2* g0, const float* pred, const WarmSpan* spans
 
 }  // namespace ddh
 
+namespace ddh {
+namespace {
+// Host arrays through the warm-start kernels: ne complex values per solution
+// (2: a diagonal pair, 4: a 2x2 matrix), two denominators per solution.
+void WarmFeaturesHost(const double* g0, const double* next1, const double* ga, const double* gb,
+                      const double* den, int32_t n_cb, int32_t n_ant, int32_t n_sub, int ne,
+                      float* features) {
+  DDH_REQUIRE(g0 && next1 && ga && gb && den && features, -1, "NULL argument");
+  DDH_REQUIRE(n_cb >= 1 && n_ant >= 1 && n_sub >= 1, -1, "empty dimension");
+  const int64_t n_pairs = int64_t(n_cb) * n_ant * n_sub;
+  Buf<double2> d0, d1, da, db;
+  Buf<double> dd;
+  Buf<float> df;
+  const size_t bytes = n_pairs * ne * sizeof(double2);
+  DDH_HIP_CHECK(hipMemcpy(d0.Get(n_pairs * ne), g0, bytes, hipMemcpyHostToDevice));
+  DDH_HIP_CHECK(hipMemcpy(d1.Get(n_pairs * ne), next1, bytes, hipMemcpyHostToDevice));
+  DDH_HIP_CHECK(hipMemcpy(da.Get(n_pairs * ne), ga, bytes, hipMemcpyHostToDevice));
+  DDH_HIP_CHECK(hipMemcpy(db.Get(n_pairs * ne), gb, bytes, hipMemcpyHostToDevice));
+  DDH_HIP_CHECK(hipMemcpy(dd.Get(n_pairs * 2), den, n_pairs * 2 * sizeof(double),
+                          hipMemcpyHostToDevice));
+  Buf<WarmSpan> span;
+  const WarmSpan one{0, n_cb, n_ant, n_sub, ne};
+  DDH_HIP_CHECK(hipMemcpy(span.Get(1), &one, sizeof(one), hipMemcpyHostToDevice));
+  LaunchWarmFeatures(d0.ptr, d1.ptr, da.ptr, db.ptr, dd.ptr, span.ptr, 1, n_pairs,
+                     df.Get(n_pairs * kWarmFeatures), nullptr);
+  DDH_HIP_CHECK(hipMemcpy(features, df.ptr, n_pairs * kWarmFeatures * sizeof(float),
+                          hipMemcpyDeviceToHost));
+}
+
+void WarmApplyHost(const double* prev, const float* pred, int64_t n_pairs, int ne, double* init) {
+  DDH_REQUIRE(prev && pred && init, -1, "NULL argument");
+  DDH_REQUIRE(n_pairs >= 1 && n_pairs <= INT32_MAX, -1, "bad pair count");
+  Buf<double2> dp, di;
+  Buf<float> dpred;
+  DDH_HIP_CHECK(hipMemcpy(dp.Get(n_pairs * ne), prev, n_pairs * ne * sizeof(double2),
+                          hipMemcpyHostToDevice));
+  DDH_HIP_CHECK(hipMemcpy(dpred.Get(n_pairs * kWarmOutputs), pred,
+                          n_pairs * kWarmOutputs * sizeof(float), hipMemcpyHostToDevice));
+  Buf<WarmSpan> span;
+  const WarmSpan one{0, 1, 1, int32_t(n_pairs), ne};
+  DDH_HIP_CHECK(hipMemcpy(span.Get(1), &one, sizeof(one), hipMemcpyHostToDevice));
+  LaunchWarmApply(dp.ptr, dpred.ptr, span.ptr, 1, n_pairs, di.Get(n_pairs * ne), nullptr);
+  DDH_HIP_CHECK(hipMemcpy(init, di.ptr, n_pairs * ne * sizeof(double2), hipMemcpyDeviceToHost));
+}
+}  // namespace
+}  // namespace ddh
+
 using namespace ddh;
 
 extern "C" {
@@ -920,42 +1001,22 @@ int ddh_gemm_bf16(const float* a, const float* b, float* c, int32_t M, int32_t N
 int ddh_warmstart_features(const double* g0, const double* next1, const double* ga,
                            const double* gb, const double* den, int32_t n_cb, int32_t n_ant,
                            int32_t n_sub, float* features) {
-  return Guard([&] {
-    DDH_REQUIRE(g0 && next1 && ga && gb && den && features, -1, "NULL argument");
-    const int64_t n_pairs = int64_t(n_cb) * n_ant * n_sub;
-    Buf<double2> d0, d1, da, db;
-    Buf<double> dd;
-    Buf<float> df;
-    const size_t bytes = n_pairs * 2 * sizeof(double2);
-    DDH_HIP_CHECK(hipMemcpy(d0.Get(n_pairs * 2), g0, bytes, hipMemcpyHostToDevice));
-    DDH_HIP_CHECK(hipMemcpy(d1.Get(n_pairs * 2), next1, bytes, hipMemcpyHostToDevice));
-    DDH_HIP_CHECK(hipMemcpy(da.Get(n_pairs * 2), ga, bytes, hipMemcpyHostToDevice));
-    DDH_HIP_CHECK(hipMemcpy(db.Get(n_pairs * 2), gb, bytes, hipMemcpyHostToDevice));
-    DDH_HIP_CHECK(hipMemcpy(dd.Get(n_pairs * 2), den, n_pairs * 2 * sizeof(double), hipMemcpyHostToDevice));
-    Buf<WarmSpan> span;
-    const WarmSpan one{0, n_cb, n_ant, n_sub, 0};
-    DDH_HIP_CHECK(hipMemcpy(span.Get(1), &one, sizeof(one), hipMemcpyHostToDevice));
-    LaunchWarmFeatures(d0.ptr, d1.ptr, da.ptr, db.ptr, dd.ptr, span.ptr, 1, n_pairs,
-                       df.Get(n_pairs * kWarmFeatures), nullptr);
-    DDH_HIP_CHECK(hipMemcpy(features, df.ptr, n_pairs * kWarmFeatures * sizeof(float),
-                            hipMemcpyDeviceToHost));
-  });
+  return Guard([&] { WarmFeaturesHost(g0, next1, ga, gb, den, n_cb, n_ant, n_sub, 2, features); });
 }
 
 int ddh_warmstart_apply(const double* prev, const float* pred, int64_t n_pairs, double* init) {
-  return Guard([&] {
-    DDH_REQUIRE(prev && pred && init, -1, "NULL argument");
-    Buf<double2> dp, di;
-    Buf<float> dpred;
-    DDH_HIP_CHECK(hipMemcpy(dp.Get(n_pairs * 2), prev, n_pairs * 2 * sizeof(double2), hipMemcpyHostToDevice));
-    DDH_HIP_CHECK(hipMemcpy(dpred.Get(n_pairs * kWarmOutputs), pred, n_pairs * kWarmOutputs * sizeof(float), hipMemcpyHostToDevice));
-    DDH_REQUIRE(n_pairs >= 1 && n_pairs <= INT32_MAX, -1, "bad pair count");
-    Buf<WarmSpan> span;
-    const WarmSpan one{0, 1, 1, int32_t(n_pairs), 0};
-    DDH_HIP_CHECK(hipMemcpy(span.Get(1), &one, sizeof(one), hipMemcpyHostToDevice));
-    LaunchWarmApply(dp.ptr, dpred.ptr, span.ptr, 1, n_pairs, di.Get(n_pairs * 2), nullptr);
-    DDH_HIP_CHECK(hipMemcpy(init, di.ptr, n_pairs * 2 * sizeof(double2), hipMemcpyDeviceToHost));
-  });
+  return Guard([&] { WarmApplyHost(prev, pred, n_pairs, 2, init); });
+}
+
+int ddh_warmstart_features_2x2(const double* g0, const double* next1, const double* ga,
+                               const double* gb, const double* den, int32_t n_cb, int32_t n_ant,
+                               int32_t n_sub, float* features) {
+  return Guard([&] { WarmFeaturesHost(g0, next1, ga, gb, den, n_cb, n_ant, n_sub, 4, features); });
+}
+
+int ddh_warmstart_apply_2x2(const double* prev, const float* pred, int64_t n_solutions,
+                            double* init) {
+  return Guard([&] { WarmApplyHost(prev, pred, n_solutions, 4, init); });
 }
 
 }  // extern "C"

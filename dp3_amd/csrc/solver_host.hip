@@ -1773,7 +1773,6 @@ size_t SolutionCount(const ddh_batch& b, int i) {
   return size_t(s.cfg.c.n_channel_blocks) * s.cfg.c.n_antennas * s.cfg.n_sub * DevicePols(s.cfg.c);
 }
 
-// Host solutions[i][cb] -> dst(i) on the batch stream.
 // Host solutions[i][cb] -> dst(i) on the batch stream: every interval's blocks
 // gathered into one pinned stage and copied with one DMA per interval
 // (scalar layouts: per block, converted).
@@ -1826,7 +1825,6 @@ void SolveBatch(ddh_batch& b, double* const* const* solutions, ddh_result* resul
 // ddh_mlp_forward and ddh_warmstart_apply.
 void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* const* solutions,
                     ddh_result* results) {
-  DDH_REQUIRE(!b.full_jones, -4, "the learned warm start serves the scalar and diagonal solvers");
   CheckWarmNetwork(net);
   RefreshBatch(b);
   const int n = int(b.solvers.size());
@@ -1842,15 +1840,18 @@ void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* cons
   b.warm_next.Resize(total);
   b.warm_a.Resize(total);
   b.warm_b.Resize(total);
-  b.warm_den.Resize(total);
+  // complex values per solution: a diagonal pair, or a 2x2 matrix (whose
+  // diagonal entries the features and the start use); two denominators each
+  const int ne = b.full_jones ? 4 : 2;
+  b.warm_den.Resize(total / ne * 2);
   // every interval's place in the concatenated buffers, for one launch each
   std::vector<int64_t> off64(off.begin(), off.end());
   std::vector<WarmSpan> spans(n);
   int64_t max_count = 0;
   for (int i = 0; i < n; ++i) {
     const ddh_config& c = b.solvers[i]->cfg.c;
-    spans[i] = WarmSpan{int64_t(off[i] / 2), c.n_channel_blocks, c.n_antennas,
-                        b.solvers[i]->cfg.n_sub, 0};
+    spans[i] = WarmSpan{int64_t(off[i] / ne), c.n_channel_blocks, c.n_antennas,
+                        b.solvers[i]->cfg.n_sub, ne};
     max_count = std::max<int64_t>(max_count, int64_t(off[i + 1] - off[i]));
   }
   const bool same_spans =
@@ -1858,7 +1859,7 @@ void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* cons
       std::equal(spans.begin(), spans.end(), b.warm_spans.begin(), b.warm_spans.end(),
                  [](const WarmSpan& x, const WarmSpan& y) {
                    return x.pair0 == y.pair0 && x.n_cb == y.n_cb && x.n_ant == y.n_ant &&
-                          x.n_sub == y.n_sub;
+                          x.n_sub == y.n_sub && x.ne == y.ne;
                  });
   if (!same_spans) {
     b.d_warm_off.Upload(off64, b.stream);
@@ -1871,7 +1872,7 @@ void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* cons
   for (int i = 0; i < n; ++i) {
     step[i].step_size = 1.0;
     step[i].phase_only = 0;
-    step[i].den_out = b.warm_den.ptr + off[i];
+    step[i].den_out = b.warm_den.ptr + off[i] / ne * 2;
     aux[i].max_iterations = stretch;
     aux[i].min_iterations = 0;
     aux[i].accuracy = 0.0;
@@ -1906,17 +1907,17 @@ void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* cons
   move(kMoveToSol0, b.warm_a.ptr);
   run_aux(b.warm_b.ptr);
   // features of all intervals as consecutive rows, one forward pass
-  const int64_t rows = int64_t(total / 2);
+  const int64_t rows = int64_t(total / ne);
   b.warm_x.Resize(size_t(rows) * kWarmFeatures);
   b.warm_y.Resize(size_t(rows) * kWarmOutputs);
   float* x = b.warm_x.ptr;
   LaunchWarmFeatures(b.warm_g0.ptr, b.warm_next.ptr, b.warm_a.ptr, b.warm_b.ptr, b.warm_den.ptr,
-                     b.d_warm_spans.ptr, n, max_count / 2, x, b.stream);
+                     b.d_warm_spans.ptr, n, max_count / ne, x, b.stream);
   MlpForwardRows(net, x, rows, b.warm_y.ptr, b.stream);
   const float* pred = b.warm_y.ptr;
   // the starts from the second stretch's iterates, through warm_a (no longer
   // needed), into every sol0
-  LaunchWarmApply(b.warm_b.ptr, pred, b.d_warm_spans.ptr, n, max_count / 2, b.warm_a.ptr,
+  LaunchWarmApply(b.warm_b.ptr, pred, b.d_warm_spans.ptr, n, max_count / ne, b.warm_a.ptr,
                   b.stream);
   move(kMoveToSol0, b.warm_a.ptr);
   DDH_HIP_CHECK(hipEventRecord(b.ev_warm1, b.stream));
@@ -2526,15 +2527,14 @@ int ddh_one_step(ddh_solver* s, ddh_problem* p, double* const* solutions_in,
                  double* const* next_out, double* const* den_out) {
   return Guard([&] {
     DDH_REQUIRE(s && p && solutions_in && next_out, -1, "NULL argument");
-    DDH_REQUIRE(s->cfg.c.n_solution_polarizations != 4, -4,
-                "one_step serves the scalar and diagonal solvers");
     CheckMatch(*s, *p);
     PrepareBuffers(*s, *p);
     const ddh_config& c = s->cfg.c;
-    const size_t per_cb = size_t(c.n_antennas) * s->cfg.n_sub * 2;
-    s->den.Resize(per_cb * c.n_channel_blocks);
+    const size_t n_sols = size_t(c.n_antennas) * s->cfg.n_sub;  // solutions per block
+    const size_t per_cb = n_sols * DevicePols(c);                // device values per block
+    s->den.Resize(n_sols * 2 * c.n_channel_blocks);              // two denominators each
     for (int cb = 0; cb < c.n_channel_blocks; ++cb)
-      SolutionsIn(*s, s->sol0.ptr + cb * per_cb, solutions_in[cb], per_cb / 2, s->stream);
+      SolutionsIn(*s, s->sol0.ptr + cb * per_cb, solutions_in[cb], n_sols, s->stream);
     IterArgs a = MakeArgs(*s, *p);
     a.step_size = 1.0;  // stepped == next: the undamped update
     a.phase_only = 0;
@@ -2542,9 +2542,14 @@ int ddh_one_step(ddh_solver* s, ddh_problem* p, double* const* solutions_in,
     LaunchSolvePrologue(*s, a, *p);
     LaunchCorrelateAndSolve(*s, a, *p);
     for (int cb = 0; cb < c.n_channel_blocks; ++cb) {
-      SolutionsOut(*s, next_out[cb], s->sol1.ptr + cb * per_cb, per_cb / 2, s->stream);
-      if (den_out)
-        SolutionsOut(*s, den_out[cb], s->den.ptr + cb * per_cb, per_cb / 2, s->stream, true);
+      SolutionsOut(*s, next_out[cb], s->sol1.ptr + cb * per_cb, n_sols, s->stream);
+      if (!den_out) continue;
+      if (c.n_solution_polarizations == 4)  // the diagonal of each 2x2 denominator
+        DDH_HIP_CHECK(hipMemcpyAsync(den_out[cb], s->den.ptr + cb * n_sols * 2,
+                                     n_sols * 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                     s->stream));
+      else
+        SolutionsOut(*s, den_out[cb], s->den.ptr + cb * n_sols * 2, n_sols, s->stream, true);
     }
     DDH_HIP_CHECK(hipStreamSynchronize(s->stream));
   });

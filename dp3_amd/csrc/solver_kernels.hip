@@ -2870,6 +2870,12 @@ __device__ __forceinline__ void FullJonesSolveBlock(const IterArgs& a, const int
       }
       nxt[sidx * 4 + k] = stepped[k];
     }
+    // the optional denominators: the diagonal of the 2x2 denominator, two
+    // per solution as for a pair (the warm start's features)
+    if (a.den_out) {
+      a.den_out[sidx * 2] = sd[0];
+      a.den_out[sidx * 2 + 1] = sd[1];
+    }
     AddMatrixTerms(old, stepped, conv_sum, conv_n, mean_sum, mean_n);
   }
   BlockSum2(conv_sum, conv_n, mean_sum, mean_n, red, redc);

@@ -184,6 +184,9 @@ SIGNATURES = {
     "ddh_warmstart_features": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_int32, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
     "ddh_warmstart_apply": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
+    "ddh_warmstart_features_2x2": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_int32, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
+    "ddh_warmstart_apply_2x2": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
 }
 
 N_WARM_FEATURES = 16
@@ -195,10 +198,11 @@ def _rows_ptrs(a):
 
 
 def one_step(solver, dproblem, solutions):
-    """Undamped solver update from `solutions` and its denominators."""
+    """Undamped solver update from `solutions` and its denominators (two per
+    solution: a diagonal pair's, or the diagonal of a 2x2 denominator)."""
     sol = np.ascontiguousarray(solutions, dtype=np.complex128)
     nxt = np.zeros_like(sol)
-    den = np.zeros(sol.shape, np.float64)
+    den = np.zeros((sol.shape[0], solver.n_antennas * solver.n_sub * 2), np.float64)
     _check(lib().ddh_one_step(solver.handle, dproblem.handle, C.cast(_rows_ptrs(sol), C.c_void_p),
                               C.cast(_rows_ptrs(nxt), C.c_void_p),
                               C.cast(_rows_ptrs(den), C.c_void_p)), "ddh_one_step")
@@ -225,21 +229,34 @@ def gemm_bf16(a, b, trans_a=False, trans_b=False):
     return c
 
 
-def warmstart_features(g0, next1, ga, gb, den, n_cb, n_ant, n_sub):
+def warmstart_features(g0, next1, ga, gb, den, n_cb, n_ant, n_sub, full_jones=False):
+    """Features per solution; full_jones: 2x2 solutions (4 values each)."""
     arrs = [np.ascontiguousarray(x, np.complex128) for x in (g0, next1, ga, gb)]
+    ne = 4 if full_jones else 2
+    for a in arrs:
+        if a.size != n_cb * n_ant * n_sub * ne:
+            raise ValueError("solution arrays do not match the shape")
     d = np.ascontiguousarray(den, np.float64)
+    if d.size != n_cb * n_ant * n_sub * 2:
+        raise ValueError("denominators do not match the shape")
     out = np.zeros((n_cb * n_ant * n_sub, N_WARM_FEATURES), np.float32)
-    _check(lib().ddh_warmstart_features(*[_ptr(a) for a in arrs], _ptr(d), n_cb, n_ant, n_sub,
-                                        _ptr(out)), "ddh_warmstart_features")
+    fn = lib().ddh_warmstart_features_2x2 if full_jones else lib().ddh_warmstart_features
+    _check(fn(*[_ptr(a) for a in arrs], _ptr(d), n_cb, n_ant, n_sub, _ptr(out)),
+           "ddh_warmstart_features")
     return out
 
 
-def warmstart_apply(prev, pred):
+def warmstart_apply(prev, pred, full_jones=False):
+    """prev * exp(pred) per solution (full_jones: row p of each 2x2 matrix
+    times exp(pred_p))."""
     p = np.ascontiguousarray(prev, np.complex128)
     q = np.ascontiguousarray(pred, np.float32)
+    ne = 4 if full_jones else 2
+    if p.size % ne or q.size != p.size // ne * N_WARM_OUTPUTS:
+        raise ValueError("predictions do not match the solutions")
     out = np.zeros_like(p)
-    _check(lib().ddh_warmstart_apply(_ptr(p), _ptr(q), p.size // 2, _ptr(out)),
-           "ddh_warmstart_apply")
+    fn = lib().ddh_warmstart_apply_2x2 if full_jones else lib().ddh_warmstart_apply
+    _check(fn(_ptr(p), _ptr(q), p.size // ne, _ptr(out)), "ddh_warmstart_apply")
     return out
 
 

@@ -8,7 +8,10 @@ third option: a small network, trained on the solver's own outputs, maps per
     interval (the data/model correlation summary, ddh_one_step), relative to
     the previous solutions, with its denominator as a confidence, and
   * the previous interval's drift
-to a multiplicative correction of the previous solutions. The plain solver
+to a multiplicative correction of the previous solutions. For the full-Jones
+solver (2x2 solutions) the same network reads the diagonal entries J_00,
+J_11 and scales row p of the matrix by the correction of polarization p.
+The plain solver
 then runs to its stopping criterion from that start, so the network only
 decides where the iteration begins, never when it ends.
 
@@ -31,7 +34,18 @@ def sibling(solver: H.Solver, max_iterations, accuracy, detect_stalling=True, it
                     n_correlations=solver.n_correlations, max_iterations=max_iterations,
                     accuracy=accuracy, step_size=c.step_size, detect_stalling=detect_stalling,
                     phase_only=bool(c.phase_only), model_storage=solver.model_storage,
-                    iteration=solver.iteration if iteration is None else iteration)
+                    iteration=solver.iteration if iteration is None else iteration,
+                    n_solution_polarizations=solver.n_solution_polarizations)
+
+
+def is_full_jones(solver: H.Solver):
+    return solver.n_solution_polarizations == 4
+
+
+def diagonal(solutions, full_jones):
+    """[n, 2] polarization values: a pair, or a 2x2 matrix's J_00, J_11."""
+    g = np.asarray(solutions, np.complex128)
+    return g.reshape(-1, 4)[:, ::3] if full_jones else g.reshape(-1, 2)
 
 
 class WarmStart:
@@ -67,15 +81,16 @@ class WarmStart:
         gb = ga.copy()
         aux.solve_problem(dproblem, gb)
         x = H.warmstart_features(g0, nxt, ga, gb, den, solver.n_channel_blocks,
-                                 solver.n_antennas, solver.n_sub)
+                                 solver.n_antennas, solver.n_sub, is_full_jones(solver))
         return (x, gb) if return_stretch else x
 
     @staticmethod
-    def targets(base, final):
+    def targets(base, final, full_jones=False):
         """log(final / base) per polarization as (log-magnitude, phase) pairs
-        (base: the second stretch's iterate gb)."""
-        p = np.asarray(base, np.complex128).reshape(-1, 2)
-        f = np.asarray(final, np.complex128).reshape(-1, 2)
+        (base: the second stretch's iterate gb; for 2x2 solutions the ratio of
+        the diagonal entries)."""
+        p = diagonal(base, full_jones)
+        f = diagonal(final, full_jones)
         with np.errstate(all="ignore"):
             r = f / p
             out = np.stack([np.log(np.abs(r[:, 0])), np.angle(r[:, 0]),
@@ -91,7 +106,7 @@ class WarmStart:
     def initial(self, solver, dproblem, g0):
         x, gb = self.features(solver, dproblem, g0, return_stretch=True)
         pred = self.net.forward(x)
-        return H.warmstart_apply(gb, pred).reshape(np.asarray(g0).shape)
+        return H.warmstart_apply(gb, pred, is_full_jones(solver)).reshape(np.asarray(g0).shape)
 
 
 def drifting_gains(w: S.Workload, rng, n_intervals, sigma_phase=0.05, sigma_amp=0.005):
@@ -161,8 +176,12 @@ class IntervalSequence:
         return self._scratch
 
     def unit(self):
-        return np.ones((self.w.n_channel_blocks, self.w.n_stations * self.w.n_sub * 2),
-                       np.complex128)
+        """Unit gains: ones, or identity matrices for 2x2 solutions."""
+        n = self.w.n_stations * self.w.n_sub
+        if is_full_jones(self.solver):
+            eye = np.tile(np.array([1, 0, 0, 1], np.complex128), n)
+            return np.tile(eye, (self.w.n_channel_blocks, 1))
+        return np.ones((self.w.n_channel_blocks, n * 2), np.complex128)
 
 
 class _LazyProblems:
@@ -207,7 +226,7 @@ def training_set(seq: IntervalSequence, ws: WarmStart, first, last, converge_ite
         xs.append(x)
         sol = prev.copy()
         long.solve_problem(dp, sol)
-        ts.append(ws.targets(gb, sol))
+        ts.append(ws.targets(gb, sol, is_full_jones(seq.solver)))
         prev = sol
     return np.concatenate(xs), np.concatenate(ts), prev
 
@@ -239,7 +258,7 @@ def training_sets_batched(seqs, ws: WarmStart, first, last, converge_iterations=
         sol = [p.copy() for p in prev]
         batch.solve(sol)
         for k in range(len(seqs)):
-            ts[k].append(ws.targets(gbs[k], sol[k]))
+            ts[k].append(ws.targets(gbs[k], sol[k], is_full_jones(seqs[k].solver)))
         prev = sol
     batch.close()
     for s in longs:

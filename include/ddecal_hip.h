@@ -292,7 +292,9 @@ int ddh_set_profiling(ddh_solver* solver, int32_t enable);
 // One undamped solver update from `solutions_in` (the solution of the
 // Jacobi step, IterativeDiagonalSolver.cc:206-221, without the damping of
 // SolverBase::Step), with its denominators: next_out[cb] like the solutions,
-// den_out[cb] -> [n_ant][n_sub][2] doubles (may be NULL). Used as the
+// den_out[cb] -> [n_ant][n_sub][2] doubles (may be NULL; for the full-Jones
+// solver the diagonal of each 2x2 denominator, the solution of
+// IterativeFullJonesSolver.cc:178-189 without the damping). Used as the
 // data/model correlation summary of the warm start.
 int ddh_one_step(ddh_solver* solver, ddh_problem* problem, double* const* solutions_in,
                  double* const* next_out, double* const* den_out);
@@ -355,7 +357,8 @@ int ddh_batch_last_time(const ddh_batch* batch, double* ms);
 // is kept). Same arithmetic as the per-interval
 // sequence ddh_one_step, 2 x ddh_solve_problem(max `stretch`, accuracy 0,
 // no stall detection), ddh_warmstart_features, ddh_mlp_forward and
-// ddh_warmstart_apply. The network must not be in use on another stream.
+// ddh_warmstart_apply (the _2x2 variants for a full-Jones batch). The
+// network must not be in use on another stream.
 // No counterpart in the reference (DDECal.cc:487-499 offers unit or
 // previous solutions only).
 typedef struct ddh_mlp ddh_mlp;
@@ -454,6 +457,17 @@ int ddh_warmstart_features(const double* g0, const double* next1, const double* 
                            const double* gb, const double* den, int32_t n_cb, int32_t n_ant,
                            int32_t n_sub, float* features);
 int ddh_warmstart_apply(const double* prev, const float* pred, int64_t n_pairs, double* init);
+// The same for 2x2 solutions ([n_cb][n_ant][n_sub][4] complex128, the
+// full-Jones layout; den two values per solution, the diagonal of the
+// one-step update's 2x2 denominator): the features from the diagonal entries
+// J_00, J_11, and the start with row p of prev times exp(pred_p) (a matrix
+// with a non-finite entry starts from the identity). No counterpart in the
+// reference.
+int ddh_warmstart_features_2x2(const double* g0, const double* next1, const double* ga,
+                               const double* gb, const double* den, int32_t n_cb, int32_t n_ant,
+                               int32_t n_sub, float* features);
+int ddh_warmstart_apply_2x2(const double* prev, const float* pred, int64_t n_solutions,
+                            double* init);
 
 // Bandwidth calibration: a plain copy and a plain streaming read over
 // `bytes` (each repeated `repeats` times). copy_gbs counts bytes read plus

@@ -233,3 +233,131 @@ def test_concurrent_batches_share_a_network():
             np.testing.assert_array_equal(con_out[k][m], seq_out[k][m])
     for b in batches:
         b.close()
+
+
+FJ = dict(n_correlations=4, n_solution_polarizations=4)
+
+
+def pairs_to_matrices(g):
+    """[n_cb, n*2] diagonal pairs -> [n_cb, n*4] row-major 2x2 matrices."""
+    g = np.asarray(g, np.complex128).reshape(g.shape[0], -1, 2)
+    m = np.zeros(g.shape[:2] + (4,), np.complex128)
+    m[..., 0], m[..., 3] = g[..., 0], g[..., 1]
+    return m.reshape(g.shape[0], -1)
+
+
+def test_full_jones_one_step_matches_oracle():
+    """ddh_one_step on the full-Jones solver: the undamped 2x2 update equals
+    one oracle iteration at step size 1 (IterativeFullJonesSolver.cc:178-189,
+    then SolverBase::Step with step 1), per matrix within 1e-4 (Frobenius);
+    the denominators (diagonal of a sum of M^H M-type products) are positive."""
+    print("case: full-Jones one step", flush=True)
+    w, geom, sky, gains, P = small_problem(n_corr=4)
+    s = H.Solver(P.n_antennas, P.n_solutions_per_direction, P.n_channel_blocks, **FJ)
+    dp = H.DeviceProblem.from_problem(s, P)
+    dp.upload(P)
+    rng = np.random.default_rng(6)
+    start = pairs_to_matrices(gains) * (1 + 0.05 * rng.normal(size=(gains.shape[0],
+                                                                      gains.shape[1] * 2)))
+    start = start.reshape(gains.shape[0], -1, 4)
+    start[..., 1:3] += 0.02 * (rng.normal(size=start[..., 1:3].shape)
+                               + 1j * rng.normal(size=start[..., 1:3].shape))
+    start = np.ascontiguousarray(start.reshape(gains.shape[0], -1))
+    nxt, den = H.one_step(s, dp, start)
+    ref = start.copy()
+    O.solve(P, ref, full_jones=True, max_iterations=1, step_size=1.0, accuracy=0.0,
+            detect_stalling=False, fp64=True)
+    d = np.linalg.norm((nxt - ref).reshape(-1, 4), axis=1)
+    r = np.linalg.norm(ref.reshape(-1, 4), axis=1)
+    assert np.max(d / r) <= 1e-4, np.max(d / r)
+    assert den.shape == (P.n_channel_blocks, P.n_antennas * P.n_sub * 2)
+    assert np.all(den > 0)
+    s.close()
+
+
+def test_full_jones_batched_warm_start_matches_host_sequence():
+    """ddh_batch_solve_warm on full-Jones intervals: the same starts and solves
+    as the per-interval host sequence one_step -> 2 x stretch iterations ->
+    ddh_warmstart_features_2x2 -> forward -> ddh_warmstart_apply_2x2,
+    followed by a batched solve (same schedule on both sides)."""
+    print("case: full-Jones batched warm start vs host sequence", flush=True)
+    w = S.Workload("wsf", 12, 3, 2, 1, 12, 3, 4, 60, 1e-5, seed=9)
+    seqs = [WS.IntervalSequence(w, 2, seed=60 + k, solver_kwargs=FJ) for k in range(3)]
+    ws = WS.WarmStart(hidden=64, seed=1, stretch=3)
+    prevs = []
+    for q in seqs:
+        g = q.unit()
+        q.solver.solve_problem(q.problems[0], g)
+        prevs.append(g)
+    solvers = [q.solver for q in seqs]
+    dps = [q.problems[1] for q in seqs]
+    host = [ws.initial(q.solver, q.problems[1], prevs[k]) for k, q in enumerate(seqs)]
+    b1 = H.Batch(solvers, dps)
+    b1.set_schedule(4, 1, 1, 8)
+    r_host = b1.solve(host)
+    dev = [p.copy() for p in prevs]
+    b2 = H.Batch(solvers, dps)
+    b2.set_schedule(4, 1, 1, 8)
+    r_dev = b2.solve_warm(ws.net, dev, stretch=ws.stretch)
+    for k in range(len(seqs)):
+        assert r_dev[k].iterations == r_host[k].iterations
+        np.testing.assert_allclose(dev[k], host[k], rtol=1e-9, atol=1e-12)
+    b1.close()
+    b2.close()
+
+
+def test_full_jones_warm_start_apply_rows():
+    """ddh_warmstart_apply_2x2 scales row p of each matrix by exp(pred_p)
+    (log-magnitude, phase pairs) and starts a matrix with a non-finite entry
+    from the identity; the features of a 2x2 solution equal those of the
+    pair of its diagonal entries."""
+    print("case: full-Jones warm start apply + features", flush=True)
+    rng = np.random.default_rng(3)
+    n_cb, n_ant, n_sub = 2, 5, 3
+    n = n_cb * n_ant * n_sub
+    m = rng.normal(size=(n, 4)) + 1j * rng.normal(size=(n, 4))
+    m[7, 2] = np.nan
+    pred = rng.normal(scale=0.2, size=(n, 4)).astype(np.float32)
+    out = H.warmstart_apply(m, pred, full_jones=True).reshape(n, 4)
+    f = np.exp(pred[:, 0::2].astype(np.float64) + 1j * pred[:, 1::2].astype(np.float64))
+    ref = m.copy()
+    ref[7] = [1, 0, 0, 1]
+    ref[:, :2] *= f[:, :1]
+    ref[:, 2:] *= f[:, 1:]
+    np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-14)
+    # features from the diagonal entries
+    its = [rng.normal(size=(n, 4)) + 1j * rng.normal(size=(n, 4)) for _ in range(4)]
+    den = rng.uniform(0.5, 2.0, size=(n, 2))
+    x4 = H.warmstart_features(*its, den, n_cb, n_ant, n_sub, full_jones=True)
+    x2 = H.warmstart_features(*[a[:, ::3] for a in its], den, n_cb, n_ant, n_sub)
+    np.testing.assert_array_equal(x4, x2)
+
+
+def test_full_jones_warm_start_converges_sooner():
+    """Consecutive full-Jones intervals: trained on the first intervals, the
+    learned start (from the second stretch's iterate) reaches the same chi^2
+    as the previous solutions in fewer solve iterations."""
+    print("case: full-Jones warm start sequence", flush=True)
+    w = S.Workload("wsfs", 12, 3, 2, 1, 12, 3, 4, 400, 1e-6, seed=7)
+    seq = WS.IntervalSequence(w, 20, seed=7, solver_kwargs=FJ)
+    ws = WS.WarmStart(hidden=128, seed=0)
+    x, t, prev = WS.training_set(seq, ws, 0, 14, converge_iterations=400)
+    loss = ws.fit(x, t, epochs=30, batch=512, lr=1e-3)
+    print(f"train samples {len(x)}, last epoch loss {loss:.3e}, target var {t.var():.3e}")
+    its = {"previous": [], "learned": []}
+    chis = {k: [] for k in its}
+    for i in range(14, 20):
+        dp = seq.problems[i]
+        starts = {"previous": prev.copy(), "learned": ws.initial(seq.solver, dp, prev)}
+        finals = {}
+        for k, sol in starts.items():
+            r = seq.solver.solve_problem(dp, sol)
+            its[k].append(r.iterations)
+            chis[k].append(H.chi2(seq.solver, dp, sol))
+            finals[k] = sol
+        prev = finals["previous"]
+    for k in its:
+        print(f"{k:9s} mean iterations {np.mean(its[k]):6.1f}  mean chi2 {np.mean(chis[k]):.6e}")
+    print(f"learned incl. feature iterations {np.mean(its['learned']) + 2 * ws.stretch + 1:.1f}")
+    np.testing.assert_allclose(chis["learned"], chis["previous"], rtol=1e-3)
+    assert np.mean(its["learned"]) < np.mean(its["previous"])
