@@ -6,7 +6,7 @@
 //
 // Buffers are in DPBuffer order for the time steps of one solution interval:
 // data [n_t][n_bl][n_ch][4] and model [n_dir][n_t][n_bl][n_ch][4] complex64,
-// solutions [n_cb][n_ant][n_dir][n_pol] complex128. One thread per
+// solutions [n_cb][n_ant][n_dir][n_pol] complex128. Two threads per
 // visibility; HBM bound: per visibility 32 B of data read and written and
 // n_dir x 32 B of model read (and written with keep_model).
 #include <hip/hip_runtime.h>
@@ -151,6 +151,155 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The solution factors of one (row, block, direction) in LDS, in double2
+// values: scalar s1 conj(s2); diagonal f_hk = s1_h conj(s2_k) (row-major);
+// full-Jones S1 then S2 (row-major 2x2 each).
+template <int NPOL>
+constexpr int RowEntry() {
+  return NPOL == 1 ? 1 : NPOL == 2 ? 4 : 8;
+}
+constexpr int kRowTableBytes = 24576;  // factor table per workgroup
+constexpr int kMaxRowChannels = 8192;  // channel -> block table (int16)
+
+// The same products over whole rows (time, baseline) of the buffers: a
+// workgroup takes `rows` consecutive rows (rows * n_ch * 64 contiguous bytes
+// of data and of each direction's model), first forms every (row, block,
+// direction) factor of the solutions in LDS, then streams the rows with two
+// lanes per visibility as above. Each lane reads its factors from LDS (two
+// 16-byte words per direction for diagonal solutions) instead of the
+// solution entries of both antennas from global memory, and the per-channel
+// block comes from a table built once per workgroup.
+template <int NPOL>
+__global__ void __launch_bounds__(256)
+    CorrectSubtractRowsKernel(float4_v* __restrict__ data, float4_v* __restrict__ model,
+                              const double2* __restrict__ sol, const int32_t* __restrict__ ant1,
+                              const int32_t* __restrict__ ant2,
+                              const int32_t* __restrict__ cb_start, int n_t, int n_bl, int n_ch,
+                              int n_cb, int n_dir, int n_ant, int rows, int subtract,
+                              int keep_model) {
+  constexpr int E = RowEntry<NPOL>();
+  extern __shared__ double2 s_dyn[];
+  double2* s_tab = s_dyn;  // [rows][n_cb][n_dir][E]
+  int16_t* s_cb = reinterpret_cast<int16_t*>(s_dyn + size_t(rows) * n_cb * n_dir * E);
+  // channel -> block: the first cb with cb_start[cb + 1] > ch
+  for (int c = threadIdx.x; c < n_ch; c += blockDim.x) {
+    int lo = 0, hi = n_cb - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (cb_start[mid + 1] > c)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    s_cb[c] = int16_t(lo);
+  }
+  const int64_t n_rows = int64_t(n_t) * n_bl;
+  const int64_t n_vis = n_rows * n_ch;
+  const int64_t n_groups = (n_rows + rows - 1) / rows;
+  const int h = threadIdx.x & 1;
+  const int per_row = n_cb * n_dir;
+  for (int64_t grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
+    const int64_t r0 = grp * rows;
+    const int nr = int(std::min<int64_t>(rows, n_rows - r0));
+    __syncthreads();  // the previous group's readers are done (and s_cb is written)
+    for (int e = threadIdx.x; e < nr * per_row; e += blockDim.x) {
+      const int d = e % n_dir;
+      const int cb = (e / n_dir) % n_cb;
+      const int bl = int((r0 + e / per_row) % n_bl);
+      const double2* p1 = sol + ((size_t(cb) * n_ant + ant1[bl]) * n_dir + d) * NPOL;
+      const double2* p2 = sol + ((size_t(cb) * n_ant + ant2[bl]) * n_dir + d) * NPOL;
+      double2* t = s_tab + size_t(e) * E;
+      if constexpr (NPOL == 1) {
+        const C2 f = MulC({p1[0].x, p1[0].y}, {p2[0].x, p2[0].y});
+        t[0] = make_double2(f.re, f.im);
+      } else if constexpr (NPOL == 2) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int k = 0; k < 2; ++k) {
+            const C2 f = MulC({p1[i].x, p1[i].y}, {p2[k].x, p2[k].y});
+            t[2 * i + k] = make_double2(f.re, f.im);
+          }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          t[k] = p1[k];
+          t[4 + k] = p2[k];
+        }
+      }
+    }
+    __syncthreads();
+    const int64_t g0 = 2 * r0 * n_ch;
+    const int span = 2 * nr * n_ch;  // lanes of this group (even: pairs stay together)
+    for (int l = threadIdx.x; l < span; l += blockDim.x) {
+      const int64_t g = g0 + l;
+      const uint32_t v = uint32_t(l) >> 1;
+      const uint32_t rr = v / uint32_t(n_ch);
+      const int ch = int(v - rr * uint32_t(n_ch));
+      const double2* tab = s_tab + (size_t(rr) * n_cb + s_cb[ch]) * n_dir * E;
+      float4_v* dp = data + size_t(g);
+      float4_v dv = {0, 0, 0, 0};
+      if (subtract) dv = __builtin_nontemporal_load(dp);
+      C2 sum[2] = {{0, 0}, {0, 0}};
+      for (int d0i = 0; d0i < n_dir; d0i += kDirBatch) {
+        float4_v mv[kDirBatch];
+#pragma unroll
+        for (int j = 0; j < kDirBatch; ++j) {
+          const int d = d0i + j < n_dir ? d0i + j : d0i;
+          mv[j] = __builtin_nontemporal_load(model + size_t(d) * n_vis * 2 + size_t(g));
+        }
+#pragma unroll
+        for (int j = 0; j < kDirBatch; ++j) {
+          const int d = d0i + j;
+          if (d >= n_dir) break;
+          const float4_v m = mv[j];
+          const C2 mr[2] = {{m.x, m.y}, {m.z, m.w}};
+          const double2* f = tab + d * E;
+          C2 out[2];
+          if constexpr (NPOL == 1) {
+            const C2 fs = {f[0].x, f[0].y};
+            out[0] = Mul(mr[0], fs);
+            out[1] = Mul(mr[1], fs);
+          } else if constexpr (NPOL == 2) {
+            const double2 f0 = f[2 * h], f1 = f[2 * h + 1];
+            out[0] = Mul(mr[0], {f0.x, f0.y});
+            out[1] = Mul(mr[1], {f1.x, f1.y});
+          } else {
+            float4_v o;
+            o.x = __shfl_xor(m.x, 1);
+            o.y = __shfl_xor(m.y, 1);
+            o.z = __shfl_xor(m.z, 1);
+            o.w = __shfl_xor(m.w, 1);
+            const C2 orow[2] = {{o.x, o.y}, {o.z, o.w}};
+            const C2* r0m = h == 0 ? mr : orow;  // row 0 of M
+            const C2* r1m = h == 0 ? orow : mr;  // row 1
+            const double2 x0 = f[2 * h], x1 = f[2 * h + 1];
+            const C2 a0 = {x0.x, x0.y}, a1v = {x1.x, x1.y};
+            const C2 b[4] = {{f[4].x, f[4].y}, {f[5].x, f[5].y}, {f[6].x, f[6].y},
+                             {f[7].x, f[7].y}};
+            const C2 t0 = Add(Mul(a0, r0m[0]), Mul(a1v, r1m[0]));
+            const C2 t1 = Add(Mul(a0, r0m[1]), Mul(a1v, r1m[1]));
+            out[0] = Add(MulC(t0, b[0]), MulC(t1, b[1]));
+            out[1] = Add(MulC(t0, b[2]), MulC(t1, b[3]));
+          }
+          sum[0] = Add(sum[0], out[0]);
+          sum[1] = Add(sum[1], out[1]);
+          if (keep_model)
+            model[size_t(d) * n_vis * 2 + size_t(g)] =
+                float4_v{float(out[0].re), float(out[0].im), float(out[1].re), float(out[1].im)};
+        }
+      }
+      if (subtract) {
+        dv.x = float(double(dv.x) - sum[0].re);
+        dv.y = float(double(dv.y) - sum[0].im);
+        dv.z = float(double(dv.z) - sum[1].re);
+        dv.w = float(double(dv.w) - sum[1].im);
+        *dp = dv;
+      }
+    }
+  }
+}
+
 template <typename T>
 struct Scratch {
   T* ptr = nullptr;
@@ -178,10 +327,36 @@ void Launch(int32_t n_t, int32_t n_bl, int32_t n_ch, int32_t n_cb, int32_t n_dir
             const double* d_sol, void* d_data, void* d_model, int subtract, int keep_model,
             hipStream_t stream) {
   const int64_t n_vis = int64_t(n_t) * n_bl * n_ch;
-  const unsigned blocks = unsigned(std::min<int64_t>((2 * n_vis + 255) / 256, 1 << 20));
   auto* data = reinterpret_cast<float4_v*>(d_data);
   auto* model = reinterpret_cast<float4_v*>(d_model);
   auto* sol = reinterpret_cast<const double2*>(d_sol);
+  // whole rows with the solution factors in LDS where one row's factors fit
+  const int entry = n_pol == 1 ? RowEntry<1>() : n_pol == 2 ? RowEntry<2>() : RowEntry<4>();
+  const size_t row_bytes = size_t(n_cb) * n_dir * entry * sizeof(double2);
+  if (row_bytes <= size_t(kRowTableBytes) && n_ch <= kMaxRowChannels && n_cb <= INT16_MAX &&
+      int64_t(2) * n_ch * (kRowTableBytes / row_bytes) < INT32_MAX) {
+    // rows per workgroup: as many as the table holds, up to about 16 passes
+    // of the workgroup's lanes
+    const int rows = int(std::max<size_t>(
+        1, std::min<size_t>(kRowTableBytes / row_bytes, size_t(16 * 256) / (2 * size_t(n_ch)))));
+    const int64_t n_rows = int64_t(n_t) * n_bl;
+    const unsigned groups = unsigned(std::min<int64_t>((n_rows + rows - 1) / rows, 1 << 20));
+    const size_t lds = size_t(rows) * row_bytes + ((size_t(n_ch) * 2 + 15) & ~size_t(15));
+#define DDH_ROWS_LAUNCH(NP)                                                                      \
+  hipLaunchKernelGGL(CorrectSubtractRowsKernel<NP>, dim3(groups), dim3(256), lds, stream, data, \
+                     model, sol, d_ant1, d_ant2, d_cb_start, n_t, n_bl, n_ch, n_cb, n_dir, n_ant, \
+                     rows, subtract, keep_model)
+    if (n_pol == 1)
+      DDH_ROWS_LAUNCH(1);
+    else if (n_pol == 2)
+      DDH_ROWS_LAUNCH(2);
+    else
+      DDH_ROWS_LAUNCH(4);
+#undef DDH_ROWS_LAUNCH
+    DDH_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  const unsigned blocks = unsigned(std::min<int64_t>((2 * n_vis + 255) / 256, 1 << 20));
   if (n_pol == 1)
     hipLaunchKernelGGL(CorrectSubtractKernel<1>, dim3(blocks), dim3(256), 0, stream, data, model,
                        sol, d_ant1, d_ant2, d_cb_start, n_t, n_bl, n_ch, n_cb, n_dir, n_ant, subtract,

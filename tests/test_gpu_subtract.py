@@ -66,3 +66,22 @@ def test_correct_and_subtract_single_channel_blocks(n_pol):
     data, model, sol, a1, a2, cbs, n_ant = case(n_pol, n_ch=11, starts=(0, 3, 3, 10))
     with pytest.raises(H.HipError, match="channel"):
         H.correct_and_subtract(data, model, sol, a1, a2, cbs, n_ant, 1, 1)
+
+
+@pytest.mark.parametrize("n_pol", [1, 2, 4])
+def test_correct_and_subtract_many_blocks(n_pol):
+    """100 uneven channel blocks of 120 channels and 4 directions: the scalar
+    solutions' factors of three rows fit one workgroup's LDS table, the
+    diagonal and full-Jones ones (25.6 and 51.2 KB per row) take the kernel
+    that reads the solution entries from global memory; both against the
+    oracle."""
+    print(f"case: correct and subtract 100 blocks n_pol={n_pol}", flush=True)
+    rng = np.random.default_rng(8)
+    inner = np.sort(rng.choice(np.arange(1, 120), size=99, replace=False))
+    starts = tuple([0] + inner.tolist())
+    data, model, sol, a1, a2, cbs, n_ant = case(n_pol, seed=9, n_t=2, n_ant=6, n_ch=120,
+                                                starts=starts)
+    d_ref, m_ref = O.correct_and_subtract(data, model, sol, a1, a2, cbs, n_ant, 1, 1)
+    d, m, ms = H.correct_and_subtract(data, model, sol, a1, a2, cbs, n_ant, 1, 1)
+    within_one_ulp(d, d_ref, model, sol)
+    within_one_ulp(m, m_ref, model, sol)
