@@ -285,8 +285,9 @@ __global__ void __launch_bounds__(256)
           sum[0] = Add(sum[0], out[0]);
           sum[1] = Add(sum[1], out[1]);
           if (keep_model)
-            model[size_t(d) * n_vis * 2 + size_t(g)] =
-                float4_v{float(out[0].re), float(out[0].im), float(out[1].re), float(out[1].im)};
+            __builtin_nontemporal_store(
+                float4_v{float(out[0].re), float(out[0].im), float(out[1].re), float(out[1].im)},
+                model + size_t(d) * n_vis * 2 + size_t(g));
         }
       }
       if (subtract) {
@@ -294,7 +295,7 @@ __global__ void __launch_bounds__(256)
         dv.y = float(double(dv.y) - sum[0].im);
         dv.z = float(double(dv.z) - sum[1].re);
         dv.w = float(double(dv.w) - sum[1].im);
-        *dp = dv;
+        __builtin_nontemporal_store(dv, dp);  // streamed out, not read again
       }
     }
   }
@@ -303,11 +304,19 @@ __global__ void __launch_bounds__(256)
 template <typename T>
 struct Scratch {
   T* ptr = nullptr;
+  size_t n = 0;
   ~Scratch() {
     if (ptr) (void)hipFree(ptr);
   }
-  T* Get(size_t n) {
-    DDH_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&ptr), std::max<size_t>(n, 1) * sizeof(T)));
+  // grows only: a later call of the same or a smaller shape reuses the memory
+  T* Get(size_t count) {
+    if (count > n) {
+      if (ptr) DDH_HIP_CHECK(hipFree(ptr));
+      ptr = nullptr;
+      n = 0;
+      DDH_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&ptr), std::max<size_t>(count, 1) * sizeof(T)));
+      n = count;
+    }
     return ptr;
   }
 };
@@ -335,10 +344,11 @@ void Launch(int32_t n_t, int32_t n_bl, int32_t n_ch, int32_t n_cb, int32_t n_dir
   const size_t row_bytes = size_t(n_cb) * n_dir * entry * sizeof(double2);
   if (row_bytes <= size_t(kRowTableBytes) && n_ch <= kMaxRowChannels && n_cb <= INT16_MAX &&
       int64_t(2) * n_ch * (kRowTableBytes / row_bytes) < INT32_MAX) {
-    // rows per workgroup: as many as the table holds, up to about 16 passes
-    // of the workgroup's lanes
+    // rows per workgroup: as many as the table holds, up to about two passes
+    // of the workgroup's lanes (at 240 channels one row: 480 lanes; more rows
+    // per group, or one pass of 512 lanes, measured slower)
     const int rows = int(std::max<size_t>(
-        1, std::min<size_t>(kRowTableBytes / row_bytes, size_t(16 * 256) / (2 * size_t(n_ch)))));
+        1, std::min<size_t>(kRowTableBytes / row_bytes, size_t(2 * 256) / (2 * size_t(n_ch)))));
     const int64_t n_rows = int64_t(n_t) * n_bl;
     const unsigned groups = unsigned(std::min<int64_t>((n_rows + rows - 1) / rows, 1 << 20));
     const size_t lds = size_t(rows) * row_bytes + ((size_t(n_ch) * 2 + 15) & ~size_t(15));
@@ -420,9 +430,10 @@ int ddh_correct_and_subtract(int32_t n_times, int32_t n_baselines, int32_t n_cha
                   -1, "antenna index out of range");
     const size_t n_vis = size_t(n_times) * n_baselines * n_channels;
     const size_t n_sol = size_t(n_channel_blocks) * n_antennas * n_directions * n_pol;
-    Scratch<int32_t> a1, a2, cbs;
-    Scratch<double> sol;
-    Scratch<float> d, m;
+    // device copies of the inputs, kept per host thread for the next call
+    static thread_local Scratch<int32_t> a1, a2, cbs;
+    static thread_local Scratch<double> sol;
+    static thread_local Scratch<float> d, m;
     hipStream_t stream = nullptr;
     DDH_HIP_CHECK(hipMemcpy(a1.Get(n_baselines), antenna1, n_baselines * 4, hipMemcpyHostToDevice));
     DDH_HIP_CHECK(hipMemcpy(a2.Get(n_baselines), antenna2, n_baselines * 4, hipMemcpyHostToDevice));

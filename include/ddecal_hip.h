@@ -407,7 +407,9 @@ int ddh_batch_autotune(ddh_batch* batch, int32_t* waves_per_tile, int32_t* rows_
 //         at most 1024 channel blocks
 // The _device form takes device pointers for everything and enqueues on
 // `stream` (a hipStream_t, NULL for the default stream); the host form copies
-// host arrays in and out and reports the kernel's device time.
+// host arrays in and out and reports the kernel's device time. Its device
+// copies are kept per host thread and reused by later calls of the same or a
+// smaller size (freed when the thread ends).
 int ddh_correct_and_subtract_device(int32_t n_times, int32_t n_baselines, int32_t n_channels,
                                     int32_t n_directions, int32_t n_antennas,
                                     int32_t n_channel_blocks, int32_t n_pol,

@@ -85,3 +85,50 @@ def test_correct_and_subtract_many_blocks(n_pol):
     d, m, ms = H.correct_and_subtract(data, model, sol, a1, a2, cbs, n_ant, 1, 1)
     within_one_ulp(d, d_ref, model, sol)
     within_one_ulp(m, m_ref, model, sol)
+
+
+def _hip_runtime():
+    import ctypes as C
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            rt = C.CDLL(name)
+            break
+        except OSError:
+            rt = None
+    assert rt is not None, "HIP runtime not found"
+    rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    rt.hipFree.argtypes = [C.c_void_p]
+    return rt
+
+
+@pytest.mark.parametrize("n_pol", [2, 4])
+def test_device_entry_equals_host_entry(n_pol):
+    """ddh_correct_and_subtract_device on caller-owned device buffers (default
+    stream) gives the host entry's outputs bit for bit."""
+    import ctypes as C
+    print(f"case: correct and subtract device entry n_pol={n_pol}", flush=True)
+    data, model, sol, a1, a2, cbs, n_ant = case(n_pol, n_ch=30, starts=(0, 7, 19))
+    d_host, m_host, _ = H.correct_and_subtract(data, model, sol, a1, a2, cbs, n_ant, 1, 1)
+    rt = _hip_runtime()
+    arrays = [np.ascontiguousarray(a1), np.ascontiguousarray(a2), np.ascontiguousarray(cbs),
+              np.ascontiguousarray(sol, np.complex128), data.copy(), model.copy()]
+    ptrs = []
+    try:
+        for a in arrays:
+            p = C.c_void_p()
+            assert rt.hipMalloc(C.byref(p), a.nbytes) == 0
+            ptrs.append(p)
+            assert rt.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0
+        n_t, n_bl, n_ch = data.shape[:3]
+        rc = H.lib().ddh_correct_and_subtract_device(n_t, n_bl, n_ch, model.shape[0], n_ant,
+                                                     len(cbs) - 1, n_pol, *ptrs, 1, 1, None)
+        assert rc == 0
+        d_dev, m_dev = np.empty_like(data), np.empty_like(model)
+        assert rt.hipMemcpy(d_dev.ctypes.data, ptrs[4], d_dev.nbytes, 2) == 0
+        assert rt.hipMemcpy(m_dev.ctypes.data, ptrs[5], m_dev.nbytes, 2) == 0
+    finally:
+        for p in ptrs:
+            rt.hipFree(p)
+    np.testing.assert_array_equal(d_dev, d_host)
+    np.testing.assert_array_equal(m_dev, m_host)
