@@ -275,14 +275,18 @@ def test_full_jones_one_step_matches_oracle():
     s.close()
 
 
-def test_full_jones_batched_warm_start_matches_host_sequence():
+@pytest.mark.parametrize("iteration", [H.ITERATION_STREAM, H.ITERATION_GRAM])
+def test_full_jones_batched_warm_start_matches_host_sequence(iteration):
     """ddh_batch_solve_warm on full-Jones intervals: the same starts and solves
     as the per-interval host sequence one_step -> 2 x stretch iterations ->
     ddh_warmstart_features_2x2 -> forward -> ddh_warmstart_apply_2x2,
-    followed by a batched solve (same schedule on both sides)."""
-    print("case: full-Jones batched warm start vs host sequence", flush=True)
+    followed by a batched solve (same schedule on both sides), with the
+    streaming and the Gram iteration."""
+    print(f"case: full-Jones batched warm start vs host sequence, iteration {iteration}",
+          flush=True)
     w = S.Workload("wsf", 12, 3, 2, 1, 12, 3, 4, 60, 1e-5, seed=9)
-    seqs = [WS.IntervalSequence(w, 2, seed=60 + k, solver_kwargs=FJ) for k in range(3)]
+    seqs = [WS.IntervalSequence(w, 2, seed=60 + k, solver_kwargs=dict(FJ, iteration=iteration))
+            for k in range(3)]
     ws = WS.WarmStart(hidden=64, seed=1, stretch=3)
     prevs = []
     for q in seqs:
