@@ -158,7 +158,9 @@ def main():
     ap.add_argument("--plain-streams", type=int, default=2,
                     help="streams the unbatched plain forms are timed on")
     ap.add_argument("--train", type=int, default=150, help="training intervals per stream")
-    ap.add_argument("--stretch", type=int, default=4)
+    ap.add_argument("--stretch", type=int, default=2,
+                    help="damped iterations per warm-start stretch (2 measured best at this "
+                         "workload: 1, 3, 4 and 6 gave fewer solves/s)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
