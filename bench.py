@@ -161,6 +161,9 @@ def main():
     ap.add_argument("--stretch", type=int, default=2,
                     help="damped iterations per warm-start stretch (2 measured best at this "
                          "workload: 1, 3, 4 and 6 gave fewer solves/s)")
+    ap.add_argument("--epochs", type=int, default=10, help="warm-start training epochs")
+    ap.add_argument("--fine-epochs", type=int, default=0,
+                    help="further warm-start training epochs at a tenth of the learning rate")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
@@ -208,7 +211,9 @@ def main():
     prevs = [s[2] for s in sets]
     log(f"{B} x {TR} training intervals solved ({time.perf_counter() - t_setup:.1f} s)")
     t_fit = time.perf_counter()
-    loss = ws.fit(x, t, epochs=10, batch=4096, lr=1e-3, seed=args.seed)
+    loss = ws.fit(x, t, epochs=args.epochs, batch=4096, lr=1e-3, seed=args.seed)
+    if args.fine_epochs > 0:  # a second phase at a tenth of the step
+        loss = ws.fit(x, t, epochs=args.fine_epochs, batch=4096, lr=1e-4, seed=args.seed + 1)
     t_fit = time.perf_counter() - t_fit
     log(f"warm start: {len(x)} samples, loss {loss:.3e}, target variance {t.var():.3e}, "
         f"fit {t_fit:.1f} s")
