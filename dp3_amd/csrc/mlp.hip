@@ -466,11 +466,15 @@ __global__ void WarmApplyKernel(const double2* __restrict__ prev_all,
       }
       continue;
     }
+    const float* pr = pred + i * kWarmOutputs;
+    // a scalar solution: both values with the outputs' mean (float, as the
+    // host helpers form it)
+    const float sym[2] = {0.5f * (pr[0] + pr[2]), 0.5f * (pr[1] + pr[3])};
     for (int p = 0; p < 2; ++p) {
       double2 g = prev[i * 2 + p];
       if (!(isfinite(g.x) && isfinite(g.y))) g = make_double2(1.0, 0.0);
-      const double lm = pred[i * kWarmOutputs + 2 * p];
-      const double an = pred[i * kWarmOutputs + 2 * p + 1];
+      const double lm = sp.scalar ? sym[0] : pr[2 * p];
+      const double an = sp.scalar ? sym[1] : pr[2 * p + 1];
       double sn, cs;
       sincos(an, &sn, &cs);
       const double sc = exp(lm);
@@ -784,7 +788,7 @@ void WarmFeaturesHost(const double* g0, const double* next1, const double* ga, c
   DDH_HIP_CHECK(hipMemcpy(dd.Get(n_pairs * 2), den, n_pairs * 2 * sizeof(double),
                           hipMemcpyHostToDevice));
   Buf<WarmSpan> span;
-  const WarmSpan one{0, n_cb, n_ant, n_sub, ne};
+  const WarmSpan one{0, n_cb, n_ant, n_sub, ne, 0};
   DDH_HIP_CHECK(hipMemcpy(span.Get(1), &one, sizeof(one), hipMemcpyHostToDevice));
   LaunchWarmFeatures(d0.ptr, d1.ptr, da.ptr, db.ptr, dd.ptr, span.ptr, 1, n_pairs,
                      df.Get(n_pairs * kWarmFeatures), nullptr);
@@ -802,7 +806,7 @@ void WarmApplyHost(const double* prev, const float* pred, int64_t n_pairs, int n
   DDH_HIP_CHECK(hipMemcpy(dpred.Get(n_pairs * kWarmOutputs), pred,
                           n_pairs * kWarmOutputs * sizeof(float), hipMemcpyHostToDevice));
   Buf<WarmSpan> span;
-  const WarmSpan one{0, 1, 1, int32_t(n_pairs), ne};
+  const WarmSpan one{0, 1, 1, int32_t(n_pairs), ne, 0};
   DDH_HIP_CHECK(hipMemcpy(span.Get(1), &one, sizeof(one), hipMemcpyHostToDevice));
   LaunchWarmApply(dp.ptr, dpred.ptr, span.ptr, 1, n_pairs, di.Get(n_pairs * ne), nullptr);
   DDH_HIP_CHECK(hipMemcpy(init, di.ptr, n_pairs * ne * sizeof(double2), hipMemcpyDeviceToHost));

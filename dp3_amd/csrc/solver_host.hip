@@ -788,23 +788,23 @@ void SolutionsIn(const ddh_solver& s, double2* dev, const double* host, size_t n
   DDH_HIP_CHECK(hipStreamSynchronize(stream));
 }
 
-// Device pairs to the caller's layout (complex values, or with `real` one
-// double per polarization, as the one-step denominators).
+// Device pairs to the caller's layout.
 void SolutionsOut(const ddh_solver& s, double* host, const void* dev, size_t n_pairs,
-                  hipStream_t stream, bool real = false) {
-  const size_t value = real ? sizeof(double) : sizeof(double2);
+                  hipStream_t stream) {
   const int np = s.cfg.c.n_solution_polarizations;
   if (np == 2 || np == 4) {
-    DDH_HIP_CHECK(hipMemcpyAsync(host, dev, n_pairs * np * value, hipMemcpyDeviceToHost, stream));
+    DDH_HIP_CHECK(hipMemcpyAsync(host, dev, n_pairs * np * sizeof(double2),
+                                 hipMemcpyDeviceToHost, stream));
     return;
   }
-  std::vector<double> tmp(n_pairs * 2 * value / sizeof(double));
-  DDH_HIP_CHECK(hipMemcpyAsync(tmp.data(), dev, n_pairs * 2 * value, hipMemcpyDeviceToHost,
-                               stream));
+  std::vector<double2> tmp(n_pairs * 2);
+  DDH_HIP_CHECK(hipMemcpyAsync(tmp.data(), dev, n_pairs * 2 * sizeof(double2),
+                               hipMemcpyDeviceToHost, stream));
   DDH_HIP_CHECK(hipStreamSynchronize(stream));
-  const size_t words = value / sizeof(double);
-  for (size_t i = 0; i < n_pairs; ++i)
-    for (size_t w = 0; w < words; ++w) host[i * words + w] = tmp[2 * i * words + w];
+  for (size_t i = 0; i < n_pairs; ++i) {
+    host[2 * i] = tmp[2 * i].x;
+    host[2 * i + 1] = tmp[2 * i].y;
+  }
 }
 
 // All channel blocks' solutions of one solver between the caller's per-block
@@ -1851,7 +1851,7 @@ void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* cons
   for (int i = 0; i < n; ++i) {
     const ddh_config& c = b.solvers[i]->cfg.c;
     spans[i] = WarmSpan{int64_t(off[i] / ne), c.n_channel_blocks, c.n_antennas,
-                        b.solvers[i]->cfg.n_sub, ne};
+                        b.solvers[i]->cfg.n_sub, ne, c.n_solution_polarizations == 1 ? 1 : 0};
     max_count = std::max<int64_t>(max_count, int64_t(off[i + 1] - off[i]));
   }
   const bool same_spans =
@@ -1859,7 +1859,7 @@ void SolveBatchWarm(ddh_batch& b, ddh_mlp& net, int stretch, double* const* cons
       std::equal(spans.begin(), spans.end(), b.warm_spans.begin(), b.warm_spans.end(),
                  [](const WarmSpan& x, const WarmSpan& y) {
                    return x.pair0 == y.pair0 && x.n_cb == y.n_cb && x.n_ant == y.n_ant &&
-                          x.n_sub == y.n_sub && x.ne == y.ne;
+                          x.n_sub == y.n_sub && x.ne == y.ne && x.scalar == y.scalar;
                  });
   if (!same_spans) {
     b.d_warm_off.Upload(off64, b.stream);
@@ -2543,13 +2543,12 @@ int ddh_one_step(ddh_solver* s, ddh_problem* p, double* const* solutions_in,
     LaunchCorrelateAndSolve(*s, a, *p);
     for (int cb = 0; cb < c.n_channel_blocks; ++cb) {
       SolutionsOut(*s, next_out[cb], s->sol1.ptr + cb * per_cb, n_sols, s->stream);
-      if (!den_out) continue;
-      if (c.n_solution_polarizations == 4)  // the diagonal of each 2x2 denominator
+      // two per solution in every layout: a pair's, a scalar solution's
+      // (both its polarizations' summed denominator), the diagonal of a 2x2 one
+      if (den_out)
         DDH_HIP_CHECK(hipMemcpyAsync(den_out[cb], s->den.ptr + cb * n_sols * 2,
                                      n_sols * 2 * sizeof(double), hipMemcpyDeviceToHost,
                                      s->stream));
-      else
-        SolutionsOut(*s, den_out[cb], s->den.ptr + cb * n_sols * 2, n_sols, s->stream, true);
     }
     DDH_HIP_CHECK(hipStreamSynchronize(s->stream));
   });

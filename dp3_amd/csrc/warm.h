@@ -27,10 +27,11 @@ void MlpForwardRows(ddh_mlp& net, const float* x, int64_t n, float* y, hipStream
 // first solution (ne complex values each: 2 for a diagonal pair, 4 for a
 // 2x2 matrix) and its shape (n_cb * n_ant * n_sub solutions). The features
 // and the start use polarizations 0 and 1: the pair's values, or the
-// matrix's diagonal entries.
+// matrix's diagonal entries. scalar: the pair holds one gain twice (the
+// scalar solver), and the start keeps it so.
 struct WarmSpan {
   int64_t pair0;
-  int32_t n_cb, n_ant, n_sub, ne;
+  int32_t n_cb, n_ant, n_sub, ne, scalar;
 };
 // Features [pairs][kWarmFeatures] of every span (device array of n_spans;
 // max_pairs: the largest span) in one launch.
@@ -38,7 +39,8 @@ void LaunchWarmFeatures(const double2* g0, const double2* next1, const double2* 
                         const double2* gb, const double* den, const WarmSpan* spans, int n_spans,
                         int64_t max_pairs, float* features, hipStream_t stream);
 // init = g0 * exp(pred) per polarization (for a 2x2 matrix: row p of g0
-// times exp(pred_p)), every span in one launch.
+// times exp(pred_p); for a scalar span both values times the factor of the
+// two outputs' mean), every span in one launch.
 void LaunchWarmApply(const double2* g0, const float* pred, const WarmSpan* spans, int n_spans,
                      int64_t max_pairs, double2* init, hipStream_t stream);
 

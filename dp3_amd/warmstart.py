@@ -42,6 +42,17 @@ def is_full_jones(solver: H.Solver):
     return solver.n_solution_polarizations == 4
 
 
+def is_scalar(solver: H.Solver):
+    return solver.n_solution_polarizations == 1
+
+
+def as_pairs(solutions, scalar):
+    """Scalar solutions [n_cb, n] as the pairs [n_cb, 2n] that hold each gain
+    twice (the layout the device solves them in); other layouts unchanged."""
+    g = np.asarray(solutions, np.complex128)
+    return np.repeat(g, 2, axis=-1) if scalar else g
+
+
 def diagonal(solutions, full_jones):
     """[n, 2] polarization values: a pair, or a 2x2 matrix's J_00, J_11."""
     g = np.asarray(solutions, np.complex128)
@@ -80,17 +91,19 @@ class WarmStart:
         aux.solve_problem(dproblem, ga)
         gb = ga.copy()
         aux.solve_problem(dproblem, gb)
-        x = H.warmstart_features(g0, nxt, ga, gb, den, solver.n_channel_blocks,
-                                 solver.n_antennas, solver.n_sub, is_full_jones(solver))
+        sc = is_scalar(solver)
+        x = H.warmstart_features(*[as_pairs(g, sc) for g in (g0, nxt, ga, gb)], den,
+                                 solver.n_channel_blocks, solver.n_antennas, solver.n_sub,
+                                 is_full_jones(solver))
         return (x, gb) if return_stretch else x
 
     @staticmethod
-    def targets(base, final, full_jones=False):
+    def targets(base, final, full_jones=False, scalar=False):
         """log(final / base) per polarization as (log-magnitude, phase) pairs
         (base: the second stretch's iterate gb; for 2x2 solutions the ratio of
-        the diagonal entries)."""
-        p = diagonal(base, full_jones)
-        f = diagonal(final, full_jones)
+        the diagonal entries; a scalar gain's ratio for both)."""
+        p = diagonal(as_pairs(base, scalar), full_jones)
+        f = diagonal(as_pairs(final, scalar), full_jones)
         with np.errstate(all="ignore"):
             r = f / p
             out = np.stack([np.log(np.abs(r[:, 0])), np.angle(r[:, 0]),
@@ -106,6 +119,12 @@ class WarmStart:
     def initial(self, solver, dproblem, g0):
         x, gb = self.features(solver, dproblem, g0, return_stretch=True)
         pred = self.net.forward(x)
+        if is_scalar(solver):
+            # one gain: both outputs' mean (in float), applied to the pair
+            sym = np.float32(0.5) * (pred[:, 0:2] + pred[:, 2:4])  # (log-magnitude, phase)
+            pred = np.ascontiguousarray(np.concatenate([sym, sym], axis=1), np.float32)
+            start = H.warmstart_apply(as_pairs(gb, True), pred)
+            return np.ascontiguousarray(start.reshape(-1, 2)[:, 0].reshape(np.asarray(g0).shape))
         return H.warmstart_apply(gb, pred, is_full_jones(solver)).reshape(np.asarray(g0).shape)
 
 
@@ -178,6 +197,8 @@ class IntervalSequence:
     def unit(self):
         """Unit gains: ones, or identity matrices for 2x2 solutions."""
         n = self.w.n_stations * self.w.n_sub
+        if is_scalar(self.solver):
+            return np.ones((self.w.n_channel_blocks, n), np.complex128)
         if is_full_jones(self.solver):
             eye = np.tile(np.array([1, 0, 0, 1], np.complex128), n)
             return np.tile(eye, (self.w.n_channel_blocks, 1))
@@ -226,7 +247,7 @@ def training_set(seq: IntervalSequence, ws: WarmStart, first, last, converge_ite
         xs.append(x)
         sol = prev.copy()
         long.solve_problem(dp, sol)
-        ts.append(ws.targets(gb, sol, is_full_jones(seq.solver)))
+        ts.append(ws.targets(gb, sol, is_full_jones(seq.solver), is_scalar(seq.solver)))
         prev = sol
     return np.concatenate(xs), np.concatenate(ts), prev
 
@@ -258,7 +279,8 @@ def training_sets_batched(seqs, ws: WarmStart, first, last, converge_iterations=
         sol = [p.copy() for p in prev]
         batch.solve(sol)
         for k in range(len(seqs)):
-            ts[k].append(ws.targets(gbs[k], sol[k], is_full_jones(seqs[k].solver)))
+            ts[k].append(ws.targets(gbs[k], sol[k], is_full_jones(seqs[k].solver),
+                                    is_scalar(seqs[k].solver)))
         prev = sol
     batch.close()
     for s in longs:

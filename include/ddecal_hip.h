@@ -292,8 +292,10 @@ int ddh_set_profiling(ddh_solver* solver, int32_t enable);
 // One undamped solver update from `solutions_in` (the solution of the
 // Jacobi step, IterativeDiagonalSolver.cc:206-221, without the damping of
 // SolverBase::Step), with its denominators: next_out[cb] like the solutions,
-// den_out[cb] -> [n_ant][n_sub][2] doubles (may be NULL; for the full-Jones
-// solver the diagonal of each 2x2 denominator, the solution of
+// den_out[cb] -> [n_ant][n_sub][2] doubles (may be NULL; for the scalar
+// solver both entries hold the sum over the polarizations that its update
+// divides by; for the full-Jones solver the diagonal of each 2x2
+// denominator, the solution of
 // IterativeFullJonesSolver.cc:178-189 without the damping). Used as the
 // data/model correlation summary of the warm start.
 int ddh_one_step(ddh_solver* solver, ddh_problem* problem, double* const* solutions_in,
@@ -455,6 +457,9 @@ int ddh_gemm_bf16(const float* a, const float* b, float* c, int32_t M, int32_t N
 // denominators den) and two later iterates ga, gb of the damped solver, each
 // relative to g0; and the initial solutions prev * exp(pred) from the
 // network's 4 outputs per pair (the batched warm start passes prev = gb).
+// Scalar solutions go through these as pairs holding each gain twice; the
+// batched warm start then scales both by the mean of the two polarizations'
+// outputs, formed in float (0.5f * (a + b)), so the pair stays one gain.
 int ddh_warmstart_features(const double* g0, const double* next1, const double* ga,
                            const double* gb, const double* den, int32_t n_cb, int32_t n_ant,
                            int32_t n_sub, float* features);

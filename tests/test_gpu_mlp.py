@@ -275,17 +275,12 @@ def test_full_jones_one_step_matches_oracle():
     s.close()
 
 
-@pytest.mark.parametrize("iteration", [H.ITERATION_STREAM, H.ITERATION_GRAM])
-def test_full_jones_batched_warm_start_matches_host_sequence(iteration):
-    """ddh_batch_solve_warm on full-Jones intervals: the same starts and solves
-    as the per-interval host sequence one_step -> 2 x stretch iterations ->
-    ddh_warmstart_features_2x2 -> forward -> ddh_warmstart_apply_2x2,
-    followed by a batched solve (same schedule on both sides), with the
-    streaming and the Gram iteration."""
-    print(f"case: full-Jones batched warm start vs host sequence, iteration {iteration}",
-          flush=True)
-    w = S.Workload("wsf", 12, 3, 2, 1, 12, 3, 4, 60, 1e-5, seed=9)
-    seqs = [WS.IntervalSequence(w, 2, seed=60 + k, solver_kwargs=dict(FJ, iteration=iteration))
+def batched_warm_matches_host(solver_kwargs, seed0):
+    """Batched warm start (ddh_batch_solve_warm) against the per-interval host
+    sequence followed by a batched solve under the same schedule: equal
+    iteration counts and solutions to FP64 rounding."""
+    w = S.Workload("wsx", 12, 3, 2, 1, 12, 3, 4, 60, 1e-5, seed=9)
+    seqs = [WS.IntervalSequence(w, 2, seed=seed0 + k, solver_kwargs=solver_kwargs)
             for k in range(3)]
     ws = WS.WarmStart(hidden=64, seed=1, stretch=3)
     prevs = []
@@ -308,6 +303,30 @@ def test_full_jones_batched_warm_start_matches_host_sequence(iteration):
         np.testing.assert_allclose(dev[k], host[k], rtol=1e-9, atol=1e-12)
     b1.close()
     b2.close()
+    return dev
+
+
+@pytest.mark.parametrize("iteration", [H.ITERATION_STREAM, H.ITERATION_GRAM])
+def test_full_jones_batched_warm_start_matches_host_sequence(iteration):
+    """ddh_batch_solve_warm on full-Jones intervals: the same starts and solves
+    as the per-interval host sequence one_step -> 2 x stretch iterations ->
+    ddh_warmstart_features_2x2 -> forward -> ddh_warmstart_apply_2x2,
+    followed by a batched solve (same schedule on both sides), with the
+    streaming and the Gram iteration."""
+    print(f"case: full-Jones batched warm start vs host sequence, iteration {iteration}",
+          flush=True)
+    batched_warm_matches_host(dict(FJ, iteration=iteration), 60)
+
+
+def test_scalar_batched_warm_start_matches_host_sequence():
+    """The scalar solver's warm start: each gain goes through the features as a
+    pair holding it twice and starts from the mean of the two polarizations'
+    corrections, so the solve runs on one gain per solution; batched equals
+    the host sequence, and the solutions are finite."""
+    print("case: scalar batched warm start vs host sequence", flush=True)
+    dev = batched_warm_matches_host(dict(n_solution_polarizations=1), 80)
+    for d in dev:
+        assert d.shape[1] == 12 * 3 and np.all(np.isfinite(d))
 
 
 def test_full_jones_warm_start_apply_rows():
