@@ -199,7 +199,8 @@ def _rows_ptrs(a):
 
 def one_step(solver, dproblem, solutions):
     """Undamped solver update from `solutions` and its denominators (two per
-    solution: a diagonal pair's, or the diagonal of a 2x2 denominator)."""
+    solution: a diagonal pair's, a scalar solution's summed denominator
+    twice, or the diagonal of a 2x2 denominator)."""
     sol = np.ascontiguousarray(solutions, dtype=np.complex128)
     nxt = np.zeros_like(sol)
     den = np.zeros((sol.shape[0], solver.n_antennas * solver.n_sub * 2), np.float64)
