@@ -161,6 +161,10 @@ def main():
     ap.add_argument("--stretch", type=int, default=2,
                     help="damped iterations per warm-start stretch (2 measured best at this "
                          "workload: 1, 3, 4 and 6 gave fewer solves/s)")
+    ap.add_argument("--poll", type=int, default=32,
+                    help="iterations the batches queue between reads of their states (32 "
+                         "measured best for the optimized batches: 4, 8, 16 and 64 gave 919, "
+                         "947, 960 and 962 solves/s against 966)")
     ap.add_argument("--epochs", type=int, default=10, help="warm-start training epochs")
     ap.add_argument("--fine-epochs", type=int, default=0,
                     help="further warm-start training epochs at a tenth of the learning rate")
@@ -310,7 +314,7 @@ def main():
         for ts in tsolvers:
             ts.close()
         for b in batches:
-            b.set_schedule(tuned[0], tuned[1], GRAPH, 8)
+            b.set_schedule(tuned[0], tuned[1], GRAPH, args.poll)
             b.set_antenna_solve_schedule(*solve_schedule)
             if iteration == H.ITERATION_GRAM:
                 b.set_gram_solve_groups(groups)
